@@ -1,0 +1,459 @@
+/*
+ * lsmps3d_amd.h — C-ABI of the MI355X (gfx950) per-step LSMPS particle pipeline.
+ *
+ * The entry points replace the C++17 module API of lsmps-3D's hot path
+ * (include/lsmps3d/{neighbor,surface,moment_matrix,provision,ppe,correction}).
+ * Each declaration names the reference function it stands in for.
+ *
+ * Conventions
+ *  - Every pointer inside a particle / neighbor / cell / PPE struct is a DEVICE
+ *    pointer (hipMalloc'd or torch-owned HBM) unless a function says "host".
+ *  - `real` of the reference is double here (LSMPS3D_USE_DOUBLE=ON, the
+ *    reference default, CMakeLists.txt:10).  The precision mode of a context
+ *    changes only internal arithmetic (operators, SpMV), never buffer types.
+ *  - Every entry returns an int status (0 = LSMPS_OK) and never exits the
+ *    process; lsmps_last_error() returns the message of the last failure on the
+ *    calling thread.  The reference printed to stderr and called std::exit
+ *    (include/lsmps3d/core/cuda_check.cuh:11-20).
+ *  - Work is enqueued on the context's stream.  Unless the context was created
+ *    with LSMPS_FLAG_ASYNC, each entry synchronises that stream before
+ *    returning, matching the reference's synchronous semantics.
+ *  - Counts are capacities, as in the reference: NeighborListView::neighbor_count
+ *    is the allocated index capacity (src/core/runtime.cu:232), and the PPE
+ *    matrix reports nnz = N + neighbor capacity (src/ppe/ppe_matrix.cu:494-499)
+ *    while row_offsets[N] holds the used count.
+ */
+#ifndef LSMPS3D_AMD_H
+#define LSMPS3D_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------- */
+enum {
+  LSMPS_OK = 0,
+  LSMPS_ERR_INVALID_ARGUMENT = 1, /* precondition violated (ref: std::exit paths) */
+  LSMPS_ERR_CAPACITY = 2,         /* neighbor / nnz capacity exceeded (neighbor_search.cu:22-30) */
+  LSMPS_ERR_DEVICE = 3,           /* HIP runtime error */
+  LSMPS_ERR_NOT_PREPARED = 4,     /* moment views used before prepare (moment_matrix.cu:571-576) */
+  LSMPS_ERR_NOT_CONVERGED = 5,    /* PPE solver hit max_iterations */
+  LSMPS_ERR_OUT_OF_MEMORY = 6,
+  LSMPS_ERR_NUMERICAL = 7         /* non-finite values / breakdown */
+};
+
+/* SurfaceType (include/lsmps3d/surface/surface_type.cuh:5-10) */
+enum {
+  LSMPS_SURFACE_INNER = 0,
+  LSMPS_SURFACE_NEAR_SURFACE = 1,
+  LSMPS_SURFACE_SURFACE = 2,
+  LSMPS_SURFACE_SPLASH = 3
+};
+
+/* MomentMatrixKind (include/lsmps3d/moment_matrix/moment_matrix.cuh:13-18) */
+enum {
+  LSMPS_MOMENT_VELOCITY_WALL_DIRICHLET_A = 0,
+  LSMPS_MOMENT_PRESSURE_WALL_NEUMANN_A = 1,
+  LSMPS_MOMENT_PRESSURE_WALL_NEUMANN_B = 2,
+  LSMPS_MOMENT_FLUID_ONLY_A = 3
+};
+
+/* MomentBasisKind (include/lsmps3d/moment_matrix/basis.cuh:11-20) */
+enum { LSMPS_BASIS_TYPE_A = 0, LSMPS_BASIS_TYPE_B = 1 };
+
+/* Arithmetic mode of a context (no counterpart: the reference picks `real` at build time). */
+enum {
+  LSMPS_PRECISION_FP64 = 0,  /* every stage in FP64 */
+  LSMPS_PRECISION_MIXED = 1  /* FP32 operator rows + FP32 inner SpMV, FP64 refinement/dots/residual */
+};
+
+/* Weight kernel w(r).  The reference uses the linear kernel everywhere
+ * (surface_detection.cu:26-31, moment_matrix.cu:145-150, ppe_matrix.cu:148-153,
+ *  explicit_update.cu:121-126, pressure_correction.cu:127-132). */
+enum { LSMPS_WEIGHT_LINEAR = 0, LSMPS_WEIGHT_QUADRATIC = 1 };
+
+/* PPE Krylov method (replaces AmgxPpeSolver's GMRES, ppe_matrix.cu:586-679).
+ * The assembled matrix is non-symmetric and indefinite near wall corners of the
+ * free surface, so the default is Jacobi-preconditioned IDR(s); BICGSTAB is
+ * IDR(1); PCG is only meaningful for symmetric positive systems. */
+enum { LSMPS_SOLVER_PCG = 0, LSMPS_SOLVER_BICGSTAB = 1, LSMPS_SOLVER_IDRS = 2 };
+
+/* Context flags */
+enum { LSMPS_FLAG_ASYNC = 1 };
+
+/* ---- data contract (POD mirrors of the reference views) ------------------ */
+
+/* FluidParticleSoA (include/lsmps3d/particle/particle_data.cuh:7-17) */
+typedef struct lsmps_fluid_soa {
+  size_t count;
+  double* x;
+  double* y;
+  double* z;
+  double* vx;
+  double* vy;
+  double* vz;
+  double* pressure;
+  int32_t* surface_type;
+} lsmps_fluid_soa;
+
+/* WallParticleSoA (particle_data.cuh:19-30) */
+typedef struct lsmps_wall_soa {
+  size_t count;
+  double* x;
+  double* y;
+  double* z;
+  double* vx;
+  double* vy;
+  double* vz;
+  double* normal_x;
+  double* normal_y;
+  double* normal_z;
+} lsmps_wall_soa;
+
+/* NeighborListView (include/lsmps3d/particle/neighbor_list.cuh:7-12) */
+typedef struct lsmps_neighbor_list {
+  size_t particle_count;
+  size_t neighbor_count; /* allocated capacity of `indices` */
+  int32_t* offsets;      /* particle_count + 1 */
+  int32_t* indices;
+} lsmps_neighbor_list;
+
+/* CellGrid (include/lsmps3d/neighbor/cell_list.cuh:7-15) */
+typedef struct lsmps_cell_grid {
+  double origin[3];
+  double cell_size;
+  int32_t dims[3];
+} lsmps_cell_grid;
+
+/* CellListView (cell_list.cuh:17-24) */
+typedef struct lsmps_cell_list {
+  size_t particle_count;
+  size_t cell_count;
+  int32_t* cell_ids;       /* sorted cell id per sorted position */
+  int32_t* sorted_indices; /* particle index per sorted position */
+  int32_t* cell_begin;     /* -1 = empty */
+  int32_t* cell_end;       /* -1 = empty */
+} lsmps_cell_list;
+
+/* SurfaceDetectionDiagnosticsView (include/lsmps3d/surface/surface_detection.cuh:10-21);
+ * every member may be NULL. */
+typedef struct lsmps_surface_diagnostics {
+  int32_t* fluid_neighbor_count;
+  int32_t* wall_neighbor_count;
+  double* number_density;
+  double* number_density_ratio;
+  double* anisotropy;
+  double* air_open_ratio;
+  double* air_anisotropy;
+  double* surface_normal_x;
+  double* surface_normal_y;
+  double* surface_normal_z;
+} lsmps_surface_diagnostics;
+
+/* CsrMatrixView (include/lsmps3d/ppe/ppe_matrix.cuh:15-22) */
+typedef struct lsmps_csr {
+  size_t rows;
+  size_t cols;
+  size_t nnz; /* capacity semantics, see header comment */
+  int32_t* row_offsets;
+  int32_t* col_indices;
+  double* values;
+} lsmps_csr;
+
+/* PpeWorkspaceView (ppe_matrix.cuh:24-30) */
+typedef struct lsmps_ppe_workspace {
+  lsmps_csr matrix;
+  double* rhs;
+  double* pressure;
+  double* divergence;
+  double* pressure_laplacian; /* holds the wall Neumann RHS term, as the reference (ppe_matrix.cu:367-369) */
+} lsmps_ppe_workspace;
+
+/* CorrectionVectorView (include/lsmps3d/correction/pressure_correction.cuh:11-16) and
+ * ProvisionVelocityView (include/lsmps3d/provision/explicit_update.cuh:11-16). */
+typedef struct lsmps_vector3 {
+  size_t count;
+  double* x;
+  double* y;
+  double* z;
+} lsmps_vector3;
+
+/* MomentMatrixView (moment_matrix.cuh:20-34).  inverse_matrices is N * n * n,
+ * column-major per particle: entry (row, col) of particle i at
+ * inverse_matrices[i*n*n + row + col*n]. */
+typedef struct lsmps_moment_view {
+  size_t particle_count;
+  int32_t matrix_size;
+  int32_t basis_kind;
+  int32_t kind;
+  int32_t is_ready;
+  double support_radius;
+  double wall_weight_scale;
+  uint64_t geometry_generation;
+  const double* inverse_matrices;
+  const double* moment_trace;
+  const double* regularization_added;
+  const int32_t* info;
+  const int32_t* inversion_count;
+} lsmps_moment_view;
+
+/* SimulationConfig (include/lsmps3d/core/config.hpp:12-82), numeric fields in
+ * declaration order; file-path / string fields are host IO and are not mirrored.
+ * `weight_kernel` is the one addition (see LSMPS_WEIGHT_*). */
+typedef struct lsmps_config {
+  /* Geometry */
+  double particle_spacing;
+  double support_radius;
+  double near_surface_radius;
+  double cell_origin[3];
+  double cell_size;
+  int32_t cell_dims[3];
+  /* Simulation */
+  double time_step;
+  double min_time_step;
+  double max_time_step;
+  double time_step_growth_factor;
+  double final_time;
+  double output_interval;
+  double density;
+  double kinematic_viscosity;
+  double cfl;
+  double gravity[3];
+  /* Surface and virtual-light diagnostics */
+  int32_t splash_neighbor_threshold;
+  int32_t include_wall_neighbors;
+  double number_density_ratio_threshold;
+  double air_open_ratio_threshold;
+  double air_anisotropy_threshold;
+  double wall_normal_independence_threshold;
+  double virtual_light_cone_cosine;
+  /* LSMPS */
+  double lsmps_regularization;
+  double lsmps_wall_weight_scale;
+  /* Pressure correction, particle shifting, and smoothing */
+  double ps_displacement_scale;
+  double ps_min_distance_ratio;
+  double ps_max_displacement_ratio;
+  double wall_clearance_ratio;
+  double velocity_smoothing_strength;
+  /* Addition: weight kernel */
+  int32_t weight_kernel;
+  int32_t reserved0;
+} lsmps_config;
+
+/* Options / report of the PPE solve (replaces the AMGX config string,
+ * ppe_matrix.cu:604-612).  The tolerance is on the true FP64 relative
+ * residual ||b - A p||_2 / ||b||_2. */
+typedef struct lsmps_solver_options {
+  int32_t method;          /* LSMPS_SOLVER_* */
+  int32_t max_iterations;  /* total Krylov iterations */
+  double tolerance;        /* relative true residual */
+  double inner_tolerance;  /* mixed mode: relative reduction per FP32 inner solve */
+  int32_t check_interval;  /* PCG: iterations between host convergence checks */
+  int32_t clamp_negative;  /* clamp p<0 after the solve, as ppe_matrix.cu:662 (default 1) */
+  int32_t shadow_dimension; /* IDR(s): s, 1..16 (default 8) */
+  int32_t reserved0;
+} lsmps_solver_options;
+
+typedef struct lsmps_solve_report {
+  int32_t method;
+  int32_t iterations;         /* Krylov iterations (inner iterations in mixed mode) */
+  int32_t refinements;        /* outer FP64 residual recomputations */
+  int32_t converged;
+  double relative_residual;   /* true FP64 relative residual at exit */
+  double rhs_norm;
+} lsmps_solve_report;
+
+/* Opaque objects */
+typedef struct lsmps_context lsmps_context;
+typedef struct lsmps_moment_matrix lsmps_moment_matrix;
+typedef struct lsmps_stepper lsmps_stepper;
+
+/* ---- context ---------------------------------------------------------- */
+/* No reference counterpart: the reference used the legacy default stream of the
+ * current device implicitly.  `precision` is LSMPS_PRECISION_*. */
+int lsmps_context_create(int device, int precision, uint32_t flags, lsmps_context** out);
+int lsmps_context_destroy(lsmps_context* ctx);
+int lsmps_context_set_stream(lsmps_context* ctx, void* hip_stream);
+int lsmps_context_synchronize(lsmps_context* ctx);
+int lsmps_context_precision(const lsmps_context* ctx);
+const char* lsmps_last_error(void);
+const char* lsmps_version(void);
+
+/* ---- config (host) ---------------------------------------------------- */
+/* default_simulation_config / SimulationConfig{} defaults (config.hpp:12-82). */
+int lsmps_default_config(lsmps_config* out);
+/* derive_cell_grid_from_particles (src/app/simulation_driver.cu:83-108); host pointers. */
+int lsmps_derive_cell_grid(const double* fluid_x, const double* fluid_y, const double* fluid_z,
+                           size_t fluid_count, const double* wall_x, const double* wall_y,
+                           const double* wall_z, size_t wall_count, lsmps_config* config);
+
+/* ---- neighbor (include/lsmps3d/neighbor/neighbor_search.cuh) ------------ */
+/* build_cell_list(const FluidParticleSoA&, const CellGrid&, CellListView) — neighbor_search.cuh:10-12 */
+int lsmps_build_cell_list_fluid(lsmps_context* ctx, const lsmps_fluid_soa* particles,
+                                const lsmps_cell_grid* grid, const lsmps_cell_list* cells);
+/* build_cell_list(const WallParticleSoA&, const CellGrid&, CellListView) — neighbor_search.cuh:14-16 */
+int lsmps_build_cell_list_wall(lsmps_context* ctx, const lsmps_wall_soa* particles,
+                               const lsmps_cell_grid* grid, const lsmps_cell_list* cells);
+/* build_neighbor_lists(...) — neighbor_search.cuh:18-24 */
+int lsmps_build_neighbor_lists(lsmps_context* ctx, const lsmps_fluid_soa* fluid,
+                               const lsmps_wall_soa* walls, const lsmps_config* config,
+                               const lsmps_cell_list* fluid_cells, const lsmps_cell_list* wall_cells,
+                               const lsmps_neighbor_list* fluid_neighbors,
+                               const lsmps_neighbor_list* wall_neighbors);
+
+/* ---- surface (include/lsmps3d/surface/surface_detection.cuh) ------------ */
+/* compute_uniform_reference_number_density(l0, re) — surface_detection.cuh:28-29 (host) */
+int lsmps_reference_number_density(double particle_spacing, double support_radius,
+                                   int32_t weight_kernel, double* out);
+/* classify_surface_particles(...) — surface_detection.cuh:31-36; diagnostics may be NULL */
+int lsmps_classify_surface(lsmps_context* ctx, const lsmps_fluid_soa* fluid,
+                           const lsmps_wall_soa* walls, const lsmps_neighbor_list* fluid_neighbors,
+                           const lsmps_neighbor_list* wall_neighbors, const lsmps_config* config,
+                           const lsmps_surface_diagnostics* diagnostics);
+
+/* ---- moment matrices (include/lsmps3d/moment_matrix/moment_matrix.cuh) -- */
+/* DeviceMomentMatrix(fluid_capacity, config) — moment_matrix.cuh:36-40 */
+int lsmps_moment_create(lsmps_context* ctx, size_t fluid_capacity, const lsmps_config* config,
+                        lsmps_moment_matrix** out);
+int lsmps_moment_destroy(lsmps_moment_matrix* moments);
+/* DeviceMomentMatrix::resize / set_config / bytes — moment_matrix.cuh:47-52 */
+int lsmps_moment_resize(lsmps_moment_matrix* moments, size_t fluid_capacity);
+int lsmps_moment_set_config(lsmps_moment_matrix* moments, const lsmps_config* config);
+size_t lsmps_moment_bytes(const lsmps_moment_matrix* moments);
+/* DeviceMomentMatrix::prepare_matrices(...) — moment_matrix.cuh:54-58.  Builds the
+ * operator rows consumed by provision / PPE / correction for the two consumed kinds;
+ * full inverses of any kind are materialised when lsmps_moment_get_view asks for them. */
+int lsmps_moment_prepare(lsmps_moment_matrix* moments, const lsmps_fluid_soa* fluid,
+                         const lsmps_wall_soa* walls, const lsmps_neighbor_list* fluid_neighbors,
+                         const lsmps_neighbor_list* wall_neighbors, uint64_t geometry_generation);
+/* velocity_type_a() / pressure_type_a() / pressure_type_b() / fluid_only_type_a() —
+ * moment_matrix.cuh:60-63.  `kind` is LSMPS_MOMENT_*. */
+int lsmps_moment_get_view(lsmps_moment_matrix* moments, int32_t kind, lsmps_moment_view* out);
+
+/* ---- provision (include/lsmps3d/provision/explicit_update.cuh) --------- */
+/* DeviceProvisionExplicitUpdate::compute_temporary_velocity(...) — explicit_update.cuh:71-78.
+ * `laplacian_workspace` may be NULL (the library then owns a workspace). */
+int lsmps_provision_compute(lsmps_context* ctx, const lsmps_config* config,
+                            const lsmps_fluid_soa* fluid, const lsmps_wall_soa* walls,
+                            const lsmps_neighbor_list* fluid_neighbors,
+                            const lsmps_neighbor_list* wall_neighbors, lsmps_moment_matrix* moments,
+                            uint64_t geometry_generation, const lsmps_fluid_soa* temporary_velocity,
+                            const lsmps_wall_soa* temporary_wall_velocity,
+                            const lsmps_vector3* laplacian_workspace);
+/* compute_provision_temporary_velocity — explicit_update.cuh:84-87 */
+int lsmps_provision_temporary_velocity(lsmps_context* ctx, const lsmps_fluid_soa* fluid,
+                                       const lsmps_config* config,
+                                       const lsmps_vector3* velocity_laplacian,
+                                       const lsmps_fluid_soa* temporary_velocity);
+/* compute_provision_temporary_wall_velocity — explicit_update.cuh:89-91 */
+int lsmps_provision_temporary_wall_velocity(lsmps_context* ctx, const lsmps_wall_soa* walls,
+                                            const lsmps_config* config,
+                                            const lsmps_wall_soa* temporary_wall_velocity);
+
+/* ---- PPE (include/lsmps3d/ppe/ppe_matrix.cuh) --------------------------- */
+/* DevicePpeMatrixAssembler::assemble(...) — ppe_matrix.cuh:82-89; writes into the
+ * caller's workspace (matrix.nnz must be >= N + fluid neighbor capacity). */
+int lsmps_ppe_assemble(lsmps_context* ctx, const lsmps_config* config,
+                       const lsmps_fluid_soa* fluid, const lsmps_wall_soa* walls,
+                       const lsmps_neighbor_list* fluid_neighbors,
+                       const lsmps_neighbor_list* wall_neighbors,
+                       const lsmps_fluid_soa* temporary_velocity,
+                       const lsmps_wall_soa* temporary_wall_velocity, lsmps_moment_matrix* moments,
+                       uint64_t geometry_generation, const lsmps_ppe_workspace* workspace);
+/* AmgxPpeSolver::solve(const CsrMatrixView&, const real* rhs, real* pressure) —
+ * ppe_matrix.cuh:112.  Hand-written Jacobi-PCG / Jacobi-BiCGStab from p = 0.
+ * options may be NULL (defaults: PCG, tol 1e-10); report may be NULL. */
+int lsmps_ppe_solve(lsmps_context* ctx, const lsmps_csr* matrix, const double* rhs,
+                    double* pressure, const lsmps_solver_options* options,
+                    lsmps_solve_report* report);
+int lsmps_default_solver_options(lsmps_solver_options* out);
+
+/* ---- correction (include/lsmps3d/correction/pressure_correction.cuh) ---- */
+/* DevicePressureCorrection::apply(...) — pressure_correction.cuh:70-77.  The three
+ * workspaces (3N each) are caller-owned here. */
+int lsmps_correction_apply(lsmps_context* ctx, const lsmps_config* config,
+                           const lsmps_fluid_soa* fluid, const lsmps_wall_soa* walls,
+                           const lsmps_neighbor_list* fluid_neighbors,
+                           const lsmps_neighbor_list* wall_neighbors,
+                           const lsmps_fluid_soa* temporary_velocity, double* pressure,
+                           lsmps_moment_matrix* moments, uint64_t geometry_generation,
+                           const lsmps_vector3* pressure_gradient,
+                           const lsmps_vector3* ps_displacement,
+                           const lsmps_vector3* smoothed_velocity);
+/* clamp_particle_pressure — pressure_correction.cuh:91 */
+int lsmps_clamp_particle_pressure(lsmps_context* ctx, const lsmps_fluid_soa* fluid,
+                                  double* pressure);
+/* compute_pressure_gradient — pressure_correction.cuh:93-100; needs prepared moments */
+int lsmps_compute_pressure_gradient(lsmps_context* ctx, const lsmps_fluid_soa* fluid,
+                                    const lsmps_wall_soa* walls,
+                                    const lsmps_neighbor_list* fluid_neighbors,
+                                    const lsmps_neighbor_list* wall_neighbors,
+                                    const double* pressure, const lsmps_config* config,
+                                    lsmps_moment_matrix* moments,
+                                    const lsmps_vector3* pressure_gradient);
+/* compute_particle_shifting_displacement — pressure_correction.cuh:102-107 */
+int lsmps_compute_particle_shifting(lsmps_context* ctx, const lsmps_fluid_soa* fluid,
+                                    const lsmps_wall_soa* walls,
+                                    const lsmps_neighbor_list* fluid_neighbors,
+                                    const lsmps_neighbor_list* wall_neighbors,
+                                    const lsmps_config* config,
+                                    const lsmps_vector3* ps_displacement);
+/* update_velocity_and_position — pressure_correction.cuh:109-113 */
+int lsmps_update_velocity_and_position(lsmps_context* ctx, const lsmps_fluid_soa* fluid,
+                                       const lsmps_fluid_soa* temporary_velocity,
+                                       const lsmps_config* config,
+                                       const lsmps_vector3* pressure_gradient,
+                                       const lsmps_vector3* ps_displacement);
+/* anti_penetration_kernel launch inside apply — pressure_correction.cu:404-444,639-643 */
+int lsmps_apply_anti_penetration(lsmps_context* ctx, const lsmps_fluid_soa* fluid,
+                                 const lsmps_wall_soa* walls,
+                                 const lsmps_neighbor_list* wall_neighbors,
+                                 const lsmps_config* config);
+/* smooth_particle_velocity — pressure_correction.cuh:115-118 */
+int lsmps_smooth_particle_velocity(lsmps_context* ctx, const lsmps_fluid_soa* fluid,
+                                   const lsmps_neighbor_list* fluid_neighbors,
+                                   const lsmps_config* config,
+                                   const lsmps_vector3* smoothed_velocity);
+
+/* ---- fused step (src/app/simulation_driver.cu:336-393 stage order) ------ */
+/* A stepper owns every intermediate buffer of one time step: cell lists, the two
+ * CSR tables (capacity `max_neighbors_per_particle` per fluid particle, 256 in the
+ * reference driver, simulation_driver.cu:29-30), moment operators, u*, the PPE
+ * system and the correction workspaces. */
+int lsmps_stepper_create(lsmps_context* ctx, size_t fluid_capacity, size_t wall_capacity,
+                         size_t max_neighbors_per_particle, const lsmps_config* config,
+                         lsmps_stepper** out);
+int lsmps_stepper_destroy(lsmps_stepper* stepper);
+int lsmps_stepper_set_config(lsmps_stepper* stepper, const lsmps_config* config);
+int lsmps_stepper_set_solver(lsmps_stepper* stepper, const lsmps_solver_options* options);
+size_t lsmps_stepper_bytes(const lsmps_stepper* stepper);
+/* One step: rebuild_geometry (neighbor CSR + surface labels, generation++),
+ * provision, PPE assembly, PPE solve, pressure correction (with shifting,
+ * anti-penetration, smoothing).  fluid x, v, pressure, surface_type are updated
+ * in place; walls are read only. */
+int lsmps_stepper_step(lsmps_stepper* stepper, const lsmps_fluid_soa* fluid,
+                       const lsmps_wall_soa* walls, lsmps_solve_report* report);
+/* Stage times of the last step in ms (hipEvents on the context stream):
+ * [0] geometry, [1] moments, [2] provision, [3] assembly, [4] solve,
+ * [5] correction, [6] total.  No reference counterpart (the reference has no timers). */
+int lsmps_stepper_stage_times(const lsmps_stepper* stepper, double* ms, int count);
+/* Views of the stepper's internal buffers (device pointers; valid until the next
+ * step or destroy) for inspection and tests. */
+int lsmps_stepper_neighbor_lists(lsmps_stepper* stepper, lsmps_neighbor_list* fluid_neighbors,
+                                 lsmps_neighbor_list* wall_neighbors);
+int lsmps_stepper_ppe_workspace(lsmps_stepper* stepper, lsmps_ppe_workspace* out);
+int lsmps_stepper_temporary_velocity(lsmps_stepper* stepper, lsmps_fluid_soa* fluid_out,
+                                     lsmps_wall_soa* wall_out);
+int lsmps_stepper_correction_vectors(lsmps_stepper* stepper, lsmps_vector3* pressure_gradient,
+                                     lsmps_vector3* ps_displacement);
+lsmps_moment_matrix* lsmps_stepper_moments(lsmps_stepper* stepper);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LSMPS3D_AMD_H */

@@ -1,0 +1,80 @@
+"""Builds liblsmps3d_amd.so (gfx950) in-tree from csrc/ with hipcc.
+
+Per-file flags: the neighbor and surface stages must round exactly like the
+FP64 oracle, so they are compiled with -ffp-contract=off; the other stages let
+the compiler contract multiply-adds into FMAs.  Packed FP32 instructions are
+disabled everywhere.
+"""
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+OBJ = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "liblsmps3d_amd.so")
+ARCH = os.environ.get("LSMPS_OFFLOAD_ARCH", "gfx950")
+
+SOURCES = {
+    "neighbor.hip": ["-ffp-contract=off"],
+    "surface.hip": ["-ffp-contract=off"],
+    "moment.hip": ["-ffp-contract=fast"],
+    "provision.hip": ["-ffp-contract=fast"],
+    "ppe.hip": ["-ffp-contract=fast"],
+    "solver.hip": ["-ffp-contract=fast"],
+    "correction.hip": ["-ffp-contract=fast"],
+    "stepper.cpp": [],
+    "capi.cpp": [],
+}
+
+COMMON = ["-O3", "-std=c++17", f"--offload-arch={ARCH}", "-fPIC", "-Wall", "-Wextra",
+          "-Wno-unused-parameter", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+          f"-I{os.path.join(ROOT, 'include')}", f"-I{CSRC}"]
+
+
+def hipcc():
+    path = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    return path
+
+
+def _compile(src, extra, verbose):
+    obj = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
+    cmd = [hipcc(), "-x", "hip", *COMMON, *extra, "-c", os.path.join(CSRC, src), "-o", obj]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"hipcc failed for {src}:\n{res.stdout}\n{res.stderr}")
+    warn = "\n".join(line for line in res.stderr.splitlines()
+                     if "packed-fp32-ops" not in line and line.strip())
+    return obj, warn
+
+
+def build(verbose=False, jobs=None):
+    os.makedirs(OBJ, exist_ok=True)
+    jobs = jobs or min(8, len(SOURCES))
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        futs = [ex.submit(_compile, s, f, verbose) for s, f in SOURCES.items()]
+        objs = []
+        for fut in futs:
+            obj, warn = fut.result()
+            if warn and verbose:
+                print(warn, flush=True)
+            objs.append(obj)
+    tmp = LIB + ".tmp"
+    cmd = [hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"link failed:\n{res.stdout}\n{res.stderr}")
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    build(verbose="-v" in sys.argv)
+    print(LIB)

@@ -1,0 +1,656 @@
+// extern "C" surface of include/lsmps3d_amd.h.  Every entry converts C++
+// exceptions into status codes and a thread-local message; nothing exits the
+// process (the reference called std::exit from library code,
+// include/lsmps3d/core/cuda_check.cuh:11-20).
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <new>
+#include <string>
+
+#include "objects.hpp"
+
+using namespace lsmps;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int handle(const char* where) {
+  try {
+    throw;
+  } catch (const Error& e) {
+    g_last_error = std::string(where) + ": " + e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_last_error = std::string(where) + ": host allocation failed";
+    return LSMPS_ERR_OUT_OF_MEMORY;
+  } catch (const std::exception& e) {
+    g_last_error = std::string(where) + ": " + e.what();
+    return LSMPS_ERR_DEVICE;
+  } catch (...) {
+    g_last_error = std::string(where) + ": unknown error";
+    return LSMPS_ERR_DEVICE;
+  }
+}
+
+#define API_BEGIN try {
+#define API_END(name)      \
+  }                        \
+  catch (...) {            \
+    return handle(name);   \
+  }                        \
+  return LSMPS_OK;
+
+Context& C(lsmps_context* ctx) {
+  require(ctx != nullptr, "null context");
+  ctx->impl.activate();
+  return ctx->impl;
+}
+
+template <typename T>
+const T& deref(const T* p, const char* what) {
+  if (!p) raise(LSMPS_ERR_INVALID_ARGUMENT, std::string("null ") + what);
+  return *p;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* lsmps_last_error(void) { return g_last_error.c_str(); }
+const char* lsmps_version(void) { return "lsmps3d_amd 0.1.0 (gfx950)"; }
+
+int lsmps_context_create(int device, int precision, uint32_t flags, lsmps_context** out) {
+  API_BEGIN
+  require(out != nullptr, "null output pointer");
+  *out = nullptr;
+  require(precision == LSMPS_PRECISION_FP64 || precision == LSMPS_PRECISION_MIXED,
+          "unknown precision mode");
+  int count = 0;
+  LSMPS_HIP(hipGetDeviceCount(&count));
+  require(device >= 0 && device < count, "device index out of range", LSMPS_ERR_DEVICE);
+  auto* ctx = new lsmps_context();
+  ctx->impl.device = device;
+  ctx->impl.precision = precision;
+  ctx->impl.flags = flags;
+  try {
+    LSMPS_HIP(hipSetDevice(device));
+    LSMPS_HIP(hipHostMalloc(reinterpret_cast<void**>(&ctx->impl.host_i32), 64));
+    LSMPS_HIP(hipHostMalloc(reinterpret_cast<void**>(&ctx->impl.host_f64), 64));
+  } catch (...) {
+    delete ctx;
+    throw;
+  }
+  *out = ctx;
+  API_END("lsmps_context_create")
+}
+
+int lsmps_context_destroy(lsmps_context* ctx) {
+  API_BEGIN
+  if (!ctx) return LSMPS_OK;
+  (void)hipSetDevice(ctx->impl.device);
+  (void)hipStreamSynchronize(ctx->impl.stream);
+  if (ctx->impl.host_i32) (void)hipHostFree(ctx->impl.host_i32);
+  if (ctx->impl.host_f64) (void)hipHostFree(ctx->impl.host_f64);
+  delete ctx;
+  API_END("lsmps_context_destroy")
+}
+
+int lsmps_context_set_stream(lsmps_context* ctx, void* hip_stream) {
+  API_BEGIN
+  C(ctx).stream = static_cast<hipStream_t>(hip_stream);
+  API_END("lsmps_context_set_stream")
+}
+
+int lsmps_context_synchronize(lsmps_context* ctx) {
+  API_BEGIN
+  C(ctx).sync();
+  API_END("lsmps_context_synchronize")
+}
+
+int lsmps_context_precision(const lsmps_context* ctx) {
+  return ctx ? ctx->impl.precision : -1;
+}
+
+int lsmps_default_config(lsmps_config* out) {
+  API_BEGIN
+  require(out != nullptr, "null config");
+  lsmps_config c{};
+  c.particle_spacing = 0.01;
+  c.support_radius = 0.031;
+  c.near_surface_radius = 0.015;
+  c.cell_size = c.support_radius;
+  c.cell_dims[0] = c.cell_dims[1] = c.cell_dims[2] = 1;
+  c.time_step = 1.0e-4;
+  c.min_time_step = 1.0e-6;
+  c.max_time_step = 1.0e-3;
+  c.time_step_growth_factor = 1.05;
+  c.final_time = 1.0;
+  c.output_interval = 0.01;
+  c.density = 1000.0;
+  c.kinematic_viscosity = 1.0e-6;
+  c.cfl = 0.2;
+  c.gravity[2] = -9.81;
+  c.splash_neighbor_threshold = 4;
+  c.include_wall_neighbors = 1;
+  c.number_density_ratio_threshold = 0.85;
+  c.air_open_ratio_threshold = 0.33;
+  c.air_anisotropy_threshold = 0.033;
+  c.wall_normal_independence_threshold = 0.25;
+  c.virtual_light_cone_cosine = 0.8660254037844386;
+  c.lsmps_regularization = 1.0e-8;
+  c.lsmps_wall_weight_scale = 1.0;
+  c.ps_displacement_scale = 0.05;
+  c.ps_min_distance_ratio = 0.85;
+  c.ps_max_displacement_ratio = 0.2;
+  c.wall_clearance_ratio = 0.25;
+  c.velocity_smoothing_strength = 0.1;
+  c.weight_kernel = LSMPS_WEIGHT_LINEAR;
+  *out = c;
+  API_END("lsmps_default_config")
+}
+
+int lsmps_derive_cell_grid(const double* fx, const double* fy, const double* fz,
+                           size_t nf, const double* wx, const double* wy, const double* wz,
+                           size_t nw, lsmps_config* config) {
+  API_BEGIN
+  require(config != nullptr, "null config");
+  bool has = false;
+  double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+  auto include = [&](const double* x, const double* y, const double* z, size_t n) {
+    for (size_t i = 0; i < n; ++i) {
+      const double p[3] = {x[i], y[i], z[i]};
+      for (int k = 0; k < 3; ++k)
+        require(std::isfinite(p[k]), "Particle coordinates must be finite");
+      if (!has) {
+        for (int k = 0; k < 3; ++k) lo[k] = hi[k] = p[k];
+        has = true;
+        continue;
+      }
+      for (int k = 0; k < 3; ++k) {
+        lo[k] = std::min(lo[k], p[k]);
+        hi[k] = std::max(hi[k], p[k]);
+      }
+    }
+  };
+  if (nf) include(fx, fy, fz, nf);
+  if (nw) include(wx, wy, wz, nw);
+  require(has, "Cannot derive cell grid from empty particle input");
+  const double pad = config->support_radius;
+  for (int k = 0; k < 3; ++k) config->cell_origin[k] = lo[k] - pad;
+  for (int k = 0; k < 3; ++k) {
+    const double span = (hi[k] + pad) - config->cell_origin[k];
+    const long long dim = static_cast<long long>(std::ceil(span / config->cell_size)) + 1;
+    require(dim > 0 && dim <= std::numeric_limits<int32_t>::max(),
+            "Derived cell-grid dimension is outside supported index range");
+    config->cell_dims[k] = static_cast<int32_t>(dim);
+  }
+  API_END("lsmps_derive_cell_grid")
+}
+
+// ---- neighbor ---------------------------------------------------------------
+int lsmps_build_cell_list_fluid(lsmps_context* ctx, const lsmps_fluid_soa* p,
+                                const lsmps_cell_grid* grid, const lsmps_cell_list* cells) {
+  API_BEGIN
+  Context& c = C(ctx);
+  const auto& f = deref(p, "particles");
+  build_cell_list(c, f.count, f.x, f.y, f.z, deref(grid, "grid"), deref(cells, "cells"));
+  c.finish();
+  API_END("lsmps_build_cell_list_fluid")
+}
+
+int lsmps_build_cell_list_wall(lsmps_context* ctx, const lsmps_wall_soa* p,
+                               const lsmps_cell_grid* grid, const lsmps_cell_list* cells) {
+  API_BEGIN
+  Context& c = C(ctx);
+  const auto& w = deref(p, "particles");
+  build_cell_list(c, w.count, w.x, w.y, w.z, deref(grid, "grid"), deref(cells, "cells"));
+  c.finish();
+  API_END("lsmps_build_cell_list_wall")
+}
+
+int lsmps_build_neighbor_lists(lsmps_context* ctx, const lsmps_fluid_soa* fluid,
+                               const lsmps_wall_soa* walls, const lsmps_config* config,
+                               const lsmps_cell_list* fc, const lsmps_cell_list* wc,
+                               const lsmps_neighbor_list* fn, const lsmps_neighbor_list* wn) {
+  API_BEGIN
+  Context& c = C(ctx);
+  build_neighbor_lists(c, deref(fluid, "fluid"), deref(walls, "walls"), deref(config, "config"),
+                       deref(fc, "fluid cells"), deref(wc, "wall cells"),
+                       deref(fn, "fluid neighbors"), deref(wn, "wall neighbors"), nullptr,
+                       nullptr);
+  c.finish();
+  API_END("lsmps_build_neighbor_lists")
+}
+
+// ---- surface ----------------------------------------------------------------
+int lsmps_reference_number_density(double l0, double re, int32_t kernel, double* out) {
+  API_BEGIN
+  require(out != nullptr, "null output");
+  *out = reference_number_density(l0, re, kernel);
+  API_END("lsmps_reference_number_density")
+}
+
+int lsmps_classify_surface(lsmps_context* ctx, const lsmps_fluid_soa* fluid,
+                           const lsmps_wall_soa* walls, const lsmps_neighbor_list* fn,
+                           const lsmps_neighbor_list* wn, const lsmps_config* config,
+                           const lsmps_surface_diagnostics* diagnostics) {
+  API_BEGIN
+  Context& c = C(ctx);
+  classify_surface(c, deref(fluid, "fluid"), deref(walls, "walls"), deref(fn, "fluid neighbors"),
+                   deref(wn, "wall neighbors"), deref(config, "config"), diagnostics);
+  c.finish();
+  API_END("lsmps_classify_surface")
+}
+
+// ---- moments ----------------------------------------------------------------
+int lsmps_moment_create(lsmps_context* ctx, size_t capacity, const lsmps_config* config,
+                        lsmps_moment_matrix** out) {
+  API_BEGIN
+  require(out != nullptr, "null output pointer");
+  Context& c = C(ctx);
+  auto* m = new lsmps_moment_matrix();
+  m->ctx = &c;
+  m->config = deref(config, "config");
+  m->resize(capacity);
+  *out = m;
+  API_END("lsmps_moment_create")
+}
+
+int lsmps_moment_destroy(lsmps_moment_matrix* m) {
+  API_BEGIN
+  if (m) {
+    (void)hipSetDevice(m->ctx->device);
+    (void)hipStreamSynchronize(m->ctx->stream);
+    delete m;
+  }
+  API_END("lsmps_moment_destroy")
+}
+
+int lsmps_moment_resize(lsmps_moment_matrix* m, size_t capacity) {
+  API_BEGIN
+  require(m != nullptr, "null moment matrix");
+  m->ctx->activate();
+  m->resize(capacity);
+  API_END("lsmps_moment_resize")
+}
+
+int lsmps_moment_set_config(lsmps_moment_matrix* m, const lsmps_config* config) {
+  API_BEGIN
+  require(m != nullptr, "null moment matrix");
+  // set_config invalidates the prepared generation (moment_matrix.cu:602-605)
+  m->config = deref(config, "config");
+  m->prepared = false;
+  API_END("lsmps_moment_set_config")
+}
+
+size_t lsmps_moment_bytes(const lsmps_moment_matrix* m) { return m ? m->bytes() : 0; }
+
+int lsmps_moment_prepare(lsmps_moment_matrix* m, const lsmps_fluid_soa* fluid,
+                         const lsmps_wall_soa* walls, const lsmps_neighbor_list* fn,
+                         const lsmps_neighbor_list* wn, uint64_t generation) {
+  API_BEGIN
+  require(m != nullptr, "null moment matrix");
+  m->ctx->activate();
+  m->prepare(deref(fluid, "fluid"), deref(walls, "walls"), deref(fn, "fluid neighbors"),
+             deref(wn, "wall neighbors"), generation);
+  m->ctx->finish();
+  API_END("lsmps_moment_prepare")
+}
+
+int lsmps_moment_get_view(lsmps_moment_matrix* m, int32_t kind, lsmps_moment_view* out) {
+  API_BEGIN
+  require(m != nullptr && out != nullptr, "null argument");
+  require(kind >= 0 && kind <= 3, "Unknown Moment matrix kind");
+  require(m->prepared, "Moment operators must prepare matrices before computing operators",
+          LSMPS_ERR_NOT_PREPARED);
+  m->ctx->activate();
+  m->materialize(kind);
+  LSMPS_HIP(hipMemcpyAsync(m->inversion_count.data(), m->host_counts, 4 * sizeof(int32_t),
+                           hipMemcpyHostToDevice, m->ctx->stream));
+  m->ctx->sync();
+  lsmps_moment_view v{};
+  v.particle_count = m->count;
+  v.matrix_size = moment_size(kind);
+  v.basis_kind = kind == LSMPS_MOMENT_PRESSURE_WALL_NEUMANN_B ? LSMPS_BASIS_TYPE_B : LSMPS_BASIS_TYPE_A;
+  v.kind = kind;
+  v.is_ready = 1;
+  v.support_radius = m->config.support_radius;
+  v.wall_weight_scale = m->config.lsmps_wall_weight_scale;
+  v.geometry_generation = m->generation;
+  v.inverse_matrices = m->full[kind].inverse.data();
+  v.moment_trace = m->full[kind].trace.data();
+  v.regularization_added = m->full[kind].regularization.data();
+  v.info = m->full[kind].info.data();
+  v.inversion_count = m->inversion_count.data() + kind;
+  *out = v;
+  API_END("lsmps_moment_get_view")
+}
+
+// ---- provision --------------------------------------------------------------
+int lsmps_provision_compute(lsmps_context* ctx, const lsmps_config* config,
+                            const lsmps_fluid_soa* fluid, const lsmps_wall_soa* walls,
+                            const lsmps_neighbor_list* fn, const lsmps_neighbor_list* wn,
+                            lsmps_moment_matrix* m, uint64_t generation,
+                            const lsmps_fluid_soa* tmp, const lsmps_wall_soa* tmpw,
+                            const lsmps_vector3* lap) {
+  API_BEGIN
+  Context& c = C(ctx);
+  require(m != nullptr, "null moment matrix");
+  const auto& cfg = deref(config, "config");
+  const auto& f = deref(fluid, "fluid");
+  const auto& w = deref(walls, "walls");
+  // compute_temporary_velocity prepares the moments with its own config (explicit_update.cu:384-387)
+  m->config = cfg;
+  m->prepare(f, w, deref(fn, "fluid neighbors"), deref(wn, "wall neighbors"), generation);
+  provision_fluid(c, cfg, f, w, *fn, *wn, m->velocity_ops(), deref(tmp, "temporary velocity"), lap);
+  provision_walls(c, cfg, w, deref(tmpw, "temporary wall velocity"));
+  c.finish();
+  API_END("lsmps_provision_compute")
+}
+
+int lsmps_provision_temporary_velocity(lsmps_context* ctx, const lsmps_fluid_soa* fluid,
+                                       const lsmps_config* config, const lsmps_vector3* lap,
+                                       const lsmps_fluid_soa* tmp) {
+  API_BEGIN
+  Context& c = C(ctx);
+  provision_combine(c, deref(config, "config"), deref(fluid, "fluid"), deref(lap, "laplacian"),
+                    deref(tmp, "temporary velocity"));
+  c.finish();
+  API_END("lsmps_provision_temporary_velocity")
+}
+
+int lsmps_provision_temporary_wall_velocity(lsmps_context* ctx, const lsmps_wall_soa* walls,
+                                            const lsmps_config* config,
+                                            const lsmps_wall_soa* tmpw) {
+  API_BEGIN
+  Context& c = C(ctx);
+  provision_walls(c, deref(config, "config"), deref(walls, "walls"),
+                  deref(tmpw, "temporary wall velocity"));
+  c.finish();
+  API_END("lsmps_provision_temporary_wall_velocity")
+}
+
+// ---- PPE --------------------------------------------------------------------
+int lsmps_ppe_assemble(lsmps_context* ctx, const lsmps_config* config,
+                       const lsmps_fluid_soa* fluid, const lsmps_wall_soa* walls,
+                       const lsmps_neighbor_list* fn, const lsmps_neighbor_list* wn,
+                       const lsmps_fluid_soa* tmp, const lsmps_wall_soa* tmpw,
+                       lsmps_moment_matrix* m, uint64_t generation,
+                       const lsmps_ppe_workspace* ws) {
+  API_BEGIN
+  Context& c = C(ctx);
+  require(m != nullptr, "null moment matrix");
+  const auto& cfg = deref(config, "config");
+  const auto& f = deref(fluid, "fluid");
+  const auto& w = deref(walls, "walls");
+  m->config = cfg;
+  m->prepare(f, w, deref(fn, "fluid neighbors"), deref(wn, "wall neighbors"), generation);
+  ppe_assemble(c, cfg, f, w, *fn, *wn, deref(tmp, "temporary velocity"),
+               deref(tmpw, "temporary wall velocity"), m->velocity_ops(), m->pressure_ops(),
+               deref(ws, "workspace"));
+  c.finish();
+  API_END("lsmps_ppe_assemble")
+}
+
+int lsmps_default_solver_options(lsmps_solver_options* out) {
+  API_BEGIN
+  require(out != nullptr, "null options");
+  lsmps_solver_options o{};
+  o.method = LSMPS_SOLVER_IDRS;
+  o.max_iterations = 50000;
+  o.tolerance = 1.0e-10;
+  o.inner_tolerance = 1.0e-4;
+  o.check_interval = 16;
+  o.clamp_negative = 1;
+  o.shadow_dimension = 8;
+  *out = o;
+  API_END("lsmps_default_solver_options")
+}
+
+namespace {
+thread_local PpeSolver* g_solver = nullptr;
+}
+
+int lsmps_ppe_solve(lsmps_context* ctx, const lsmps_csr* matrix, const double* rhs,
+                    double* pressure, const lsmps_solver_options* options,
+                    lsmps_solve_report* report) {
+  API_BEGIN
+  Context& c = C(ctx);
+  lsmps_solver_options opt{};
+  if (options) {
+    opt = *options;
+  } else {
+    lsmps_default_solver_options(&opt);
+  }
+  // The free solve keeps one workspace per host thread.
+  if (!g_solver) g_solver = new PpeSolver();
+  ppe_solve(c, *g_solver, deref(matrix, "matrix"), rhs, pressure, opt, report);
+  API_END("lsmps_ppe_solve")
+}
+
+// ---- correction -------------------------------------------------------------
+int lsmps_correction_apply(lsmps_context* ctx, const lsmps_config* config,
+                           const lsmps_fluid_soa* fluid, const lsmps_wall_soa* walls,
+                           const lsmps_neighbor_list* fn, const lsmps_neighbor_list* wn,
+                           const lsmps_fluid_soa* tmp, double* pressure, lsmps_moment_matrix* m,
+                           uint64_t generation, const lsmps_vector3* grad,
+                           const lsmps_vector3* ps, const lsmps_vector3* smoothed) {
+  API_BEGIN
+  Context& c = C(ctx);
+  require(m != nullptr, "null moment matrix");
+  const auto& cfg = deref(config, "config");
+  const auto& f = deref(fluid, "fluid");
+  const auto& w = deref(walls, "walls");
+  const auto& fnl = deref(fn, "fluid neighbors");
+  const auto& wnl = deref(wn, "wall neighbors");
+  clamp_pressure(c, f, pressure);
+  m->config = cfg;
+  m->prepare(f, w, fnl, wnl, generation);
+  pressure_gradient(c, cfg, f, w, fnl, wnl, pressure, m->pressure_ops(), deref(grad, "gradient"));
+  particle_shifting(c, cfg, f, w, fnl, wnl, deref(ps, "shifting"));
+  update_velocity_position(c, cfg, f, deref(tmp, "temporary velocity"), *grad, *ps);
+  anti_penetration(c, cfg, f, w, wnl);
+  smooth_velocity(c, cfg, f, fnl, deref(smoothed, "smoothed velocity"));
+  c.finish();
+  API_END("lsmps_correction_apply")
+}
+
+int lsmps_clamp_particle_pressure(lsmps_context* ctx, const lsmps_fluid_soa* fluid,
+                                  double* pressure) {
+  API_BEGIN
+  Context& c = C(ctx);
+  clamp_pressure(c, deref(fluid, "fluid"), pressure);
+  c.finish();
+  API_END("lsmps_clamp_particle_pressure")
+}
+
+int lsmps_compute_pressure_gradient(lsmps_context* ctx, const lsmps_fluid_soa* fluid,
+                                    const lsmps_wall_soa* walls, const lsmps_neighbor_list* fn,
+                                    const lsmps_neighbor_list* wn, const double* pressure,
+                                    const lsmps_config* config, lsmps_moment_matrix* m,
+                                    const lsmps_vector3* grad) {
+  API_BEGIN
+  Context& c = C(ctx);
+  require(m != nullptr, "null moment matrix");
+  require(m->prepared, "Pressure correction requires a prepared pressure Type-A moment matrix",
+          LSMPS_ERR_NOT_PREPARED);
+  require(m->count >= deref(fluid, "fluid").count,
+          "Pressure correction requires a prepared pressure Type-A moment matrix");
+  pressure_gradient(c, deref(config, "config"), *fluid, deref(walls, "walls"),
+                    deref(fn, "fluid neighbors"), deref(wn, "wall neighbors"), pressure,
+                    m->pressure_ops(), deref(grad, "gradient"));
+  c.finish();
+  API_END("lsmps_compute_pressure_gradient")
+}
+
+int lsmps_compute_particle_shifting(lsmps_context* ctx, const lsmps_fluid_soa* fluid,
+                                    const lsmps_wall_soa* walls, const lsmps_neighbor_list* fn,
+                                    const lsmps_neighbor_list* wn, const lsmps_config* config,
+                                    const lsmps_vector3* ps) {
+  API_BEGIN
+  Context& c = C(ctx);
+  particle_shifting(c, deref(config, "config"), deref(fluid, "fluid"), deref(walls, "walls"),
+                    deref(fn, "fluid neighbors"), deref(wn, "wall neighbors"),
+                    deref(ps, "shifting"));
+  c.finish();
+  API_END("lsmps_compute_particle_shifting")
+}
+
+int lsmps_update_velocity_and_position(lsmps_context* ctx, const lsmps_fluid_soa* fluid,
+                                       const lsmps_fluid_soa* tmp, const lsmps_config* config,
+                                       const lsmps_vector3* grad, const lsmps_vector3* ps) {
+  API_BEGIN
+  Context& c = C(ctx);
+  update_velocity_position(c, deref(config, "config"), deref(fluid, "fluid"),
+                           deref(tmp, "temporary velocity"), deref(grad, "gradient"),
+                           deref(ps, "shifting"));
+  c.finish();
+  API_END("lsmps_update_velocity_and_position")
+}
+
+int lsmps_apply_anti_penetration(lsmps_context* ctx, const lsmps_fluid_soa* fluid,
+                                 const lsmps_wall_soa* walls, const lsmps_neighbor_list* wn,
+                                 const lsmps_config* config) {
+  API_BEGIN
+  Context& c = C(ctx);
+  anti_penetration(c, deref(config, "config"), deref(fluid, "fluid"), deref(walls, "walls"),
+                   deref(wn, "wall neighbors"));
+  c.finish();
+  API_END("lsmps_apply_anti_penetration")
+}
+
+int lsmps_smooth_particle_velocity(lsmps_context* ctx, const lsmps_fluid_soa* fluid,
+                                   const lsmps_neighbor_list* fn, const lsmps_config* config,
+                                   const lsmps_vector3* smoothed) {
+  API_BEGIN
+  Context& c = C(ctx);
+  smooth_velocity(c, deref(config, "config"), deref(fluid, "fluid"), deref(fn, "fluid neighbors"),
+                  deref(smoothed, "smoothed velocity"));
+  c.finish();
+  API_END("lsmps_smooth_particle_velocity")
+}
+
+// ---- stepper ----------------------------------------------------------------
+int lsmps_stepper_create(lsmps_context* ctx, size_t fluid_capacity, size_t wall_capacity,
+                         size_t max_neighbors, const lsmps_config* config, lsmps_stepper** out) {
+  API_BEGIN
+  require(out != nullptr, "null output pointer");
+  *out = nullptr;
+  Context& c = C(ctx);
+  require(max_neighbors > 0, "max_neighbors_per_particle must be positive");
+  const auto& cfg = deref(config, "config");
+  auto* s = new lsmps_stepper();
+  try {
+    s->ctx = &c;
+    s->config = cfg;
+    s->max_neighbors = max_neighbors;
+    s->moments.ctx = &c;
+    lsmps_default_solver_options(&s->solver_options);
+    const size_t cells = static_cast<size_t>(cfg.cell_dims[0]) * cfg.cell_dims[1] * cfg.cell_dims[2];
+    s->allocate(fluid_capacity, wall_capacity, cells);
+  } catch (...) {
+    delete s;
+    throw;
+  }
+  *out = s;
+  API_END("lsmps_stepper_create")
+}
+
+int lsmps_stepper_destroy(lsmps_stepper* s) {
+  API_BEGIN
+  if (s) {
+    (void)hipSetDevice(s->ctx->device);
+    (void)hipStreamSynchronize(s->ctx->stream);
+    delete s;
+  }
+  API_END("lsmps_stepper_destroy")
+}
+
+int lsmps_stepper_set_config(lsmps_stepper* s, const lsmps_config* config) {
+  API_BEGIN
+  require(s != nullptr, "null stepper");
+  s->config = deref(config, "config");
+  s->moments.prepared = false;
+  API_END("lsmps_stepper_set_config")
+}
+
+int lsmps_stepper_set_solver(lsmps_stepper* s, const lsmps_solver_options* options) {
+  API_BEGIN
+  require(s != nullptr, "null stepper");
+  s->solver_options = deref(options, "options");
+  API_END("lsmps_stepper_set_solver")
+}
+
+size_t lsmps_stepper_bytes(const lsmps_stepper* s) { return s ? s->bytes() : 0; }
+
+int lsmps_stepper_step(lsmps_stepper* s, const lsmps_fluid_soa* fluid, const lsmps_wall_soa* walls,
+                       lsmps_solve_report* report) {
+  API_BEGIN
+  require(s != nullptr, "null stepper");
+  s->ctx->activate();
+  int rc = LSMPS_OK;
+  try {
+    s->step(deref(fluid, "fluid"), deref(walls, "walls"));
+  } catch (const Error& e) {
+    if (e.code != LSMPS_ERR_NOT_CONVERGED) throw;
+    g_last_error = std::string("lsmps_stepper_step: ") + e.what();
+    rc = e.code;
+  }
+  if (report) *report = s->last_report;
+  return rc;
+  API_END("lsmps_stepper_step")
+}
+
+int lsmps_stepper_stage_times(const lsmps_stepper* s, double* ms, int count) {
+  API_BEGIN
+  require(s != nullptr && ms != nullptr, "null argument");
+  for (int k = 0; k < count && k < 7; ++k) ms[k] = s->stage_ms[k];
+  API_END("lsmps_stepper_stage_times")
+}
+
+int lsmps_stepper_neighbor_lists(lsmps_stepper* s, lsmps_neighbor_list* fn,
+                                 lsmps_neighbor_list* wn) {
+  API_BEGIN
+  require(s != nullptr, "null stepper");
+  if (fn) *fn = s->fluid_neighbors();
+  if (wn) *wn = s->wall_neighbors();
+  API_END("lsmps_stepper_neighbor_lists")
+}
+
+int lsmps_stepper_ppe_workspace(lsmps_stepper* s, lsmps_ppe_workspace* out) {
+  API_BEGIN
+  require(s != nullptr && out != nullptr, "null argument");
+  *out = s->ppe_workspace(s->moments.count);
+  API_END("lsmps_stepper_ppe_workspace")
+}
+
+int lsmps_stepper_temporary_velocity(lsmps_stepper* s, lsmps_fluid_soa* f, lsmps_wall_soa* w) {
+  API_BEGIN
+  require(s != nullptr, "null stepper");
+  const size_t nf1 = std::max<size_t>(s->fluid_capacity, 1);
+  const size_t nw1 = std::max<size_t>(s->wall_capacity, 1);
+  double* tv = s->tmp_fluid.data();
+  double* tw = s->tmp_wall.data();
+  if (f) *f = lsmps_fluid_soa{s->moments.count, nullptr, nullptr, nullptr, tv, tv + nf1,
+                              tv + 2 * nf1, nullptr, nullptr};
+  if (w) *w = lsmps_wall_soa{s->wall_capacity, nullptr, nullptr, nullptr, tw, tw + nw1,
+                             tw + 2 * nw1, nullptr, nullptr, nullptr};
+  API_END("lsmps_stepper_temporary_velocity")
+}
+
+int lsmps_stepper_correction_vectors(lsmps_stepper* s, lsmps_vector3* grad, lsmps_vector3* ps) {
+  API_BEGIN
+  require(s != nullptr, "null stepper");
+  const size_t nf1 = std::max<size_t>(s->fluid_capacity, 1);
+  double* cv = s->corr.data();
+  const size_t n = s->moments.count;
+  if (grad) *grad = lsmps_vector3{n, cv, cv + nf1, cv + 2 * nf1};
+  if (ps) *ps = lsmps_vector3{n, cv + 3 * nf1, cv + 4 * nf1, cv + 5 * nf1};
+  API_END("lsmps_stepper_correction_vectors")
+}
+
+lsmps_moment_matrix* lsmps_stepper_moments(lsmps_stepper* s) { return s ? &s->moments : nullptr; }
+
+}  // extern "C"

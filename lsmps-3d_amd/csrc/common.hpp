@@ -1,0 +1,132 @@
+// Shared host/device infrastructure of the gfx950 LSMPS library: error
+// propagation (C++ exceptions inside, status codes at the C-ABI), RAII device
+// arrays, the context (device, stream, precision, scratch) and launch helpers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <utility>
+
+#include "lsmps3d_amd.h"
+
+namespace lsmps {
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+[[noreturn]] inline void raise(int code, const std::string& message) { throw Error(code, message); }
+
+inline void require(bool ok, const char* message, int code = LSMPS_ERR_INVALID_ARGUMENT) {
+  if (!ok) raise(code, message);
+}
+
+#define LSMPS_HIP(call)                                                                   \
+  do {                                                                                    \
+    hipError_t lsmps_e_ = (call);                                                         \
+    if (lsmps_e_ != hipSuccess)                                                           \
+      ::lsmps::raise(lsmps_e_ == hipErrorOutOfMemory ? LSMPS_ERR_OUT_OF_MEMORY            \
+                                                     : LSMPS_ERR_DEVICE,                  \
+                     std::string(#call) + ": " + hipGetErrorString(lsmps_e_));            \
+  } while (0)
+
+#define LSMPS_LAUNCH_CHECK() LSMPS_HIP(hipGetLastError())
+
+// Move-only owner of one device allocation.  Reallocates only when the
+// requested element count grows (reserve) or changes (resize).
+template <typename T>
+class DeviceArray {
+ public:
+  DeviceArray() = default;
+  explicit DeviceArray(size_t n) { resize(n); }
+  ~DeviceArray() { release(); }
+  DeviceArray(const DeviceArray&) = delete;
+  DeviceArray& operator=(const DeviceArray&) = delete;
+  DeviceArray(DeviceArray&& o) noexcept : ptr_(o.ptr_), n_(o.n_) {
+    o.ptr_ = nullptr;
+    o.n_ = 0;
+  }
+  DeviceArray& operator=(DeviceArray&& o) noexcept {
+    if (this != &o) {
+      release();
+      std::swap(ptr_, o.ptr_);
+      std::swap(n_, o.n_);
+    }
+    return *this;
+  }
+  void resize(size_t n) {
+    if (n == n_) return;
+    release();
+    if (n > 0) LSMPS_HIP(hipMalloc(reinterpret_cast<void**>(&ptr_), n * sizeof(T)));
+    n_ = n;
+  }
+  void reserve(size_t n) {
+    if (n > n_) resize(n);
+  }
+  void release() noexcept {
+    if (ptr_) (void)hipFree(ptr_);
+    ptr_ = nullptr;
+    n_ = 0;
+  }
+  T* data() const { return ptr_; }
+  size_t size() const { return n_; }
+  size_t bytes() const { return n_ * sizeof(T); }
+
+ private:
+  T* ptr_ = nullptr;
+  size_t n_ = 0;
+};
+
+constexpr int kBlock = 256;
+
+inline unsigned blocks_for(size_t n, int block = kBlock) {
+  const size_t b = (n + static_cast<size_t>(block) - 1) / static_cast<size_t>(block);
+  return static_cast<unsigned>(b > 0 ? b : 1);
+}
+
+// Fill kernels replace hipMemsetAsync so that every clear is an ordinary
+// kernel node if the stream is captured into a graph.
+void fill_i32(hipStream_t s, int32_t* p, size_t n, int32_t v);
+void fill_f64(hipStream_t s, double* p, size_t n, double v);
+void fill_f32(hipStream_t s, float* p, size_t n, float v);
+void copy_f64(hipStream_t s, double* dst, const double* src, size_t n);
+
+// Device-wide exclusive scan of n int32 values into out[0..n]; out[n] = total.
+// `in` and `out` may alias.  Uses ctx scratch.
+struct Context;
+void exclusive_scan_i32(Context& ctx, const int32_t* in, int32_t* out, size_t n);
+
+struct Context {
+  int device = 0;
+  int precision = LSMPS_PRECISION_FP64;
+  uint32_t flags = 0;
+  hipStream_t stream = nullptr;  // null = legacy default stream of `device`
+  // scratch owned by the context
+  DeviceArray<int32_t> scan_partials;
+  DeviceArray<int32_t> tmp_counts;
+  DeviceArray<int32_t> tmp_cursor;
+  DeviceArray<int32_t> tmp_labels;   // unsorted cell id per particle
+  DeviceArray<int32_t> tmp_primary;  // primary surface labels before the near-surface pass
+  DeviceArray<char> tmp_points;      // sorted source points for the neighbor scan
+  int32_t* host_i32 = nullptr;   // pinned host scratch
+  double* host_f64 = nullptr;
+
+  void activate() const { LSMPS_HIP(hipSetDevice(device)); }
+  void finish() const {
+    if (!(flags & LSMPS_FLAG_ASYNC)) LSMPS_HIP(hipStreamSynchronize(stream));
+  }
+  void sync() const { LSMPS_HIP(hipStreamSynchronize(stream)); }
+};
+
+inline double config_radius(const lsmps_config& c) { return c.support_radius; }
+
+}  // namespace lsmps
+
+struct lsmps_context {
+  lsmps::Context impl;
+};

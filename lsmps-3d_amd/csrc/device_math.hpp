@@ -1,0 +1,75 @@
+// Device-side LSMPS building blocks shared by the stage kernels: weight
+// kernels, the Type-A basis and the wall Neumann vector.  Each mirrors the
+// formula of the reference function cited beside it, in the same operation
+// order, so FP64 results track the oracle to rounding.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "lsmps3d_amd.h"
+
+namespace lsmps {
+
+// particle_weight (surface_detection.cu:26-31) / moment_weight (moment_matrix.cu:145-150) /
+// provision_weight (explicit_update.cu:121-126): zero at r >= re.
+template <typename T>
+__device__ __forceinline__ T weight_of(T distance, T re, int kernel) {
+  if (distance >= re) return T(0);
+  const T t = T(1) - distance / re;
+  return kernel == LSMPS_WEIGHT_QUADRATIC ? t * t : t;
+}
+
+// correction_weight (pressure_correction.cu:127-132): also zero at r <= 0.
+template <typename T>
+__device__ __forceinline__ T correction_weight_of(T distance, T re, int kernel) {
+  if (distance >= re || distance <= T(0)) return T(0);
+  const T t = T(1) - distance / re;
+  return kernel == LSMPS_WEIGHT_QUADRATIC ? t * t : t;
+}
+
+// Type-A basis with s = d * (1/re): provision_type_a_basis (explicit_update.cu:128-146),
+// ppe_type_a_basis (ppe_matrix.cu:155-173), correction_type_a_basis (pressure_correction.cu:138-156).
+template <typename T>
+__device__ __forceinline__ void type_a_basis(T dx, T dy, T dz, T inv_re, T b[9]) {
+  const T sx = dx * inv_re, sy = dy * inv_re, sz = dz * inv_re;
+  b[0] = sx;
+  b[1] = sy;
+  b[2] = sz;
+  b[3] = sx * sx;
+  b[4] = sy * sy;
+  b[5] = sz * sz;
+  b[6] = sx * sy;
+  b[7] = sy * sz;
+  b[8] = sz * sx;
+}
+
+// Wall Neumann vector q of Type-A (wall_neumann_vector, moment_matrix.cu:177-201;
+// ppe_wall_neumann_vector, ppe_matrix.cu:175-193).
+template <typename T>
+__device__ __forceinline__ void wall_neumann_q(T dx, T dy, T dz, T nx, T ny, T nz, T inv_re,
+                                               T q[9]) {
+  q[0] = nx;
+  q[1] = ny;
+  q[2] = nz;
+  q[3] = T(2) * dx * nx * inv_re;
+  q[4] = T(2) * dy * ny * inv_re;
+  q[5] = T(2) * dz * nz * inv_re;
+  q[6] = (dy * nx + dx * ny) * inv_re;
+  q[7] = (dz * ny + dy * nz) * inv_re;
+  q[8] = (dx * nz + dz * nx) * inv_re;
+}
+
+// Operator rows kept per particle and moment kind (SoA, index k*N + i):
+//   k = 0..8   row 0 of M^-1 (x-gradient coefficients)
+//   k = 9..17  row 1 (y-gradient)
+//   k = 18..26 row 2 (z-gradient)
+//   k = 27..35 rows 3+4+5 summed (Laplacian coefficients)
+// These are the only parts of the inverse the consumers read:
+// explicit_update.cu:164-176 (rows 3..5), ppe_matrix.cu:203-215 (3+4+5) and
+// :355-361 (rows 0..2), pressure_correction.cu:268-275 (rows 0..2).
+constexpr int kOpRows = 36;
+constexpr int kOpLap = 27;
+
+}  // namespace lsmps

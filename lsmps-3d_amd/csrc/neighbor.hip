@@ -1,0 +1,460 @@
+// Neighbor search on gfx950: cell list + fluid-fluid / fluid-wall CSR tables.
+//
+// Replaces build_cell_list / build_neighbor_lists (src/neighbor/neighbor_search.cu:255-367).
+// Output is bit-identical to a stable sort of the cell ids and a 27-cell scan in
+// (dz, dy, dx) order, within a cell in ascending particle index:
+//   1. cell id per particle (FP64, same operations as clamp_cell_index/flatten_cell),
+//      per-cell counts with atomics;
+//   2. device-wide exclusive scan -> cell starts;
+//   3. scatter with per-cell atomic cursors, then each cell's slice is sorted
+//      ascending (the slot order from atomics is arbitrary, the sorted slice is
+//      what a stable sort produces because input order = index order);
+//   4. sorted source points packed as 32-byte {x, y, z, index} records so a
+//      candidate costs one contiguous load;
+//   5. count pass, scan, capacity check, write pass.  Rows are visited in the
+//      fluid cell order so a wavefront's 64 rows share the same neighbor cells.
+// Compiled with -ffp-contract=off: the distance test must round exactly like
+// the oracle's.
+#include "common.hpp"
+#include "stages.hpp"
+
+namespace lsmps {
+namespace {
+
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kBlock * kScanItems;  // 2048
+
+struct alignas(32) SortedPoint {
+  double x, y, z;
+  int32_t index;
+  int32_t pad;
+};
+
+struct GridDev {
+  double ox, oy, oz, h;
+  int32_t nx, ny, nz;
+};
+
+__global__ void fill_i32_kernel(int32_t* p, size_t n, int32_t v) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+__global__ void fill_f64_kernel(double* p, size_t n, double v) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+__global__ void fill_f32_kernel(float* p, size_t n, float v) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+__global__ void copy_f64_kernel(double* dst, const double* src, size_t n) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
+// ---- block scan helpers (64-lane wavefronts, 256-thread blocks) ----------
+__device__ __forceinline__ int32_t wave_inclusive_scan(int32_t v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int32_t t = __shfl_up(v, off, 64);
+    if (lane >= off) v += t;
+  }
+  return v;
+}
+
+// Exclusive scan over the block; returns the prefix of this thread, total in `total`.
+__device__ __forceinline__ int32_t block_exclusive_scan(int32_t v, int32_t& total) {
+  __shared__ int32_t wave_sums[kBlock / 64];
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int32_t inc = wave_inclusive_scan(v);
+  if (lane == 63) wave_sums[wave] = inc;
+  __syncthreads();
+  int32_t offset = 0, sum = 0;
+#pragma unroll
+  for (int w = 0; w < kBlock / 64; ++w) {
+    const int32_t s = wave_sums[w];
+    offset += (w < wave) ? s : 0;
+    sum += s;
+  }
+  __syncthreads();
+  total = sum;
+  return inc - v + offset;
+}
+
+__global__ void __launch_bounds__(kBlock) scan_reduce_kernel(const int32_t* in, size_t n,
+                                                             int32_t* partials) {
+  const size_t base = static_cast<size_t>(blockIdx.x) * kScanTile;
+  int32_t s = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    const size_t idx = base + static_cast<size_t>(k) * kBlock + threadIdx.x;
+    if (idx < n) s += in[idx];
+  }
+  int32_t total;
+  (void)block_exclusive_scan(s, total);
+  if (threadIdx.x == 0) partials[blockIdx.x] = total;
+}
+
+// Scans one tile per block, adding the block's carry from `carries` (or 0).
+__global__ void __launch_bounds__(kBlock) scan_tile_kernel(const int32_t* in, int32_t* out,
+                                                           size_t n, const int32_t* carries,
+                                                           int write_total) {
+  __shared__ int32_t tile[kScanTile];
+  const size_t base = static_cast<size_t>(blockIdx.x) * kScanTile;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    const size_t idx = base + static_cast<size_t>(k) * kBlock + threadIdx.x;
+    tile[k * kBlock + threadIdx.x] = idx < n ? in[idx] : 0;
+  }
+  __syncthreads();
+  int32_t local[kScanItems];
+  int32_t s = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    local[k] = tile[threadIdx.x * kScanItems + k];
+    s += local[k];
+  }
+  int32_t total;
+  int32_t prefix = block_exclusive_scan(s, total);
+  const int32_t carry = carries ? carries[blockIdx.x] : 0;
+  prefix += carry;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    tile[threadIdx.x * kScanItems + k] = prefix;
+    prefix += local[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    const size_t idx = base + static_cast<size_t>(k) * kBlock + threadIdx.x;
+    if (idx < n) out[idx] = tile[k * kBlock + threadIdx.x];
+  }
+  if (write_total && blockIdx.x == gridDim.x - 1 && threadIdx.x == kBlock - 1) {
+    out[n] = carry + total;
+  }
+}
+
+// ---- cell list ------------------------------------------------------------
+
+// clamp_cell_index (neighbor_search.cu:36-46)
+__device__ __forceinline__ int32_t clamp_cell(double coordinate, double origin, double h,
+                                              int32_t dim) {
+  const double scaled = (coordinate - origin) / h;
+  const int32_t index = static_cast<int32_t>(floor(scaled));
+  if (index < 0) return 0;
+  if (index >= dim) return dim - 1;
+  return index;
+}
+
+// flatten_cell (neighbor_search.cu:32-34)
+__device__ __forceinline__ int32_t flatten(int32_t ix, int32_t iy, int32_t iz, const GridDev& g) {
+  return (iz * g.ny + iy) * g.nx + ix;
+}
+
+// assign_cells_kernel (neighbor_search.cu:61-73) + per-cell counting.
+__global__ void assign_cells_kernel(size_t n, const double* __restrict__ x,
+                                    const double* __restrict__ y, const double* __restrict__ z,
+                                    GridDev g, int32_t* __restrict__ cell_of,
+                                    int32_t* __restrict__ counts) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int32_t ix = clamp_cell(x[i], g.ox, g.h, g.nx);
+  const int32_t iy = clamp_cell(y[i], g.oy, g.h, g.ny);
+  const int32_t iz = clamp_cell(z[i], g.oz, g.h, g.nz);
+  const int32_t c = flatten(ix, iy, iz, g);
+  cell_of[i] = c;
+  atomicAdd(&counts[c], 1);
+}
+
+__global__ void scatter_cells_kernel(size_t n, const int32_t* __restrict__ cell_of,
+                                     const int32_t* __restrict__ starts,
+                                     int32_t* __restrict__ cursor,
+                                     int32_t* __restrict__ sorted) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int32_t c = cell_of[i];
+  const int32_t slot = atomicAdd(&cursor[c], 1);
+  sorted[starts[c] + slot] = static_cast<int32_t>(i);
+}
+
+// Sorts each cell's slice ascending and writes cell_ids / cell_begin / cell_end
+// (build_cell_ranges_kernel, neighbor_search.cu:85-98).
+__global__ void finalize_cells_kernel(size_t cells, const int32_t* __restrict__ starts,
+                                      int32_t* __restrict__ sorted,
+                                      int32_t* __restrict__ cell_ids,
+                                      int32_t* __restrict__ begin, int32_t* __restrict__ end) {
+  const size_t c = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (c >= cells) return;
+  const int32_t b = starts[c];
+  const int32_t e = starts[c + 1];
+  if (b == e) return;  // ranges were filled with -1
+  for (int32_t p = b + 1; p < e; ++p) {
+    const int32_t v = sorted[p];
+    int32_t q = p - 1;
+    while (q >= b && sorted[q] > v) {
+      sorted[q + 1] = sorted[q];
+      --q;
+    }
+    sorted[q + 1] = v;
+  }
+  for (int32_t p = b; p < e; ++p) cell_ids[p] = static_cast<int32_t>(c);
+  begin[c] = b;
+  end[c] = e;
+}
+
+__global__ void gather_points_kernel(size_t n, const int32_t* __restrict__ sorted,
+                                     const double* __restrict__ x, const double* __restrict__ y,
+                                     const double* __restrict__ z,
+                                     SortedPoint* __restrict__ points) {
+  const size_t p = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int32_t j = sorted[p];
+  SortedPoint sp;
+  sp.x = x[j];
+  sp.y = y[j];
+  sp.z = z[j];
+  sp.index = j;
+  sp.pad = 0;
+  points[p] = sp;
+}
+
+// count_neighbors_kernel / write_neighbors_kernel (neighbor_search.cu:221-253) with
+// count/write_source_neighbors_for_fluid (:100-219) inlined.  Row t of the launch
+// handles fluid particle order[t] (cell order) so neighboring rows share cells.
+template <bool kWrite>
+__global__ void __launch_bounds__(kBlock) neighbor_scan_kernel(
+    size_t n_fluid, const int32_t* __restrict__ order, const double* __restrict__ fx,
+    const double* __restrict__ fy, const double* __restrict__ fz, GridDev g,
+    const int32_t* __restrict__ src_begin, const int32_t* __restrict__ src_end,
+    const SortedPoint* __restrict__ src, double radius_squared, int exclude_self,
+    int32_t* __restrict__ offsets, int32_t* __restrict__ indices) {
+  const size_t t = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= n_fluid) return;
+  const int32_t i = order ? order[t] : static_cast<int32_t>(t);
+  const double px = fx[i];
+  const double py = fy[i];
+  const double pz = fz[i];
+  const int32_t cx = clamp_cell(px, g.ox, g.h, g.nx);
+  const int32_t cy = clamp_cell(py, g.oy, g.h, g.ny);
+  const int32_t cz = clamp_cell(pz, g.oz, g.h, g.nz);
+  int32_t count = 0;
+  int32_t out = kWrite ? offsets[i] : 0;
+  for (int32_t dz = -1; dz <= 1; ++dz) {
+    const int32_t nz = cz + dz;
+    if (nz < 0 || nz >= g.nz) continue;
+    for (int32_t dy = -1; dy <= 1; ++dy) {
+      const int32_t ny = cy + dy;
+      if (ny < 0 || ny >= g.ny) continue;
+      for (int32_t dx = -1; dx <= 1; ++dx) {
+        const int32_t nx = cx + dx;
+        if (nx < 0 || nx >= g.nx) continue;
+        const int32_t cell = flatten(nx, ny, nz, g);
+        const int32_t b = src_begin[cell];
+        const int32_t e = src_end[cell];
+        if (b < 0 || e < 0) continue;
+        for (int32_t p = b; p < e; ++p) {
+          const SortedPoint q = src[p];
+          if (exclude_self && q.index == i) continue;
+          // within_radius (neighbor_search.cu:48-59)
+          const double ddx = px - q.x;
+          const double ddy = py - q.y;
+          const double ddz = pz - q.z;
+          if (ddx * ddx + ddy * ddy + ddz * ddz <= radius_squared) {
+            if (kWrite) {
+              indices[out++] = q.index;
+            } else {
+              ++count;
+            }
+          }
+        }
+      }
+    }
+  }
+  if (!kWrite) offsets[i] = count;
+}
+
+GridDev to_dev(const lsmps_cell_grid& g) {
+  GridDev d;
+  d.ox = g.origin[0];
+  d.oy = g.origin[1];
+  d.oz = g.origin[2];
+  d.h = g.cell_size;
+  d.nx = g.dims[0];
+  d.ny = g.dims[1];
+  d.nz = g.dims[2];
+  return d;
+}
+
+}  // namespace
+
+void fill_i32(hipStream_t s, int32_t* p, size_t n, int32_t v) {
+  if (n == 0) return;
+  fill_i32_kernel<<<blocks_for(n), kBlock, 0, s>>>(p, n, v);
+  LSMPS_LAUNCH_CHECK();
+}
+void fill_f64(hipStream_t s, double* p, size_t n, double v) {
+  if (n == 0) return;
+  fill_f64_kernel<<<blocks_for(n), kBlock, 0, s>>>(p, n, v);
+  LSMPS_LAUNCH_CHECK();
+}
+void fill_f32(hipStream_t s, float* p, size_t n, float v) {
+  if (n == 0) return;
+  fill_f32_kernel<<<blocks_for(n), kBlock, 0, s>>>(p, n, v);
+  LSMPS_LAUNCH_CHECK();
+}
+void copy_f64(hipStream_t s, double* dst, const double* src, size_t n) {
+  if (n == 0) return;
+  copy_f64_kernel<<<blocks_for(n), kBlock, 0, s>>>(dst, src, n);
+  LSMPS_LAUNCH_CHECK();
+}
+
+namespace {
+// Recursive three-phase scan: tile sums, scan of the sums, tile scans with carries.
+void scan_level(Context& ctx, const int32_t* in, int32_t* out, size_t n, int32_t* scratch,
+                bool write_total) {
+  const size_t tiles = (n + kScanTile - 1) / kScanTile;
+  if (tiles <= 1) {
+    scan_tile_kernel<<<1, kBlock, 0, ctx.stream>>>(in, out, n, nullptr, write_total ? 1 : 0);
+    LSMPS_LAUNCH_CHECK();
+    return;
+  }
+  int32_t* partials = scratch;
+  int32_t* next = scratch + tiles + 1;
+  scan_reduce_kernel<<<static_cast<unsigned>(tiles), kBlock, 0, ctx.stream>>>(in, n, partials);
+  LSMPS_LAUNCH_CHECK();
+  scan_level(ctx, partials, partials, tiles, next, false);
+  scan_tile_kernel<<<static_cast<unsigned>(tiles), kBlock, 0, ctx.stream>>>(
+      in, out, n, partials, write_total ? 1 : 0);
+  LSMPS_LAUNCH_CHECK();
+}
+}  // namespace
+
+void exclusive_scan_i32(Context& ctx, const int32_t* in, int32_t* out, size_t n) {
+  if (n == 0) {
+    fill_i32(ctx.stream, out, 1, 0);
+    return;
+  }
+  size_t need = 0;
+  for (size_t m = n; m > 1;) {
+    const size_t tiles = (m + kScanTile - 1) / kScanTile;
+    need += tiles + 1;
+    if (tiles <= 1) break;
+    m = tiles;
+  }
+  ctx.scan_partials.reserve(need + 16);
+  scan_level(ctx, in, out, n, ctx.scan_partials.data(), true);
+}
+
+void build_cell_list(Context& ctx, size_t count, const double* x, const double* y,
+                     const double* z, const lsmps_cell_grid& grid, const lsmps_cell_list& cells) {
+  // validation messages follow build_cell_list_impl (neighbor_search.cu:257-273)
+  require(grid.cell_size > 0.0 && grid.dims[0] > 0 && grid.dims[1] > 0 && grid.dims[2] > 0,
+          "Invalid cell grid configuration");
+  const int64_t cell_count = static_cast<int64_t>(grid.dims[0]) * grid.dims[1] * grid.dims[2];
+  require(cell_count == static_cast<int64_t>(cells.cell_count),
+          "Cell list capacity does not match grid cell count");
+  require(count <= cells.particle_count,
+          "Cell list particle capacity is smaller than particle count");
+  require(count <= static_cast<size_t>(INT32_MAX), "Particle count exceeds 32-bit index capacity");
+  require(cell_count < static_cast<int64_t>(INT32_MAX), "Cell count exceeds 32-bit index capacity");
+  const size_t cc = static_cast<size_t>(cell_count);
+  fill_i32(ctx.stream, cells.cell_begin, cc, -1);
+  fill_i32(ctx.stream, cells.cell_end, cc, -1);
+  if (count == 0) return;
+  require(x && y && z && cells.cell_ids && cells.sorted_indices && cells.cell_begin && cells.cell_end,
+          "Cell list requires position and cell buffers");
+  const GridDev g = to_dev(grid);
+  ctx.tmp_counts.reserve(cc + 1);
+  ctx.tmp_cursor.reserve(cc + 1);
+  ctx.tmp_labels.reserve(count);
+  int32_t* counts = ctx.tmp_counts.data();
+  int32_t* starts = ctx.tmp_cursor.data();
+  int32_t* cell_of = ctx.tmp_labels.data();
+  fill_i32(ctx.stream, counts, cc, 0);
+  assign_cells_kernel<<<blocks_for(count), kBlock, 0, ctx.stream>>>(count, x, y, z, g, cell_of,
+                                                                    counts);
+  LSMPS_LAUNCH_CHECK();
+  exclusive_scan_i32(ctx, counts, starts, cc);
+  fill_i32(ctx.stream, counts, cc, 0);
+  scatter_cells_kernel<<<blocks_for(count), kBlock, 0, ctx.stream>>>(count, cell_of, starts,
+                                                                     counts, cells.sorted_indices);
+  LSMPS_LAUNCH_CHECK();
+  finalize_cells_kernel<<<blocks_for(cc), kBlock, 0, ctx.stream>>>(
+      cc, starts, cells.sorted_indices, cells.cell_ids, cells.cell_begin, cells.cell_end);
+  LSMPS_LAUNCH_CHECK();
+}
+
+namespace {
+// build_one_neighbor_list (neighbor_search.cu:294-333)
+void one_neighbor_list(Context& ctx, const lsmps_fluid_soa& fluid, size_t source_count,
+                       const double* sx, const double* sy, const double* sz,
+                       const lsmps_cell_list& source_cells, const lsmps_cell_list& fluid_cells,
+                       const lsmps_cell_grid& grid, double radius, bool exclude_self,
+                       const lsmps_neighbor_list& nbrs, int32_t* device_total) {
+  require(fluid.count <= nbrs.particle_count,
+          "Neighbor list particle capacity is smaller than fluid count");
+  require(nbrs.offsets != nullptr, "Neighbor list requires an offsets buffer");
+  const double radius_squared = radius * radius;
+  if (fluid.count == 0) {
+    fill_i32(ctx.stream, nbrs.offsets, 1, 0);
+    return;
+  }
+  const GridDev g = to_dev(grid);
+  ctx.tmp_points.reserve(std::max<size_t>(source_count, 1) * sizeof(SortedPoint));
+  SortedPoint* points = reinterpret_cast<SortedPoint*>(ctx.tmp_points.data());
+  if (source_count > 0) {
+    gather_points_kernel<<<blocks_for(source_count), kBlock, 0, ctx.stream>>>(
+        source_count, source_cells.sorted_indices, sx, sy, sz, points);
+    LSMPS_LAUNCH_CHECK();
+  }
+  const int32_t* order = fluid_cells.sorted_indices;
+  neighbor_scan_kernel<false><<<blocks_for(fluid.count), kBlock, 0, ctx.stream>>>(
+      fluid.count, order, fluid.x, fluid.y, fluid.z, g, source_cells.cell_begin,
+      source_cells.cell_end, points, radius_squared, exclude_self ? 1 : 0, nbrs.offsets, nullptr);
+  LSMPS_LAUNCH_CHECK();
+  exclusive_scan_i32(ctx, nbrs.offsets, nbrs.offsets, fluid.count);
+  // Capacity check (require_neighbor_capacity, neighbor_search.cu:22-30,323-328).
+  LSMPS_HIP(hipMemcpyAsync(ctx.host_i32, nbrs.offsets + fluid.count, sizeof(int32_t),
+                           hipMemcpyDeviceToHost, ctx.stream));
+  ctx.sync();
+  const int64_t required = ctx.host_i32[0];
+  if (device_total) *device_total = static_cast<int32_t>(required);
+  if (required < 0 || static_cast<size_t>(required) > nbrs.neighbor_count)
+    raise(LSMPS_ERR_CAPACITY, "Neighbor list capacity exceeded: required " +
+                                  std::to_string(required) + ", capacity " +
+                                  std::to_string(nbrs.neighbor_count));
+  require(required == 0 || nbrs.indices != nullptr, "Neighbor list requires an indices buffer");
+  neighbor_scan_kernel<true><<<blocks_for(fluid.count), kBlock, 0, ctx.stream>>>(
+      fluid.count, order, fluid.x, fluid.y, fluid.z, g, source_cells.cell_begin,
+      source_cells.cell_end, points, radius_squared, exclude_self ? 1 : 0, nbrs.offsets,
+      nbrs.indices);
+  LSMPS_LAUNCH_CHECK();
+}
+}  // namespace
+
+void build_neighbor_lists(Context& ctx, const lsmps_fluid_soa& fluid, const lsmps_wall_soa& walls,
+                          const lsmps_config& config, const lsmps_cell_list& fluid_cells,
+                          const lsmps_cell_list& wall_cells,
+                          const lsmps_neighbor_list& fluid_neighbors,
+                          const lsmps_neighbor_list& wall_neighbors, int32_t* fluid_total,
+                          int32_t* wall_total) {
+  lsmps_cell_grid grid;
+  for (int k = 0; k < 3; ++k) {
+    grid.origin[k] = config.cell_origin[k];
+    grid.dims[k] = config.cell_dims[k];
+  }
+  grid.cell_size = config.cell_size;
+  const double radius = config.support_radius;
+  require(radius > 0.0, "Neighbor search radius must be positive");
+  require(radius <= grid.cell_size, "Neighbor search currently requires radius <= cell_size");
+  build_cell_list(ctx, fluid.count, fluid.x, fluid.y, fluid.z, grid, fluid_cells);
+  build_cell_list(ctx, walls.count, walls.x, walls.y, walls.z, grid, wall_cells);
+  one_neighbor_list(ctx, fluid, fluid.count, fluid.x, fluid.y, fluid.z, fluid_cells, fluid_cells,
+                    grid, radius, true, fluid_neighbors, fluid_total);
+  one_neighbor_list(ctx, fluid, walls.count, walls.x, walls.y, walls.z, wall_cells, fluid_cells,
+                    grid, radius, false, wall_neighbors, wall_total);
+}
+
+}  // namespace lsmps

@@ -1,0 +1,97 @@
+// Opaque objects behind the C-ABI: the moment-matrix cache and the fused stepper.
+#pragma once
+
+#include "common.hpp"
+#include "solver.hpp"
+#include "stages.hpp"
+
+// DeviceMomentMatrix (include/lsmps3d/moment_matrix/moment_matrix.cuh:36-68).
+// The step path keeps only the operator rows of the two consumed kinds; full
+// inverses of any kind are materialised on request for MomentMatrixView parity
+// from the inputs of the last prepare (which must not have changed since).
+struct lsmps_moment_matrix {
+  lsmps::Context* ctx = nullptr;
+  lsmps_config config{};
+  size_t capacity = 0;
+  size_t count = 0;
+  uint64_t generation = 0;
+  bool prepared = false;
+  lsmps::DeviceArray<double> vel_ops;  // kOpRows x capacity
+  lsmps::DeviceArray<double> pre_ops;
+  struct Full {
+    lsmps::DeviceArray<double> inverse;
+    lsmps::DeviceArray<double> trace;
+    lsmps::DeviceArray<double> regularization;
+    lsmps::DeviceArray<int32_t> info;
+    uint64_t generation = 0;
+    bool ready = false;
+  } full[4];
+  lsmps::DeviceArray<int32_t> inversion_count;  // 4 counters, one per kind
+  int32_t host_counts[4] = {0, 0, 0, 0};
+  lsmps_fluid_soa fluid{};
+  lsmps_wall_soa walls{};
+  lsmps_neighbor_list fn{};
+  lsmps_neighbor_list wn{};
+
+  size_t bytes() const {
+    size_t b = vel_ops.bytes() + pre_ops.bytes() + inversion_count.bytes();
+    for (const auto& f : full)
+      b += f.inverse.bytes() + f.trace.bytes() + f.regularization.bytes() + f.info.bytes();
+    return b;
+  }
+  void resize(size_t cap) {
+    if (cap != capacity) {
+      vel_ops.release();
+      pre_ops.release();
+      for (auto& f : full) {
+        f.inverse.release();
+        f.trace.release();
+        f.regularization.release();
+        f.info.release();
+        f.ready = false;
+      }
+      capacity = cap;
+      prepared = false;
+    }
+  }
+  void prepare(const lsmps_fluid_soa& fluid_in, const lsmps_wall_soa& walls_in,
+               const lsmps_neighbor_list& fn_in, const lsmps_neighbor_list& wn_in,
+               uint64_t gen);
+  void materialize(int kind);
+  const double* velocity_ops() const;
+  const double* pressure_ops() const;
+};
+
+// Owns every intermediate buffer of one time step.
+struct lsmps_stepper {
+  lsmps::Context* ctx = nullptr;
+  lsmps_config config{};
+  lsmps_solver_options solver_options{};
+  size_t fluid_capacity = 0;
+  size_t wall_capacity = 0;
+  size_t max_neighbors = 0;
+  size_t cell_capacity = 0;
+  uint64_t generation = 0;
+  lsmps::DeviceArray<int32_t> fc_ids, fc_sorted, fc_begin, fc_end;
+  lsmps::DeviceArray<int32_t> wc_ids, wc_sorted, wc_begin, wc_end;
+  lsmps::DeviceArray<int32_t> f_off, f_idx, w_off, w_idx;
+  lsmps_moment_matrix moments;
+  lsmps::DeviceArray<double> tmp_fluid;  // 3 x fluid capacity
+  lsmps::DeviceArray<double> tmp_wall;   // 3 x wall capacity
+  lsmps::DeviceArray<int32_t> ppe_rows, ppe_cols;
+  lsmps::DeviceArray<double> ppe_vals, ppe_vec;  // ppe_vec: rhs, pressure, divergence, wall term
+  lsmps::PpeSolver solver;
+  lsmps::DeviceArray<double> corr;  // grad, ps, smoothed: 9 x fluid capacity
+  hipEvent_t events[8] = {};
+  int n_events = 0;
+  double stage_ms[7] = {0, 0, 0, 0, 0, 0, 0};
+  lsmps_solve_report last_report{};
+
+  ~lsmps_stepper();
+  void allocate(size_t nf, size_t nw, size_t cells);
+  size_t bytes() const;
+  void step(const lsmps_fluid_soa& fluid, const lsmps_wall_soa& walls);
+  lsmps_neighbor_list fluid_neighbors() const;
+  lsmps_neighbor_list wall_neighbors() const;
+  lsmps_ppe_workspace ppe_workspace(size_t n) const;
+};

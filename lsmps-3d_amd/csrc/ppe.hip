@@ -1,0 +1,227 @@
+// PPE assembly on gfx950.
+//
+// Replaces assemble_ppe_matrix_and_rhs (src/ppe/ppe_matrix.cu:517-551):
+// build_ppe_row_offsets_kernel (:133-146) and assemble_ppe_system_kernel
+// (:243-370).  Row layout and values follow the reference exactly: row i
+// starts at i + fluid_offsets[i], the diagonal comes first, then one entry per
+// fluid neighbor in CSR order (value 0 when the weight vanishes), Dirichlet rows
+// (Surface / Splash) are e_i with their off-diagonal columns kept at value 0.
+// The pressure Laplacian row (rows 3+4+5 of the pressure Type-A inverse) and
+// rows 0..2 of the velocity inverse come from the operator rows.
+#include "common.hpp"
+#include "device_math.hpp"
+#include "stages.hpp"
+
+namespace lsmps {
+namespace {
+
+struct PpeArgs {
+  size_t n;
+  const double* x;
+  const double* y;
+  const double* z;
+  const int32_t* surface_type;
+  const double* tvx;
+  const double* tvy;
+  const double* tvz;
+  const double* wx;
+  const double* wy;
+  const double* wz;
+  const double* wnx;
+  const double* wny;
+  const double* wnz;
+  const double* twx;
+  const double* twy;
+  const double* twz;
+  const int32_t* f_off;
+  const int32_t* f_idx;
+  const int32_t* w_off;
+  const int32_t* w_idx;
+  const double* vel_ops;
+  const double* pre_ops;
+  double re;
+  double density;
+  double dt;
+  double wall_scale;
+  double g[3];
+  int32_t kernel;
+  int32_t* row_offsets;
+  int32_t* cols;
+  double* vals;
+  double* rhs;
+  double* divergence;
+  double* wall_term;
+};
+
+__global__ void row_offsets_kernel(size_t n, const int32_t* f_off, int32_t* row_offsets) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i > n) return;
+  row_offsets[i] = static_cast<int32_t>(i) + f_off[i];
+}
+
+__global__ void __launch_bounds__(kBlock) assemble_kernel(PpeArgs a) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  const int32_t row_begin = static_cast<int32_t>(i) + a.f_off[i];
+  const int32_t nb = a.f_off[i], ne = a.f_off[i + 1];
+  const int32_t st = a.surface_type ? a.surface_type[i] : LSMPS_SURFACE_INNER;
+  if (st == LSMPS_SURFACE_SURFACE || st == LSMPS_SURFACE_SPLASH) {
+    a.cols[row_begin] = static_cast<int32_t>(i);
+    a.vals[row_begin] = 1.0;
+    for (int32_t c = nb; c < ne; ++c) {
+      const int32_t out = row_begin + 1 + c - nb;
+      a.cols[out] = a.f_idx[c];
+      a.vals[out] = 0.0;
+    }
+    a.rhs[i] = 0.0;
+    a.divergence[i] = 0.0;
+    if (a.wall_term) a.wall_term[i] = 0.0;
+    return;
+  }
+  double L[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) L[k] = a.pre_ops[static_cast<size_t>(kOpLap + k) * a.n + i];
+  const double xi = a.x[i], yi = a.y[i], zi = a.z[i];
+  const double ui = a.tvx[i], vi = a.tvy[i], wi = a.tvz[i];
+  const double inv_re = 1.0 / a.re;
+  const double coef = 2.0 / (a.re * a.re * a.density);
+  double rx[9], ry[9], rz[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    rx[k] = 0.0;
+    ry[k] = 0.0;
+    rz[k] = 0.0;
+  }
+  double diagonal = 0.0;
+  double b[9];
+  for (int32_t c = nb; c < ne; ++c) {
+    const int32_t j = a.f_idx[c];
+    const double dx = a.x[j] - xi, dy = a.y[j] - yi, dz = a.z[j] - zi;
+    const double w = weight_of(sqrt(dx * dx + dy * dy + dz * dz), a.re, a.kernel);
+    double off = 0.0;
+    if (w > 0.0) {
+      type_a_basis(dx, dy, dz, inv_re, b);
+      double lb = 0.0;
+#pragma unroll
+      for (int k = 0; k < 9; ++k) lb += L[k] * b[k];
+      off = coef * w * lb;
+      diagonal -= off;
+      const double du = a.tvx[j] - ui, dv = a.tvy[j] - vi, dw = a.tvz[j] - wi;
+#pragma unroll
+      for (int k = 0; k < 9; ++k) {
+        const double wb = w * b[k];
+        rx[k] += wb * du;
+        ry[k] += wb * dv;
+        rz[k] += wb * dw;
+      }
+    }
+    const int32_t out = row_begin + 1 + c - nb;
+    a.cols[out] = j;
+    a.vals[out] = off;
+  }
+  double wall_rhs = 0.0;
+  double q[9];
+  for (int32_t c = a.w_off[i]; c < a.w_off[i + 1]; ++c) {
+    const int32_t j = a.w_idx[c];
+    const double dx = a.wx[j] - xi, dy = a.wy[j] - yi, dz = a.wz[j] - zi;
+    const double bw = weight_of(sqrt(dx * dx + dy * dy + dz * dz), a.re, a.kernel);
+    if (bw <= 0.0) continue;
+    type_a_basis(dx, dy, dz, inv_re, b);
+    const double w = a.wall_scale * bw;
+    const double du = a.twx[j] - ui, dv = a.twy[j] - vi, dw = a.twz[j] - wi;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      const double wb = w * b[k];
+      rx[k] += wb * du;
+      ry[k] += wb * dv;
+      rz[k] += wb * dw;
+    }
+    const double nx = a.wnx[j], ny = a.wny[j], nz = a.wnz[j];
+    const double ng = a.g[0] * nx + a.g[1] * ny + a.g[2] * nz;
+    wall_neumann_q(dx, dy, dz, nx, ny, nz, inv_re, q);
+    double lq = 0.0;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) lq += L[k] * q[k];
+    wall_rhs += 2.0 * a.wall_scale * bw * ng * lq / a.re;
+  }
+  // divergence from velocity rows 0..2 (first_coefficient_from_inverse_rhs, ppe_matrix.cu:233-241)
+  double d0 = 0.0, d1 = 0.0, d2 = 0.0;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    d0 += a.vel_ops[static_cast<size_t>(k) * a.n + i] * rx[k];
+    d1 += a.vel_ops[static_cast<size_t>(9 + k) * a.n + i] * ry[k];
+    d2 += a.vel_ops[static_cast<size_t>(18 + k) * a.n + i] * rz[k];
+  }
+  const double divergence = (d0 + d1 + d2) / a.re;
+  a.cols[row_begin] = static_cast<int32_t>(i);
+  a.vals[row_begin] = diagonal;
+  a.divergence[i] = divergence;
+  a.rhs[i] = divergence / a.dt - wall_rhs;
+  if (a.wall_term) a.wall_term[i] = wall_rhs;
+}
+
+}  // namespace
+
+void ppe_assemble(Context& ctx, const lsmps_config& cfg, const lsmps_fluid_soa& fluid,
+                  const lsmps_wall_soa& walls, const lsmps_neighbor_list& fn,
+                  const lsmps_neighbor_list& wn, const lsmps_fluid_soa& tmp,
+                  const lsmps_wall_soa& tmpw, const double* vel_ops, const double* pre_ops,
+                  const lsmps_ppe_workspace& ws) {
+  // validate_matrix_workspace / validate_rhs_workspace (ppe_matrix.cu:62-106)
+  require(ws.matrix.rows >= fluid.count && ws.matrix.cols >= fluid.count &&
+              ws.matrix.nnz >= fluid.count + fn.neighbor_count,
+          "PPE matrix workspace is smaller than the requested fluid system");
+  if (fluid.count == 0) return;
+  require(ws.matrix.row_offsets && ws.matrix.col_indices && ws.matrix.values && ws.rhs &&
+              ws.pressure && ws.divergence,
+          "PPE workspace requires matrix, rhs, pressure, and diagnostic buffers");
+  require(fn.particle_count >= fluid.count && fn.offsets,
+          "PPE assembly requires a valid fluid-neighbor CSR table");
+  require(tmp.vx && tmp.vy && tmp.vz,
+          "PPE RHS assembly requires temporary velocity, divergence and rhs buffers");
+  require(tmpw.count == 0 || (tmpw.vx && tmpw.vy && tmpw.vz),
+          "PPE RHS assembly requires temporary wall velocity buffers");
+  row_offsets_kernel<<<blocks_for(fluid.count + 1), kBlock, 0, ctx.stream>>>(
+      fluid.count, fn.offsets, ws.matrix.row_offsets);
+  LSMPS_LAUNCH_CHECK();
+  PpeArgs a;
+  a.n = fluid.count;
+  a.x = fluid.x;
+  a.y = fluid.y;
+  a.z = fluid.z;
+  a.surface_type = fluid.surface_type;
+  a.tvx = tmp.vx;
+  a.tvy = tmp.vy;
+  a.tvz = tmp.vz;
+  a.wx = walls.x;
+  a.wy = walls.y;
+  a.wz = walls.z;
+  a.wnx = walls.normal_x;
+  a.wny = walls.normal_y;
+  a.wnz = walls.normal_z;
+  a.twx = tmpw.vx;
+  a.twy = tmpw.vy;
+  a.twz = tmpw.vz;
+  a.f_off = fn.offsets;
+  a.f_idx = fn.indices;
+  a.w_off = wn.offsets;
+  a.w_idx = wn.indices;
+  a.vel_ops = vel_ops;
+  a.pre_ops = pre_ops;
+  a.re = cfg.support_radius;
+  a.density = cfg.density;
+  a.dt = cfg.time_step;
+  a.wall_scale = cfg.lsmps_wall_weight_scale;
+  for (int k = 0; k < 3; ++k) a.g[k] = cfg.gravity[k];
+  a.kernel = cfg.weight_kernel;
+  a.row_offsets = ws.matrix.row_offsets;
+  a.cols = ws.matrix.col_indices;
+  a.vals = ws.matrix.values;
+  a.rhs = ws.rhs;
+  a.divergence = ws.divergence;
+  a.wall_term = ws.pressure_laplacian;
+  assemble_kernel<<<blocks_for(fluid.count), kBlock, 0, ctx.stream>>>(a);
+  LSMPS_LAUNCH_CHECK();
+}
+
+}  // namespace lsmps

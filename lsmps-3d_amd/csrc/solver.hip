@@ -1,0 +1,732 @@
+// PPE solve on gfx950: replaces AmgxPpeSolver::solve (src/ppe/ppe_matrix.cu:586-679).
+//
+// The assembled PPE matrix is non-symmetric and, near wall corners of the free
+// surface, indefinite (NearSurface rows with a positive diagonal), so the
+// default method is Jacobi-preconditioned IDR(s) with biorthogonalisation
+// (Sonneveld & van Gijzen, SIAM J. Sci. Comput. 31, 2008; van Gijzen &
+// Sonneveld, ACM TOMS 38, 2011), written from the published algorithm:
+// short recurrences, s+1 SpMVs per cycle, s-dimensional shadow space.  IDR(1)
+// is mathematically BiCGStab, which LSMPS_SOLVER_BICGSTAB selects.
+// Jacobi-PCG (Hestenes-Stiefel) is kept for symmetric systems.
+//
+// GPU structure:
+//  * SpMV: 16 lanes per CSR row, grid-stride over rows, fused epilogue dots
+//    (shadow-space products P^T (A u), or t.r / t.t) into per-block partials.
+//  * all s x s algebra (M, f, c, alphas, beta, omega) lives in a device struct
+//    updated by one-block "scalar" kernels that reduce the partials in a fixed
+//    order: no host round trip per iteration and bit-reproducible reductions.
+//    The host reads one double per cycle to test convergence.
+//  * FP64 mode: IDR(s) on the FP64 matrix, FP64 vectors; at convergence of the
+//    recursive residual the true FP64 residual b - A x is recomputed and must
+//    meet the tolerance.
+//  * mixed mode: defect correction.  Outer loop in FP64 (r = b - A x with the
+//    FP64 matrix, x += e); inner IDR(s) on an FP32 copy of the values with FP32
+//    vectors and FP64 dot accumulation, to `inner_tolerance`.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "common.hpp"
+#include "solver.hpp"
+#include "stages.hpp"
+
+namespace lsmps {
+
+PpeSolver::~PpeSolver() {
+  if (host) (void)hipHostFree(host);
+}
+
+namespace {
+
+constexpr int kLanes = 16;
+constexpr uint32_t kShadowSeed = 0x5eedu;
+
+__device__ __forceinline__ double hashed_sign(uint32_t row, int q, uint32_t seed) {
+  uint32_t h = row * 0x9E3779B1u + static_cast<uint32_t>(q) * 0x85EBCA77u + seed;
+  h ^= h >> 16;
+  h *= 0x7FEB352Du;
+  h ^= h >> 15;
+  h *= 0x846CA68Bu;
+  h ^= h >> 16;
+  return (h & 1u) ? 1.0 : -1.0;
+}
+
+// Shadow vector q: P_0 = initial residual (stored), P_q (q >= 1) random signs.
+template <typename T>
+__device__ __forceinline__ double shadow(const T* p0, size_t row, int q) {
+  return q == 0 ? static_cast<double>(p0[row]) : hashed_sign(static_cast<uint32_t>(row), q, kShadowSeed);
+}
+
+template <int NQ>
+__device__ __forceinline__ void block_reduce_store(const double (&acc)[NQ], int nq,
+                                                   double* partials) {
+  __shared__ double red[kBlock / 64][kPartialStride];
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    if (q < nq) {
+      double v = acc[q];
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+      if (lane == 0) red[wave][q] = v;
+    }
+  }
+  __syncthreads();
+  if (static_cast<int>(threadIdx.x) < nq) {
+    double s = 0.0;
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; ++w) s += red[w][threadIdx.x];
+    partials[static_cast<size_t>(blockIdx.x) * kPartialStride + threadIdx.x] = s;
+  }
+}
+
+// Fixed-order sum of partials[b][q] over b < nb (one block).  Result in all threads.
+__device__ double block_sum_partials(const double* partials, int nb, int q) {
+  __shared__ double red[kBlock];
+  double s = 0.0;
+  for (int b = threadIdx.x; b < nb; b += kBlock) s += partials[static_cast<size_t>(b) * kPartialStride + q];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if (static_cast<int>(threadIdx.x) < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  const double total = red[0];
+  __syncthreads();
+  return total;
+}
+
+template <typename T>
+struct SpmvArgs {
+  size_t n;
+  const int32_t* rows;
+  const int32_t* cols;
+  const T* vals;
+  const T* x;
+  T* y;
+  const T* p0;  // mode 1
+  int s;        // mode 1
+  const T* r;   // mode 2
+  double* partials;
+};
+
+// MODE 0: y = A x.  1: + shadow dots P_q . y.  2: + y.r, y.y.  3: + y.x (CG p.Ap).
+template <typename T, int MODE>
+__global__ void __launch_bounds__(kBlock) spmv_kernel(SpmvArgs<T> a) {
+  constexpr int NQ = MODE == 1 ? kMaxShadow : (MODE == 2 ? 2 : 1);
+  const int lane = threadIdx.x & (kLanes - 1);
+  const size_t group = (static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x) / kLanes;
+  const size_t ngroups = static_cast<size_t>(gridDim.x) * blockDim.x / kLanes;
+  double acc[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) acc[q] = 0.0;
+  for (size_t row = group; row < a.n; row += ngroups) {
+    const int32_t b = a.rows[row];
+    const int32_t e = a.rows[row + 1];
+    T sum = T(0);
+    for (int32_t c = b + lane; c < e; c += kLanes) sum += a.vals[c] * a.x[a.cols[c]];
+#pragma unroll
+    for (int off = kLanes / 2; off > 0; off >>= 1) sum += __shfl_xor(sum, off, kLanes);
+    if (lane == 0) {
+      a.y[row] = sum;
+      const double ys = static_cast<double>(sum);
+      if (MODE == 1) {
+#pragma unroll
+        for (int q = 0; q < NQ; ++q)
+          if (q < a.s) acc[q] += shadow(a.p0, row, q) * ys;
+      } else if (MODE == 2) {
+        acc[0] += ys * static_cast<double>(a.r[row]);
+        acc[1 % NQ] += ys * ys;
+      } else if (MODE == 3) {
+        acc[0] += ys * static_cast<double>(a.x[row]);
+      }
+    }
+  }
+  if (MODE != 0) block_reduce_store<NQ>(acc, MODE == 1 ? a.s : (MODE == 2 ? 2 : 1), a.partials);
+}
+
+// r = b - A x in FP64; optional FP32 copy of r; partial r.r.
+__global__ void __launch_bounds__(kBlock) residual_kernel(size_t n, const int32_t* rows,
+                                                          const int32_t* cols, const double* vals,
+                                                          const double* x, const double* b,
+                                                          double* r, float* r32,
+                                                          double* partials) {
+  const int lane = threadIdx.x & (kLanes - 1);
+  const size_t group = (static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x) / kLanes;
+  const size_t ngroups = static_cast<size_t>(gridDim.x) * blockDim.x / kLanes;
+  double acc[1] = {0.0};
+  for (size_t row = group; row < n; row += ngroups) {
+    double sum = 0.0;
+    for (int32_t c = rows[row] + lane; c < rows[row + 1]; c += kLanes) sum += vals[c] * x[cols[c]];
+#pragma unroll
+    for (int off = kLanes / 2; off > 0; off >>= 1) sum += __shfl_xor(sum, off, kLanes);
+    if (lane == 0) {
+      const double rv = b[row] - sum;
+      if (r) r[row] = rv;
+      if (r32) r32[row] = static_cast<float>(rv);
+      acc[0] += rv * rv;
+    }
+  }
+  block_reduce_store<1>(acc, 1, partials);
+}
+
+template <typename T>
+__global__ void jacobi_kernel(size_t n, const int32_t* rows, const int32_t* cols,
+                              const double* vals, T* dinv) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double d = 0.0;
+  for (int32_t c = rows[i]; c < rows[i + 1]; ++c)
+    if (cols[c] == static_cast<int32_t>(i)) d += vals[c];
+  dinv[i] = static_cast<T>(d != 0.0 ? 1.0 / d : 1.0);
+}
+
+__global__ void to_f32_kernel(size_t n, const double* src, float* dst) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = static_cast<float>(src[i]);
+}
+
+__global__ void add_f32_to_f64_kernel(size_t n, const float* e, double* x) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) x[i] += static_cast<double>(e[i]);
+}
+
+template <typename T>
+__global__ void fill_kernel(size_t n, T* p, T v) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+template <typename T>
+__global__ void copy_kernel(size_t n, const T* src, T* dst) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
+__global__ void dot_self_kernel(size_t n, const double* v, double* partials) {
+  double acc[1] = {0.0};
+  for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<size_t>(gridDim.x) * blockDim.x)
+    acc[0] += v[i] * v[i];
+  block_reduce_store<1>(acc, 1, partials);
+}
+
+__global__ void clamp_kernel(size_t n, double* x) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n && x[i] < 0.0) x[i] = 0.0;
+}
+
+// ---- IDR(s) -------------------------------------------------------------
+
+template <typename T>
+struct IdrVec {
+  size_t n;
+  size_t ld;
+  int s;
+  T* r;
+  T* x;
+  T* v;
+  T* t;
+  T* G;
+  T* U;
+  const T* p0;
+  const T* dinv;
+  SolverScalars* sc;
+  double* partials;
+};
+
+__global__ void idr_init_scalars(SolverScalars* sc, int s) {
+  const int t = threadIdx.x;
+  for (int e = t; e < kMaxShadow * kMaxShadow; e += blockDim.x) {
+    const int i = e / kMaxShadow, k = e % kMaxShadow;
+    sc->M[e] = (i == k && i < s) ? 1.0 : 0.0;
+  }
+  if (t == 0) {
+    sc->om = 1.0;
+    sc->breakdown = 0;
+    sc->beta = 0.0;
+  }
+}
+
+// Lower-triangular solve M[k:, k:] c[k:] = f[k:] (single thread).
+__device__ void solve_c(SolverScalars* sc, int s, int k) {
+  for (int i = k; i < s; ++i) {
+    double t = sc->f[i];
+    for (int l = k; l < i; ++l) t -= sc->M[i * kMaxShadow + l] * sc->c[l];
+    const double d = sc->M[i * kMaxShadow + i];
+    sc->c[i] = d != 0.0 ? t / d : 0.0;
+    if (d == 0.0) sc->breakdown = 1;
+  }
+}
+
+// Cycle start: f = P^T r (from partials), c = M^-1 f.
+__global__ void idr_scalar_f(SolverScalars* sc, const double* partials, int nb, int s) {
+  for (int q = 0; q < s; ++q) {
+    const double v = block_sum_partials(partials, nb, q);
+    if (threadIdx.x == 0) sc->f[q] = v;
+  }
+  if (threadIdx.x == 0) solve_c(sc, s, 0);
+}
+
+// After G_k = A U_k: biorthogonalisation coefficients, new column k of M,
+// beta, f update and c for the next k.
+__global__ void idr_scalar_after(SolverScalars* sc, const double* partials, int nb, int s, int k) {
+  __shared__ double d[kMaxShadow];
+  for (int q = 0; q < s; ++q) {
+    const double v = block_sum_partials(partials, nb, q);
+    if (threadIdx.x == 0) d[q] = v;
+  }
+  if (threadIdx.x != 0) return;
+  double alpha[kMaxShadow];
+  for (int i = 0; i < k; ++i) {
+    double t = d[i];
+    for (int l = 0; l < i; ++l) t -= alpha[l] * sc->M[i * kMaxShadow + l];
+    const double mii = sc->M[i * kMaxShadow + i];
+    alpha[i] = mii != 0.0 ? t / mii : 0.0;
+    sc->alpha[i] = alpha[i];
+  }
+  for (int i = k; i < s; ++i) {
+    double t = d[i];
+    for (int l = 0; l < k; ++l) t -= alpha[l] * sc->M[i * kMaxShadow + l];
+    sc->M[i * kMaxShadow + k] = t;
+  }
+  const double mkk = sc->M[k * kMaxShadow + k];
+  if (mkk == 0.0 || !isfinite(mkk)) {
+    sc->breakdown = 1;
+    sc->beta = 0.0;
+  } else {
+    sc->beta = sc->f[k] / mkk;
+  }
+  for (int i = k + 1; i < s; ++i) sc->f[i] -= sc->beta * sc->M[i * kMaxShadow + k];
+  if (k + 1 < s) solve_c(sc, s, k + 1);
+}
+
+// omega = (t.r)/(t.t) with the 0.7 "maintaining the convergence" safeguard.
+__global__ void idr_scalar_omega(SolverScalars* sc, const double* part_t, const double* part_r,
+                                 int nb_t, int nb_r) {
+  const double tr = block_sum_partials(part_t, nb_t, 0);
+  const double tt = block_sum_partials(part_t, nb_t, 1);
+  const double rr = block_sum_partials(part_r, nb_r, 0);
+  if (threadIdx.x != 0) return;
+  if (!(tt > 0.0) || !(rr > 0.0)) {
+    sc->om = 0.0;
+    if (!(rr > 0.0)) return;  // already converged exactly
+    sc->breakdown = 1;
+    return;
+  }
+  double om = tr / tt;
+  const double rho = fabs(tr) / (sqrt(tt) * sqrt(rr));
+  if (rho < 0.7) om *= 0.7 / rho;
+  sc->om = om;
+}
+
+__global__ void scalar_norm(SolverScalars* sc, const double* partials, int nb) {
+  const double v = block_sum_partials(partials, nb, 0);
+  if (threadIdx.x == 0) sc->rr = v;
+}
+
+// v = D^-1 (r - sum_{q>=k} c_q G_q);  U_k = sum_{q>=k} c_q U_q + om v
+template <typename T>
+__global__ void __launch_bounds__(kBlock) idr_a_kernel(IdrVec<T> a, int k) {
+  const SolverScalars* sc = a.sc;
+  const double om = sc->om;
+  for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < a.n;
+       i += static_cast<size_t>(gridDim.x) * blockDim.x) {
+    double v = static_cast<double>(a.r[i]);
+    double u = 0.0;
+    for (int q = k; q < a.s; ++q) {
+      const double c = sc->c[q];
+      v -= c * static_cast<double>(a.G[q * a.ld + i]);
+      u += c * static_cast<double>(a.U[q * a.ld + i]);
+    }
+    v *= static_cast<double>(a.dinv[i]);
+    a.U[k * a.ld + i] = static_cast<T>(u + om * v);
+  }
+}
+
+// G_k, U_k biorthogonalised; r -= beta G_k; x += beta U_k; partial r.r;
+// optionally v = D^-1 r for the dimension-reduction step.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) idr_b_kernel(IdrVec<T> a, int k, int write_v) {
+  const SolverScalars* sc = a.sc;
+  const double beta = sc->beta;
+  double acc[1] = {0.0};
+  for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < a.n;
+       i += static_cast<size_t>(gridDim.x) * blockDim.x) {
+    double g = static_cast<double>(a.G[k * a.ld + i]);
+    double u = static_cast<double>(a.U[k * a.ld + i]);
+    for (int q = 0; q < k; ++q) {
+      const double al = sc->alpha[q];
+      g -= al * static_cast<double>(a.G[q * a.ld + i]);
+      u -= al * static_cast<double>(a.U[q * a.ld + i]);
+    }
+    a.G[k * a.ld + i] = static_cast<T>(g);
+    a.U[k * a.ld + i] = static_cast<T>(u);
+    const double r = static_cast<double>(a.r[i]) - beta * g;
+    a.r[i] = static_cast<T>(r);
+    a.x[i] = static_cast<T>(static_cast<double>(a.x[i]) + beta * u);
+    acc[0] += r * r;
+    if (write_v) a.v[i] = static_cast<T>(static_cast<double>(a.dinv[i]) * r);
+  }
+  block_reduce_store<1>(acc, 1, a.partials);
+}
+
+// x += om v; r -= om t; partial r.r
+template <typename T>
+__global__ void __launch_bounds__(kBlock) idr_c_kernel(IdrVec<T> a) {
+  const double om = a.sc->om;
+  double acc[1] = {0.0};
+  for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < a.n;
+       i += static_cast<size_t>(gridDim.x) * blockDim.x) {
+    a.x[i] = static_cast<T>(static_cast<double>(a.x[i]) + om * static_cast<double>(a.v[i]));
+    const double r = static_cast<double>(a.r[i]) - om * static_cast<double>(a.t[i]);
+    a.r[i] = static_cast<T>(r);
+    acc[0] += r * r;
+  }
+  block_reduce_store<1>(acc, 1, a.partials);
+}
+
+// f partials: P_q . r
+template <typename T>
+__global__ void __launch_bounds__(kBlock) shadow_dots_kernel(IdrVec<T> a) {
+  double acc[kMaxShadow];
+#pragma unroll
+  for (int q = 0; q < kMaxShadow; ++q) acc[q] = 0.0;
+  for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < a.n;
+       i += static_cast<size_t>(gridDim.x) * blockDim.x) {
+    const double r = static_cast<double>(a.r[i]);
+#pragma unroll
+    for (int q = 0; q < kMaxShadow; ++q)
+      if (q < a.s) acc[q] += shadow(a.p0, i, q) * r;
+  }
+  block_reduce_store<kMaxShadow>(acc, a.s, a.partials);
+}
+
+// ---- PCG ----------------------------------------------------------------
+__global__ void cg_scalar_rz(SolverScalars* sc, const double* part, int nb) {
+  const double rz = block_sum_partials(part, nb, 0);
+  const double rr = block_sum_partials(part, nb, 1);
+  if (threadIdx.x == 0) {
+    sc->cg_beta = sc->rz != 0.0 ? rz / sc->rz : 0.0;
+    sc->rz = rz;
+    sc->rr = rr;
+  }
+}
+__global__ void cg_scalar_alpha(SolverScalars* sc, const double* part, int nb) {
+  const double pq = block_sum_partials(part, nb, 0);
+  if (threadIdx.x == 0) {
+    if (pq == 0.0 || !isfinite(pq)) {
+      sc->breakdown = 1;
+      sc->cg_alpha = 0.0;
+    } else {
+      sc->cg_alpha = sc->rz / pq;
+    }
+  }
+}
+// x += alpha p; r -= alpha q; partials r.z, r.r
+__global__ void __launch_bounds__(kBlock) cg_update_kernel(size_t n, double* x, double* r,
+                                                           const double* p, const double* q,
+                                                           const double* dinv,
+                                                           const SolverScalars* sc,
+                                                           double* partials) {
+  const double alpha = sc->cg_alpha;
+  double acc[2] = {0.0, 0.0};
+  for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<size_t>(gridDim.x) * blockDim.x) {
+    x[i] += alpha * p[i];
+    const double rv = r[i] - alpha * q[i];
+    r[i] = rv;
+    acc[0] += rv * rv * dinv[i];
+    acc[1] += rv * rv;
+  }
+  block_reduce_store<2>(acc, 2, partials);
+}
+// p = D^-1 r + beta p  (first call with beta_zero: p = D^-1 r)
+__global__ void cg_direction_kernel(size_t n, double* p, const double* r, const double* dinv,
+                                    const SolverScalars* sc, int first) {
+  const double beta = first ? 0.0 : sc->cg_beta;
+  for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<size_t>(gridDim.x) * blockDim.x)
+    p[i] = dinv[i] * r[i] + (first ? 0.0 : beta * p[i]);
+}
+__global__ void rz_kernel(size_t n, const double* r, const double* dinv, double* partials) {
+  double acc[2] = {0.0, 0.0};
+  for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<size_t>(gridDim.x) * blockDim.x) {
+    acc[0] += r[i] * r[i] * dinv[i];
+    acc[1] += r[i] * r[i];
+  }
+  block_reduce_store<2>(acc, 2, partials);
+}
+
+// ---- host orchestration -------------------------------------------------
+
+unsigned vec_blocks(size_t n) {
+  const size_t b = (n + kBlock - 1) / kBlock;
+  return static_cast<unsigned>(std::max<size_t>(1, std::min<size_t>(b, kMaxReduceBlocks)));
+}
+unsigned spmv_blocks(size_t n) {
+  const size_t rows_per_block = kBlock / kLanes;
+  const size_t b = (n + rows_per_block - 1) / rows_per_block;
+  return static_cast<unsigned>(std::max<size_t>(1, std::min<size_t>(b, kMaxReduceBlocks)));
+}
+
+double read_rr(Context& ctx, PpeSolver& S) {
+  LSMPS_HIP(hipMemcpyAsync(S.host, &S.scalars.data()->rr, sizeof(double), hipMemcpyDeviceToHost,
+                           ctx.stream));
+  ctx.sync();
+  return S.host[0];
+}
+
+// FP64 true residual r = b - A x; returns ||r||^2.
+double true_residual(Context& ctx, PpeSolver& S, const lsmps_csr& A, const double* x,
+                     const double* b, double* r, float* r32) {
+  const size_t n = A.rows;
+  const unsigned nb = spmv_blocks(n);
+  residual_kernel<<<nb, kBlock, 0, ctx.stream>>>(n, A.row_offsets, A.col_indices, A.values, x, b,
+                                                 r, r32, S.part_b.data());
+  LSMPS_LAUNCH_CHECK();
+  scalar_norm<<<1, kBlock, 0, ctx.stream>>>(S.scalars.data(), S.part_b.data(), static_cast<int>(nb));
+  LSMPS_LAUNCH_CHECK();
+  return read_rr(ctx, S);
+}
+
+// IDR(s) on (rows, cols, vals) of type T.  x and r hold the initial iterate and
+// its residual.  Stops when ||r|| <= target (recursive residual); when `A64`
+// is given (FP64 mode) the true residual is recomputed and must agree.
+template <typename T>
+int idr_solve(Context& ctx, PpeSolver& S, size_t n, const int32_t* rows, const int32_t* cols,
+              const T* vals, const T* dinv, T* x, T* r, T* work, size_t ld, int s,
+              double rr0, double target, int max_iterations, const lsmps_csr* A64,
+              const double* b64, int* iterations_out, double* rr_out) {
+  T* v = work;
+  T* t = work + ld;
+  T* p0 = work + 2 * ld;
+  T* G = work + 3 * ld;
+  T* U = work + (3 + s) * ld;
+  IdrVec<T> iv{n, ld, s, r, x, v, t, G, U, p0, dinv, S.scalars.data(), S.part_b.data()};
+  const unsigned vb = vec_blocks(n);
+  const unsigned sb = spmv_blocks(n);
+  fill_kernel<T><<<blocks_for(2 * s * ld), kBlock, 0, ctx.stream>>>(2 * s * ld, G, T(0));
+  copy_kernel<T><<<blocks_for(n), kBlock, 0, ctx.stream>>>(n, r, p0);
+  idr_init_scalars<<<1, kBlock, 0, ctx.stream>>>(S.scalars.data(), s);
+  LSMPS_LAUNCH_CHECK();
+  double rr = rr0;
+  int it = 0;
+  const double target2 = target * target;
+  while (rr > target2 && it < max_iterations) {
+    iv.partials = S.part_a.data();
+    shadow_dots_kernel<T><<<vb, kBlock, 0, ctx.stream>>>(iv);
+    idr_scalar_f<<<1, kBlock, 0, ctx.stream>>>(S.scalars.data(), S.part_a.data(),
+                                               static_cast<int>(vb), s);
+    for (int k = 0; k < s; ++k) {
+      idr_a_kernel<T><<<vb, kBlock, 0, ctx.stream>>>(iv, k);
+      SpmvArgs<T> sa{n, rows, cols, vals, U + k * ld, G + k * ld, p0, s, nullptr, S.part_a.data()};
+      spmv_kernel<T, 1><<<sb, kBlock, 0, ctx.stream>>>(sa);
+      idr_scalar_after<<<1, kBlock, 0, ctx.stream>>>(S.scalars.data(), S.part_a.data(),
+                                                     static_cast<int>(sb), s, k);
+      iv.partials = S.part_b.data();
+      idr_b_kernel<T><<<vb, kBlock, 0, ctx.stream>>>(iv, k, k == s - 1 ? 1 : 0);
+      ++it;
+    }
+    SpmvArgs<T> st{n, rows, cols, vals, v, t, nullptr, s, r, S.part_a.data()};
+    spmv_kernel<T, 2><<<sb, kBlock, 0, ctx.stream>>>(st);
+    idr_scalar_omega<<<1, kBlock, 0, ctx.stream>>>(S.scalars.data(), S.part_a.data(), S.part_b.data(),
+                                                   static_cast<int>(sb), static_cast<int>(vb));
+    iv.partials = S.part_b.data();
+    idr_c_kernel<T><<<vb, kBlock, 0, ctx.stream>>>(iv);
+    scalar_norm<<<1, kBlock, 0, ctx.stream>>>(S.scalars.data(), S.part_b.data(), static_cast<int>(vb));
+    LSMPS_LAUNCH_CHECK();
+    ++it;
+    rr = read_rr(ctx, S);
+    LSMPS_HIP(hipMemcpyAsync(S.host + 2, &S.scalars.data()->breakdown, sizeof(int32_t),
+                             hipMemcpyDeviceToHost, ctx.stream));
+    ctx.sync();
+    int32_t breakdown = 0;
+    std::memcpy(&breakdown, S.host + 2, sizeof(int32_t));
+    if (!std::isfinite(rr)) break;
+    if (rr <= target2 && A64 != nullptr) {
+      // replace the recursive residual by the true FP64 residual
+      rr = true_residual(ctx, S, *A64, reinterpret_cast<const double*>(x), b64,
+                         reinterpret_cast<double*>(r), nullptr);
+    }
+    if (breakdown) {
+      fill_kernel<T><<<blocks_for(2 * s * ld), kBlock, 0, ctx.stream>>>(2 * s * ld, G, T(0));
+      copy_kernel<T><<<blocks_for(n), kBlock, 0, ctx.stream>>>(n, r, p0);
+      idr_init_scalars<<<1, kBlock, 0, ctx.stream>>>(S.scalars.data(), s);
+      LSMPS_LAUNCH_CHECK();
+    }
+  }
+  *iterations_out = it;
+  *rr_out = rr;
+  return rr <= target2 ? 1 : 0;
+}
+
+void pcg_solve(Context& ctx, PpeSolver& S, const lsmps_csr& A, const double* b, double* x,
+               double target, int max_iterations, int check, lsmps_solve_report& rep) {
+  const size_t n = A.rows;
+  const size_t ld = (n + 63) / 64 * 64;
+  S.vec64.reserve(3 * ld);
+  double* r = S.vec64.data();
+  double* p = r + ld;
+  double* q = p + ld;
+  const unsigned vb = vec_blocks(n), sb = spmv_blocks(n);
+  const double* dinv = S.dinv64.data();
+  double rr = true_residual(ctx, S, A, x, b, r, nullptr);
+  idr_init_scalars<<<1, kBlock, 0, ctx.stream>>>(S.scalars.data(), 1);
+  rz_kernel<<<vb, kBlock, 0, ctx.stream>>>(n, r, dinv, S.part_b.data());
+  cg_scalar_rz<<<1, kBlock, 0, ctx.stream>>>(S.scalars.data(), S.part_b.data(), static_cast<int>(vb));
+  cg_direction_kernel<<<vb, kBlock, 0, ctx.stream>>>(n, p, r, dinv, S.scalars.data(), 1);
+  LSMPS_LAUNCH_CHECK();
+  int it = 0;
+  const double target2 = target * target;
+  while (rr > target2 && it < max_iterations) {
+    SpmvArgs<double> sa{n, A.row_offsets, A.col_indices, A.values, p, q, nullptr, 0, nullptr,
+                        S.part_a.data()};
+    spmv_kernel<double, 3><<<sb, kBlock, 0, ctx.stream>>>(sa);
+    cg_scalar_alpha<<<1, kBlock, 0, ctx.stream>>>(S.scalars.data(), S.part_a.data(), static_cast<int>(sb));
+    cg_update_kernel<<<vb, kBlock, 0, ctx.stream>>>(n, x, r, p, q, dinv, S.scalars.data(),
+                                                    S.part_b.data());
+    cg_scalar_rz<<<1, kBlock, 0, ctx.stream>>>(S.scalars.data(), S.part_b.data(), static_cast<int>(vb));
+    cg_direction_kernel<<<vb, kBlock, 0, ctx.stream>>>(n, p, r, dinv, S.scalars.data(), 0);
+    LSMPS_LAUNCH_CHECK();
+    ++it;
+    if (it % check == 0 || it >= max_iterations) {
+      rr = read_rr(ctx, S);
+      if (!std::isfinite(rr)) break;
+      if (rr <= target2) rr = true_residual(ctx, S, A, x, b, r, nullptr);
+    }
+  }
+  rep.iterations = it;
+  rep.relative_residual = std::sqrt(rr);
+}
+
+}  // namespace
+
+void ppe_solve(Context& ctx, PpeSolver& S, const lsmps_csr& A, const double* b, double* x,
+               const lsmps_solver_options& opt, lsmps_solve_report* report) {
+  lsmps_solve_report rep{};
+  rep.method = opt.method;
+  const size_t n = A.rows;
+  require(A.rows == A.cols, "PPE solve requires a square CSR matrix, rhs, and solution buffer");
+  if (n == 0) {
+    rep.converged = 1;
+    if (report) *report = rep;
+    return;
+  }
+  require(A.row_offsets && A.col_indices && A.values && b && x,
+          "PPE solve requires a square CSR matrix, rhs, and solution buffer");
+  require(opt.tolerance > 0.0, "PPE solve tolerance must be positive");
+  require(n <= static_cast<size_t>(INT32_MAX), "PPE solve row count exceeds 32-bit indices");
+  if (!S.host) LSMPS_HIP(hipHostMalloc(reinterpret_cast<void**>(&S.host), 8 * sizeof(double)));
+  S.scalars.reserve(1);
+  S.part_a.reserve(static_cast<size_t>(kMaxReduceBlocks) * kPartialStride);
+  S.part_b.reserve(static_cast<size_t>(kMaxReduceBlocks) * kPartialStride);
+  S.dinv64.reserve(n);
+  jacobi_kernel<double><<<blocks_for(n), kBlock, 0, ctx.stream>>>(n, A.row_offsets, A.col_indices,
+                                                                   A.values, S.dinv64.data());
+  LSMPS_LAUNCH_CHECK();
+  // ||b||
+  dot_self_kernel<<<vec_blocks(n), kBlock, 0, ctx.stream>>>(n, b, S.part_b.data());
+  scalar_norm<<<1, kBlock, 0, ctx.stream>>>(S.scalars.data(), S.part_b.data(),
+                                            static_cast<int>(vec_blocks(n)));
+  LSMPS_LAUNCH_CHECK();
+  const double bnorm = std::sqrt(read_rr(ctx, S));
+  rep.rhs_norm = bnorm;
+  fill_kernel<double><<<blocks_for(n), kBlock, 0, ctx.stream>>>(n, x, 0.0);
+  LSMPS_LAUNCH_CHECK();
+  if (bnorm == 0.0) {
+    rep.converged = 1;
+    if (report) *report = rep;
+    ctx.finish();
+    return;
+  }
+  const double target = opt.tolerance * bnorm;
+  const int max_it = opt.max_iterations > 0 ? opt.max_iterations : 20000;
+  const size_t ld = (n + 63) / 64 * 64;
+  int shadow_dim = opt.method == LSMPS_SOLVER_BICGSTAB ? 1 : opt.shadow_dimension;
+  if (shadow_dim <= 0) shadow_dim = 8;
+  if (shadow_dim > kMaxShadow) shadow_dim = kMaxShadow;
+  if (static_cast<size_t>(shadow_dim) > n) shadow_dim = static_cast<int>(n);
+
+  if (opt.method == LSMPS_SOLVER_PCG) {
+    pcg_solve(ctx, S, A, b, x, target, max_it, opt.check_interval > 0 ? opt.check_interval : 16,
+              rep);
+    rep.relative_residual /= bnorm;
+    rep.converged = rep.relative_residual <= opt.tolerance ? 1 : 0;
+  } else if (ctx.precision == LSMPS_PRECISION_FP64) {
+    S.vec64.reserve((4 + 2 * static_cast<size_t>(shadow_dim)) * ld);
+    double* r = S.vec64.data();
+    double* work = r + ld;
+    copy_kernel<double><<<blocks_for(n), kBlock, 0, ctx.stream>>>(n, b, r);
+    LSMPS_LAUNCH_CHECK();
+    int it = 0;
+    double rr = 0.0;
+    idr_solve<double>(ctx, S, n, A.row_offsets, A.col_indices, A.values, S.dinv64.data(), x, r,
+                      work, ld, shadow_dim, bnorm * bnorm, target, max_it, &A, b, &it, &rr);
+    rep.iterations = it;
+    rep.refinements = 1;
+    rep.relative_residual = std::sqrt(rr) / bnorm;
+    rep.converged = rep.relative_residual <= opt.tolerance ? 1 : 0;
+  } else {
+    // mixed: FP32 values and vectors inside, FP64 defect correction outside
+    const size_t nnz = static_cast<size_t>(0);
+    (void)nnz;
+    LSMPS_HIP(hipMemcpyAsync(ctx.host_i32, A.row_offsets + n, sizeof(int32_t),
+                             hipMemcpyDeviceToHost, ctx.stream));
+    ctx.sync();
+    const size_t used = static_cast<size_t>(ctx.host_i32[0]);
+    S.vals32.reserve(std::max<size_t>(used, 1));
+    S.dinv32.reserve(n);
+    to_f32_kernel<<<blocks_for(used), kBlock, 0, ctx.stream>>>(used, A.values, S.vals32.data());
+    to_f32_kernel<<<blocks_for(n), kBlock, 0, ctx.stream>>>(n, S.dinv64.data(), S.dinv32.data());
+    LSMPS_LAUNCH_CHECK();
+    S.outer64.reserve(n);
+    S.vec32.reserve((5 + 2 * static_cast<size_t>(shadow_dim)) * ld);
+    double* r64 = S.outer64.data();
+    float* e = S.vec32.data();
+    float* r32 = e + ld;
+    float* work = r32 + ld;
+    double rr = true_residual(ctx, S, A, x, b, r64, r32);
+    const double inner_tol = opt.inner_tolerance > 0.0 ? opt.inner_tolerance : 1.0e-4;
+    int total = 0, outer = 0;
+    while (rr > target * target && total < max_it && outer < 64) {
+      fill_kernel<float><<<blocks_for(n), kBlock, 0, ctx.stream>>>(n, e, 0.0f);
+      LSMPS_LAUNCH_CHECK();
+      int it = 0;
+      double rr_inner = 0.0;
+      const double rnorm = std::sqrt(rr);
+      idr_solve<float>(ctx, S, n, A.row_offsets, A.col_indices, S.vals32.data(), S.dinv32.data(),
+                       e, r32, work, ld, shadow_dim, rr, std::max(inner_tol * rnorm, 0.5 * target),
+                       max_it - total, nullptr, nullptr, &it, &rr_inner);
+      total += it;
+      add_f32_to_f64_kernel<<<blocks_for(n), kBlock, 0, ctx.stream>>>(n, e, x);
+      LSMPS_LAUNCH_CHECK();
+      const double rr_new = true_residual(ctx, S, A, x, b, r64, r32);
+      ++outer;
+      if (!std::isfinite(rr_new)) {
+        rr = rr_new;
+        break;
+      }
+      rr = rr_new;
+      if (it == 0) break;
+    }
+    rep.iterations = total;
+    rep.refinements = outer;
+    rep.relative_residual = std::sqrt(rr) / bnorm;
+    rep.converged = rep.relative_residual <= opt.tolerance ? 1 : 0;
+  }
+  if (opt.clamp_negative) {
+    // clamp_pressure_kernel (ppe_matrix.cu:373-379)
+    clamp_kernel<<<blocks_for(n), kBlock, 0, ctx.stream>>>(n, x);
+    LSMPS_LAUNCH_CHECK();
+  }
+  if (report) *report = rep;
+  ctx.finish();
+  if (!rep.converged)
+    raise(LSMPS_ERR_NOT_CONVERGED, "PPE solver did not reach the tolerance (relative residual " +
+                                       std::to_string(rep.relative_residual) + ")");
+}
+
+}  // namespace lsmps

@@ -1,0 +1,82 @@
+// Stage entry points of the per-step pipeline, shared by the C-ABI (capi.cpp)
+// and the fused stepper (stepper.cpp).  All functions enqueue on ctx.stream and
+// throw lsmps::Error on failure.
+#pragma once
+
+#include "common.hpp"
+#include "device_math.hpp"
+
+namespace lsmps {
+
+// ---- neighbor.hip
+void build_cell_list(Context& ctx, size_t count, const double* x, const double* y,
+                     const double* z, const lsmps_cell_grid& grid, const lsmps_cell_list& cells);
+void build_neighbor_lists(Context& ctx, const lsmps_fluid_soa& fluid, const lsmps_wall_soa& walls,
+                          const lsmps_config& config, const lsmps_cell_list& fluid_cells,
+                          const lsmps_cell_list& wall_cells,
+                          const lsmps_neighbor_list& fluid_neighbors,
+                          const lsmps_neighbor_list& wall_neighbors, int32_t* fluid_total,
+                          int32_t* wall_total);
+
+// ---- surface.hip
+double reference_number_density(double l0, double re, int kernel);
+void classify_surface(Context& ctx, const lsmps_fluid_soa& fluid, const lsmps_wall_soa& walls,
+                      const lsmps_neighbor_list& fn, const lsmps_neighbor_list& wn,
+                      const lsmps_config& config, const lsmps_surface_diagnostics* diagnostics);
+
+// ---- moment.hip
+struct MomentOutputs {
+  double* ops = nullptr;             // kOpRows x N operator rows (Type-A kinds)
+  double* inverse = nullptr;         // N x n x n column-major
+  double* matrices = nullptr;        // N x n x n column-major
+  double* trace = nullptr;
+  double* regularization = nullptr;
+  int32_t* info = nullptr;
+};
+int moment_size(int kind);
+void compute_moments(Context& ctx, int kind, const lsmps_config& cfg, const lsmps_fluid_soa& fluid,
+                     const lsmps_wall_soa& walls, const lsmps_neighbor_list& fn,
+                     const lsmps_neighbor_list& wn, const MomentOutputs& out);
+
+// ---- provision.hip
+// velocity Laplacian (rows of the velocity Type-A operator) fused with the
+// explicit update u* = u + dt (nu_eff lap + g); `lap` may be null.
+void provision_fluid(Context& ctx, const lsmps_config& cfg, const lsmps_fluid_soa& fluid,
+                     const lsmps_wall_soa& walls, const lsmps_neighbor_list& fn,
+                     const lsmps_neighbor_list& wn, const double* vel_ops,
+                     const lsmps_fluid_soa& tmp, const lsmps_vector3* lap);
+void provision_combine(Context& ctx, const lsmps_config& cfg, const lsmps_fluid_soa& fluid,
+                       const lsmps_vector3& lap, const lsmps_fluid_soa& tmp);
+void provision_walls(Context& ctx, const lsmps_config& cfg, const lsmps_wall_soa& walls,
+                     const lsmps_wall_soa& tmp);
+
+// ---- ppe.hip
+void ppe_assemble(Context& ctx, const lsmps_config& cfg, const lsmps_fluid_soa& fluid,
+                  const lsmps_wall_soa& walls, const lsmps_neighbor_list& fn,
+                  const lsmps_neighbor_list& wn, const lsmps_fluid_soa& tmp,
+                  const lsmps_wall_soa& tmpw, const double* vel_ops, const double* pre_ops,
+                  const lsmps_ppe_workspace& ws);
+
+// ---- solver.hip
+class PpeSolver;  // owns Krylov workspaces; defined in solver.hpp
+void ppe_solve(Context& ctx, PpeSolver& solver, const lsmps_csr& a, const double* b, double* x,
+               const lsmps_solver_options& opt, lsmps_solve_report* report);
+
+// ---- correction.hip
+void clamp_pressure(Context& ctx, const lsmps_fluid_soa& fluid, double* pressure);
+void pressure_gradient(Context& ctx, const lsmps_config& cfg, const lsmps_fluid_soa& fluid,
+                       const lsmps_wall_soa& walls, const lsmps_neighbor_list& fn,
+                       const lsmps_neighbor_list& wn, const double* pressure,
+                       const double* pre_ops, const lsmps_vector3& grad);
+void particle_shifting(Context& ctx, const lsmps_config& cfg, const lsmps_fluid_soa& fluid,
+                       const lsmps_wall_soa& walls, const lsmps_neighbor_list& fn,
+                       const lsmps_neighbor_list& wn, const lsmps_vector3& ps);
+void update_velocity_position(Context& ctx, const lsmps_config& cfg, const lsmps_fluid_soa& fluid,
+                              const lsmps_fluid_soa& tmp, const lsmps_vector3& grad,
+                              const lsmps_vector3& ps);
+void anti_penetration(Context& ctx, const lsmps_config& cfg, const lsmps_fluid_soa& fluid,
+                      const lsmps_wall_soa& walls, const lsmps_neighbor_list& wn);
+void smooth_velocity(Context& ctx, const lsmps_config& cfg, const lsmps_fluid_soa& fluid,
+                     const lsmps_neighbor_list& fn, const lsmps_vector3& smoothed);
+
+}  // namespace lsmps

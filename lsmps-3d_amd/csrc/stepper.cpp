@@ -1,0 +1,229 @@
+// Moment-matrix cache and the fused one-step pipeline.
+//
+// Stage order follows the reference driver (src/app/simulation_driver.cu:336-393
+// with rebuild_geometry :252-264 moved to the start of the step): neighbor CSR
+// and surface labels (generation + 1), moment operators, provision, PPE
+// assembly, PPE solve, pressure correction.  Every stage is enqueued on the
+// context stream; the only host synchronisations are the two neighbor-capacity
+// checks and the solver's once-per-cycle convergence test.
+#include <algorithm>
+#include <cstring>
+
+#include "objects.hpp"
+
+using namespace lsmps;
+
+// ---- moment matrix -------------------------------------------------------
+
+void lsmps_moment_matrix::prepare(const lsmps_fluid_soa& fluid_in, const lsmps_wall_soa& walls_in,
+                                  const lsmps_neighbor_list& fn_in,
+                                  const lsmps_neighbor_list& wn_in, uint64_t gen) {
+  require(fluid_in.count <= fn_in.particle_count && fluid_in.count <= wn_in.particle_count,
+          "Moment matrix neighbor lists are smaller than fluid particle count");
+  if (fluid_in.count > capacity) resize(fluid_in.count);
+  // cache hit: same generation and geometry (moment_matrix.cu:117-123, 410-412)
+  if (prepared && gen == generation && fluid_in.count == count) return;
+  fluid = fluid_in;
+  walls = walls_in;
+  fn = fn_in;
+  wn = wn_in;
+  count = fluid_in.count;
+  generation = gen;
+  for (auto& f : full) f.ready = false;
+  vel_ops.reserve(static_cast<size_t>(kOpRows) * std::max<size_t>(capacity, 1));
+  pre_ops.reserve(static_cast<size_t>(kOpRows) * std::max<size_t>(capacity, 1));
+  inversion_count.reserve(4);
+  if (fluid.count > 0) {
+    MomentOutputs vo;
+    vo.ops = vel_ops.data();
+    compute_moments(*ctx, LSMPS_MOMENT_VELOCITY_WALL_DIRICHLET_A, config, fluid, walls, fn, wn, vo);
+    MomentOutputs po;
+    po.ops = pre_ops.data();
+    compute_moments(*ctx, LSMPS_MOMENT_PRESSURE_WALL_NEUMANN_A, config, fluid, walls, fn, wn, po);
+  }
+  host_counts[LSMPS_MOMENT_VELOCITY_WALL_DIRICHLET_A]++;
+  host_counts[LSMPS_MOMENT_PRESSURE_WALL_NEUMANN_A]++;
+  prepared = true;
+}
+
+// Operator rows are stored with stride `count` (the prepared particle count).
+const double* lsmps_moment_matrix::velocity_ops() const { return vel_ops.data(); }
+const double* lsmps_moment_matrix::pressure_ops() const { return pre_ops.data(); }
+
+void lsmps_moment_matrix::materialize(int kind) {
+  require(prepared, "Moment operators must prepare matrices before computing operators",
+          LSMPS_ERR_NOT_PREPARED);
+  Full& f = full[kind];
+  if (f.ready && f.generation == generation) return;
+  const int n = moment_size(kind);
+  const size_t cap = std::max<size_t>(capacity, 1);
+  f.inverse.reserve(cap * n * n);
+  f.trace.reserve(cap);
+  f.regularization.reserve(cap);
+  f.info.reserve(cap);
+  if (count > 0) {
+    MomentOutputs out;
+    out.inverse = f.inverse.data();
+    out.trace = f.trace.data();
+    out.regularization = f.regularization.data();
+    out.info = f.info.data();
+    compute_moments(*ctx, kind, config, fluid, walls, fn, wn, out);
+  }
+  if (kind == LSMPS_MOMENT_PRESSURE_WALL_NEUMANN_B || kind == LSMPS_MOMENT_FLUID_ONLY_A)
+    host_counts[kind]++;
+  f.generation = generation;
+  f.ready = true;
+}
+
+// ---- stepper -------------------------------------------------------------
+
+lsmps_stepper::~lsmps_stepper() {
+  for (int k = 0; k < n_events; ++k)
+    if (events[k]) (void)hipEventDestroy(events[k]);
+}
+
+void lsmps_stepper::allocate(size_t nf, size_t nw, size_t cells) {
+  const size_t cap = nf * max_neighbors;
+  const size_t nf1 = std::max<size_t>(nf, 1), nw1 = std::max<size_t>(nw, 1);
+  fc_ids.reserve(nf1);
+  fc_sorted.reserve(nf1);
+  wc_ids.reserve(nw1);
+  wc_sorted.reserve(nw1);
+  fc_begin.reserve(cells);
+  fc_end.reserve(cells);
+  wc_begin.reserve(cells);
+  wc_end.reserve(cells);
+  f_off.reserve(nf + 1);
+  w_off.reserve(nf + 1);
+  f_idx.reserve(std::max<size_t>(cap, 1));
+  w_idx.reserve(std::max<size_t>(cap, 1));
+  tmp_fluid.reserve(3 * nf1);
+  tmp_wall.reserve(3 * nw1);
+  ppe_rows.reserve(nf + 1);
+  ppe_cols.reserve(std::max<size_t>(nf + cap, 1));
+  ppe_vals.reserve(std::max<size_t>(nf + cap, 1));
+  ppe_vec.reserve(4 * nf1);
+  corr.reserve(9 * nf1);
+  fluid_capacity = std::max(fluid_capacity, nf);
+  wall_capacity = std::max(wall_capacity, nw);
+  cell_capacity = std::max(cell_capacity, cells);
+  moments.resize(std::max(moments.capacity, nf));
+}
+
+size_t lsmps_stepper::bytes() const {
+  return fc_ids.bytes() + fc_sorted.bytes() + fc_begin.bytes() + fc_end.bytes() +
+         wc_ids.bytes() + wc_sorted.bytes() + wc_begin.bytes() + wc_end.bytes() + f_off.bytes() +
+         f_idx.bytes() + w_off.bytes() + w_idx.bytes() + moments.bytes() + tmp_fluid.bytes() +
+         tmp_wall.bytes() + ppe_rows.bytes() + ppe_cols.bytes() + ppe_vals.bytes() +
+         ppe_vec.bytes() + solver.bytes() + corr.bytes();
+}
+
+lsmps_neighbor_list lsmps_stepper::fluid_neighbors() const {
+  return lsmps_neighbor_list{fluid_capacity, fluid_capacity * max_neighbors, f_off.data(),
+                             f_idx.data()};
+}
+lsmps_neighbor_list lsmps_stepper::wall_neighbors() const {
+  return lsmps_neighbor_list{fluid_capacity, fluid_capacity * max_neighbors, w_off.data(),
+                             w_idx.data()};
+}
+lsmps_ppe_workspace lsmps_stepper::ppe_workspace(size_t n) const {
+  const size_t ld = std::max<size_t>(fluid_capacity, 1);
+  lsmps_ppe_workspace ws{};
+  ws.matrix.rows = n;
+  ws.matrix.cols = n;
+  ws.matrix.nnz = n + fluid_capacity * max_neighbors;
+  ws.matrix.row_offsets = ppe_rows.data();
+  ws.matrix.col_indices = ppe_cols.data();
+  ws.matrix.values = ppe_vals.data();
+  ws.rhs = ppe_vec.data();
+  ws.pressure = ppe_vec.data() + ld;
+  ws.divergence = ppe_vec.data() + 2 * ld;
+  ws.pressure_laplacian = ppe_vec.data() + 3 * ld;
+  return ws;
+}
+
+void lsmps_stepper::step(const lsmps_fluid_soa& fluid, const lsmps_wall_soa& walls) {
+  Context& c = *ctx;
+  const size_t nf = fluid.count, nw = walls.count;
+  const size_t cells = static_cast<size_t>(config.cell_dims[0]) * config.cell_dims[1] *
+                       config.cell_dims[2];
+  if (nf > fluid_capacity || nw > wall_capacity || cells > cell_capacity) allocate(nf, nw, cells);
+  if (n_events == 0) {
+    for (int k = 0; k < 8; ++k) LSMPS_HIP(hipEventCreate(&events[k]));
+    n_events = 8;
+  }
+  const size_t nf1 = std::max<size_t>(fluid_capacity, 1), nw1 = std::max<size_t>(wall_capacity, 1);
+  lsmps_cell_list fc{fluid_capacity, cells, fc_ids.data(), fc_sorted.data(), fc_begin.data(),
+                     fc_end.data()};
+  lsmps_cell_list wc{wall_capacity, cells, wc_ids.data(), wc_sorted.data(), wc_begin.data(),
+                     wc_end.data()};
+  const lsmps_neighbor_list fn = fluid_neighbors();
+  const lsmps_neighbor_list wn = wall_neighbors();
+  LSMPS_HIP(hipEventRecord(events[0], c.stream));
+  // 1. geometry: rebuild_geometry (simulation_driver.cu:252-264)
+  build_neighbor_lists(c, fluid, walls, config, fc, wc, fn, wn, nullptr, nullptr);
+  ++generation;
+  classify_surface(c, fluid, walls, fn, wn, config, nullptr);
+  LSMPS_HIP(hipEventRecord(events[1], c.stream));
+  // 2. moment operators (prepare_matrices, moment_matrix.cu:615-654)
+  moments.config = config;
+  moments.prepare(fluid, walls, fn, wn, generation);
+  LSMPS_HIP(hipEventRecord(events[2], c.stream));
+  // 3. provision (explicit_update.cu:366-399)
+  double* tv = tmp_fluid.data();
+  double* tw = tmp_wall.data();
+  lsmps_fluid_soa tmp{nf, nullptr, nullptr, nullptr, tv, tv + nf1, tv + 2 * nf1, nullptr, nullptr};
+  lsmps_wall_soa tmpw{nw, nullptr, nullptr, nullptr, tw, tw + nw1, tw + 2 * nw1,
+                      nullptr, nullptr, nullptr};
+  provision_fluid(c, config, fluid, walls, fn, wn, moments.velocity_ops(), tmp, nullptr);
+  provision_walls(c, config, walls, tmpw);
+  LSMPS_HIP(hipEventRecord(events[3], c.stream));
+  // 4. PPE assembly (ppe_matrix.cu:486-515)
+  const lsmps_ppe_workspace ws = ppe_workspace(nf);
+  ppe_assemble(c, config, fluid, walls, fn, wn, tmp, tmpw, moments.velocity_ops(),
+               moments.pressure_ops(), ws);
+  LSMPS_HIP(hipEventRecord(events[4], c.stream));
+  // 5. PPE solve (replaces the AMGX solve, simulation_driver.cu:364-370)
+  // As in the reference driver, the step continues with whatever pressure the
+  // solver produced; a non-converged solve is reported after the correction.
+  last_report = lsmps_solve_report{};
+  const uint32_t saved_flags = c.flags;
+  c.flags |= LSMPS_FLAG_ASYNC;  // the stepper synchronises once at the end
+  bool converged = true;
+  try {
+    ppe_solve(c, solver, ws.matrix, ws.rhs, ws.pressure, solver_options, &last_report);
+  } catch (const Error& e) {
+    c.flags = saved_flags;
+    if (e.code != LSMPS_ERR_NOT_CONVERGED) throw;
+    converged = false;
+  } catch (...) {
+    c.flags = saved_flags;
+    throw;
+  }
+  c.flags = saved_flags;
+  LSMPS_HIP(hipEventRecord(events[5], c.stream));
+  // 6. correction (pressure_correction.cu:598-645)
+  double* cv = corr.data();
+  lsmps_vector3 grad{nf, cv, cv + nf1, cv + 2 * nf1};
+  lsmps_vector3 ps{nf, cv + 3 * nf1, cv + 4 * nf1, cv + 5 * nf1};
+  lsmps_vector3 sm{nf, cv + 6 * nf1, cv + 7 * nf1, cv + 8 * nf1};
+  clamp_pressure(c, fluid, ws.pressure);
+  pressure_gradient(c, config, fluid, walls, fn, wn, ws.pressure, moments.pressure_ops(), grad);
+  particle_shifting(c, config, fluid, walls, fn, wn, ps);
+  update_velocity_position(c, config, fluid, tmp, grad, ps);
+  anti_penetration(c, config, fluid, walls, wn);
+  smooth_velocity(c, config, fluid, fn, sm);
+  LSMPS_HIP(hipEventRecord(events[6], c.stream));
+  LSMPS_HIP(hipEventSynchronize(events[6]));
+  for (int k = 0; k < 6; ++k) {
+    float ms = 0.0f;
+    LSMPS_HIP(hipEventElapsedTime(&ms, events[k], events[k + 1]));
+    stage_ms[k] = ms;
+  }
+  float total = 0.0f;
+  LSMPS_HIP(hipEventElapsedTime(&total, events[0], events[6]));
+  stage_ms[6] = total;
+  if (!converged)
+    raise(LSMPS_ERR_NOT_CONVERGED, "PPE solver did not reach the tolerance (relative residual " +
+                                       std::to_string(last_report.relative_residual) + ")");
+}

@@ -1,0 +1,322 @@
+"""ctypes bindings of liblsmps3d_amd.so and device-buffer plumbing.
+
+torch is used only to own device memory and streams; every computation runs in
+the HIP library.  Loading fails loudly when the library is not built.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liblsmps3d_amd.so")
+
+EXPORTS = [
+    "lsmps_context_create", "lsmps_context_destroy", "lsmps_context_set_stream",
+    "lsmps_context_synchronize", "lsmps_context_precision", "lsmps_last_error", "lsmps_version",
+    "lsmps_default_config", "lsmps_derive_cell_grid", "lsmps_build_cell_list_fluid",
+    "lsmps_build_cell_list_wall", "lsmps_build_neighbor_lists", "lsmps_reference_number_density",
+    "lsmps_classify_surface", "lsmps_moment_create", "lsmps_moment_destroy", "lsmps_moment_resize",
+    "lsmps_moment_set_config", "lsmps_moment_bytes", "lsmps_moment_prepare",
+    "lsmps_moment_get_view", "lsmps_provision_compute", "lsmps_provision_temporary_velocity",
+    "lsmps_provision_temporary_wall_velocity", "lsmps_ppe_assemble", "lsmps_ppe_solve",
+    "lsmps_default_solver_options", "lsmps_correction_apply", "lsmps_clamp_particle_pressure",
+    "lsmps_compute_pressure_gradient", "lsmps_compute_particle_shifting",
+    "lsmps_update_velocity_and_position", "lsmps_apply_anti_penetration",
+    "lsmps_smooth_particle_velocity", "lsmps_stepper_create", "lsmps_stepper_destroy",
+    "lsmps_stepper_set_config", "lsmps_stepper_set_solver", "lsmps_stepper_bytes",
+    "lsmps_stepper_step", "lsmps_stepper_stage_times", "lsmps_stepper_neighbor_lists",
+    "lsmps_stepper_ppe_workspace", "lsmps_stepper_temporary_velocity",
+    "lsmps_stepper_correction_vectors", "lsmps_stepper_moments",
+]
+
+_LIB = None
+
+
+def load(path=LIB_PATH):
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build the HIP library first "
+                           "(python __graft_entry__.py or lsmps-3d_amd/build.py)")
+    lib = C.CDLL(path)
+    lib.lsmps_last_error.restype = C.c_char_p
+    lib.lsmps_version.restype = C.c_char_p
+    lib.lsmps_moment_bytes.restype = C.c_size_t
+    lib.lsmps_stepper_bytes.restype = C.c_size_t
+    lib.lsmps_stepper_moments.restype = C.c_void_p
+    lib.lsmps_stepper_moments.argtypes = [C.c_void_p]
+    lib.lsmps_context_precision.restype = C.c_int
+    lib.lsmps_moment_prepare.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                         C.c_void_p, C.c_uint64]
+    lib.lsmps_context_create.argtypes = [C.c_int, C.c_int, C.c_uint32, C.c_void_p]
+    lib.lsmps_derive_cell_grid.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t,
+                                           C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t,
+                                           C.c_void_p]
+    lib.lsmps_reference_number_density.argtypes = [C.c_double, C.c_double, C.c_int32,
+                                                   C.POINTER(C.c_double)]
+    _LIB = lib
+    return lib
+
+
+class LsmpsError(RuntimeError):
+    def __init__(self, code, message):
+        super().__init__(f"[{code}] {message}")
+        self.code = code
+
+
+def check(rc, what=""):
+    if rc != abi.LSMPS_OK:
+        msg = load().lsmps_last_error().decode()
+        raise LsmpsError(rc, f"{what}: {msg}" if what else msg)
+    return rc
+
+
+# ---- host helpers (no GPU needed) ------------------------------------------
+
+def reference_number_density(l0, re, kernel=abi.WEIGHT_LINEAR):
+    out = C.c_double()
+    check(load().lsmps_reference_number_density(l0, re, kernel, C.byref(out)))
+    return out.value
+
+
+def derive_cell_grid(config, fluid, walls):
+    f = [np.ascontiguousarray(fluid[k], np.float64) for k in ("x", "y", "z")]
+    w = [np.ascontiguousarray(walls[k], np.float64) for k in ("x", "y", "z")]
+    cfg = config.copy()
+    check(load().lsmps_derive_cell_grid(*(a.ctypes.data for a in f), f[0].shape[0],
+                                        *(a.ctypes.data for a in w), w[0].shape[0],
+                                        C.byref(cfg)))
+    return cfg
+
+
+# ---- device plumbing --------------------------------------------------------
+
+def _torch():
+    import torch
+    return torch
+
+
+def to_device(a, device="cuda:0"):
+    torch = _torch()
+    return torch.from_numpy(np.ascontiguousarray(a)).to(device)
+
+
+def ptr(t):
+    return t.data_ptr() if t is not None else 0
+
+
+class DeviceState:
+    """Fluid and wall SoA buffers of a case on the GPU."""
+
+    FLUID_KEYS = ("x", "y", "z", "vx", "vy", "vz", "pressure")
+    WALL_KEYS = ("x", "y", "z", "vx", "vy", "vz", "normal_x", "normal_y", "normal_z")
+
+    def __init__(self, case, device="cuda:0"):
+        self.device = device
+        self.n_fluid = case.n_fluid
+        self.n_wall = case.n_wall
+        self.fluid = {k: to_device(case.fluid[k].astype(np.float64), device) for k in self.FLUID_KEYS}
+        self.fluid["surface_type"] = to_device(case.fluid["surface_type"].astype(np.int32), device)
+        self.walls = {k: to_device(case.walls[k].astype(np.float64), device) for k in self.WALL_KEYS}
+
+    def fluid_soa(self):
+        f = self.fluid
+        return abi.FluidSoA(self.n_fluid, *(abi.dptr(ptr(f[k])) for k in self.FLUID_KEYS),
+                            abi.iptr(ptr(f["surface_type"])))
+
+    def wall_soa(self):
+        w = self.walls
+        return abi.WallSoA(self.n_wall, *(abi.dptr(ptr(w[k])) for k in self.WALL_KEYS))
+
+    def snapshot(self):
+        return {k: v.clone() for k, v in self.fluid.items()}
+
+    def restore(self, snap):
+        for k, v in snap.items():
+            self.fluid[k].copy_(v)
+
+    def fluid_host(self):
+        return {k: v.cpu().numpy() for k, v in self.fluid.items()}
+
+
+class Context:
+    def __init__(self, precision=abi.PRECISION_FP64, device=0, flags=0, stream=None):
+        torch = _torch()
+        self.lib = load()
+        self.handle = C.c_void_p()
+        check(self.lib.lsmps_context_create(device, precision, flags, C.byref(self.handle)),
+              "context_create")
+        s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        check(self.lib.lsmps_context_set_stream(self.handle, C.c_void_p(s)), "set_stream")
+
+    def close(self):
+        if self.handle:
+            self.lib.lsmps_context_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Stepper:
+    def __init__(self, ctx, state, config, max_neighbors=256, solver=None):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.state = state
+        self.config = config.copy()
+        self.max_neighbors = max_neighbors
+        self.handle = C.c_void_p()
+        check(self.lib.lsmps_stepper_create(ctx.handle, C.c_size_t(state.n_fluid),
+                                            C.c_size_t(state.n_wall), C.c_size_t(max_neighbors),
+                                            C.byref(self.config), C.byref(self.handle)),
+              "stepper_create")
+        if solver is not None:
+            self.set_solver(solver)
+
+    def set_solver(self, options):
+        check(self.lib.lsmps_stepper_set_solver(self.handle, C.byref(options)), "set_solver")
+
+    def step(self, raise_on_nonconvergence=True):
+        rep = abi.SolveReport()
+        fs = self.state.fluid_soa()
+        ws = self.state.wall_soa()
+        rc = self.lib.lsmps_stepper_step(self.handle, C.byref(fs), C.byref(ws), C.byref(rep))
+        if rc != abi.LSMPS_OK and (rc != abi.LSMPS_ERR_NOT_CONVERGED or raise_on_nonconvergence):
+            check(rc, "stepper_step")
+        return rep.as_dict()
+
+    def stage_times(self):
+        ms = (C.c_double * 7)()
+        check(self.lib.lsmps_stepper_stage_times(self.handle, ms, 7))
+        return list(ms)
+
+    def bytes(self):
+        return int(self.lib.lsmps_stepper_bytes(self.handle))
+
+    def neighbor_lists(self):
+        """(fluid_offsets, fluid_indices, wall_offsets, wall_indices) as numpy arrays."""
+        fn, wn = abi.NeighborList(), abi.NeighborList()
+        check(self.lib.lsmps_stepper_neighbor_lists(self.handle, C.byref(fn), C.byref(wn)))
+        n = self.state.n_fluid
+        f_off = device_to_numpy(fn.offsets, n + 1, np.int32)
+        w_off = device_to_numpy(wn.offsets, n + 1, np.int32)
+        f_idx = device_to_numpy(fn.indices, int(f_off[-1]), np.int32)
+        w_idx = device_to_numpy(wn.indices, int(w_off[-1]), np.int32)
+        return f_off, f_idx, w_off, w_idx
+
+    def ppe(self):
+        ws = abi.PpeWorkspace()
+        check(self.lib.lsmps_stepper_ppe_workspace(self.handle, C.byref(ws)))
+        n = self.state.n_fluid
+        rows = device_to_numpy(ws.matrix.row_offsets, n + 1, np.int32)
+        used = int(rows[-1])
+        return {
+            "row_offsets": rows,
+            "col_indices": device_to_numpy(ws.matrix.col_indices, used, np.int32),
+            "values": device_to_numpy(ws.matrix.values, used, np.float64),
+            "nnz": ws.matrix.nnz,
+            "rhs": device_to_numpy(ws.rhs, n, np.float64),
+            "pressure": device_to_numpy(ws.pressure, n, np.float64),
+            "divergence": device_to_numpy(ws.divergence, n, np.float64),
+            "pressure_laplacian": device_to_numpy(ws.pressure_laplacian, n, np.float64),
+        }
+
+    def temporary_velocity(self):
+        f, w = abi.FluidSoA(), abi.WallSoA()
+        check(self.lib.lsmps_stepper_temporary_velocity(self.handle, C.byref(f), C.byref(w)))
+        n, nw = self.state.n_fluid, self.state.n_wall
+        return {"vx": device_to_numpy(f.vx, n, np.float64), "vy": device_to_numpy(f.vy, n, np.float64),
+                "vz": device_to_numpy(f.vz, n, np.float64),
+                "wall_vx": device_to_numpy(w.vx, nw, np.float64),
+                "wall_vy": device_to_numpy(w.vy, nw, np.float64),
+                "wall_vz": device_to_numpy(w.vz, nw, np.float64)}
+
+    def correction_vectors(self):
+        g, p = abi.Vector3(), abi.Vector3()
+        check(self.lib.lsmps_stepper_correction_vectors(self.handle, C.byref(g), C.byref(p)))
+        n = self.state.n_fluid
+        return ([device_to_numpy(a, n, np.float64) for a in (g.x, g.y, g.z)],
+                [device_to_numpy(a, n, np.float64) for a in (p.x, p.y, p.z)])
+
+    def moments_handle(self):
+        return C.c_void_p(self.lib.lsmps_stepper_moments(self.handle))
+
+    def close(self):
+        if self.handle:
+            self.lib.lsmps_stepper_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def device_to_numpy(p, n, dtype):
+    """Copy n elements from a device pointer (ctypes pointer or int) to a numpy array."""
+    torch = _torch()
+    n = int(n)
+    addr = C.cast(p, C.c_void_p).value if not isinstance(p, int) else p
+    out = np.empty(n, dtype=dtype)
+    if n == 0:
+        return out
+    hip = _hip_runtime()
+    torch.cuda.synchronize()
+    rc = hip.hipMemcpy(C.c_void_p(out.ctypes.data), C.c_void_p(addr), C.c_size_t(out.nbytes), 2)
+    if rc != 0:
+        raise RuntimeError(f"hipMemcpy D2H failed ({rc})")
+    return out
+
+
+_HIP = None
+
+
+def _hip_runtime():
+    global _HIP
+    if _HIP is None:
+        # by soname, so the runtime torch already loaded is reused
+        for name in ("libamdhip64.so.7", "/opt/rocm/lib/libamdhip64.so.7"):
+            try:
+                _HIP = C.CDLL(name)
+                break
+            except OSError:
+                continue
+        if _HIP is None:
+            raise RuntimeError("libamdhip64.so not found")
+        _HIP.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    return _HIP
+
+
+def default_solver(precision=abi.PRECISION_FP64):
+    o = abi.default_solver_options()
+    return o
+
+
+def run_step_on_device(case, precision=abi.PRECISION_FP64, solver=None, max_neighbors=256,
+                       device="cuda:0"):
+    """One fused step of `case` on the GPU; returns every inspectable output as numpy."""
+    ctx = Context(precision)
+    state = DeviceState(case, device)
+    st = Stepper(ctx, state, case.config, max_neighbors, solver or abi.default_solver_options())
+    report = st.step(raise_on_nonconvergence=False)
+    f_off, f_idx, w_off, w_idx = st.neighbor_lists()
+    ppe = st.ppe()
+    grad, ps = st.correction_vectors()
+    out = {
+        "fluid_offsets": f_off, "fluid_indices": f_idx, "wall_offsets": w_off,
+        "wall_indices": w_idx, "ppe": ppe, "pressure": state.fluid["pressure"].cpu().numpy(),
+        "surface_type": state.fluid["surface_type"].cpu().numpy(), "fluid": state.fluid_host(),
+        "provision": st.temporary_velocity(), "grad": grad, "ps": ps, "report": report,
+        "stage_ms": st.stage_times(),
+    }
+    st.close()
+    ctx.close()
+    return out
