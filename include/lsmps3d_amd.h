@@ -81,8 +81,10 @@ enum { LSMPS_WEIGHT_LINEAR = 0, LSMPS_WEIGHT_QUADRATIC = 1 };
  * IDR(1); PCG is only meaningful for symmetric positive systems. */
 enum { LSMPS_SOLVER_PCG = 0, LSMPS_SOLVER_BICGSTAB = 1, LSMPS_SOLVER_IDRS = 2 };
 
-/* Context flags */
-enum { LSMPS_FLAG_ASYNC = 1 };
+/* Context flags.  KERNEL_TIMING brackets every PPE SpMV launch with hipEvents on
+ * the context stream and accumulates durations and algorithmic bytes
+ * (lsmps_context_kernel_stats). */
+enum { LSMPS_FLAG_ASYNC = 1, LSMPS_FLAG_KERNEL_TIMING = 2 };
 
 /* ---- data contract (POD mirrors of the reference views) ------------------ */
 
@@ -251,7 +253,8 @@ typedef struct lsmps_solver_options {
   int32_t method;          /* LSMPS_SOLVER_* */
   int32_t max_iterations;  /* total Krylov iterations */
   double tolerance;        /* relative true residual */
-  double inner_tolerance;  /* mixed mode: relative reduction per FP32 inner solve */
+  double inner_tolerance;  /* mixed mode: factor by which the recursive residual falls
+                              between replacements by the true FP64 residual (1e-3) */
   int32_t check_interval;  /* PCG: iterations between host convergence checks */
   int32_t clamp_negative;  /* clamp p<0 after the solve, as ppe_matrix.cu:662 (default 1) */
   int32_t shadow_dimension; /* IDR(s): s, 1..16 (default 8) */
@@ -280,6 +283,12 @@ int lsmps_context_destroy(lsmps_context* ctx);
 int lsmps_context_set_stream(lsmps_context* ctx, void* hip_stream);
 int lsmps_context_synchronize(lsmps_context* ctx);
 int lsmps_context_precision(const lsmps_context* ctx);
+/* PPE SpMV statistics gathered under LSMPS_FLAG_KERNEL_TIMING since the last
+ * reset: launches, summed duration (ms) and summed algorithmic bytes
+ * (values + column indices + row pointers, each vector element read or written
+ * once).  No reference counterpart. */
+int lsmps_context_kernel_stats(lsmps_context* ctx, int64_t* launches, double* total_ms,
+                               double* total_bytes, int reset);
 const char* lsmps_last_error(void);
 const char* lsmps_version(void);
 

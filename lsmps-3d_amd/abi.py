@@ -32,6 +32,7 @@ SOLVER_BICGSTAB = 1
 SOLVER_IDRS = 2
 
 FLAG_ASYNC = 1
+FLAG_KERNEL_TIMING = 2
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
@@ -192,7 +193,7 @@ def default_solver_options():
     o.method = SOLVER_IDRS
     o.max_iterations = 50000
     o.tolerance = 1.0e-10
-    o.inner_tolerance = 1.0e-4
+    o.inner_tolerance = 1.0e-3
     o.check_interval = 16
     o.clamp_negative = 1
     o.shadow_dimension = 8

@@ -91,6 +91,8 @@ int lsmps_context_destroy(lsmps_context* ctx) {
   if (!ctx) return LSMPS_OK;
   (void)hipSetDevice(ctx->impl.device);
   (void)hipStreamSynchronize(ctx->impl.stream);
+  if (ctx->impl.timing_ready)
+    for (auto& e : ctx->impl.timing_events) (void)hipEventDestroy(e);
   if (ctx->impl.host_i32) (void)hipHostFree(ctx->impl.host_i32);
   if (ctx->impl.host_f64) (void)hipHostFree(ctx->impl.host_f64);
   delete ctx;
@@ -111,6 +113,21 @@ int lsmps_context_synchronize(lsmps_context* ctx) {
 
 int lsmps_context_precision(const lsmps_context* ctx) {
   return ctx ? ctx->impl.precision : -1;
+}
+
+int lsmps_context_kernel_stats(lsmps_context* ctx, int64_t* launches, double* total_ms,
+                               double* total_bytes, int reset) {
+  API_BEGIN
+  Context& c = C(ctx);
+  if (launches) *launches = c.spmv_launches;
+  if (total_ms) *total_ms = c.spmv_ms;
+  if (total_bytes) *total_bytes = c.spmv_bytes;
+  if (reset) {
+    c.spmv_launches = 0;
+    c.spmv_ms = 0.0;
+    c.spmv_bytes = 0.0;
+  }
+  API_END("lsmps_context_kernel_stats")
 }
 
 int lsmps_default_config(lsmps_config* out) {
@@ -401,7 +418,7 @@ int lsmps_default_solver_options(lsmps_solver_options* out) {
   o.method = LSMPS_SOLVER_IDRS;
   o.max_iterations = 50000;
   o.tolerance = 1.0e-10;
-  o.inner_tolerance = 1.0e-4;
+  o.inner_tolerance = 1.0e-3;
   o.check_interval = 16;
   o.clamp_negative = 1;
   o.shadow_dimension = 8;

@@ -115,6 +115,14 @@ struct Context {
   DeviceArray<char> tmp_points;      // sorted source points for the neighbor scan
   int32_t* host_i32 = nullptr;   // pinned host scratch
   double* host_f64 = nullptr;
+  // PPE SpMV timing (LSMPS_FLAG_KERNEL_TIMING)
+  hipEvent_t timing_events[2 * 17] = {};
+  bool timing_ready = false;
+  int64_t spmv_launches = 0;
+  double spmv_ms = 0.0;
+  double spmv_bytes = 0.0;
+
+  bool timing() const { return (flags & LSMPS_FLAG_KERNEL_TIMING) != 0; }
 
   void activate() const { LSMPS_HIP(hipSetDevice(device)); }
   void finish() const {

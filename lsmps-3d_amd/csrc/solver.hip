@@ -19,9 +19,11 @@
 //  * FP64 mode: IDR(s) on the FP64 matrix, FP64 vectors; at convergence of the
 //    recursive residual the true FP64 residual b - A x is recomputed and must
 //    meet the tolerance.
-//  * mixed mode: defect correction.  Outer loop in FP64 (r = b - A x with the
-//    FP64 matrix, x += e); inner IDR(s) on an FP32 copy of the values with FP32
-//    vectors and FP64 dot accumulation, to `inner_tolerance`.
+//  * mixed mode: the SpMVs of IDR(s) read an FP32 copy of the matrix values
+//    (4 instead of 8 bytes per nonzero); vectors, dots and the true residual
+//    stay FP64, and the recursive residual is replaced by the true FP64
+//    residual b - A x each time it falls by `inner_tolerance` (reliable updates),
+//    so the solve converges to the FP64 system at FP64 accuracy.
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -97,12 +99,12 @@ __device__ double block_sum_partials(const double* partials, int nb, int q) {
   return total;
 }
 
-template <typename T>
+template <typename T, typename V>
 struct SpmvArgs {
   size_t n;
   const int32_t* rows;
   const int32_t* cols;
-  const T* vals;
+  const V* vals;
   const T* x;
   T* y;
   const T* p0;  // mode 1
@@ -112,8 +114,8 @@ struct SpmvArgs {
 };
 
 // MODE 0: y = A x.  1: + shadow dots P_q . y.  2: + y.r, y.y.  3: + y.x (CG p.Ap).
-template <typename T, int MODE>
-__global__ void __launch_bounds__(kBlock) spmv_kernel(SpmvArgs<T> a) {
+template <typename T, typename V, int MODE>
+__global__ void __launch_bounds__(kBlock) spmv_kernel(SpmvArgs<T, V> a) {
   constexpr int NQ = MODE == 1 ? kMaxShadow : (MODE == 2 ? 2 : 1);
   const int lane = threadIdx.x & (kLanes - 1);
   const size_t group = (static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x) / kLanes;
@@ -125,7 +127,7 @@ __global__ void __launch_bounds__(kBlock) spmv_kernel(SpmvArgs<T> a) {
     const int32_t b = a.rows[row];
     const int32_t e = a.rows[row + 1];
     T sum = T(0);
-    for (int32_t c = b + lane; c < e; c += kLanes) sum += a.vals[c] * a.x[a.cols[c]];
+    for (int32_t c = b + lane; c < e; c += kLanes) sum += static_cast<T>(a.vals[c]) * a.x[a.cols[c]];
 #pragma unroll
     for (int off = kLanes / 2; off > 0; off >>= 1) sum += __shfl_xor(sum, off, kLanes);
     if (lane == 0) {
@@ -492,14 +494,18 @@ double true_residual(Context& ctx, PpeSolver& S, const lsmps_csr& A, const doubl
   return read_rr(ctx, S);
 }
 
-// IDR(s) on (rows, cols, vals) of type T.  x and r hold the initial iterate and
-// its residual.  Stops when ||r|| <= target (recursive residual); when `A64`
-// is given (FP64 mode) the true residual is recomputed and must agree.
-template <typename T>
+// IDR(s) with vectors of type T and matrix values of type V.  x and r hold the
+// initial iterate and its residual.  Each time the recursive residual has
+// dropped by `replace_factor` since the last replacement (and when it meets the
+// target) it is replaced by the true FP64 residual b - A64 x, keeping the
+// Krylov state (reliable updates); the solve stops when the true residual meets
+// the target.
+template <typename T, typename V>
 int idr_solve(Context& ctx, PpeSolver& S, size_t n, const int32_t* rows, const int32_t* cols,
-              const T* vals, const T* dinv, T* x, T* r, T* work, size_t ld, int s,
-              double rr0, double target, int max_iterations, const lsmps_csr* A64,
-              const double* b64, int* iterations_out, double* rr_out) {
+              const V* vals, const T* dinv, T* x, T* r, T* work, size_t ld, int s, double rr0,
+              double target, int max_iterations, const lsmps_csr& A64, const double* b64,
+              double replace_factor, int* iterations_out, int* replacements_out,
+              double* rr_out) {
   T* v = work;
   T* t = work + ld;
   T* p0 = work + 2 * ld;
@@ -513,8 +519,25 @@ int idr_solve(Context& ctx, PpeSolver& S, size_t n, const int32_t* rows, const i
   idr_init_scalars<<<1, kBlock, 0, ctx.stream>>>(S.scalars.data(), s);
   LSMPS_LAUNCH_CHECK();
   double rr = rr0;
-  int it = 0;
+  double rr_replaced = rr0;
+  int it = 0, replacements = 0;
   const double target2 = target * target;
+  const double factor2 = replace_factor * replace_factor;
+  // Optional per-launch timing of the SpMVs (events bracket each launch).
+  const bool timing = ctx.timing();
+  if (timing && !ctx.timing_ready) {
+    for (auto& e : ctx.timing_events) LSMPS_HIP(hipEventCreate(&e));
+    ctx.timing_ready = true;
+  }
+  LSMPS_HIP(hipMemcpyAsync(ctx.host_i32, rows + n, sizeof(int32_t), hipMemcpyDeviceToHost,
+                           ctx.stream));
+  ctx.sync();
+  const double nnz = static_cast<double>(ctx.host_i32[0]);
+  const double spmv_bytes = nnz * (sizeof(V) + sizeof(int32_t)) + (n + 1) * sizeof(int32_t) +
+                            2.0 * n * sizeof(T);
+  auto mark = [&](int slot) {
+    if (timing) LSMPS_HIP(hipEventRecord(ctx.timing_events[slot], ctx.stream));
+  };
   while (rr > target2 && it < max_iterations) {
     iv.partials = S.part_a.data();
     shadow_dots_kernel<T><<<vb, kBlock, 0, ctx.stream>>>(iv);
@@ -522,34 +545,53 @@ int idr_solve(Context& ctx, PpeSolver& S, size_t n, const int32_t* rows, const i
                                                static_cast<int>(vb), s);
     for (int k = 0; k < s; ++k) {
       idr_a_kernel<T><<<vb, kBlock, 0, ctx.stream>>>(iv, k);
-      SpmvArgs<T> sa{n, rows, cols, vals, U + k * ld, G + k * ld, p0, s, nullptr, S.part_a.data()};
-      spmv_kernel<T, 1><<<sb, kBlock, 0, ctx.stream>>>(sa);
+      SpmvArgs<T, V> sa{n, rows, cols, vals, U + k * ld, G + k * ld, p0, s, nullptr,
+                        S.part_a.data()};
+      mark(2 * k);
+      spmv_kernel<T, V, 1><<<sb, kBlock, 0, ctx.stream>>>(sa);
+      mark(2 * k + 1);
       idr_scalar_after<<<1, kBlock, 0, ctx.stream>>>(S.scalars.data(), S.part_a.data(),
                                                      static_cast<int>(sb), s, k);
       iv.partials = S.part_b.data();
       idr_b_kernel<T><<<vb, kBlock, 0, ctx.stream>>>(iv, k, k == s - 1 ? 1 : 0);
       ++it;
     }
-    SpmvArgs<T> st{n, rows, cols, vals, v, t, nullptr, s, r, S.part_a.data()};
-    spmv_kernel<T, 2><<<sb, kBlock, 0, ctx.stream>>>(st);
-    idr_scalar_omega<<<1, kBlock, 0, ctx.stream>>>(S.scalars.data(), S.part_a.data(), S.part_b.data(),
-                                                   static_cast<int>(sb), static_cast<int>(vb));
+    SpmvArgs<T, V> st{n, rows, cols, vals, v, t, nullptr, s, r, S.part_a.data()};
+    mark(2 * s);
+    spmv_kernel<T, V, 2><<<sb, kBlock, 0, ctx.stream>>>(st);
+    mark(2 * s + 1);
+    idr_scalar_omega<<<1, kBlock, 0, ctx.stream>>>(S.scalars.data(), S.part_a.data(),
+                                                   S.part_b.data(), static_cast<int>(sb),
+                                                   static_cast<int>(vb));
     iv.partials = S.part_b.data();
     idr_c_kernel<T><<<vb, kBlock, 0, ctx.stream>>>(iv);
-    scalar_norm<<<1, kBlock, 0, ctx.stream>>>(S.scalars.data(), S.part_b.data(), static_cast<int>(vb));
+    scalar_norm<<<1, kBlock, 0, ctx.stream>>>(S.scalars.data(), S.part_b.data(),
+                                              static_cast<int>(vb));
     LSMPS_LAUNCH_CHECK();
     ++it;
-    rr = read_rr(ctx, S);
+    LSMPS_HIP(hipMemcpyAsync(S.host, &S.scalars.data()->rr, sizeof(double), hipMemcpyDeviceToHost,
+                             ctx.stream));
     LSMPS_HIP(hipMemcpyAsync(S.host + 2, &S.scalars.data()->breakdown, sizeof(int32_t),
                              hipMemcpyDeviceToHost, ctx.stream));
     ctx.sync();
+    if (timing) {
+      for (int k = 0; k <= s; ++k) {
+        float ms = 0.0f;
+        LSMPS_HIP(hipEventElapsedTime(&ms, ctx.timing_events[2 * k], ctx.timing_events[2 * k + 1]));
+        ctx.spmv_ms += ms;
+        ctx.spmv_bytes += spmv_bytes + (k == s ? n * sizeof(T) : 0.0);  // mode 2 also reads r
+        ++ctx.spmv_launches;
+      }
+    }
+    rr = S.host[0];
     int32_t breakdown = 0;
     std::memcpy(&breakdown, S.host + 2, sizeof(int32_t));
     if (!std::isfinite(rr)) break;
-    if (rr <= target2 && A64 != nullptr) {
-      // replace the recursive residual by the true FP64 residual
-      rr = true_residual(ctx, S, *A64, reinterpret_cast<const double*>(x), b64,
+    if (rr <= target2 || rr <= factor2 * rr_replaced) {
+      rr = true_residual(ctx, S, A64, reinterpret_cast<const double*>(x), b64,
                          reinterpret_cast<double*>(r), nullptr);
+      rr_replaced = rr;
+      ++replacements;
     }
     if (breakdown) {
       fill_kernel<T><<<blocks_for(2 * s * ld), kBlock, 0, ctx.stream>>>(2 * s * ld, G, T(0));
@@ -559,6 +601,7 @@ int idr_solve(Context& ctx, PpeSolver& S, size_t n, const int32_t* rows, const i
     }
   }
   *iterations_out = it;
+  *replacements_out = replacements;
   *rr_out = rr;
   return rr <= target2 ? 1 : 0;
 }
@@ -582,9 +625,9 @@ void pcg_solve(Context& ctx, PpeSolver& S, const lsmps_csr& A, const double* b, 
   int it = 0;
   const double target2 = target * target;
   while (rr > target2 && it < max_iterations) {
-    SpmvArgs<double> sa{n, A.row_offsets, A.col_indices, A.values, p, q, nullptr, 0, nullptr,
-                        S.part_a.data()};
-    spmv_kernel<double, 3><<<sb, kBlock, 0, ctx.stream>>>(sa);
+    SpmvArgs<double, double> sa{n, A.row_offsets, A.col_indices, A.values, p, q, nullptr, 0,
+                                nullptr, S.part_a.data()};
+    spmv_kernel<double, double, 3><<<sb, kBlock, 0, ctx.stream>>>(sa);
     cg_scalar_alpha<<<1, kBlock, 0, ctx.stream>>>(S.scalars.data(), S.part_a.data(), static_cast<int>(sb));
     cg_update_kernel<<<vb, kBlock, 0, ctx.stream>>>(n, x, r, p, q, dinv, S.scalars.data(),
                                                     S.part_b.data());
@@ -655,65 +698,34 @@ void ppe_solve(Context& ctx, PpeSolver& S, const lsmps_csr& A, const double* b, 
               rep);
     rep.relative_residual /= bnorm;
     rep.converged = rep.relative_residual <= opt.tolerance ? 1 : 0;
-  } else if (ctx.precision == LSMPS_PRECISION_FP64) {
+  } else {
     S.vec64.reserve((4 + 2 * static_cast<size_t>(shadow_dim)) * ld);
     double* r = S.vec64.data();
     double* work = r + ld;
     copy_kernel<double><<<blocks_for(n), kBlock, 0, ctx.stream>>>(n, b, r);
     LSMPS_LAUNCH_CHECK();
-    int it = 0;
+    int it = 0, replacements = 0;
     double rr = 0.0;
-    idr_solve<double>(ctx, S, n, A.row_offsets, A.col_indices, A.values, S.dinv64.data(), x, r,
-                      work, ld, shadow_dim, bnorm * bnorm, target, max_it, &A, b, &it, &rr);
-    rep.iterations = it;
-    rep.refinements = 1;
-    rep.relative_residual = std::sqrt(rr) / bnorm;
-    rep.converged = rep.relative_residual <= opt.tolerance ? 1 : 0;
-  } else {
-    // mixed: FP32 values and vectors inside, FP64 defect correction outside
-    const size_t nnz = static_cast<size_t>(0);
-    (void)nnz;
-    LSMPS_HIP(hipMemcpyAsync(ctx.host_i32, A.row_offsets + n, sizeof(int32_t),
-                             hipMemcpyDeviceToHost, ctx.stream));
-    ctx.sync();
-    const size_t used = static_cast<size_t>(ctx.host_i32[0]);
-    S.vals32.reserve(std::max<size_t>(used, 1));
-    S.dinv32.reserve(n);
-    to_f32_kernel<<<blocks_for(used), kBlock, 0, ctx.stream>>>(used, A.values, S.vals32.data());
-    to_f32_kernel<<<blocks_for(n), kBlock, 0, ctx.stream>>>(n, S.dinv64.data(), S.dinv32.data());
-    LSMPS_LAUNCH_CHECK();
-    S.outer64.reserve(n);
-    S.vec32.reserve((5 + 2 * static_cast<size_t>(shadow_dim)) * ld);
-    double* r64 = S.outer64.data();
-    float* e = S.vec32.data();
-    float* r32 = e + ld;
-    float* work = r32 + ld;
-    double rr = true_residual(ctx, S, A, x, b, r64, r32);
-    const double inner_tol = opt.inner_tolerance > 0.0 ? opt.inner_tolerance : 1.0e-4;
-    int total = 0, outer = 0;
-    while (rr > target * target && total < max_it && outer < 64) {
-      fill_kernel<float><<<blocks_for(n), kBlock, 0, ctx.stream>>>(n, e, 0.0f);
+    if (ctx.precision == LSMPS_PRECISION_FP64) {
+      idr_solve<double, double>(ctx, S, n, A.row_offsets, A.col_indices, A.values,
+                                S.dinv64.data(), x, r, work, ld, shadow_dim, bnorm * bnorm,
+                                target, max_it, A, b, 0.0, &it, &replacements, &rr);
+    } else {
+      // mixed: SpMV on an FP32 copy of the values; FP64 vectors, dots and reliable updates
+      LSMPS_HIP(hipMemcpyAsync(ctx.host_i32, A.row_offsets + n, sizeof(int32_t),
+                               hipMemcpyDeviceToHost, ctx.stream));
+      ctx.sync();
+      const size_t used = static_cast<size_t>(ctx.host_i32[0]);
+      S.vals32.reserve(std::max<size_t>(used, 1));
+      to_f32_kernel<<<blocks_for(used), kBlock, 0, ctx.stream>>>(used, A.values, S.vals32.data());
       LSMPS_LAUNCH_CHECK();
-      int it = 0;
-      double rr_inner = 0.0;
-      const double rnorm = std::sqrt(rr);
-      idr_solve<float>(ctx, S, n, A.row_offsets, A.col_indices, S.vals32.data(), S.dinv32.data(),
-                       e, r32, work, ld, shadow_dim, rr, std::max(inner_tol * rnorm, 0.5 * target),
-                       max_it - total, nullptr, nullptr, &it, &rr_inner);
-      total += it;
-      add_f32_to_f64_kernel<<<blocks_for(n), kBlock, 0, ctx.stream>>>(n, e, x);
-      LSMPS_LAUNCH_CHECK();
-      const double rr_new = true_residual(ctx, S, A, x, b, r64, r32);
-      ++outer;
-      if (!std::isfinite(rr_new)) {
-        rr = rr_new;
-        break;
-      }
-      rr = rr_new;
-      if (it == 0) break;
+      const double factor = opt.inner_tolerance > 0.0 ? opt.inner_tolerance : 1.0e-3;
+      idr_solve<double, float>(ctx, S, n, A.row_offsets, A.col_indices, S.vals32.data(),
+                               S.dinv64.data(), x, r, work, ld, shadow_dim, bnorm * bnorm, target,
+                               max_it, A, b, factor, &it, &replacements, &rr);
     }
-    rep.iterations = total;
-    rep.refinements = outer;
+    rep.iterations = it;
+    rep.refinements = replacements;
     rep.relative_residual = std::sqrt(rr) / bnorm;
     rep.converged = rep.relative_residual <= opt.tolerance ? 1 : 0;
   }

@@ -29,7 +29,7 @@ EXPORTS = [
     "lsmps_stepper_set_config", "lsmps_stepper_set_solver", "lsmps_stepper_bytes",
     "lsmps_stepper_step", "lsmps_stepper_stage_times", "lsmps_stepper_neighbor_lists",
     "lsmps_stepper_ppe_workspace", "lsmps_stepper_temporary_velocity",
-    "lsmps_stepper_correction_vectors", "lsmps_stepper_moments",
+    "lsmps_stepper_correction_vectors", "lsmps_stepper_moments", "lsmps_context_kernel_stats",
 ]
 
 _LIB = None
@@ -56,6 +56,8 @@ def load(path=LIB_PATH):
     lib.lsmps_derive_cell_grid.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t,
                                            C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t,
                                            C.c_void_p]
+    lib.lsmps_context_kernel_stats.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                               C.c_int]
     lib.lsmps_reference_number_density.argtypes = [C.c_double, C.c_double, C.c_int32,
                                                    C.POINTER(C.c_double)]
     _LIB = lib

@@ -1,0 +1,135 @@
+"""HIP path vs the FP64 oracle, through the C-ABI (liblsmps3d_amd.so).
+
+Bars: neighbor CSR and surface labels bit-exact; FP64-mode floating results
+within 1e-9 relative L2 (the oracle and the GPU both solve the PPE to a 1e-10
+relative true residual); mixed-mode pressure within the 1e-5 relative L2 the
+target sets.  Velocities after the correction are compared relative to the
+magnitude of the predictor u*, because in near-hydrostatic states
+u = u* - dt/rho grad p cancels to a tiny remainder.
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+from helpers import brute_force_neighbors
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-9
+MIXED_PRESSURE_TOL = 1e-5
+
+
+def _small_cases(cases):
+    return [
+        cases.hydrostatic(n=8, l0=0.01),
+        cases.hydrostatic(n=20, l0=0.01),
+        cases.dam_break(block=(30, 20, 20), tank=(60, 20, 40), l0=0.01, seed=0),
+        cases.splash(n=24, l0=0.01, seed=3, top_layers=3),
+    ]
+
+
+@pytest.fixture(scope="module")
+def small(cases):
+    return _small_cases(cases)
+
+
+@pytest.fixture(scope="module")
+def oracle_runs(oracle, small):
+    return {c.name: oracle.pipeline(c) for c in small}
+
+
+def _vec_err(got, ref):
+    """Relative L2 error of a 3-component field, normalised by the whole field
+    (a single component can be ~1e-9 of the others, e.g. grad_x in a hydrostatic box)."""
+    num = sum(float(np.sum((np.asarray(a) - np.asarray(b)) ** 2)) for a, b in zip(got, ref))
+    den = sum(float(np.sum(np.asarray(b) ** 2)) for b in ref)
+    return float(np.sqrt(num / den)) if den > 0 else float(np.sqrt(num))
+
+
+def _check_geometry(g, ref):
+    nb = ref["neighbors"]
+    assert np.array_equal(g["fluid_offsets"], nb["fluid_offsets"])
+    assert np.array_equal(g["fluid_indices"], nb["fluid_indices"])
+    assert np.array_equal(g["wall_offsets"], nb["wall_offsets"])
+    assert np.array_equal(g["wall_indices"], nb["wall_indices"])
+    assert np.array_equal(g["surface_type"], ref["surface"]["surface_type"])
+
+
+@pytest.mark.parametrize("idx", [0, 1, 2, 3])
+def test_step_fp64_parity(rt, abi, small, oracle_runs, idx):
+    case = small[idx]
+    ref = oracle_runs[case.name]
+    g = rt.run_step_on_device(case, precision=abi.PRECISION_FP64)
+    print(case.name, case.n_fluid, g["report"], flush=True)
+    _check_geometry(g, ref)
+    pv = ref["provision"]
+    for k in ("vx", "vy", "vz", "wall_vx", "wall_vy", "wall_vz"):
+        assert rel_l2(g["provision"][k], pv[k]) <= FP64_TOL, k
+    pp = ref["ppe"]
+    assert np.array_equal(g["ppe"]["row_offsets"], pp["row_offsets"])
+    assert np.array_equal(g["ppe"]["col_indices"], pp["col_indices"])
+    assert g["ppe"]["nnz"] == pp["nnz"]
+    assert rel_l2(g["ppe"]["values"], pp["values"]) <= FP64_TOL
+    assert rel_l2(g["ppe"]["rhs"], pp["rhs"]) <= FP64_TOL
+    assert rel_l2(g["ppe"]["divergence"], pp["divergence"]) <= FP64_TOL
+    assert g["report"]["converged"] == 1
+    assert rel_l2(g["pressure"], ref["pressure"]) <= FP64_TOL
+    cg = ref["correction"]
+    assert _vec_err(g["grad"], cg["grad"]) <= 1e-8
+    assert _vec_err(g["ps"], cg["ps"]) <= 1e-8
+    for k in ("x", "y", "z"):
+        assert rel_l2(g["fluid"][k], cg["fluid"][k]) <= 1e-12
+    scale = max(np.linalg.norm(pv["vz"]), 1e-300)
+    for k in ("vx", "vy", "vz"):
+        assert np.linalg.norm(g["fluid"][k] - cg["fluid"][k]) / scale <= 1e-8, k
+
+
+@pytest.mark.parametrize("idx", [1, 2, 3])
+def test_step_mixed_parity(rt, abi, small, oracle_runs, idx):
+    case = small[idx]
+    ref = oracle_runs[case.name]
+    g = rt.run_step_on_device(case, precision=abi.PRECISION_MIXED)
+    print(case.name, case.n_fluid, g["report"], flush=True)
+    _check_geometry(g, ref)
+    assert g["report"]["converged"] == 1
+    assert g["report"]["relative_residual"] <= 1e-10
+    assert rel_l2(g["pressure"], ref["pressure"]) <= MIXED_PRESSURE_TOL
+    cg = ref["correction"]
+    assert _vec_err(g["grad"], cg["grad"]) <= 1e-5
+    scale = max(np.linalg.norm(ref["provision"]["vz"]), 1e-300)
+    for k in ("vx", "vy", "vz"):
+        assert np.linalg.norm(g["fluid"][k] - cg["fluid"][k]) / scale <= 1e-5, k
+
+
+def test_neighbor_brute_force_jittered(rt, abi, cases):
+    """Neighbor sets against O(N^2) distance tests; order = oracle canonical order."""
+    case = cases.dam_break(block=(14, 9, 9), tank=(20, 9, 14), l0=0.01, seed=5, amplitude=0.3)
+    g = rt.run_step_on_device(case, precision=abi.PRECISION_FP64)
+    f = case.fluid
+    w = case.walls
+    re = case.config.support_radius
+    rows = brute_force_neighbors(f["x"], f["y"], f["z"], f["x"], f["y"], f["z"], re, True)
+    wrows = brute_force_neighbors(f["x"], f["y"], f["z"], w["x"], w["y"], w["z"], re, False)
+    for i in range(case.n_fluid):
+        got = g["fluid_indices"][g["fluid_offsets"][i]:g["fluid_offsets"][i + 1]]
+        assert sorted(got.tolist()) == rows[i].tolist()
+        got = g["wall_indices"][g["wall_offsets"][i]:g["wall_offsets"][i + 1]]
+        assert sorted(got.tolist()) == wrows[i].tolist()
+
+
+def test_mixed_vs_fp64_at_250k(rt, abi, cases, oracle):
+    """Dam-break C2 (250,000 fluid): CSR and labels bit-exact against the oracle;
+    mixed and FP64 pressures agree to 1e-5 and both meet the 1e-10 residual."""
+    case = cases.dam_break()
+    nb = oracle.neighbors(case)
+    cls = oracle.classify(case, nb, diagnostics=False)
+    g64 = rt.run_step_on_device(case, precision=abi.PRECISION_FP64)
+    print("fp64", g64["report"], g64["stage_ms"], flush=True)
+    assert np.array_equal(g64["fluid_offsets"], nb["fluid_offsets"])
+    assert np.array_equal(g64["fluid_indices"], nb["fluid_indices"])
+    assert np.array_equal(g64["wall_indices"], nb["wall_indices"])
+    assert np.array_equal(g64["surface_type"], cls["surface_type"])
+    g32 = rt.run_step_on_device(case, precision=abi.PRECISION_MIXED)
+    print("mixed", g32["report"], g32["stage_ms"], flush=True)
+    assert g64["report"]["converged"] == 1 and g32["report"]["converged"] == 1
+    assert rel_l2(g32["pressure"], g64["pressure"]) <= MIXED_PRESSURE_TOL
