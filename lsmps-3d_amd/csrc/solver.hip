@@ -148,6 +148,131 @@ __global__ void __launch_bounds__(kBlock) spmv_kernel(SpmvArgs<T, V> a) {
   if (MODE != 0) block_reduce_store<NQ>(acc, MODE == 1 ? a.s : (MODE == 2 ? 2 : 1), a.partials);
 }
 
+// ---- sliced ELLPACK (SELL-64) ---------------------------------------------
+
+// Width of each 64-row chunk (one wavefront per chunk); also width * 64 for the scan.
+__global__ void sell_width_kernel(size_t n, size_t chunks, const int32_t* rows, int32_t* width,
+                                  int32_t* scaled) {
+  const size_t t = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const size_t chunk = t / kSellChunk;
+  const size_t r = t;
+  int32_t len = r < n ? rows[r + 1] - rows[r] : 0;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) len = max(len, __shfl_xor(len, off, 64));
+  if ((threadIdx.x & 63) == 0 && chunk < chunks) {
+    width[chunk] = len;
+    scaled[chunk] = len * kSellChunk;
+  }
+}
+
+template <typename V>
+__global__ void sell_fill_kernel(size_t n, size_t chunks, const int32_t* rows,
+                                 const int32_t* cols, const double* vals,
+                                 const int32_t* chunk_start, const int32_t* width,
+                                 int32_t* out_cols, V* out_vals) {
+  const size_t r = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (r >= chunks * kSellChunk) return;
+  const size_t chunk = r / kSellChunk;
+  const int lane = static_cast<int>(r % kSellChunk);
+  const int64_t base = chunk_start[chunk];
+  const int32_t w = width[chunk];
+  const int32_t b = r < n ? rows[r] : 0;
+  const int32_t len = r < n ? rows[r + 1] - b : 0;
+  const int32_t self = static_cast<int32_t>(r < n ? r : n - 1);
+  for (int32_t k = 0; k < w; ++k) {
+    const int64_t idx = base + static_cast<int64_t>(k) * kSellChunk + lane;
+    if (k < len) {
+      out_cols[idx] = cols[b + k];
+      out_vals[idx] = static_cast<V>(vals[b + k]);
+    } else {
+      out_cols[idx] = self;
+      out_vals[idx] = V(0);
+    }
+  }
+}
+
+template <typename T, typename V>
+struct SellArgs {
+  size_t n;
+  size_t chunks;
+  const int32_t* chunk_start;
+  const int32_t* width;
+  const int32_t* cols;
+  const V* vals;
+  const T* x;
+  T* y;
+  const T* p0;  // mode 1
+  int s;        // mode 1
+  const T* r;   // mode 2
+  double* partials;
+};
+
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.
+// Give XCD k a contiguous slab of the rows (and so of the gathered x entries,
+// which are spatial neighbors) by renumbering block b as
+// (b % 8) * (gridDim / 8) + b / 8; the grid is a multiple of 8.
+constexpr int kXcds = 8;
+__device__ __forceinline__ unsigned xcd_block() {
+  const unsigned per = gridDim.x / kXcds;
+  return (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+}
+
+// One wavefront per 64-row chunk, lane = row.  Each workgroup owns a contiguous
+// range of chunks (XCD-aware order above); its 4 waves interleave within it.
+// MODE 1: shadow dots P_q . y.  MODE 2: y.r and y.y.
+template <typename T, typename V, int MODE>
+__global__ void __launch_bounds__(kBlock) sell_spmv_kernel(SellArgs<T, V> a) {
+  constexpr int NQ = MODE == 1 ? kMaxShadow : (MODE == 2 ? 2 : 1);
+  const int lane = threadIdx.x & 63;
+  const unsigned wave_in_block = threadIdx.x >> 6;
+  const size_t per_block = (a.chunks + gridDim.x - 1) / gridDim.x;
+  const size_t first = static_cast<size_t>(xcd_block()) * per_block;
+  const size_t last = first + per_block < a.chunks ? first + per_block : a.chunks;
+  double acc[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) acc[q] = 0.0;
+  for (size_t chunk = first + wave_in_block; chunk < last; chunk += kBlock / 64) {
+    const int64_t base = a.chunk_start[chunk];
+    const int32_t w = a.width[chunk];
+    const V* vp = a.vals + base + lane;
+    const int32_t* cp = a.cols + base + lane;
+    // 8 columns in flight per lane: all column and value loads of a group are
+    // issued before the dependent gathers of x.
+    constexpr int U = 8;
+    T s[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) s[u] = T(0);
+    int32_t k = 0;
+    for (; k + U <= w; k += U) {
+      int32_t c[U];
+      V v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        c[u] = __builtin_nontemporal_load(cp + (k + u) * kSellChunk);
+        v[u] = __builtin_nontemporal_load(vp + (k + u) * kSellChunk);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) s[u] += static_cast<T>(v[u]) * a.x[c[u]];
+    }
+    for (; k < w; ++k) s[0] += static_cast<T>(vp[k * kSellChunk]) * a.x[cp[k * kSellChunk]];
+    const T sum = ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
+    const size_t row = chunk * kSellChunk + lane;
+    if (row < a.n) {
+      a.y[row] = sum;
+      const double ys = static_cast<double>(sum);
+      if (MODE == 1) {
+#pragma unroll
+        for (int q = 0; q < NQ; ++q)
+          if (q < a.s) acc[q] += shadow(a.p0, row, q) * ys;
+      } else if (MODE == 2) {
+        acc[0] += ys * static_cast<double>(a.r[row]);
+        acc[1 % NQ] += ys * ys;
+      }
+    }
+  }
+  if (MODE != 0) block_reduce_store<NQ>(acc, MODE == 1 ? a.s : (MODE == 2 ? 2 : 1), a.partials);
+}
+
 // r = b - A x in FP64; optional FP32 copy of r; partial r.r.
 __global__ void __launch_bounds__(kBlock) residual_kernel(size_t n, const int32_t* rows,
                                                           const int32_t* cols, const double* vals,
@@ -182,16 +307,6 @@ __global__ void jacobi_kernel(size_t n, const int32_t* rows, const int32_t* cols
   for (int32_t c = rows[i]; c < rows[i + 1]; ++c)
     if (cols[c] == static_cast<int32_t>(i)) d += vals[c];
   dinv[i] = static_cast<T>(d != 0.0 ? 1.0 / d : 1.0);
-}
-
-__global__ void to_f32_kernel(size_t n, const double* src, float* dst) {
-  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i < n) dst[i] = static_cast<float>(src[i]);
-}
-
-__global__ void add_f32_to_f64_kernel(size_t n, const float* e, double* x) {
-  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i < n) x[i] += static_cast<double>(e[i]);
 }
 
 template <typename T>
@@ -474,6 +589,52 @@ unsigned spmv_blocks(size_t n) {
   return static_cast<unsigned>(std::max<size_t>(1, std::min<size_t>(b, kMaxReduceBlocks)));
 }
 
+// A multiple of the XCD count (see xcd_block), at most kMaxReduceBlocks.
+unsigned sell_blocks(size_t chunks) {
+  const size_t waves_per_block = kBlock / 64;
+  size_t b = (chunks + waves_per_block - 1) / waves_per_block;
+  b = std::min<size_t>(b, kMaxReduceBlocks);
+  b = (b + kXcds - 1) / kXcds * kXcds;
+  return static_cast<unsigned>(std::max<size_t>(kXcds, std::min<size_t>(b, kMaxReduceBlocks)));
+}
+
+// Converts the CSR (FP64 values) into the SELL-64 layout with values of type V.
+template <typename V>
+void build_sell(Context& ctx, SellMatrix& M, const lsmps_csr& A, V* out_vals_dummy_type) {
+  (void)out_vals_dummy_type;
+  const size_t n = A.rows;
+  M.n = n;
+  M.chunks = (n + kSellChunk - 1) / kSellChunk;
+  M.chunk_width.reserve(M.chunks);
+  M.chunk_scaled.reserve(M.chunks);
+  M.chunk_start.reserve(M.chunks + 1);
+  sell_width_kernel<<<blocks_for(M.chunks * kSellChunk), kBlock, 0, ctx.stream>>>(
+      n, M.chunks, A.row_offsets, M.chunk_width.data(), M.chunk_scaled.data());
+  LSMPS_LAUNCH_CHECK();
+  exclusive_scan_i32(ctx, M.chunk_scaled.data(), M.chunk_start.data(), M.chunks);
+  LSMPS_HIP(hipMemcpyAsync(ctx.host_i32, M.chunk_start.data() + M.chunks, sizeof(int32_t),
+                           hipMemcpyDeviceToHost, ctx.stream));
+  LSMPS_HIP(hipMemcpyAsync(ctx.host_i32 + 1, A.row_offsets + n, sizeof(int32_t),
+                           hipMemcpyDeviceToHost, ctx.stream));
+  ctx.sync();
+  require(ctx.host_i32[0] >= 0, "PPE matrix too large for the 32-bit SELL layout");
+  M.entries = static_cast<size_t>(ctx.host_i32[0]);
+  M.nnz = static_cast<size_t>(ctx.host_i32[1]);
+  M.cols.reserve(std::max<size_t>(M.entries, 1));
+  V* vals;
+  if (sizeof(V) == sizeof(double)) {
+    M.vals64.reserve(std::max<size_t>(M.entries, 1));
+    vals = reinterpret_cast<V*>(M.vals64.data());
+  } else {
+    M.vals32.reserve(std::max<size_t>(M.entries, 1));
+    vals = reinterpret_cast<V*>(M.vals32.data());
+  }
+  sell_fill_kernel<V><<<blocks_for(M.chunks * kSellChunk), kBlock, 0, ctx.stream>>>(
+      n, M.chunks, A.row_offsets, A.col_indices, A.values, M.chunk_start.data(),
+      M.chunk_width.data(), M.cols.data(), vals);
+  LSMPS_LAUNCH_CHECK();
+}
+
 double read_rr(Context& ctx, PpeSolver& S) {
   LSMPS_HIP(hipMemcpyAsync(S.host, &S.scalars.data()->rr, sizeof(double), hipMemcpyDeviceToHost,
                            ctx.stream));
@@ -501,8 +662,8 @@ double true_residual(Context& ctx, PpeSolver& S, const lsmps_csr& A, const doubl
 // Krylov state (reliable updates); the solve stops when the true residual meets
 // the target.
 template <typename T, typename V>
-int idr_solve(Context& ctx, PpeSolver& S, size_t n, const int32_t* rows, const int32_t* cols,
-              const V* vals, const T* dinv, T* x, T* r, T* work, size_t ld, int s, double rr0,
+int idr_solve(Context& ctx, PpeSolver& S, size_t n, const SellMatrix& M, const V* vals,
+              const T* dinv, T* x, T* r, T* work, size_t ld, int s, double rr0,
               double target, int max_iterations, const lsmps_csr& A64, const double* b64,
               double replace_factor, int* iterations_out, int* replacements_out,
               double* rr_out) {
@@ -513,7 +674,7 @@ int idr_solve(Context& ctx, PpeSolver& S, size_t n, const int32_t* rows, const i
   T* U = work + (3 + s) * ld;
   IdrVec<T> iv{n, ld, s, r, x, v, t, G, U, p0, dinv, S.scalars.data(), S.part_b.data()};
   const unsigned vb = vec_blocks(n);
-  const unsigned sb = spmv_blocks(n);
+  const unsigned sb = sell_blocks(M.chunks);
   fill_kernel<T><<<blocks_for(2 * s * ld), kBlock, 0, ctx.stream>>>(2 * s * ld, G, T(0));
   copy_kernel<T><<<blocks_for(n), kBlock, 0, ctx.stream>>>(n, r, p0);
   idr_init_scalars<<<1, kBlock, 0, ctx.stream>>>(S.scalars.data(), s);
@@ -529,12 +690,10 @@ int idr_solve(Context& ctx, PpeSolver& S, size_t n, const int32_t* rows, const i
     for (auto& e : ctx.timing_events) LSMPS_HIP(hipEventCreate(&e));
     ctx.timing_ready = true;
   }
-  LSMPS_HIP(hipMemcpyAsync(ctx.host_i32, rows + n, sizeof(int32_t), hipMemcpyDeviceToHost,
-                           ctx.stream));
-  ctx.sync();
-  const double nnz = static_cast<double>(ctx.host_i32[0]);
-  const double spmv_bytes = nnz * (sizeof(V) + sizeof(int32_t)) + (n + 1) * sizeof(int32_t) +
-                            2.0 * n * sizeof(T);
+  // Algorithmic bytes of one SpMV: the stored nonzeros (value + column), the
+  // per-chunk descriptors, x read once and y written once.  Padding is not counted.
+  const double spmv_bytes = static_cast<double>(M.nnz) * (sizeof(V) + sizeof(int32_t)) +
+                            M.chunks * 2.0 * sizeof(int32_t) + 2.0 * n * sizeof(T);
   auto mark = [&](int slot) {
     if (timing) LSMPS_HIP(hipEventRecord(ctx.timing_events[slot], ctx.stream));
   };
@@ -545,10 +704,10 @@ int idr_solve(Context& ctx, PpeSolver& S, size_t n, const int32_t* rows, const i
                                                static_cast<int>(vb), s);
     for (int k = 0; k < s; ++k) {
       idr_a_kernel<T><<<vb, kBlock, 0, ctx.stream>>>(iv, k);
-      SpmvArgs<T, V> sa{n, rows, cols, vals, U + k * ld, G + k * ld, p0, s, nullptr,
-                        S.part_a.data()};
+      SellArgs<T, V> sa{n, M.chunks, M.chunk_start.data(), M.chunk_width.data(), M.cols.data(),
+                        vals, U + k * ld, G + k * ld, p0, s, nullptr, S.part_a.data()};
       mark(2 * k);
-      spmv_kernel<T, V, 1><<<sb, kBlock, 0, ctx.stream>>>(sa);
+      sell_spmv_kernel<T, V, 1><<<sb, kBlock, 0, ctx.stream>>>(sa);
       mark(2 * k + 1);
       idr_scalar_after<<<1, kBlock, 0, ctx.stream>>>(S.scalars.data(), S.part_a.data(),
                                                      static_cast<int>(sb), s, k);
@@ -556,9 +715,10 @@ int idr_solve(Context& ctx, PpeSolver& S, size_t n, const int32_t* rows, const i
       idr_b_kernel<T><<<vb, kBlock, 0, ctx.stream>>>(iv, k, k == s - 1 ? 1 : 0);
       ++it;
     }
-    SpmvArgs<T, V> st{n, rows, cols, vals, v, t, nullptr, s, r, S.part_a.data()};
+    SellArgs<T, V> st{n, M.chunks, M.chunk_start.data(), M.chunk_width.data(), M.cols.data(),
+                      vals, v, t, nullptr, s, r, S.part_a.data()};
     mark(2 * s);
-    spmv_kernel<T, V, 2><<<sb, kBlock, 0, ctx.stream>>>(st);
+    sell_spmv_kernel<T, V, 2><<<sb, kBlock, 0, ctx.stream>>>(st);
     mark(2 * s + 1);
     idr_scalar_omega<<<1, kBlock, 0, ctx.stream>>>(S.scalars.data(), S.part_a.data(),
                                                    S.part_b.data(), static_cast<int>(sb),
@@ -707,22 +867,17 @@ void ppe_solve(Context& ctx, PpeSolver& S, const lsmps_csr& A, const double* b, 
     int it = 0, replacements = 0;
     double rr = 0.0;
     if (ctx.precision == LSMPS_PRECISION_FP64) {
-      idr_solve<double, double>(ctx, S, n, A.row_offsets, A.col_indices, A.values,
-                                S.dinv64.data(), x, r, work, ld, shadow_dim, bnorm * bnorm,
-                                target, max_it, A, b, 0.0, &it, &replacements, &rr);
+      build_sell<double>(ctx, S.sell, A, nullptr);
+      idr_solve<double, double>(ctx, S, n, S.sell, S.sell.vals64.data(), S.dinv64.data(), x, r,
+                                work, ld, shadow_dim, bnorm * bnorm, target, max_it, A, b, 0.0,
+                                &it, &replacements, &rr);
     } else {
       // mixed: SpMV on an FP32 copy of the values; FP64 vectors, dots and reliable updates
-      LSMPS_HIP(hipMemcpyAsync(ctx.host_i32, A.row_offsets + n, sizeof(int32_t),
-                               hipMemcpyDeviceToHost, ctx.stream));
-      ctx.sync();
-      const size_t used = static_cast<size_t>(ctx.host_i32[0]);
-      S.vals32.reserve(std::max<size_t>(used, 1));
-      to_f32_kernel<<<blocks_for(used), kBlock, 0, ctx.stream>>>(used, A.values, S.vals32.data());
-      LSMPS_LAUNCH_CHECK();
+      build_sell<float>(ctx, S.sell, A, nullptr);
       const double factor = opt.inner_tolerance > 0.0 ? opt.inner_tolerance : 1.0e-3;
-      idr_solve<double, float>(ctx, S, n, A.row_offsets, A.col_indices, S.vals32.data(),
-                               S.dinv64.data(), x, r, work, ld, shadow_dim, bnorm * bnorm, target,
-                               max_it, A, b, factor, &it, &replacements, &rr);
+      idr_solve<double, float>(ctx, S, n, S.sell, S.sell.vals32.data(), S.dinv64.data(), x, r,
+                               work, ld, shadow_dim, bnorm * bnorm, target, max_it, A, b, factor,
+                               &it, &replacements, &rr);
     }
     rep.iterations = it;
     rep.refinements = replacements;

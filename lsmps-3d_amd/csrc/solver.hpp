@@ -7,7 +7,7 @@ namespace lsmps {
 
 constexpr int kMaxShadow = 16;           // largest IDR(s) shadow dimension
 constexpr int kPartialStride = kMaxShadow + 4;
-constexpr int kMaxReduceBlocks = 2048;
+constexpr int kMaxReduceBlocks = 4096;
 
 // Small device-resident state of one Krylov solve; kernels read and write it
 // directly so no host round trip happens per iteration.
@@ -26,8 +26,30 @@ struct SolverScalars {
   int32_t pad;
 };
 
+// Sliced ELLPACK with 64-row chunks (one wavefront per chunk): entry k of the
+// chunk's row r sits at chunk_start[c] + k * 64 + r, padded to the chunk's
+// widest row with value 0 and the row's own index as column.
+constexpr int kSellChunk = 64;
+struct SellMatrix {
+  size_t n = 0;
+  size_t chunks = 0;
+  size_t entries = 0;  // padded
+  size_t nnz = 0;      // stored CSR nonzeros
+  DeviceArray<int32_t> chunk_start;  // padded entry count < 2^31 is checked at build
+  DeviceArray<int32_t> chunk_width;
+  DeviceArray<int32_t> chunk_scaled;
+  DeviceArray<int32_t> cols;
+  DeviceArray<double> vals64;
+  DeviceArray<float> vals32;
+  size_t bytes() const {
+    return chunk_start.bytes() + chunk_width.bytes() + cols.bytes() + vals64.bytes() +
+           vals32.bytes();
+  }
+};
+
 class PpeSolver {
  public:
+  SellMatrix sell;
   DeviceArray<double> dinv64;
   DeviceArray<float> dinv32;
   DeviceArray<float> vals32;
@@ -44,8 +66,8 @@ class PpeSolver {
   PpeSolver(const PpeSolver&) = delete;
   PpeSolver& operator=(const PpeSolver&) = delete;
   size_t bytes() const {
-    return dinv64.bytes() + dinv32.bytes() + vals32.bytes() + vec64.bytes() + vec32.bytes() +
-           outer64.bytes() + part_a.bytes() + part_b.bytes() + scalars.bytes();
+    return sell.bytes() + dinv64.bytes() + dinv32.bytes() + vals32.bytes() + vec64.bytes() +
+           vec32.bytes() + outer64.bytes() + part_a.bytes() + part_b.bytes() + scalars.bytes();
   }
 };
 
