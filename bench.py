@@ -192,9 +192,11 @@ def main():
                                 "C2": "dam-break 100x50x50 (250,000 fluid) in 200x50x100 tank",
                                 "C1": "hydrostatic 20x20x20 (8,000 fluid)"}[args.case],
                    "n_fluid": case.n_fluid, "n_wall": case.n_wall, "l0_m": case.config.particle_spacing,
-                   "re_over_l0": 3.1, "dt_s": case.config.time_step, "solver": "Jacobi-IDR(8)",
+                   "re_over_l0": 3.1, "dt_s": case.config.time_step, "solver": "IDR(8); Schwarz blocks on wrong-sign rows, Jacobi elsewhere",
                    "tolerance": 1e-10, "precision": args.precision},
         "solver_iterations": iters,
+        "schwarz_blocks": main_run["reports"][-1]["schwarz_blocks"],
+        "schwarz_rows": main_run["reports"][-1]["schwarz_rows"],
         "relative_residual": [r["relative_residual"] for r in main_run["reports"]],
         "stage_ms": dict(zip(["geometry", "moments", "provision", "assembly", "solve",
                               "correction", "total"], np.round(main_run["stage_ms"], 3).tolist())),

@@ -258,8 +258,14 @@ typedef struct lsmps_solver_options {
   int32_t check_interval;  /* PCG: iterations between host convergence checks */
   int32_t clamp_negative;  /* clamp p<0 after the solve, as ppe_matrix.cu:662 (default 1) */
   int32_t shadow_dimension; /* IDR(s): s, 1..16 (default 8) */
-  int32_t reserved0;
+  int32_t preconditioner;   /* LSMPS_PRECOND_* (default SCHWARZ) */
 } lsmps_solver_options;
+
+/* Preconditioner of the Krylov PPE solve.  JACOBI: D^-1.  SCHWARZ: Jacobi plus
+ * exact local solves (restricted additive Schwarz) on the 2-hop neighbourhoods
+ * of the rows with a positive diagonal, which make the reference's PPE
+ * indefinite.  Either way the matrix is the reference's. */
+enum { LSMPS_PRECOND_JACOBI = 0, LSMPS_PRECOND_SCHWARZ = 1 };
 
 typedef struct lsmps_solve_report {
   int32_t method;
@@ -268,6 +274,8 @@ typedef struct lsmps_solve_report {
   int32_t converged;
   double relative_residual;   /* true FP64 relative residual at exit */
   double rhs_norm;
+  int32_t schwarz_blocks;     /* LSMPS_PRECOND_SCHWARZ: dense blocks (wrong-sign rows) */
+  int32_t schwarz_rows;       /* LSMPS_PRECOND_SCHWARZ: rows summed over all blocks */
 } lsmps_solve_report;
 
 /* Opaque objects */

@@ -31,6 +31,9 @@ SOLVER_PCG = 0
 SOLVER_BICGSTAB = 1
 SOLVER_IDRS = 2
 
+PRECOND_JACOBI = 0
+PRECOND_SCHWARZ = 1
+
 FLAG_ASYNC = 1
 FLAG_KERNEL_TIMING = 2
 
@@ -137,13 +140,14 @@ class SolverOptions(C.Structure):
     _fields_ = [("method", C.c_int32), ("max_iterations", C.c_int32),
                 ("tolerance", C.c_double), ("inner_tolerance", C.c_double),
                 ("check_interval", C.c_int32), ("clamp_negative", C.c_int32),
-                ("shadow_dimension", C.c_int32), ("reserved0", C.c_int32)]
+                ("shadow_dimension", C.c_int32), ("preconditioner", C.c_int32)]
 
 
 class SolveReport(C.Structure):
     _fields_ = [("method", C.c_int32), ("iterations", C.c_int32), ("refinements", C.c_int32),
                 ("converged", C.c_int32), ("relative_residual", C.c_double),
-                ("rhs_norm", C.c_double)]
+                ("rhs_norm", C.c_double), ("schwarz_blocks", C.c_int32),
+                ("schwarz_rows", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -197,7 +201,7 @@ def default_solver_options():
     o.check_interval = 16
     o.clamp_negative = 1
     o.shadow_dimension = 8
-    o.reserved0 = 0
+    o.preconditioner = PRECOND_SCHWARZ
     return o
 
 

@@ -26,6 +26,7 @@ SOURCES = {
     "ppe.hip": ["-ffp-contract=fast"],
     "solver.hip": ["-ffp-contract=fast"],
     "correction.hip": ["-ffp-contract=fast"],
+    "schwarz.hip": ["-ffp-contract=fast"],
     "stepper.cpp": [],
     "capi.cpp": [],
 }

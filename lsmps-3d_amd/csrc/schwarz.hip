@@ -1,0 +1,403 @@
+// Restricted additive Schwarz correction for the PPE's wrong-sign rows.
+//
+// The reference's PPE (src/ppe/ppe_matrix.cu:243-370) has a few hundred rows
+// with a positive diagonal: NearSurface particles in wall corners whose
+// truncated, wall-dominated neighbourhoods give the LSMPS Laplacian row the
+// wrong sign.  They make the system indefinite and dominate Krylov
+// convergence (Jacobi-IDR(8) on the 250,000-particle dam-break: ~4000 SpMVs).
+// This preconditioner keeps Jacobi on every other row and, around each
+// wrong-sign row, solves the local problem exactly: the block of its 2-hop
+// neighbourhood (1-hop if that exceeds kMaxBlock rows) is factored by LU with
+// partial pivoting into an explicit inverse; rows covered by several blocks take
+// the average of their blocks' values, summed in block order (restricted
+// additive Schwarz with averaging).  The
+// matrix itself is not modified, so the solution is that of the reference system.
+//
+// Setup, per solve: one workgroup per seed row builds the member set in an LDS
+// hash table, sorts it (deterministic membership and order), assembles the dense
+// block, factors it and forms the inverse (column-major, so the application's
+// dense matvec reads coalesced columns).
+#include <algorithm>
+#include <vector>
+
+#include "common.hpp"
+#include "schwarz.hpp"
+
+namespace lsmps {
+namespace {
+
+constexpr int kTable = 4096;  // LDS hash table entries (power of two)
+
+// Per row: 1 when eligible to join a block (has off-diagonal couplings),
+// 2 when it is also a seed (positive diagonal).  Dirichlet rows (identity rows
+// of Surface / Splash particles) have no off-diagonal values and stay Jacobi.
+__global__ void classify_rows_kernel(size_t n, const int32_t* rows, const int32_t* cols,
+                                     const double* vals, int32_t* eligible, int32_t* seed) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double diag = 0.0, off = 0.0;
+  for (int32_t c = rows[i]; c < rows[i + 1]; ++c) {
+    if (cols[c] == static_cast<int32_t>(i)) {
+      diag += vals[c];
+    } else {
+      off += fabs(vals[c]);
+    }
+  }
+  eligible[i] = off > 0.0 ? 1 : 0;
+  seed[i] = (off > 0.0 && diag > 0.0) ? 1 : 0;
+}
+
+__global__ void compact_seeds_kernel(size_t n, const int32_t* flag, const int32_t* pos,
+                                     int32_t* seeds) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n && flag[i]) seeds[pos[i]] = static_cast<int32_t>(i);
+}
+
+// Linear probing, bounded by the table size: the callers stop inserting once
+// more than kMaxBlock keys are present, far below kTable, so a probe sequence
+// always finds its key or a free slot; the bound only guarantees termination.
+__device__ void table_insert(int32_t* table, int32_t key, int32_t* count) {
+  uint32_t h = (static_cast<uint32_t>(key) * 2654435761u) & (kTable - 1);
+  for (int probe = 0; probe < kTable; ++probe) {
+    const int32_t prev = atomicCAS(&table[h], -1, key);
+    if (prev == -1) {
+      atomicAdd(count, 1);
+      return;
+    }
+    if (prev == key) return;
+    h = (h + 1) & (kTable - 1);
+  }
+}
+
+// Member set of block b (1 workgroup of kMaxBlock threads per block).
+__global__ void __launch_bounds__(kMaxBlock) members_kernel(
+    const int32_t* seeds, const int32_t* rows, const int32_t* cols, const int32_t* eligible,
+    int32_t* members, int32_t* sizes) {
+  __shared__ int32_t table[kTable];
+  __shared__ int32_t list[kMaxBlock];
+  __shared__ int32_t count;
+  __shared__ int32_t filled;
+  const int t = threadIdx.x;
+  const int32_t seed = seeds[blockIdx.x];
+  for (int e = t; e < kTable; e += blockDim.x) table[e] = -1;
+  if (t == 0) count = 0;
+  __syncthreads();
+  // 1-hop: the seed row's columns (the seed itself is its diagonal entry)
+  const int32_t sb = rows[seed], se = rows[seed + 1];
+  for (int32_t c = sb + t; c < se; c += blockDim.x) {
+    const int32_t j = cols[c];
+    if (eligible[j]) table_insert(table, j, &count);
+  }
+  __syncthreads();
+  // 2-hop: columns of every eligible 1-hop member.  `count` is read between two
+  // barriers so that every thread sees the same value (no insert of the next
+  // row can overtake a slower wave's read) and the loop exits uniformly.
+  bool two_hop = true;
+  for (int32_t m = sb; m < se; ++m) {
+    const int32_t j = cols[m];
+    if (!eligible[j]) continue;
+    for (int32_t c = rows[j] + t; c < rows[j + 1]; c += blockDim.x) {
+      const int32_t k = cols[c];
+      if (eligible[k]) table_insert(table, k, &count);
+    }
+    __syncthreads();
+    const int32_t now = count;
+    __syncthreads();
+    if (now > kMaxBlock) {
+      two_hop = false;
+      break;
+    }
+  }
+  if (!two_hop) {  // fall back to the 1-hop set
+    for (int e = t; e < kTable; e += blockDim.x) table[e] = -1;
+    if (t == 0) count = 0;
+    __syncthreads();
+    for (int32_t c = sb + t; c < se; c += blockDim.x) {
+      const int32_t j = cols[c];
+      if (eligible[j]) table_insert(table, j, &count);
+    }
+    __syncthreads();
+  }
+  // compact and sort ascending (bitonic sort over kMaxBlock slots)
+  if (t == 0) filled = 0;
+  list[t] = INT32_MAX;
+  __syncthreads();
+  for (int e = t; e < kTable; e += blockDim.x) {
+    const int32_t key = table[e];
+    if (key >= 0) {
+      const int32_t slot = atomicAdd(&filled, 1);
+      if (slot < kMaxBlock) list[slot] = key;  // a >kMaxBlock 1-hop row is truncated
+    }
+  }
+  __syncthreads();
+  if (t == 0 && filled > kMaxBlock) filled = kMaxBlock;
+  for (int size = 2; size <= kMaxBlock; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      const int partner = t ^ stride;
+      if (partner > t) {
+        const bool up = (t & size) == 0;
+        const int32_t a = list[t], b = list[partner];
+        if ((a > b) == up) {
+          list[t] = b;
+          list[partner] = a;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  members[static_cast<size_t>(blockIdx.x) * kMaxBlock + t] = list[t];
+  if (t == 0) sizes[blockIdx.x] = filled;
+}
+
+__device__ int find_member(const int32_t* list, int n, int32_t key) {
+  int lo = 0, hi = n - 1;
+  while (lo <= hi) {
+    const int mid = (lo + hi) >> 1;
+    const int32_t v = list[mid];
+    if (v == key) return mid;
+    if (v < key) lo = mid + 1; else hi = mid - 1;
+  }
+  return -1;
+}
+
+// Dense block, LU with partial pivoting, explicit inverse (column-major).
+template <typename V>
+__global__ void __launch_bounds__(kMaxBlock) factor_kernel(
+    const int32_t* rows, const int32_t* cols, const double* vals, const int32_t* members,
+    const int32_t* sizes, double* scratch, V* inverse) {
+  __shared__ int32_t list[kMaxBlock];
+  __shared__ int32_t piv[kMaxBlock];
+  __shared__ double red_v[kMaxBlock];
+  __shared__ int32_t red_i[kMaxBlock];
+  const int t = threadIdx.x;
+  const int n = sizes[blockIdx.x];
+  constexpr int ld = kMaxBlock;
+  double* a = scratch + static_cast<size_t>(blockIdx.x) * ld * ld;  // row-major
+  V* inv = inverse + static_cast<size_t>(blockIdx.x) * ld * ld;     // column-major
+  list[t] = members[static_cast<size_t>(blockIdx.x) * kMaxBlock + t];
+  for (int e = t; e < ld * ld; e += blockDim.x) a[e] = 0.0;
+  __syncthreads();
+  if (t < n) {
+    const int32_t row = list[t];
+    for (int32_t c = rows[row]; c < rows[row + 1]; ++c) {
+      const int p = find_member(list, n, cols[c]);
+      if (p >= 0) a[t * ld + p] += vals[c];
+    }
+  }
+  __syncthreads();
+  for (int k = 0; k < n; ++k) {
+    // pivot: largest |a[r][k]|, r >= k (ties -> smallest r)
+    const double v = (t >= k && t < n) ? fabs(a[t * ld + k]) : -1.0;
+    red_v[t] = v;
+    red_i[t] = t;
+    __syncthreads();
+    for (int w = kMaxBlock / 2; w > 0; w >>= 1) {
+      if (t < w) {
+        const double o = red_v[t + w];
+        const int32_t oi = red_i[t + w];
+        if (o > red_v[t] || (o == red_v[t] && oi < red_i[t])) {
+          red_v[t] = o;
+          red_i[t] = oi;
+        }
+      }
+      __syncthreads();
+    }
+    const int p = red_i[0];
+    if (t == 0) piv[k] = p;
+    if (p != k) {
+      for (int j = t; j < n; j += blockDim.x) {
+        const double x = a[k * ld + j];
+        a[k * ld + j] = a[p * ld + j];
+        a[p * ld + j] = x;
+      }
+    }
+    __syncthreads();
+    double akk = a[k * ld + k];
+    if (akk == 0.0) akk = 1.0e-300;  // singular block: keep finite, Krylov absorbs it
+    for (int r = k + 1 + t; r < n; r += blockDim.x) a[r * ld + k] /= akk;
+    __syncthreads();
+    const int m = n - k - 1;
+    for (int e = t; e < m * m; e += blockDim.x) {
+      const int r = k + 1 + e / m, j = k + 1 + e % m;
+      a[r * ld + j] -= a[r * ld + k] * a[k * ld + j];
+    }
+    __syncthreads();
+  }
+  // Inverse column t (thread t): solve P A x = e_t, i.e. L U x = P e_t.  The
+  // right-hand sides are stored X[i * ld + t] so that the substitutions, which
+  // advance i in lockstep over all threads, read and write coalesced rows while
+  // the factor entries a[i][j] are broadcast.
+  if (t < n) {
+    double* X = scratch + static_cast<size_t>(gridDim.x) * ld * ld +
+                static_cast<size_t>(blockIdx.x) * ld * ld;
+    for (int i = 0; i < n; ++i) X[i * ld + t] = (i == t) ? 1.0 : 0.0;
+    for (int k = 0; k < n; ++k) {
+      const int p = piv[k];
+      if (p != k) {
+        const double x = X[k * ld + t];
+        X[k * ld + t] = X[p * ld + t];
+        X[p * ld + t] = x;
+      }
+    }
+    for (int i = 0; i < n; ++i) {  // unit lower
+      double s = X[i * ld + t];
+      for (int j = 0; j < i; ++j) s -= a[i * ld + j] * X[j * ld + t];
+      X[i * ld + t] = s;
+    }
+    for (int i = n - 1; i >= 0; --i) {  // upper
+      double s = X[i * ld + t];
+      for (int j = i + 1; j < n; ++j) s -= a[i * ld + j] * X[j * ld + t];
+      double d = a[i * ld + i];
+      if (d == 0.0) d = 1.0e-300;
+      X[i * ld + t] = s / d;
+    }
+    // inverse(i, t) at t * ld + i (column-major): the application kernel reads
+    // column j as inv[j * ld + row], coalesced over its threads.
+    for (int i = 0; i < n; ++i) inv[static_cast<size_t>(t) * ld + i] = static_cast<V>(X[i * ld + t]);
+  }
+}
+
+// Number of blocks covering each row.
+__global__ void cover_count_kernel(const int32_t* members, const int32_t* sizes, int32_t* cover) {
+  const int t = threadIdx.x;
+  if (t < sizes[blockIdx.x]) atomicAdd(&cover[members[static_cast<size_t>(blockIdx.x) * kMaxBlock + t]], 1);
+}
+
+// Row -> covering block slots (b * kMaxBlock + position), unordered.
+__global__ void cover_fill_kernel(const int32_t* members, const int32_t* sizes,
+                                  const int32_t* start, int32_t* cursor, int32_t* entries) {
+  const int t = threadIdx.x;
+  if (t >= sizes[blockIdx.x]) return;
+  const int32_t row = members[static_cast<size_t>(blockIdx.x) * kMaxBlock + t];
+  const int32_t k = atomicAdd(&cursor[row], 1);
+  entries[start[row] + k] = static_cast<int32_t>(blockIdx.x) * kMaxBlock + t;
+}
+
+// Ascending slot order per row, so the averaged value is summed in block order
+// and is the same on every run.
+__global__ void cover_sort_kernel(size_t n, const int32_t* start, int32_t* entries) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int32_t b = start[i], e = start[i + 1];
+  for (int32_t p = b + 1; p < e; ++p) {
+    const int32_t v = entries[p];
+    int32_t q = p - 1;
+    while (q >= b && entries[q] > v) {
+      entries[q + 1] = entries[q];
+      --q;
+    }
+    entries[q + 1] = v;
+  }
+}
+
+// y_b = inv_b w_b for every block b; slot b * kMaxBlock + position.
+template <typename T, typename V>
+__global__ void __launch_bounds__(kMaxBlock) apply_kernel(const int32_t* members,
+                                                          const int32_t* sizes, const V* inverse,
+                                                          const T* w, double* ybuf) {
+  __shared__ double wb[kMaxBlock];
+  const int t = threadIdx.x;
+  const int n = sizes[blockIdx.x];
+  constexpr int ld = kMaxBlock;
+  const int32_t* mem = members + static_cast<size_t>(blockIdx.x) * kMaxBlock;
+  const V* inv = inverse + static_cast<size_t>(blockIdx.x) * ld * ld;
+  const int32_t me = t < n ? mem[t] : 0;
+  wb[t] = t < n ? static_cast<double>(w[me]) : 0.0;
+  __syncthreads();
+  if (t < n) {
+    double y = 0.0;
+    for (int j = 0; j < n; ++j) y += static_cast<double>(inv[static_cast<size_t>(j) * ld + t]) * wb[j];
+    ybuf[static_cast<size_t>(blockIdx.x) * kMaxBlock + t] = y;
+  }
+}
+
+}  // namespace
+
+void schwarz_setup(Context& ctx, SchwarzPrec& P, const lsmps_csr& A, bool fp32_inverse) {
+  const size_t n = A.rows;
+  P.n = n;
+  P.blocks = 0;
+  P.fp32 = fp32_inverse;
+  P.eligible.reserve(n);
+  P.seed_flag.reserve(n);
+  P.seed_pos.reserve(n + 1);
+  classify_rows_kernel<<<blocks_for(n), kBlock, 0, ctx.stream>>>(
+      n, A.row_offsets, A.col_indices, A.values, P.eligible.data(), P.seed_flag.data());
+  LSMPS_LAUNCH_CHECK();
+  exclusive_scan_i32(ctx, P.seed_flag.data(), P.seed_pos.data(), n);
+  LSMPS_HIP(hipMemcpyAsync(ctx.host_i32, P.seed_pos.data() + n, sizeof(int32_t),
+                           hipMemcpyDeviceToHost, ctx.stream));
+  ctx.sync();
+  const size_t nb = static_cast<size_t>(ctx.host_i32[0]);
+  P.blocks = nb;
+  if (nb == 0) return;
+  P.seeds.reserve(nb);
+  compact_seeds_kernel<<<blocks_for(n), kBlock, 0, ctx.stream>>>(n, P.seed_flag.data(),
+                                                                 P.seed_pos.data(), P.seeds.data());
+  P.members.reserve(nb * kMaxBlock);
+  P.sizes.reserve(nb);
+  members_kernel<<<static_cast<unsigned>(nb), kMaxBlock, 0, ctx.stream>>>(
+      P.seeds.data(), A.row_offsets, A.col_indices, P.eligible.data(), P.members.data(),
+      P.sizes.data());
+  LSMPS_LAUNCH_CHECK();
+  const size_t dense = nb * kMaxBlock * kMaxBlock;
+  P.scratch.reserve(2 * dense);
+  if (fp32_inverse) {
+    P.inv32.reserve(dense);
+    factor_kernel<float><<<static_cast<unsigned>(nb), kMaxBlock, 0, ctx.stream>>>(
+        A.row_offsets, A.col_indices, A.values, P.members.data(), P.sizes.data(),
+        P.scratch.data(), P.inv32.data());
+  } else {
+    P.inv64.reserve(dense);
+    factor_kernel<double><<<static_cast<unsigned>(nb), kMaxBlock, 0, ctx.stream>>>(
+        A.row_offsets, A.col_indices, A.values, P.members.data(), P.sizes.data(),
+        P.scratch.data(), P.inv64.data());
+  }
+  LSMPS_LAUNCH_CHECK();
+  // Row -> covering slots (CSR over rows), sorted per row.
+  P.cover.reserve(n);
+  P.cover_start.reserve(n + 1);
+  P.ybuf.reserve(nb * kMaxBlock);
+  fill_i32(ctx.stream, P.cover.data(), n, 0);
+  cover_count_kernel<<<static_cast<unsigned>(nb), kMaxBlock, 0, ctx.stream>>>(
+      P.members.data(), P.sizes.data(), P.cover.data());
+  LSMPS_LAUNCH_CHECK();
+  exclusive_scan_i32(ctx, P.cover.data(), P.cover_start.data(), n);
+  P.cover_entries.reserve(nb * kMaxBlock);
+  fill_i32(ctx.stream, P.cover.data(), n, 0);
+  cover_fill_kernel<<<static_cast<unsigned>(nb), kMaxBlock, 0, ctx.stream>>>(
+      P.members.data(), P.sizes.data(), P.cover_start.data(), P.cover.data(),
+      P.cover_entries.data());
+  cover_sort_kernel<<<blocks_for(n), kBlock, 0, ctx.stream>>>(n, P.cover_start.data(),
+                                                             P.cover_entries.data());
+  LSMPS_LAUNCH_CHECK();
+}
+
+template <typename T>
+void schwarz_accumulate(Context& ctx, const SchwarzPrec& P, const T* w) {
+  if (P.blocks == 0) return;
+  if (P.fp32) {
+    apply_kernel<T, float><<<static_cast<unsigned>(P.blocks), kMaxBlock, 0, ctx.stream>>>(
+        P.members.data(), P.sizes.data(), P.inv32.data(), w, P.ybuf.data());
+  } else {
+    apply_kernel<T, double><<<static_cast<unsigned>(P.blocks), kMaxBlock, 0, ctx.stream>>>(
+        P.members.data(), P.sizes.data(), P.inv64.data(), w, P.ybuf.data());
+  }
+  LSMPS_LAUNCH_CHECK();
+}
+
+template void schwarz_accumulate<double>(Context&, const SchwarzPrec&, const double*);
+template void schwarz_accumulate<float>(Context&, const SchwarzPrec&, const float*);
+
+void schwarz_stats(Context& ctx, const SchwarzPrec& P, int64_t* blocks, int64_t* members) {
+  *blocks = static_cast<int64_t>(P.blocks);
+  *members = 0;
+  if (P.blocks == 0) return;
+  std::vector<int32_t> sizes(P.blocks);
+  LSMPS_HIP(hipMemcpyAsync(sizes.data(), P.sizes.data(), P.blocks * sizeof(int32_t),
+                           hipMemcpyDeviceToHost, ctx.stream));
+  ctx.sync();
+  for (int32_t s : sizes) *members += s;
+}
+
+}  // namespace lsmps

@@ -75,28 +75,44 @@ __device__ __forceinline__ void block_reduce_store(const double (&acc)[NQ], int 
     }
   }
   __syncthreads();
-  if (static_cast<int>(threadIdx.x) < nq) {
+  // All kPartialStride slots are written (unused ones as 0) so the all-column
+  // reduction below only ever reads initialised memory.
+  if (static_cast<int>(threadIdx.x) < kPartialStride) {
     double s = 0.0;
+    if (static_cast<int>(threadIdx.x) < nq) {
 #pragma unroll
-    for (int w = 0; w < kBlock / 64; ++w) s += red[w][threadIdx.x];
+      for (int w = 0; w < kBlock / 64; ++w) s += red[w][threadIdx.x];
+    }
     partials[static_cast<size_t>(blockIdx.x) * kPartialStride + threadIdx.x] = s;
   }
 }
 
-// Fixed-order sum of partials[b][q] over b < nb (one block).  Result in all threads.
-__device__ double block_sum_partials(const double* partials, int nb, int q) {
-  __shared__ double red[kBlock];
-  double s = 0.0;
-  for (int b = threadIdx.x; b < nb; b += kBlock) s += partials[static_cast<size_t>(b) * kPartialStride + q];
-  red[threadIdx.x] = s;
+// Fixed-order sums over b < nb of partials[b][q] for every q < kPartialStride
+// at once, by one block of kReduceThreads.  The partials are one contiguous
+// array of nb * kPartialStride doubles and kReduceThreads is a multiple of
+// kPartialStride, so thread t always reads column t % kPartialStride: every
+// pass is one coalesced sweep.  Results land in out[0 .. kPartialStride).
+constexpr int kReduceThreads = 1024;
+static_assert(kReduceThreads % kPartialStride == 0, "column-stable sweep");
+__device__ void reduce_partials(const double* partials, int nb, double* out) {
+  __shared__ double red[kReduceThreads];
+  const int t = threadIdx.x;
+  const int total = nb * kPartialStride;
+  double s0 = 0.0, s1 = 0.0;
+  int e = t;
+  for (; e + kReduceThreads < total; e += 2 * kReduceThreads) {
+    s0 += partials[e];
+    s1 += partials[e + kReduceThreads];
+  }
+  if (e < total) s0 += partials[e];
+  red[t] = s0 + s1;
   __syncthreads();
-  for (int w = kBlock / 2; w > 0; w >>= 1) {
-    if (static_cast<int>(threadIdx.x) < w) red[threadIdx.x] += red[threadIdx.x + w];
+  for (int w = kReduceThreads / 2; w >= kPartialStride; w >>= 1) {
+    if (t < w) red[t] += red[t + w];
     __syncthreads();
   }
-  const double total = red[0];
+  if (t < kPartialStride) out[t] = red[t];
   __syncthreads();
-  return total;
 }
 
 template <typename T, typename V>
@@ -351,7 +367,58 @@ struct IdrVec {
   const T* dinv;
   SolverScalars* sc;
   double* partials;
+  const int32_t* cover_start;    // Schwarz (or null: Jacobi only): row -> block slots
+  const int32_t* cover_entries;
+  const double* ybuf;            // Schwarz block products of the last application
 };
+
+// Preconditioned value of row i of w: the mean of its Schwarz block values
+// (summed in block order) where blocks cover the row, Jacobi elsewhere.
+template <typename T>
+__device__ __forceinline__ double precondition(const IdrVec<T>& a, size_t i, double w) {
+  if (a.cover_start) {
+    const int32_t lo = a.cover_start[i], hi = a.cover_start[i + 1];
+    if (hi > lo) {
+      double z = 0.0;
+      for (int32_t e = lo; e < hi; ++e) z += a.ybuf[a.cover_entries[e]];
+      return z / static_cast<double>(hi - lo);
+    }
+  }
+  return static_cast<double>(a.dinv[i]) * w;
+}
+
+// Schwarz path, first half of idr_a: v = r - sum_{q>=k} c_q G_q (unpreconditioned).
+template <typename T>
+__global__ void __launch_bounds__(kBlock) idr_a1_kernel(IdrVec<T> a, int k) {
+  const SolverScalars* sc = a.sc;
+  for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < a.n;
+       i += static_cast<size_t>(gridDim.x) * blockDim.x) {
+    double v = static_cast<double>(a.r[i]);
+    for (int q = k; q < a.s; ++q) v -= sc->c[q] * static_cast<double>(a.G[q * a.ld + i]);
+    a.v[i] = static_cast<T>(v);
+  }
+}
+
+// Schwarz path, second half: U_k = sum_{q>=k} c_q U_q + om M^-1 v.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) idr_a2_kernel(IdrVec<T> a, int k) {
+  const SolverScalars* sc = a.sc;
+  const double om = sc->om;
+  for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < a.n;
+       i += static_cast<size_t>(gridDim.x) * blockDim.x) {
+    double u = 0.0;
+    for (int q = k; q < a.s; ++q) u += sc->c[q] * static_cast<double>(a.U[q * a.ld + i]);
+    a.U[k * a.ld + i] = static_cast<T>(u + om * precondition(a, i, static_cast<double>(a.v[i])));
+  }
+}
+
+// v = M^-1 v in place (dimension-reduction step on the Schwarz path).
+template <typename T>
+__global__ void precondition_kernel(IdrVec<T> a) {
+  for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < a.n;
+       i += static_cast<size_t>(gridDim.x) * blockDim.x)
+    a.v[i] = static_cast<T>(precondition(a, i, static_cast<double>(a.v[i])));
+}
 
 __global__ void idr_init_scalars(SolverScalars* sc, int s) {
   const int t = threadIdx.x;
@@ -378,22 +445,21 @@ __device__ void solve_c(SolverScalars* sc, int s, int k) {
 }
 
 // Cycle start: f = P^T r (from partials), c = M^-1 f.
-__global__ void idr_scalar_f(SolverScalars* sc, const double* partials, int nb, int s) {
-  for (int q = 0; q < s; ++q) {
-    const double v = block_sum_partials(partials, nb, q);
-    if (threadIdx.x == 0) sc->f[q] = v;
-  }
-  if (threadIdx.x == 0) solve_c(sc, s, 0);
+__global__ void __launch_bounds__(kReduceThreads)
+    idr_scalar_f(SolverScalars* sc, const double* partials, int nb, int s) {
+  __shared__ double d[kPartialStride];
+  reduce_partials(partials, nb, d);
+  if (threadIdx.x != 0) return;
+  for (int q = 0; q < s; ++q) sc->f[q] = d[q];
+  solve_c(sc, s, 0);
 }
 
 // After G_k = A U_k: biorthogonalisation coefficients, new column k of M,
 // beta, f update and c for the next k.
-__global__ void idr_scalar_after(SolverScalars* sc, const double* partials, int nb, int s, int k) {
-  __shared__ double d[kMaxShadow];
-  for (int q = 0; q < s; ++q) {
-    const double v = block_sum_partials(partials, nb, q);
-    if (threadIdx.x == 0) d[q] = v;
-  }
+__global__ void __launch_bounds__(kReduceThreads)
+    idr_scalar_after(SolverScalars* sc, const double* partials, int nb, int s, int k) {
+  __shared__ double d[kPartialStride];
+  reduce_partials(partials, nb, d);
   if (threadIdx.x != 0) return;
   double alpha[kMaxShadow];
   for (int i = 0; i < k; ++i) {
@@ -420,12 +486,15 @@ __global__ void idr_scalar_after(SolverScalars* sc, const double* partials, int 
 }
 
 // omega = (t.r)/(t.t) with the 0.7 "maintaining the convergence" safeguard.
-__global__ void idr_scalar_omega(SolverScalars* sc, const double* part_t, const double* part_r,
-                                 int nb_t, int nb_r) {
-  const double tr = block_sum_partials(part_t, nb_t, 0);
-  const double tt = block_sum_partials(part_t, nb_t, 1);
-  const double rr = block_sum_partials(part_r, nb_r, 0);
+__global__ void __launch_bounds__(kReduceThreads)
+    idr_scalar_omega(SolverScalars* sc, const double* part_t, const double* part_r, int nb_t,
+                     int nb_r) {
+  __shared__ double dt[kPartialStride];
+  __shared__ double dr[kPartialStride];
+  reduce_partials(part_t, nb_t, dt);
+  reduce_partials(part_r, nb_r, dr);
   if (threadIdx.x != 0) return;
+  const double tr = dt[0], tt = dt[1], rr = dr[0];
   if (!(tt > 0.0) || !(rr > 0.0)) {
     sc->om = 0.0;
     if (!(rr > 0.0)) return;  // already converged exactly
@@ -438,9 +507,11 @@ __global__ void idr_scalar_omega(SolverScalars* sc, const double* part_t, const 
   sc->om = om;
 }
 
-__global__ void scalar_norm(SolverScalars* sc, const double* partials, int nb) {
-  const double v = block_sum_partials(partials, nb, 0);
-  if (threadIdx.x == 0) sc->rr = v;
+__global__ void __launch_bounds__(kReduceThreads)
+    scalar_norm(SolverScalars* sc, const double* partials, int nb) {
+  __shared__ double d[kPartialStride];
+  reduce_partials(partials, nb, d);
+  if (threadIdx.x == 0) sc->rr = d[0];
 }
 
 // v = D^-1 (r - sum_{q>=k} c_q G_q);  U_k = sum_{q>=k} c_q U_q + om v
@@ -484,7 +555,8 @@ __global__ void __launch_bounds__(kBlock) idr_b_kernel(IdrVec<T> a, int k, int w
     a.r[i] = static_cast<T>(r);
     a.x[i] = static_cast<T>(static_cast<double>(a.x[i]) + beta * u);
     acc[0] += r * r;
-    if (write_v) a.v[i] = static_cast<T>(static_cast<double>(a.dinv[i]) * r);
+    // write_v: 1 = Jacobi-preconditioned residual, 2 = raw residual (Schwarz path)
+    if (write_v) a.v[i] = static_cast<T>(write_v == 1 ? static_cast<double>(a.dinv[i]) * r : r);
   }
   block_reduce_store<1>(acc, 1, a.partials);
 }
@@ -521,17 +593,22 @@ __global__ void __launch_bounds__(kBlock) shadow_dots_kernel(IdrVec<T> a) {
 }
 
 // ---- PCG ----------------------------------------------------------------
-__global__ void cg_scalar_rz(SolverScalars* sc, const double* part, int nb) {
-  const double rz = block_sum_partials(part, nb, 0);
-  const double rr = block_sum_partials(part, nb, 1);
+__global__ void __launch_bounds__(kReduceThreads)
+    cg_scalar_rz(SolverScalars* sc, const double* part, int nb) {
+  __shared__ double d[kPartialStride];
+  reduce_partials(part, nb, d);
+  const double rz = d[0], rr = d[1];
   if (threadIdx.x == 0) {
     sc->cg_beta = sc->rz != 0.0 ? rz / sc->rz : 0.0;
     sc->rz = rz;
     sc->rr = rr;
   }
 }
-__global__ void cg_scalar_alpha(SolverScalars* sc, const double* part, int nb) {
-  const double pq = block_sum_partials(part, nb, 0);
+__global__ void __launch_bounds__(kReduceThreads)
+    cg_scalar_alpha(SolverScalars* sc, const double* part, int nb) {
+  __shared__ double d[kPartialStride];
+  reduce_partials(part, nb, d);
+  const double pq = d[0];
   if (threadIdx.x == 0) {
     if (pq == 0.0 || !isfinite(pq)) {
       sc->breakdown = 1;
@@ -650,7 +727,7 @@ double true_residual(Context& ctx, PpeSolver& S, const lsmps_csr& A, const doubl
   residual_kernel<<<nb, kBlock, 0, ctx.stream>>>(n, A.row_offsets, A.col_indices, A.values, x, b,
                                                  r, r32, S.part_b.data());
   LSMPS_LAUNCH_CHECK();
-  scalar_norm<<<1, kBlock, 0, ctx.stream>>>(S.scalars.data(), S.part_b.data(), static_cast<int>(nb));
+  scalar_norm<<<1, kReduceThreads, 0, ctx.stream>>>(S.scalars.data(), S.part_b.data(), static_cast<int>(nb));
   LSMPS_LAUNCH_CHECK();
   return read_rr(ctx, S);
 }
@@ -672,7 +749,11 @@ int idr_solve(Context& ctx, PpeSolver& S, size_t n, const SellMatrix& M, const V
   T* p0 = work + 2 * ld;
   T* G = work + 3 * ld;
   T* U = work + (3 + s) * ld;
-  IdrVec<T> iv{n, ld, s, r, x, v, t, G, U, p0, dinv, S.scalars.data(), S.part_b.data()};
+  const bool schwarz = S.schwarz.blocks > 0 && S.schwarz.n == n;
+  IdrVec<T> iv{n, ld, s, r, x, v, t, G, U, p0, dinv, S.scalars.data(), S.part_b.data(),
+               schwarz ? S.schwarz.cover_start.data() : nullptr,
+               schwarz ? S.schwarz.cover_entries.data() : nullptr,
+               schwarz ? S.schwarz.ybuf.data() : nullptr};
   const unsigned vb = vec_blocks(n);
   const unsigned sb = sell_blocks(M.chunks);
   fill_kernel<T><<<blocks_for(2 * s * ld), kBlock, 0, ctx.stream>>>(2 * s * ld, G, T(0));
@@ -700,32 +781,42 @@ int idr_solve(Context& ctx, PpeSolver& S, size_t n, const SellMatrix& M, const V
   while (rr > target2 && it < max_iterations) {
     iv.partials = S.part_a.data();
     shadow_dots_kernel<T><<<vb, kBlock, 0, ctx.stream>>>(iv);
-    idr_scalar_f<<<1, kBlock, 0, ctx.stream>>>(S.scalars.data(), S.part_a.data(),
+    idr_scalar_f<<<1, kReduceThreads, 0, ctx.stream>>>(S.scalars.data(), S.part_a.data(),
                                                static_cast<int>(vb), s);
     for (int k = 0; k < s; ++k) {
-      idr_a_kernel<T><<<vb, kBlock, 0, ctx.stream>>>(iv, k);
+      if (schwarz) {
+        idr_a1_kernel<T><<<vb, kBlock, 0, ctx.stream>>>(iv, k);
+        schwarz_accumulate<T>(ctx, S.schwarz, v);
+        idr_a2_kernel<T><<<vb, kBlock, 0, ctx.stream>>>(iv, k);
+      } else {
+        idr_a_kernel<T><<<vb, kBlock, 0, ctx.stream>>>(iv, k);
+      }
       SellArgs<T, V> sa{n, M.chunks, M.chunk_start.data(), M.chunk_width.data(), M.cols.data(),
                         vals, U + k * ld, G + k * ld, p0, s, nullptr, S.part_a.data()};
       mark(2 * k);
       sell_spmv_kernel<T, V, 1><<<sb, kBlock, 0, ctx.stream>>>(sa);
       mark(2 * k + 1);
-      idr_scalar_after<<<1, kBlock, 0, ctx.stream>>>(S.scalars.data(), S.part_a.data(),
+      idr_scalar_after<<<1, kReduceThreads, 0, ctx.stream>>>(S.scalars.data(), S.part_a.data(),
                                                      static_cast<int>(sb), s, k);
       iv.partials = S.part_b.data();
-      idr_b_kernel<T><<<vb, kBlock, 0, ctx.stream>>>(iv, k, k == s - 1 ? 1 : 0);
+      idr_b_kernel<T><<<vb, kBlock, 0, ctx.stream>>>(iv, k, k == s - 1 ? (schwarz ? 2 : 1) : 0);
       ++it;
+    }
+    if (schwarz) {
+      schwarz_accumulate<T>(ctx, S.schwarz, v);
+      precondition_kernel<T><<<vb, kBlock, 0, ctx.stream>>>(iv);
     }
     SellArgs<T, V> st{n, M.chunks, M.chunk_start.data(), M.chunk_width.data(), M.cols.data(),
                       vals, v, t, nullptr, s, r, S.part_a.data()};
     mark(2 * s);
     sell_spmv_kernel<T, V, 2><<<sb, kBlock, 0, ctx.stream>>>(st);
     mark(2 * s + 1);
-    idr_scalar_omega<<<1, kBlock, 0, ctx.stream>>>(S.scalars.data(), S.part_a.data(),
+    idr_scalar_omega<<<1, kReduceThreads, 0, ctx.stream>>>(S.scalars.data(), S.part_a.data(),
                                                    S.part_b.data(), static_cast<int>(sb),
                                                    static_cast<int>(vb));
     iv.partials = S.part_b.data();
     idr_c_kernel<T><<<vb, kBlock, 0, ctx.stream>>>(iv);
-    scalar_norm<<<1, kBlock, 0, ctx.stream>>>(S.scalars.data(), S.part_b.data(),
+    scalar_norm<<<1, kReduceThreads, 0, ctx.stream>>>(S.scalars.data(), S.part_b.data(),
                                               static_cast<int>(vb));
     LSMPS_LAUNCH_CHECK();
     ++it;
@@ -779,7 +870,7 @@ void pcg_solve(Context& ctx, PpeSolver& S, const lsmps_csr& A, const double* b, 
   double rr = true_residual(ctx, S, A, x, b, r, nullptr);
   idr_init_scalars<<<1, kBlock, 0, ctx.stream>>>(S.scalars.data(), 1);
   rz_kernel<<<vb, kBlock, 0, ctx.stream>>>(n, r, dinv, S.part_b.data());
-  cg_scalar_rz<<<1, kBlock, 0, ctx.stream>>>(S.scalars.data(), S.part_b.data(), static_cast<int>(vb));
+  cg_scalar_rz<<<1, kReduceThreads, 0, ctx.stream>>>(S.scalars.data(), S.part_b.data(), static_cast<int>(vb));
   cg_direction_kernel<<<vb, kBlock, 0, ctx.stream>>>(n, p, r, dinv, S.scalars.data(), 1);
   LSMPS_LAUNCH_CHECK();
   int it = 0;
@@ -788,10 +879,10 @@ void pcg_solve(Context& ctx, PpeSolver& S, const lsmps_csr& A, const double* b, 
     SpmvArgs<double, double> sa{n, A.row_offsets, A.col_indices, A.values, p, q, nullptr, 0,
                                 nullptr, S.part_a.data()};
     spmv_kernel<double, double, 3><<<sb, kBlock, 0, ctx.stream>>>(sa);
-    cg_scalar_alpha<<<1, kBlock, 0, ctx.stream>>>(S.scalars.data(), S.part_a.data(), static_cast<int>(sb));
+    cg_scalar_alpha<<<1, kReduceThreads, 0, ctx.stream>>>(S.scalars.data(), S.part_a.data(), static_cast<int>(sb));
     cg_update_kernel<<<vb, kBlock, 0, ctx.stream>>>(n, x, r, p, q, dinv, S.scalars.data(),
                                                     S.part_b.data());
-    cg_scalar_rz<<<1, kBlock, 0, ctx.stream>>>(S.scalars.data(), S.part_b.data(), static_cast<int>(vb));
+    cg_scalar_rz<<<1, kReduceThreads, 0, ctx.stream>>>(S.scalars.data(), S.part_b.data(), static_cast<int>(vb));
     cg_direction_kernel<<<vb, kBlock, 0, ctx.stream>>>(n, p, r, dinv, S.scalars.data(), 0);
     LSMPS_LAUNCH_CHECK();
     ++it;
@@ -832,7 +923,7 @@ void ppe_solve(Context& ctx, PpeSolver& S, const lsmps_csr& A, const double* b, 
   LSMPS_LAUNCH_CHECK();
   // ||b||
   dot_self_kernel<<<vec_blocks(n), kBlock, 0, ctx.stream>>>(n, b, S.part_b.data());
-  scalar_norm<<<1, kBlock, 0, ctx.stream>>>(S.scalars.data(), S.part_b.data(),
+  scalar_norm<<<1, kReduceThreads, 0, ctx.stream>>>(S.scalars.data(), S.part_b.data(),
                                             static_cast<int>(vec_blocks(n)));
   LSMPS_LAUNCH_CHECK();
   const double bnorm = std::sqrt(read_rr(ctx, S));
@@ -866,6 +957,14 @@ void ppe_solve(Context& ctx, PpeSolver& S, const lsmps_csr& A, const double* b, 
     LSMPS_LAUNCH_CHECK();
     int it = 0, replacements = 0;
     double rr = 0.0;
+    S.schwarz.blocks = 0;
+    if (opt.preconditioner == LSMPS_PRECOND_SCHWARZ) {
+      schwarz_setup(ctx, S.schwarz, A, ctx.precision == LSMPS_PRECISION_MIXED);
+      int64_t blocks = 0, rows = 0;
+      schwarz_stats(ctx, S.schwarz, &blocks, &rows);
+      rep.schwarz_blocks = static_cast<int32_t>(blocks);
+      rep.schwarz_rows = static_cast<int32_t>(std::min<int64_t>(rows, INT32_MAX));
+    }
     if (ctx.precision == LSMPS_PRECISION_FP64) {
       build_sell<double>(ctx, S.sell, A, nullptr);
       idr_solve<double, double>(ctx, S, n, S.sell, S.sell.vals64.data(), S.dinv64.data(), x, r,

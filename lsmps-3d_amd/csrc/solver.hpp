@@ -2,11 +2,12 @@
 #pragma once
 
 #include "common.hpp"
+#include "schwarz.hpp"
 
 namespace lsmps {
 
 constexpr int kMaxShadow = 16;           // largest IDR(s) shadow dimension
-constexpr int kPartialStride = kMaxShadow + 4;
+constexpr int kPartialStride = kMaxShadow;  // doubles per block partial record
 constexpr int kMaxReduceBlocks = 4096;
 
 // Small device-resident state of one Krylov solve; kernels read and write it
@@ -50,6 +51,7 @@ struct SellMatrix {
 class PpeSolver {
  public:
   SellMatrix sell;
+  SchwarzPrec schwarz;
   DeviceArray<double> dinv64;
   DeviceArray<float> dinv32;
   DeviceArray<float> vals32;
@@ -66,7 +68,7 @@ class PpeSolver {
   PpeSolver(const PpeSolver&) = delete;
   PpeSolver& operator=(const PpeSolver&) = delete;
   size_t bytes() const {
-    return sell.bytes() + dinv64.bytes() + dinv32.bytes() + vals32.bytes() + vec64.bytes() +
+    return sell.bytes() + schwarz.bytes() + dinv64.bytes() + dinv32.bytes() + vals32.bytes() + vec64.bytes() +
            vec32.bytes() + outer64.bytes() + part_a.bytes() + part_b.bytes() + scalars.bytes();
   }
 };
