@@ -39,6 +39,9 @@ def parse():
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     p.add_argument("--cpu-threads", type=int, default=0)
     p.add_argument("--cpu-solver-iterations", type=int, default=40)
+    p.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
+                   help="only run a plain device-to-device copy of GIB GiB (5 launches) and "
+                        "print its rate: the reference kernel for scaling HBM byte counters")
     return p.parse_args()
 
 
@@ -125,7 +128,8 @@ def cpu_baseline(pkg, case, gpu_iterations, threads, solver_iterations):
     t0 = time.perf_counter()
     p, rep = orc.solve(ppe["row_offsets"], ppe["col_indices"], ppe["values"], ppe["rhs"], opt)
     t_solve = time.perf_counter() - t0
-    per_iter = t_solve / max(rep["iterations"], 1)
+    t["solver_setup"] = rep["setup_seconds"]
+    per_iter = (t_solve - rep["setup_seconds"]) / max(rep["iterations"], 1)
     t0 = time.perf_counter()
     orc.correction(c2, nb, tmp, np.maximum(p, 0.0), pre["inverse"])
     t["correction"] = time.perf_counter() - t0
@@ -137,13 +141,34 @@ def cpu_baseline(pkg, case, gpu_iterations, threads, solver_iterations):
         "cores": orc.threads,
         "kind": "port",
         "sample": (f"FP64 oracle on the same {case.n_fluid}-particle workload: geometry, moments, "
-                   f"provision, assembly and correction timed in full ({fixed:.2f} s); IDR(8) "
-                   f"solver timed for {rep['iterations']} iterations ({per_iter * 1e3:.1f} ms each) "
-                   f"and scaled to the GPU's {gpu_iterations} iterations"),
+                   f"provision, assembly, correction and the solver's Schwarz block setup timed "
+                   f"in full ({fixed:.2f} s); the same IDR(8) solver timed for {rep['iterations']} "
+                   f"iterations ({per_iter * 1e3:.1f} ms each) and scaled to the GPU's "
+                   f"{gpu_iterations} iterations"),
         "stage_seconds": {k: round(v, 3) for k, v in t.items()},
         "solver_ms_per_iteration": per_iter * 1e3,
         "est_step_seconds": est,
     }
+
+
+def copy_calibration(gib):
+    """A plain copy of known size: it reads and writes exactly `nbytes` each
+    launch, so FETCH_SIZE / WRITE_SIZE of this kernel give the counters' scale."""
+    import torch
+    nbytes = int(gib * (1 << 30)) // 8 * 8
+    src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda:0")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(5):
+        dst.copy_(src)
+    stop.record()
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(stop) / 5
+    print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "avg_ms": ms,
+                      "GBps_read_plus_write": 2 * nbytes / (ms * 1e-3) / 1e9}), flush=True)
 
 
 def main():
@@ -152,6 +177,9 @@ def main():
         print(f"bench.py measures one GPU: this build has no multi-GPU path "
               f"(requested --gpus {args.gpus}); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration > 0:
+        copy_calibration(args.copy_calibration)
+        return
     import numpy as np
     from __graft_entry__ import load_package
     pkg = load_package()

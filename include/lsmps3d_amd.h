@@ -77,8 +77,8 @@ enum { LSMPS_WEIGHT_LINEAR = 0, LSMPS_WEIGHT_QUADRATIC = 1 };
 
 /* PPE Krylov method (replaces AmgxPpeSolver's GMRES, ppe_matrix.cu:586-679).
  * The assembled matrix is non-symmetric and indefinite near wall corners of the
- * free surface, so the default is Jacobi-preconditioned IDR(s); BICGSTAB is
- * IDR(1); PCG is only meaningful for symmetric positive systems. */
+ * free surface, so the default is preconditioned IDR(s); BICGSTAB is IDR(1);
+ * PCG (Jacobi only) is only meaningful for symmetric positive systems. */
 enum { LSMPS_SOLVER_PCG = 0, LSMPS_SOLVER_BICGSTAB = 1, LSMPS_SOLVER_IDRS = 2 };
 
 /* Context flags.  KERNEL_TIMING brackets every PPE SpMV launch with hipEvents on
@@ -382,8 +382,11 @@ int lsmps_ppe_assemble(lsmps_context* ctx, const lsmps_config* config,
                        const lsmps_wall_soa* temporary_wall_velocity, lsmps_moment_matrix* moments,
                        uint64_t geometry_generation, const lsmps_ppe_workspace* workspace);
 /* AmgxPpeSolver::solve(const CsrMatrixView&, const real* rhs, real* pressure) —
- * ppe_matrix.cuh:112.  Hand-written Jacobi-PCG / Jacobi-BiCGStab from p = 0.
- * options may be NULL (defaults: PCG, tol 1e-10); report may be NULL. */
+ * ppe_matrix.cuh:112.  Hand-written IDR(s) / BiCGStab / PCG from p = 0, then the
+ * reference's clamp p = max(p, 0) (ppe_matrix.cu:662).  options may be NULL
+ * (lsmps_default_solver_options: IDR(8), Schwarz preconditioner, relative true
+ * residual 1e-10); report may be NULL.  Returns LSMPS_ERR_NOT_CONVERGED (with the
+ * iterate written and the report filled) when max_iterations is reached. */
 int lsmps_ppe_solve(lsmps_context* ctx, const lsmps_csr* matrix, const double* rhs,
                     double* pressure, const lsmps_solver_options* options,
                     lsmps_solve_report* report);

@@ -427,9 +427,6 @@ int lsmps_default_solver_options(lsmps_solver_options* out) {
   API_END("lsmps_default_solver_options")
 }
 
-namespace {
-thread_local PpeSolver* g_solver = nullptr;
-}
 
 int lsmps_ppe_solve(lsmps_context* ctx, const lsmps_csr* matrix, const double* rhs,
                     double* pressure, const lsmps_solver_options* options,
@@ -442,9 +439,9 @@ int lsmps_ppe_solve(lsmps_context* ctx, const lsmps_csr* matrix, const double* r
   } else {
     lsmps_default_solver_options(&opt);
   }
-  // The free solve keeps one workspace per host thread.
-  if (!g_solver) g_solver = new PpeSolver();
-  ppe_solve(c, *g_solver, deref(matrix, "matrix"), rhs, pressure, opt, report);
+  // The free solve keeps its workspace in the context (reused across calls).
+  if (!ctx->solver) ctx->solver = std::make_unique<PpeSolver>();
+  ppe_solve(c, *ctx->solver, deref(matrix, "matrix"), rhs, pressure, opt, report);
   API_END("lsmps_ppe_solve")
 }
 

@@ -7,6 +7,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -135,6 +136,12 @@ inline double config_radius(const lsmps_config& c) { return c.support_radius; }
 
 }  // namespace lsmps
 
+namespace lsmps {
+class PpeSolver;
+}
 struct lsmps_context {
   lsmps::Context impl;
+  // Workspace of the free lsmps_ppe_solve entry (created on first use; the
+  // complete type is visible where contexts are destroyed).
+  std::unique_ptr<lsmps::PpeSolver> solver;
 };

@@ -2,16 +2,20 @@
 //
 // The assembled PPE matrix is non-symmetric and, near wall corners of the free
 // surface, indefinite (NearSurface rows with a positive diagonal), so the
-// default method is Jacobi-preconditioned IDR(s) with biorthogonalisation
+// default method is preconditioned IDR(s) with biorthogonalisation
 // (Sonneveld & van Gijzen, SIAM J. Sci. Comput. 31, 2008; van Gijzen &
 // Sonneveld, ACM TOMS 38, 2011), written from the published algorithm:
 // short recurrences, s+1 SpMVs per cycle, s-dimensional shadow space.  IDR(1)
-// is mathematically BiCGStab, which LSMPS_SOLVER_BICGSTAB selects.
-// Jacobi-PCG (Hestenes-Stiefel) is kept for symmetric systems.
+// is mathematically BiCGStab, which LSMPS_SOLVER_BICGSTAB selects.  The
+// preconditioner is Jacobi, or Jacobi plus restricted additive Schwarz blocks
+// around the wrong-sign rows (schwarz.hip).  Jacobi-PCG (Hestenes-Stiefel) is
+// kept for symmetric systems.
 //
 // GPU structure:
-//  * SpMV: 16 lanes per CSR row, grid-stride over rows, fused epilogue dots
-//    (shadow-space products P^T (A u), or t.r / t.t) into per-block partials.
+//  * SpMV: sliced ELLPACK with 64-row chunks, one wavefront per chunk and one
+//    row per lane, 8 loads in flight per lane, XCD-aware chunk slabs, fused
+//    epilogue dots (shadow-space products P^T (A u), or t.r / t.t) into
+//    per-block partial records of 16 doubles.
 //  * all s x s algebra (M, f, c, alphas, beta, omega) lives in a device struct
 //    updated by one-block "scalar" kernels that reduce the partials in a fixed
 //    order: no host round trip per iteration and bit-reproducible reductions.
@@ -20,10 +24,11 @@
 //    recursive residual the true FP64 residual b - A x is recomputed and must
 //    meet the tolerance.
 //  * mixed mode: the SpMVs of IDR(s) read an FP32 copy of the matrix values
-//    (4 instead of 8 bytes per nonzero); vectors, dots and the true residual
-//    stay FP64, and the recursive residual is replaced by the true FP64
-//    residual b - A x each time it falls by `inner_tolerance` (reliable updates),
-//    so the solve converges to the FP64 system at FP64 accuracy.
+//    (4 instead of 8 bytes per nonzero); the Krylov basis, the iterate, the
+//    residual, every dot product and every accumulation stay FP64, and the
+//    recursive residual is replaced by the true FP64 residual b - A x each time
+//    it falls by `inner_tolerance` (reliable updates), so the solve converges to
+//    the FP64 system at FP64 accuracy.
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -59,10 +64,10 @@ __device__ __forceinline__ double shadow(const T* p0, size_t row, int q) {
   return q == 0 ? static_cast<double>(p0[row]) : hashed_sign(static_cast<uint32_t>(row), q, kShadowSeed);
 }
 
-template <int NQ>
+template <int NQ, int BLOCK = kBlock>
 __device__ __forceinline__ void block_reduce_store(const double (&acc)[NQ], int nq,
                                                    double* partials) {
-  __shared__ double red[kBlock / 64][kPartialStride];
+  __shared__ double red[BLOCK / 64][kPartialStride];
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
 #pragma unroll
@@ -81,7 +86,7 @@ __device__ __forceinline__ void block_reduce_store(const double (&acc)[NQ], int 
     double s = 0.0;
     if (static_cast<int>(threadIdx.x) < nq) {
 #pragma unroll
-      for (int w = 0; w < kBlock / 64; ++w) s += red[w][threadIdx.x];
+      for (int w = 0; w < BLOCK / 64; ++w) s += red[w][threadIdx.x];
     }
     partials[static_cast<size_t>(blockIdx.x) * kPartialStride + threadIdx.x] = s;
   }
@@ -98,14 +103,17 @@ __device__ void reduce_partials(const double* partials, int nb, double* out) {
   __shared__ double red[kReduceThreads];
   const int t = threadIdx.x;
   const int total = nb * kPartialStride;
-  double s0 = 0.0, s1 = 0.0;
+  // four independent loads in flight per thread
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
   int e = t;
-  for (; e + kReduceThreads < total; e += 2 * kReduceThreads) {
+  for (; e + 3 * kReduceThreads < total; e += 4 * kReduceThreads) {
     s0 += partials[e];
     s1 += partials[e + kReduceThreads];
+    s2 += partials[e + 2 * kReduceThreads];
+    s3 += partials[e + 3 * kReduceThreads];
   }
-  if (e < total) s0 += partials[e];
-  red[t] = s0 + s1;
+  for (; e < total; e += kReduceThreads) s0 += partials[e];
+  red[t] = (s0 + s1) + (s2 + s3);
   __syncthreads();
   for (int w = kReduceThreads / 2; w >= kPartialStride; w >>= 1) {
     if (t < w) red[t] += red[t + w];
@@ -181,28 +189,79 @@ __global__ void sell_width_kernel(size_t n, size_t chunks, const int32_t* rows, 
   }
 }
 
+// Windows of one chunk (one wavefront per chunk, lane = row): repeatedly the
+// smallest column not yet covered opens the next window [base, base + 4096).
+// More than kSellWindows windows -> 32-bit columns for this chunk.
+__global__ void sell_window_kernel(size_t n, size_t chunks, const int32_t* rows,
+                                   const int32_t* cols, const int32_t* width, int32_t* nwin,
+                                   int32_t* windows, int32_t* col_rows) {
+  const size_t chunk = (static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x) / kSellChunk;
+  if (chunk >= chunks) return;  // uniform per wavefront
+  const int lane = threadIdx.x & 63;
+  const size_t r = chunk * kSellChunk + lane;
+  const int32_t b = r < n ? rows[r] : 0;
+  const int32_t e = r < n ? rows[r + 1] : 0;
+  int32_t* wb = windows + chunk * kSellWindows;
+  int64_t lo = 0;  // smallest column not yet covered
+  int count = 0;
+  bool wide = false;
+  while (true) {
+    int32_t m = INT32_MAX;
+    for (int32_t c = b; c < e; ++c) {
+      const int32_t v = cols[c];
+      if (v >= lo && v < m) m = v;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = min(m, __shfl_xor(m, off, 64));
+    if (m == INT32_MAX) break;  // every column is covered
+    if (count == kSellWindows) {
+      wide = true;
+      break;
+    }
+    if (lane == 0) wb[count] = m;
+    ++count;
+    lo = static_cast<int64_t>(m) + (1 << kSellWindowBits);
+  }
+  if (count == 0) {  // a chunk without entries: one window at 0 for the padding
+    if (lane == 0) wb[0] = 0;
+    count = 1;
+  }
+  if (lane == 0) {
+    for (int j = count; j < kSellWindows; ++j) wb[j] = wb[count - 1];
+    nwin[chunk] = wide ? -1 : count;
+    col_rows[chunk] = width[chunk] * (wide ? 2 : 1);
+  }
+}
+
+// One thread per (padded) row writes its entries: values, and columns encoded
+// against the chunk's windows (or as 32-bit indices).  Padding points at base_0.
 template <typename V>
 __global__ void sell_fill_kernel(size_t n, size_t chunks, const int32_t* rows,
                                  const int32_t* cols, const double* vals,
                                  const int32_t* chunk_start, const int32_t* width,
-                                 int32_t* out_cols, V* out_vals) {
+                                 const int32_t* col_start, const int32_t* nwin,
+                                 const int32_t* windows, uint16_t* out_cols, V* out_vals) {
   const size_t r = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (r >= chunks * kSellChunk) return;
   const size_t chunk = r / kSellChunk;
   const int lane = static_cast<int>(r % kSellChunk);
-  const int64_t base = chunk_start[chunk];
+  const int64_t vbase = chunk_start[chunk];
+  const int64_t cbase = static_cast<int64_t>(col_start[chunk]) * kSellChunk;
   const int32_t w = width[chunk];
+  const int32_t nw = nwin[chunk];
+  const int32_t* wb = windows + chunk * kSellWindows;
   const int32_t b = r < n ? rows[r] : 0;
   const int32_t len = r < n ? rows[r + 1] - b : 0;
-  const int32_t self = static_cast<int32_t>(r < n ? r : n - 1);
   for (int32_t k = 0; k < w; ++k) {
-    const int64_t idx = base + static_cast<int64_t>(k) * kSellChunk + lane;
-    if (k < len) {
-      out_cols[idx] = cols[b + k];
-      out_vals[idx] = static_cast<V>(vals[b + k]);
+    const int64_t slot = static_cast<int64_t>(k) * kSellChunk + lane;
+    const int32_t c = k < len ? cols[b + k] : wb[0];
+    out_vals[vbase + slot] = k < len ? static_cast<V>(vals[b + k]) : V(0);
+    if (nw < 0) {
+      reinterpret_cast<int32_t*>(out_cols + cbase)[slot] = c;
     } else {
-      out_cols[idx] = self;
-      out_vals[idx] = V(0);
+      int j = 0;
+      while (j + 1 < nw && wb[j + 1] <= c) ++j;
+      out_cols[cbase + slot] = static_cast<uint16_t>((j << kSellWindowBits) | (c - wb[j]));
     }
   }
 }
@@ -213,13 +272,16 @@ struct SellArgs {
   size_t chunks;
   const int32_t* chunk_start;
   const int32_t* width;
-  const int32_t* cols;
+  const int32_t* col_start;
+  const int32_t* nwin;
+  const int32_t* windows;
+  const uint16_t* cols;
   const V* vals;
   const T* x;
   T* y;
-  const T* p0;  // mode 1
-  int s;        // mode 1
-  const T* r;   // mode 2
+  const T* p0;       // mode 1
+  int s;             // mode 1
+  const double* r;   // mode 2: the FP64 residual
   double* partials;
 };
 
@@ -234,10 +296,11 @@ __device__ __forceinline__ unsigned xcd_block() {
 }
 
 // One wavefront per 64-row chunk, lane = row.  Each workgroup owns a contiguous
-// range of chunks (XCD-aware order above); its 4 waves interleave within it.
-// MODE 1: shadow dots P_q . y.  MODE 2: y.r and y.y.
+// range of chunks (XCD-aware order above); its kSellBlock/64 waves interleave
+// within it.  MODE 1: shadow dots P_q . y.  MODE 2: y.r and y.y.
+constexpr int kSellBlock = 512;
 template <typename T, typename V, int MODE>
-__global__ void __launch_bounds__(kBlock) sell_spmv_kernel(SellArgs<T, V> a) {
+__global__ void __launch_bounds__(kSellBlock) sell_spmv_kernel(SellArgs<T, V> a) {
   constexpr int NQ = MODE == 1 ? kMaxShadow : (MODE == 2 ? 2 : 1);
   const int lane = threadIdx.x & 63;
   const unsigned wave_in_block = threadIdx.x >> 6;
@@ -247,46 +310,78 @@ __global__ void __launch_bounds__(kBlock) sell_spmv_kernel(SellArgs<T, V> a) {
   double acc[NQ];
 #pragma unroll
   for (int q = 0; q < NQ; ++q) acc[q] = 0.0;
-  for (size_t chunk = first + wave_in_block; chunk < last; chunk += kBlock / 64) {
-    const int64_t base = a.chunk_start[chunk];
+  for (size_t chunk = first + wave_in_block; chunk < last; chunk += kSellBlock / 64) {
     const int32_t w = a.width[chunk];
-    const V* vp = a.vals + base + lane;
-    const int32_t* cp = a.cols + base + lane;
+    const int32_t nw = a.nwin[chunk];  // uniform per wavefront
+    const V* vp = a.vals + a.chunk_start[chunk] + lane;
+    const int64_t cbase = static_cast<int64_t>(a.col_start[chunk]) * kSellChunk;
     // 8 columns in flight per lane: all column and value loads of a group are
     // issued before the dependent gathers of x.
+    // FP64 accumulation whatever the storage types: the SpMV is bandwidth bound.
     constexpr int U = 8;
-    T s[U];
+    double s[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) s[u] = T(0);
+    for (int u = 0; u < U; ++u) s[u] = 0.0;
     int32_t k = 0;
-    for (; k + U <= w; k += U) {
-      int32_t c[U];
-      V v[U];
+    if (nw >= 0) {
+      // lane l holds window base l % 16; an entry's base is fetched across lanes
+      const int32_t wbase = a.windows[chunk * kSellWindows + (lane & (kSellWindows - 1))];
+      const uint16_t* cp = a.cols + cbase + lane;
+      constexpr int kMask = (1 << kSellWindowBits) - 1;
+      for (; k + U <= w; k += U) {
+        uint16_t c[U];
+        V v[U];
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        c[u] = __builtin_nontemporal_load(cp + (k + u) * kSellChunk);
-        v[u] = __builtin_nontemporal_load(vp + (k + u) * kSellChunk);
+        for (int u = 0; u < U; ++u) {
+          c[u] = __builtin_nontemporal_load(cp + (k + u) * kSellChunk);
+          v[u] = __builtin_nontemporal_load(vp + (k + u) * kSellChunk);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int32_t col = __shfl(wbase, c[u] >> kSellWindowBits, 64) + (c[u] & kMask);
+          s[u] += static_cast<double>(v[u]) * static_cast<double>(a.x[col]);
+        }
       }
+      for (; k < w; ++k) {
+        const uint16_t c = cp[k * kSellChunk];
+        const int32_t col = __shfl(wbase, c >> kSellWindowBits, 64) + (c & kMask);
+        s[0] += static_cast<double>(vp[k * kSellChunk]) * static_cast<double>(a.x[col]);
+      }
+    } else {
+      const int32_t* cp = reinterpret_cast<const int32_t*>(a.cols + cbase) + lane;
+      for (; k + U <= w; k += U) {
+        int32_t c[U];
+        V v[U];
 #pragma unroll
-      for (int u = 0; u < U; ++u) s[u] += static_cast<T>(v[u]) * a.x[c[u]];
+        for (int u = 0; u < U; ++u) {
+          c[u] = __builtin_nontemporal_load(cp + (k + u) * kSellChunk);
+          v[u] = __builtin_nontemporal_load(vp + (k + u) * kSellChunk);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+          s[u] += static_cast<double>(v[u]) * static_cast<double>(a.x[c[u]]);
+      }
+      for (; k < w; ++k)
+        s[0] += static_cast<double>(vp[k * kSellChunk]) * static_cast<double>(a.x[cp[k * kSellChunk]]);
     }
-    for (; k < w; ++k) s[0] += static_cast<T>(vp[k * kSellChunk]) * a.x[cp[k * kSellChunk]];
-    const T sum = ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
+    const double sum = ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
     const size_t row = chunk * kSellChunk + lane;
     if (row < a.n) {
-      a.y[row] = sum;
-      const double ys = static_cast<double>(sum);
+      // the dots use the stored (rounded) value, which the later updates read
+      const T yv = static_cast<T>(sum);
+      a.y[row] = yv;
+      const double ys = static_cast<double>(yv);
       if (MODE == 1) {
 #pragma unroll
         for (int q = 0; q < NQ; ++q)
           if (q < a.s) acc[q] += shadow(a.p0, row, q) * ys;
       } else if (MODE == 2) {
-        acc[0] += ys * static_cast<double>(a.r[row]);
+        acc[0] += ys * a.r[row];
         acc[1 % NQ] += ys * ys;
       }
     }
   }
-  if (MODE != 0) block_reduce_store<NQ>(acc, MODE == 1 ? a.s : (MODE == 2 ? 2 : 1), a.partials);
+  if (MODE != 0) block_reduce_store<NQ, kSellBlock>(acc, MODE == 1 ? a.s : (MODE == 2 ? 2 : 1), a.partials);
 }
 
 // r = b - A x in FP64; optional FP32 copy of r; partial r.r.
@@ -331,10 +426,10 @@ __global__ void fill_kernel(size_t n, T* p, T v) {
   if (i < n) p[i] = v;
 }
 
-template <typename T>
-__global__ void copy_kernel(size_t n, const T* src, T* dst) {
+template <typename S, typename D>
+__global__ void copy_kernel(size_t n, const S* src, D* dst) {
   const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i < n) dst[i] = src[i];
+  if (i < n) dst[i] = static_cast<D>(src[i]);
 }
 
 __global__ void dot_self_kernel(size_t n, const double* v, double* partials) {
@@ -352,19 +447,22 @@ __global__ void clamp_kernel(size_t n, double* x) {
 
 // ---- IDR(s) -------------------------------------------------------------
 
+// IDR(s) state: the residual r and iterate x are FP64; the Krylov basis
+// (G, U), the work vectors v, t and the shadow vector P_0 have type T (FP32 in
+// mixed mode, so the basis updates move half the bytes).
 template <typename T>
 struct IdrVec {
   size_t n;
   size_t ld;
   int s;
-  T* r;
-  T* x;
+  double* r;
+  double* x;
   T* v;
   T* t;
   T* G;
   T* U;
   const T* p0;
-  const T* dinv;
+  const double* dinv;
   SolverScalars* sc;
   double* partials;
   const int32_t* cover_start;    // Schwarz (or null: Jacobi only): row -> block slots
@@ -384,7 +482,7 @@ __device__ __forceinline__ double precondition(const IdrVec<T>& a, size_t i, dou
       return z / static_cast<double>(hi - lo);
     }
   }
-  return static_cast<double>(a.dinv[i]) * w;
+  return a.dinv[i] * w;
 }
 
 // Schwarz path, first half of idr_a: v = r - sum_{q>=k} c_q G_q (unpreconditioned).
@@ -393,7 +491,7 @@ __global__ void __launch_bounds__(kBlock) idr_a1_kernel(IdrVec<T> a, int k) {
   const SolverScalars* sc = a.sc;
   for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < a.n;
        i += static_cast<size_t>(gridDim.x) * blockDim.x) {
-    double v = static_cast<double>(a.r[i]);
+    double v = a.r[i];
     for (int q = k; q < a.s; ++q) v -= sc->c[q] * static_cast<double>(a.G[q * a.ld + i]);
     a.v[i] = static_cast<T>(v);
   }
@@ -521,14 +619,14 @@ __global__ void __launch_bounds__(kBlock) idr_a_kernel(IdrVec<T> a, int k) {
   const double om = sc->om;
   for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < a.n;
        i += static_cast<size_t>(gridDim.x) * blockDim.x) {
-    double v = static_cast<double>(a.r[i]);
+    double v = a.r[i];
     double u = 0.0;
     for (int q = k; q < a.s; ++q) {
       const double c = sc->c[q];
       v -= c * static_cast<double>(a.G[q * a.ld + i]);
       u += c * static_cast<double>(a.U[q * a.ld + i]);
     }
-    v *= static_cast<double>(a.dinv[i]);
+    v *= a.dinv[i];
     a.U[k * a.ld + i] = static_cast<T>(u + om * v);
   }
 }
@@ -551,12 +649,12 @@ __global__ void __launch_bounds__(kBlock) idr_b_kernel(IdrVec<T> a, int k, int w
     }
     a.G[k * a.ld + i] = static_cast<T>(g);
     a.U[k * a.ld + i] = static_cast<T>(u);
-    const double r = static_cast<double>(a.r[i]) - beta * g;
-    a.r[i] = static_cast<T>(r);
-    a.x[i] = static_cast<T>(static_cast<double>(a.x[i]) + beta * u);
+    const double r = a.r[i] - beta * g;
+    a.r[i] = r;
+    a.x[i] += beta * u;
     acc[0] += r * r;
     // write_v: 1 = Jacobi-preconditioned residual, 2 = raw residual (Schwarz path)
-    if (write_v) a.v[i] = static_cast<T>(write_v == 1 ? static_cast<double>(a.dinv[i]) * r : r);
+    if (write_v) a.v[i] = static_cast<T>(write_v == 1 ? a.dinv[i] * r : r);
   }
   block_reduce_store<1>(acc, 1, a.partials);
 }
@@ -568,9 +666,9 @@ __global__ void __launch_bounds__(kBlock) idr_c_kernel(IdrVec<T> a) {
   double acc[1] = {0.0};
   for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < a.n;
        i += static_cast<size_t>(gridDim.x) * blockDim.x) {
-    a.x[i] = static_cast<T>(static_cast<double>(a.x[i]) + om * static_cast<double>(a.v[i]));
-    const double r = static_cast<double>(a.r[i]) - om * static_cast<double>(a.t[i]);
-    a.r[i] = static_cast<T>(r);
+    a.x[i] += om * static_cast<double>(a.v[i]);
+    const double r = a.r[i] - om * static_cast<double>(a.t[i]);
+    a.r[i] = r;
     acc[0] += r * r;
   }
   block_reduce_store<1>(acc, 1, a.partials);
@@ -584,7 +682,7 @@ __global__ void __launch_bounds__(kBlock) shadow_dots_kernel(IdrVec<T> a) {
   for (int q = 0; q < kMaxShadow; ++q) acc[q] = 0.0;
   for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < a.n;
        i += static_cast<size_t>(gridDim.x) * blockDim.x) {
-    const double r = static_cast<double>(a.r[i]);
+    const double r = a.r[i];
 #pragma unroll
     for (int q = 0; q < kMaxShadow; ++q)
       if (q < a.s) acc[q] += shadow(a.p0, i, q) * r;
@@ -658,7 +756,7 @@ __global__ void rz_kernel(size_t n, const double* r, const double* dinv, double*
 
 unsigned vec_blocks(size_t n) {
   const size_t b = (n + kBlock - 1) / kBlock;
-  return static_cast<unsigned>(std::max<size_t>(1, std::min<size_t>(b, kMaxReduceBlocks)));
+  return static_cast<unsigned>(std::max<size_t>(1, std::min<size_t>(b, kMaxVectorBlocks)));
 }
 unsigned spmv_blocks(size_t n) {
   const size_t rows_per_block = kBlock / kLanes;
@@ -668,7 +766,7 @@ unsigned spmv_blocks(size_t n) {
 
 // A multiple of the XCD count (see xcd_block), at most kMaxReduceBlocks.
 unsigned sell_blocks(size_t chunks) {
-  const size_t waves_per_block = kBlock / 64;
+  const size_t waves_per_block = kSellBlock / 64;
   size_t b = (chunks + waves_per_block - 1) / waves_per_block;
   b = std::min<size_t>(b, kMaxReduceBlocks);
   b = (b + kXcds - 1) / kXcds * kXcds;
@@ -685,19 +783,32 @@ void build_sell(Context& ctx, SellMatrix& M, const lsmps_csr& A, V* out_vals_dum
   M.chunk_width.reserve(M.chunks);
   M.chunk_scaled.reserve(M.chunks);
   M.chunk_start.reserve(M.chunks + 1);
-  sell_width_kernel<<<blocks_for(M.chunks * kSellChunk), kBlock, 0, ctx.stream>>>(
+  M.col_rows.reserve(M.chunks);
+  M.col_start.reserve(M.chunks + 1);
+  M.nwin.reserve(M.chunks);
+  M.windows.reserve(M.chunks * kSellWindows);
+  const unsigned row_blocks = blocks_for(M.chunks * kSellChunk);
+  sell_width_kernel<<<row_blocks, kBlock, 0, ctx.stream>>>(
       n, M.chunks, A.row_offsets, M.chunk_width.data(), M.chunk_scaled.data());
+  sell_window_kernel<<<row_blocks, kBlock, 0, ctx.stream>>>(
+      n, M.chunks, A.row_offsets, A.col_indices, M.chunk_width.data(), M.nwin.data(),
+      M.windows.data(), M.col_rows.data());
   LSMPS_LAUNCH_CHECK();
   exclusive_scan_i32(ctx, M.chunk_scaled.data(), M.chunk_start.data(), M.chunks);
+  exclusive_scan_i32(ctx, M.col_rows.data(), M.col_start.data(), M.chunks);
   LSMPS_HIP(hipMemcpyAsync(ctx.host_i32, M.chunk_start.data() + M.chunks, sizeof(int32_t),
                            hipMemcpyDeviceToHost, ctx.stream));
   LSMPS_HIP(hipMemcpyAsync(ctx.host_i32 + 1, A.row_offsets + n, sizeof(int32_t),
                            hipMemcpyDeviceToHost, ctx.stream));
+  LSMPS_HIP(hipMemcpyAsync(ctx.host_i32 + 2, M.col_start.data() + M.chunks, sizeof(int32_t),
+                           hipMemcpyDeviceToHost, ctx.stream));
   ctx.sync();
-  require(ctx.host_i32[0] >= 0, "PPE matrix too large for the 32-bit SELL layout");
+  require(ctx.host_i32[0] >= 0 && ctx.host_i32[2] >= 0,
+          "PPE matrix too large for the 32-bit SELL layout");
   M.entries = static_cast<size_t>(ctx.host_i32[0]);
   M.nnz = static_cast<size_t>(ctx.host_i32[1]);
-  M.cols.reserve(std::max<size_t>(M.entries, 1));
+  M.col_units = static_cast<size_t>(ctx.host_i32[2]) * kSellChunk;
+  M.cols.reserve(std::max<size_t>(M.col_units, 1));
   V* vals;
   if (sizeof(V) == sizeof(double)) {
     M.vals64.reserve(std::max<size_t>(M.entries, 1));
@@ -706,9 +817,10 @@ void build_sell(Context& ctx, SellMatrix& M, const lsmps_csr& A, V* out_vals_dum
     M.vals32.reserve(std::max<size_t>(M.entries, 1));
     vals = reinterpret_cast<V*>(M.vals32.data());
   }
-  sell_fill_kernel<V><<<blocks_for(M.chunks * kSellChunk), kBlock, 0, ctx.stream>>>(
+  sell_fill_kernel<V><<<row_blocks, kBlock, 0, ctx.stream>>>(
       n, M.chunks, A.row_offsets, A.col_indices, A.values, M.chunk_start.data(),
-      M.chunk_width.data(), M.cols.data(), vals);
+      M.chunk_width.data(), M.col_start.data(), M.nwin.data(), M.windows.data(), M.cols.data(),
+      vals);
   LSMPS_LAUNCH_CHECK();
 }
 
@@ -732,15 +844,15 @@ double true_residual(Context& ctx, PpeSolver& S, const lsmps_csr& A, const doubl
   return read_rr(ctx, S);
 }
 
-// IDR(s) with vectors of type T and matrix values of type V.  x and r hold the
-// initial iterate and its residual.  Each time the recursive residual has
+// IDR(s) with an FP64 iterate and residual, a Krylov basis of type T and matrix
+// values of type V.  x and r hold the initial iterate and its residual.  Each time the recursive residual has
 // dropped by `replace_factor` since the last replacement (and when it meets the
 // target) it is replaced by the true FP64 residual b - A64 x, keeping the
 // Krylov state (reliable updates); the solve stops when the true residual meets
 // the target.
 template <typename T, typename V>
 int idr_solve(Context& ctx, PpeSolver& S, size_t n, const SellMatrix& M, const V* vals,
-              const T* dinv, T* x, T* r, T* work, size_t ld, int s, double rr0,
+              const double* dinv, double* x, double* r, T* work, size_t ld, int s, double rr0,
               double target, int max_iterations, const lsmps_csr& A64, const double* b64,
               double replace_factor, int* iterations_out, int* replacements_out,
               double* rr_out) {
@@ -757,7 +869,7 @@ int idr_solve(Context& ctx, PpeSolver& S, size_t n, const SellMatrix& M, const V
   const unsigned vb = vec_blocks(n);
   const unsigned sb = sell_blocks(M.chunks);
   fill_kernel<T><<<blocks_for(2 * s * ld), kBlock, 0, ctx.stream>>>(2 * s * ld, G, T(0));
-  copy_kernel<T><<<blocks_for(n), kBlock, 0, ctx.stream>>>(n, r, p0);
+  copy_kernel<double, T><<<blocks_for(n), kBlock, 0, ctx.stream>>>(n, r, p0);
   idr_init_scalars<<<1, kBlock, 0, ctx.stream>>>(S.scalars.data(), s);
   LSMPS_LAUNCH_CHECK();
   double rr = rr0;
@@ -771,10 +883,15 @@ int idr_solve(Context& ctx, PpeSolver& S, size_t n, const SellMatrix& M, const V
     for (auto& e : ctx.timing_events) LSMPS_HIP(hipEventCreate(&e));
     ctx.timing_ready = true;
   }
-  // Algorithmic bytes of one SpMV: the stored nonzeros (value + column), the
-  // per-chunk descriptors, x read once and y written once.  Padding is not counted.
-  const double spmv_bytes = static_cast<double>(M.nnz) * (sizeof(V) + sizeof(int32_t)) +
-                            M.chunks * 2.0 * sizeof(int32_t) + 2.0 * n * sizeof(T);
+  // Algorithmic bytes of one SpMV: the stored nonzeros (value + 16-bit column,
+  // or 32-bit in the chunks that need it, in the layout's proportion), the
+  // per-chunk descriptors and window bases, x read once and y written once.
+  // Padding is not counted.
+  const double col_bytes =
+      2.0 * static_cast<double>(M.col_units) / static_cast<double>(std::max<size_t>(M.entries, 1));
+  const double spmv_bytes = static_cast<double>(M.nnz) * (sizeof(V) + col_bytes) +
+                            M.chunks * (4.0 + kSellWindows) * sizeof(int32_t) +
+                            2.0 * n * sizeof(T);
   auto mark = [&](int slot) {
     if (timing) LSMPS_HIP(hipEventRecord(ctx.timing_events[slot], ctx.stream));
   };
@@ -791,10 +908,11 @@ int idr_solve(Context& ctx, PpeSolver& S, size_t n, const SellMatrix& M, const V
       } else {
         idr_a_kernel<T><<<vb, kBlock, 0, ctx.stream>>>(iv, k);
       }
-      SellArgs<T, V> sa{n, M.chunks, M.chunk_start.data(), M.chunk_width.data(), M.cols.data(),
+      SellArgs<T, V> sa{n, M.chunks, M.chunk_start.data(), M.chunk_width.data(),
+                        M.col_start.data(), M.nwin.data(), M.windows.data(), M.cols.data(),
                         vals, U + k * ld, G + k * ld, p0, s, nullptr, S.part_a.data()};
       mark(2 * k);
-      sell_spmv_kernel<T, V, 1><<<sb, kBlock, 0, ctx.stream>>>(sa);
+      sell_spmv_kernel<T, V, 1><<<sb, kSellBlock, 0, ctx.stream>>>(sa);
       mark(2 * k + 1);
       idr_scalar_after<<<1, kReduceThreads, 0, ctx.stream>>>(S.scalars.data(), S.part_a.data(),
                                                      static_cast<int>(sb), s, k);
@@ -806,10 +924,11 @@ int idr_solve(Context& ctx, PpeSolver& S, size_t n, const SellMatrix& M, const V
       schwarz_accumulate<T>(ctx, S.schwarz, v);
       precondition_kernel<T><<<vb, kBlock, 0, ctx.stream>>>(iv);
     }
-    SellArgs<T, V> st{n, M.chunks, M.chunk_start.data(), M.chunk_width.data(), M.cols.data(),
-                      vals, v, t, nullptr, s, r, S.part_a.data()};
+    SellArgs<T, V> st{n, M.chunks, M.chunk_start.data(), M.chunk_width.data(),
+                      M.col_start.data(), M.nwin.data(), M.windows.data(), M.cols.data(), vals,
+                      v, t, nullptr, s, r, S.part_a.data()};
     mark(2 * s);
-    sell_spmv_kernel<T, V, 2><<<sb, kBlock, 0, ctx.stream>>>(st);
+    sell_spmv_kernel<T, V, 2><<<sb, kSellBlock, 0, ctx.stream>>>(st);
     mark(2 * s + 1);
     idr_scalar_omega<<<1, kReduceThreads, 0, ctx.stream>>>(S.scalars.data(), S.part_a.data(),
                                                    S.part_b.data(), static_cast<int>(sb),
@@ -830,7 +949,7 @@ int idr_solve(Context& ctx, PpeSolver& S, size_t n, const SellMatrix& M, const V
         float ms = 0.0f;
         LSMPS_HIP(hipEventElapsedTime(&ms, ctx.timing_events[2 * k], ctx.timing_events[2 * k + 1]));
         ctx.spmv_ms += ms;
-        ctx.spmv_bytes += spmv_bytes + (k == s ? n * sizeof(T) : 0.0);  // mode 2 also reads r
+        ctx.spmv_bytes += spmv_bytes + (k == s ? n * sizeof(double) : 0.0);  // mode 2 also reads r
         ++ctx.spmv_launches;
       }
     }
@@ -839,14 +958,13 @@ int idr_solve(Context& ctx, PpeSolver& S, size_t n, const SellMatrix& M, const V
     std::memcpy(&breakdown, S.host + 2, sizeof(int32_t));
     if (!std::isfinite(rr)) break;
     if (rr <= target2 || rr <= factor2 * rr_replaced) {
-      rr = true_residual(ctx, S, A64, reinterpret_cast<const double*>(x), b64,
-                         reinterpret_cast<double*>(r), nullptr);
+      rr = true_residual(ctx, S, A64, x, b64, r, nullptr);
       rr_replaced = rr;
       ++replacements;
     }
     if (breakdown) {
       fill_kernel<T><<<blocks_for(2 * s * ld), kBlock, 0, ctx.stream>>>(2 * s * ld, G, T(0));
-      copy_kernel<T><<<blocks_for(n), kBlock, 0, ctx.stream>>>(n, r, p0);
+      copy_kernel<double, T><<<blocks_for(n), kBlock, 0, ctx.stream>>>(n, r, p0);
       idr_init_scalars<<<1, kBlock, 0, ctx.stream>>>(S.scalars.data(), s);
       LSMPS_LAUNCH_CHECK();
     }
@@ -950,10 +1068,10 @@ void ppe_solve(Context& ctx, PpeSolver& S, const lsmps_csr& A, const double* b, 
     rep.relative_residual /= bnorm;
     rep.converged = rep.relative_residual <= opt.tolerance ? 1 : 0;
   } else {
+    // r, then the basis: v, t, P_0, G[s], U[s]
     S.vec64.reserve((4 + 2 * static_cast<size_t>(shadow_dim)) * ld);
     double* r = S.vec64.data();
-    double* work = r + ld;
-    copy_kernel<double><<<blocks_for(n), kBlock, 0, ctx.stream>>>(n, b, r);
+    copy_kernel<double, double><<<blocks_for(n), kBlock, 0, ctx.stream>>>(n, b, r);
     LSMPS_LAUNCH_CHECK();
     int it = 0, replacements = 0;
     double rr = 0.0;
@@ -968,15 +1086,18 @@ void ppe_solve(Context& ctx, PpeSolver& S, const lsmps_csr& A, const double* b, 
     if (ctx.precision == LSMPS_PRECISION_FP64) {
       build_sell<double>(ctx, S.sell, A, nullptr);
       idr_solve<double, double>(ctx, S, n, S.sell, S.sell.vals64.data(), S.dinv64.data(), x, r,
-                                work, ld, shadow_dim, bnorm * bnorm, target, max_it, A, b, 0.0,
+                                r + ld, ld, shadow_dim, bnorm * bnorm, target, max_it, A, b, 0.0,
                                 &it, &replacements, &rr);
     } else {
-      // mixed: SpMV on an FP32 copy of the values; FP64 vectors, dots and reliable updates
+      // mixed: FP32 matrix values in the Krylov SpMVs; FP64 basis, iterate,
+      // residual, dots and reliable updates against the FP64 matrix.  (An FP32
+      // basis stagnates: r -= beta G_k with a rounded G_k leaves an error of
+      // 1e-7 |beta G_k|, far above the target when the recurrences grow.)
       build_sell<float>(ctx, S.sell, A, nullptr);
       const double factor = opt.inner_tolerance > 0.0 ? opt.inner_tolerance : 1.0e-3;
       idr_solve<double, float>(ctx, S, n, S.sell, S.sell.vals32.data(), S.dinv64.data(), x, r,
-                               work, ld, shadow_dim, bnorm * bnorm, target, max_it, A, b, factor,
-                               &it, &replacements, &rr);
+                               r + ld, ld, shadow_dim, bnorm * bnorm, target, max_it, A, b,
+                               factor, &it, &replacements, &rr);
     }
     rep.iterations = it;
     rep.refinements = replacements;

@@ -8,7 +8,8 @@ namespace lsmps {
 
 constexpr int kMaxShadow = 16;           // largest IDR(s) shadow dimension
 constexpr int kPartialStride = kMaxShadow;  // doubles per block partial record
-constexpr int kMaxReduceBlocks = 4096;
+constexpr int kMaxReduceBlocks = 2048;  // partial records per reduction (SpMV grid cap)
+constexpr int kMaxVectorBlocks = 1024;  // grid cap of the streaming vector kernels
 
 // Small device-resident state of one Krylov solve; kernels read and write it
 // directly so no host round trip happens per iteration.
@@ -29,21 +30,38 @@ struct SolverScalars {
 
 // Sliced ELLPACK with 64-row chunks (one wavefront per chunk): entry k of the
 // chunk's row r sits at chunk_start[c] + k * 64 + r, padded to the chunk's
-// widest row with value 0 and the row's own index as column.
+// widest row with value 0.
+//
+// Column indices are stored in 16 bits where the chunk allows it: the chunk's
+// columns are covered by at most kSellWindows windows [base_j, base_j + 4096)
+// (found greedily, which is optimal for covering points by fixed-length
+// intervals), and an entry holds (j << 12) | (column - base_j).  Particles in
+// lattice or cell order need about 7-9 windows (one per z layer of the 7x7x7
+// neighbourhood).  A chunk needing more windows keeps 32-bit columns
+// (nwin = -1) in the same array, two 16-bit slots per entry.  Padding entries
+// point at base_0 with value 0.
 constexpr int kSellChunk = 64;
+constexpr int kSellWindows = 16;
+constexpr int kSellWindowBits = 12;
 struct SellMatrix {
   size_t n = 0;
   size_t chunks = 0;
-  size_t entries = 0;  // padded
-  size_t nnz = 0;      // stored CSR nonzeros
-  DeviceArray<int32_t> chunk_start;  // padded entry count < 2^31 is checked at build
+  size_t entries = 0;    // padded value slots
+  size_t col_units = 0;  // 16-bit column slots (entries + entries of 32-bit chunks)
+  size_t nnz = 0;        // stored CSR nonzeros
+  DeviceArray<int32_t> chunk_start;  // value slot offset; padded entry count < 2^31 is checked
   DeviceArray<int32_t> chunk_width;
   DeviceArray<int32_t> chunk_scaled;
-  DeviceArray<int32_t> cols;
+  DeviceArray<int32_t> col_rows;     // per chunk: width * (1 or 2), units of 64 slots
+  DeviceArray<int32_t> col_start;    // scan of col_rows
+  DeviceArray<int32_t> nwin;         // windows used, or -1 for 32-bit columns
+  DeviceArray<int32_t> windows;      // chunks x kSellWindows bases
+  DeviceArray<uint16_t> cols;
   DeviceArray<double> vals64;
   DeviceArray<float> vals32;
   size_t bytes() const {
-    return chunk_start.bytes() + chunk_width.bytes() + cols.bytes() + vals64.bytes() +
+    return chunk_start.bytes() + chunk_width.bytes() + chunk_scaled.bytes() + col_rows.bytes() +
+           col_start.bytes() + nwin.bytes() + windows.bytes() + cols.bytes() + vals64.bytes() +
            vals32.bytes();
   }
 };
@@ -52,12 +70,8 @@ class PpeSolver {
  public:
   SellMatrix sell;
   SchwarzPrec schwarz;
-  DeviceArray<double> dinv64;
-  DeviceArray<float> dinv32;
-  DeviceArray<float> vals32;
-  DeviceArray<double> vec64;   // FP64 Krylov vectors
-  DeviceArray<float> vec32;    // FP32 Krylov vectors (mixed inner solve)
-  DeviceArray<double> outer64; // mixed: FP64 residual
+  DeviceArray<double> dinv64;  // Jacobi: 1 / diagonal
+  DeviceArray<double> vec64;   // residual + Krylov basis (FP64)
   DeviceArray<double> part_a;
   DeviceArray<double> part_b;
   DeviceArray<SolverScalars> scalars;
@@ -68,8 +82,8 @@ class PpeSolver {
   PpeSolver(const PpeSolver&) = delete;
   PpeSolver& operator=(const PpeSolver&) = delete;
   size_t bytes() const {
-    return sell.bytes() + schwarz.bytes() + dinv64.bytes() + dinv32.bytes() + vals32.bytes() + vec64.bytes() +
-           vec32.bytes() + outer64.bytes() + part_a.bytes() + part_b.bytes() + scalars.bytes();
+    return sell.bytes() + schwarz.bytes() + dinv64.bytes() + vec64.bytes() + part_a.bytes() +
+           part_b.bytes() + scalars.bytes();
   }
 };
 

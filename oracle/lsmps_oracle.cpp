@@ -16,9 +16,10 @@
 //    counting sort: within a cell, ascending particle index.
 //  * cuSOLVER potrf/potrs (moment_matrix.cu:430-471) are restated as a textbook
 //    Cholesky-Banachiewicz factorisation and per-column triangular solves.
-//  * AMGX GMRES (ppe_matrix.cu:586-679) is replaced by Jacobi-PCG or
-//    Jacobi-BiCGStab to a relative true-residual tolerance, from p = 0, followed
-//    by the same clamp p = max(p, 0) (ppe_matrix.cu:662).
+//  * AMGX GMRES (ppe_matrix.cu:586-679) is replaced by IDR(s) (default; with
+//    Jacobi or restricted-additive-Schwarz preconditioning), BiCGStab or PCG to
+//    a relative true-residual tolerance, from p = 0, followed by the same clamp
+//    p = max(p, 0) (ppe_matrix.cu:662).
 //
 // Parity status per function is recorded in the oracle section of DESIGN.md and
 // in tests/test_oracle_*.py, which pin each function against the reference's own
@@ -841,6 +842,146 @@ double true_residual(const lsmps_csr& a, const double* b, const double* x, doubl
   return std::sqrt(s);
 }
 
+// Restricted additive Schwarz preconditioner (no reference counterpart: the
+// reference hands the PPE to AMGX, ppe_matrix.cu:586-679).  Rows with
+// off-diagonal couplings are "eligible"; eligible rows with a positive diagonal
+// (wrong-sign LSMPS Laplacian rows at wall corners of the free surface) seed one
+// dense block each: the eligible rows among the seed's columns and their
+// columns (2 hops), or the 1-hop set alone when the 2-hop set has more than
+// kSchwarzMaxBlock rows.  Each block is inverted by LU with partial pivoting;
+// a covered row takes the mean of its blocks' values (summed in ascending block
+// order), every other row Jacobi.
+constexpr int kSchwarzMaxBlock = 256;
+
+struct SchwarzBlocks {
+  std::vector<std::vector<int32_t>> members;  // ascending row indices
+  std::vector<std::vector<double>> inverse;   // row-major n x n
+  std::vector<int32_t> cover;                 // blocks covering each row
+};
+
+// LU with partial pivoting (largest |a|, first row on ties) and n column
+// solves into an explicit inverse.  A zero pivot is replaced by 1e-300.
+void lu_inverse(std::vector<double>& a, int n, std::vector<double>& inv) {
+  std::vector<int> piv(n);
+  for (int k = 0; k < n; ++k) {
+    int p = k;
+    double best = std::fabs(a[static_cast<size_t>(k) * n + k]);
+    for (int r = k + 1; r < n; ++r) {
+      const double v = std::fabs(a[static_cast<size_t>(r) * n + k]);
+      if (v > best) {
+        best = v;
+        p = r;
+      }
+    }
+    piv[k] = p;
+    if (p != k)
+      for (int j = 0; j < n; ++j)
+        std::swap(a[static_cast<size_t>(k) * n + j], a[static_cast<size_t>(p) * n + j]);
+    double akk = a[static_cast<size_t>(k) * n + k];
+    if (akk == 0.0) akk = 1.0e-300;
+    for (int r = k + 1; r < n; ++r) a[static_cast<size_t>(r) * n + k] /= akk;
+    for (int r = k + 1; r < n; ++r) {
+      const double l = a[static_cast<size_t>(r) * n + k];
+      for (int j = k + 1; j < n; ++j)
+        a[static_cast<size_t>(r) * n + j] -= l * a[static_cast<size_t>(k) * n + j];
+    }
+  }
+  inv.assign(static_cast<size_t>(n) * n, 0.0);
+  std::vector<double> col(n);
+  for (int t = 0; t < n; ++t) {
+    for (int i = 0; i < n; ++i) col[i] = i == t ? 1.0 : 0.0;
+    for (int k = 0; k < n; ++k)
+      if (piv[k] != k) std::swap(col[k], col[piv[k]]);
+    for (int i = 0; i < n; ++i) {
+      double s = col[i];
+      for (int j = 0; j < i; ++j) s -= a[static_cast<size_t>(i) * n + j] * col[j];
+      col[i] = s;
+    }
+    for (int i = n - 1; i >= 0; --i) {
+      double s = col[i];
+      for (int j = i + 1; j < n; ++j) s -= a[static_cast<size_t>(i) * n + j] * col[j];
+      double d = a[static_cast<size_t>(i) * n + i];
+      if (d == 0.0) d = 1.0e-300;
+      col[i] = s / d;
+    }
+    for (int i = 0; i < n; ++i) inv[static_cast<size_t>(i) * n + t] = col[i];
+  }
+}
+
+void schwarz_build(const lsmps_csr& a, SchwarzBlocks& S) {
+  const int64_t n = static_cast<int64_t>(a.rows);
+  std::vector<char> eligible(n, 0);
+  std::vector<int32_t> seeds;
+  for (int64_t i = 0; i < n; ++i) {
+    double diag = 0.0, off = 0.0;
+    for (int32_t c = a.row_offsets[i]; c < a.row_offsets[i + 1]; ++c) {
+      if (a.col_indices[c] == i) diag += a.values[c];
+      else off += std::fabs(a.values[c]);
+    }
+    eligible[i] = off > 0.0;
+    if (off > 0.0 && diag > 0.0) seeds.push_back(static_cast<int32_t>(i));
+  }
+  const size_t nb = seeds.size();
+  S.members.assign(nb, {});
+  S.inverse.assign(nb, {});
+#pragma omp parallel for schedule(dynamic)
+  for (size_t b = 0; b < nb; ++b) {
+    const int32_t seed = seeds[b];
+    std::vector<int32_t> one, two;
+    for (int32_t c = a.row_offsets[seed]; c < a.row_offsets[seed + 1]; ++c)
+      if (eligible[a.col_indices[c]]) one.push_back(a.col_indices[c]);
+    std::sort(one.begin(), one.end());
+    one.erase(std::unique(one.begin(), one.end()), one.end());
+    two = one;
+    for (int32_t j : one)
+      for (int32_t c = a.row_offsets[j]; c < a.row_offsets[j + 1]; ++c)
+        if (eligible[a.col_indices[c]]) two.push_back(a.col_indices[c]);
+    std::sort(two.begin(), two.end());
+    two.erase(std::unique(two.begin(), two.end()), two.end());
+    std::vector<int32_t>& m = S.members[b];
+    m = two.size() <= static_cast<size_t>(kSchwarzMaxBlock) ? two : one;
+    if (m.size() > static_cast<size_t>(kSchwarzMaxBlock)) m.resize(kSchwarzMaxBlock);
+    const int bn = static_cast<int>(m.size());
+    std::vector<double> dense(static_cast<size_t>(bn) * bn, 0.0);
+    for (int r = 0; r < bn; ++r) {
+      const int32_t row = m[r];
+      for (int32_t c = a.row_offsets[row]; c < a.row_offsets[row + 1]; ++c) {
+        const auto it = std::lower_bound(m.begin(), m.end(), a.col_indices[c]);
+        if (it != m.end() && *it == a.col_indices[c])
+          dense[static_cast<size_t>(r) * bn + (it - m.begin())] += a.values[c];
+      }
+    }
+    lu_inverse(dense, bn, S.inverse[b]);
+  }
+  S.cover.assign(n, 0);
+  for (const auto& m : S.members)
+    for (int32_t r : m) ++S.cover[r];
+}
+
+// z = M^-1 v (z and v distinct).
+void schwarz_apply(const SchwarzBlocks& S, const std::vector<double>& dinv, const double* v,
+                   double* z, int64_t n) {
+  for (int64_t i = 0; i < n; ++i) z[i] = S.cover[i] ? 0.0 : dinv[i] * v[i];
+  std::vector<std::vector<double>> y(S.members.size());
+#pragma omp parallel for schedule(dynamic)
+  for (size_t b = 0; b < S.members.size(); ++b) {
+    const auto& m = S.members[b];
+    const int bn = static_cast<int>(m.size());
+    y[b].assign(bn, 0.0);
+    for (int r = 0; r < bn; ++r) {
+      double s = 0.0;
+      for (int j = 0; j < bn; ++j) s += S.inverse[b][static_cast<size_t>(r) * bn + j] * v[m[j]];
+      y[b][r] = s;
+    }
+  }
+  for (size_t b = 0; b < S.members.size(); ++b)  // ascending block order
+    for (size_t r = 0; r < S.members[b].size(); ++r) z[S.members[b][r]] += y[b][r];
+  for (int64_t i = 0; i < n; ++i)
+    if (S.cover[i]) z[i] /= static_cast<double>(S.cover[i]);
+}
+
+double g_solver_setup_seconds = 0.0;
+
 int ppe_solve(const lsmps_csr& a, const double* b, double* x, const lsmps_solver_options& opt,
               lsmps_solve_report* rep) {
   const int64_t n = static_cast<int64_t>(a.rows);
@@ -887,6 +1028,24 @@ int ppe_solve(const lsmps_csr& a, const double* b, double* x, const lsmps_solver
     };
     std::vector<double> p0(r), G(static_cast<size_t>(s) * n, 0.0), U(static_cast<size_t>(s) * n, 0.0);
     std::vector<double> M(static_cast<size_t>(s) * s, 0.0), f(s), c(s), v(n), tv(n), d(s);
+    // M^-1: Jacobi, or Schwarz blocks on the wrong-sign rows plus Jacobi elsewhere
+    SchwarzBlocks blocks;
+    const bool use_schwarz = opt.preconditioner == LSMPS_PRECOND_SCHWARZ;
+    {
+      const double t0 = omp_get_wtime();
+      if (use_schwarz) schwarz_build(a, blocks);
+      g_solver_setup_seconds = omp_get_wtime() - t0;
+      report.schwarz_blocks = static_cast<int32_t>(blocks.members.size());
+      for (const auto& m : blocks.members) report.schwarz_rows += static_cast<int32_t>(m.size());
+    }
+    std::vector<double> w(n);
+    auto precondition = [&](const double* in, double* out) {
+      if (use_schwarz) {
+        schwarz_apply(blocks, dinv, in, out, n);
+      } else {
+        for (int64_t i = 0; i < n; ++i) out[i] = in[i] * dinv[i];
+      }
+    };
     auto pdot = [&](int q, const double* w) {
       double acc = 0.0;
       if (q == 0) {
@@ -920,12 +1079,16 @@ int ppe_solve(const lsmps_csr& a, const double* b, double* x, const lsmps_solver
         double* Gk = G.data() + static_cast<size_t>(k) * n;
 #pragma omp parallel for schedule(static)
         for (int64_t i = 0; i < n; ++i) {
-          double vv = r[i], u = 0.0;
-          for (int q = k; q < s; ++q) {
-            vv -= c[q] * G[static_cast<size_t>(q) * n + i];
-            u += c[q] * U[static_cast<size_t>(q) * n + i];
-          }
-          Uk[i] = u + om * (vv * dinv[i]);
+          double vv = r[i];
+          for (int q = k; q < s; ++q) vv -= c[q] * G[static_cast<size_t>(q) * n + i];
+          v[i] = vv;
+        }
+        precondition(v.data(), w.data());
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < n; ++i) {
+          double u = 0.0;
+          for (int q = k; q < s; ++q) u += c[q] * U[static_cast<size_t>(q) * n + i];
+          Uk[i] = u + om * w[i];
         }
         spmv(a, Uk, Gk);
         for (int q = 0; q < s; ++q) d[q] = pdot(q, Gk);
@@ -964,7 +1127,7 @@ int ppe_solve(const lsmps_csr& a, const double* b, double* x, const lsmps_solver
         ++it;
       }
       if (!breakdown) {
-        for (int64_t i = 0; i < n; ++i) v[i] = r[i] * dinv[i];
+        precondition(r.data(), v.data());
         spmv(a, v.data(), tv.data());
         const double tr = dot(tv.data(), r.data(), n);
         const double tt = dot(tv.data(), tv.data(), n);
@@ -1380,6 +1543,31 @@ int orc_ppe_assemble(const lsmps_config* cfg, const lsmps_fluid_soa* fluid,
 int orc_ppe_solve(const lsmps_csr* a, const double* b, double* x,
                   const lsmps_solver_options* opt, lsmps_solve_report* rep) {
   return ppe_solve(*a, b, x, *opt, rep);
+}
+
+// Seconds spent building the preconditioner in the last IDR(s) solve.
+double orc_solver_setup_seconds(void) { return g_solver_setup_seconds; }
+
+// z = M^-1 v with the Schwarz preconditioner of `a` (built on each call).
+// Returns the number of blocks.
+int orc_schwarz_apply(const lsmps_csr* a, const double* v, double* z, int* blocks_out,
+                      int* rows_out) {
+  const int64_t n = static_cast<int64_t>(a->rows);
+  std::vector<double> dinv(n);
+  for (int64_t i = 0; i < n; ++i) {
+    double d = 0.0;
+    for (int32_t c = a->row_offsets[i]; c < a->row_offsets[i + 1]; ++c)
+      if (a->col_indices[c] == i) d += a->values[c];
+    dinv[i] = d != 0.0 ? 1.0 / d : 1.0;
+  }
+  SchwarzBlocks S;
+  schwarz_build(*a, S);
+  schwarz_apply(S, dinv, v, z, n);
+  int rows = 0;
+  for (const auto& m : S.members) rows += static_cast<int>(m.size());
+  if (blocks_out) *blocks_out = static_cast<int>(S.members.size());
+  if (rows_out) *rows_out = rows;
+  return LSMPS_OK;
 }
 
 // DevicePressureCorrection::apply (pressure_correction.cu:598-645)

@@ -35,6 +35,7 @@ class Oracle:
         L.orc_reference_number_density.restype = C.c_double
         L.orc_reference_number_density.argtypes = [C.c_double, C.c_double, C.c_int]
         L.orc_stage_seconds.restype = C.POINTER(C.c_double)
+        L.orc_solver_setup_seconds.restype = C.c_double
         L.orc_set_threads.restype = C.c_int
         L.orc_set_threads.argtypes = [C.c_int]
         if threads:
@@ -259,7 +260,25 @@ class Oracle:
                                     C.byref(rep))
         if rc not in (0, a.LSMPS_ERR_NOT_CONVERGED):
             self._check(rc, "ppe_solve")
-        return x, rep.as_dict()
+        out = rep.as_dict()
+        out["setup_seconds"] = self.lib.orc_solver_setup_seconds()
+        return x, out
+
+    def schwarz_apply(self, row_offsets, col_indices, values, v):
+        """z = M^-1 v for the Schwarz preconditioner of the CSR matrix; returns
+        (z, blocks, rows summed over blocks)."""
+        a = self.abi
+        n = v.shape[0]
+        rows = np.ascontiguousarray(row_offsets, np.int32)
+        cols = np.ascontiguousarray(col_indices, np.int32)
+        vals = np.ascontiguousarray(values, np.float64)
+        vv = np.ascontiguousarray(v, np.float64)
+        z = np.zeros(n)
+        csr = a.Csr(n, n, vals.shape[0], a.iptr(_p(rows)), a.iptr(_p(cols)), a.dptr(_p(vals)))
+        nb, nr = C.c_int(), C.c_int()
+        self._check(self.lib.orc_schwarz_apply(C.byref(csr), a.dptr(_p(vv)), a.dptr(_p(z)),
+                                               C.byref(nb), C.byref(nr)), "schwarz_apply")
+        return z, nb.value, nr.value
 
     def correction(self, case, nb, tmp, pressure, pre_inverse, config=None, surface_type=None):
         """DevicePressureCorrection::apply on a copy of the case's fluid; returns the new state."""
