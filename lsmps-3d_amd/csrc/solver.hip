@@ -189,9 +189,13 @@ __global__ void sell_width_kernel(size_t n, size_t chunks, const int32_t* rows, 
   }
 }
 
-// Windows of one chunk (one wavefront per chunk, lane = row): repeatedly the
-// smallest column not yet covered opens the next window [base, base + 4096).
-// More than kSellWindows windows -> 32-bit columns for this chunk.
+// Windows of one chunk (one wavefront per chunk, lane = row), in one pass over
+// the rows: each lane marks the 2048-wide column buckets its row touches in a
+// 128-bit mask relative to the chunk's first row, the masks are OR-reduced
+// over the wavefront, and windows of two buckets (4096 columns) are laid
+// greedily from the lowest uncovered bucket.  Columns outside the 128-bucket
+// span, or more than kSellWindows windows, give the chunk 32-bit columns.
+constexpr int kBucketBits = kSellWindowBits - 1;
 __global__ void sell_window_kernel(size_t n, size_t chunks, const int32_t* rows,
                                    const int32_t* cols, const int32_t* width, int32_t* nwin,
                                    int32_t* windows, int32_t* col_rows) {
@@ -201,29 +205,46 @@ __global__ void sell_window_kernel(size_t n, size_t chunks, const int32_t* rows,
   const size_t r = chunk * kSellChunk + lane;
   const int32_t b = r < n ? rows[r] : 0;
   const int32_t e = r < n ? rows[r + 1] : 0;
-  int32_t* wb = windows + chunk * kSellWindows;
-  int64_t lo = 0;  // smallest column not yet covered
-  int count = 0;
-  bool wide = false;
-  while (true) {
-    int32_t m = INT32_MAX;
-    for (int32_t c = b; c < e; ++c) {
-      const int32_t v = cols[c];
-      if (v >= lo && v < m) m = v;
+  const int64_t first_bucket = static_cast<int64_t>(chunk * kSellChunk >> kBucketBits) - 64;
+  uint64_t lo = 0, hi = 0;
+  int outside = 0;
+  for (int32_t c = b; c < e; ++c) {
+    const int64_t rel = (static_cast<int64_t>(cols[c]) >> kBucketBits) - first_bucket;
+    if (rel < 0 || rel >= 128) {
+      outside = 1;
+    } else if (rel < 64) {
+      lo |= 1ull << rel;
+    } else {
+      hi |= 1ull << (rel - 64);
     }
+  }
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) m = min(m, __shfl_xor(m, off, 64));
-    if (m == INT32_MAX) break;  // every column is covered
+  for (int off = 32; off > 0; off >>= 1) {
+    lo |= __shfl_xor(lo, off, 64);
+    hi |= __shfl_xor(hi, off, 64);
+    outside |= __shfl_xor(outside, off, 64);
+  }
+  // every lane now holds the chunk's masks; lay the windows
+  int32_t* wb = windows + chunk * kSellWindows;
+  int count = 0;
+  bool wide = outside != 0;
+  for (int bit = 0; bit < 128 && !wide; ++bit) {
+    const bool set = bit < 64 ? ((lo >> bit) & 1ull) : ((hi >> (bit - 64)) & 1ull);
+    if (!set) continue;
     if (count == kSellWindows) {
       wide = true;
       break;
     }
-    if (lane == 0) wb[count] = m;
+    if (lane == 0) wb[count] = static_cast<int32_t>((first_bucket + bit) << kBucketBits);
     ++count;
-    lo = static_cast<int64_t>(m) + (1 << kSellWindowBits);
+    ++bit;  // the window also covers the next bucket
   }
-  if (count == 0) {  // a chunk without entries: one window at 0 for the padding
+  if (!wide && count == 0) {  // a chunk without entries: one window at 0 for the padding
     if (lane == 0) wb[0] = 0;
+    count = 1;
+  }
+  if (wide) {  // 32-bit columns; padding points at the chunk's first row
+    if (lane == 0) wb[0] = static_cast<int32_t>(chunk * kSellChunk);
     count = 1;
   }
   if (lane == 0) {
@@ -774,10 +795,121 @@ unsigned sell_blocks(size_t chunks) {
 }
 
 // Converts the CSR (FP64 values) into the SELL-64 layout with values of type V.
+// Bitonic sort of one row's (column, position) keys held by a wavefront, E keys
+// per lane (key i = lane * E + e); P = 64 E keys, padded with all-ones keys.
+template <int E>
+__device__ __forceinline__ void wave_bitonic_sort(uint64_t (&key)[E], int lane) {
+  constexpr int P = 64 * E;
+#pragma unroll
+  for (int k = 2; k <= P; k <<= 1) {
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      if (j < E) {  // partner in the same lane
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          const int f = e ^ j;
+          if (f > e) {
+            const bool up = ((lane * E + e) & k) == 0;
+            const uint64_t a = key[e], b = key[f];
+            if ((a > b) == up) {
+              key[e] = b;
+              key[f] = a;
+            }
+          }
+        }
+      } else {  // partner lane = lane ^ (j / E), same slot
+        const int m = j / E;
+        const bool lower = (lane & m) == 0;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          const uint64_t other = __shfl_xor(key[e], m, 64);
+          const bool up = ((lane * E + e) & k) == 0;
+          const bool keep_min = lower == up;
+          key[e] = keep_min ? (other < key[e] ? other : key[e]) : (other > key[e] ? other : key[e]);
+        }
+      }
+    }
+  }
+}
+
+template <int E>
+__device__ __forceinline__ void sort_row(int32_t b, int32_t len, const int32_t* cols,
+                                         const double* vals, int32_t* out_cols, double* out_vals,
+                                         int lane) {
+  uint64_t key[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int i = lane * E + e;
+    key[e] = i < len ? (static_cast<uint64_t>(static_cast<uint32_t>(cols[b + i])) << 32) |
+                           static_cast<uint32_t>(i)
+                     : ~0ull;
+  }
+  wave_bitonic_sort<E>(key, lane);
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int i = lane * E + e;
+    if (i < len) {
+      out_cols[b + i] = static_cast<int32_t>(key[e] >> 32);
+      out_vals[b + i] = vals[b + static_cast<int32_t>(key[e] & 0xFFFFFFFFu)];
+    }
+  }
+}
+
+// Copy of the CSR with every row's entries in ascending column order (one
+// wavefront per row).  In the SELL layout, lane r of a chunk then reads the
+// k-th smallest column of its row, and neighbouring particles' k-th columns
+// are neighbours too, so the gathers of x coalesce; in CSR (neighbor-list)
+// order they scatter over ~50 cache lines per load.  Rows longer than 512
+// entries are copied unsorted (still correct).
+__global__ void __launch_bounds__(kBlock) sort_rows_kernel(size_t n, const int32_t* rows,
+                                                           const int32_t* cols,
+                                                           const double* vals,
+                                                           int32_t* out_cols, double* out_vals) {
+  const int lane = threadIdx.x & 63;
+  const size_t wave = (static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+  const size_t waves = (static_cast<size_t>(gridDim.x) * blockDim.x) >> 6;
+  for (size_t row = wave; row < n; row += waves) {
+    const int32_t b = rows[row];
+    const int32_t len = rows[row + 1] - b;
+    if (len <= 64) {
+      sort_row<1>(b, len, cols, vals, out_cols, out_vals, lane);
+    } else if (len <= 128) {
+      sort_row<2>(b, len, cols, vals, out_cols, out_vals, lane);
+    } else if (len <= 256) {
+      sort_row<4>(b, len, cols, vals, out_cols, out_vals, lane);
+    } else if (len <= 512) {
+      sort_row<8>(b, len, cols, vals, out_cols, out_vals, lane);
+    } else {
+      for (int32_t i = lane; i < len; i += 64) {
+        out_cols[b + i] = cols[b + i];
+        out_vals[b + i] = vals[b + i];
+      }
+    }
+  }
+}
+
 template <typename V>
-void build_sell(Context& ctx, SellMatrix& M, const lsmps_csr& A, V* out_vals_dummy_type) {
+void build_sell(Context& ctx, SellMatrix& M, const lsmps_csr& A_in, V* out_vals_dummy_type) {
   (void)out_vals_dummy_type;
-  const size_t n = A.rows;
+  const size_t n = A_in.rows;
+  // row-sorted copy of the matrix (same rows and values, columns ascending)
+  LSMPS_HIP(hipMemcpyAsync(ctx.host_i32, A_in.row_offsets + n, sizeof(int32_t),
+                           hipMemcpyDeviceToHost, ctx.stream));
+  ctx.sync();
+  const size_t used = static_cast<size_t>(std::max(ctx.host_i32[0], 0));
+  M.sorted_cols.reserve(std::max<size_t>(used, 1));
+  M.sorted_vals.reserve(std::max<size_t>(used, 1));
+  {
+    const size_t waves = std::min<size_t>(n, static_cast<size_t>(kMaxVectorBlocks) * 16);
+    const unsigned blocks = static_cast<unsigned>(std::max<size_t>(1, (waves * 64 + kBlock - 1) / kBlock));
+    sort_rows_kernel<<<blocks, kBlock, 0, ctx.stream>>>(n, A_in.row_offsets, A_in.col_indices,
+                                                        A_in.values, M.sorted_cols.data(),
+                                                        M.sorted_vals.data());
+    LSMPS_LAUNCH_CHECK();
+  }
+  lsmps_csr A = A_in;
+  A.col_indices = M.sorted_cols.data();
+  A.values = M.sorted_vals.data();
   M.n = n;
   M.chunks = (n + kSellChunk - 1) / kSellChunk;
   M.chunk_width.reserve(M.chunks);

@@ -33,11 +33,12 @@ struct SolverScalars {
 // widest row with value 0.
 //
 // Column indices are stored in 16 bits where the chunk allows it: the chunk's
-// columns are covered by at most kSellWindows windows [base_j, base_j + 4096)
-// (found greedily, which is optimal for covering points by fixed-length
-// intervals), and an entry holds (j << 12) | (column - base_j).  Particles in
-// lattice or cell order need about 7-9 windows (one per z layer of the 7x7x7
-// neighbourhood).  A chunk needing more windows keeps 32-bit columns
+// columns are covered by at most kSellWindows windows [base_j, base_j + 4096),
+// each two aligned 2048-column buckets wide and laid greedily over the buckets
+// the chunk touches, and an entry holds (j << 12) | (column - base_j).
+// Particles in lattice or cell order need about 7-14 windows (one or two per z
+// layer of the 7x7x7 neighbourhood).  A chunk needing more windows, or with
+// columns more than 64 buckets from its first row, keeps 32-bit columns
 // (nwin = -1) in the same array, two 16-bit slots per entry.  Padding entries
 // point at base_0 with value 0.
 constexpr int kSellChunk = 64;
@@ -59,10 +60,12 @@ struct SellMatrix {
   DeviceArray<uint16_t> cols;
   DeviceArray<double> vals64;
   DeviceArray<float> vals32;
+  DeviceArray<int32_t> sorted_cols;  // the CSR with each row's columns ascending
+  DeviceArray<double> sorted_vals;
   size_t bytes() const {
     return chunk_start.bytes() + chunk_width.bytes() + chunk_scaled.bytes() + col_rows.bytes() +
            col_start.bytes() + nwin.bytes() + windows.bytes() + cols.bytes() + vals64.bytes() +
-           vals32.bytes();
+           vals32.bytes() + sorted_cols.bytes() + sorted_vals.bytes();
   }
 };
 
