@@ -174,18 +174,208 @@ __global__ void __launch_bounds__(kBlock) spmv_kernel(SpmvArgs<T, V> a) {
 
 // ---- sliced ELLPACK (SELL-64) ---------------------------------------------
 
-// Width of each 64-row chunk (one wavefront per chunk); also width * 64 for the scan.
-__global__ void sell_width_kernel(size_t n, size_t chunks, const int32_t* rows, int32_t* width,
-                                  int32_t* scaled) {
-  const size_t t = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  const size_t chunk = t / kSellChunk;
-  const size_t r = t;
-  int32_t len = r < n ? rows[r + 1] - rows[r] : 0;
+// Bitonic sort of one row's (column, position) keys held by a wavefront, E keys
+// per lane (key i = lane * E + e); P = 64 E keys, padded with all-ones keys.
+template <int E>
+__device__ __forceinline__ void wave_bitonic_sort(uint64_t (&key)[E], int lane) {
+  constexpr int P = 64 * E;
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) len = max(len, __shfl_xor(len, off, 64));
-  if ((threadIdx.x & 63) == 0 && chunk < chunks) {
-    width[chunk] = len;
-    scaled[chunk] = len * kSellChunk;
+  for (int k = 2; k <= P; k <<= 1) {
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      if (j < E) {  // partner in the same lane
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          const int f = e ^ j;
+          if (f > e) {
+            const bool up = ((lane * E + e) & k) == 0;
+            const uint64_t a = key[e], b = key[f];
+            if ((a > b) == up) {
+              key[e] = b;
+              key[f] = a;
+            }
+          }
+        }
+      } else {  // partner lane = lane ^ (j / E), same slot
+        const int m = j / E;
+        const bool lower = (lane & m) == 0;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          const uint64_t other = __shfl_xor(key[e], m, 64);
+          const bool up = ((lane * E + e) & k) == 0;
+          const bool keep_min = lower == up;
+          key[e] = keep_min ? (other < key[e] ? other : key[e]) : (other > key[e] ? other : key[e]);
+        }
+      }
+    }
+  }
+}
+
+template <int E>
+__device__ __forceinline__ void sort_row(int32_t b, int32_t len, const int32_t* cols,
+                                         const double* vals, int32_t* out_cols, double* out_vals,
+                                         int lane) {
+  uint64_t key[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int i = lane * E + e;
+    key[e] = i < len ? (static_cast<uint64_t>(static_cast<uint32_t>(cols[b + i])) << 32) |
+                           static_cast<uint32_t>(i)
+                     : ~0ull;
+  }
+  wave_bitonic_sort<E>(key, lane);
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int i = lane * E + e;
+    if (i < len) {
+      out_cols[b + i] = static_cast<int32_t>(key[e] >> 32);
+      out_vals[b + i] = vals[b + static_cast<int32_t>(key[e] & 0xFFFFFFFFu)];
+    }
+  }
+}
+
+// Copy of the CSR with every row's entries in ascending column order (one
+// wavefront per row).  In the SELL layout, lane r of a chunk then reads the
+// k-th smallest column of its row, and neighbouring particles' k-th columns
+// are neighbours too, so the gathers of x coalesce; in CSR (neighbor-list)
+// order they scatter over ~50 cache lines per load.  Rows longer than 512
+// entries are copied unsorted (still correct).
+__global__ void __launch_bounds__(kBlock) sort_rows_kernel(size_t n, const int32_t* rows,
+                                                           const int32_t* cols,
+                                                           const double* vals,
+                                                           int32_t* out_cols, double* out_vals) {
+  const int lane = threadIdx.x & 63;
+  const size_t wave = (static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+  const size_t waves = (static_cast<size_t>(gridDim.x) * blockDim.x) >> 6;
+  for (size_t row = wave; row < n; row += waves) {
+    const int32_t b = rows[row];
+    const int32_t len = rows[row + 1] - b;
+    if (len <= 64) {
+      sort_row<1>(b, len, cols, vals, out_cols, out_vals, lane);
+    } else if (len <= 128) {
+      sort_row<2>(b, len, cols, vals, out_cols, out_vals, lane);
+    } else if (len <= 256) {
+      sort_row<4>(b, len, cols, vals, out_cols, out_vals, lane);
+    } else if (len <= 512) {
+      sort_row<8>(b, len, cols, vals, out_cols, out_vals, lane);
+    } else {
+      for (int32_t i = lane; i < len; i += 64) {
+        out_cols[b + i] = cols[b + i];
+        out_vals[b + i] = vals[b + i];
+      }
+    }
+  }
+}
+
+// Index of offset d in a chunk's ascending DIA list, or -1.
+__device__ __forceinline__ int dia_find(const int32_t* list, int nd, int32_t d) {
+  int lo = 0, hi = nd - 1;
+  while (lo <= hi) {
+    const int mid = (lo + hi) >> 1;
+    const int32_t v = list[mid];
+    if (v == d) return mid;
+    if (v < d) lo = mid + 1; else hi = mid - 1;
+  }
+  return -1;
+}
+
+constexpr int kDiaTable = 2048;  // LDS hash slots (power of two)
+constexpr int32_t kDiaEmpty = INT32_MIN;
+
+// One 64-thread block per chunk, lane = row: counts the offsets column - row
+// of the chunk's entries in an LDS hash table, keeps those present in at least
+// kDiaMinRows rows as the chunk's diagonals (ascending), and sizes the
+// remainder.  A table overflow or more than kMaxDia qualifying offsets leaves
+// the chunk without diagonals, so the selection never depends on the order of
+// the atomics.
+__global__ void __launch_bounds__(kSellChunk) dia_select_kernel(
+    size_t n, const int32_t* rows, const int32_t* cols, int32_t* dia_count,
+    int32_t* dia_offsets, int32_t* dia_slots, int32_t* rem_width, int32_t* rem_scaled,
+    unsigned long long* rem_total) {
+  __shared__ int32_t keys[kDiaTable];
+  __shared__ int32_t counts[kDiaTable];
+  __shared__ int32_t list[kMaxDia];
+  __shared__ int32_t nlist;
+  __shared__ int32_t failed;
+  const size_t chunk = blockIdx.x;
+  const int lane = threadIdx.x;
+  const size_t r = chunk * kSellChunk + lane;
+  const int32_t b = r < n ? rows[r] : 0;
+  const int32_t e = r < n ? rows[r + 1] : 0;
+  for (int t = lane; t < kDiaTable; t += kSellChunk) {
+    keys[t] = kDiaEmpty;
+    counts[t] = 0;
+  }
+  if (lane == 0) {
+    nlist = 0;
+    failed = 0;
+  }
+  __syncthreads();
+  for (int32_t c = b; c < e; ++c) {
+    const int32_t d = cols[c] - static_cast<int32_t>(r);
+    uint32_t h = (static_cast<uint32_t>(d) * 2654435761u) & (kDiaTable - 1);
+    bool placed = false;
+    for (int probe = 0; probe < kDiaTable; ++probe) {
+      const int32_t prev = atomicCAS(&keys[h], kDiaEmpty, d);
+      if (prev == kDiaEmpty || prev == d) {
+        atomicAdd(&counts[h], 1);
+        placed = true;
+        break;
+      }
+      h = (h + 1) & (kDiaTable - 1);
+    }
+    if (!placed) failed = 1;
+  }
+  __syncthreads();
+  if (!failed) {
+    for (int t = lane; t < kDiaTable; t += kSellChunk) {
+      if (keys[t] != kDiaEmpty && counts[t] >= kDiaMinRows) {
+        const int32_t slot = atomicAdd(&nlist, 1);
+        if (slot < kMaxDia) list[slot] = keys[t]; else failed = 1;
+      }
+    }
+  }
+  __syncthreads();
+  const int nd = failed ? 0 : nlist;
+  // sort the list ascending (signed order via the sign-flipped high word)
+  {
+    uint64_t key[kMaxDia / kSellChunk];
+#pragma unroll
+    for (int q = 0; q < kMaxDia / kSellChunk; ++q) {
+      const int i = lane * (kMaxDia / kSellChunk) + q;
+      key[q] = i < nd ? static_cast<uint64_t>(static_cast<uint32_t>(list[i]) ^ 0x80000000u) << 32
+                      : ~0ull;
+    }
+    wave_bitonic_sort<kMaxDia / kSellChunk>(key, lane);
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kMaxDia / kSellChunk; ++q) {
+      const int i = lane * (kMaxDia / kSellChunk) + q;
+      if (i < nd) {
+        const int32_t d = static_cast<int32_t>(static_cast<uint32_t>(key[q] >> 32) ^ 0x80000000u);
+        list[i] = d;
+        dia_offsets[chunk * kMaxDia + i] = d;
+      }
+    }
+  }
+  __syncthreads();
+  // remainder entries of this row, and the chunk's widest remainder row
+  int32_t rem = 0;
+  for (int32_t c = b; c < e; ++c)
+    if (dia_find(list, nd, cols[c] - static_cast<int32_t>(r)) < 0) ++rem;
+  int32_t width = rem;
+  unsigned long long total = static_cast<unsigned long long>(rem);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    width = max(width, __shfl_xor(width, off, 64));
+    total += __shfl_xor(total, off, 64);
+  }
+  if (lane == 0) {
+    dia_count[chunk] = nd;
+    dia_slots[chunk] = nd * kSellChunk;
+    rem_width[chunk] = width;
+    rem_scaled[chunk] = width * kSellChunk;
+    atomicAdd(rem_total, total);
   }
 }
 
@@ -197,8 +387,9 @@ __global__ void sell_width_kernel(size_t n, size_t chunks, const int32_t* rows, 
 // span, or more than kSellWindows windows, give the chunk 32-bit columns.
 constexpr int kBucketBits = kSellWindowBits - 1;
 __global__ void sell_window_kernel(size_t n, size_t chunks, const int32_t* rows,
-                                   const int32_t* cols, const int32_t* width, int32_t* nwin,
-                                   int32_t* windows, int32_t* col_rows) {
+                                   const int32_t* cols, const int32_t* dia_count,
+                                   const int32_t* dia_offsets, const int32_t* width,
+                                   int32_t* nwin, int32_t* windows, int32_t* col_rows) {
   const size_t chunk = (static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x) / kSellChunk;
   if (chunk >= chunks) return;  // uniform per wavefront
   const int lane = threadIdx.x & 63;
@@ -206,9 +397,12 @@ __global__ void sell_window_kernel(size_t n, size_t chunks, const int32_t* rows,
   const int32_t b = r < n ? rows[r] : 0;
   const int32_t e = r < n ? rows[r + 1] : 0;
   const int64_t first_bucket = static_cast<int64_t>(chunk * kSellChunk >> kBucketBits) - 64;
+  const int nd = dia_count[chunk];
+  const int32_t* dl = dia_offsets + chunk * kMaxDia;
   uint64_t lo = 0, hi = 0;
   int outside = 0;
   for (int32_t c = b; c < e; ++c) {
+    if (dia_find(dl, nd, cols[c] - static_cast<int32_t>(r)) >= 0) continue;  // a DIA entry
     const int64_t rel = (static_cast<int64_t>(cols[c]) >> kBucketBits) - first_bucket;
     if (rel < 0 || rel >= 128) {
       outside = 1;
@@ -256,9 +450,15 @@ __global__ void sell_window_kernel(size_t n, size_t chunks, const int32_t* rows,
 
 // One thread per (padded) row writes its entries: values, and columns encoded
 // against the chunk's windows (or as 32-bit indices).  Padding points at base_0.
+// One thread per (padded) row: DIA values into their diagonal slots (the DIA
+// array is zeroed beforehand), every other entry into the remainder with its
+// column encoded against the chunk's windows (or as a 32-bit index); remainder
+// padding points at base_0 with value 0.
 template <typename V>
 __global__ void sell_fill_kernel(size_t n, size_t chunks, const int32_t* rows,
                                  const int32_t* cols, const double* vals,
+                                 const int32_t* dia_count, const int32_t* dia_offsets,
+                                 const int32_t* dia_start, V* dia_vals,
                                  const int32_t* chunk_start, const int32_t* width,
                                  const int32_t* col_start, const int32_t* nwin,
                                  const int32_t* windows, uint16_t* out_cols, V* out_vals) {
@@ -266,6 +466,9 @@ __global__ void sell_fill_kernel(size_t n, size_t chunks, const int32_t* rows,
   if (r >= chunks * kSellChunk) return;
   const size_t chunk = r / kSellChunk;
   const int lane = static_cast<int>(r % kSellChunk);
+  const int nd = dia_count[chunk];
+  const int32_t* dl = dia_offsets + chunk * kMaxDia;
+  const int64_t dbase = dia_start[chunk];
   const int64_t vbase = chunk_start[chunk];
   const int64_t cbase = static_cast<int64_t>(col_start[chunk]) * kSellChunk;
   const int32_t w = width[chunk];
@@ -273,10 +476,10 @@ __global__ void sell_fill_kernel(size_t n, size_t chunks, const int32_t* rows,
   const int32_t* wb = windows + chunk * kSellWindows;
   const int32_t b = r < n ? rows[r] : 0;
   const int32_t len = r < n ? rows[r + 1] - b : 0;
-  for (int32_t k = 0; k < w; ++k) {
+  int32_t k = 0;  // next remainder slot
+  auto put = [&](int32_t c, V v) {
     const int64_t slot = static_cast<int64_t>(k) * kSellChunk + lane;
-    const int32_t c = k < len ? cols[b + k] : wb[0];
-    out_vals[vbase + slot] = k < len ? static_cast<V>(vals[b + k]) : V(0);
+    out_vals[vbase + slot] = v;
     if (nw < 0) {
       reinterpret_cast<int32_t*>(out_cols + cbase)[slot] = c;
     } else {
@@ -284,13 +487,28 @@ __global__ void sell_fill_kernel(size_t n, size_t chunks, const int32_t* rows,
       while (j + 1 < nw && wb[j + 1] <= c) ++j;
       out_cols[cbase + slot] = static_cast<uint16_t>((j << kSellWindowBits) | (c - wb[j]));
     }
+    ++k;
+  };
+  for (int32_t q = 0; q < len; ++q) {
+    const int32_t c = cols[b + q];
+    const int j = dia_find(dl, nd, c - static_cast<int32_t>(r));
+    if (j >= 0) {
+      dia_vals[dbase + static_cast<int64_t>(j) * kSellChunk + lane] = static_cast<V>(vals[b + q]);
+    } else {
+      put(c, static_cast<V>(vals[b + q]));
+    }
   }
+  while (k < w) put(wb[0], V(0));
 }
 
 template <typename T, typename V>
 struct SellArgs {
   size_t n;
   size_t chunks;
+  const int32_t* dia_count;
+  const int32_t* dia_offsets;
+  const int32_t* dia_start;
+  const V* dia_vals;
   const int32_t* chunk_start;
   const int32_t* width;
   const int32_t* col_start;
@@ -332,17 +550,41 @@ __global__ void __launch_bounds__(kSellBlock) sell_spmv_kernel(SellArgs<T, V> a)
 #pragma unroll
   for (int q = 0; q < NQ; ++q) acc[q] = 0.0;
   for (size_t chunk = first + wave_in_block; chunk < last; chunk += kSellBlock / 64) {
+    const size_t row = chunk * kSellChunk + lane;
+    const int32_t irow = static_cast<int32_t>(row);
+    const int32_t last_row = static_cast<int32_t>(a.n) - 1;
     const int32_t w = a.width[chunk];
     const int32_t nw = a.nwin[chunk];  // uniform per wavefront
     const V* vp = a.vals + a.chunk_start[chunk] + lane;
     const int64_t cbase = static_cast<int64_t>(a.col_start[chunk]) * kSellChunk;
-    // 8 columns in flight per lane: all column and value loads of a group are
-    // issued before the dependent gathers of x.
     // FP64 accumulation whatever the storage types: the SpMV is bandwidth bound.
     constexpr int U = 8;
     double s[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) s[u] = 0.0;
+    // DIA part: column = row + offset (clamped into range for padding slots,
+    // whose value is 0), so the wavefront's gathers are contiguous
+    {
+      const int nd = a.dia_count[chunk];
+      const int32_t* dl = a.dia_offsets + chunk * kMaxDia;
+      const V* dp = a.dia_vals + a.dia_start[chunk] + lane;
+      int j = 0;
+      for (; j + U <= nd; j += U) {
+        V v[U];
+        int32_t c[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          v[u] = __builtin_nontemporal_load(dp + (j + u) * kSellChunk);
+          c[u] = min(max(irow + dl[j + u], 0), last_row);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) s[u] += static_cast<double>(v[u]) * static_cast<double>(a.x[c[u]]);
+      }
+      for (; j < nd; ++j) {
+        const int32_t c = min(max(irow + dl[j], 0), last_row);
+        s[0] += static_cast<double>(dp[j * kSellChunk]) * static_cast<double>(a.x[c]);
+      }
+    }
     int32_t k = 0;
     if (nw >= 0) {
       // lane l holds window base l % 16; an entry's base is fetched across lanes
@@ -386,7 +628,6 @@ __global__ void __launch_bounds__(kSellBlock) sell_spmv_kernel(SellArgs<T, V> a)
         s[0] += static_cast<double>(vp[k * kSellChunk]) * static_cast<double>(a.x[cp[k * kSellChunk]]);
     }
     const double sum = ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
-    const size_t row = chunk * kSellChunk + lane;
     if (row < a.n) {
       // the dots use the stored (rounded) value, which the later updates read
       const T yv = static_cast<T>(sum);
@@ -535,8 +776,9 @@ __global__ void __launch_bounds__(kBlock) idr_a2_kernel(IdrVec<T> a, int k) {
 template <typename T>
 __global__ void precondition_kernel(IdrVec<T> a) {
   for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < a.n;
-       i += static_cast<size_t>(gridDim.x) * blockDim.x)
+       i += static_cast<size_t>(gridDim.x) * blockDim.x) {
     a.v[i] = static_cast<T>(precondition(a, i, static_cast<double>(a.v[i])));
+  }
 }
 
 __global__ void idr_init_scalars(SolverScalars* sc, int s) {
@@ -795,99 +1037,6 @@ unsigned sell_blocks(size_t chunks) {
 }
 
 // Converts the CSR (FP64 values) into the SELL-64 layout with values of type V.
-// Bitonic sort of one row's (column, position) keys held by a wavefront, E keys
-// per lane (key i = lane * E + e); P = 64 E keys, padded with all-ones keys.
-template <int E>
-__device__ __forceinline__ void wave_bitonic_sort(uint64_t (&key)[E], int lane) {
-  constexpr int P = 64 * E;
-#pragma unroll
-  for (int k = 2; k <= P; k <<= 1) {
-#pragma unroll
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      if (j < E) {  // partner in the same lane
-#pragma unroll
-        for (int e = 0; e < E; ++e) {
-          const int f = e ^ j;
-          if (f > e) {
-            const bool up = ((lane * E + e) & k) == 0;
-            const uint64_t a = key[e], b = key[f];
-            if ((a > b) == up) {
-              key[e] = b;
-              key[f] = a;
-            }
-          }
-        }
-      } else {  // partner lane = lane ^ (j / E), same slot
-        const int m = j / E;
-        const bool lower = (lane & m) == 0;
-#pragma unroll
-        for (int e = 0; e < E; ++e) {
-          const uint64_t other = __shfl_xor(key[e], m, 64);
-          const bool up = ((lane * E + e) & k) == 0;
-          const bool keep_min = lower == up;
-          key[e] = keep_min ? (other < key[e] ? other : key[e]) : (other > key[e] ? other : key[e]);
-        }
-      }
-    }
-  }
-}
-
-template <int E>
-__device__ __forceinline__ void sort_row(int32_t b, int32_t len, const int32_t* cols,
-                                         const double* vals, int32_t* out_cols, double* out_vals,
-                                         int lane) {
-  uint64_t key[E];
-#pragma unroll
-  for (int e = 0; e < E; ++e) {
-    const int i = lane * E + e;
-    key[e] = i < len ? (static_cast<uint64_t>(static_cast<uint32_t>(cols[b + i])) << 32) |
-                           static_cast<uint32_t>(i)
-                     : ~0ull;
-  }
-  wave_bitonic_sort<E>(key, lane);
-#pragma unroll
-  for (int e = 0; e < E; ++e) {
-    const int i = lane * E + e;
-    if (i < len) {
-      out_cols[b + i] = static_cast<int32_t>(key[e] >> 32);
-      out_vals[b + i] = vals[b + static_cast<int32_t>(key[e] & 0xFFFFFFFFu)];
-    }
-  }
-}
-
-// Copy of the CSR with every row's entries in ascending column order (one
-// wavefront per row).  In the SELL layout, lane r of a chunk then reads the
-// k-th smallest column of its row, and neighbouring particles' k-th columns
-// are neighbours too, so the gathers of x coalesce; in CSR (neighbor-list)
-// order they scatter over ~50 cache lines per load.  Rows longer than 512
-// entries are copied unsorted (still correct).
-__global__ void __launch_bounds__(kBlock) sort_rows_kernel(size_t n, const int32_t* rows,
-                                                           const int32_t* cols,
-                                                           const double* vals,
-                                                           int32_t* out_cols, double* out_vals) {
-  const int lane = threadIdx.x & 63;
-  const size_t wave = (static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
-  const size_t waves = (static_cast<size_t>(gridDim.x) * blockDim.x) >> 6;
-  for (size_t row = wave; row < n; row += waves) {
-    const int32_t b = rows[row];
-    const int32_t len = rows[row + 1] - b;
-    if (len <= 64) {
-      sort_row<1>(b, len, cols, vals, out_cols, out_vals, lane);
-    } else if (len <= 128) {
-      sort_row<2>(b, len, cols, vals, out_cols, out_vals, lane);
-    } else if (len <= 256) {
-      sort_row<4>(b, len, cols, vals, out_cols, out_vals, lane);
-    } else if (len <= 512) {
-      sort_row<8>(b, len, cols, vals, out_cols, out_vals, lane);
-    } else {
-      for (int32_t i = lane; i < len; i += 64) {
-        out_cols[b + i] = cols[b + i];
-        out_vals[b + i] = vals[b + i];
-      }
-    }
-  }
-}
-
 template <typename V>
 void build_sell(Context& ctx, SellMatrix& M, const lsmps_csr& A_in, V* out_vals_dummy_type) {
   (void)out_vals_dummy_type;
@@ -912,45 +1061,72 @@ void build_sell(Context& ctx, SellMatrix& M, const lsmps_csr& A_in, V* out_vals_
   A.values = M.sorted_vals.data();
   M.n = n;
   M.chunks = (n + kSellChunk - 1) / kSellChunk;
-  M.chunk_width.reserve(M.chunks);
-  M.chunk_scaled.reserve(M.chunks);
-  M.chunk_start.reserve(M.chunks + 1);
-  M.col_rows.reserve(M.chunks);
-  M.col_start.reserve(M.chunks + 1);
-  M.nwin.reserve(M.chunks);
-  M.windows.reserve(M.chunks * kSellWindows);
-  const unsigned row_blocks = blocks_for(M.chunks * kSellChunk);
-  sell_width_kernel<<<row_blocks, kBlock, 0, ctx.stream>>>(
-      n, M.chunks, A.row_offsets, M.chunk_width.data(), M.chunk_scaled.data());
-  sell_window_kernel<<<row_blocks, kBlock, 0, ctx.stream>>>(
-      n, M.chunks, A.row_offsets, A.col_indices, M.chunk_width.data(), M.nwin.data(),
-      M.windows.data(), M.col_rows.data());
+  const size_t chunks = M.chunks;
+  M.dia_count.reserve(chunks);
+  M.dia_offsets.reserve(chunks * kMaxDia);
+  M.dia_slots.reserve(chunks);
+  M.dia_start.reserve(chunks + 1);
+  M.chunk_width.reserve(chunks);
+  M.chunk_scaled.reserve(chunks);
+  M.chunk_start.reserve(chunks + 1);
+  M.col_rows.reserve(chunks);
+  M.col_start.reserve(chunks + 1);
+  M.nwin.reserve(chunks);
+  M.windows.reserve(chunks * kSellWindows);
+  M.rem_counter.reserve(1);
+  fill_kernel<unsigned long long><<<1, kBlock, 0, ctx.stream>>>(1, M.rem_counter.data(), 0ull);
+  // diagonals and remainder widths
+  dia_select_kernel<<<static_cast<unsigned>(chunks), kSellChunk, 0, ctx.stream>>>(
+      n, A.row_offsets, A.col_indices, M.dia_count.data(), M.dia_offsets.data(),
+      M.dia_slots.data(), M.chunk_width.data(), M.chunk_scaled.data(), M.rem_counter.data());
   LSMPS_LAUNCH_CHECK();
-  exclusive_scan_i32(ctx, M.chunk_scaled.data(), M.chunk_start.data(), M.chunks);
-  exclusive_scan_i32(ctx, M.col_rows.data(), M.col_start.data(), M.chunks);
-  LSMPS_HIP(hipMemcpyAsync(ctx.host_i32, M.chunk_start.data() + M.chunks, sizeof(int32_t),
+  const unsigned row_blocks = blocks_for(chunks * kSellChunk);
+  sell_window_kernel<<<row_blocks, kBlock, 0, ctx.stream>>>(
+      n, chunks, A.row_offsets, A.col_indices, M.dia_count.data(), M.dia_offsets.data(),
+      M.chunk_width.data(), M.nwin.data(), M.windows.data(), M.col_rows.data());
+  LSMPS_LAUNCH_CHECK();
+  exclusive_scan_i32(ctx, M.dia_slots.data(), M.dia_start.data(), chunks);
+  exclusive_scan_i32(ctx, M.chunk_scaled.data(), M.chunk_start.data(), chunks);
+  exclusive_scan_i32(ctx, M.col_rows.data(), M.col_start.data(), chunks);
+  LSMPS_HIP(hipMemcpyAsync(ctx.host_i32, M.chunk_start.data() + chunks, sizeof(int32_t),
                            hipMemcpyDeviceToHost, ctx.stream));
   LSMPS_HIP(hipMemcpyAsync(ctx.host_i32 + 1, A.row_offsets + n, sizeof(int32_t),
                            hipMemcpyDeviceToHost, ctx.stream));
-  LSMPS_HIP(hipMemcpyAsync(ctx.host_i32 + 2, M.col_start.data() + M.chunks, sizeof(int32_t),
+  LSMPS_HIP(hipMemcpyAsync(ctx.host_i32 + 2, M.col_start.data() + chunks, sizeof(int32_t),
+                           hipMemcpyDeviceToHost, ctx.stream));
+  LSMPS_HIP(hipMemcpyAsync(ctx.host_i32 + 3, M.dia_start.data() + chunks, sizeof(int32_t),
+                           hipMemcpyDeviceToHost, ctx.stream));
+  LSMPS_HIP(hipMemcpyAsync(ctx.host_f64, M.rem_counter.data(), sizeof(unsigned long long),
                            hipMemcpyDeviceToHost, ctx.stream));
   ctx.sync();
-  require(ctx.host_i32[0] >= 0 && ctx.host_i32[2] >= 0,
+  require(ctx.host_i32[0] >= 0 && ctx.host_i32[2] >= 0 && ctx.host_i32[3] >= 0,
           "PPE matrix too large for the 32-bit SELL layout");
   M.entries = static_cast<size_t>(ctx.host_i32[0]);
   M.nnz = static_cast<size_t>(ctx.host_i32[1]);
   M.col_units = static_cast<size_t>(ctx.host_i32[2]) * kSellChunk;
+  M.dia_entries = static_cast<size_t>(ctx.host_i32[3]);
+  unsigned long long rem = 0;
+  std::memcpy(&rem, ctx.host_f64, sizeof(rem));
+  M.rem_nnz = static_cast<size_t>(rem);
   M.cols.reserve(std::max<size_t>(M.col_units, 1));
   V* vals;
+  V* dia;
   if (sizeof(V) == sizeof(double)) {
     M.vals64.reserve(std::max<size_t>(M.entries, 1));
+    M.dia64.reserve(std::max<size_t>(M.dia_entries, 1));
     vals = reinterpret_cast<V*>(M.vals64.data());
+    dia = reinterpret_cast<V*>(M.dia64.data());
   } else {
     M.vals32.reserve(std::max<size_t>(M.entries, 1));
+    M.dia32.reserve(std::max<size_t>(M.dia_entries, 1));
     vals = reinterpret_cast<V*>(M.vals32.data());
+    dia = reinterpret_cast<V*>(M.dia32.data());
   }
+  fill_kernel<V><<<blocks_for(std::max<size_t>(M.dia_entries, 1)), kBlock, 0, ctx.stream>>>(
+      M.dia_entries, dia, V(0));
   sell_fill_kernel<V><<<row_blocks, kBlock, 0, ctx.stream>>>(
-      n, M.chunks, A.row_offsets, A.col_indices, A.values, M.chunk_start.data(),
+      n, chunks, A.row_offsets, A.col_indices, A.values, M.dia_count.data(),
+      M.dia_offsets.data(), M.dia_start.data(), dia, M.chunk_start.data(),
       M.chunk_width.data(), M.col_start.data(), M.nwin.data(), M.windows.data(), M.cols.data(),
       vals);
   LSMPS_LAUNCH_CHECK();
@@ -994,6 +1170,8 @@ int idr_solve(Context& ctx, PpeSolver& S, size_t n, const SellMatrix& M, const V
   T* G = work + 3 * ld;
   T* U = work + (3 + s) * ld;
   const bool schwarz = S.schwarz.blocks > 0 && S.schwarz.n == n;
+  const V* dia = sizeof(V) == sizeof(double) ? reinterpret_cast<const V*>(M.dia64.data())
+                                             : reinterpret_cast<const V*>(M.dia32.data());
   IdrVec<T> iv{n, ld, s, r, x, v, t, G, U, p0, dinv, S.scalars.data(), S.part_b.data(),
                schwarz ? S.schwarz.cover_start.data() : nullptr,
                schwarz ? S.schwarz.cover_entries.data() : nullptr,
@@ -1015,14 +1193,17 @@ int idr_solve(Context& ctx, PpeSolver& S, size_t n, const SellMatrix& M, const V
     for (auto& e : ctx.timing_events) LSMPS_HIP(hipEventCreate(&e));
     ctx.timing_ready = true;
   }
-  // Algorithmic bytes of one SpMV: the stored nonzeros (value + 16-bit column,
-  // or 32-bit in the chunks that need it, in the layout's proportion), the
-  // per-chunk descriptors and window bases, x read once and y written once.
-  // Padding is not counted.
+  // Algorithmic bytes of one SpMV: the stored nonzeros' values, the column
+  // indices of the remainder entries (16-bit, or 32-bit in the chunks that need
+  // it, in the layout's proportion; DIA columns are implicit), the per-chunk
+  // descriptors, diagonal offsets and window bases, x read once and y written
+  // once.  Padding is not counted.
   const double col_bytes =
       2.0 * static_cast<double>(M.col_units) / static_cast<double>(std::max<size_t>(M.entries, 1));
-  const double spmv_bytes = static_cast<double>(M.nnz) * (sizeof(V) + col_bytes) +
-                            M.chunks * (4.0 + kSellWindows) * sizeof(int32_t) +
+  const double spmv_bytes = static_cast<double>(M.nnz) * sizeof(V) +
+                            static_cast<double>(M.rem_nnz) * col_bytes +
+                            M.chunks * (6.0 + kSellWindows) * sizeof(int32_t) +
+                            static_cast<double>(M.dia_entries) / kSellChunk * sizeof(int32_t) +
                             2.0 * n * sizeof(T);
   auto mark = [&](int slot) {
     if (timing) LSMPS_HIP(hipEventRecord(ctx.timing_events[slot], ctx.stream));
@@ -1040,7 +1221,8 @@ int idr_solve(Context& ctx, PpeSolver& S, size_t n, const SellMatrix& M, const V
       } else {
         idr_a_kernel<T><<<vb, kBlock, 0, ctx.stream>>>(iv, k);
       }
-      SellArgs<T, V> sa{n, M.chunks, M.chunk_start.data(), M.chunk_width.data(),
+      SellArgs<T, V> sa{n, M.chunks, M.dia_count.data(), M.dia_offsets.data(),
+                        M.dia_start.data(), dia, M.chunk_start.data(), M.chunk_width.data(),
                         M.col_start.data(), M.nwin.data(), M.windows.data(), M.cols.data(),
                         vals, U + k * ld, G + k * ld, p0, s, nullptr, S.part_a.data()};
       mark(2 * k);
@@ -1056,7 +1238,8 @@ int idr_solve(Context& ctx, PpeSolver& S, size_t n, const SellMatrix& M, const V
       schwarz_accumulate<T>(ctx, S.schwarz, v);
       precondition_kernel<T><<<vb, kBlock, 0, ctx.stream>>>(iv);
     }
-    SellArgs<T, V> st{n, M.chunks, M.chunk_start.data(), M.chunk_width.data(),
+    SellArgs<T, V> st{n, M.chunks, M.dia_count.data(), M.dia_offsets.data(),
+                      M.dia_start.data(), dia, M.chunk_start.data(), M.chunk_width.data(),
                       M.col_start.data(), M.nwin.data(), M.windows.data(), M.cols.data(), vals,
                       v, t, nullptr, s, r, S.part_a.data()};
     mark(2 * s);

@@ -28,44 +28,61 @@ struct SolverScalars {
   int32_t pad;
 };
 
-// Sliced ELLPACK with 64-row chunks (one wavefront per chunk): entry k of the
-// chunk's row r sits at chunk_start[c] + k * 64 + r, padded to the chunk's
-// widest row with value 0.
+// PPE matrix for the Krylov SpMVs: a hybrid of a per-chunk diagonal (DIA)
+// part and a sliced-ELLPACK remainder, over 64-row chunks (one wavefront per
+// chunk, one row per lane).
 //
-// Column indices are stored in 16 bits where the chunk allows it: the chunk's
-// columns are covered by at most kSellWindows windows [base_j, base_j + 4096),
-// each two aligned 2048-column buckets wide and laid greedily over the buckets
-// the chunk touches, and an entry holds (j << 12) | (column - base_j).
-// Particles in lattice or cell order need about 7-14 windows (one or two per z
-// layer of the 7x7x7 neighbourhood).  A chunk needing more windows, or with
-// columns more than 64 buckets from its first row, keeps 32-bit columns
-// (nwin = -1) in the same array, two 16-bit slots per entry.  Padding entries
-// point at base_0 with value 0.
+// DIA part: the offsets d = column - row that at least kDiaMinRows of the
+// chunk's rows have (particles in lattice or cell order share the ~123 offsets
+// of the 7x7x7 neighbourhood) are listed per chunk, ascending; their values sit
+// at dia_start[c] + j * 64 + lane, zero where a row lacks offset j.  Columns are
+// implicit, and the gathers x[row + d] of a wavefront are contiguous.
+//
+// Remainder: every other entry, in row-sorted order, padded to the chunk's
+// widest remainder row (value 0): entry k of the chunk's row r sits at
+// chunk_start[c] + k * 64 + r.  Its column indices are stored in 16 bits where
+// the chunk allows it: the chunk's columns are covered by at most kSellWindows
+// windows [base_j, base_j + 4096), each two aligned 2048-column buckets wide
+// and laid greedily over the buckets the chunk touches, and an entry holds
+// (j << 12) | (column - base_j).  A chunk needing more windows, or with columns
+// more than 64 buckets from its first row, keeps 32-bit columns (nwin = -1) in
+// the same array, two 16-bit slots per entry.  Padding points at base_0.
 constexpr int kSellChunk = 64;
 constexpr int kSellWindows = 16;
 constexpr int kSellWindowBits = 12;
+constexpr int kMaxDia = 256;
+constexpr int kDiaMinRows = 16;
 struct SellMatrix {
   size_t n = 0;
   size_t chunks = 0;
-  size_t entries = 0;    // padded value slots
-  size_t col_units = 0;  // 16-bit column slots (entries + entries of 32-bit chunks)
-  size_t nnz = 0;        // stored CSR nonzeros
-  DeviceArray<int32_t> chunk_start;  // value slot offset; padded entry count < 2^31 is checked
-  DeviceArray<int32_t> chunk_width;
+  size_t entries = 0;     // padded remainder value slots
+  size_t dia_entries = 0; // DIA value slots
+  size_t col_units = 0;   // 16-bit column slots (entries + entries of 32-bit chunks)
+  size_t nnz = 0;         // stored CSR nonzeros
+  size_t rem_nnz = 0;     // of which in the remainder
+  DeviceArray<int32_t> dia_count;    // diagonals per chunk
+  DeviceArray<int32_t> dia_offsets;  // chunks x kMaxDia, ascending
+  DeviceArray<int32_t> dia_slots;    // dia_count * 64 (scan input)
+  DeviceArray<int32_t> dia_start;    // scan of dia_slots
+  DeviceArray<int32_t> chunk_start;  // remainder value slot offset (< 2^31 is checked)
+  DeviceArray<int32_t> chunk_width;  // remainder width
   DeviceArray<int32_t> chunk_scaled;
   DeviceArray<int32_t> col_rows;     // per chunk: width * (1 or 2), units of 64 slots
   DeviceArray<int32_t> col_start;    // scan of col_rows
   DeviceArray<int32_t> nwin;         // windows used, or -1 for 32-bit columns
   DeviceArray<int32_t> windows;      // chunks x kSellWindows bases
+  DeviceArray<unsigned long long> rem_counter;
   DeviceArray<uint16_t> cols;
-  DeviceArray<double> vals64;
-  DeviceArray<float> vals32;
+  DeviceArray<double> vals64, dia64;
+  DeviceArray<float> vals32, dia32;
   DeviceArray<int32_t> sorted_cols;  // the CSR with each row's columns ascending
   DeviceArray<double> sorted_vals;
   size_t bytes() const {
-    return chunk_start.bytes() + chunk_width.bytes() + chunk_scaled.bytes() + col_rows.bytes() +
-           col_start.bytes() + nwin.bytes() + windows.bytes() + cols.bytes() + vals64.bytes() +
-           vals32.bytes() + sorted_cols.bytes() + sorted_vals.bytes();
+    return dia_count.bytes() + dia_offsets.bytes() + dia_slots.bytes() + dia_start.bytes() +
+           chunk_start.bytes() + chunk_width.bytes() + chunk_scaled.bytes() + col_rows.bytes() +
+           col_start.bytes() + nwin.bytes() + windows.bytes() + rem_counter.bytes() +
+           cols.bytes() + vals64.bytes() + dia64.bytes() + vals32.bytes() + dia32.bytes() +
+           sorted_cols.bytes() + sorted_vals.bytes();
   }
 };
 
@@ -85,8 +102,8 @@ class PpeSolver {
   PpeSolver(const PpeSolver&) = delete;
   PpeSolver& operator=(const PpeSolver&) = delete;
   size_t bytes() const {
-    return sell.bytes() + schwarz.bytes() + dinv64.bytes() + vec64.bytes() + part_a.bytes() +
-           part_b.bytes() + scalars.bytes();
+    return sell.bytes() + schwarz.bytes() + dinv64.bytes() + vec64.bytes() +
+           part_a.bytes() + part_b.bytes() + scalars.bytes();
   }
 };
 
