@@ -15,8 +15,10 @@
 //
 // Setup, per solve: one workgroup per seed row builds the member set in an LDS
 // hash table, sorts it (deterministic membership and order), assembles the dense
-// block, factors it and forms the inverse (column-major, so the application's
-// dense matvec reads coalesced columns).
+// block, factors it and forms the inverse (column-major).  The application
+// multiplies every block's inverse with its input slice (kApplyParts workgroups
+// per block) into partial products, and each covered row then takes the mean of
+// its blocks' values, summed in block order, in place.
 #include <algorithm>
 #include <vector>
 
@@ -251,8 +253,8 @@ __global__ void __launch_bounds__(kMaxBlock) factor_kernel(
       if (d == 0.0) d = 1.0e-300;
       X[i * ld + t] = s / d;
     }
-    // inverse(i, t) at t * ld + i (column-major): the application kernel reads
-    // column j as inv[j * ld + row], coalesced over its threads.
+    // inverse(i, t) at t * ld + i (column-major): the application reads column j
+    // as inv[j * ld + row], coalesced over its threads.
     for (int i = 0; i < n; ++i) inv[static_cast<size_t>(t) * ld + i] = static_cast<V>(X[i * ld + t]);
   }
 }
@@ -290,30 +292,78 @@ __global__ void cover_sort_kernel(size_t n, const int32_t* start, int32_t* entri
   }
 }
 
-// y_b = inv_b w_b for every block b; slot b * kMaxBlock + position.
+// y_b = inv_b w_b, split over kApplyParts workgroups per block b so that a
+// few hundred blocks still fill the GPU: workgroup (b, p) multiplies the column
+// range [p * 64, p * 64 + 64) and thread t (row t) writes its partial sum to
+// ybuf[(b * kApplyParts + p) * kMaxBlock + t].  The column slices are read
+// coalesced across the rows.
 template <typename T, typename V>
 __global__ void __launch_bounds__(kMaxBlock) apply_kernel(const int32_t* members,
                                                           const int32_t* sizes, const V* inverse,
                                                           const T* w, double* ybuf) {
-  __shared__ double wb[kMaxBlock];
+  __shared__ double wb[kApplyCols];
   const int t = threadIdx.x;
-  const int n = sizes[blockIdx.x];
+  const size_t b = blockIdx.x / kApplyParts;
+  const int part = static_cast<int>(blockIdx.x % kApplyParts);
+  const int n = sizes[b];
+  const int j0 = part * kApplyCols;
+  const int j1 = min(n, j0 + kApplyCols);
   constexpr int ld = kMaxBlock;
-  const int32_t* mem = members + static_cast<size_t>(blockIdx.x) * kMaxBlock;
-  const V* inv = inverse + static_cast<size_t>(blockIdx.x) * ld * ld;
-  const int32_t me = t < n ? mem[t] : 0;
-  wb[t] = t < n ? static_cast<double>(w[me]) : 0.0;
+  const int32_t* mem = members + b * kMaxBlock;
+  const V* inv = inverse + b * ld * ld;
+  if (t < kApplyCols) wb[t] = j0 + t < j1 ? static_cast<double>(w[mem[j0 + t]]) : 0.0;
   __syncthreads();
-  if (t < n) {
-    double y = 0.0;
-    for (int j = 0; j < n; ++j) y += static_cast<double>(inv[static_cast<size_t>(j) * ld + t]) * wb[j];
-    ybuf[static_cast<size_t>(blockIdx.x) * kMaxBlock + t] = y;
+  if (t >= n) return;
+  double y[4] = {0.0, 0.0, 0.0, 0.0};
+  int j = j0;
+  for (; j + 4 <= j1; j += 4) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      y[u] += static_cast<double>(inv[static_cast<size_t>(j + u) * ld + t]) * wb[j + u - j0];
   }
+  for (; j < j1; ++j) y[0] += static_cast<double>(inv[static_cast<size_t>(j) * ld + t]) * wb[j - j0];
+  ybuf[(b * kApplyParts + part) * kMaxBlock + t] = (y[0] + y[1]) + (y[2] + y[3]);
+}
+
+// Covered rows take the mean of their blocks' values (each the sum of its
+// kApplyParts partials), in ascending block order.
+template <typename T>
+__global__ void gather_kernel(size_t count, const int32_t* covered, const int32_t* start,
+                              const int32_t* entries, const double* ybuf, T* v) {
+  const size_t k = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k >= count) return;
+  const int32_t i = covered[k];
+  const int32_t lo = start[i], hi = start[i + 1];
+  double z = 0.0;
+  for (int32_t e = lo; e < hi; ++e) {
+    const int32_t slot = entries[e];  // block * kMaxBlock + position
+    const size_t base = static_cast<size_t>(slot / kMaxBlock) * kApplyParts * kMaxBlock +
+                        static_cast<size_t>(slot % kMaxBlock);
+#pragma unroll
+    for (int p = 0; p < kApplyParts; ++p) z += ybuf[base + static_cast<size_t>(p) * kMaxBlock];
+  }
+  v[i] = static_cast<T>(z / static_cast<double>(hi - lo));
+}
+
+// pdiag and the flags of covered rows.
+__global__ void pdiag_kernel(size_t n, const int32_t* start, const double* dinv, double* pdiag,
+                             int32_t* flag) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const bool covered = start[i + 1] > start[i];
+  pdiag[i] = covered ? 1.0 : dinv[i];
+  flag[i] = covered ? 1 : 0;
+}
+
+__global__ void compact_kernel(size_t n, const int32_t* flag, const int32_t* pos, int32_t* out) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n && flag[i]) out[pos[i]] = static_cast<int32_t>(i);
 }
 
 }  // namespace
 
-void schwarz_setup(Context& ctx, SchwarzPrec& P, const lsmps_csr& A, bool fp32_inverse) {
+void schwarz_setup(Context& ctx, SchwarzPrec& P, const lsmps_csr& A, const double* dinv,
+                   bool fp32_inverse) {
   const size_t n = A.rows;
   P.n = n;
   P.blocks = 0;
@@ -330,6 +380,7 @@ void schwarz_setup(Context& ctx, SchwarzPrec& P, const lsmps_csr& A, bool fp32_i
   ctx.sync();
   const size_t nb = static_cast<size_t>(ctx.host_i32[0]);
   P.blocks = nb;
+  P.covered_count = 0;
   if (nb == 0) return;
   P.seeds.reserve(nb);
   compact_seeds_kernel<<<blocks_for(n), kBlock, 0, ctx.stream>>>(n, P.seed_flag.data(),
@@ -357,7 +408,7 @@ void schwarz_setup(Context& ctx, SchwarzPrec& P, const lsmps_csr& A, bool fp32_i
   // Row -> covering slots (CSR over rows), sorted per row.
   P.cover.reserve(n);
   P.cover_start.reserve(n + 1);
-  P.ybuf.reserve(nb * kMaxBlock);
+  P.ybuf.reserve(nb * kApplyParts * kMaxBlock);
   fill_i32(ctx.stream, P.cover.data(), n, 0);
   cover_count_kernel<<<static_cast<unsigned>(nb), kMaxBlock, 0, ctx.stream>>>(
       P.members.data(), P.sizes.data(), P.cover.data());
@@ -370,24 +421,42 @@ void schwarz_setup(Context& ctx, SchwarzPrec& P, const lsmps_csr& A, bool fp32_i
       P.cover_entries.data());
   cover_sort_kernel<<<blocks_for(n), kBlock, 0, ctx.stream>>>(n, P.cover_start.data(),
                                                              P.cover_entries.data());
+  // preconditioner diagonal and the list of covered rows (flags and scan reuse
+  // the seed arrays, which are no longer needed)
+  P.pdiag.reserve(n);
+  pdiag_kernel<<<blocks_for(n), kBlock, 0, ctx.stream>>>(n, P.cover_start.data(), dinv,
+                                                         P.pdiag.data(), P.seed_flag.data());
+  LSMPS_LAUNCH_CHECK();
+  exclusive_scan_i32(ctx, P.seed_flag.data(), P.seed_pos.data(), n);
+  LSMPS_HIP(hipMemcpyAsync(ctx.host_i32, P.seed_pos.data() + n, sizeof(int32_t),
+                           hipMemcpyDeviceToHost, ctx.stream));
+  ctx.sync();
+  P.covered_count = static_cast<size_t>(ctx.host_i32[0]);
+  P.covered.reserve(std::max<size_t>(P.covered_count, 1));
+  compact_kernel<<<blocks_for(n), kBlock, 0, ctx.stream>>>(n, P.seed_flag.data(),
+                                                           P.seed_pos.data(), P.covered.data());
   LSMPS_LAUNCH_CHECK();
 }
 
 template <typename T>
-void schwarz_accumulate(Context& ctx, const SchwarzPrec& P, const T* w) {
+void schwarz_apply(Context& ctx, const SchwarzPrec& P, T* v) {
   if (P.blocks == 0) return;
+  const unsigned grid = static_cast<unsigned>(P.blocks * kApplyParts);
   if (P.fp32) {
-    apply_kernel<T, float><<<static_cast<unsigned>(P.blocks), kMaxBlock, 0, ctx.stream>>>(
-        P.members.data(), P.sizes.data(), P.inv32.data(), w, P.ybuf.data());
+    apply_kernel<T, float><<<grid, kMaxBlock, 0, ctx.stream>>>(P.members.data(), P.sizes.data(),
+                                                               P.inv32.data(), v, P.ybuf.data());
   } else {
-    apply_kernel<T, double><<<static_cast<unsigned>(P.blocks), kMaxBlock, 0, ctx.stream>>>(
-        P.members.data(), P.sizes.data(), P.inv64.data(), w, P.ybuf.data());
+    apply_kernel<T, double><<<grid, kMaxBlock, 0, ctx.stream>>>(P.members.data(), P.sizes.data(),
+                                                                P.inv64.data(), v, P.ybuf.data());
   }
+  gather_kernel<T><<<blocks_for(std::max<size_t>(P.covered_count, 1)), kBlock, 0, ctx.stream>>>(
+      P.covered_count, P.covered.data(), P.cover_start.data(), P.cover_entries.data(),
+      P.ybuf.data(), v);
   LSMPS_LAUNCH_CHECK();
 }
 
-template void schwarz_accumulate<double>(Context&, const SchwarzPrec&, const double*);
-template void schwarz_accumulate<float>(Context&, const SchwarzPrec&, const float*);
+template void schwarz_apply<double>(Context&, const SchwarzPrec&, double*);
+template void schwarz_apply<float>(Context&, const SchwarzPrec&, float*);
 
 void schwarz_stats(Context& ctx, const SchwarzPrec& P, int64_t* blocks, int64_t* members) {
   *blocks = static_cast<int64_t>(P.blocks);

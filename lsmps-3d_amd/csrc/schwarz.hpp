@@ -6,6 +6,8 @@
 namespace lsmps {
 
 constexpr int kMaxBlock = 256;  // rows per dense block (= workgroup size of its kernels)
+constexpr int kApplyParts = 4;                       // workgroups per block in the application
+constexpr int kApplyCols = kMaxBlock / kApplyParts;  // columns per workgroup
 
 struct SchwarzPrec {
   size_t n = 0;
@@ -20,19 +22,26 @@ struct SchwarzPrec {
   DeviceArray<int32_t> cover;          // per-row counter (setup only)
   DeviceArray<int32_t> cover_start;    // n + 1: row i's slots are cover_entries[start[i], start[i+1])
   DeviceArray<int32_t> cover_entries;  // slots b * kMaxBlock + position, ascending per row
-  DeviceArray<double> ybuf;            // blocks x kMaxBlock: last block products
+  DeviceArray<int32_t> covered;        // the covered rows, ascending
+  size_t covered_count = 0;
+  DeviceArray<double> pdiag;           // 1 on covered rows, 1 / A_ii elsewhere
+  DeviceArray<double> ybuf;            // blocks x kApplyParts x kMaxBlock: partial products
   size_t bytes() const {
     return eligible.bytes() + seed_flag.bytes() + seed_pos.bytes() + seeds.bytes() +
            members.bytes() + sizes.bytes() + scratch.bytes() + inv64.bytes() + inv32.bytes() +
-           cover.bytes() + cover_start.bytes() + cover_entries.bytes() + ybuf.bytes();
+           cover.bytes() + cover_start.bytes() + cover_entries.bytes() + covered.bytes() +
+           pdiag.bytes() + ybuf.bytes();
   }
 };
 
-void schwarz_setup(Context& ctx, SchwarzPrec& P, const lsmps_csr& A, bool fp32_inverse);
-// ybuf[b * kMaxBlock + p] = (inv_b w_b)_p for every block b; rows covered by
-// blocks then take the mean of their slots (cover_start / cover_entries).
+// Builds the blocks of matrix A; dinv is its Jacobi diagonal 1 / A_ii.
+void schwarz_setup(Context& ctx, SchwarzPrec& P, const lsmps_csr& A, const double* dinv,
+                   bool fp32_inverse);
+// In-place completion of a preconditioned vector: on entry v = pdiag * w (w on
+// covered rows, Jacobi values elsewhere); on exit each covered row holds the mean
+// of its blocks' values (inv_b w_b), summed in block order.
 template <typename T>
-void schwarz_accumulate(Context& ctx, const SchwarzPrec& P, const T* w);
+void schwarz_apply(Context& ctx, const SchwarzPrec& P, T* v);
 void schwarz_stats(Context& ctx, const SchwarzPrec& P, int64_t* blocks, int64_t* members);
 
 }  // namespace lsmps

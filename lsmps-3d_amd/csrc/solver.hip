@@ -540,15 +540,15 @@ __device__ __forceinline__ unsigned xcd_block() {
 constexpr int kSellBlock = 512;
 template <typename T, typename V, int MODE>
 __global__ void __launch_bounds__(kSellBlock) sell_spmv_kernel(SellArgs<T, V> a) {
-  constexpr int NQ = MODE == 1 ? kMaxShadow : (MODE == 2 ? 2 : 1);
   const int lane = threadIdx.x & 63;
   const unsigned wave_in_block = threadIdx.x >> 6;
   const size_t per_block = (a.chunks + gridDim.x - 1) / gridDim.x;
   const size_t first = static_cast<size_t>(xcd_block()) * per_block;
   const size_t last = first + per_block < a.chunks ? first + per_block : a.chunks;
-  double acc[NQ];
-#pragma unroll
-  for (int q = 0; q < NQ; ++q) acc[q] = 0.0;
+  const int nq = MODE == 1 ? a.s : (MODE == 2 ? 2 : 0);
+  // Lane q keeps the wavefront's running dot product q (one register instead of
+  // one per dot in every lane, which keeps occupancy up).
+  double lane_acc = 0.0;
   for (size_t chunk = first + wave_in_block; chunk < last; chunk += kSellBlock / 64) {
     const size_t row = chunk * kSellChunk + lane;
     const int32_t irow = static_cast<int32_t>(row);
@@ -628,22 +628,35 @@ __global__ void __launch_bounds__(kSellBlock) sell_spmv_kernel(SellArgs<T, V> a)
         s[0] += static_cast<double>(vp[k * kSellChunk]) * static_cast<double>(a.x[cp[k * kSellChunk]]);
     }
     const double sum = ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
+    double ys = 0.0;
     if (row < a.n) {
       // the dots use the stored (rounded) value, which the later updates read
       const T yv = static_cast<T>(sum);
       a.y[row] = yv;
-      const double ys = static_cast<double>(yv);
-      if (MODE == 1) {
-#pragma unroll
-        for (int q = 0; q < NQ; ++q)
-          if (q < a.s) acc[q] += shadow(a.p0, row, q) * ys;
-      } else if (MODE == 2) {
-        acc[0] += ys * a.r[row];
-        acc[1 % NQ] += ys * ys;
+      ys = static_cast<double>(yv);
+    }
+    for (int q = 0; q < nq; ++q) {  // uniform trip count
+      double d = 0.0;
+      if (row < a.n) {
+        if (MODE == 1) d = shadow(a.p0, row, q) * ys;
+        else d = (q == 0 ? a.r[row] : ys) * ys;
       }
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) d += __shfl_xor(d, off, 64);
+      if (lane == q) lane_acc += d;
     }
   }
-  if (MODE != 0) block_reduce_store<NQ, kSellBlock>(acc, MODE == 1 ? a.s : (MODE == 2 ? 2 : 1), a.partials);
+  if (MODE != 0) {
+    __shared__ double red[kSellBlock / 64][kPartialStride];
+    if (lane < kPartialStride) red[wave_in_block][lane] = lane < nq ? lane_acc : 0.0;
+    __syncthreads();
+    if (static_cast<int>(threadIdx.x) < kPartialStride) {
+      double t = 0.0;
+#pragma unroll
+      for (int w = 0; w < kSellBlock / 64; ++w) t += red[w][threadIdx.x];
+      a.partials[static_cast<size_t>(blockIdx.x) * kPartialStride + threadIdx.x] = t;
+    }
+  }
 }
 
 // r = b - A x in FP64; optional FP32 copy of r; partial r.r.
@@ -724,30 +737,15 @@ struct IdrVec {
   T* G;
   T* U;
   const T* p0;
-  const double* dinv;
   SolverScalars* sc;
   double* partials;
-  const int32_t* cover_start;    // Schwarz (or null: Jacobi only): row -> block slots
-  const int32_t* cover_entries;
-  const double* ybuf;            // Schwarz block products of the last application
+  // Diagonal part of the preconditioner: 1 / A_ii, except 1 on the rows the
+  // Schwarz blocks cover, whose values the block application then replaces.
+  const double* pdiag;
 };
 
-// Preconditioned value of row i of w: the mean of its Schwarz block values
-// (summed in block order) where blocks cover the row, Jacobi elsewhere.
-template <typename T>
-__device__ __forceinline__ double precondition(const IdrVec<T>& a, size_t i, double w) {
-  if (a.cover_start) {
-    const int32_t lo = a.cover_start[i], hi = a.cover_start[i + 1];
-    if (hi > lo) {
-      double z = 0.0;
-      for (int32_t e = lo; e < hi; ++e) z += a.ybuf[a.cover_entries[e]];
-      return z / static_cast<double>(hi - lo);
-    }
-  }
-  return a.dinv[i] * w;
-}
-
-// Schwarz path, first half of idr_a: v = r - sum_{q>=k} c_q G_q (unpreconditioned).
+// Schwarz path, first half of idr_a: v = pdiag (r - sum_{q>=k} c_q G_q): Jacobi
+// on uncovered rows, the raw value (the block application's input) on covered ones.
 template <typename T>
 __global__ void __launch_bounds__(kBlock) idr_a1_kernel(IdrVec<T> a, int k) {
   const SolverScalars* sc = a.sc;
@@ -755,11 +753,11 @@ __global__ void __launch_bounds__(kBlock) idr_a1_kernel(IdrVec<T> a, int k) {
        i += static_cast<size_t>(gridDim.x) * blockDim.x) {
     double v = a.r[i];
     for (int q = k; q < a.s; ++q) v -= sc->c[q] * static_cast<double>(a.G[q * a.ld + i]);
-    a.v[i] = static_cast<T>(v);
+    a.v[i] = static_cast<T>(a.pdiag[i] * v);
   }
 }
 
-// Schwarz path, second half: U_k = sum_{q>=k} c_q U_q + om M^-1 v.
+// Schwarz path, second half (v already preconditioned): U_k = sum_{q>=k} c_q U_q + om v.
 template <typename T>
 __global__ void __launch_bounds__(kBlock) idr_a2_kernel(IdrVec<T> a, int k) {
   const SolverScalars* sc = a.sc;
@@ -768,16 +766,7 @@ __global__ void __launch_bounds__(kBlock) idr_a2_kernel(IdrVec<T> a, int k) {
        i += static_cast<size_t>(gridDim.x) * blockDim.x) {
     double u = 0.0;
     for (int q = k; q < a.s; ++q) u += sc->c[q] * static_cast<double>(a.U[q * a.ld + i]);
-    a.U[k * a.ld + i] = static_cast<T>(u + om * precondition(a, i, static_cast<double>(a.v[i])));
-  }
-}
-
-// v = M^-1 v in place (dimension-reduction step on the Schwarz path).
-template <typename T>
-__global__ void precondition_kernel(IdrVec<T> a) {
-  for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < a.n;
-       i += static_cast<size_t>(gridDim.x) * blockDim.x) {
-    a.v[i] = static_cast<T>(precondition(a, i, static_cast<double>(a.v[i])));
+    a.U[k * a.ld + i] = static_cast<T>(u + om * static_cast<double>(a.v[i]));
   }
 }
 
@@ -889,7 +878,7 @@ __global__ void __launch_bounds__(kBlock) idr_a_kernel(IdrVec<T> a, int k) {
       v -= c * static_cast<double>(a.G[q * a.ld + i]);
       u += c * static_cast<double>(a.U[q * a.ld + i]);
     }
-    v *= a.dinv[i];
+    v *= a.pdiag[i];
     a.U[k * a.ld + i] = static_cast<T>(u + om * v);
   }
 }
@@ -916,8 +905,8 @@ __global__ void __launch_bounds__(kBlock) idr_b_kernel(IdrVec<T> a, int k, int w
     a.r[i] = r;
     a.x[i] += beta * u;
     acc[0] += r * r;
-    // write_v: 1 = Jacobi-preconditioned residual, 2 = raw residual (Schwarz path)
-    if (write_v) a.v[i] = static_cast<T>(write_v == 1 ? a.dinv[i] * r : r);
+    // dimension-reduction input: pdiag r (the Schwarz blocks then fill their rows)
+    if (write_v) a.v[i] = static_cast<T>(a.pdiag[i] * r);
   }
   block_reduce_store<1>(acc, 1, a.partials);
 }
@@ -1172,10 +1161,8 @@ int idr_solve(Context& ctx, PpeSolver& S, size_t n, const SellMatrix& M, const V
   const bool schwarz = S.schwarz.blocks > 0 && S.schwarz.n == n;
   const V* dia = sizeof(V) == sizeof(double) ? reinterpret_cast<const V*>(M.dia64.data())
                                              : reinterpret_cast<const V*>(M.dia32.data());
-  IdrVec<T> iv{n, ld, s, r, x, v, t, G, U, p0, dinv, S.scalars.data(), S.part_b.data(),
-               schwarz ? S.schwarz.cover_start.data() : nullptr,
-               schwarz ? S.schwarz.cover_entries.data() : nullptr,
-               schwarz ? S.schwarz.ybuf.data() : nullptr};
+  IdrVec<T> iv{n, ld, s, r, x, v, t, G, U, p0, S.scalars.data(), S.part_b.data(),
+               schwarz ? S.schwarz.pdiag.data() : dinv};
   const unsigned vb = vec_blocks(n);
   const unsigned sb = sell_blocks(M.chunks);
   fill_kernel<T><<<blocks_for(2 * s * ld), kBlock, 0, ctx.stream>>>(2 * s * ld, G, T(0));
@@ -1216,7 +1203,7 @@ int idr_solve(Context& ctx, PpeSolver& S, size_t n, const SellMatrix& M, const V
     for (int k = 0; k < s; ++k) {
       if (schwarz) {
         idr_a1_kernel<T><<<vb, kBlock, 0, ctx.stream>>>(iv, k);
-        schwarz_accumulate<T>(ctx, S.schwarz, v);
+        schwarz_apply<T>(ctx, S.schwarz, v);
         idr_a2_kernel<T><<<vb, kBlock, 0, ctx.stream>>>(iv, k);
       } else {
         idr_a_kernel<T><<<vb, kBlock, 0, ctx.stream>>>(iv, k);
@@ -1231,12 +1218,11 @@ int idr_solve(Context& ctx, PpeSolver& S, size_t n, const SellMatrix& M, const V
       idr_scalar_after<<<1, kReduceThreads, 0, ctx.stream>>>(S.scalars.data(), S.part_a.data(),
                                                      static_cast<int>(sb), s, k);
       iv.partials = S.part_b.data();
-      idr_b_kernel<T><<<vb, kBlock, 0, ctx.stream>>>(iv, k, k == s - 1 ? (schwarz ? 2 : 1) : 0);
+      idr_b_kernel<T><<<vb, kBlock, 0, ctx.stream>>>(iv, k, k == s - 1 ? 1 : 0);
       ++it;
     }
     if (schwarz) {
-      schwarz_accumulate<T>(ctx, S.schwarz, v);
-      precondition_kernel<T><<<vb, kBlock, 0, ctx.stream>>>(iv);
+      schwarz_apply<T>(ctx, S.schwarz, v);
     }
     SellArgs<T, V> st{n, M.chunks, M.dia_count.data(), M.dia_offsets.data(),
                       M.dia_start.data(), dia, M.chunk_start.data(), M.chunk_width.data(),
@@ -1392,7 +1378,7 @@ void ppe_solve(Context& ctx, PpeSolver& S, const lsmps_csr& A, const double* b, 
     double rr = 0.0;
     S.schwarz.blocks = 0;
     if (opt.preconditioner == LSMPS_PRECOND_SCHWARZ) {
-      schwarz_setup(ctx, S.schwarz, A, ctx.precision == LSMPS_PRECISION_MIXED);
+      schwarz_setup(ctx, S.schwarz, A, S.dinv64.data(), ctx.precision == LSMPS_PRECISION_MIXED);
       int64_t blocks = 0, rows = 0;
       schwarz_stats(ctx, S.schwarz, &blocks, &rows);
       rep.schwarz_blocks = static_cast<int32_t>(blocks);
