@@ -171,6 +171,19 @@ def copy_calibration(gib):
                       "GBps_read_plus_write": 2 * nbytes / (ms * 1e-3) / 1e9}), flush=True)
 
 
+def measured_traffic(case, precision):
+    """HBM bytes per SpMV launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    summary of this workload (scaled by a plain-copy calibration), or None."""
+    path = os.path.join(ROOT, "profiles", "spmv_dia_sell_m1_mixed_traffic.json")
+    if case != "M1" or precision != "mixed" or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    fetch = d["fetch_kib"] * 1024 * d["copy_bytes"] / (d["copy_fetch_kib"] * 1024)
+    write = d["write_kib"] * 1024 * d["copy_bytes"] / (d["copy_write_kib"] * 1024)
+    return fetch + write
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -229,7 +242,8 @@ def main():
         "stage_ms": dict(zip(["geometry", "moments", "provision", "assembly", "solve",
                               "correction", "total"], np.round(main_run["stage_ms"], 3).tolist())),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(args.case, args.precision),
                      "kernel": "PPE SpMV (IDR shadow-dot epilogue)",
                      "avg_launch_ms": avg_ms, "algorithmic_bytes_per_launch": avg_bytes,
                      "launches": main_run["spmv_launches"]},
