@@ -33,7 +33,9 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=3)
     p.add_argument("--warmup", type=int, default=1)
-    p.add_argument("--case", default="M1", help="M1 (1M dam-break), C2 (250k) or C1 (8k)")
+    p.add_argument("--case", default="M1",
+                   help="M1 (1M dam-break), C2 (250k dam-break), C1 (8k hydrostatic), "
+                        "C3 (64 batched tanks), C5 (4.1M splash)")
     p.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
     p.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
@@ -52,6 +54,10 @@ def make_case(cases, name):
         return cases.dam_break()
     if name == "C1":
         return cases.hydrostatic()
+    if name == "C3":
+        return cases.batched_tanks()
+    if name == "C5":
+        return cases.splash()
     raise SystemExit(f"unknown case {name}")
 
 
@@ -63,6 +69,9 @@ def run_gpu(pkg, rt, case, precision, steps, warmup, timing=True):
     ctx = rt.Context(precision, flags=flags)
     state = rt.DeviceState(case)
     st = rt.Stepper(ctx, state, case.config, 256, abi.default_solver_options())
+    if "fluid_offsets" in case.extra:  # batched tanks
+        st.set_segments(case.extra["fluid_offsets"], case.extra["wall_offsets"],
+                        case.extra["grids"])
     snap = state.snapshot()
     reports = []
     for _ in range(warmup):
@@ -231,7 +240,9 @@ def main():
         "data": "synthetic",
         "config": {"workload": {"M1": "dam-break 100x100x100 (1,000,000 fluid) in 200x100x200 tank",
                                 "C2": "dam-break 100x50x50 (250,000 fluid) in 200x50x100 tank",
-                                "C1": "hydrostatic 20x20x20 (8,000 fluid)"}[args.case],
+                                "C1": "hydrostatic 20x20x20 (8,000 fluid)",
+                                "C3": "64 batched tanks, 2k-60k fluid each, one launch per stage",
+                                "C5": "splash 160^3 (4,096,000 fluid), top 8 layers lifted"}[args.case],
                    "n_fluid": case.n_fluid, "n_wall": case.n_wall, "l0_m": case.config.particle_spacing,
                    "re_over_l0": 3.1, "dt_s": case.config.time_step, "solver": "IDR(8); Schwarz blocks on wrong-sign rows, Jacobi elsewhere",
                    "tolerance": 1e-10, "precision": args.precision},

@@ -450,6 +450,20 @@ int lsmps_stepper_create(lsmps_context* ctx, size_t fluid_capacity, size_t wall_
 int lsmps_stepper_destroy(lsmps_stepper* stepper);
 int lsmps_stepper_set_config(lsmps_stepper* stepper, const lsmps_config* config);
 int lsmps_stepper_set_solver(lsmps_stepper* stepper, const lsmps_solver_options* options);
+/* Batched independent tanks (the many-tanks workload; no reference counterpart:
+ * the reference steps one system per process).  The SoAs given to
+ * lsmps_stepper_step then hold the tanks back to back: tank k owns fluid
+ * particles [fluid_offsets[k], fluid_offsets[k+1]) and wall particles
+ * [wall_offsets[k], wall_offsets[k+1]), and its own cell grid grids[k]; the
+ * other config fields are shared.  Every stage runs once over all tanks: the
+ * neighbor search is segment-aware (each particle scans its tank's grid only,
+ * with the arithmetic of a single-tank run), and the PPE is one block-diagonal
+ * system solved to the tolerance relative to the whole right-hand side.  Host
+ * arrays of segment_count + 1 offsets and segment_count grids; segment_count = 0
+ * returns to one system on the config's grid. */
+int lsmps_stepper_set_segments(lsmps_stepper* stepper, int32_t segment_count,
+                               const int64_t* fluid_offsets, const int64_t* wall_offsets,
+                               const lsmps_cell_grid* grids);
 size_t lsmps_stepper_bytes(const lsmps_stepper* stepper);
 /* One step: rebuild_geometry (neighbor CSR + surface labels, generation++),
  * provision, PPE assembly, PPE solve, pressure correction (with shifting,

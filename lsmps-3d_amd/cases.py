@@ -228,10 +228,48 @@ def splash(n=160, l0=0.0025, seed=3, top_layers=8):
     return Case(f"splash_{n}", fluid, walls, cfg, {"l0": l0})
 
 
+def batched_tanks(count=64, seed=1, l0=0.005, min_fluid=2000, max_fluid=60000,
+                  amplitude=0.05):
+    """C3: `count` independent tanks, fluid counts log-uniform in [min_fluid,
+    max_fluid], block aspect ratios x:y:z = U(1,4):1:U(1,2), alternating
+    hydrostatic (block fills the open tank, no jitter) and dam-break (block in
+    the corner of a 2x-wider, 2x-taller tank, jittered) with shared l0 / re.
+
+    Draws (SplitMix64, `seed`): three uniforms per tank in tank order (size,
+    x ratio, z ratio); dam-break tank k is jittered with seed `seed` + 1 + k.
+    Every tank is generated at the origin with its own cell grid; the batched
+    case concatenates them, tank by tank.
+    """
+    u = uniform01(seed, 3 * count).reshape(count, 3)
+    tanks = []
+    for k in range(count):
+        n_target = math.exp(math.log(min_fluid) + u[k, 0] * (math.log(max_fluid) - math.log(min_fluid)))
+        ax = 1.0 + 3.0 * u[k, 1]
+        az = 1.0 + 1.0 * u[k, 2]
+        ny = max(2, int(round((n_target / (ax * az)) ** (1.0 / 3.0))))
+        nx = max(2, int(round(ax * ny)))
+        nz = max(2, int(round(az * ny)))
+        if k % 2 == 0:
+            t = block_in_tank(f"tank{k}_hydrostatic", (nx, ny, nz), (nx, ny, nz), l0)
+        else:
+            t = block_in_tank(f"tank{k}_dam_break", (nx, ny, nz), (2 * nx, ny, 2 * nz), l0,
+                              amplitude, seed + 1 + k)
+        tanks.append(t)
+    fluid = {key: np.concatenate([t.fluid[key] for t in tanks]) for key in tanks[0].fluid}
+    walls = {key: np.concatenate([t.walls[key] for t in tanks]) for key in tanks[0].walls}
+    fo = np.concatenate([[0], np.cumsum([t.n_fluid for t in tanks])]).astype(np.int64)
+    wo = np.concatenate([[0], np.cumsum([t.n_wall for t in tanks])]).astype(np.int64)
+    cfg = tanks[0].config.copy()  # shared physics; per-tank grids in `extra`
+    return Case(f"batched_{count}", fluid, walls, cfg,
+                {"fluid_offsets": fo, "wall_offsets": wo,
+                 "grids": [t.config.grid() for t in tanks], "tanks": tanks, "l0": l0})
+
+
 def by_name(name):
     table = {
         "C1": hydrostatic,
         "C2": dam_break,
+        "C3": batched_tanks,
         "M1": metric_case,
         "C4": sloshing,
         "C5": splash,

@@ -598,6 +598,16 @@ int lsmps_stepper_set_solver(lsmps_stepper* s, const lsmps_solver_options* optio
   API_END("lsmps_stepper_set_solver")
 }
 
+int lsmps_stepper_set_segments(lsmps_stepper* s, int32_t segment_count,
+                               const int64_t* fluid_offsets, const int64_t* wall_offsets,
+                               const lsmps_cell_grid* grids) {
+  API_BEGIN
+  require(s != nullptr, "null stepper");
+  s->ctx->activate();
+  s->set_segments(segment_count, fluid_offsets, wall_offsets, grids);
+  API_END("lsmps_stepper_set_segments")
+}
+
 size_t lsmps_stepper_bytes(const lsmps_stepper* s) { return s ? s->bytes() : 0; }
 
 int lsmps_stepper_step(lsmps_stepper* s, const lsmps_fluid_soa* fluid, const lsmps_wall_soa* walls,

@@ -153,17 +153,41 @@ __device__ __forceinline__ int32_t flatten(int32_t ix, int32_t iy, int32_t iz, c
   return (iz * g.ny + iy) * g.nx + ix;
 }
 
-// assign_cells_kernel (neighbor_search.cu:61-73) + per-cell counting.
+__device__ __forceinline__ GridDev seg_grid(const SegmentGrid& s) {
+  GridDev g;
+  g.ox = s.ox;
+  g.oy = s.oy;
+  g.oz = s.oz;
+  g.h = s.h;
+  g.nx = s.nx;
+  g.ny = s.ny;
+  g.nz = s.nz;
+  return g;
+}
+
+// assign_cells_kernel (neighbor_search.cu:61-73) + per-cell counting.  With
+// segments, particle i uses the grid of its tank seg[i] (same arithmetic as a
+// single-tank run, cell ids offset by the tank's base).
+template <bool kSeg>
 __global__ void assign_cells_kernel(size_t n, const double* __restrict__ x,
                                     const double* __restrict__ y, const double* __restrict__ z,
-                                    GridDev g, int32_t* __restrict__ cell_of,
+                                    GridDev g0, const SegmentGrid* __restrict__ grids,
+                                    const int32_t* __restrict__ seg,
+                                    int32_t* __restrict__ cell_of,
                                     int32_t* __restrict__ counts) {
   const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  GridDev g = g0;
+  int32_t base = 0;
+  if (kSeg) {
+    const SegmentGrid sg = grids[seg[i]];
+    g = seg_grid(sg);
+    base = sg.base;
+  }
   const int32_t ix = clamp_cell(x[i], g.ox, g.h, g.nx);
   const int32_t iy = clamp_cell(y[i], g.oy, g.h, g.ny);
   const int32_t iz = clamp_cell(z[i], g.oz, g.h, g.nz);
-  const int32_t c = flatten(ix, iy, iz, g);
+  const int32_t c = base + flatten(ix, iy, iz, g);
   cell_of[i] = c;
   atomicAdd(&counts[c], 1);
 }
@@ -223,16 +247,24 @@ __global__ void gather_points_kernel(size_t n, const int32_t* __restrict__ sorte
 // count_neighbors_kernel / write_neighbors_kernel (neighbor_search.cu:221-253) with
 // count/write_source_neighbors_for_fluid (:100-219) inlined.  Row t of the launch
 // handles fluid particle order[t] (cell order) so neighboring rows share cells.
-template <bool kWrite>
+template <bool kWrite, bool kSeg>
 __global__ void __launch_bounds__(kBlock) neighbor_scan_kernel(
     size_t n_fluid, const int32_t* __restrict__ order, const double* __restrict__ fx,
-    const double* __restrict__ fy, const double* __restrict__ fz, GridDev g,
+    const double* __restrict__ fy, const double* __restrict__ fz, GridDev g0,
+    const SegmentGrid* __restrict__ grids, const int32_t* __restrict__ fseg,
     const int32_t* __restrict__ src_begin, const int32_t* __restrict__ src_end,
     const SortedPoint* __restrict__ src, double radius_squared, int exclude_self,
     int32_t* __restrict__ offsets, int32_t* __restrict__ indices) {
   const size_t t = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (t >= n_fluid) return;
   const int32_t i = order ? order[t] : static_cast<int32_t>(t);
+  GridDev g = g0;
+  int32_t base = 0;
+  if (kSeg) {  // the tank's own grid: sources of other tanks are never visited
+    const SegmentGrid sg = grids[fseg[i]];
+    g = seg_grid(sg);
+    base = sg.base;
+  }
   const double px = fx[i];
   const double py = fy[i];
   const double pz = fz[i];
@@ -250,7 +282,7 @@ __global__ void __launch_bounds__(kBlock) neighbor_scan_kernel(
       for (int32_t dx = -1; dx <= 1; ++dx) {
         const int32_t nx = cx + dx;
         if (nx < 0 || nx >= g.nx) continue;
-        const int32_t cell = flatten(nx, ny, nz, g);
+        const int32_t cell = base + flatten(nx, ny, nz, g);
         const int32_t b = src_begin[cell];
         const int32_t e = src_end[cell];
         if (b < 0 || e < 0) continue;
@@ -347,6 +379,43 @@ void exclusive_scan_i32(Context& ctx, const int32_t* in, int32_t* out, size_t n)
   scan_level(ctx, in, out, n, ctx.scan_partials.data(), true);
 }
 
+namespace {
+// Counting sort of `count` particles into `cc` cells (one grid, or per-particle
+// segment grids when `grids` is set).
+void cell_list_impl(Context& ctx, size_t count, const double* x, const double* y,
+                    const double* z, const GridDev& g, const SegmentGrid* grids,
+                    const int32_t* seg, size_t cc, const lsmps_cell_list& cells) {
+  fill_i32(ctx.stream, cells.cell_begin, cc, -1);
+  fill_i32(ctx.stream, cells.cell_end, cc, -1);
+  if (count == 0) return;
+  require(x && y && z && cells.cell_ids && cells.sorted_indices && cells.cell_begin && cells.cell_end,
+          "Cell list requires position and cell buffers");
+  ctx.tmp_counts.reserve(cc + 1);
+  ctx.tmp_cursor.reserve(cc + 1);
+  ctx.tmp_labels.reserve(count);
+  int32_t* counts = ctx.tmp_counts.data();
+  int32_t* starts = ctx.tmp_cursor.data();
+  int32_t* cell_of = ctx.tmp_labels.data();
+  fill_i32(ctx.stream, counts, cc, 0);
+  if (grids) {
+    assign_cells_kernel<true><<<blocks_for(count), kBlock, 0, ctx.stream>>>(
+        count, x, y, z, g, grids, seg, cell_of, counts);
+  } else {
+    assign_cells_kernel<false><<<blocks_for(count), kBlock, 0, ctx.stream>>>(
+        count, x, y, z, g, nullptr, nullptr, cell_of, counts);
+  }
+  LSMPS_LAUNCH_CHECK();
+  exclusive_scan_i32(ctx, counts, starts, cc);
+  fill_i32(ctx.stream, counts, cc, 0);
+  scatter_cells_kernel<<<blocks_for(count), kBlock, 0, ctx.stream>>>(count, cell_of, starts,
+                                                                     counts, cells.sorted_indices);
+  LSMPS_LAUNCH_CHECK();
+  finalize_cells_kernel<<<blocks_for(cc), kBlock, 0, ctx.stream>>>(
+      cc, starts, cells.sorted_indices, cells.cell_ids, cells.cell_begin, cells.cell_end);
+  LSMPS_LAUNCH_CHECK();
+}
+}  // namespace
+
 void build_cell_list(Context& ctx, size_t count, const double* x, const double* y,
                      const double* z, const lsmps_cell_grid& grid, const lsmps_cell_list& cells) {
   // validation messages follow build_cell_list_impl (neighbor_search.cu:257-273)
@@ -359,31 +428,8 @@ void build_cell_list(Context& ctx, size_t count, const double* x, const double* 
           "Cell list particle capacity is smaller than particle count");
   require(count <= static_cast<size_t>(INT32_MAX), "Particle count exceeds 32-bit index capacity");
   require(cell_count < static_cast<int64_t>(INT32_MAX), "Cell count exceeds 32-bit index capacity");
-  const size_t cc = static_cast<size_t>(cell_count);
-  fill_i32(ctx.stream, cells.cell_begin, cc, -1);
-  fill_i32(ctx.stream, cells.cell_end, cc, -1);
-  if (count == 0) return;
-  require(x && y && z && cells.cell_ids && cells.sorted_indices && cells.cell_begin && cells.cell_end,
-          "Cell list requires position and cell buffers");
-  const GridDev g = to_dev(grid);
-  ctx.tmp_counts.reserve(cc + 1);
-  ctx.tmp_cursor.reserve(cc + 1);
-  ctx.tmp_labels.reserve(count);
-  int32_t* counts = ctx.tmp_counts.data();
-  int32_t* starts = ctx.tmp_cursor.data();
-  int32_t* cell_of = ctx.tmp_labels.data();
-  fill_i32(ctx.stream, counts, cc, 0);
-  assign_cells_kernel<<<blocks_for(count), kBlock, 0, ctx.stream>>>(count, x, y, z, g, cell_of,
-                                                                    counts);
-  LSMPS_LAUNCH_CHECK();
-  exclusive_scan_i32(ctx, counts, starts, cc);
-  fill_i32(ctx.stream, counts, cc, 0);
-  scatter_cells_kernel<<<blocks_for(count), kBlock, 0, ctx.stream>>>(count, cell_of, starts,
-                                                                     counts, cells.sorted_indices);
-  LSMPS_LAUNCH_CHECK();
-  finalize_cells_kernel<<<blocks_for(cc), kBlock, 0, ctx.stream>>>(
-      cc, starts, cells.sorted_indices, cells.cell_ids, cells.cell_begin, cells.cell_end);
-  LSMPS_LAUNCH_CHECK();
+  cell_list_impl(ctx, count, x, y, z, to_dev(grid), nullptr, nullptr,
+                 static_cast<size_t>(cell_count), cells);
 }
 
 namespace {
@@ -391,7 +437,8 @@ namespace {
 void one_neighbor_list(Context& ctx, const lsmps_fluid_soa& fluid, size_t source_count,
                        const double* sx, const double* sy, const double* sz,
                        const lsmps_cell_list& source_cells, const lsmps_cell_list& fluid_cells,
-                       const lsmps_cell_grid& grid, double radius, bool exclude_self,
+                       const lsmps_cell_grid& grid, const SegmentGrid* grids,
+                       const int32_t* fluid_seg, double radius, bool exclude_self,
                        const lsmps_neighbor_list& nbrs, int32_t* device_total) {
   require(fluid.count <= nbrs.particle_count,
           "Neighbor list particle capacity is smaller than fluid count");
@@ -410,9 +457,17 @@ void one_neighbor_list(Context& ctx, const lsmps_fluid_soa& fluid, size_t source
     LSMPS_LAUNCH_CHECK();
   }
   const int32_t* order = fluid_cells.sorted_indices;
-  neighbor_scan_kernel<false><<<blocks_for(fluid.count), kBlock, 0, ctx.stream>>>(
-      fluid.count, order, fluid.x, fluid.y, fluid.z, g, source_cells.cell_begin,
-      source_cells.cell_end, points, radius_squared, exclude_self ? 1 : 0, nbrs.offsets, nullptr);
+  if (grids) {
+    neighbor_scan_kernel<false, true><<<blocks_for(fluid.count), kBlock, 0, ctx.stream>>>(
+        fluid.count, order, fluid.x, fluid.y, fluid.z, g, grids, fluid_seg,
+        source_cells.cell_begin, source_cells.cell_end, points, radius_squared,
+        exclude_self ? 1 : 0, nbrs.offsets, nullptr);
+  } else {
+    neighbor_scan_kernel<false, false><<<blocks_for(fluid.count), kBlock, 0, ctx.stream>>>(
+        fluid.count, order, fluid.x, fluid.y, fluid.z, g, nullptr, nullptr,
+        source_cells.cell_begin, source_cells.cell_end, points, radius_squared,
+        exclude_self ? 1 : 0, nbrs.offsets, nullptr);
+  }
   LSMPS_LAUNCH_CHECK();
   exclusive_scan_i32(ctx, nbrs.offsets, nbrs.offsets, fluid.count);
   // Capacity check (require_neighbor_capacity, neighbor_search.cu:22-30,323-328).
@@ -426,10 +481,17 @@ void one_neighbor_list(Context& ctx, const lsmps_fluid_soa& fluid, size_t source
                                   std::to_string(required) + ", capacity " +
                                   std::to_string(nbrs.neighbor_count));
   require(required == 0 || nbrs.indices != nullptr, "Neighbor list requires an indices buffer");
-  neighbor_scan_kernel<true><<<blocks_for(fluid.count), kBlock, 0, ctx.stream>>>(
-      fluid.count, order, fluid.x, fluid.y, fluid.z, g, source_cells.cell_begin,
-      source_cells.cell_end, points, radius_squared, exclude_self ? 1 : 0, nbrs.offsets,
-      nbrs.indices);
+  if (grids) {
+    neighbor_scan_kernel<true, true><<<blocks_for(fluid.count), kBlock, 0, ctx.stream>>>(
+        fluid.count, order, fluid.x, fluid.y, fluid.z, g, grids, fluid_seg,
+        source_cells.cell_begin, source_cells.cell_end, points, radius_squared,
+        exclude_self ? 1 : 0, nbrs.offsets, nbrs.indices);
+  } else {
+    neighbor_scan_kernel<true, false><<<blocks_for(fluid.count), kBlock, 0, ctx.stream>>>(
+        fluid.count, order, fluid.x, fluid.y, fluid.z, g, nullptr, nullptr,
+        source_cells.cell_begin, source_cells.cell_end, points, radius_squared,
+        exclude_self ? 1 : 0, nbrs.offsets, nbrs.indices);
+  }
   LSMPS_LAUNCH_CHECK();
 }
 }  // namespace
@@ -452,9 +514,41 @@ void build_neighbor_lists(Context& ctx, const lsmps_fluid_soa& fluid, const lsmp
   build_cell_list(ctx, fluid.count, fluid.x, fluid.y, fluid.z, grid, fluid_cells);
   build_cell_list(ctx, walls.count, walls.x, walls.y, walls.z, grid, wall_cells);
   one_neighbor_list(ctx, fluid, fluid.count, fluid.x, fluid.y, fluid.z, fluid_cells, fluid_cells,
-                    grid, radius, true, fluid_neighbors, fluid_total);
+                    grid, nullptr, nullptr, radius, true, fluid_neighbors, fluid_total);
   one_neighbor_list(ctx, fluid, walls.count, walls.x, walls.y, walls.z, wall_cells, fluid_cells,
-                    grid, radius, false, wall_neighbors, wall_total);
+                    grid, nullptr, nullptr, radius, false, wall_neighbors, wall_total);
+}
+
+void build_neighbor_lists_segmented(Context& ctx, const lsmps_fluid_soa& fluid,
+                                    const lsmps_wall_soa& walls, const lsmps_config& config,
+                                    const Segments& segments, const lsmps_cell_list& fluid_cells,
+                                    const lsmps_cell_list& wall_cells,
+                                    const lsmps_neighbor_list& fluid_neighbors,
+                                    const lsmps_neighbor_list& wall_neighbors) {
+  const double radius = config.support_radius;
+  require(radius > 0.0, "Neighbor search radius must be positive");
+  require(segments.count > 0 && segments.grids && segments.fluid_seg &&
+              (walls.count == 0 || segments.wall_seg),
+          "Batched neighbor search requires segment grids and particle segment ids");
+  require(segments.cells == fluid_cells.cell_count && segments.cells == wall_cells.cell_count,
+          "Cell list capacity does not match the segments' cell count");
+  require(segments.cells < static_cast<size_t>(INT32_MAX), "Cell count exceeds 32-bit index capacity");
+  require(fluid.count <= fluid_cells.particle_count && walls.count <= wall_cells.particle_count,
+          "Cell list particle capacity is smaller than particle count");
+  const GridDev none{};
+  cell_list_impl(ctx, fluid.count, fluid.x, fluid.y, fluid.z, none, segments.grids,
+                 segments.fluid_seg, segments.cells, fluid_cells);
+  cell_list_impl(ctx, walls.count, walls.x, walls.y, walls.z, none, segments.grids,
+                 segments.wall_seg, segments.cells, wall_cells);
+  lsmps_cell_grid unused{};
+  unused.cell_size = 1.0;
+  unused.dims[0] = unused.dims[1] = unused.dims[2] = 1;
+  one_neighbor_list(ctx, fluid, fluid.count, fluid.x, fluid.y, fluid.z, fluid_cells, fluid_cells,
+                    unused, segments.grids, segments.fluid_seg, radius, true, fluid_neighbors,
+                    nullptr);
+  one_neighbor_list(ctx, fluid, walls.count, walls.x, walls.y, walls.z, wall_cells, fluid_cells,
+                    unused, segments.grids, segments.fluid_seg, radius, false, wall_neighbors,
+                    nullptr);
 }
 
 }  // namespace lsmps

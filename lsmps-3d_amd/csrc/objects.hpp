@@ -1,6 +1,8 @@
 // Opaque objects behind the C-ABI: the moment-matrix cache and the fused stepper.
 #pragma once
 
+#include <vector>
+
 #include "common.hpp"
 #include "solver.hpp"
 #include "stages.hpp"
@@ -82,6 +84,12 @@ struct lsmps_stepper {
   lsmps::DeviceArray<double> ppe_vals, ppe_vec;  // ppe_vec: rhs, pressure, divergence, wall term
   lsmps::PpeSolver solver;
   lsmps::DeviceArray<double> corr;  // grad, ps, smoothed: 9 x fluid capacity
+  // batched tanks (segment_count > 0): per-tank grids and particle tank ids
+  int segment_count = 0;
+  size_t segment_cells = 0;
+  std::vector<int64_t> fluid_offsets, wall_offsets;
+  lsmps::DeviceArray<lsmps::SegmentGrid> segment_grids;
+  lsmps::DeviceArray<int32_t> fluid_segment, wall_segment;
   hipEvent_t events[8] = {};
   int n_events = 0;
   double stage_ms[7] = {0, 0, 0, 0, 0, 0, 0};
@@ -89,6 +97,8 @@ struct lsmps_stepper {
 
   ~lsmps_stepper();
   void allocate(size_t nf, size_t nw, size_t cells);
+  void set_segments(int count, const int64_t* fluid_offsets, const int64_t* wall_offsets,
+                    const lsmps_cell_grid* grids);
   size_t bytes() const;
   void step(const lsmps_fluid_soa& fluid, const lsmps_wall_soa& walls);
   lsmps_neighbor_list fluid_neighbors() const;

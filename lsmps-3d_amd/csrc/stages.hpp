@@ -18,6 +18,27 @@ void build_neighbor_lists(Context& ctx, const lsmps_fluid_soa& fluid, const lsmp
                           const lsmps_neighbor_list& wall_neighbors, int32_t* fluid_total,
                           int32_t* wall_total);
 
+// Batched independent tanks: tank k's cells occupy [base, base + nx ny nz) of a
+// combined cell numbering; fluid_seg / wall_seg give each particle's tank.  All
+// arrays are device arrays; neighbors never cross tanks.
+struct SegmentGrid {
+  double ox, oy, oz, h;
+  int32_t nx, ny, nz, base;
+};
+struct Segments {
+  int count = 0;
+  const SegmentGrid* grids = nullptr;
+  const int32_t* fluid_seg = nullptr;
+  const int32_t* wall_seg = nullptr;
+  size_t cells = 0;
+};
+void build_neighbor_lists_segmented(Context& ctx, const lsmps_fluid_soa& fluid,
+                                    const lsmps_wall_soa& walls, const lsmps_config& config,
+                                    const Segments& segments, const lsmps_cell_list& fluid_cells,
+                                    const lsmps_cell_list& wall_cells,
+                                    const lsmps_neighbor_list& fluid_neighbors,
+                                    const lsmps_neighbor_list& wall_neighbors);
+
 // ---- surface.hip
 double reference_number_density(double l0, double re, int kernel);
 void classify_surface(Context& ctx, const lsmps_fluid_soa& fluid, const lsmps_wall_soa& walls,

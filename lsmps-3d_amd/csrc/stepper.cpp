@@ -142,11 +142,65 @@ lsmps_ppe_workspace lsmps_stepper::ppe_workspace(size_t n) const {
   return ws;
 }
 
+void lsmps_stepper::set_segments(int count, const int64_t* fo, const int64_t* wo,
+                                 const lsmps_cell_grid* grids) {
+  if (count <= 0) {
+    segment_count = 0;
+    segment_cells = 0;
+    fluid_offsets.clear();
+    wall_offsets.clear();
+    return;
+  }
+  require(fo && wo && grids, "segments require fluid offsets, wall offsets and grids");
+  require(fo[0] == 0 && wo[0] == 0, "segment offsets must start at 0");
+  std::vector<SegmentGrid> host(count);
+  int64_t base = 0;
+  for (int k = 0; k < count; ++k) {
+    require(fo[k + 1] >= fo[k] && wo[k + 1] >= wo[k], "segment offsets must be non-decreasing");
+    const lsmps_cell_grid& g = grids[k];
+    require(g.cell_size > 0.0 && g.dims[0] > 0 && g.dims[1] > 0 && g.dims[2] > 0,
+            "Invalid cell grid configuration");
+    require(config.support_radius <= g.cell_size,
+            "Neighbor search currently requires radius <= cell_size");
+    host[k] = SegmentGrid{g.origin[0], g.origin[1], g.origin[2], g.cell_size,
+                          g.dims[0],   g.dims[1],   g.dims[2],   static_cast<int32_t>(base)};
+    base += static_cast<int64_t>(g.dims[0]) * g.dims[1] * g.dims[2];
+    require(base < INT32_MAX, "Cell count exceeds 32-bit index capacity");
+  }
+  const size_t nf = static_cast<size_t>(fo[count]), nw = static_cast<size_t>(wo[count]);
+  std::vector<int32_t> fseg(std::max<size_t>(nf, 1)), wseg(std::max<size_t>(nw, 1));
+  for (int k = 0; k < count; ++k) {
+    for (int64_t i = fo[k]; i < fo[k + 1]; ++i) fseg[i] = k;
+    for (int64_t i = wo[k]; i < wo[k + 1]; ++i) wseg[i] = k;
+  }
+  Context& c = *ctx;
+  segment_grids.reserve(count);
+  fluid_segment.reserve(fseg.size());
+  wall_segment.reserve(wseg.size());
+  LSMPS_HIP(hipMemcpyAsync(segment_grids.data(), host.data(), count * sizeof(SegmentGrid),
+                           hipMemcpyHostToDevice, c.stream));
+  LSMPS_HIP(hipMemcpyAsync(fluid_segment.data(), fseg.data(), fseg.size() * sizeof(int32_t),
+                           hipMemcpyHostToDevice, c.stream));
+  LSMPS_HIP(hipMemcpyAsync(wall_segment.data(), wseg.data(), wseg.size() * sizeof(int32_t),
+                           hipMemcpyHostToDevice, c.stream));
+  c.sync();  // the host vectors go out of scope
+  fluid_offsets.assign(fo, fo + count + 1);
+  wall_offsets.assign(wo, wo + count + 1);
+  segment_count = count;
+  segment_cells = static_cast<size_t>(base);
+}
+
 void lsmps_stepper::step(const lsmps_fluid_soa& fluid, const lsmps_wall_soa& walls) {
   Context& c = *ctx;
   const size_t nf = fluid.count, nw = walls.count;
-  const size_t cells = static_cast<size_t>(config.cell_dims[0]) * config.cell_dims[1] *
-                       config.cell_dims[2];
+  if (segment_count > 0)
+    require(static_cast<int64_t>(nf) == fluid_offsets.back() &&
+                static_cast<int64_t>(nw) == wall_offsets.back(),
+            "particle counts do not match the stepper's segments");
+  const size_t cells = segment_count > 0
+                           ? segment_cells
+                           : static_cast<size_t>(config.cell_dims[0]) * config.cell_dims[1] *
+                                 config.cell_dims[2];
   if (nf > fluid_capacity || nw > wall_capacity || cells > cell_capacity) allocate(nf, nw, cells);
   if (n_events == 0) {
     for (int k = 0; k < 8; ++k) LSMPS_HIP(hipEventCreate(&events[k]));
@@ -161,7 +215,17 @@ void lsmps_stepper::step(const lsmps_fluid_soa& fluid, const lsmps_wall_soa& wal
   const lsmps_neighbor_list wn = wall_neighbors();
   LSMPS_HIP(hipEventRecord(events[0], c.stream));
   // 1. geometry: rebuild_geometry (simulation_driver.cu:252-264)
-  build_neighbor_lists(c, fluid, walls, config, fc, wc, fn, wn, nullptr, nullptr);
+  if (segment_count > 0) {
+    Segments seg;
+    seg.count = segment_count;
+    seg.grids = segment_grids.data();
+    seg.fluid_seg = fluid_segment.data();
+    seg.wall_seg = wall_segment.data();
+    seg.cells = segment_cells;
+    build_neighbor_lists_segmented(c, fluid, walls, config, seg, fc, wc, fn, wn);
+  } else {
+    build_neighbor_lists(c, fluid, walls, config, fc, wc, fn, wn, nullptr, nullptr);
+  }
   ++generation;
   classify_surface(c, fluid, walls, fn, wn, config, nullptr);
   LSMPS_HIP(hipEventRecord(events[1], c.stream));

@@ -30,6 +30,7 @@ EXPORTS = [
     "lsmps_stepper_step", "lsmps_stepper_stage_times", "lsmps_stepper_neighbor_lists",
     "lsmps_stepper_ppe_workspace", "lsmps_stepper_temporary_velocity",
     "lsmps_stepper_correction_vectors", "lsmps_stepper_moments", "lsmps_context_kernel_stats",
+    "lsmps_stepper_set_segments",
 ]
 
 _LIB = None
@@ -182,6 +183,16 @@ class Stepper:
         if solver is not None:
             self.set_solver(solver)
 
+    def set_segments(self, fluid_offsets, wall_offsets, grids):
+        """Batched tanks: offsets (len T+1) into the fluid / wall SoAs and T cell grids."""
+        fo = np.ascontiguousarray(fluid_offsets, np.int64)
+        wo = np.ascontiguousarray(wall_offsets, np.int64)
+        t = fo.shape[0] - 1
+        arr = (abi.CellGrid * max(t, 1))(*grids) if t > 0 else None
+        check(self.lib.lsmps_stepper_set_segments(
+            self.handle, C.c_int32(t), fo.ctypes.data_as(C.c_void_p),
+            wo.ctypes.data_as(C.c_void_p), arr), "set_segments")
+
     def set_solver(self, options):
         check(self.lib.lsmps_stepper_set_solver(self.handle, C.byref(options)), "set_solver")
 
@@ -308,6 +319,9 @@ def run_step_on_device(case, precision=abi.PRECISION_FP64, solver=None, max_neig
     ctx = Context(precision)
     state = DeviceState(case, device)
     st = Stepper(ctx, state, case.config, max_neighbors, solver or abi.default_solver_options())
+    if "fluid_offsets" in case.extra:  # batched tanks
+        st.set_segments(case.extra["fluid_offsets"], case.extra["wall_offsets"],
+                        case.extra["grids"])
     report = st.step(raise_on_nonconvergence=False)
     f_off, f_idx, w_off, w_idx = st.neighbor_lists()
     ppe = st.ppe()

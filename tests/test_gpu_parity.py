@@ -159,3 +159,29 @@ def test_schwarz_matches_jacobi(rt, abi, cases, precision):
     # deterministic: a second Schwarz run reproduces the pressure bit for bit
     gs2 = rt.run_step_on_device(case, precision=prec, solver=sch)
     assert np.array_equal(gs2["pressure"], gs["pressure"])
+
+
+@pytest.mark.parametrize("precision", ["FP64", "MIXED"])
+def test_batched_tanks_match_single_tanks(rt, abi, cases, oracle, precision):
+    """Four independent tanks (hydrostatic and dam-break, uneven sizes) stepped
+    as one batched system: every tank's neighbor rows and labels equal its own
+    single-tank oracle run bit for bit (indices offset by the tank's start), and
+    its pressure agrees to the tolerance of the union solve."""
+    case = cases.batched_tanks(count=4, min_fluid=2000, max_fluid=8000)
+    g = rt.run_step_on_device(case, precision=getattr(abi, "PRECISION_" + precision))
+    print(precision, case.n_fluid, g["report"], flush=True)
+    assert g["report"]["converged"] == 1
+    fo, wo = case.extra["fluid_offsets"], case.extra["wall_offsets"]
+    tol = 1e-7 if precision == "FP64" else 1e-5
+    for k, tank in enumerate(case.extra["tanks"]):
+        ref = oracle.pipeline(tank)
+        nb = ref["neighbors"]
+        a, b = fo[k], fo[k + 1]
+        f_off = g["fluid_offsets"][a:b + 1]
+        w_off = g["wall_offsets"][a:b + 1]
+        assert np.array_equal(f_off - f_off[0], nb["fluid_offsets"]), k
+        assert np.array_equal(w_off - w_off[0], nb["wall_offsets"]), k
+        assert np.array_equal(g["fluid_indices"][f_off[0]:f_off[-1]] - a, nb["fluid_indices"]), k
+        assert np.array_equal(g["wall_indices"][w_off[0]:w_off[-1]] - wo[k], nb["wall_indices"]), k
+        assert np.array_equal(g["surface_type"][a:b], ref["surface"]["surface_type"]), k
+        assert rel_l2(g["pressure"][a:b], ref["pressure"]) <= tol, k
