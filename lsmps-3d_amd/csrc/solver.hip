@@ -41,6 +41,7 @@ namespace lsmps {
 
 PpeSolver::~PpeSolver() {
   if (host) (void)hipHostFree(host);
+  if (seg_host) (void)hipHostFree(seg_host);
 }
 
 namespace {
@@ -742,15 +743,23 @@ struct IdrVec {
   // Diagonal part of the preconditioner: 1 / A_ii, except 1 on the rows the
   // Schwarz blocks cover, whose values the block application then replaces.
   const double* pdiag;
+  const int32_t* row_seg;  // batched systems: each row's segment (scalars sc[row_seg[i]])
 };
+
+// Scalars that govern row i: the system's, or its segment's in a batched solve.
+template <bool kSeg, typename T>
+__device__ __forceinline__ const SolverScalars* row_sc(const IdrVec<T>& a, size_t i) {
+  if (kSeg) return a.sc + a.row_seg[i];
+  return a.sc;
+}
 
 // Schwarz path, first half of idr_a: v = pdiag (r - sum_{q>=k} c_q G_q): Jacobi
 // on uncovered rows, the raw value (the block application's input) on covered ones.
-template <typename T>
+template <typename T, bool kSeg = false>
 __global__ void __launch_bounds__(kBlock) idr_a1_kernel(IdrVec<T> a, int k) {
-  const SolverScalars* sc = a.sc;
   for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < a.n;
        i += static_cast<size_t>(gridDim.x) * blockDim.x) {
+    const SolverScalars* sc = row_sc<kSeg>(a, i);
     double v = a.r[i];
     for (int q = k; q < a.s; ++q) v -= sc->c[q] * static_cast<double>(a.G[q * a.ld + i]);
     a.v[i] = static_cast<T>(a.pdiag[i] * v);
@@ -758,19 +767,23 @@ __global__ void __launch_bounds__(kBlock) idr_a1_kernel(IdrVec<T> a, int k) {
 }
 
 // Schwarz path, second half (v already preconditioned): U_k = sum_{q>=k} c_q U_q + om v.
-template <typename T>
+template <typename T, bool kSeg = false>
 __global__ void __launch_bounds__(kBlock) idr_a2_kernel(IdrVec<T> a, int k) {
-  const SolverScalars* sc = a.sc;
-  const double om = sc->om;
   for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < a.n;
        i += static_cast<size_t>(gridDim.x) * blockDim.x) {
+    const SolverScalars* sc = row_sc<kSeg>(a, i);
+    const double om = sc->om;
     double u = 0.0;
     for (int q = k; q < a.s; ++q) u += sc->c[q] * static_cast<double>(a.U[q * a.ld + i]);
     a.U[k * a.ld + i] = static_cast<T>(u + om * static_cast<double>(a.v[i]));
   }
 }
 
-__global__ void idr_init_scalars(SolverScalars* sc, int s) {
+// One block per system (segment); the grid is 1 for a single system.  `only`
+// (optional, per segment) restricts the reset to the flagged segments.
+__global__ void idr_init_scalars(SolverScalars* sc, int s, const int32_t* only = nullptr) {
+  if (only && !only[blockIdx.x]) return;
+  sc += blockIdx.x;
   const int t = threadIdx.x;
   for (int e = t; e < kMaxShadow * kMaxShadow; e += blockDim.x) {
     const int i = e / kMaxShadow, k = e % kMaxShadow;
@@ -780,7 +793,19 @@ __global__ void idr_init_scalars(SolverScalars* sc, int s) {
     sc->om = 1.0;
     sc->breakdown = 0;
     sc->beta = 0.0;
+    if (!only) sc->done = 0;
   }
+}
+
+// A frozen (converged) segment keeps every coefficient at zero, so the vector
+// kernels leave its rows unchanged.
+__device__ void freeze(SolverScalars* sc) {
+  for (int q = 0; q < kMaxShadow; ++q) {
+    sc->c[q] = 0.0;
+    sc->alpha[q] = 0.0;
+  }
+  sc->beta = 0.0;
+  sc->om = 0.0;
 }
 
 // Lower-triangular solve M[k:, k:] c[k:] = f[k:] (single thread).
@@ -797,9 +822,15 @@ __device__ void solve_c(SolverScalars* sc, int s, int k) {
 // Cycle start: f = P^T r (from partials), c = M^-1 f.
 __global__ void __launch_bounds__(kReduceThreads)
     idr_scalar_f(SolverScalars* sc, const double* partials, int nb, int s) {
+  sc += blockIdx.x;
+  partials += static_cast<size_t>(blockIdx.x) * nb * kPartialStride;
   __shared__ double d[kPartialStride];
   reduce_partials(partials, nb, d);
   if (threadIdx.x != 0) return;
+  if (sc->done) {
+    freeze(sc);
+    return;
+  }
   for (int q = 0; q < s; ++q) sc->f[q] = d[q];
   solve_c(sc, s, 0);
 }
@@ -808,9 +839,15 @@ __global__ void __launch_bounds__(kReduceThreads)
 // beta, f update and c for the next k.
 __global__ void __launch_bounds__(kReduceThreads)
     idr_scalar_after(SolverScalars* sc, const double* partials, int nb, int s, int k) {
+  sc += blockIdx.x;
+  partials += static_cast<size_t>(blockIdx.x) * nb * kPartialStride;
   __shared__ double d[kPartialStride];
   reduce_partials(partials, nb, d);
   if (threadIdx.x != 0) return;
+  if (sc->done) {
+    freeze(sc);
+    return;
+  }
   double alpha[kMaxShadow];
   for (int i = 0; i < k; ++i) {
     double t = d[i];
@@ -838,13 +875,20 @@ __global__ void __launch_bounds__(kReduceThreads)
 // omega = (t.r)/(t.t) with the 0.7 "maintaining the convergence" safeguard.
 __global__ void __launch_bounds__(kReduceThreads)
     idr_scalar_omega(SolverScalars* sc, const double* part_t, const double* part_r, int nb_t,
-                     int nb_r) {
+                     int nb_r, int rr_index = 0) {
+  sc += blockIdx.x;
+  part_t += static_cast<size_t>(blockIdx.x) * nb_t * kPartialStride;
+  part_r += static_cast<size_t>(blockIdx.x) * nb_r * kPartialStride;
   __shared__ double dt[kPartialStride];
   __shared__ double dr[kPartialStride];
   reduce_partials(part_t, nb_t, dt);
   reduce_partials(part_r, nb_r, dr);
   if (threadIdx.x != 0) return;
-  const double tr = dt[0], tt = dt[1], rr = dr[0];
+  if (sc->done) {
+    freeze(sc);
+    return;
+  }
+  const double tr = dt[0], tt = dt[1], rr = dr[rr_index];
   if (!(tt > 0.0) || !(rr > 0.0)) {
     sc->om = 0.0;
     if (!(rr > 0.0)) return;  // already converged exactly
@@ -859,18 +903,20 @@ __global__ void __launch_bounds__(kReduceThreads)
 
 __global__ void __launch_bounds__(kReduceThreads)
     scalar_norm(SolverScalars* sc, const double* partials, int nb) {
+  sc += blockIdx.x;
+  partials += static_cast<size_t>(blockIdx.x) * nb * kPartialStride;
   __shared__ double d[kPartialStride];
   reduce_partials(partials, nb, d);
   if (threadIdx.x == 0) sc->rr = d[0];
 }
 
 // v = D^-1 (r - sum_{q>=k} c_q G_q);  U_k = sum_{q>=k} c_q U_q + om v
-template <typename T>
+template <typename T, bool kSeg = false>
 __global__ void __launch_bounds__(kBlock) idr_a_kernel(IdrVec<T> a, int k) {
-  const SolverScalars* sc = a.sc;
-  const double om = sc->om;
   for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < a.n;
        i += static_cast<size_t>(gridDim.x) * blockDim.x) {
+    const SolverScalars* sc = row_sc<kSeg>(a, i);
+    const double om = sc->om;
     double v = a.r[i];
     double u = 0.0;
     for (int q = k; q < a.s; ++q) {
@@ -885,13 +931,13 @@ __global__ void __launch_bounds__(kBlock) idr_a_kernel(IdrVec<T> a, int k) {
 
 // G_k, U_k biorthogonalised; r -= beta G_k; x += beta U_k; partial r.r;
 // optionally v = D^-1 r for the dimension-reduction step.
-template <typename T>
+template <typename T, bool kSeg = false>
 __global__ void __launch_bounds__(kBlock) idr_b_kernel(IdrVec<T> a, int k, int write_v) {
-  const SolverScalars* sc = a.sc;
-  const double beta = sc->beta;
   double acc[1] = {0.0};
   for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < a.n;
        i += static_cast<size_t>(gridDim.x) * blockDim.x) {
+    const SolverScalars* sc = row_sc<kSeg>(a, i);
+    const double beta = sc->beta;
     double g = static_cast<double>(a.G[k * a.ld + i]);
     double u = static_cast<double>(a.U[k * a.ld + i]);
     for (int q = 0; q < k; ++q) {
@@ -912,12 +958,12 @@ __global__ void __launch_bounds__(kBlock) idr_b_kernel(IdrVec<T> a, int k, int w
 }
 
 // x += om v; r -= om t; partial r.r
-template <typename T>
+template <typename T, bool kSeg = false>
 __global__ void __launch_bounds__(kBlock) idr_c_kernel(IdrVec<T> a) {
-  const double om = a.sc->om;
   double acc[1] = {0.0};
   for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < a.n;
        i += static_cast<size_t>(gridDim.x) * blockDim.x) {
+    const double om = row_sc<kSeg>(a, i)->om;
     a.x[i] += om * static_cast<double>(a.v[i]);
     const double r = a.r[i] - om * static_cast<double>(a.t[i]);
     a.r[i] = r;
@@ -940,6 +986,72 @@ __global__ void __launch_bounds__(kBlock) shadow_dots_kernel(IdrVec<T> a) {
       if (q < a.s) acc[q] += shadow(a.p0, i, q) * r;
   }
   block_reduce_store<kMaxShadow>(acc, a.s, a.partials);
+}
+
+// ---- batched systems: per-segment reductions and bookkeeping -------------
+
+constexpr int kSegBlocks = 16;  // partial records per segment
+
+// Per-segment dot products, kSegBlocks blocks per segment, fixed order.
+// MODE 1: P_q . y (shadow dots of an SpMV result); 2: t.r, t.t, r.r (y = t);
+// 3: r.r; 4: P_q . r.
+template <typename T, int MODE>
+__global__ void __launch_bounds__(kBlock) seg_reduce_kernel(const int32_t* start, int s,
+                                                            const T* p0, const T* y,
+                                                            const double* r,
+                                                            double* partials) {
+  constexpr int NQ = (MODE == 1 || MODE == 4) ? kMaxShadow : (MODE == 2 ? 3 : 1);
+  const int seg = blockIdx.x / kSegBlocks;
+  const int part = blockIdx.x % kSegBlocks;
+  const int64_t b = start[seg], e = start[seg + 1];
+  double acc[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) acc[q] = 0.0;
+  for (int64_t i = b + static_cast<int64_t>(part) * blockDim.x + threadIdx.x; i < e;
+       i += static_cast<int64_t>(kSegBlocks) * blockDim.x) {
+    if (MODE == 1 || MODE == 4) {
+      const double w = MODE == 1 ? static_cast<double>(y[i]) : r[i];
+#pragma unroll
+      for (int q = 0; q < NQ; ++q)
+        if (q < s) acc[q] += shadow(p0, static_cast<size_t>(i), q) * w;
+    } else if (MODE == 2) {
+      const double t = static_cast<double>(y[i]);
+      acc[0] += t * r[i];
+      acc[1 % NQ] += t * t;
+      acc[2 % NQ] += r[i] * r[i];
+    } else {
+      acc[0] += r[i] * r[i];
+    }
+  }
+  block_reduce_store<NQ>(acc, (MODE == 1 || MODE == 4) ? s : NQ, partials);
+}
+
+// Per segment: rr and the breakdown flag, packed for one device-to-host copy.
+__global__ void seg_status_kernel(const SolverScalars* sc, int count, double* out) {
+  const int k = static_cast<int>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k >= count) return;
+  out[2 * k] = sc[k].rr;
+  out[2 * k + 1] = static_cast<double>(sc[k].breakdown);
+}
+
+// Marks segments converged (flag 1) or, for flag 2, due for a restart.
+__global__ void seg_flags_kernel(SolverScalars* sc, int count, const int32_t* flags) {
+  const int k = static_cast<int>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k >= count) return;
+  if (flags[k] == 1) sc[k].done = 1;
+}
+
+// Restart of the flagged segments' Krylov state: G = U = 0 and P_0 = r on their rows.
+template <typename T>
+__global__ void seg_restart_kernel(size_t n, const int32_t* row_seg, const int32_t* flags,
+                                   size_t ld, int s, const double* r, T* p0, T* G, T* U) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n || flags[row_seg[i]] != 2) return;
+  p0[i] = static_cast<T>(r[i]);
+  for (int q = 0; q < s; ++q) {
+    G[q * ld + i] = T(0);
+    U[q * ld + i] = T(0);
+  }
 }
 
 // ---- PCG ----------------------------------------------------------------
@@ -1162,7 +1274,7 @@ int idr_solve(Context& ctx, PpeSolver& S, size_t n, const SellMatrix& M, const V
   const V* dia = sizeof(V) == sizeof(double) ? reinterpret_cast<const V*>(M.dia64.data())
                                              : reinterpret_cast<const V*>(M.dia32.data());
   IdrVec<T> iv{n, ld, s, r, x, v, t, G, U, p0, S.scalars.data(), S.part_b.data(),
-               schwarz ? S.schwarz.pdiag.data() : dinv};
+               schwarz ? S.schwarz.pdiag.data() : dinv, nullptr};
   const unsigned vb = vec_blocks(n);
   const unsigned sb = sell_blocks(M.chunks);
   fill_kernel<T><<<blocks_for(2 * s * ld), kBlock, 0, ctx.stream>>>(2 * s * ld, G, T(0));
@@ -1276,6 +1388,166 @@ int idr_solve(Context& ctx, PpeSolver& S, size_t n, const SellMatrix& M, const V
   return rr <= target2 ? 1 : 0;
 }
 
+// Batched IDR(s): every segment (independent system) runs its own recurrences
+// and convergence test through shared launches; converged segments freeze.
+// Per-segment dot products come from seg_reduce_kernel (the SpMVs run without
+// fused dots); reliable updates recompute the true residual of all rows at once.
+template <typename T, typename V>
+void idr_solve_segmented(Context& ctx, PpeSolver& S, size_t n, const SellMatrix& M,
+                         const V* vals, const double* dinv, double* x, double* r, T* work,
+                         size_t ld, int s, double tolerance, int max_iterations,
+                         const lsmps_csr& A64, const double* b64, double replace_factor,
+                         lsmps_solve_report& rep) {
+  const SolverSegments& g = S.segments;
+  const int nseg = g.count;
+  T* v = work;
+  T* t = work + ld;
+  T* p0 = work + 2 * ld;
+  T* G = work + 3 * ld;
+  T* U = work + (3 + s) * ld;
+  const bool schwarz = S.schwarz.blocks > 0 && S.schwarz.n == n;
+  const V* dia = sizeof(V) == sizeof(double) ? reinterpret_cast<const V*>(M.dia64.data())
+                                             : reinterpret_cast<const V*>(M.dia32.data());
+  IdrVec<T> iv{n, ld, s, r, x, v, t, G, U, p0, S.scalars.data(), S.part_b.data(),
+               schwarz ? S.schwarz.pdiag.data() : dinv, g.row_seg};
+  const unsigned vb = vec_blocks(n);
+  const unsigned sb = sell_blocks(M.chunks);
+  const unsigned rb = static_cast<unsigned>(nseg * kSegBlocks);
+  const unsigned fb = static_cast<unsigned>((nseg + kBlock - 1) / kBlock);
+  double* part = S.part_a.data();
+  S.seg_status.reserve(2 * static_cast<size_t>(nseg));
+  std::vector<int32_t> flags(nseg, 0);
+  DeviceArray<int32_t>& dflags = S.seg_flags;
+  dflags.reserve(nseg);
+  auto read_status = [&](std::vector<double>& rr, std::vector<int>& brk) {
+    seg_status_kernel<<<fb, kBlock, 0, ctx.stream>>>(S.scalars.data(), nseg, S.seg_status.data());
+    LSMPS_HIP(hipMemcpyAsync(S.seg_host, S.seg_status.data(), 2 * nseg * sizeof(double),
+                             hipMemcpyDeviceToHost, ctx.stream));
+    ctx.sync();
+    for (int k = 0; k < nseg; ++k) {
+      rr[k] = S.seg_host[2 * k];
+      brk[k] = S.seg_host[2 * k + 1] != 0.0 ? 1 : 0;
+    }
+  };
+  auto push_flags = [&]() {
+    LSMPS_HIP(hipMemcpyAsync(dflags.data(), flags.data(), nseg * sizeof(int32_t),
+                             hipMemcpyHostToDevice, ctx.stream));
+  };
+  auto segment_norms = [&]() {  // sc[k].rr = |r_k|^2
+    seg_reduce_kernel<T, 3><<<rb, kBlock, 0, ctx.stream>>>(g.start, s, p0, nullptr, r, part);
+    scalar_norm<<<nseg, kReduceThreads, 0, ctx.stream>>>(S.scalars.data(), part, kSegBlocks);
+    LSMPS_LAUNCH_CHECK();
+  };
+  auto spmv = [&](const T* in, T* out) {
+    SellArgs<T, V> a{n, M.chunks, M.dia_count.data(), M.dia_offsets.data(), M.dia_start.data(),
+                     dia, M.chunk_start.data(), M.chunk_width.data(), M.col_start.data(),
+                     M.nwin.data(), M.windows.data(), M.cols.data(), vals, in, out, p0, s,
+                     nullptr, nullptr};
+    sell_spmv_kernel<T, V, 0><<<sb, kSellBlock, 0, ctx.stream>>>(a);
+  };
+  // |b_k|^2 per segment (r = b on entry, x = 0)
+  std::vector<double> bnorm2(nseg), rr(nseg), rr_rep(nseg), target2(nseg);
+  std::vector<int> brk(nseg), done(nseg, 0);
+  fill_kernel<T><<<blocks_for(2 * s * ld), kBlock, 0, ctx.stream>>>(2 * s * ld, G, T(0));
+  copy_kernel<double, T><<<blocks_for(n), kBlock, 0, ctx.stream>>>(n, r, p0);
+  idr_init_scalars<<<nseg, kBlock, 0, ctx.stream>>>(S.scalars.data(), s);
+  segment_norms();
+  read_status(bnorm2, brk);
+  for (int k = 0; k < nseg; ++k) {
+    target2[k] = tolerance * tolerance * bnorm2[k];
+    rr[k] = rr_rep[k] = bnorm2[k];
+    done[k] = !(bnorm2[k] > 0.0);
+    flags[k] = done[k];
+  }
+  push_flags();
+  seg_flags_kernel<<<fb, kBlock, 0, ctx.stream>>>(S.scalars.data(), nseg, dflags.data());
+  const double factor2 = replace_factor * replace_factor;
+  int it = 0, replacements = 0;
+  auto active = [&]() {
+    for (int k = 0; k < nseg; ++k)
+      if (!done[k]) return true;
+    return false;
+  };
+  while (active() && it < max_iterations) {
+    seg_reduce_kernel<T, 4><<<rb, kBlock, 0, ctx.stream>>>(g.start, s, p0, nullptr, r, part);
+    idr_scalar_f<<<nseg, kReduceThreads, 0, ctx.stream>>>(S.scalars.data(), part, kSegBlocks, s);
+    for (int k = 0; k < s; ++k) {
+      if (schwarz) {
+        idr_a1_kernel<T, true><<<vb, kBlock, 0, ctx.stream>>>(iv, k);
+        schwarz_apply<T>(ctx, S.schwarz, v);
+        idr_a2_kernel<T, true><<<vb, kBlock, 0, ctx.stream>>>(iv, k);
+      } else {
+        idr_a_kernel<T, true><<<vb, kBlock, 0, ctx.stream>>>(iv, k);
+      }
+      spmv(U + k * ld, G + k * ld);
+      seg_reduce_kernel<T, 1><<<rb, kBlock, 0, ctx.stream>>>(g.start, s, p0, G + k * ld, r, part);
+      idr_scalar_after<<<nseg, kReduceThreads, 0, ctx.stream>>>(S.scalars.data(), part,
+                                                                kSegBlocks, s, k);
+      idr_b_kernel<T, true><<<vb, kBlock, 0, ctx.stream>>>(iv, k, k == s - 1 ? 1 : 0);
+      ++it;
+    }
+    if (schwarz) schwarz_apply<T>(ctx, S.schwarz, v);
+    spmv(v, t);
+    seg_reduce_kernel<T, 2><<<rb, kBlock, 0, ctx.stream>>>(g.start, s, p0, t, r, part);
+    idr_scalar_omega<<<nseg, kReduceThreads, 0, ctx.stream>>>(S.scalars.data(), part, part,
+                                                              kSegBlocks, kSegBlocks, 2);
+    idr_c_kernel<T, true><<<vb, kBlock, 0, ctx.stream>>>(iv);
+    segment_norms();
+    ++it;
+    read_status(rr, brk);
+    bool need_true = false;
+    for (int k = 0; k < nseg; ++k) {
+      if (done[k]) continue;
+      if (!std::isfinite(rr[k])) raise(LSMPS_ERR_NUMERICAL, "PPE solve produced a non-finite residual");
+      if (rr[k] <= target2[k] || rr[k] <= factor2 * rr_rep[k]) need_true = true;
+    }
+    if (need_true) {
+      const unsigned nb = spmv_blocks(n);
+      residual_kernel<<<nb, kBlock, 0, ctx.stream>>>(n, A64.row_offsets, A64.col_indices,
+                                                     A64.values, x, b64, r, nullptr,
+                                                     S.part_b.data());
+      segment_norms();
+      read_status(rr, brk);
+      for (int k = 0; k < nseg; ++k) rr_rep[k] = rr[k];
+      ++replacements;
+    }
+    bool restart = false;
+    for (int k = 0; k < nseg; ++k) {
+      flags[k] = 0;
+      if (done[k]) continue;
+      if (rr[k] <= target2[k]) {
+        done[k] = 1;
+        flags[k] = 1;
+      } else if (brk[k]) {
+        flags[k] = 2;
+        restart = true;
+      }
+    }
+    push_flags();
+    seg_flags_kernel<<<fb, kBlock, 0, ctx.stream>>>(S.scalars.data(), nseg, dflags.data());
+    if (restart) {
+      seg_restart_kernel<T><<<blocks_for(n), kBlock, 0, ctx.stream>>>(n, g.row_seg, dflags.data(),
+                                                                     ld, s, r, p0, G, U);
+      idr_init_scalars<<<nseg, kBlock, 0, ctx.stream>>>(S.scalars.data(), s, dflags.data());
+    }
+    LSMPS_LAUNCH_CHECK();
+  }
+  double total = 0.0, bn = 0.0, worst = 0.0;
+  bool all = true;
+  for (int k = 0; k < nseg; ++k) {
+    total += rr[k];
+    bn += bnorm2[k];
+    if (bnorm2[k] > 0.0) worst = std::max(worst, std::sqrt(rr[k] / bnorm2[k]));
+    all = all && done[k];
+  }
+  rep.iterations = it;
+  rep.refinements = replacements;
+  rep.rhs_norm = std::sqrt(bn);
+  rep.relative_residual = worst;  // the largest per-segment relative residual
+  rep.converged = all ? 1 : 0;
+  (void)total;
+}
+
 void pcg_solve(Context& ctx, PpeSolver& S, const lsmps_csr& A, const double* b, double* x,
                double target, int max_iterations, int check, lsmps_solve_report& rep) {
   const size_t n = A.rows;
@@ -1384,26 +1656,52 @@ void ppe_solve(Context& ctx, PpeSolver& S, const lsmps_csr& A, const double* b, 
       rep.schwarz_blocks = static_cast<int32_t>(blocks);
       rep.schwarz_rows = static_cast<int32_t>(std::min<int64_t>(rows, INT32_MAX));
     }
-    if (ctx.precision == LSMPS_PRECISION_FP64) {
-      build_sell<double>(ctx, S.sell, A, nullptr);
-      idr_solve<double, double>(ctx, S, n, S.sell, S.sell.vals64.data(), S.dinv64.data(), x, r,
-                                r + ld, ld, shadow_dim, bnorm * bnorm, target, max_it, A, b, 0.0,
-                                &it, &replacements, &rr);
-    } else {
-      // mixed: FP32 matrix values in the Krylov SpMVs; FP64 basis, iterate,
-      // residual, dots and reliable updates against the FP64 matrix.  (An FP32
-      // basis stagnates: r -= beta G_k with a rounded G_k leaves an error of
-      // 1e-7 |beta G_k|, far above the target when the recurrences grow.)
+    const bool mixed = ctx.precision == LSMPS_PRECISION_MIXED;
+    // mixed: FP32 matrix values in the Krylov SpMVs; FP64 basis, iterate,
+    // residual, dots and reliable updates against the FP64 matrix.  (An FP32
+    // basis stagnates: r -= beta G_k with a rounded G_k leaves an error of
+    // 1e-7 |beta G_k|, far above the target when the recurrences grow.)
+    const double factor = mixed ? (opt.inner_tolerance > 0.0 ? opt.inner_tolerance : 1.0e-3) : 0.0;
+    if (mixed) {
       build_sell<float>(ctx, S.sell, A, nullptr);
-      const double factor = opt.inner_tolerance > 0.0 ? opt.inner_tolerance : 1.0e-3;
-      idr_solve<double, float>(ctx, S, n, S.sell, S.sell.vals32.data(), S.dinv64.data(), x, r,
-                               r + ld, ld, shadow_dim, bnorm * bnorm, target, max_it, A, b,
-                               factor, &it, &replacements, &rr);
+    } else {
+      build_sell<double>(ctx, S.sell, A, nullptr);
     }
-    rep.iterations = it;
-    rep.refinements = replacements;
-    rep.relative_residual = std::sqrt(rr) / bnorm;
-    rep.converged = rep.relative_residual <= opt.tolerance ? 1 : 0;
+    if (S.segments.count > 0) {
+      // independent systems stacked in one matrix: per-segment recurrences
+      if (!S.seg_host || S.seg_host_count < static_cast<size_t>(S.segments.count)) {
+        if (S.seg_host) LSMPS_HIP(hipHostFree(S.seg_host));
+        S.seg_host = nullptr;
+        LSMPS_HIP(hipHostMalloc(reinterpret_cast<void**>(&S.seg_host),
+                                2 * S.segments.count * sizeof(double)));
+        S.seg_host_count = static_cast<size_t>(S.segments.count);
+      }
+      S.scalars.reserve(static_cast<size_t>(S.segments.count));
+      S.part_a.reserve(static_cast<size_t>(S.segments.count) * kSegBlocks * kPartialStride);
+      if (mixed) {
+        idr_solve_segmented<double, float>(ctx, S, n, S.sell, S.sell.vals32.data(),
+                                           S.dinv64.data(), x, r, r + ld, ld, shadow_dim,
+                                           opt.tolerance, max_it, A, b, factor, rep);
+      } else {
+        idr_solve_segmented<double, double>(ctx, S, n, S.sell, S.sell.vals64.data(),
+                                            S.dinv64.data(), x, r, r + ld, ld, shadow_dim,
+                                            opt.tolerance, max_it, A, b, factor, rep);
+      }
+    } else {
+      if (mixed) {
+        idr_solve<double, float>(ctx, S, n, S.sell, S.sell.vals32.data(), S.dinv64.data(), x,
+                                 r, r + ld, ld, shadow_dim, bnorm * bnorm, target, max_it, A, b,
+                                 factor, &it, &replacements, &rr);
+      } else {
+        idr_solve<double, double>(ctx, S, n, S.sell, S.sell.vals64.data(), S.dinv64.data(), x,
+                                  r, r + ld, ld, shadow_dim, bnorm * bnorm, target, max_it, A,
+                                  b, factor, &it, &replacements, &rr);
+      }
+      rep.iterations = it;
+      rep.refinements = replacements;
+      rep.relative_residual = std::sqrt(rr) / bnorm;
+      rep.converged = rep.relative_residual <= opt.tolerance ? 1 : 0;
+    }
   }
   if (opt.clamp_negative) {
     // clamp_pressure_kernel (ppe_matrix.cu:373-379)

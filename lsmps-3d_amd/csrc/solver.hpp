@@ -25,7 +25,7 @@ struct SolverScalars {
   double cg_alpha;  // CG step
   double cg_beta;
   int32_t breakdown;
-  int32_t pad;
+  int32_t done;     // batched solves: this segment has converged and is frozen
 };
 
 // PPE matrix for the Krylov SpMVs: a hybrid of a per-chunk diagonal (DIA)
@@ -86,8 +86,23 @@ struct SellMatrix {
   }
 };
 
+// Batched solve of independent systems (tanks) stacked in one matrix: segment k
+// owns rows [start[k], start[k+1]); row_seg[i] is row i's segment.  Every segment
+// runs its own IDR(s) recurrences (own scalars, dot products and convergence
+// test) through shared kernel launches.
+struct SolverSegments {
+  int count = 0;
+  const int32_t* start = nullptr;    // device, count + 1
+  const int32_t* row_seg = nullptr;  // device, n
+};
+
 class PpeSolver {
  public:
+  SolverSegments segments;         // set by the caller before ppe_solve (count 0: one system)
+  DeviceArray<double> seg_status;  // per segment: rr, breakdown (packed for one copy)
+  DeviceArray<int32_t> seg_flags;   // per segment: 1 converged, 2 restart
+  double* seg_host = nullptr;      // pinned copy of seg_status
+  size_t seg_host_count = 0;
   SellMatrix sell;
   SchwarzPrec schwarz;
   DeviceArray<double> dinv64;  // Jacobi: 1 / diagonal

@@ -174,6 +174,14 @@ void lsmps_stepper::set_segments(int count, const int64_t* fo, const int64_t* wo
     for (int64_t i = wo[k]; i < wo[k + 1]; ++i) wseg[i] = k;
   }
   Context& c = *ctx;
+  std::vector<int32_t> rows32(count + 1);
+  for (int k = 0; k <= count; ++k) {
+    require(fo[k] <= INT32_MAX, "Particle count exceeds 32-bit index capacity");
+    rows32[k] = static_cast<int32_t>(fo[k]);
+  }
+  segment_rows.reserve(count + 1);
+  LSMPS_HIP(hipMemcpyAsync(segment_rows.data(), rows32.data(), (count + 1) * sizeof(int32_t),
+                           hipMemcpyHostToDevice, c.stream));
   segment_grids.reserve(count);
   fluid_segment.reserve(fseg.size());
   wall_segment.reserve(wseg.size());
@@ -254,6 +262,13 @@ void lsmps_stepper::step(const lsmps_fluid_soa& fluid, const lsmps_wall_soa& wal
   const uint32_t saved_flags = c.flags;
   c.flags |= LSMPS_FLAG_ASYNC;  // the stepper synchronises once at the end
   bool converged = true;
+  // batched tanks: one Krylov recurrence per tank (PPE rows = fluid particles)
+  solver.segments = SolverSegments{};
+  if (segment_count > 0) {
+    solver.segments.count = segment_count;
+    solver.segments.start = segment_rows.data();
+    solver.segments.row_seg = fluid_segment.data();
+  }
   try {
     ppe_solve(c, solver, ws.matrix, ws.rhs, ws.pressure, solver_options, &last_report);
   } catch (const Error& e) {
