@@ -486,6 +486,46 @@ int lsmps_stepper_correction_vectors(lsmps_stepper* stepper, lsmps_vector3* pres
                                      lsmps_vector3* ps_displacement);
 lsmps_moment_matrix* lsmps_stepper_moments(lsmps_stepper* stepper);
 
+/* ---- adjoint of one step (no reference counterpart) -------------------- */
+/* Stepper modes (bit flags, default 0):
+ *  LSMPS_STEP_RECORD           keep what the adjoint needs and the step
+ *                              overwrites (start positions and velocities, the
+ *                              PPE solution before the clamp): 7 doubles per
+ *                              fluid particle.
+ *  LSMPS_STEP_FROZEN_GEOMETRY  skip the neighbor search and the surface
+ *                              classification and reuse the lists of the
+ *                              previous step and the labels already in
+ *                              fluid.surface_type (same particle counts).  The
+ *                              step is then a smooth function of positions,
+ *                              velocities and gravity, which is what central
+ *                              finite differences of the adjoint need. */
+enum { LSMPS_STEP_RECORD = 1u, LSMPS_STEP_FROZEN_GEOMETRY = 2u };
+int lsmps_stepper_set_mode(lsmps_stepper* stepper, uint32_t mode);
+
+/* Outputs of lsmps_adjoint_step.  Device pointers may be NULL to skip a
+ * gradient; gravity and the report are host values. */
+typedef struct lsmps_adjoint_grads {
+  lsmps_vector3 position;   /* dL/dx at the start of the step (fluid) */
+  lsmps_vector3 velocity;   /* dL/dv at the start of the step (fluid) */
+  double* pressure;         /* dL/dp for the clamped pressure the correction read */
+  double gravity[3];        /* dL/dg (body force) */
+  lsmps_solve_report transpose_report; /* the A^T solve of the adjoint */
+} lsmps_adjoint_grads;
+
+/* Reverse mode of the stepper's last step, which must have been taken with
+ * LSMPS_STEP_RECORD.  L is any scalar function of the step's outputs; the caller
+ * passes dL/dv_new (3N) and optionally dL/dx_new (NULL = 0).  Neighbor lists and
+ * surface labels are held fixed; every stage is differentiated: moment
+ * operators (both kinds), provision, PPE assembly, PPE solve (one solve with
+ * A^T, same solver and tolerance, no clamp), the pressure clamp, pressure
+ * gradient, particle shifting, update, anti-penetration and smoothing.
+ * `fluid` is the fluid after the step (its surface_type holds the step's
+ * labels), `walls` the walls of the step.  Arithmetic is FP64 in both precision
+ * modes except the A^T solve, which follows the context precision. */
+int lsmps_adjoint_step(lsmps_stepper* stepper, const lsmps_fluid_soa* fluid,
+                       const lsmps_wall_soa* walls, const lsmps_vector3* grad_velocity_out,
+                       const lsmps_vector3* grad_position_out, lsmps_adjoint_grads* grads);
+
 #ifdef __cplusplus
 }
 #endif

@@ -34,6 +34,9 @@ SOLVER_IDRS = 2
 PRECOND_JACOBI = 0
 PRECOND_SCHWARZ = 1
 
+STEP_RECORD = 1
+STEP_FROZEN_GEOMETRY = 2
+
 FLAG_ASYNC = 1
 FLAG_KERNEL_TIMING = 2
 
@@ -151,6 +154,11 @@ class SolveReport(C.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class AdjointGrads(C.Structure):
+    _fields_ = [("position", Vector3), ("velocity", Vector3), ("pressure", _dp),
+                ("gravity", C.c_double * 3), ("transpose_report", SolveReport)]
 
 
 def default_config():

@@ -182,7 +182,7 @@ def block_in_tank(name, block, tank, l0, jitter_amplitude=0.0, seed=0, layers=3,
 
 
 def hydrostatic(n=20, l0=0.01, layers=3, weight_kernel=abi.WEIGHT_LINEAR):
-    """20^3 hydrostatic box, open top, 3 wall layers (BASELINE case C1)."""
+    """20^3 hydrostatic box, open top, 3 wall layers (case C1)."""
     return block_in_tank(f"hydrostatic_{n}", (n, n, n), (n, n, n), l0, 0.0, 0, layers,
                          weight_kernel)
 
@@ -212,6 +212,30 @@ def sloshing(n=50, l0=0.005, seed=2, tank_height=None):
     derive_cell_grid(cfg, fluid, walls)
     target = normal(seed, 3 * x.shape[0]).reshape(-1, 3) * 0.01
     return Case(f"sloshing_{n}", fluid, walls, cfg, {"v_target": target, "l0": l0})
+
+
+def adjoint_stress(n=12, l0=0.005, seed=5, weight_kernel=abi.WEIGHT_LINEAR):
+    """Small jittered block that reaches every branch of the step and of its
+    adjoint: pairs closer than the shifting distance (jitter 0.3 l0), particles
+    inside the wall clearance, random velocities (some into the walls), a
+    shifting scale large enough to reach the displacement cap, and a few top
+    particles lifted by 3.5 l0 (splash labels next to surface rows: four top
+    particles at least 3.5 l0 from the side walls and 5 l0 from each other)."""
+    assert n >= 10
+    x, y, z = lattice_block(n, n, n, l0)
+    x, y, z = jitter(x, y, z, l0, 0.3, seed)
+    z = z.copy()
+    lo, hi = 3, n - 4
+    pick = [((n - 1) * n + iy) * n + ix for iy in (lo, hi) for ix in (lo, hi)]
+    z[pick] += 3.5 * l0
+    v = normal(seed + 1, 3 * x.shape[0]).reshape(-1, 3) * 0.05
+    walls = tank_walls(n, n, n + 4, l0)
+    fluid = _fluid_dict(x, y, z, vx=v[:, 0].copy(), vy=v[:, 1].copy(), vz=v[:, 2].copy())
+    cfg = case_config(l0, weight_kernel=weight_kernel)
+    cfg.ps_displacement_scale = 0.5
+    derive_cell_grid(cfg, fluid, walls)
+    target = normal(seed + 2, 3 * x.shape[0]).reshape(-1, 3) * 0.01
+    return Case(f"adjoint_stress_{n}", fluid, walls, cfg, {"v_target": target, "l0": l0})
 
 
 def splash(n=160, l0=0.0025, seed=3, top_layers=8):

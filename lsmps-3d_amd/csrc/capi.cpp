@@ -610,6 +610,35 @@ int lsmps_stepper_set_segments(lsmps_stepper* s, int32_t segment_count,
 
 size_t lsmps_stepper_bytes(const lsmps_stepper* s) { return s ? s->bytes() : 0; }
 
+int lsmps_stepper_set_mode(lsmps_stepper* s, uint32_t mode) {
+  API_BEGIN
+  require(s != nullptr, "null stepper");
+  require((mode & ~static_cast<uint32_t>(LSMPS_STEP_RECORD | LSMPS_STEP_FROZEN_GEOMETRY)) == 0,
+          "unknown stepper mode bits");
+  s->mode = mode;
+  if (!(mode & LSMPS_STEP_RECORD)) s->recorded = false;
+  API_END("lsmps_stepper_set_mode")
+}
+
+int lsmps_adjoint_step(lsmps_stepper* s, const lsmps_fluid_soa* fluid, const lsmps_wall_soa* walls,
+                       const lsmps_vector3* grad_velocity_out,
+                       const lsmps_vector3* grad_position_out, lsmps_adjoint_grads* grads) {
+  API_BEGIN
+  require(s != nullptr && grads != nullptr, "null argument");
+  s->ctx->activate();
+  int rc = LSMPS_OK;
+  try {
+    adjoint_step(*s->ctx, *s, deref(fluid, "fluid"), deref(walls, "walls"),
+                 deref(grad_velocity_out, "grad_velocity_out"), grad_position_out, *grads);
+  } catch (const Error& e) {
+    if (e.code != LSMPS_ERR_NOT_CONVERGED) throw;
+    g_last_error = std::string("lsmps_adjoint_step: ") + e.what();
+    rc = e.code;
+  }
+  return rc;
+  API_END("lsmps_adjoint_step")
+}
+
 int lsmps_stepper_step(lsmps_stepper* s, const lsmps_fluid_soa* fluid, const lsmps_wall_soa* walls,
                        lsmps_solve_report* report) {
   API_BEGIN

@@ -61,6 +61,42 @@ __device__ __forceinline__ void wall_neumann_q(T dx, T dy, T dz, T nx, T ny, T n
   q[8] = (dx * nz + dz * nx) * inv_re;
 }
 
+// ---- derivatives used by the adjoint of the step ----------------------------
+
+// dw/dr of weight_of inside its support (0 for r >= re).  At r = 0 the pair
+// vector has no direction and callers drop the dw/dr term.
+template <typename T>
+__device__ __forceinline__ T weight_slope(T distance, T re, int kernel) {
+  if (distance >= re) return T(0);
+  return kernel == LSMPS_WEIGHT_QUADRATIC ? T(-2) * (T(1) - distance / re) / re : T(-1) / re;
+}
+
+// dw/dr of correction_weight_of (0 outside (0, re)).
+template <typename T>
+__device__ __forceinline__ T correction_weight_slope(T distance, T re, int kernel) {
+  if (distance <= T(0)) return T(0);
+  return weight_slope(distance, re, kernel);
+}
+
+// J_b^T u for the Type-A basis b(d) (s = d / re): out_c = sum_k u_k db_k/dd_c.
+template <typename T>
+__device__ __forceinline__ void type_a_basis_jt(T dx, T dy, T dz, T inv_re, const T u[9],
+                                                T out[3]) {
+  const T sx = dx * inv_re, sy = dy * inv_re, sz = dz * inv_re;
+  out[0] = (u[0] + T(2) * sx * u[3] + sy * u[6] + sz * u[8]) * inv_re;
+  out[1] = (u[1] + T(2) * sy * u[4] + sx * u[6] + sz * u[7]) * inv_re;
+  out[2] = (u[2] + T(2) * sz * u[5] + sy * u[7] + sx * u[8]) * inv_re;
+}
+
+// J_q^T u for the wall Neumann vector q(d, n) (linear in d).
+template <typename T>
+__device__ __forceinline__ void wall_neumann_q_jt(T nx, T ny, T nz, T inv_re, const T u[9],
+                                                  T out[3]) {
+  out[0] = (T(2) * nx * u[3] + ny * u[6] + nz * u[8]) * inv_re;
+  out[1] = (T(2) * ny * u[4] + nx * u[6] + nz * u[7]) * inv_re;
+  out[2] = (T(2) * nz * u[5] + ny * u[7] + nx * u[8]) * inv_re;
+}
+
 // Operator rows kept per particle and moment kind (SoA, index k*N + i):
 //   k = 0..8   row 0 of M^-1 (x-gradient coefficients)
 //   k = 9..17  row 1 (y-gradient)

@@ -299,6 +299,143 @@ __global__ void __launch_bounds__(kBlock) moment_kernel(MomentArgs a) {
   }
 }
 
+// Adjoint of the operator rows of a Type-A kind with respect to the fluid
+// positions.  Given Rbar (the gradient of a loss with respect to the four
+// stored rows R_k = M^-1 e_k), dR_k = -M^-1 dM R_k gives
+// Mbar = -sum_k (M^-1 Rbar_k) R_k^T; with the regularisation lambda tr(M)/n I
+// the raw moment gradient is S = sym(Mbar) + lambda/n tr(Mbar) I, and each
+// sample w(|d|) v(d) v(d)^T contributes w' d/|d| (v.S v) + 2 w J_v^T S v to the
+// gradient with respect to its pair vector d = x_j - x_i.  The matrix is
+// re-accumulated and factored in FP64 (nothing of the forward factor is kept).
+struct MomentAdjArgs {
+  MomentArgs m;
+  const double* ops_bar;  // kOpRows x n
+  double* f;              // pair gradients, 3 x fcap
+  size_t fcap;
+  double* xbar;  // 3 x n
+};
+
+__global__ void __launch_bounds__(kBlock) moment_adjoint_kernel(MomentAdjArgs A) {
+  const MomentArgs& a = A.m;
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  constexpr int N = 9;
+  constexpr int P = Packed<N>::kSize;
+  double m[P];
+#pragma unroll
+  for (int k = 0; k < P; ++k) m[k] = 0.0;
+  const double xi = a.x[i], yi = a.y[i], zi = a.z[i];
+  const double inv_re = 1.0 / a.re;
+  double v[N];
+  for (int32_t c = a.f_off[i]; c < a.f_off[i + 1]; ++c) {
+    const int32_t j = a.f_idx[c];
+    const double dx = a.x[j] - xi, dy = a.y[j] - yi, dz = a.z[j] - zi;
+    const double d2 = dx * dx + dy * dy + dz * dz;
+    if (d2 <= 0.0) continue;
+    const double w = weight_of(sqrt(d2), a.re, a.kernel);
+    if (w <= 0.0) continue;
+    basis_full<N>(dx, dy, dz, inv_re, v);
+    add_outer<N>(m, v, w);
+  }
+  if (a.boundary != 0) {
+    for (int32_t c = a.w_off[i]; c < a.w_off[i + 1]; ++c) {
+      const int32_t j = a.w_idx[c];
+      const double dx = a.wx[j] - xi, dy = a.wy[j] - yi, dz = a.wz[j] - zi;
+      const double d2 = dx * dx + dy * dy + dz * dz;
+      if (d2 <= 0.0) continue;
+      const double w = a.wall_scale * weight_of(sqrt(d2), a.re, a.kernel);
+      if (w <= 0.0) continue;
+      if (a.boundary == 2) neumann_full<N>(dx, dy, dz, a.wnx[j], a.wny[j], a.wnz[j], inv_re, v);
+      else basis_full<N>(dx, dy, dz, inv_re, v);
+      add_outer<N>(m, v, w);
+    }
+  }
+  double trace = 0.0;
+#pragma unroll
+  for (int r = 0; r < N; ++r) trace += m[Packed<N>::at(r, r)];
+  const double reg = a.lambda * (trace > 0.0 ? trace / static_cast<double>(N) : 1.0);
+#pragma unroll
+  for (int r = 0; r < N; ++r) m[Packed<N>::at(r, r)] += reg;
+  if (cholesky<N, double>(m) != 0) return;  // the step's operators were NaN already
+  double S[P];
+#pragma unroll
+  for (int k = 0; k < P; ++k) S[k] = 0.0;
+#pragma unroll
+  for (int which = 0; which < 4; ++which) {
+    double R[N], y[N];
+    rhs_vector<N>(which, R);
+    chol_solve<N, double>(m, R);
+#pragma unroll
+    for (int k = 0; k < N; ++k) y[k] = A.ops_bar[static_cast<size_t>(which * 9 + k) * a.n + i];
+    chol_solve<N, double>(m, y);
+#pragma unroll
+    for (int r = 0; r < N; ++r)
+#pragma unroll
+      for (int c = 0; c <= r; ++c) S[Packed<N>::at(r, c)] -= 0.5 * (y[r] * R[c] + R[r] * y[c]);
+  }
+  if (trace > 0.0) {
+    double ts = 0.0;
+#pragma unroll
+    for (int r = 0; r < N; ++r) ts += S[Packed<N>::at(r, r)];
+    const double add = a.lambda / static_cast<double>(N) * ts;
+#pragma unroll
+    for (int r = 0; r < N; ++r) S[Packed<N>::at(r, r)] += add;
+  }
+  double sv[N], jt[3];
+  const size_t fc = A.fcap;
+  for (int32_t c = a.f_off[i]; c < a.f_off[i + 1]; ++c) {
+    const int32_t j = a.f_idx[c];
+    const double dx = a.x[j] - xi, dy = a.y[j] - yi, dz = a.z[j] - zi;
+    const double d2 = dx * dx + dy * dy + dz * dz;
+    if (d2 <= 0.0) continue;
+    const double r = sqrt(d2);
+    const double w = weight_of(r, a.re, a.kernel);
+    if (w <= 0.0) continue;
+    const double dwr = weight_slope(r, a.re, a.kernel) / r;
+    basis_full<N>(dx, dy, dz, inv_re, v);
+    sym_matvec<N>(S, v, sv);
+    double vsv = 0.0;
+#pragma unroll
+    for (int k = 0; k < N; ++k) vsv += v[k] * sv[k];
+    type_a_basis_jt(dx, dy, dz, inv_re, sv, jt);
+    A.f[c] += dwr * vsv * dx + 2.0 * w * jt[0];
+    A.f[fc + c] += dwr * vsv * dy + 2.0 * w * jt[1];
+    A.f[2 * fc + c] += dwr * vsv * dz + 2.0 * w * jt[2];
+  }
+  if (a.boundary != 0) {
+    double xl[3] = {0.0, 0.0, 0.0};
+    for (int32_t c = a.w_off[i]; c < a.w_off[i + 1]; ++c) {
+      const int32_t j = a.w_idx[c];
+      const double dx = a.wx[j] - xi, dy = a.wy[j] - yi, dz = a.wz[j] - zi;
+      const double d2 = dx * dx + dy * dy + dz * dz;
+      if (d2 <= 0.0) continue;
+      const double r = sqrt(d2);
+      const double bw = weight_of(r, a.re, a.kernel);
+      const double w = a.wall_scale * bw;
+      if (w <= 0.0) continue;
+      const double dwr = a.wall_scale * weight_slope(r, a.re, a.kernel) / r;
+      if (a.boundary == 2) {
+        neumann_full<N>(dx, dy, dz, a.wnx[j], a.wny[j], a.wnz[j], inv_re, v);
+        sym_matvec<N>(S, v, sv);
+        wall_neumann_q_jt(a.wnx[j], a.wny[j], a.wnz[j], inv_re, sv, jt);
+      } else {
+        basis_full<N>(dx, dy, dz, inv_re, v);
+        sym_matvec<N>(S, v, sv);
+        type_a_basis_jt(dx, dy, dz, inv_re, sv, jt);
+      }
+      double vsv = 0.0;
+#pragma unroll
+      for (int k = 0; k < N; ++k) vsv += v[k] * sv[k];
+      // d = x_wall - x_i
+      xl[0] -= dwr * vsv * dx + 2.0 * w * jt[0];
+      xl[1] -= dwr * vsv * dy + 2.0 * w * jt[1];
+      xl[2] -= dwr * vsv * dz + 2.0 * w * jt[2];
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) A.xbar[k * a.n + i] += xl[k];
+  }
+}
+
 }  // namespace
 
 // make_matrix_request (moment_matrix.cu:494-538): boundary handling and size per kind.
@@ -312,19 +449,10 @@ static int moment_boundary(int kind) {
   }
 }
 
-void compute_moments(Context& ctx, int kind, const lsmps_config& cfg, const lsmps_fluid_soa& fluid,
-                     const lsmps_wall_soa& walls, const lsmps_neighbor_list& fn,
-                     const lsmps_neighbor_list& wn, const MomentOutputs& out) {
-  // validate_matrix_request (moment_matrix.cu:102-115)
-  require(cfg.support_radius > 0.0, "Moment support radius must be positive");
-  require(cfg.lsmps_regularization >= 0.0, "Moment regularization must be non-negative");
-  require(cfg.lsmps_wall_weight_scale >= 0.0, "Moment wall weight scale must be non-negative");
-  require(fluid.count <= fn.particle_count && fluid.count <= wn.particle_count,
-          "Moment matrix neighbor lists are smaller than fluid particle count");
-  if (fluid.count == 0) return;
-  const int size = moment_size(kind);
-  require(out.ops == nullptr || size == 9, "Operator rows exist for Type-A kinds only");
-  MomentArgs a;
+static MomentArgs moment_args(Context& ctx, int kind, const lsmps_config& cfg,
+                              const lsmps_fluid_soa& fluid, const lsmps_wall_soa& walls,
+                              const lsmps_neighbor_list& fn, const lsmps_neighbor_list& wn) {
+  MomentArgs a{};
   a.n = fluid.count;
   a.x = fluid.x;
   a.y = fluid.y;
@@ -345,6 +473,22 @@ void compute_moments(Context& ctx, int kind, const lsmps_config& cfg, const lsmp
   a.kernel = cfg.weight_kernel;
   a.boundary = moment_boundary(kind);
   a.mixed = ctx.precision == LSMPS_PRECISION_MIXED ? 1 : 0;
+  return a;
+}
+
+void compute_moments(Context& ctx, int kind, const lsmps_config& cfg, const lsmps_fluid_soa& fluid,
+                     const lsmps_wall_soa& walls, const lsmps_neighbor_list& fn,
+                     const lsmps_neighbor_list& wn, const MomentOutputs& out) {
+  // validate_matrix_request (moment_matrix.cu:102-115)
+  require(cfg.support_radius > 0.0, "Moment support radius must be positive");
+  require(cfg.lsmps_regularization >= 0.0, "Moment regularization must be non-negative");
+  require(cfg.lsmps_wall_weight_scale >= 0.0, "Moment wall weight scale must be non-negative");
+  require(fluid.count <= fn.particle_count && fluid.count <= wn.particle_count,
+          "Moment matrix neighbor lists are smaller than fluid particle count");
+  if (fluid.count == 0) return;
+  const int size = moment_size(kind);
+  require(out.ops == nullptr || size == 9, "Operator rows exist for Type-A kinds only");
+  MomentArgs a = moment_args(ctx, kind, cfg, fluid, walls, fn, wn);
   a.ops = out.ops;
   a.inverse = out.inverse;
   a.matrices = out.matrices;
@@ -359,6 +503,22 @@ void compute_moments(Context& ctx, int kind, const lsmps_config& cfg, const lsmp
     if (a.mixed) moment_kernel<9, true><<<grid, kBlock, 0, ctx.stream>>>(a);
     else moment_kernel<9, false><<<grid, kBlock, 0, ctx.stream>>>(a);
   }
+  LSMPS_LAUNCH_CHECK();
+}
+
+void moment_adjoint(Context& ctx, int kind, const lsmps_config& cfg, const lsmps_fluid_soa& fluid,
+                    const lsmps_wall_soa& walls, const lsmps_neighbor_list& fn,
+                    const lsmps_neighbor_list& wn, const double* ops_bar, double* pair_grad,
+                    size_t pair_stride, double* xbar) {
+  require(moment_size(kind) == 9, "Operator-row adjoints exist for Type-A kinds only");
+  if (fluid.count == 0) return;
+  MomentAdjArgs A;
+  A.m = moment_args(ctx, kind, cfg, fluid, walls, fn, wn);
+  A.ops_bar = ops_bar;
+  A.f = pair_grad;
+  A.fcap = pair_stride;
+  A.xbar = xbar;
+  moment_adjoint_kernel<<<blocks_for(fluid.count), kBlock, 0, ctx.stream>>>(A);
   LSMPS_LAUNCH_CHECK();
 }
 

@@ -95,6 +95,26 @@ struct lsmps_stepper {
   int n_events = 0;
   double stage_ms[7] = {0, 0, 0, 0, 0, 0, 0};
   lsmps_solve_report last_report{};
+  // LSMPS_STEP_* mode.  RECORD keeps what the adjoint of the step reads and the
+  // step overwrites: positions and velocities at the start of the step and the
+  // PPE solution before the clamp.  FROZEN_GEOMETRY reuses the previous step's
+  // neighbor lists and surface labels.
+  uint32_t mode = 0;
+  bool geometry_ready = false;
+  size_t geometry_fluid = 0, geometry_walls = 0;
+  bool recorded = false;
+  size_t recorded_fluid = 0, recorded_walls = 0;
+  lsmps::DeviceArray<double> start_state;   // x, y, z, vx, vy, vz: 6 x fluid capacity
+  lsmps::DeviceArray<double> raw_pressure;  // fluid capacity
+  struct Adjoint {
+    lsmps::DeviceArray<int32_t> rev, flags;
+    lsmps::DeviceArray<double> pair, work, at_vals, sums;
+    lsmps::PpeSolver solver;
+    size_t bytes() const {
+      return rev.bytes() + flags.bytes() + pair.bytes() + work.bytes() + at_vals.bytes() +
+             sums.bytes() + solver.bytes();
+    }
+  } adjoint;
 
   ~lsmps_stepper();
   void allocate(size_t nf, size_t nw, size_t cells);
@@ -106,3 +126,12 @@ struct lsmps_stepper {
   lsmps_neighbor_list wall_neighbors() const;
   lsmps_ppe_workspace ppe_workspace(size_t n) const;
 };
+
+namespace lsmps {
+// Per-particle rows of adjoint work arrays (adjoint.hip).
+constexpr int kAdjointWork = 142;
+// Reverse mode of the stepper's last (recorded) step; adjoint.hip.
+void adjoint_step(Context& ctx, lsmps_stepper& stepper, const lsmps_fluid_soa& fluid,
+                  const lsmps_wall_soa& walls, const lsmps_vector3& grad_velocity_out,
+                  const lsmps_vector3* grad_position_out, lsmps_adjoint_grads& grads);
+}  // namespace lsmps

@@ -58,6 +58,14 @@ int moment_size(int kind);
 void compute_moments(Context& ctx, int kind, const lsmps_config& cfg, const lsmps_fluid_soa& fluid,
                      const lsmps_wall_soa& walls, const lsmps_neighbor_list& fn,
                      const lsmps_neighbor_list& wn, const MomentOutputs& out);
+// Reverse mode of the operator rows of a Type-A kind: ops_bar (kOpRows x N, the
+// loss gradient of the rows) is propagated to the fluid positions.  Pair terms
+// are added to pair_grad (3 x pair_stride, gradient w.r.t. x_j - x_i of each
+// fluid CSR entry), wall terms to xbar (3 x N).
+void moment_adjoint(Context& ctx, int kind, const lsmps_config& cfg, const lsmps_fluid_soa& fluid,
+                    const lsmps_wall_soa& walls, const lsmps_neighbor_list& fn,
+                    const lsmps_neighbor_list& wn, const double* ops_bar, double* pair_grad,
+                    size_t pair_stride, double* xbar);
 
 // ---- provision.hip
 // velocity Laplacian (rows of the velocity Type-A operator) fused with the

@@ -115,7 +115,8 @@ size_t lsmps_stepper::bytes() const {
          wc_ids.bytes() + wc_sorted.bytes() + wc_begin.bytes() + wc_end.bytes() + f_off.bytes() +
          f_idx.bytes() + w_off.bytes() + w_idx.bytes() + moments.bytes() + tmp_fluid.bytes() +
          tmp_wall.bytes() + ppe_rows.bytes() + ppe_cols.bytes() + ppe_vals.bytes() +
-         ppe_vec.bytes() + solver.bytes() + corr.bytes();
+         ppe_vec.bytes() + solver.bytes() + corr.bytes() + start_state.bytes() +
+         raw_pressure.bytes() + adjoint.bytes();
 }
 
 lsmps_neighbor_list lsmps_stepper::fluid_neighbors() const {
@@ -221,9 +222,24 @@ void lsmps_stepper::step(const lsmps_fluid_soa& fluid, const lsmps_wall_soa& wal
                      wc_end.data()};
   const lsmps_neighbor_list fn = fluid_neighbors();
   const lsmps_neighbor_list wn = wall_neighbors();
+  const bool record = (mode & LSMPS_STEP_RECORD) != 0;
+  const bool frozen = (mode & LSMPS_STEP_FROZEN_GEOMETRY) != 0;
+  if (frozen)
+    require(geometry_ready && geometry_fluid == nf && geometry_walls == nw,
+            "frozen geometry needs a previous step over the same particles");
+  recorded = false;
+  if (record && nf > 0) {
+    require(fluid.vx && fluid.vy && fluid.vz, "recording needs fluid velocities");
+    start_state.reserve(6 * nf1);
+    raw_pressure.reserve(nf1);
+    const double* src[6] = {fluid.x, fluid.y, fluid.z, fluid.vx, fluid.vy, fluid.vz};
+    for (int k = 0; k < 6; ++k) copy_f64(c.stream, start_state.data() + k * nf1, src[k], nf);
+  }
   LSMPS_HIP(hipEventRecord(events[0], c.stream));
   // 1. geometry: rebuild_geometry (simulation_driver.cu:252-264)
-  if (segment_count > 0) {
+  if (frozen) {
+    // lists and labels of the previous step
+  } else if (segment_count > 0) {
     Segments seg;
     seg.count = segment_count;
     seg.grids = segment_grids.data();
@@ -235,7 +251,10 @@ void lsmps_stepper::step(const lsmps_fluid_soa& fluid, const lsmps_wall_soa& wal
     build_neighbor_lists(c, fluid, walls, config, fc, wc, fn, wn, nullptr, nullptr);
   }
   ++generation;
-  classify_surface(c, fluid, walls, fn, wn, config, nullptr);
+  if (!frozen) classify_surface(c, fluid, walls, fn, wn, config, nullptr);
+  geometry_ready = true;
+  geometry_fluid = nf;
+  geometry_walls = nw;
   LSMPS_HIP(hipEventRecord(events[1], c.stream));
   // 2. moment operators (prepare_matrices, moment_matrix.cu:615-654)
   moments.config = config;
@@ -269,8 +288,10 @@ void lsmps_stepper::step(const lsmps_fluid_soa& fluid, const lsmps_wall_soa& wal
     solver.segments.start = segment_rows.data();
     solver.segments.row_seg = fluid_segment.data();
   }
+  lsmps_solver_options opts = solver_options;
+  if (record) opts.clamp_negative = 0;  // the correction clamps the same way
   try {
-    ppe_solve(c, solver, ws.matrix, ws.rhs, ws.pressure, solver_options, &last_report);
+    ppe_solve(c, solver, ws.matrix, ws.rhs, ws.pressure, opts, &last_report);
   } catch (const Error& e) {
     c.flags = saved_flags;
     if (e.code != LSMPS_ERR_NOT_CONVERGED) throw;
@@ -280,6 +301,7 @@ void lsmps_stepper::step(const lsmps_fluid_soa& fluid, const lsmps_wall_soa& wal
     throw;
   }
   c.flags = saved_flags;
+  if (record && nf > 0) copy_f64(c.stream, raw_pressure.data(), ws.pressure, nf);
   LSMPS_HIP(hipEventRecord(events[5], c.stream));
   // 6. correction (pressure_correction.cu:598-645)
   double* cv = corr.data();
@@ -302,6 +324,11 @@ void lsmps_stepper::step(const lsmps_fluid_soa& fluid, const lsmps_wall_soa& wal
   float total = 0.0f;
   LSMPS_HIP(hipEventElapsedTime(&total, events[0], events[6]));
   stage_ms[6] = total;
+  if (record) {
+    recorded = true;
+    recorded_fluid = nf;
+    recorded_walls = nw;
+  }
   if (!converged)
     raise(LSMPS_ERR_NOT_CONVERGED, "PPE solver did not reach the tolerance (relative residual " +
                                        std::to_string(last_report.relative_residual) + ")");

@@ -30,7 +30,7 @@ EXPORTS = [
     "lsmps_stepper_step", "lsmps_stepper_stage_times", "lsmps_stepper_neighbor_lists",
     "lsmps_stepper_ppe_workspace", "lsmps_stepper_temporary_velocity",
     "lsmps_stepper_correction_vectors", "lsmps_stepper_moments", "lsmps_context_kernel_stats",
-    "lsmps_stepper_set_segments",
+    "lsmps_stepper_set_segments", "lsmps_stepper_set_mode", "lsmps_adjoint_step",
 ]
 
 _LIB = None
@@ -195,6 +195,51 @@ class Stepper:
 
     def set_solver(self, options):
         check(self.lib.lsmps_stepper_set_solver(self.handle, C.byref(options)), "set_solver")
+
+    def set_config(self, config):
+        self.config = config.copy()
+        check(self.lib.lsmps_stepper_set_config(self.handle, C.byref(self.config)), "set_config")
+
+    def set_mode(self, mode):
+        """abi.STEP_RECORD (keep what the adjoint needs) | abi.STEP_FROZEN_GEOMETRY."""
+        check(self.lib.lsmps_stepper_set_mode(self.handle, C.c_uint32(mode)), "set_mode")
+
+    def adjoint(self, grad_v, grad_x=None, want=("position", "velocity", "pressure"),
+                raise_on_nonconvergence=True):
+        """Reverse mode of the last recorded step.  grad_v / grad_x: (3, N) float64
+        device tensors (dL/dv_new, dL/dx_new).  Returns device tensors for the
+        requested gradients, the host dL/dg and the A^T solve report."""
+        torch = _torch()
+        n = self.state.n_fluid
+        dev = self.state.device
+        gv = grad_v.contiguous()
+        assert gv.shape == (3, n) and gv.dtype == torch.float64 and gv.is_cuda
+        gvs = abi.Vector3(n, *(abi.dptr(ptr(gv[k])) for k in range(3)))
+        gxs = None
+        if grad_x is not None:
+            gx = grad_x.contiguous()
+            assert gx.shape == (3, n) and gx.dtype == torch.float64 and gx.is_cuda
+            gxs = abi.Vector3(n, *(abi.dptr(ptr(gx[k])) for k in range(3)))
+        out = {}
+        g = abi.AdjointGrads()
+        if "position" in want:
+            out["position"] = torch.empty((3, n), dtype=torch.float64, device=dev)
+            g.position = abi.Vector3(n, *(abi.dptr(ptr(out["position"][k])) for k in range(3)))
+        if "velocity" in want:
+            out["velocity"] = torch.empty((3, n), dtype=torch.float64, device=dev)
+            g.velocity = abi.Vector3(n, *(abi.dptr(ptr(out["velocity"][k])) for k in range(3)))
+        if "pressure" in want:
+            out["pressure"] = torch.empty(n, dtype=torch.float64, device=dev)
+            g.pressure = abi.dptr(ptr(out["pressure"]))
+        fs = self.state.fluid_soa()
+        ws = self.state.wall_soa()
+        rc = self.lib.lsmps_adjoint_step(self.handle, C.byref(fs), C.byref(ws), C.byref(gvs),
+                                         C.byref(gxs) if gxs is not None else None, C.byref(g))
+        if rc != abi.LSMPS_OK and (rc != abi.LSMPS_ERR_NOT_CONVERGED or raise_on_nonconvergence):
+            check(rc, "adjoint_step")
+        out["gravity"] = np.array(list(g.gravity))
+        out["report"] = g.transpose_report.as_dict()
+        return out
 
     def step(self, raise_on_nonconvergence=True):
         rep = abi.SolveReport()
