@@ -39,6 +39,8 @@ def parse():
     p.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
     p.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    p.add_argument("--no-adjoint", action="store_true",
+                   help="skip timing the adjoint of one step (after the timed steps)")
     p.add_argument("--cpu-threads", type=int, default=0)
     p.add_argument("--cpu-solver-iterations", type=int, default=40)
     p.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
@@ -61,7 +63,7 @@ def make_case(cases, name):
     raise SystemExit(f"unknown case {name}")
 
 
-def run_gpu(pkg, rt, case, precision, steps, warmup, timing=True):
+def run_gpu(pkg, rt, case, precision, steps, warmup, timing=True, adjoint=False):
     import numpy as np
     import torch
     abi = pkg.abi
@@ -92,6 +94,22 @@ def run_gpu(pkg, rt, case, precision, steps, warmup, timing=True):
     rt.check(rt.load().lsmps_context_kernel_stats(ctx.handle, C.byref(launches), C.byref(ms),
                                                   C.byref(nbytes), 0))
     pressure = state.fluid["pressure"].cpu().numpy()
+    adj = None
+    if adjoint:
+        # one recorded step, then the reverse mode of it (outside the timed region)
+        st.set_mode(abi.STEP_RECORD)
+        state.restore(snap)
+        st.step(raise_on_nonconvergence=False)
+        f = state.fluid
+        vbar = 2.0 * torch.stack([f["vx"], f["vy"], f["vz"]])
+        st.adjoint(vbar, raise_on_nonconvergence=False)  # allocates the adjoint workspace
+        torch.cuda.synchronize()
+        ta = time.perf_counter()
+        res = st.adjoint(vbar, raise_on_nonconvergence=False)
+        torch.cuda.synchronize()
+        adj = {"ms": (time.perf_counter() - ta) * 1e3,
+               "transpose_iterations": res["report"]["iterations"],
+               "transpose_relative_residual": res["report"]["relative_residual"]}
     out = {
         "ms_per_step": (t1 - t0) * 1e3 / steps,
         "stage_ms": (stage / steps).tolist(),
@@ -101,6 +119,7 @@ def run_gpu(pkg, rt, case, precision, steps, warmup, timing=True):
         "spmv_bytes": nbytes.value,
         "pressure": pressure,
         "bytes": st.bytes(),
+        "adjoint": adj,
     }
     st.close()
     ctx.close()
@@ -212,7 +231,8 @@ def main():
     prec = abi.PRECISION_MIXED if args.precision == "mixed" else abi.PRECISION_FP64
     print(f"[bench] case {args.case}: {case.n_fluid} fluid, {case.n_wall} wall particles; "
           f"grid {list(case.config.cell_dims)}", file=sys.stderr, flush=True)
-    main_run = run_gpu(pkg, rt, case, prec, args.steps, args.warmup)
+    main_run = run_gpu(pkg, rt, case, prec, args.steps, args.warmup,
+                       adjoint=not args.no_adjoint)
     iters = [r["iterations"] for r in main_run["reports"]]
     print(f"[bench] {args.precision}: {main_run['ms_per_step']:.1f} ms/step, iterations {iters}, "
           f"stages {np.round(main_run['stage_ms'], 2).tolist()}", file=sys.stderr, flush=True)
@@ -259,6 +279,9 @@ def main():
                      "avg_launch_ms": avg_ms, "algorithmic_bytes_per_launch": avg_bytes,
                      "launches": main_run["spmv_launches"]},
     }
+    if main_run["adjoint"] is not None:
+        line["adjoint"] = main_run["adjoint"]
+        print(f"[bench] adjoint of one step: {main_run['adjoint']}", file=sys.stderr, flush=True)
     if fp64_run is not None:
         line["fp64_ms_per_step"] = fp64_run["ms_per_step"]
         line["fp64_iterations"] = [r["iterations"] for r in fp64_run["reports"]]
