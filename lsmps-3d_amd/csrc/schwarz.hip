@@ -259,6 +259,154 @@ __global__ void __launch_bounds__(kMaxBlock) factor_kernel(
   }
 }
 
+// Mixed mode: the block inverse by Gauss-Jordan elimination with partial
+// pivoting in FP32, the whole block held in registers.  1024 threads; thread
+// (ty, tx) = (t % 32, t / 32) owns rows ty + 32 a and columns tx + 32 b
+// (a, b < 8), 64 entries.  Step k: the half-wave owning column k picks the
+// pivot row p (largest |a_rk|, r >= k, ties to the smaller row) and publishes
+// column k, the owners of rows k and p publish those rows (LDS, double
+// buffered by k parity, so two barriers per step), then every thread updates
+// its 64 entries: row k becomes row p / pivot, every other row r loses
+// f_r times it, and column k takes -f_r / pivot (in-place inversion).  The
+// row interchanges become column interchanges of the result, applied when the
+// inverse is written out (column-major, like the FP64 path).  The block is
+// assembled into the output buffer first; nothing else is allocated.
+constexpr int kGjThreads = 1024;
+
+__global__ void __launch_bounds__(kGjThreads) gj_inverse_kernel(
+    const int32_t* rows, const int32_t* cols, const double* vals, const int32_t* members,
+    const int32_t* sizes, float* inverse) {
+  __shared__ int32_t list[kMaxBlock];
+  __shared__ int32_t piv[kMaxBlock];
+  __shared__ int32_t pos[kMaxBlock];
+  __shared__ float row_k[2][kMaxBlock];
+  __shared__ float row_p[2][kMaxBlock];
+  __shared__ float col_k[2][kMaxBlock];
+  __shared__ int32_t pivot_row[2];
+  constexpr int ld = kMaxBlock;
+  const int t = threadIdx.x;
+  const int n = sizes[blockIdx.x];
+  float* buf = inverse + static_cast<size_t>(blockIdx.x) * ld * ld;
+  if (t < kMaxBlock) list[t] = members[static_cast<size_t>(blockIdx.x) * kMaxBlock + t];
+  for (int e = t; e < ld * ld; e += kGjThreads) buf[e] = 0.0f;
+  __syncthreads();
+  {
+    const int r = t >> 2, q = t & 3;  // 4 threads per block row
+    if (r < n) {
+      const int32_t row = list[r];
+      for (int32_t c = rows[row] + q; c < rows[row + 1]; c += 4) {
+        const int p = find_member(list, n, cols[c]);
+        if (p >= 0) buf[static_cast<size_t>(p) * ld + r] = static_cast<float>(vals[c]);
+      }
+    }
+  }
+  __syncthreads();
+  const int ty = t & 31, tx = t >> 5;
+  float a[8][8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i)
+#pragma unroll
+    for (int j = 0; j < 8; ++j) a[i][j] = buf[static_cast<size_t>(tx + 32 * j) * ld + ty + 32 * i];
+  for (int k = 0; k < n; ++k) {
+    const int s = k & 1;
+    const int kb = k >> 5, kt = k & 31;
+    if (tx == kt) {
+      float best = -1.0f;
+      int best_r = INT32_MAX;
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        float v = 0.0f;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v = (j == kb) ? a[i][j] : v;
+        const int r = ty + 32 * i;
+        col_k[s][r] = v;
+        const float m = fabsf(v);
+        if (r >= k && r < n && m > best) {
+          best = m;
+          best_r = r;
+        }
+      }
+#pragma unroll
+      for (int off = 16; off > 0; off >>= 1) {
+        const float ob = __shfl_xor(best, off, 32);
+        const int orow = __shfl_xor(best_r, off, 32);
+        if (ob > best || (ob == best && orow < best_r)) {
+          best = ob;
+          best_r = orow;
+        }
+      }
+      if (ty == 0) {
+        pivot_row[s] = best_r < n ? best_r : k;
+        piv[k] = best_r < n ? best_r : k;
+      }
+    }
+    if (ty == kt) {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        float v = 0.0f;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) v = (i == kb) ? a[i][j] : v;
+        row_k[s][tx + 32 * j] = v;
+      }
+    }
+    __syncthreads();
+    const int p = pivot_row[s];
+    if (ty == (p & 31)) {
+      const int pb = p >> 5;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        float v = 0.0f;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) v = (i == pb) ? a[i][j] : v;
+        row_p[s][tx + 32 * j] = v;
+      }
+    }
+    __syncthreads();
+    float pivot = row_p[s][k];
+    if (pivot == 0.0f) pivot = 1.0e-30f;  // singular block: keep finite, Krylov absorbs it
+    const float ip = 1.0f / pivot;
+    float u[8], rk[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int col = tx + 32 * j;
+      u[j] = col == k ? ip : row_p[s][col] * ip;
+      rk[j] = row_k[s][col];
+    }
+    const float fk = row_k[s][k];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int r = ty + 32 * i;
+      const bool is_k = r == k, is_p = r == p;
+      const float f = is_p ? fk : col_k[s][r];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int col = tx + 32 * j;
+        const float src = is_p ? rk[j] : a[i][j];
+        const float v = col == k ? -f * ip : src - f * u[j];
+        a[i][j] = is_k ? u[j] : v;
+      }
+    }
+  }
+  __syncthreads();
+  if (t == 0) {
+    // column order of the inverse: undo the row interchanges in reverse
+    for (int c = 0; c < kMaxBlock; ++c) list[c] = c;
+    for (int k = n - 1; k >= 0; --k) {
+      const int q = piv[k];
+      const int32_t x = list[k];
+      list[k] = list[q];
+      list[q] = x;
+    }
+    for (int c = 0; c < kMaxBlock; ++c) pos[list[c]] = c;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < 8; ++i)
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      buf[static_cast<size_t>(pos[tx + 32 * j]) * ld + ty + 32 * i] = a[i][j];
+}
+
 // Number of blocks covering each row.
 __global__ void cover_count_kernel(const int32_t* members, const int32_t* sizes, int32_t* cover) {
   const int t = threadIdx.x;
@@ -392,13 +540,13 @@ void schwarz_setup(Context& ctx, SchwarzPrec& P, const lsmps_csr& A, const doubl
       P.sizes.data());
   LSMPS_LAUNCH_CHECK();
   const size_t dense = nb * kMaxBlock * kMaxBlock;
-  P.scratch.reserve(2 * dense);
   if (fp32_inverse) {
     P.inv32.reserve(dense);
-    factor_kernel<float><<<static_cast<unsigned>(nb), kMaxBlock, 0, ctx.stream>>>(
+    gj_inverse_kernel<<<static_cast<unsigned>(nb), kGjThreads, 0, ctx.stream>>>(
         A.row_offsets, A.col_indices, A.values, P.members.data(), P.sizes.data(),
-        P.scratch.data(), P.inv32.data());
+        P.inv32.data());
   } else {
+    P.scratch.reserve(2 * dense);
     P.inv64.reserve(dense);
     factor_kernel<double><<<static_cast<unsigned>(nb), kMaxBlock, 0, ctx.stream>>>(
         A.row_offsets, A.col_indices, A.values, P.members.data(), P.sizes.data(),
