@@ -481,16 +481,7 @@ __global__ void gather_kernel(size_t count, const int32_t* covered, const int32_
   const size_t k = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (k >= count) return;
   const int32_t i = covered[k];
-  const int32_t lo = start[i], hi = start[i + 1];
-  double z = 0.0;
-  for (int32_t e = lo; e < hi; ++e) {
-    const int32_t slot = entries[e];  // block * kMaxBlock + position
-    const size_t base = static_cast<size_t>(slot / kMaxBlock) * kApplyParts * kMaxBlock +
-                        static_cast<size_t>(slot % kMaxBlock);
-#pragma unroll
-    for (int p = 0; p < kApplyParts; ++p) z += ybuf[base + static_cast<size_t>(p) * kMaxBlock];
-  }
-  v[i] = static_cast<T>(z / static_cast<double>(hi - lo));
+  v[i] = static_cast<T>(schwarz_row_mean(start, entries, ybuf, i));
 }
 
 // pdiag and the flags of covered rows.
@@ -587,7 +578,7 @@ void schwarz_setup(Context& ctx, SchwarzPrec& P, const lsmps_csr& A, const doubl
 }
 
 template <typename T>
-void schwarz_apply(Context& ctx, const SchwarzPrec& P, T* v) {
+void schwarz_products(Context& ctx, const SchwarzPrec& P, const T* v) {
   if (P.blocks == 0) return;
   const unsigned grid = static_cast<unsigned>(P.blocks * kApplyParts);
   if (P.fp32) {
@@ -597,6 +588,13 @@ void schwarz_apply(Context& ctx, const SchwarzPrec& P, T* v) {
     apply_kernel<T, double><<<grid, kMaxBlock, 0, ctx.stream>>>(P.members.data(), P.sizes.data(),
                                                                 P.inv64.data(), v, P.ybuf.data());
   }
+  LSMPS_LAUNCH_CHECK();
+}
+
+template <typename T>
+void schwarz_apply(Context& ctx, const SchwarzPrec& P, T* v) {
+  if (P.blocks == 0) return;
+  schwarz_products<T>(ctx, P, v);
   gather_kernel<T><<<blocks_for(std::max<size_t>(P.covered_count, 1)), kBlock, 0, ctx.stream>>>(
       P.covered_count, P.covered.data(), P.cover_start.data(), P.cover_entries.data(),
       P.ybuf.data(), v);
@@ -605,6 +603,8 @@ void schwarz_apply(Context& ctx, const SchwarzPrec& P, T* v) {
 
 template void schwarz_apply<double>(Context&, const SchwarzPrec&, double*);
 template void schwarz_apply<float>(Context&, const SchwarzPrec&, float*);
+template void schwarz_products<double>(Context&, const SchwarzPrec&, const double*);
+template void schwarz_products<float>(Context&, const SchwarzPrec&, const float*);
 
 void schwarz_stats(Context& ctx, const SchwarzPrec& P, int64_t* blocks, int64_t* members) {
   *blocks = static_cast<int64_t>(P.blocks);

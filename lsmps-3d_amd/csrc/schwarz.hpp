@@ -34,6 +34,22 @@ struct SchwarzPrec {
   }
 };
 
+// Value of covered row i: the mean of its blocks' products (each the sum of
+// its kApplyParts partials in ybuf), in ascending block order.
+__device__ __forceinline__ double schwarz_row_mean(const int32_t* start, const int32_t* entries,
+                                                   const double* ybuf, int32_t i) {
+  const int32_t lo = start[i], hi = start[i + 1];
+  double z = 0.0;
+  for (int32_t e = lo; e < hi; ++e) {
+    const int32_t slot = entries[e];  // block * kMaxBlock + position
+    const size_t base = static_cast<size_t>(slot / kMaxBlock) * kApplyParts * kMaxBlock +
+                        static_cast<size_t>(slot % kMaxBlock);
+#pragma unroll
+    for (int p = 0; p < kApplyParts; ++p) z += ybuf[base + static_cast<size_t>(p) * kMaxBlock];
+  }
+  return z / static_cast<double>(hi - lo);
+}
+
 // Builds the blocks of matrix A; dinv is its Jacobi diagonal 1 / A_ii.
 void schwarz_setup(Context& ctx, SchwarzPrec& P, const lsmps_csr& A, const double* dinv,
                    bool fp32_inverse);
@@ -42,6 +58,10 @@ void schwarz_setup(Context& ctx, SchwarzPrec& P, const lsmps_csr& A, const doubl
 // of its blocks' values (inv_b w_b), summed in block order.
 template <typename T>
 void schwarz_apply(Context& ctx, const SchwarzPrec& P, T* v);
+// The block products alone (into P.ybuf); a consumer that reads v row by row
+// can then take covered rows from schwarz_row_mean instead of the completion.
+template <typename T>
+void schwarz_products(Context& ctx, const SchwarzPrec& P, const T* v);
 void schwarz_stats(Context& ctx, const SchwarzPrec& P, int64_t* blocks, int64_t* members);
 
 }  // namespace lsmps

@@ -281,6 +281,48 @@ __device__ __forceinline__ int dia_find(const int32_t* list, int nd, int32_t d) 
 }
 
 constexpr int kDiaTable = 2048;  // LDS hash slots (power of two)
+// Stored entries of one row that a lane reads with one vector load: 4 values
+// (16 bytes in FP32) and 4 16-bit columns (8 bytes).  Slot k of a chunk part is
+// at (k / kQuad) * kQuad * kSellChunk + lane * kQuad + k % kQuad.
+constexpr int kQuad = 4;
+__device__ __forceinline__ int64_t quad_slot(int32_t k, int lane) {
+  return static_cast<int64_t>(k / kQuad) * (kQuad * kSellChunk) + lane * kQuad + k % kQuad;
+}
+typedef float quad_f32 __attribute__((ext_vector_type(4)));
+typedef double pair_f64 __attribute__((ext_vector_type(2)));
+typedef unsigned short quad_u16 __attribute__((ext_vector_type(4)));
+typedef int quad_i32 __attribute__((ext_vector_type(4)));
+// A lane's quad in one streaming load (two for FP64 values); p is aligned to
+// the quad's size by the layout (chunk parts start at multiples of kQuad rows).
+__device__ __forceinline__ void load_quad(const float* p, float v[kQuad]) {
+  const quad_f32 q = __builtin_nontemporal_load(reinterpret_cast<const quad_f32*>(p));
+  v[0] = q.x;
+  v[1] = q.y;
+  v[2] = q.z;
+  v[3] = q.w;
+}
+__device__ __forceinline__ void load_quad(const double* p, double v[kQuad]) {
+  const pair_f64 a = __builtin_nontemporal_load(reinterpret_cast<const pair_f64*>(p));
+  const pair_f64 b = __builtin_nontemporal_load(reinterpret_cast<const pair_f64*>(p) + 1);
+  v[0] = a.x;
+  v[1] = a.y;
+  v[2] = b.x;
+  v[3] = b.y;
+}
+__device__ __forceinline__ void load_quad(const uint16_t* p, uint16_t v[kQuad]) {
+  const quad_u16 q = __builtin_nontemporal_load(reinterpret_cast<const quad_u16*>(p));
+  v[0] = q.x;
+  v[1] = q.y;
+  v[2] = q.z;
+  v[3] = q.w;
+}
+__device__ __forceinline__ void load_quad(const int32_t* p, int32_t v[kQuad]) {
+  const quad_i32 q = __builtin_nontemporal_load(reinterpret_cast<const quad_i32*>(p));
+  v[0] = q.x;
+  v[1] = q.y;
+  v[2] = q.z;
+  v[3] = q.w;
+}
 constexpr int32_t kDiaEmpty = INT32_MIN;
 
 // One 64-thread block per chunk, lane = row: counts the offsets column - row
@@ -371,11 +413,19 @@ __global__ void __launch_bounds__(kSellChunk) dia_select_kernel(
     width = max(width, __shfl_xor(width, off, 64));
     total += __shfl_xor(total, off, 64);
   }
+  // Both parts are stored in quads (kQuad consecutive entries of one row per
+  // lane), so the diagonal count and the width are padded to multiples of
+  // kQuad.  Padding diagonals repeat the last offset (the list stays sorted for
+  // dia_find) and keep value 0.
+  const int nd4 = (nd + kQuad - 1) / kQuad * kQuad;
+  const int32_t w4 = (width + kQuad - 1) / kQuad * kQuad;
+  for (int i = nd + lane; i < nd4; i += kSellChunk)
+    dia_offsets[chunk * kMaxDia + i] = nd > 0 ? list[nd - 1] : 0;
   if (lane == 0) {
-    dia_count[chunk] = nd;
-    dia_slots[chunk] = nd * kSellChunk;
-    rem_width[chunk] = width;
-    rem_scaled[chunk] = width * kSellChunk;
+    dia_count[chunk] = nd4;
+    dia_slots[chunk] = nd4 * kSellChunk;
+    rem_width[chunk] = w4;
+    rem_scaled[chunk] = w4 * kSellChunk;
     atomicAdd(rem_total, total);
   }
 }
@@ -479,7 +529,7 @@ __global__ void sell_fill_kernel(size_t n, size_t chunks, const int32_t* rows,
   const int32_t len = r < n ? rows[r + 1] - b : 0;
   int32_t k = 0;  // next remainder slot
   auto put = [&](int32_t c, V v) {
-    const int64_t slot = static_cast<int64_t>(k) * kSellChunk + lane;
+    const int64_t slot = quad_slot(k, lane);
     out_vals[vbase + slot] = v;
     if (nw < 0) {
       reinterpret_cast<int32_t*>(out_cols + cbase)[slot] = c;
@@ -494,7 +544,7 @@ __global__ void sell_fill_kernel(size_t n, size_t chunks, const int32_t* rows,
     const int32_t c = cols[b + q];
     const int j = dia_find(dl, nd, c - static_cast<int32_t>(r));
     if (j >= 0) {
-      dia_vals[dbase + static_cast<int64_t>(j) * kSellChunk + lane] = static_cast<V>(vals[b + q]);
+      dia_vals[dbase + quad_slot(j, lane)] = static_cast<V>(vals[b + q]);
     } else {
       put(c, static_cast<V>(vals[b + q]));
     }
@@ -556,77 +606,82 @@ __global__ void __launch_bounds__(kSellBlock) sell_spmv_kernel(SellArgs<T, V> a)
     const int32_t last_row = static_cast<int32_t>(a.n) - 1;
     const int32_t w = a.width[chunk];
     const int32_t nw = a.nwin[chunk];  // uniform per wavefront
-    const V* vp = a.vals + a.chunk_start[chunk] + lane;
+    const V* vp = a.vals + a.chunk_start[chunk] + lane * kQuad;
     const int64_t cbase = static_cast<int64_t>(a.col_start[chunk]) * kSellChunk;
     // FP64 accumulation whatever the storage types: the SpMV is bandwidth bound.
-    constexpr int U = 8;
+    constexpr int U = 8;  // entries (two quads) in flight per lane
     double s[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) s[u] = 0.0;
     // DIA part: column = row + offset (clamped into range for padding slots,
-    // whose value is 0), so the wavefront's gathers are contiguous
+    // whose value is 0), so the wavefront's gathers are contiguous.  nd and w
+    // are multiples of kQuad; each lane reads its row's values a quad at a time.
     {
       const int nd = a.dia_count[chunk];
       const int32_t* dl = a.dia_offsets + chunk * kMaxDia;
-      const V* dp = a.dia_vals + a.dia_start[chunk] + lane;
+      const V* dp = a.dia_vals + a.dia_start[chunk] + lane * kQuad;
       int j = 0;
       for (; j + U <= nd; j += U) {
         V v[U];
         int32_t c[U];
+        load_quad(dp + j * kSellChunk, v);
+        load_quad(dp + (j + kQuad) * kSellChunk, v + kQuad);
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-          v[u] = __builtin_nontemporal_load(dp + (j + u) * kSellChunk);
-          c[u] = min(max(irow + dl[j + u], 0), last_row);
-        }
+        for (int u = 0; u < U; ++u) c[u] = min(max(irow + dl[j + u], 0), last_row);
 #pragma unroll
         for (int u = 0; u < U; ++u) s[u] += static_cast<double>(v[u]) * static_cast<double>(a.x[c[u]]);
       }
-      for (; j < nd; ++j) {
-        const int32_t c = min(max(irow + dl[j], 0), last_row);
-        s[0] += static_cast<double>(dp[j * kSellChunk]) * static_cast<double>(a.x[c]);
+      for (; j < nd; j += kQuad) {
+        V v[kQuad];
+        load_quad(dp + j * kSellChunk, v);
+#pragma unroll
+        for (int u = 0; u < kQuad; ++u) {
+          const int32_t c = min(max(irow + dl[j + u], 0), last_row);
+          s[u] += static_cast<double>(v[u]) * static_cast<double>(a.x[c]);
+        }
       }
     }
     int32_t k = 0;
     if (nw >= 0) {
       // lane l holds window base l % 16; an entry's base is fetched across lanes
       const int32_t wbase = a.windows[chunk * kSellWindows + (lane & (kSellWindows - 1))];
-      const uint16_t* cp = a.cols + cbase + lane;
+      const uint16_t* cp = a.cols + cbase + lane * kQuad;
       constexpr int kMask = (1 << kSellWindowBits) - 1;
       for (; k + U <= w; k += U) {
         uint16_t c[U];
         V v[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-          c[u] = __builtin_nontemporal_load(cp + (k + u) * kSellChunk);
-          v[u] = __builtin_nontemporal_load(vp + (k + u) * kSellChunk);
-        }
+        load_quad(cp + k * kSellChunk, c);
+        load_quad(cp + (k + kQuad) * kSellChunk, c + kQuad);
+        load_quad(vp + k * kSellChunk, v);
+        load_quad(vp + (k + kQuad) * kSellChunk, v + kQuad);
 #pragma unroll
         for (int u = 0; u < U; ++u) {
           const int32_t col = __shfl(wbase, c[u] >> kSellWindowBits, 64) + (c[u] & kMask);
           s[u] += static_cast<double>(v[u]) * static_cast<double>(a.x[col]);
         }
       }
-      for (; k < w; ++k) {
-        const uint16_t c = cp[k * kSellChunk];
-        const int32_t col = __shfl(wbase, c >> kSellWindowBits, 64) + (c & kMask);
-        s[0] += static_cast<double>(vp[k * kSellChunk]) * static_cast<double>(a.x[col]);
+      for (; k < w; k += kQuad) {
+        uint16_t c[kQuad];
+        V v[kQuad];
+        load_quad(cp + k * kSellChunk, c);
+        load_quad(vp + k * kSellChunk, v);
+#pragma unroll
+        for (int u = 0; u < kQuad; ++u) {
+          const int32_t col = __shfl(wbase, c[u] >> kSellWindowBits, 64) + (c[u] & kMask);
+          s[u] += static_cast<double>(v[u]) * static_cast<double>(a.x[col]);
+        }
       }
     } else {
-      const int32_t* cp = reinterpret_cast<const int32_t*>(a.cols + cbase) + lane;
-      for (; k + U <= w; k += U) {
-        int32_t c[U];
-        V v[U];
+      const int32_t* cp = reinterpret_cast<const int32_t*>(a.cols + cbase) + lane * kQuad;
+      for (; k < w; k += kQuad) {
+        int32_t c[kQuad];
+        V v[kQuad];
+        load_quad(cp + k * kSellChunk, c);
+        load_quad(vp + k * kSellChunk, v);
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-          c[u] = __builtin_nontemporal_load(cp + (k + u) * kSellChunk);
-          v[u] = __builtin_nontemporal_load(vp + (k + u) * kSellChunk);
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u)
+        for (int u = 0; u < kQuad; ++u)
           s[u] += static_cast<double>(v[u]) * static_cast<double>(a.x[c[u]]);
       }
-      for (; k < w; ++k)
-        s[0] += static_cast<double>(vp[k * kSellChunk]) * static_cast<double>(a.x[cp[k * kSellChunk]]);
     }
     const double sum = ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
     double ys = 0.0;
@@ -744,6 +799,10 @@ struct IdrVec {
   // Schwarz blocks cover, whose values the block application then replaces.
   const double* pdiag;
   const int32_t* row_seg;  // batched systems: each row's segment (scalars sc[row_seg[i]])
+  // Schwarz completion read by idr_a2g_kernel (null without blocks)
+  const int32_t* cover_start;
+  const int32_t* cover_entries;
+  const double* ybuf;
 };
 
 // Scalars that govern row i: the system's, or its segment's in a batched solve.
@@ -776,6 +835,23 @@ __global__ void __launch_bounds__(kBlock) idr_a2_kernel(IdrVec<T> a, int k) {
     double u = 0.0;
     for (int q = k; q < a.s; ++q) u += sc->c[q] * static_cast<double>(a.U[q * a.ld + i]);
     a.U[k * a.ld + i] = static_cast<T>(u + om * static_cast<double>(a.v[i]));
+  }
+}
+
+// idr_a2_kernel with the Schwarz completion fused: covered rows take their
+// blocks' mean (schwarz_row_mean) in place of v, which is not written back.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) idr_a2g_kernel(IdrVec<T> a, int k) {
+  const double om = a.sc->om;
+  for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < a.n;
+       i += static_cast<size_t>(gridDim.x) * blockDim.x) {
+    double u = 0.0;
+    for (int q = k; q < a.s; ++q) u += a.sc->c[q] * static_cast<double>(a.U[q * a.ld + i]);
+    double v = static_cast<double>(a.v[i]);
+    if (a.cover_start[i + 1] > a.cover_start[i])
+      v = static_cast<double>(static_cast<T>(
+          schwarz_row_mean(a.cover_start, a.cover_entries, a.ybuf, static_cast<int32_t>(i))));
+    a.U[k * a.ld + i] = static_cast<T>(u + om * v);
   }
 }
 
@@ -953,6 +1029,44 @@ __global__ void __launch_bounds__(kBlock) idr_b_kernel(IdrVec<T> a, int k, int w
     acc[0] += r * r;
     // dimension-reduction input: pdiag r (the Schwarz blocks then fill their rows)
     if (write_v) a.v[i] = static_cast<T>(a.pdiag[i] * r);
+  }
+  block_reduce_store<1>(acc, 1, a.partials);
+}
+
+// idr_b_kernel(k) fused with the first half of step k + 1 (both read only the
+// scalars idr_scalar_after(k) wrote): after the biorthogonalisation and the r, x
+// updates, v = pdiag (r - sum_{q>k} c_q G_q); without Schwarz blocks also
+// U_{k+1} = sum_{q>k} c_q U_q + om v (idr_a_kernel).
+template <typename T, bool kBlocks>
+__global__ void __launch_bounds__(kBlock) idr_ba_kernel(IdrVec<T> a, int k) {
+  double acc[1] = {0.0};
+  const SolverScalars* sc = a.sc;
+  const double beta = sc->beta, om = sc->om;
+  for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < a.n;
+       i += static_cast<size_t>(gridDim.x) * blockDim.x) {
+    double g = static_cast<double>(a.G[k * a.ld + i]);
+    double u = static_cast<double>(a.U[k * a.ld + i]);
+    for (int q = 0; q < k; ++q) {
+      const double al = sc->alpha[q];
+      g -= al * static_cast<double>(a.G[q * a.ld + i]);
+      u -= al * static_cast<double>(a.U[q * a.ld + i]);
+    }
+    a.G[k * a.ld + i] = static_cast<T>(g);
+    a.U[k * a.ld + i] = static_cast<T>(u);
+    const double r = a.r[i] - beta * g;
+    a.r[i] = r;
+    a.x[i] += beta * u;
+    acc[0] += r * r;
+    double v = r;
+    for (int q = k + 1; q < a.s; ++q) v -= sc->c[q] * static_cast<double>(a.G[q * a.ld + i]);
+    v *= a.pdiag[i];
+    if (kBlocks) {
+      a.v[i] = static_cast<T>(v);
+    } else {
+      double un = 0.0;
+      for (int q = k + 1; q < a.s; ++q) un += sc->c[q] * static_cast<double>(a.U[q * a.ld + i]);
+      a.U[(k + 1) * a.ld + i] = static_cast<T>(un + om * v);
+    }
   }
   block_reduce_store<1>(acc, 1, a.partials);
 }
@@ -1274,7 +1388,10 @@ int idr_solve(Context& ctx, PpeSolver& S, size_t n, const SellMatrix& M, const V
   const V* dia = sizeof(V) == sizeof(double) ? reinterpret_cast<const V*>(M.dia64.data())
                                              : reinterpret_cast<const V*>(M.dia32.data());
   IdrVec<T> iv{n, ld, s, r, x, v, t, G, U, p0, S.scalars.data(), S.part_b.data(),
-               schwarz ? S.schwarz.pdiag.data() : dinv, nullptr};
+               schwarz ? S.schwarz.pdiag.data() : dinv, nullptr,
+               schwarz ? S.schwarz.cover_start.data() : nullptr,
+               schwarz ? S.schwarz.cover_entries.data() : nullptr,
+               schwarz ? S.schwarz.ybuf.data() : nullptr};
   const unsigned vb = vec_blocks(n);
   const unsigned sb = sell_blocks(M.chunks);
   fill_kernel<T><<<blocks_for(2 * s * ld), kBlock, 0, ctx.stream>>>(2 * s * ld, G, T(0));
@@ -1312,13 +1429,16 @@ int idr_solve(Context& ctx, PpeSolver& S, size_t n, const SellMatrix& M, const V
     shadow_dots_kernel<T><<<vb, kBlock, 0, ctx.stream>>>(iv);
     idr_scalar_f<<<1, kReduceThreads, 0, ctx.stream>>>(S.scalars.data(), S.part_a.data(),
                                                static_cast<int>(vb), s);
+    // step 0's first half; later steps get it fused into the previous step's b
+    if (schwarz) {
+      idr_a1_kernel<T><<<vb, kBlock, 0, ctx.stream>>>(iv, 0);
+    } else {
+      idr_a_kernel<T><<<vb, kBlock, 0, ctx.stream>>>(iv, 0);
+    }
     for (int k = 0; k < s; ++k) {
       if (schwarz) {
-        idr_a1_kernel<T><<<vb, kBlock, 0, ctx.stream>>>(iv, k);
-        schwarz_apply<T>(ctx, S.schwarz, v);
-        idr_a2_kernel<T><<<vb, kBlock, 0, ctx.stream>>>(iv, k);
-      } else {
-        idr_a_kernel<T><<<vb, kBlock, 0, ctx.stream>>>(iv, k);
+        schwarz_products<T>(ctx, S.schwarz, v);
+        idr_a2g_kernel<T><<<vb, kBlock, 0, ctx.stream>>>(iv, k);
       }
       SellArgs<T, V> sa{n, M.chunks, M.dia_count.data(), M.dia_offsets.data(),
                         M.dia_start.data(), dia, M.chunk_start.data(), M.chunk_width.data(),
@@ -1330,7 +1450,13 @@ int idr_solve(Context& ctx, PpeSolver& S, size_t n, const SellMatrix& M, const V
       idr_scalar_after<<<1, kReduceThreads, 0, ctx.stream>>>(S.scalars.data(), S.part_a.data(),
                                                      static_cast<int>(sb), s, k);
       iv.partials = S.part_b.data();
-      idr_b_kernel<T><<<vb, kBlock, 0, ctx.stream>>>(iv, k, k == s - 1 ? 1 : 0);
+      if (k == s - 1) {
+        idr_b_kernel<T><<<vb, kBlock, 0, ctx.stream>>>(iv, k, 1);
+      } else if (schwarz) {
+        idr_ba_kernel<T, true><<<vb, kBlock, 0, ctx.stream>>>(iv, k);
+      } else {
+        idr_ba_kernel<T, false><<<vb, kBlock, 0, ctx.stream>>>(iv, k);
+      }
       ++it;
     }
     if (schwarz) {
