@@ -39,6 +39,8 @@ def parse():
     p.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
     p.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    p.add_argument("--shadow-dimension", type=int, default=0,
+                   help="IDR(s) shadow dimension (0: the library default)")
     p.add_argument("--no-adjoint", action="store_true",
                    help="skip timing the adjoint of one step (after the timed steps)")
     p.add_argument("--cpu-threads", type=int, default=0)
@@ -63,14 +65,17 @@ def make_case(cases, name):
     raise SystemExit(f"unknown case {name}")
 
 
-def run_gpu(pkg, rt, case, precision, steps, warmup, timing=True, adjoint=False):
+def run_gpu(pkg, rt, case, precision, steps, warmup, timing=True, adjoint=False, shadow=0):
     import numpy as np
     import torch
     abi = pkg.abi
     flags = abi.FLAG_KERNEL_TIMING if timing else 0
     ctx = rt.Context(precision, flags=flags)
     state = rt.DeviceState(case)
-    st = rt.Stepper(ctx, state, case.config, 256, abi.default_solver_options())
+    opts = abi.default_solver_options()
+    if shadow > 0:
+        opts.shadow_dimension = shadow
+    st = rt.Stepper(ctx, state, case.config, 256, opts)
     if "fluid_offsets" in case.extra:  # batched tanks
         st.set_segments(case.extra["fluid_offsets"], case.extra["wall_offsets"],
                         case.extra["grids"])
@@ -170,7 +175,7 @@ def cpu_baseline(pkg, case, gpu_iterations, threads, solver_iterations):
         "kind": "port",
         "sample": (f"FP64 oracle on the same {case.n_fluid}-particle workload: geometry, moments, "
                    f"provision, assembly, correction and the solver's Schwarz block setup timed "
-                   f"in full ({fixed:.2f} s); the same IDR(8) solver timed for {rep['iterations']} "
+                   f"in full ({fixed:.2f} s); the same IDR(s) solver timed for {rep['iterations']} "
                    f"iterations ({per_iter * 1e3:.1f} ms each) and scaled to the GPU's "
                    f"{gpu_iterations} iterations"),
         "stage_seconds": {k: round(v, 3) for k, v in t.items()},
@@ -232,13 +237,14 @@ def main():
     print(f"[bench] case {args.case}: {case.n_fluid} fluid, {case.n_wall} wall particles; "
           f"grid {list(case.config.cell_dims)}", file=sys.stderr, flush=True)
     main_run = run_gpu(pkg, rt, case, prec, args.steps, args.warmup,
-                       adjoint=not args.no_adjoint)
+                       adjoint=not args.no_adjoint, shadow=args.shadow_dimension)
     iters = [r["iterations"] for r in main_run["reports"]]
     print(f"[bench] {args.precision}: {main_run['ms_per_step']:.1f} ms/step, iterations {iters}, "
           f"stages {np.round(main_run['stage_ms'], 2).tolist()}", file=sys.stderr, flush=True)
     fp64_run = None
     if not args.no_fp64 and prec != abi.PRECISION_FP64:
-        fp64_run = run_gpu(pkg, rt, case, abi.PRECISION_FP64, max(1, min(args.steps, 2)), 0)
+        fp64_run = run_gpu(pkg, rt, case, abi.PRECISION_FP64, max(1, min(args.steps, 2)), 0,
+                           shadow=args.shadow_dimension)
         print(f"[bench] fp64: {fp64_run['ms_per_step']:.1f} ms/step, iterations "
               f"{[r['iterations'] for r in fp64_run['reports']]}", file=sys.stderr, flush=True)
     avg_ms = main_run["spmv_ms"] / max(main_run["spmv_launches"], 1)
@@ -264,7 +270,7 @@ def main():
                                 "C3": "64 batched tanks, 2k-60k fluid each, one launch per stage",
                                 "C5": "splash 160^3 (4,096,000 fluid), top 8 layers lifted"}[args.case],
                    "n_fluid": case.n_fluid, "n_wall": case.n_wall, "l0_m": case.config.particle_spacing,
-                   "re_over_l0": 3.1, "dt_s": case.config.time_step, "solver": "IDR(8); Schwarz blocks on wrong-sign rows, Jacobi elsewhere",
+                   "re_over_l0": 3.1, "dt_s": case.config.time_step, "solver": f"IDR({args.shadow_dimension or pkg.abi.default_solver_options().shadow_dimension}); Schwarz blocks on wrong-sign rows, Jacobi elsewhere",
                    "tolerance": 1e-10, "precision": args.precision},
         "solver_iterations": iters,
         "schwarz_blocks": main_run["reports"][-1]["schwarz_blocks"],

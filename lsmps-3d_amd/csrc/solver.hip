@@ -1757,7 +1757,7 @@ void ppe_solve(Context& ctx, PpeSolver& S, const lsmps_csr& A, const double* b, 
   const int max_it = opt.max_iterations > 0 ? opt.max_iterations : 20000;
   const size_t ld = (n + 63) / 64 * 64;
   int shadow_dim = opt.method == LSMPS_SOLVER_BICGSTAB ? 1 : opt.shadow_dimension;
-  if (shadow_dim <= 0) shadow_dim = 8;
+  if (shadow_dim <= 0) shadow_dim = 12;
   if (shadow_dim > kMaxShadow) shadow_dim = kMaxShadow;
   if (static_cast<size_t>(shadow_dim) > n) shadow_dim = static_cast<int>(n);
 
