@@ -104,17 +104,18 @@ __device__ void reduce_partials(const double* partials, int nb, double* out) {
   __shared__ double red[kReduceThreads];
   const int t = threadIdx.x;
   const int total = nb * kPartialStride;
-  // four independent loads in flight per thread
-  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  // eight independent loads in flight per thread
+  constexpr int L = 8;
+  double acc[L];
+#pragma unroll
+  for (int u = 0; u < L; ++u) acc[u] = 0.0;
   int e = t;
-  for (; e + 3 * kReduceThreads < total; e += 4 * kReduceThreads) {
-    s0 += partials[e];
-    s1 += partials[e + kReduceThreads];
-    s2 += partials[e + 2 * kReduceThreads];
-    s3 += partials[e + 3 * kReduceThreads];
+  for (; e + (L - 1) * kReduceThreads < total; e += L * kReduceThreads) {
+#pragma unroll
+    for (int u = 0; u < L; ++u) acc[u] += partials[e + u * kReduceThreads];
   }
-  for (; e < total; e += kReduceThreads) s0 += partials[e];
-  red[t] = (s0 + s1) + (s2 + s3);
+  for (; e < total; e += kReduceThreads) acc[0] += partials[e];
+  red[t] = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
   __syncthreads();
   for (int w = kReduceThreads / 2; w >= kPartialStride; w >>= 1) {
     if (t < w) red[t] += red[t + w];
@@ -895,42 +896,67 @@ __device__ void solve_c(SolverScalars* sc, int s, int k) {
   }
 }
 
+// The single-thread recurrences below work on an LDS copy of the system's
+// scalars (all threads copy in and out), not on global memory.
+static_assert(sizeof(SolverScalars) % sizeof(double) == 0, "copied as doubles");
+__device__ void scalars_copy(const SolverScalars* from, SolverScalars* to) {
+  const double* src = reinterpret_cast<const double*>(from);
+  double* dst = reinterpret_cast<double*>(to);
+  for (int e = threadIdx.x; e < static_cast<int>(sizeof(SolverScalars) / sizeof(double));
+       e += blockDim.x)
+    dst[e] = src[e];
+  __syncthreads();
+}
+
 // Cycle start: f = P^T r (from partials), c = M^-1 f.
 __global__ void __launch_bounds__(kReduceThreads)
-    idr_scalar_f(SolverScalars* sc, const double* partials, int nb, int s) {
-  sc += blockIdx.x;
+    idr_scalar_f(SolverScalars* sc_g, const double* partials, int nb, int s) {
+  sc_g += blockIdx.x;
   partials += static_cast<size_t>(blockIdx.x) * nb * kPartialStride;
   __shared__ double d[kPartialStride];
+  __shared__ SolverScalars lds;
   reduce_partials(partials, nb, d);
-  if (threadIdx.x != 0) return;
-  if (sc->done) {
-    freeze(sc);
-    return;
+  scalars_copy(sc_g, &lds);
+  SolverScalars* sc = &lds;
+  if (threadIdx.x == 0) {
+    if (sc->done) {
+      freeze(sc);
+    } else {
+      for (int q = 0; q < s; ++q) sc->f[q] = d[q];
+      solve_c(sc, s, 0);
+    }
   }
-  for (int q = 0; q < s; ++q) sc->f[q] = d[q];
-  solve_c(sc, s, 0);
+  __syncthreads();
+  scalars_copy(&lds, sc_g);
 }
 
 // After G_k = A U_k: biorthogonalisation coefficients, new column k of M,
 // beta, f update and c for the next k.
+__device__ void scalar_after_body(SolverScalars* sc, const double* d, int s, int k);
+
 __global__ void __launch_bounds__(kReduceThreads)
-    idr_scalar_after(SolverScalars* sc, const double* partials, int nb, int s, int k) {
-  sc += blockIdx.x;
+    idr_scalar_after(SolverScalars* sc_g, const double* partials, int nb, int s, int k) {
+  sc_g += blockIdx.x;
   partials += static_cast<size_t>(blockIdx.x) * nb * kPartialStride;
   __shared__ double d[kPartialStride];
+  __shared__ SolverScalars lds;
   reduce_partials(partials, nb, d);
-  if (threadIdx.x != 0) return;
-  if (sc->done) {
-    freeze(sc);
-    return;
+  scalars_copy(sc_g, &lds);
+  if (threadIdx.x == 0) {
+    if (lds.done) freeze(&lds);
+    else scalar_after_body(&lds, d, s, k);
   }
-  double alpha[kMaxShadow];
+  __syncthreads();
+  scalars_copy(&lds, sc_g);
+}
+
+__device__ void scalar_after_body(SolverScalars* sc, const double* d, int s, int k) {
+  double* alpha = sc->alpha;
   for (int i = 0; i < k; ++i) {
     double t = d[i];
     for (int l = 0; l < i; ++l) t -= alpha[l] * sc->M[i * kMaxShadow + l];
     const double mii = sc->M[i * kMaxShadow + i];
     alpha[i] = mii != 0.0 ? t / mii : 0.0;
-    sc->alpha[i] = alpha[i];
   }
   for (int i = k; i < s; ++i) {
     double t = d[i];

@@ -34,13 +34,19 @@ struct SolverScalars {
 //
 // DIA part: the offsets d = column - row that at least kDiaMinRows of the
 // chunk's rows have (particles in lattice or cell order share the ~123 offsets
-// of the 7x7x7 neighbourhood) are listed per chunk, ascending; their values sit
-// at dia_start[c] + j * 64 + lane, zero where a row lacks offset j.  Columns are
-// implicit, and the gathers x[row + d] of a wavefront are contiguous.
+// of the 7x7x7 neighbourhood) are listed per chunk, ascending, padded to a
+// multiple of 4 by repeating the last offset; zero where a row lacks offset j.
+// Columns are implicit, and the gathers x[row + d] of a wavefront are
+// contiguous.
 //
 // Remainder: every other entry, in row-sorted order, padded to the chunk's
-// widest remainder row (value 0): entry k of the chunk's row r sits at
-// chunk_start[c] + k * 64 + r.  Its column indices are stored in 16 bits where
+// widest remainder row rounded up to a multiple of 4 (value 0).
+//
+// Both parts are stored in per-lane quads so that a lane reads 4 entries of its
+// row with one 16-byte load (8 bytes for 16-bit columns): entry k of the chunk's
+// row r sits at start + (k / 4) * 256 + r * 4 + k % 4, with start = dia_start[c]
+// for diagonal k, chunk_start[c] for remainder entry k, and the same slot
+// numbering for the remainder columns.  Its column indices are stored in 16 bits where
 // the chunk allows it: the chunk's columns are covered by at most kSellWindows
 // windows [base_j, base_j + 4096), each two aligned 2048-column buckets wide
 // and laid greedily over the buckets the chunk touches, and an entry holds
