@@ -486,6 +486,85 @@ int lsmps_stepper_correction_vectors(lsmps_stepper* stepper, lsmps_vector3* pres
                                      lsmps_vector3* ps_displacement);
 lsmps_moment_matrix* lsmps_stepper_moments(lsmps_stepper* stepper);
 
+/* ---- multi-step driver --------------------------------------------------- */
+/* TimeStepLimits (include/lsmps3d/core/time_step.hpp:8-17). */
+typedef struct lsmps_time_limits {
+  double current_time_step;
+  double min_time_step;
+  double max_time_step;
+  double growth_factor;
+  double cfl;
+  double particle_spacing;
+  double final_time;
+  double output_interval;
+} lsmps_time_limits;
+
+/* TimeStepStatus (time_step.hpp:19-29); the two flags are 0 / 1. */
+typedef struct lsmps_time_status {
+  size_t step_index;
+  size_t output_index;
+  double previous_time;
+  double current_time;
+  double time_step;
+  double cfl_time_step;
+  double max_velocity;
+  int32_t should_output;
+  int32_t reached_final_time;
+} lsmps_time_status;
+
+/* SimulationTimeManager (time_step.hpp:31-82, src/core/time_step.cpp:47-172):
+ * a host-side clock, no device work. */
+typedef struct lsmps_time_manager lsmps_time_manager;
+/* make_time_step_limits (time_step.cpp:47-58) */
+int lsmps_time_limits_from_config(const lsmps_config* config, lsmps_time_limits* out);
+/* validate_time_step_limits (time_step.cpp:60-77): LSMPS_ERR_INVALID_ARGUMENT
+ * with the reference's message where it would throw std::invalid_argument. */
+int lsmps_time_limits_validate(const lsmps_time_limits* limits);
+/* SimulationTimeManager(limits) / reset(limits) (time_step.cpp:79-100) */
+int lsmps_time_manager_create(const lsmps_time_limits* limits, lsmps_time_manager** out);
+int lsmps_time_manager_destroy(lsmps_time_manager* clock);
+int lsmps_time_manager_reset(lsmps_time_manager* clock, const lsmps_time_limits* limits);
+/* current_time / current_time_step / step_index / output_index / output_due /
+ * finished (time_step.hpp:44-64, time_step.cpp:102-104); NULL outputs are skipped. */
+int lsmps_time_manager_state(const lsmps_time_manager* clock, double* current_time,
+                             double* current_time_step, size_t* step_index,
+                             size_t* output_index, int32_t* output_due, int32_t* finished);
+/* cfl_limited_time_step / next_time_step (time_step.cpp:106-124) */
+int lsmps_time_manager_next_time_step(const lsmps_time_manager* clock, double max_velocity,
+                                      double* time_step, double* cfl_time_step);
+/* advance / mark_initial_output (time_step.cpp:126-170) */
+int lsmps_time_manager_advance(lsmps_time_manager* clock, double max_velocity,
+                               lsmps_time_status* status);
+int lsmps_time_manager_mark_initial_output(lsmps_time_manager* clock, lsmps_time_status* status);
+
+/* max_velocity (src/app/simulation_driver.cu:122-143): the largest |v| of the
+ * fluid, non-finite magnitudes skipped, as a device reduction (one double read
+ * back). */
+int lsmps_max_velocity(lsmps_context* ctx, const lsmps_fluid_soa* fluid, double* out);
+
+/* Output hook of lsmps_stepper_run: called for the initial state and after every
+ * step whose status has should_output or reached_final_time, with the fluid
+ * buffers holding that state (lists and labels rebuilt at its positions).
+ * Returning non-zero stops the run. */
+typedef int (*lsmps_output_callback)(const lsmps_time_status* status, void* user);
+
+/* The driver loop (run_simulation, simulation_driver.cu:340-391) on the
+ * stepper: while the clock has not reached final_time (and at most max_steps
+ * steps, < 0 for no limit): max velocity, clock advance, config.time_step = dt,
+ * one step, and on output steps a rebuild of the neighbor lists and labels at
+ * the new positions (the reference's end-of-step rebuild_geometry) followed by
+ * the callback.  The clock's initial output, if still pending, is reported
+ * first.  `last` (may be NULL) receives the last status.  Returns
+ * LSMPS_ERR_NOT_CONVERGED when some step's solve missed the tolerance (the run
+ * continues with that pressure, as the reference does). */
+int lsmps_stepper_run(lsmps_stepper* stepper, const lsmps_fluid_soa* fluid,
+                      const lsmps_wall_soa* walls, lsmps_time_manager* clock, int64_t max_steps,
+                      lsmps_output_callback callback, void* user, lsmps_time_status* last);
+/* rebuild_geometry (simulation_driver.cu:252-264) at the fluid's current
+ * positions: neighbor lists and labels (fluid.surface_type), generation + 1. */
+int lsmps_stepper_refresh_geometry(lsmps_stepper* stepper, const lsmps_fluid_soa* fluid,
+                                   const lsmps_wall_soa* walls);
+
 /* ---- adjoint of one step (no reference counterpart) -------------------- */
 /* Stepper modes (bit flags, default 0):
  *  LSMPS_STEP_RECORD           keep what the adjoint needs and the step

@@ -156,6 +156,27 @@ class SolveReport(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class TimeLimits(C.Structure):
+    _fields_ = [("current_time_step", C.c_double), ("min_time_step", C.c_double),
+                ("max_time_step", C.c_double), ("growth_factor", C.c_double), ("cfl", C.c_double),
+                ("particle_spacing", C.c_double), ("final_time", C.c_double),
+                ("output_interval", C.c_double)]
+
+
+class TimeStatus(C.Structure):
+    _fields_ = [("step_index", C.c_size_t), ("output_index", C.c_size_t),
+                ("previous_time", C.c_double), ("current_time", C.c_double),
+                ("time_step", C.c_double), ("cfl_time_step", C.c_double),
+                ("max_velocity", C.c_double), ("should_output", C.c_int32),
+                ("reached_final_time", C.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+OutputCallback = C.CFUNCTYPE(C.c_int, C.POINTER(TimeStatus), C.c_void_p)
+
+
 class AdjointGrads(C.Structure):
     _fields_ = [("position", Vector3), ("velocity", Vector3), ("pressure", _dp),
                 ("gravity", C.c_double * 3), ("transpose_report", SolveReport)]

@@ -28,6 +28,7 @@ SOURCES = {
     "correction.hip": ["-ffp-contract=fast"],
     "schwarz.hip": ["-ffp-contract=fast"],
     "adjoint.hip": ["-ffp-contract=fast"],
+    "driver.hip": ["-ffp-contract=off"],
     "stepper.cpp": [],
     "capi.cpp": [],
 }

@@ -610,6 +610,110 @@ int lsmps_stepper_set_segments(lsmps_stepper* s, int32_t segment_count,
 
 size_t lsmps_stepper_bytes(const lsmps_stepper* s) { return s ? s->bytes() : 0; }
 
+int lsmps_time_limits_from_config(const lsmps_config* config, lsmps_time_limits* out) {
+  API_BEGIN
+  require(out != nullptr, "null output");
+  *out = time_limits_from_config(deref(config, "config"));
+  API_END("lsmps_time_limits_from_config")
+}
+
+int lsmps_time_limits_validate(const lsmps_time_limits* limits) {
+  API_BEGIN
+  validate_time_limits(deref(limits, "limits"));
+  API_END("lsmps_time_limits_validate")
+}
+
+int lsmps_time_manager_create(const lsmps_time_limits* limits, lsmps_time_manager** out) {
+  API_BEGIN
+  require(out != nullptr, "null output");
+  auto clock = std::make_unique<lsmps_time_manager>();
+  clock->reset(deref(limits, "limits"));
+  *out = clock.release();
+  API_END("lsmps_time_manager_create")
+}
+
+int lsmps_time_manager_destroy(lsmps_time_manager* clock) {
+  delete clock;
+  return LSMPS_OK;
+}
+
+int lsmps_time_manager_reset(lsmps_time_manager* clock, const lsmps_time_limits* limits) {
+  API_BEGIN
+  require(clock != nullptr, "null clock");
+  clock->reset(deref(limits, "limits"));
+  API_END("lsmps_time_manager_reset")
+}
+
+int lsmps_time_manager_state(const lsmps_time_manager* clock, double* current_time,
+                             double* current_time_step, size_t* step_index,
+                             size_t* output_index, int32_t* output_due, int32_t* finished) {
+  API_BEGIN
+  require(clock != nullptr, "null clock");
+  if (current_time) *current_time = clock->current_time;
+  if (current_time_step) *current_time_step = clock->current_time_step;
+  if (step_index) *step_index = clock->step_index;
+  if (output_index) *output_index = clock->output_index;
+  if (output_due) *output_due = clock->output_due() ? 1 : 0;
+  if (finished) *finished = clock->finished() ? 1 : 0;
+  API_END("lsmps_time_manager_state")
+}
+
+int lsmps_time_manager_next_time_step(const lsmps_time_manager* clock, double max_velocity,
+                                      double* time_step, double* cfl_time_step) {
+  API_BEGIN
+  require(clock != nullptr, "null clock");
+  if (time_step) *time_step = clock->next_dt(max_velocity);
+  if (cfl_time_step) *cfl_time_step = clock->cfl_dt(max_velocity);
+  API_END("lsmps_time_manager_next_time_step")
+}
+
+int lsmps_time_manager_advance(lsmps_time_manager* clock, double max_velocity,
+                               lsmps_time_status* status) {
+  API_BEGIN
+  require(clock != nullptr && status != nullptr, "null argument");
+  *status = clock->advance(max_velocity);
+  API_END("lsmps_time_manager_advance")
+}
+
+int lsmps_time_manager_mark_initial_output(lsmps_time_manager* clock, lsmps_time_status* status) {
+  API_BEGIN
+  require(clock != nullptr && status != nullptr, "null argument");
+  *status = clock->mark_initial_output();
+  API_END("lsmps_time_manager_mark_initial_output")
+}
+
+int lsmps_max_velocity(lsmps_context* ctx, const lsmps_fluid_soa* fluid, double* out) {
+  API_BEGIN
+  require(out != nullptr, "null output");
+  Context& c = C(ctx);
+  *out = max_speed(c, deref(fluid, "fluid"), c.reduce_scratch);
+  API_END("lsmps_max_velocity")
+}
+
+int lsmps_stepper_run(lsmps_stepper* s, const lsmps_fluid_soa* fluid, const lsmps_wall_soa* walls,
+                      lsmps_time_manager* clock, int64_t max_steps,
+                      lsmps_output_callback callback, void* user, lsmps_time_status* last) {
+  API_BEGIN
+  require(s != nullptr && clock != nullptr, "null argument");
+  s->ctx->activate();
+  const int rc = run_steps(*s, deref(fluid, "fluid"), deref(walls, "walls"), *clock, max_steps,
+                           callback, user, last);
+  if (rc != LSMPS_OK) {
+    g_last_error = "lsmps_stepper_run: a PPE solve did not reach the tolerance";
+    return rc;
+  }
+  API_END("lsmps_stepper_run")
+}
+
+int lsmps_stepper_refresh_geometry(lsmps_stepper* s, const lsmps_fluid_soa* fluid,
+                                   const lsmps_wall_soa* walls) {
+  API_BEGIN
+  require(s != nullptr, "null stepper");
+  s->ctx->activate();
+  s->refresh_geometry(deref(fluid, "fluid"), deref(walls, "walls"));
+  API_END("lsmps_stepper_refresh_geometry")
+}
+
 int lsmps_stepper_set_mode(lsmps_stepper* s, uint32_t mode) {
   API_BEGIN
   require(s != nullptr, "null stepper");

@@ -114,6 +114,7 @@ struct Context {
   DeviceArray<int32_t> tmp_labels;   // unsorted cell id per particle
   DeviceArray<int32_t> tmp_primary;  // primary surface labels before the near-surface pass
   DeviceArray<char> tmp_points;      // sorted source points for the neighbor scan
+  DeviceArray<double> reduce_scratch;  // block maxima of lsmps_max_velocity
   int32_t* host_i32 = nullptr;   // pinned host scratch
   double* host_f64 = nullptr;
   // PPE SpMV timing (LSMPS_FLAG_KERNEL_TIMING)

@@ -115,6 +115,9 @@ struct lsmps_stepper {
              sums.bytes() + solver.bytes();
     }
   } adjoint;
+  lsmps::DeviceArray<double> speed_scratch;  // max-velocity block maxima
+
+  void refresh_geometry(const lsmps_fluid_soa& fluid, const lsmps_wall_soa& walls);
 
   ~lsmps_stepper();
   void allocate(size_t nf, size_t nw, size_t cells);
@@ -127,7 +130,34 @@ struct lsmps_stepper {
   lsmps_ppe_workspace ppe_workspace(size_t n) const;
 };
 
+// SimulationTimeManager (src/core/time_step.cpp:79-170); driver.hip.
+struct lsmps_time_manager {
+  lsmps_time_limits limits{};
+  double current_time = 0.0;
+  double current_time_step = 0.0;
+  double next_output_time = 0.0;
+  size_t step_index = 0;
+  size_t output_index = 0;
+  bool initial_output_pending = true;
+
+  void reset(const lsmps_time_limits& l);
+  double clamp_dt(double v) const;
+  double cfl_dt(double vmax) const;
+  double next_dt(double vmax) const;
+  bool finished() const { return current_time >= limits.final_time; }
+  bool output_due() const { return initial_output_pending || current_time >= next_output_time; }
+  lsmps_time_status advance(double vmax);
+  lsmps_time_status mark_initial_output();
+};
+
 namespace lsmps {
+// driver.hip
+lsmps_time_limits time_limits_from_config(const lsmps_config& c);
+void validate_time_limits(const lsmps_time_limits& l);
+double max_speed(Context& ctx, const lsmps_fluid_soa& fluid, DeviceArray<double>& scratch);
+int run_steps(lsmps_stepper& stepper, const lsmps_fluid_soa& fluid, const lsmps_wall_soa& walls,
+              lsmps_time_manager& clock, int64_t max_steps, lsmps_output_callback cb, void* user,
+              lsmps_time_status* last);
 // Per-particle rows of adjoint work arrays (adjoint.hip).
 constexpr int kAdjointWork = 142;
 // Reverse mode of the stepper's last (recorded) step; adjoint.hip.

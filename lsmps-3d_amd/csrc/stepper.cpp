@@ -116,7 +116,7 @@ size_t lsmps_stepper::bytes() const {
          f_idx.bytes() + w_off.bytes() + w_idx.bytes() + moments.bytes() + tmp_fluid.bytes() +
          tmp_wall.bytes() + ppe_rows.bytes() + ppe_cols.bytes() + ppe_vals.bytes() +
          ppe_vec.bytes() + solver.bytes() + corr.bytes() + start_state.bytes() +
-         raw_pressure.bytes() + adjoint.bytes();
+         raw_pressure.bytes() + adjoint.bytes() + speed_scratch.bytes();
 }
 
 lsmps_neighbor_list lsmps_stepper::fluid_neighbors() const {

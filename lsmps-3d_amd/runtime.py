@@ -31,6 +31,11 @@ EXPORTS = [
     "lsmps_stepper_ppe_workspace", "lsmps_stepper_temporary_velocity",
     "lsmps_stepper_correction_vectors", "lsmps_stepper_moments", "lsmps_context_kernel_stats",
     "lsmps_stepper_set_segments", "lsmps_stepper_set_mode", "lsmps_adjoint_step",
+    "lsmps_time_limits_from_config", "lsmps_time_limits_validate", "lsmps_time_manager_create",
+    "lsmps_time_manager_destroy", "lsmps_time_manager_reset", "lsmps_time_manager_state",
+    "lsmps_time_manager_next_time_step", "lsmps_time_manager_advance",
+    "lsmps_time_manager_mark_initial_output", "lsmps_max_velocity", "lsmps_stepper_run",
+    "lsmps_stepper_refresh_geometry",
 ]
 
 _LIB = None
@@ -61,6 +66,11 @@ def load(path=LIB_PATH):
                                                C.c_int]
     lib.lsmps_reference_number_density.argtypes = [C.c_double, C.c_double, C.c_int32,
                                                    C.POINTER(C.c_double)]
+    lib.lsmps_time_manager_advance.argtypes = [C.c_void_p, C.c_double, C.c_void_p]
+    lib.lsmps_time_manager_next_time_step.argtypes = [C.c_void_p, C.c_double, C.c_void_p,
+                                                      C.c_void_p]
+    lib.lsmps_stepper_run.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                      C.c_int64, abi.OutputCallback, C.c_void_p, C.c_void_p]
     _LIB = lib
     return lib
 
@@ -94,6 +104,62 @@ def derive_cell_grid(config, fluid, walls):
                                         *(a.ctypes.data for a in w), w[0].shape[0],
                                         C.byref(cfg)))
     return cfg
+
+
+class TimeManager:
+    """SimulationTimeManager over the C-ABI (host-side clock, no GPU needed)."""
+
+    def __init__(self, config=None, limits=None):
+        self.lib = load()
+        if limits is None:
+            limits = abi.TimeLimits()
+            check(self.lib.lsmps_time_limits_from_config(C.byref(config), C.byref(limits)),
+                  "time_limits_from_config")
+        self.limits = limits
+        self.handle = C.c_void_p()
+        check(self.lib.lsmps_time_manager_create(C.byref(limits), C.byref(self.handle)),
+              "time_manager_create")
+
+    @staticmethod
+    def validate(limits):
+        check(load().lsmps_time_limits_validate(C.byref(limits)), "time_limits_validate")
+
+    def advance(self, max_velocity):
+        st = abi.TimeStatus()
+        check(self.lib.lsmps_time_manager_advance(self.handle, float(max_velocity), C.byref(st)))
+        return st.as_dict()
+
+    def mark_initial_output(self):
+        st = abi.TimeStatus()
+        check(self.lib.lsmps_time_manager_mark_initial_output(self.handle, C.byref(st)))
+        return st.as_dict()
+
+    def next_time_step(self, max_velocity):
+        dt, cdt = C.c_double(), C.c_double()
+        check(self.lib.lsmps_time_manager_next_time_step(self.handle, float(max_velocity),
+                                                         C.byref(dt), C.byref(cdt)))
+        return dt.value, cdt.value
+
+    def state(self):
+        t, dt = C.c_double(), C.c_double()
+        si, oi = C.c_size_t(), C.c_size_t()
+        due, fin = C.c_int32(), C.c_int32()
+        check(self.lib.lsmps_time_manager_state(self.handle, C.byref(t), C.byref(dt), C.byref(si),
+                                                C.byref(oi), C.byref(due), C.byref(fin)))
+        return {"current_time": t.value, "current_time_step": dt.value, "step_index": si.value,
+                "output_index": oi.value, "output_due": bool(due.value),
+                "finished": bool(fin.value)}
+
+    def close(self):
+        if self.handle:
+            self.lib.lsmps_time_manager_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # ---- device plumbing --------------------------------------------------------
@@ -195,6 +261,30 @@ class Stepper:
 
     def set_solver(self, options):
         check(self.lib.lsmps_stepper_set_solver(self.handle, C.byref(options)), "set_solver")
+
+    def run(self, clock, max_steps=-1, on_output=None):
+        """lsmps_stepper_run: the driver loop until the clock's final time (or
+        max_steps); on_output(status_dict) is called for every output state and
+        may return True to stop.  Returns (last status, return code)."""
+        def cb(status, user):
+            if on_output is None:
+                return 0
+            return 1 if on_output(status.contents.as_dict()) else 0
+        fn = abi.OutputCallback(cb)
+        last = abi.TimeStatus()
+        fs = self.state.fluid_soa()
+        ws = self.state.wall_soa()
+        rc = self.lib.lsmps_stepper_run(self.handle, C.byref(fs), C.byref(ws), clock.handle,
+                                        C.c_int64(max_steps), fn, None, C.byref(last))
+        if rc not in (abi.LSMPS_OK, abi.LSMPS_ERR_NOT_CONVERGED):
+            check(rc, "stepper_run")
+        return last.as_dict(), rc
+
+    def refresh_geometry(self):
+        fs = self.state.fluid_soa()
+        ws = self.state.wall_soa()
+        check(self.lib.lsmps_stepper_refresh_geometry(self.handle, C.byref(fs), C.byref(ws)),
+              "refresh_geometry")
 
     def set_config(self, config):
         self.config = config.copy()
@@ -316,6 +406,14 @@ class Stepper:
             self.close()
         except Exception:
             pass
+
+
+def max_velocity(ctx, state):
+    """lsmps_max_velocity: largest |v| of the fluid on the device."""
+    out = C.c_double()
+    fs = state.fluid_soa()
+    check(ctx.lib.lsmps_max_velocity(ctx.handle, C.byref(fs), C.byref(out)), "max_velocity")
+    return out.value
 
 
 def device_to_numpy(p, n, dtype):

@@ -114,6 +114,24 @@ def main():
             "expected_x_center": 0.125,
             "tolerance": 2e-3,
         },
+        "time_step": {
+            "source": "tests/unit/smoke_time_step.cpp:24-98",
+            "config": {"particle_spacing": 0.02, "time_step": 0.001, "min_time_step": 0.0001,
+                       "max_time_step": 0.01, "time_step_growth_factor": 2.0, "cfl": 0.25,
+                       "final_time": 0.012, "output_interval": 0.004},
+            "initial_output_due": True,
+            "initial_output_index": 1,
+            "advances": [
+                {"max_velocity": 2.0, "expected_dt": 0.002, "should_output": False,
+                 "reached_final_time": False, "label": "growth-limited dt"},
+                {"max_velocity": 2.0, "expected_dt": 0.0025, "should_output": True,
+                 "output_index": 2, "label": "CFL-limited dt"},
+                {"max_velocity": 1000.0, "expected_dt": 0.0001, "label": "minimum-clamped dt"},
+            ],
+            "then_max_velocity": 0.0, "guard": 20, "expected_final_time": 0.012,
+            "tolerance": 1e-12,
+            "invalid_growth_factor": 0.5,
+        },
     }
     with open(os.path.join(HERE, "reference_unit_tests.json"), "w") as fh:
         json.dump(fixtures, fh, indent=1)
