@@ -735,4 +735,116 @@ inline void smooth_particle_velocity(const FluidParticleSoA& fluid,
                 "smooth_particle_velocity");
 }
 
+// ---- simulation clock (core/time_step.hpp) -------------------------------
+using TimeStepLimits = lsmps_time_limits;
+
+// TimeStepStatus (time_step.hpp:19-29) with the reference's bool flags.
+struct TimeStepStatus {
+  size_type step_index{};
+  size_type output_index{};
+  real previous_time{};
+  real current_time{};
+  real time_step{};
+  real cfl_time_step{};
+  real max_velocity{};
+  bool should_output{};
+  bool reached_final_time{};
+};
+
+inline TimeStepLimits make_time_step_limits(const SimulationConfig& config) noexcept {
+  const lsmps_config c = detail::to_c(config);
+  TimeStepLimits l{};
+  (void)lsmps_time_limits_from_config(&c, &l);
+  return l;
+}
+// Throws std::invalid_argument, as the reference (time_step.cpp:60-77).
+inline void validate_time_step_limits(const TimeStepLimits& limits) {
+  if (lsmps_time_limits_validate(&limits) != LSMPS_OK) throw std::invalid_argument(lsmps_last_error());
+}
+
+// SimulationTimeManager (time_step.hpp:31-82) over the library's clock.
+class SimulationTimeManager {
+ public:
+  SimulationTimeManager() = default;
+  explicit SimulationTimeManager(const TimeStepLimits& limits) { reset(limits); }
+  explicit SimulationTimeManager(const SimulationConfig& config) { reset(config); }
+  ~SimulationTimeManager() {
+    if (clock_) lsmps_time_manager_destroy(clock_);
+  }
+  SimulationTimeManager(const SimulationTimeManager&) = delete;
+  SimulationTimeManager& operator=(const SimulationTimeManager&) = delete;
+
+  void reset(const TimeStepLimits& limits) {
+    validate_time_step_limits(limits);
+    if (clock_) {
+      detail::check(lsmps_time_manager_reset(clock_, &limits), "SimulationTimeManager::reset");
+    } else {
+      detail::check(lsmps_time_manager_create(&limits, &clock_), "SimulationTimeManager");
+    }
+    limits_ = limits;
+  }
+  void reset(const SimulationConfig& config) { reset(make_time_step_limits(config)); }
+  const TimeStepLimits& limits() const noexcept { return limits_; }
+  real current_time() const { return state().current_time; }
+  real current_time_step() const { return state().current_time_step; }
+  size_type step_index() const { return state().step_index; }
+  size_type output_index() const { return state().output_index; }
+  bool finished() const { return state().finished; }
+  bool output_due() const { return state().output_due; }
+  real cfl_limited_time_step(real max_velocity) const {
+    double dt = 0.0, cdt = 0.0;
+    detail::check(lsmps_time_manager_next_time_step(clock_, max_velocity, &dt, &cdt),
+                  "cfl_limited_time_step");
+    return cdt;
+  }
+  real next_time_step(real max_velocity) const {
+    double dt = 0.0, cdt = 0.0;
+    detail::check(lsmps_time_manager_next_time_step(clock_, max_velocity, &dt, &cdt),
+                  "next_time_step");
+    return dt;
+  }
+  TimeStepStatus advance(real max_velocity) {
+    lsmps_time_status st{};
+    detail::check(lsmps_time_manager_advance(clock_, max_velocity, &st), "advance");
+    return convert(st);
+  }
+  TimeStepStatus mark_initial_output() {
+    lsmps_time_status st{};
+    detail::check(lsmps_time_manager_mark_initial_output(clock_, &st), "mark_initial_output");
+    return convert(st);
+  }
+  lsmps_time_manager* handle() noexcept { return clock_; }
+
+ private:
+  struct State {
+    double current_time, current_time_step;
+    size_t step_index, output_index;
+    bool output_due, finished;
+  };
+  State state() const {
+    State s{};
+    int32_t due = 0, fin = 0;
+    detail::check(lsmps_time_manager_state(clock_, &s.current_time, &s.current_time_step,
+                                           &s.step_index, &s.output_index, &due, &fin),
+                  "SimulationTimeManager state");
+    s.output_due = due != 0;
+    s.finished = fin != 0;
+    return s;
+  }
+  static TimeStepStatus convert(const lsmps_time_status& st) {
+    return TimeStepStatus{st.step_index,  st.output_index,  st.previous_time,
+                          st.current_time, st.time_step,    st.cfl_time_step,
+                          st.max_velocity, st.should_output != 0, st.reached_final_time != 0};
+  }
+  lsmps_time_manager* clock_ = nullptr;
+  TimeStepLimits limits_{};
+};
+
+// max_velocity (src/app/simulation_driver.cu:122-143): a device reduction here.
+inline real max_velocity(const FluidParticleSoA& fluid) {
+  double v = 0.0;
+  detail::check(lsmps_max_velocity(detail::context(), &fluid, &v), "max_velocity");
+  return v;
+}
+
 }  // namespace lsmps3d

@@ -34,14 +34,15 @@ def test_header_functions_exported(rt):
 STRUCTS = ["lsmps_fluid_soa", "lsmps_wall_soa", "lsmps_neighbor_list", "lsmps_cell_grid",
            "lsmps_cell_list", "lsmps_surface_diagnostics", "lsmps_csr", "lsmps_ppe_workspace",
            "lsmps_vector3", "lsmps_moment_view", "lsmps_config", "lsmps_solver_options",
-           "lsmps_solve_report", "lsmps_adjoint_grads"]
+           "lsmps_solve_report", "lsmps_adjoint_grads", "lsmps_time_limits", "lsmps_time_status"]
 MIRRORS = {"lsmps_fluid_soa": "FluidSoA", "lsmps_wall_soa": "WallSoA",
            "lsmps_neighbor_list": "NeighborList", "lsmps_cell_grid": "CellGrid",
            "lsmps_cell_list": "CellList", "lsmps_surface_diagnostics": "SurfaceDiagnostics",
            "lsmps_csr": "Csr", "lsmps_ppe_workspace": "PpeWorkspace", "lsmps_vector3": "Vector3",
            "lsmps_moment_view": "MomentView", "lsmps_config": "Config",
            "lsmps_solver_options": "SolverOptions", "lsmps_solve_report": "SolveReport",
-           "lsmps_adjoint_grads": "AdjointGrads"}
+           "lsmps_adjoint_grads": "AdjointGrads", "lsmps_time_limits": "TimeLimits",
+           "lsmps_time_status": "TimeStatus"}
 
 
 @pytest.fixture(scope="module")
@@ -164,3 +165,57 @@ def test_cpp_wrapper_compiles():
                           f"-I{os.path.join(ROOT, 'include')}", "--offload-arch=gfx950", "-"],
                          input=src, capture_output=True, text=True)
     assert res.returncode == 0, res.stderr
+
+
+def test_cpp_wrapper_clock_known_answers(golden, tmp_path):
+    """lsmps3d::SimulationTimeManager (header-only layer over the C-ABI clock)
+    replays the sequence of the reference's tests/unit/smoke_time_step.cpp with
+    its known answers; host code only."""
+    fx = golden["time_step"]
+    c = fx["config"]
+    a = fx["advances"]
+    src = f"""
+#include <cmath>
+#include <cstdio>
+#include "lsmps3d_amd.hpp"
+static bool near(double x, double y) {{ return std::abs(x - y) <= {fx['tolerance']!r}; }}
+int main() {{
+  lsmps3d::SimulationConfig config;
+  config.particle_spacing = {c['particle_spacing']!r};
+  config.time_step = {c['time_step']!r};
+  config.min_time_step = {c['min_time_step']!r};
+  config.max_time_step = {c['max_time_step']!r};
+  config.time_step_growth_factor = {c['time_step_growth_factor']!r};
+  config.cfl = {c['cfl']!r};
+  config.final_time = {c['final_time']!r};
+  config.output_interval = {c['output_interval']!r};
+  lsmps3d::SimulationTimeManager clock(config);
+  if (!clock.output_due()) return 1;
+  auto init = clock.mark_initial_output();
+  if (!init.should_output || init.output_index != {fx['initial_output_index']} || clock.output_due()) return 2;
+  auto s = clock.advance({a[0]['max_velocity']!r});
+  if (!near(s.time_step, {a[0]['expected_dt']!r}) || s.should_output || s.reached_final_time) return 3;
+  s = clock.advance({a[1]['max_velocity']!r});
+  if (!near(s.time_step, {a[1]['expected_dt']!r}) || !s.should_output || s.output_index != {a[1]['output_index']}) return 4;
+  s = clock.advance({a[2]['max_velocity']!r});
+  if (!near(s.time_step, {a[2]['expected_dt']!r})) return 5;
+  int guard = 0;
+  while (!s.reached_final_time && guard < {fx['guard']}) {{ s = clock.advance({fx['then_max_velocity']!r}); ++guard; }}
+  if (!s.reached_final_time || !near(clock.current_time(), {fx['expected_final_time']!r})) return 6;
+  auto bad = lsmps3d::make_time_step_limits(config);
+  bad.growth_factor = {fx['invalid_growth_factor']!r};
+  try {{ lsmps3d::validate_time_step_limits(bad); return 7; }} catch (const std::invalid_argument&) {{}}
+  std::puts("ok");
+  return 0;
+}}
+"""
+    cpp = tmp_path / "clock.cpp"
+    cpp.write_text(src)
+    exe = tmp_path / "clock"
+    lib_dir = os.path.join(ROOT, "lsmps-3d_amd")
+    res = subprocess.run(["/opt/rocm/bin/hipcc", "-std=c++17", "-O1", str(cpp),
+                          f"-I{os.path.join(ROOT, 'include')}", f"-L{lib_dir}", "-llsmps3d_amd",
+                          f"-Wl,-rpath,{lib_dir}", "-o", str(exe)], capture_output=True, text=True)
+    assert res.returncode == 0, res.stderr
+    run = subprocess.run([str(exe)], capture_output=True, text=True)
+    assert run.returncode == 0 and run.stdout.strip() == "ok", (run.returncode, run.stdout, run.stderr)
