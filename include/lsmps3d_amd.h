@@ -42,7 +42,8 @@ enum {
   LSMPS_ERR_NOT_PREPARED = 4,     /* moment views used before prepare (moment_matrix.cu:571-576) */
   LSMPS_ERR_NOT_CONVERGED = 5,    /* PPE solver hit max_iterations */
   LSMPS_ERR_OUT_OF_MEMORY = 6,
-  LSMPS_ERR_NUMERICAL = 7         /* non-finite values / breakdown */
+  LSMPS_ERR_NUMERICAL = 7,        /* non-finite values / breakdown */
+  LSMPS_ERR_IO = 8                /* a file could not be opened or written (std::runtime_error paths) */
 };
 
 /* SurfaceType (include/lsmps3d/surface/surface_type.cuh:5-10) */
@@ -564,6 +565,62 @@ int lsmps_stepper_run(lsmps_stepper* stepper, const lsmps_fluid_soa* fluid,
  * positions: neighbor lists and labels (fluid.surface_type), generation + 1. */
 int lsmps_stepper_refresh_geometry(lsmps_stepper* stepper, const lsmps_fluid_soa* fluid,
                                    const lsmps_wall_soa* walls);
+
+/* ---- on-disk formats (host code) ----------------------------------------- */
+/* The string and flag fields of SimulationConfig (config.hpp:73-80) that
+ * lsmps_config leaves out, NUL-terminated. */
+typedef struct lsmps_config_files {
+  char fluid_particle_file[1024];
+  char wall_particle_file[1024];
+  char output_directory[1024];
+  char vtk_file_prefix[256];
+  char amgx_config_path[1024];
+  int32_t vtk_write_point_fields;
+  int32_t amgx_print_solve_stats;
+} lsmps_config_files;
+/* SimulationConfig{} file-field defaults (config.hpp:73-80). */
+int lsmps_default_config_files(lsmps_config_files* out);
+/* validate_simulation_config (src/core/config.cpp:278-335); `files` may be NULL
+ * to check the numeric fields only. */
+int lsmps_config_validate(const lsmps_config* config, const lsmps_config_files* files);
+/* load_simulation_config (config.cpp:337-385): INI sections geometry,
+ * simulation, surface, lsmps, correction, files; '#' comments; defaults for
+ * absent keys; unknown keys and sections are errors.  weight_kernel (not an INI
+ * key) is left at its default. */
+int lsmps_config_load(const char* path, lsmps_config* config, lsmps_config_files* files);
+/* save_simulation_config (config.cpp:387-431), same text as the reference. */
+int lsmps_config_save(const char* path, const lsmps_config* config,
+                      const lsmps_config_files* files);
+/* FileManager::load_fluid_particles / load_wall_particles (src/io/file_manager.cpp:
+ * 126-197): CSV rows of x,y,z,vx,vy,vz (walls: + nx,ny,nz); '#' comments,
+ * blank lines and one header row before the first record are skipped.  HOST
+ * arrays: with host == NULL (or host->x == NULL) only the record count is
+ * returned; otherwise host->count is the capacity, and pressure / surface_type
+ * (if given) are set to 0 / Inner. */
+int lsmps_read_fluid_csv(const char* path, const lsmps_fluid_soa* host, size_t* count);
+int lsmps_read_wall_csv(const char* path, const lsmps_wall_soa* host, size_t* count);
+/* Point field of a VTK file (HostVtkPointFields, vtk_writer.hpp:17-46): REAL
+ * scalars read x, INT scalars ivalues, VECTORs x, y, z; host arrays of count
+ * values. */
+enum { LSMPS_VTK_REAL_SCALAR = 0, LSMPS_VTK_INT_SCALAR = 1, LSMPS_VTK_VECTOR = 2 };
+typedef struct lsmps_vtk_field {
+  const char* name;
+  int32_t kind;
+  size_t count;
+  const double* x;
+  const double* y;
+  const double* z;
+  const int32_t* ivalues;
+} lsmps_vtk_field;
+/* LegacyVtkWriter::make_path (vtk_writer.cpp:83-88): directory/prefix_NNNNNN.vtk */
+int lsmps_vtk_path(const char* output_directory, const char* prefix, size_t step, char* out,
+                   size_t capacity);
+/* LegacyVtkWriter::write (vtk_writer.cpp:90-149): legacy ASCII POLYDATA with
+ * one vertex per particle; point fields (vectors, then real scalars, then
+ * integer scalars) when write_point_fields is non-zero. */
+int lsmps_write_vtk(const char* output_directory, const char* prefix, int32_t write_point_fields,
+                    size_t step, size_t count, const double* x, const double* y, const double* z,
+                    const lsmps_vtk_field* fields, int32_t field_count);
 
 /* ---- adjoint of one step (no reference counterpart) -------------------- */
 /* Stepper modes (bit flags, default 0):

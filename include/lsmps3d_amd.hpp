@@ -29,6 +29,7 @@
 #include <stdexcept>
 #include <string>
 #include <utility>
+#include <vector>
 
 #include "lsmps3d_amd.h"
 
@@ -254,6 +255,70 @@ inline lsmps_cell_grid to_c(const CellGrid& g) {
 }
 inline lsmps_vector3 to_c(const ProvisionVelocityView& v) {
   return lsmps_vector3{v.count, v.vx, v.vy, v.vz};
+}
+inline void copy_text(char* dst, size_t cap, const std::string& v, const char* what) {
+  if (v.size() + 1 > cap) throw std::invalid_argument(std::string("SimulationConfig field '") + what + "' is too long");
+  for (size_t i = 0; i <= v.size(); ++i) dst[i] = i < v.size() ? v[i] : '\0';
+}
+inline lsmps_config_files files_to_c(const SimulationConfig& c) {
+  lsmps_config_files f{};
+  copy_text(f.fluid_particle_file, sizeof(f.fluid_particle_file), c.fluid_particle_file.string(), "fluid_particle_file");
+  copy_text(f.wall_particle_file, sizeof(f.wall_particle_file), c.wall_particle_file.string(), "wall_particle_file");
+  copy_text(f.output_directory, sizeof(f.output_directory), c.output_directory.string(), "output_directory");
+  copy_text(f.vtk_file_prefix, sizeof(f.vtk_file_prefix), c.vtk_file_prefix, "vtk_file_prefix");
+  copy_text(f.amgx_config_path, sizeof(f.amgx_config_path), c.amgx_config_path, "amgx_config_path");
+  f.vtk_write_point_fields = c.vtk_write_point_fields ? 1 : 0;
+  f.amgx_print_solve_stats = c.amgx_print_solve_stats ? 1 : 0;
+  return f;
+}
+inline SimulationConfig from_c(const lsmps_config& o, const lsmps_config_files& f) {
+  SimulationConfig c;
+  c.particle_spacing = o.particle_spacing;
+  c.support_radius = o.support_radius;
+  c.near_surface_radius = o.near_surface_radius;
+  c.cell_origin = Vec3{o.cell_origin[0], o.cell_origin[1], o.cell_origin[2]};
+  c.cell_size = o.cell_size;
+  c.cell_dims = Int3{o.cell_dims[0], o.cell_dims[1], o.cell_dims[2]};
+  c.time_step = o.time_step;
+  c.min_time_step = o.min_time_step;
+  c.max_time_step = o.max_time_step;
+  c.time_step_growth_factor = o.time_step_growth_factor;
+  c.final_time = o.final_time;
+  c.output_interval = o.output_interval;
+  c.density = o.density;
+  c.kinematic_viscosity = o.kinematic_viscosity;
+  c.cfl = o.cfl;
+  c.gravity = Vec3{o.gravity[0], o.gravity[1], o.gravity[2]};
+  c.splash_neighbor_threshold = o.splash_neighbor_threshold;
+  c.include_wall_neighbors = o.include_wall_neighbors != 0;
+  c.number_density_ratio_threshold = o.number_density_ratio_threshold;
+  c.air_open_ratio_threshold = o.air_open_ratio_threshold;
+  c.air_anisotropy_threshold = o.air_anisotropy_threshold;
+  c.wall_normal_independence_threshold = o.wall_normal_independence_threshold;
+  c.virtual_light_cone_cosine = o.virtual_light_cone_cosine;
+  c.lsmps_regularization = o.lsmps_regularization;
+  c.lsmps_wall_weight_scale = o.lsmps_wall_weight_scale;
+  c.ps_displacement_scale = o.ps_displacement_scale;
+  c.ps_min_distance_ratio = o.ps_min_distance_ratio;
+  c.ps_max_displacement_ratio = o.ps_max_displacement_ratio;
+  c.wall_clearance_ratio = o.wall_clearance_ratio;
+  c.velocity_smoothing_strength = o.velocity_smoothing_strength;
+  c.weight_kernel = o.weight_kernel;
+  c.fluid_particle_file = f.fluid_particle_file;
+  c.wall_particle_file = f.wall_particle_file;
+  c.output_directory = f.output_directory;
+  c.vtk_file_prefix = f.vtk_file_prefix;
+  c.vtk_write_point_fields = f.vtk_write_point_fields != 0;
+  c.amgx_config_path = f.amgx_config_path;
+  c.amgx_print_solve_stats = f.amgx_print_solve_stats != 0;
+  return c;
+}
+// The host formats throw the reference's exception types: std::invalid_argument
+// for content errors, std::runtime_error for files that cannot be opened.
+inline void check_io(int rc) {
+  if (rc == LSMPS_OK) return;
+  if (rc == LSMPS_ERR_IO) throw std::runtime_error(lsmps_last_error());
+  throw std::invalid_argument(lsmps_last_error());
 }
 template <typename T>
 T* device_alloc(size_type count) {
@@ -734,6 +799,168 @@ inline void smooth_particle_velocity(const FluidParticleSoA& fluid,
                                                &smoothed_velocity),
                 "smooth_particle_velocity");
 }
+
+// ---- on-disk formats (core/config.hpp, io/file_manager.hpp, io/vtk_writer.hpp)
+inline void validate_simulation_config(const SimulationConfig& config) {
+  const lsmps_config c = detail::to_c(config);
+  const lsmps_config_files f = detail::files_to_c(config);
+  detail::check_io(lsmps_config_validate(&c, &f));
+}
+inline SimulationConfig load_simulation_config(const std::filesystem::path& path) {
+  lsmps_config c{};
+  lsmps_config_files f{};
+  detail::check_io(lsmps_config_load(path.string().c_str(), &c, &f));
+  return detail::from_c(c, f);
+}
+inline void save_simulation_config(const SimulationConfig& config, const std::filesystem::path& path) {
+  const lsmps_config c = detail::to_c(config);
+  const lsmps_config_files f = detail::files_to_c(config);
+  detail::check_io(lsmps_config_save(path.string().c_str(), &c, &f));
+}
+
+struct HostParticleSnapshot {
+  std::vector<real> x;
+  std::vector<real> y;
+  std::vector<real> z;
+};
+struct HostRealScalarField {
+  std::string name;
+  std::vector<real> values;
+};
+struct HostIntScalarField {
+  std::string name;
+  std::vector<int> values;
+};
+struct HostVectorField {
+  std::string name;
+  std::vector<real> x;
+  std::vector<real> y;
+  std::vector<real> z;
+};
+struct HostVtkPointFields {
+  std::vector<HostRealScalarField> real_scalars;
+  std::vector<HostIntScalarField> int_scalars;
+  std::vector<HostVectorField> vectors;
+  void add_real_scalar(std::string name, std::vector<real> values) {
+    real_scalars.push_back(HostRealScalarField{std::move(name), std::move(values)});
+  }
+  void add_int_scalar(std::string name, std::vector<int> values) {
+    int_scalars.push_back(HostIntScalarField{std::move(name), std::move(values)});
+  }
+  void add_scalar(std::string name, std::vector<real> values) { add_real_scalar(std::move(name), std::move(values)); }
+  void add_scalar(std::string name, std::vector<int> values) { add_int_scalar(std::move(name), std::move(values)); }
+  void add_vector(std::string name, std::vector<real> x, std::vector<real> y, std::vector<real> z) {
+    vectors.push_back(HostVectorField{std::move(name), std::move(x), std::move(y), std::move(z)});
+  }
+};
+
+// LegacyVtkWriter (vtk_writer.hpp:48-61) over lsmps_write_vtk.
+class LegacyVtkWriter {
+ public:
+  explicit LegacyVtkWriter(SimulationConfig config = {}) : config_(std::move(config)) {}
+  std::filesystem::path make_path(size_type step) const {
+    char buf[4096];
+    detail::check_io(lsmps_vtk_path(config_.output_directory.string().c_str(),
+                                    config_.vtk_file_prefix.c_str(), step, buf, sizeof(buf)));
+    return std::filesystem::path(buf);
+  }
+  void write(size_type step, const HostParticleSnapshot& particles,
+             const HostVtkPointFields& fields = {}) const {
+    const size_t n = particles.x.size();
+    if (particles.y.size() != n || particles.z.size() != n)
+      throw std::invalid_argument("HostParticleSnapshot fields must have matching lengths.");
+    std::vector<lsmps_vtk_field> f;
+    for (const auto& v : fields.vectors) {
+      const size_t m = (v.x.size() == v.y.size() && v.y.size() == v.z.size()) ? v.x.size() : size_t(-1);
+      f.push_back(lsmps_vtk_field{v.name.c_str(), LSMPS_VTK_VECTOR, m, v.x.data(), v.y.data(), v.z.data(), nullptr});
+    }
+    for (const auto& r : fields.real_scalars)
+      f.push_back(lsmps_vtk_field{r.name.c_str(), LSMPS_VTK_REAL_SCALAR, r.values.size(), r.values.data(), nullptr, nullptr, nullptr});
+    std::vector<std::vector<int32_t>> ints;
+    for (const auto& i : fields.int_scalars) ints.emplace_back(i.values.begin(), i.values.end());
+    for (size_t k = 0; k < fields.int_scalars.size(); ++k)
+      f.push_back(lsmps_vtk_field{fields.int_scalars[k].name.c_str(), LSMPS_VTK_INT_SCALAR, ints[k].size(), nullptr, nullptr, nullptr, ints[k].data()});
+    detail::check_io(lsmps_write_vtk(config_.output_directory.string().c_str(), config_.vtk_file_prefix.c_str(),
+                                     config_.vtk_write_point_fields ? 1 : 0, step, n, particles.x.data(),
+                                     particles.y.data(), particles.z.data(), f.data(), static_cast<int32_t>(f.size())));
+  }
+
+ private:
+  SimulationConfig config_{};
+};
+
+struct HostFluidParticles {
+  std::vector<real> x, y, z, vx, vy, vz, pressure;
+  std::vector<int> surface_type;
+  size_type count() const noexcept { return x.size(); }
+  HostParticleSnapshot snapshot() const { return HostParticleSnapshot{x, y, z}; }
+};
+struct HostWallParticles {
+  std::vector<real> x, y, z, normal_x, normal_y, normal_z, vx, vy, vz;
+  size_type count() const noexcept { return x.size(); }
+  HostParticleSnapshot snapshot() const { return HostParticleSnapshot{x, y, z}; }
+};
+struct ParticleInputData {
+  HostFluidParticles fluid;
+  HostWallParticles walls;
+};
+
+// FileManager (io/file_manager.hpp:57-87) over the library's readers and writer.
+class FileManager {
+ public:
+  explicit FileManager(SimulationConfig config = {}) : config_(std::move(config)) {}
+  const SimulationConfig& config() const noexcept { return config_; }
+  void set_config(SimulationConfig config) { config_ = std::move(config); }
+  SimulationConfig load_config(const std::filesystem::path& path) const { return load_simulation_config(path); }
+  void save_config(const SimulationConfig& config, const std::filesystem::path& path) const {
+    save_simulation_config(config, path);
+  }
+  HostFluidParticles load_fluid_particles(const std::filesystem::path& path) const {
+    size_t n = 0;
+    detail::check_io(lsmps_read_fluid_csv(path.string().c_str(), nullptr, &n));
+    HostFluidParticles p;
+    p.x.resize(n); p.y.resize(n); p.z.resize(n); p.vx.resize(n); p.vy.resize(n); p.vz.resize(n);
+    p.pressure.resize(n);
+    std::vector<int32_t> st(n);
+    lsmps_fluid_soa h{n, p.x.data(), p.y.data(), p.z.data(), p.vx.data(), p.vy.data(), p.vz.data(),
+                      p.pressure.data(), st.data()};
+    detail::check_io(lsmps_read_fluid_csv(path.string().c_str(), &h, &n));
+    p.surface_type.assign(st.begin(), st.end());
+    return p;
+  }
+  HostWallParticles load_wall_particles(const std::filesystem::path& path) const {
+    size_t n = 0;
+    detail::check_io(lsmps_read_wall_csv(path.string().c_str(), nullptr, &n));
+    HostWallParticles p;
+    for (auto* v : {&p.x, &p.y, &p.z, &p.vx, &p.vy, &p.vz, &p.normal_x, &p.normal_y, &p.normal_z}) v->resize(n);
+    lsmps_wall_soa h{n, p.x.data(), p.y.data(), p.z.data(), p.vx.data(), p.vy.data(), p.vz.data(),
+                     p.normal_x.data(), p.normal_y.data(), p.normal_z.data()};
+    detail::check_io(lsmps_read_wall_csv(path.string().c_str(), &h, &n));
+    return p;
+  }
+  ParticleInputData load_particle_input(const std::filesystem::path& fluid_path,
+                                        const std::filesystem::path& wall_path) const {
+    return ParticleInputData{load_fluid_particles(fluid_path), load_wall_particles(wall_path)};
+  }
+  ParticleInputData load_particle_input() const {
+    return load_particle_input(config_.fluid_particle_file, config_.wall_particle_file);
+  }
+  void write_fluid_result(size_type step, const HostFluidParticles& particles,
+                          const HostVtkPointFields& fields = {}) const {
+    SimulationConfig c = config_;
+    c.vtk_file_prefix += "_fluid";
+    LegacyVtkWriter(c).write(step, particles.snapshot(), fields);
+  }
+  void write_wall_result(size_type step, const HostWallParticles& particles,
+                         const HostVtkPointFields& fields = {}) const {
+    SimulationConfig c = config_;
+    c.vtk_file_prefix += "_wall";
+    LegacyVtkWriter(c).write(step, particles.snapshot(), fields);
+  }
+
+ private:
+  SimulationConfig config_{};
+};
 
 // ---- simulation clock (core/time_step.hpp) -------------------------------
 using TimeStepLimits = lsmps_time_limits;

@@ -13,6 +13,7 @@ LSMPS_ERR_NOT_PREPARED = 4
 LSMPS_ERR_NOT_CONVERGED = 5
 LSMPS_ERR_OUT_OF_MEMORY = 6
 LSMPS_ERR_NUMERICAL = 7
+LSMPS_ERR_IO = 8
 
 INNER, NEAR_SURFACE, SURFACE, SPLASH = 0, 1, 2, 3
 
@@ -154,6 +155,21 @@ class SolveReport(C.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ConfigFiles(C.Structure):
+    _fields_ = [("fluid_particle_file", C.c_char * 1024), ("wall_particle_file", C.c_char * 1024),
+                ("output_directory", C.c_char * 1024), ("vtk_file_prefix", C.c_char * 256),
+                ("amgx_config_path", C.c_char * 1024), ("vtk_write_point_fields", C.c_int32),
+                ("amgx_print_solve_stats", C.c_int32)]
+
+
+VTK_REAL_SCALAR, VTK_INT_SCALAR, VTK_VECTOR = 0, 1, 2
+
+
+class VtkField(C.Structure):
+    _fields_ = [("name", C.c_char_p), ("kind", C.c_int32), ("count", C.c_size_t),
+                ("x", _dp), ("y", _dp), ("z", _dp), ("ivalues", _ip)]
 
 
 class TimeLimits(C.Structure):

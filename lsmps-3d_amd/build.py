@@ -30,6 +30,7 @@ SOURCES = {
     "adjoint.hip": ["-ffp-contract=fast"],
     "driver.hip": ["-ffp-contract=off"],
     "stepper.cpp": [],
+    "io.cpp": [],
     "capi.cpp": [],
 }
 

@@ -610,6 +610,96 @@ int lsmps_stepper_set_segments(lsmps_stepper* s, int32_t segment_count,
 
 size_t lsmps_stepper_bytes(const lsmps_stepper* s) { return s ? s->bytes() : 0; }
 
+int lsmps_default_config_files(lsmps_config_files* out) {
+  API_BEGIN
+  require(out != nullptr, "null output");
+  default_config_files(*out);
+  API_END("lsmps_default_config_files")
+}
+
+int lsmps_config_validate(const lsmps_config* config, const lsmps_config_files* files) {
+  API_BEGIN
+  validate_config(deref(config, "config"), files);
+  API_END("lsmps_config_validate")
+}
+
+int lsmps_config_load(const char* path, lsmps_config* config, lsmps_config_files* files) {
+  API_BEGIN
+  require(path != nullptr && config != nullptr, "null argument");
+  lsmps_config c{};
+  lsmps_config_files f{};
+  load_config(path, c, f);
+  *config = c;
+  if (files) *files = f;
+  API_END("lsmps_config_load")
+}
+
+int lsmps_config_save(const char* path, const lsmps_config* config,
+                      const lsmps_config_files* files) {
+  API_BEGIN
+  require(path != nullptr, "null path");
+  lsmps_config_files f{};
+  if (files) f = *files; else default_config_files(f);
+  save_config(path, deref(config, "config"), f);
+  API_END("lsmps_config_save")
+}
+
+int lsmps_read_fluid_csv(const char* path, const lsmps_fluid_soa* host, size_t* count) {
+  API_BEGIN
+  require(path != nullptr, "null path");
+  const bool fill = host && host->x;
+  double* out[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  if (fill) {
+    out[0] = host->x; out[1] = host->y; out[2] = host->z;
+    out[3] = host->vx; out[4] = host->vy; out[5] = host->vz;
+  }
+  const size_t rows = read_particle_csv(path, 6, fill ? out : nullptr, fill ? host->count : 0);
+  if (fill) {
+    for (size_t i = 0; i < rows; ++i) {
+      if (host->pressure) host->pressure[i] = 0.0;
+      if (host->surface_type) host->surface_type[i] = LSMPS_SURFACE_INNER;
+    }
+  }
+  if (count) *count = rows;
+  API_END("lsmps_read_fluid_csv")
+}
+
+int lsmps_read_wall_csv(const char* path, const lsmps_wall_soa* host, size_t* count) {
+  API_BEGIN
+  require(path != nullptr, "null path");
+  const bool fill = host && host->x;
+  double* out[9] = {};
+  if (fill) {
+    out[0] = host->x; out[1] = host->y; out[2] = host->z;
+    out[3] = host->vx; out[4] = host->vy; out[5] = host->vz;
+    out[6] = host->normal_x; out[7] = host->normal_y; out[8] = host->normal_z;
+  }
+  const size_t rows = read_particle_csv(path, 9, fill ? out : nullptr, fill ? host->count : 0);
+  if (count) *count = rows;
+  API_END("lsmps_read_wall_csv")
+}
+
+int lsmps_vtk_path(const char* output_directory, const char* prefix, size_t step, char* out,
+                   size_t capacity) {
+  API_BEGIN
+  require(output_directory && prefix && out, "null argument");
+  const std::string p = vtk_path(output_directory, prefix, step);
+  require(p.size() + 1 <= capacity, "output buffer too small", LSMPS_ERR_CAPACITY);
+  std::memcpy(out, p.c_str(), p.size() + 1);
+  API_END("lsmps_vtk_path")
+}
+
+int lsmps_write_vtk(const char* output_directory, const char* prefix, int32_t write_point_fields,
+                    size_t step, size_t count, const double* x, const double* y, const double* z,
+                    const lsmps_vtk_field* fields, int32_t field_count) {
+  API_BEGIN
+  require(output_directory && prefix, "null argument");
+  require(field_count == 0 || fields != nullptr, "null fields");
+  write_vtk(output_directory, prefix, write_point_fields != 0, step, count, x, y, z, fields,
+            field_count);
+  API_END("lsmps_write_vtk")
+}
+
 int lsmps_time_limits_from_config(const lsmps_config* config, lsmps_time_limits* out) {
   API_BEGIN
   require(out != nullptr, "null output");

@@ -151,6 +151,16 @@ struct lsmps_time_manager {
 };
 
 namespace lsmps {
+// io.cpp
+void default_config_files(lsmps_config_files& f);
+void validate_config(const lsmps_config& c, const lsmps_config_files* f);
+void load_config(const char* path, lsmps_config& c, lsmps_config_files& f);
+void save_config(const char* path, const lsmps_config& c, const lsmps_config_files& f);
+size_t read_particle_csv(const char* path, int columns, double* const* out, size_t capacity);
+std::string vtk_path(const char* directory, const char* prefix, size_t step);
+void write_vtk(const char* directory, const char* prefix, bool point_fields, size_t step,
+               size_t count, const double* x, const double* y, const double* z,
+               const lsmps_vtk_field* fields, int nfields);
 // driver.hip
 lsmps_time_limits time_limits_from_config(const lsmps_config& c);
 void validate_time_limits(const lsmps_time_limits& l);

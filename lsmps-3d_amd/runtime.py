@@ -35,7 +35,9 @@ EXPORTS = [
     "lsmps_time_manager_destroy", "lsmps_time_manager_reset", "lsmps_time_manager_state",
     "lsmps_time_manager_next_time_step", "lsmps_time_manager_advance",
     "lsmps_time_manager_mark_initial_output", "lsmps_max_velocity", "lsmps_stepper_run",
-    "lsmps_stepper_refresh_geometry",
+    "lsmps_stepper_refresh_geometry", "lsmps_default_config_files", "lsmps_config_validate",
+    "lsmps_config_load", "lsmps_config_save", "lsmps_read_fluid_csv", "lsmps_read_wall_csv",
+    "lsmps_vtk_path", "lsmps_write_vtk",
 ]
 
 _LIB = None
@@ -104,6 +106,96 @@ def derive_cell_grid(config, fluid, walls):
                                         *(a.ctypes.data for a in w), w[0].shape[0],
                                         C.byref(cfg)))
     return cfg
+
+
+# ---- on-disk formats (host code) ---------------------------------------------
+
+def default_config_files():
+    f = abi.ConfigFiles()
+    check(load().lsmps_default_config_files(C.byref(f)))
+    return f
+
+
+def validate_config(config, files=None):
+    check(load().lsmps_config_validate(C.byref(config), C.byref(files) if files else None),
+          "config_validate")
+
+
+def load_config(path):
+    """load_simulation_config: (lsmps_config, ConfigFiles)."""
+    cfg, files = abi.Config(), abi.ConfigFiles()
+    check(load().lsmps_config_load(os.fsencode(path), C.byref(cfg), C.byref(files)),
+          "config_load")
+    return cfg, files
+
+
+def save_config(path, config, files=None):
+    check(load().lsmps_config_save(os.fsencode(path), C.byref(config),
+                                   C.byref(files) if files else None), "config_save")
+
+
+def read_fluid_csv(path):
+    """FileManager::load_fluid_particles into numpy arrays (x..vz, pressure, surface_type)."""
+    lib = load()
+    n = C.c_size_t()
+    check(lib.lsmps_read_fluid_csv(os.fsencode(path), None, C.byref(n)), "read_fluid_csv")
+    out = {k: np.zeros(n.value) for k in ("x", "y", "z", "vx", "vy", "vz", "pressure")}
+    out["surface_type"] = np.full(n.value, -1, np.int32)
+    soa = abi.FluidSoA(n.value, *(abi.dptr(out[k].ctypes.data) for k in
+                                  ("x", "y", "z", "vx", "vy", "vz", "pressure")),
+                       abi.iptr(out["surface_type"].ctypes.data))
+    check(lib.lsmps_read_fluid_csv(os.fsencode(path), C.byref(soa), C.byref(n)), "read_fluid_csv")
+    return out
+
+
+def read_wall_csv(path):
+    lib = load()
+    n = C.c_size_t()
+    check(lib.lsmps_read_wall_csv(os.fsencode(path), None, C.byref(n)), "read_wall_csv")
+    keys = ("x", "y", "z", "vx", "vy", "vz", "normal_x", "normal_y", "normal_z")
+    out = {k: np.zeros(n.value) for k in keys}
+    soa = abi.WallSoA(n.value, *(abi.dptr(out[k].ctypes.data) for k in keys))
+    check(lib.lsmps_read_wall_csv(os.fsencode(path), C.byref(soa), C.byref(n)), "read_wall_csv")
+    return out
+
+
+def vtk_path(directory, prefix, step):
+    buf = C.create_string_buffer(4096)
+    check(load().lsmps_vtk_path(os.fsencode(directory), prefix.encode(), C.c_size_t(step), buf,
+                                C.c_size_t(4096)), "vtk_path")
+    return buf.value.decode()
+
+
+def write_vtk(directory, prefix, step, x, y, z, vectors=(), real_scalars=(), int_scalars=(),
+              point_fields=True):
+    """LegacyVtkWriter::write of host arrays; fields as (name, arrays) pairs."""
+    keep = []
+    def arr(a, dt):
+        a = np.ascontiguousarray(a, dtype=dt)
+        keep.append(a)
+        return a
+    xs, ys, zs = (arr(a, np.float64) for a in (x, y, z))
+    fields = []
+    for name, (fx, fy, fz) in vectors:
+        a, b, c = (arr(v, np.float64) for v in (fx, fy, fz))
+        # the library checks the count against the particles; a short component fails it
+        fields.append(abi.VtkField(name.encode(), abi.VTK_VECTOR, min(len(a), len(b), len(c)),
+                                   abi.dptr(a.ctypes.data),
+                                   abi.dptr(b.ctypes.data), abi.dptr(c.ctypes.data), abi.iptr(0)))
+    for name, v in real_scalars:
+        a = arr(v, np.float64)
+        fields.append(abi.VtkField(name.encode(), abi.VTK_REAL_SCALAR, len(a),
+                                   abi.dptr(a.ctypes.data), abi.dptr(0), abi.dptr(0), abi.iptr(0)))
+    for name, v in int_scalars:
+        a = arr(v, np.int32)
+        fields.append(abi.VtkField(name.encode(), abi.VTK_INT_SCALAR, len(a), abi.dptr(0),
+                                   abi.dptr(0), abi.dptr(0), abi.iptr(a.ctypes.data)))
+    farr = (abi.VtkField * max(len(fields), 1))(*fields)
+    check(load().lsmps_write_vtk(os.fsencode(directory), prefix.encode(), 1 if point_fields else 0,
+                                 C.c_size_t(step), C.c_size_t(len(xs)),
+                                 abi.dptr(xs.ctypes.data), abi.dptr(ys.ctypes.data),
+                                 abi.dptr(zs.ctypes.data), farr, len(fields)), "write_vtk")
+    return vtk_path(directory, prefix, step)
 
 
 class TimeManager:

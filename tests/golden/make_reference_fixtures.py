@@ -114,6 +114,71 @@ def main():
             "expected_x_center": 0.125,
             "tolerance": 2e-3,
         },
+        "config": {
+            "source": "tests/unit/smoke_config.cpp:17-135",
+            "default_ratios": {"support_radius_ratio": 3.1, "particle_volume": 1.0e-6},
+            "ini": ("# comments and whitespace are allowed\n[geometry]\nparticle_spacing = 0.02\n"
+                    "support_radius = 0.062\nnear_surface_radius = 0.04\ncell_size = 0.062\n\n"
+                    "[simulation]\ntime_step = 0.0005\nmin_time_step = 0.0001\n"
+                    "max_time_step = 0.002\ntime_step_growth_factor = 1.2\nfinal_time = 0.5\n"
+                    "output_interval = 0.05\ndensity = 998\nkinematic_viscosity = 0.0000011\n"
+                    "cfl = 0.25\ngravity_z = -9.8\n\n[surface]\nsplash_neighbor_threshold = 12\n"
+                    "number_density_ratio_threshold = 0.82\nair_open_ratio_threshold = 0.31\n"
+                    "air_anisotropy_threshold = 0.04\ninclude_wall_neighbors = true\n"
+                    "virtual_light_cone_cosine = 0.8\n\n[lsmps]\nregularization = 0.000000001\n"
+                    "wall_weight_scale = 0.75\n\n[correction]\nps_displacement_scale = 0.02\n"
+                    "ps_min_distance_ratio = 0.8\nps_max_displacement_ratio = 0.15\n"
+                    "wall_clearance_ratio = 0.3\nvelocity_smoothing_strength = 0.05\n\n[files]\n"
+                    "output_directory = /tmp/lsmps3d_output\nvtk_file_prefix = smoke\n"
+                    "vtk_write_point_fields = false\namgx_config_path = configs/custom.json\n"
+                    "amgx_print_solve_stats = true\n"),
+            "expected": {"particle_spacing": 0.02, "support_radius": 0.062, "cell_size": 0.062,
+                         "min_time_step": 0.0001, "max_time_step": 0.002,
+                         "time_step_growth_factor": 1.2, "final_time": 0.5,
+                         "output_interval": 0.05, "ps_displacement_scale": 0.02,
+                         "velocity_smoothing_strength": 0.05},
+            "expected_files": {"vtk_write_point_fields": 0, "amgx_print_solve_stats": 1,
+                               "vtk_file_prefix": "smoke",
+                               "amgx_config_path": "configs/custom.json"},
+            "invalid": [{"support_radius": -1.0},
+                        {"min_time_step": 0.01, "max_time_step": 0.001}],
+            "bad_ini": "[geometry]\nunknown_key=1\n",
+            "tolerance": 1e-10,
+        },
+        "file_manager": {
+            "source": "tests/unit/smoke_file_manager.cpp:34-137",
+            "fluid_csv": "# x,y,z,vx,vy,vz\nx,y,z,vx,vy,vz\n0,0,0,0,0,0\n1,0,0,0.5,0.25,0.125\n",
+            "wall_csv": ("x,y,z,vx,vy,vz,nx,ny,nz\n0,-1,0,0,0,0,0,1,0\n"
+                         "1,-1,0,0.1,0.2,0.3,0,1,0\n"),
+            "expected": {"fluid_count": 2, "wall_count": 2, "fluid_vx": [0.0, 0.5],
+                         "fluid_pressure": [0.0, 0.0], "fluid_surface_type": [0, 0],
+                         "wall_normal_y0": 1.0, "wall_vz1": 0.3},
+            "vtk_prefix": "case", "vtk_step": 3,
+            "expected_files": ["case_fluid_000003.vtk", "case_wall_000003.vtk"],
+            "expected_fluid_tokens": ["VECTORS velocity float", "SCALARS pressure float 1"],
+            "expected_wall_tokens": ["VECTORS normal float"],
+            "bad_csv": "x,y,z,vx,vy,vz\n0,not-a-number,0,0,0,0\n",
+            "extra_column_csv": "x,y,z,vx,vy,vz,pressure\n0,0,0,0,0,0,100\n",
+            "tolerance": 1e-10,
+        },
+        "vtk_writer": {
+            "source": "tests/unit/smoke_vtk_writer.cpp:29-88",
+            "prefix": "particles", "step": 7,
+            "points": {"x": [0.0, 1.0, 0.0], "y": [0.0, 0.0, 1.0], "z": [0.0, 0.0, 0.5]},
+            "vectors": {"velocity": [[1.0, 0.0, -1.0], [0.0, 2.0, 0.0], [0.0, 0.0, 3.0]],
+                        "pressure_gradient": [[0.5, 0.0, -0.5], [0.0, 1.0, 0.0], [0.0, 0.0, 1.5]]},
+            "real_scalars": {"pressure": [0.0, 10.0, 20.0], "number_density": [1.0, 0.55, 0.0],
+                             "anisotropy": [0.1, 0.8, 0.0]},
+            "int_scalars": {"surface_type": [0, 2, 3], "fluid_neighbor_count": [6, 4, 0],
+                            "wall_neighbor_count": [2, 1, 0]},
+            "expected_file": "particles_000007.vtk",
+            "expected_tokens": ["DATASET POLYDATA", "POINTS 3 float", "VECTORS velocity float",
+                                "VECTORS pressure_gradient float", "SCALARS pressure float 1",
+                                "SCALARS surface_type int 1", "SCALARS fluid_neighbor_count int 1",
+                                "SCALARS wall_neighbor_count int 1",
+                                "SCALARS number_density float 1", "SCALARS anisotropy float 1"],
+            "mismatched_vector": "velocity",
+        },
         "time_step": {
             "source": "tests/unit/smoke_time_step.cpp:24-98",
             "config": {"particle_spacing": 0.02, "time_step": 0.001, "min_time_step": 0.0001,

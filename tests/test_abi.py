@@ -34,7 +34,8 @@ def test_header_functions_exported(rt):
 STRUCTS = ["lsmps_fluid_soa", "lsmps_wall_soa", "lsmps_neighbor_list", "lsmps_cell_grid",
            "lsmps_cell_list", "lsmps_surface_diagnostics", "lsmps_csr", "lsmps_ppe_workspace",
            "lsmps_vector3", "lsmps_moment_view", "lsmps_config", "lsmps_solver_options",
-           "lsmps_solve_report", "lsmps_adjoint_grads", "lsmps_time_limits", "lsmps_time_status"]
+           "lsmps_solve_report", "lsmps_adjoint_grads", "lsmps_time_limits", "lsmps_time_status",
+           "lsmps_config_files", "lsmps_vtk_field"]
 MIRRORS = {"lsmps_fluid_soa": "FluidSoA", "lsmps_wall_soa": "WallSoA",
            "lsmps_neighbor_list": "NeighborList", "lsmps_cell_grid": "CellGrid",
            "lsmps_cell_list": "CellList", "lsmps_surface_diagnostics": "SurfaceDiagnostics",
@@ -42,7 +43,8 @@ MIRRORS = {"lsmps_fluid_soa": "FluidSoA", "lsmps_wall_soa": "WallSoA",
            "lsmps_moment_view": "MomentView", "lsmps_config": "Config",
            "lsmps_solver_options": "SolverOptions", "lsmps_solve_report": "SolveReport",
            "lsmps_adjoint_grads": "AdjointGrads", "lsmps_time_limits": "TimeLimits",
-           "lsmps_time_status": "TimeStatus"}
+           "lsmps_time_status": "TimeStatus", "lsmps_config_files": "ConfigFiles",
+           "lsmps_vtk_field": "VtkField"}
 
 
 @pytest.fixture(scope="module")
