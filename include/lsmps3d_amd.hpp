@@ -26,6 +26,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <filesystem>
+#include <iostream>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -1066,6 +1067,115 @@ class SimulationTimeManager {
   lsmps_time_manager* clock_ = nullptr;
   TimeStepLimits limits_{};
 };
+
+// ---- the application loop (src/app/simulation_driver.cu:302-398) ---------
+inline void print_simulation_usage(std::ostream& output, const char* program_name) {
+  output << "Usage: " << program_name << " [config/simulation.ini]\n";
+}
+
+namespace detail {
+struct DeviceBuffers {
+  std::vector<void*> ptrs;
+  template <typename T>
+  T* copy_in(const std::vector<T>& host) {
+    T* d = device_alloc<T>(host.size());
+    if (!host.empty()) hip_check(hipMemcpy(d, host.data(), host.size() * sizeof(T), hipMemcpyHostToDevice), "hipMemcpy");
+    ptrs.push_back(d);
+    return d;
+  }
+  ~DeviceBuffers() {
+    for (void* p : ptrs) device_free(p);
+  }
+};
+template <typename T>
+std::vector<T> copy_out(const T* device, size_t n) {
+  std::vector<T> h(n);
+  if (n) hip_check(hipMemcpy(h.data(), device, n * sizeof(T), hipMemcpyDeviceToHost), "hipMemcpy");
+  return h;
+}
+}  // namespace detail
+
+// run_simulation (simulation_driver.cu:302-398): config, CSV input, derived cell
+// grid, the stepper under the clock, and VTK output of the fluid (velocity,
+// pressure, surface_type; after steps also pressure_gradient and
+// ps_displacement) at the initial state and at every output time, walls once.
+// The pressure solve is the library's (no AMGX); the loop is lsmps_stepper_run.
+inline int run_simulation(const std::filesystem::path& config_path, std::ostream& log = std::cout) {
+  SimulationConfig config = load_simulation_config(config_path);
+  FileManager files(config);
+  const ParticleInputData input = files.load_particle_input();
+  const size_t nf = input.fluid.count(), nw = input.walls.count();
+  lsmps_config c = detail::to_c(config);
+  detail::check(lsmps_derive_cell_grid(input.fluid.x.data(), input.fluid.y.data(), input.fluid.z.data(), nf,
+                                       input.walls.x.data(), input.walls.y.data(), input.walls.z.data(), nw, &c),
+                "derive_cell_grid");
+  config.cell_origin = Vec3{c.cell_origin[0], c.cell_origin[1], c.cell_origin[2]};
+  config.cell_dims = Int3{c.cell_dims[0], c.cell_dims[1], c.cell_dims[2]};
+  detail::DeviceBuffers mem;
+  std::vector<int32_t> st(input.fluid.surface_type.begin(), input.fluid.surface_type.end());
+  FluidParticleSoA fluid{nf, mem.copy_in(input.fluid.x), mem.copy_in(input.fluid.y), mem.copy_in(input.fluid.z),
+                         mem.copy_in(input.fluid.vx), mem.copy_in(input.fluid.vy), mem.copy_in(input.fluid.vz),
+                         mem.copy_in(input.fluid.pressure), mem.copy_in(st)};
+  WallParticleSoA walls{nw, mem.copy_in(input.walls.x), mem.copy_in(input.walls.y), mem.copy_in(input.walls.z),
+                        mem.copy_in(input.walls.vx), mem.copy_in(input.walls.vy), mem.copy_in(input.walls.vz),
+                        mem.copy_in(input.walls.normal_x), mem.copy_in(input.walls.normal_y),
+                        mem.copy_in(input.walls.normal_z)};
+  lsmps_stepper* stepper = nullptr;
+  detail::check(lsmps_stepper_create(detail::context(), nf, nw, 256, &c, &stepper), "lsmps_stepper_create");
+  SimulationTimeManager clock(config);
+  log << "LSMPS3D simulation\n  config: " << config_path << "\n  fluid particles: " << nf
+      << "\n  wall particles: " << nw << "\n  cell dims: (" << c.cell_dims[0] << ", " << c.cell_dims[1]
+      << ", " << c.cell_dims[2] << ")\n  output directory: " << config.output_directory.string() << std::endl;
+  struct Out {
+    const FileManager* files;
+    const FluidParticleSoA* fluid;
+    lsmps_stepper* stepper;
+    std::ostream* log;
+  } out{&files, &fluid, stepper, &log};
+  auto write = [](const lsmps_time_status* s, void* user) -> int {
+    Out& o = *static_cast<Out*>(user);
+    const size_t n = o.fluid->count;
+    HostFluidParticles h;
+    h.x = detail::copy_out(o.fluid->x, n);
+    h.y = detail::copy_out(o.fluid->y, n);
+    h.z = detail::copy_out(o.fluid->z, n);
+    h.vx = detail::copy_out(o.fluid->vx, n);
+    h.vy = detail::copy_out(o.fluid->vy, n);
+    h.vz = detail::copy_out(o.fluid->vz, n);
+    h.pressure = detail::copy_out(o.fluid->pressure, n);
+    const auto labels = detail::copy_out(o.fluid->surface_type, n);
+    h.surface_type.assign(labels.begin(), labels.end());
+    HostVtkPointFields f;
+    f.add_vector("velocity", h.vx, h.vy, h.vz);
+    f.add_real_scalar("pressure", h.pressure);
+    f.add_int_scalar("surface_type", h.surface_type);
+    if (s->step_index > 0) {
+      lsmps_vector3 grad{}, ps{};
+      if (lsmps_stepper_correction_vectors(o.stepper, &grad, &ps) == LSMPS_OK && grad.count >= n) {
+        f.add_vector("pressure_gradient", detail::copy_out(grad.x, n), detail::copy_out(grad.y, n),
+                     detail::copy_out(grad.z, n));
+        f.add_vector("ps_displacement", detail::copy_out(ps.x, n), detail::copy_out(ps.y, n),
+                     detail::copy_out(ps.z, n));
+      }
+      *o.log << "step " << s->step_index << " time " << s->current_time << " dt " << s->time_step
+             << " max_velocity " << s->max_velocity << std::endl;
+    }
+    o.files->write_fluid_result(s->step_index, h, f);
+    return 0;
+  };
+  // walls once, with the initial output
+  HostVtkPointFields wf;
+  wf.add_vector("velocity", input.walls.vx, input.walls.vy, input.walls.vz);
+  wf.add_vector("normal", input.walls.normal_x, input.walls.normal_y, input.walls.normal_z);
+  files.write_wall_result(0, input.walls, wf);
+  lsmps_time_status last{};
+  const int rc = lsmps_stepper_run(stepper, &fluid, &walls, clock.handle(), -1, write, &out, &last);
+  lsmps_stepper_destroy(stepper);
+  if (rc != LSMPS_OK && rc != LSMPS_ERR_NOT_CONVERGED) detail::check(rc, "lsmps_stepper_run");
+  log << "Simulation complete at time " << last.current_time << " after " << last.step_index << " steps"
+      << std::endl;
+  return EXIT_SUCCESS;
+}
 
 // max_velocity (src/app/simulation_driver.cu:122-143): a device reduction here.
 inline real max_velocity(const FluidParticleSoA& fluid) {

@@ -1,4 +1,5 @@
-"""Compiles the C++ module-API test program (tests/cpp/module_step.cpp) against
+"""Compiles the C++ test programs (tests/cpp/module_step.cpp, the module API;
+tests/cpp/run_simulation.cpp, the application loop) against
 include/lsmps3d_amd.hpp and lsmps-3d_amd/liblsmps3d_amd.so into tests/cpp/_build/."""
 import os
 import shutil
@@ -10,19 +11,24 @@ LIBDIR = os.path.join(ROOT, "lsmps-3d_amd")
 OUT = os.path.join(HERE, "_build", "module_step")
 
 
+PROGRAMS = ("module_step", "run_simulation")
+
+
 def build(verbose=False):
     os.makedirs(os.path.dirname(OUT), exist_ok=True)
     hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
-    cmd = [hipcc, "-std=c++17", "-O2", "--offload-arch=gfx950", "-Wall", "-Wextra",
-           f"-I{os.path.join(ROOT, 'include')}", os.path.join(HERE, "module_step.cpp"),
-           f"-L{LIBDIR}", "-llsmps3d_amd", f"-Wl,-rpath,{LIBDIR}", "-Wl,-rpath,$ORIGIN/../../../lsmps-3d_amd",
-           "-o", OUT + ".tmp"]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    res = subprocess.run(cmd, capture_output=True, text=True)
-    if res.returncode != 0:
-        raise RuntimeError(f"hipcc failed for module_step.cpp:\n{res.stdout}\n{res.stderr}")
-    os.replace(OUT + ".tmp", OUT)
+    for name in PROGRAMS:
+        out = os.path.join(HERE, "_build", name)
+        cmd = [hipcc, "-std=c++17", "-O2", "--offload-arch=gfx950", "-Wall", "-Wextra",
+               f"-I{os.path.join(ROOT, 'include')}", os.path.join(HERE, name + ".cpp"),
+               f"-L{LIBDIR}", "-llsmps3d_amd", f"-Wl,-rpath,{LIBDIR}",
+               "-Wl,-rpath,$ORIGIN/../../../lsmps-3d_amd", "-o", out + ".tmp"]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        if res.returncode != 0:
+            raise RuntimeError(f"hipcc failed for {name}.cpp:\n{res.stdout}\n{res.stderr}")
+        os.replace(out + ".tmp", out)
     return OUT
 
 
