@@ -333,6 +333,26 @@ int lsmps_classify_surface(lsmps_context* ctx, const lsmps_fluid_soa* fluid,
                            const lsmps_neighbor_list* wall_neighbors, const lsmps_config* config,
                            const lsmps_surface_diagnostics* diagnostics);
 
+/* VirtualLightDiagnosticsView (surface_detection.cuh:23-26): per fluid particle,
+ * how many of the 14 virtual-light directions no neighbor blocks, as a count and
+ * as count / 14.  Either member may be NULL, not both. */
+typedef struct lsmps_virtual_light_diagnostics {
+  int32_t* open_direction_count;
+  double* open_fraction;
+} lsmps_virtual_light_diagnostics;
+/* compute_virtual_light_diagnostics(...) — surface_detection.cuh:38-43,
+ * src/surface/surface_detection.cu:350-475,555-578.  A direction is blocked when
+ * some neighbor within the support radius lies inside the cone of half-angle
+ * acos(virtual_light_cone_cosine) around it (wall neighbors only when
+ * include_wall_neighbors is set).  LSMPS_ERR_INVALID_ARGUMENT with the
+ * reference's message for a cone cosine outside [-1, 1] or no output buffer. */
+int lsmps_compute_virtual_light_diagnostics(lsmps_context* ctx, const lsmps_fluid_soa* fluid,
+                                            const lsmps_wall_soa* walls,
+                                            const lsmps_neighbor_list* fluid_neighbors,
+                                            const lsmps_neighbor_list* wall_neighbors,
+                                            const lsmps_config* config,
+                                            const lsmps_virtual_light_diagnostics* diagnostics);
+
 /* ---- moment matrices (include/lsmps3d/moment_matrix/moment_matrix.cuh) -- */
 /* DeviceMomentMatrix(fluid_capacity, config) — moment_matrix.cuh:36-40 */
 int lsmps_moment_create(lsmps_context* ctx, size_t fluid_capacity, const lsmps_config* config,

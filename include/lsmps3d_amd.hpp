@@ -22,10 +22,12 @@
 
 #include <hip/hip_runtime_api.h>
 
+#include <array>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <filesystem>
+#include <initializer_list>
 #include <iostream>
 #include <stdexcept>
 #include <string>
@@ -340,6 +342,274 @@ inline void set_precision(int precision_mode) { detail::settings().precision = p
 inline void set_exit_on_error(bool on) { detail::settings().exit_on_error = on; }
 inline lsmps_context* default_context() { return detail::context(); }
 
+// ---- device containers (particle_data.cuh, neighbor_list.cuh, cell_list.cuh,
+// core/workspace.cuh; src/core/runtime.cu) ----------------------------------------
+// Owners of the views above, with the reference's sizes: a neighbor list holds
+// particle_count + 1 offsets and neighbor_capacity indices, a cell list
+// particle_count ids and sorted indices plus cell_count begin / end entries.
+// resize() to the current shape keeps the buffers; contents are not initialised.
+namespace detail {
+template <typename T>
+void alloc_into(T*& p, size_type count) {
+  p = device_alloc<T>(count);
+}
+template <typename T>
+void free_into(T*& p) noexcept {
+  device_free(p);
+  p = nullptr;
+}
+}  // namespace detail
+
+class DeviceFluidParticles {
+ public:
+  DeviceFluidParticles() = default;
+  explicit DeviceFluidParticles(size_type count) { resize(count); }
+  ~DeviceFluidParticles() { release(); }
+  DeviceFluidParticles(const DeviceFluidParticles&) = delete;
+  DeviceFluidParticles& operator=(const DeviceFluidParticles&) = delete;
+  DeviceFluidParticles(DeviceFluidParticles&& o) noexcept : view_(std::exchange(o.view_, {})) {}
+  DeviceFluidParticles& operator=(DeviceFluidParticles&& o) noexcept {
+    if (this != &o) {
+      release();
+      view_ = std::exchange(o.view_, {});
+    }
+    return *this;
+  }
+  void resize(size_type count) {
+    if (count == view_.count) return;
+    release();
+    for (real** f : {&view_.x, &view_.y, &view_.z, &view_.vx, &view_.vy, &view_.vz, &view_.pressure})
+      detail::alloc_into(*f, count);
+    detail::alloc_into(view_.surface_type, count);
+    view_.count = count;
+  }
+  void release() noexcept {
+    for (real** f : {&view_.x, &view_.y, &view_.z, &view_.vx, &view_.vy, &view_.vz, &view_.pressure})
+      detail::free_into(*f);
+    detail::free_into(view_.surface_type);
+    view_.count = 0;
+  }
+  size_type count() const noexcept { return view_.count; }
+  size_type capacity() const noexcept { return view_.count; }
+  size_type bytes() const noexcept { return view_.count * (7 * sizeof(real) + sizeof(int)); }
+  FluidParticleSoA view() noexcept { return view_; }
+  const FluidParticleSoA& view() const noexcept { return view_; }
+
+ private:
+  FluidParticleSoA view_{};
+};
+
+class DeviceWallParticles {
+ public:
+  DeviceWallParticles() = default;
+  explicit DeviceWallParticles(size_type count) { resize(count); }
+  ~DeviceWallParticles() { release(); }
+  DeviceWallParticles(const DeviceWallParticles&) = delete;
+  DeviceWallParticles& operator=(const DeviceWallParticles&) = delete;
+  DeviceWallParticles(DeviceWallParticles&& o) noexcept : view_(std::exchange(o.view_, {})) {}
+  DeviceWallParticles& operator=(DeviceWallParticles&& o) noexcept {
+    if (this != &o) {
+      release();
+      view_ = std::exchange(o.view_, {});
+    }
+    return *this;
+  }
+  void resize(size_type count) {
+    if (count == view_.count) return;
+    release();
+    for (real** f : fields()) detail::alloc_into(*f, count);
+    view_.count = count;
+  }
+  void release() noexcept {
+    for (real** f : fields()) detail::free_into(*f);
+    view_.count = 0;
+  }
+  size_type count() const noexcept { return view_.count; }
+  size_type capacity() const noexcept { return view_.count; }
+  size_type bytes() const noexcept { return view_.count * 9 * sizeof(real); }
+  WallParticleSoA view() noexcept { return view_; }
+  const WallParticleSoA& view() const noexcept { return view_; }
+
+ private:
+  std::array<real**, 9> fields() noexcept {
+    return {&view_.x,  &view_.y,        &view_.z,        &view_.vx,      &view_.vy,
+            &view_.vz, &view_.normal_x, &view_.normal_y, &view_.normal_z};
+  }
+  WallParticleSoA view_{};
+};
+
+class DeviceNeighborList {
+ public:
+  DeviceNeighborList() = default;
+  DeviceNeighborList(size_type particle_count, size_type neighbor_capacity) {
+    resize(particle_count, neighbor_capacity);
+  }
+  ~DeviceNeighborList() { release(); }
+  DeviceNeighborList(const DeviceNeighborList&) = delete;
+  DeviceNeighborList& operator=(const DeviceNeighborList&) = delete;
+  DeviceNeighborList(DeviceNeighborList&& o) noexcept
+      : view_(std::exchange(o.view_, {})), capacity_(std::exchange(o.capacity_, 0)) {}
+  DeviceNeighborList& operator=(DeviceNeighborList&& o) noexcept {
+    if (this != &o) {
+      release();
+      view_ = std::exchange(o.view_, {});
+      capacity_ = std::exchange(o.capacity_, 0);
+    }
+    return *this;
+  }
+  void resize(size_type particle_count, size_type neighbor_capacity) {
+    if (particle_count == view_.particle_count && neighbor_capacity == capacity_) return;
+    release();
+    detail::alloc_into(view_.offsets, particle_count + 1);
+    detail::alloc_into(view_.indices, neighbor_capacity);
+    view_.particle_count = particle_count;
+    view_.neighbor_count = neighbor_capacity;
+    capacity_ = neighbor_capacity;
+  }
+  void release() noexcept {
+    detail::free_into(view_.offsets);
+    detail::free_into(view_.indices);
+    view_.particle_count = 0;
+    view_.neighbor_count = 0;
+    capacity_ = 0;
+  }
+  size_type particle_capacity() const noexcept { return view_.particle_count; }
+  size_type neighbor_capacity() const noexcept { return capacity_; }
+  size_type bytes() const noexcept {
+    return (view_.particle_count + 1) * sizeof(index_t) + capacity_ * sizeof(index_t);
+  }
+  NeighborListView view() noexcept { return view_; }
+  const NeighborListView& view() const noexcept { return view_; }
+
+ private:
+  NeighborListView view_{};
+  size_type capacity_ = 0;
+};
+
+class DeviceCellList {
+ public:
+  DeviceCellList() = default;
+  DeviceCellList(size_type particle_count, size_type cell_count) { resize(particle_count, cell_count); }
+  ~DeviceCellList() { release(); }
+  DeviceCellList(const DeviceCellList&) = delete;
+  DeviceCellList& operator=(const DeviceCellList&) = delete;
+  DeviceCellList(DeviceCellList&& o) noexcept : view_(std::exchange(o.view_, {})) {}
+  DeviceCellList& operator=(DeviceCellList&& o) noexcept {
+    if (this != &o) {
+      release();
+      view_ = std::exchange(o.view_, {});
+    }
+    return *this;
+  }
+  void resize(size_type particle_count, size_type cell_count) {
+    if (particle_count == view_.particle_count && cell_count == view_.cell_count) return;
+    release();
+    detail::alloc_into(view_.cell_ids, particle_count);
+    detail::alloc_into(view_.sorted_indices, particle_count);
+    detail::alloc_into(view_.cell_begin, cell_count);
+    detail::alloc_into(view_.cell_end, cell_count);
+    view_.particle_count = particle_count;
+    view_.cell_count = cell_count;
+  }
+  void release() noexcept {
+    detail::free_into(view_.cell_ids);
+    detail::free_into(view_.sorted_indices);
+    detail::free_into(view_.cell_begin);
+    detail::free_into(view_.cell_end);
+    view_.particle_count = 0;
+    view_.cell_count = 0;
+  }
+  size_type particle_capacity() const noexcept { return view_.particle_count; }
+  size_type cell_capacity() const noexcept { return view_.cell_count; }
+  size_type bytes() const noexcept {
+    return (view_.particle_count + view_.cell_count) * 2 * sizeof(index_t);
+  }
+  CellListView view() noexcept { return view_; }
+  const CellListView& view() const noexcept { return view_; }
+
+ private:
+  CellListView view_{};
+};
+
+// core/workspace.cuh:10-26: capacities of one simulation's device state.
+struct WorkspaceSpec {
+  size_type fluid_capacity{};
+  size_type wall_capacity{};
+  size_type max_fluid_neighbors_per_particle{};
+  size_type max_wall_neighbors_per_particle{};
+  size_type cell_capacity{};
+  size_type fluid_neighbor_capacity() const noexcept {
+    return fluid_capacity * max_fluid_neighbors_per_particle;
+  }
+  size_type wall_neighbor_capacity() const noexcept {
+    return fluid_capacity * max_wall_neighbors_per_particle;
+  }
+  size_type bytes() const noexcept {
+    const size_type offsets = (fluid_capacity + 1) * sizeof(index_t);
+    return fluid_capacity * (7 * sizeof(real) + sizeof(int)) + wall_capacity * 9 * sizeof(real) +
+           2 * offsets + (fluid_neighbor_capacity() + wall_neighbor_capacity()) * sizeof(index_t) +
+           (fluid_capacity + wall_capacity) * 2 * sizeof(index_t) +
+           2 * cell_capacity * 2 * sizeof(index_t);
+  }
+};
+
+struct SimulationWorkspaceView {
+  FluidParticleSoA fluid{};
+  WallParticleSoA walls{};
+  NeighborListView fluid_neighbors{};
+  NeighborListView wall_neighbors{};
+  CellListView fluid_cells{};
+  CellListView wall_cells{};
+};
+
+// core/workspace.cuh:28-61: the particle arrays, both neighbor tables and both
+// cell lists of one simulation.
+class SimulationWorkspace {
+ public:
+  SimulationWorkspace() = default;
+  explicit SimulationWorkspace(const WorkspaceSpec& spec) { resize(spec); }
+  SimulationWorkspace(const SimulationWorkspace&) = delete;
+  SimulationWorkspace& operator=(const SimulationWorkspace&) = delete;
+  SimulationWorkspace(SimulationWorkspace&&) noexcept = default;
+  SimulationWorkspace& operator=(SimulationWorkspace&&) noexcept = default;
+  void resize(const WorkspaceSpec& spec) {
+    fluid_.resize(spec.fluid_capacity);
+    walls_.resize(spec.wall_capacity);
+    fluid_neighbors_.resize(spec.fluid_capacity, spec.fluid_neighbor_capacity());
+    wall_neighbors_.resize(spec.fluid_capacity, spec.wall_neighbor_capacity());
+    fluid_cells_.resize(spec.fluid_capacity, spec.cell_capacity);
+    wall_cells_.resize(spec.wall_capacity, spec.cell_capacity);
+    spec_ = spec;
+  }
+  void release() noexcept {
+    fluid_.release();
+    walls_.release();
+    fluid_neighbors_.release();
+    wall_neighbors_.release();
+    fluid_cells_.release();
+    wall_cells_.release();
+    spec_ = {};
+  }
+  const WorkspaceSpec& spec() const noexcept { return spec_; }
+  size_type bytes() const noexcept {
+    return fluid_.bytes() + walls_.bytes() + fluid_neighbors_.bytes() + wall_neighbors_.bytes() +
+           fluid_cells_.bytes() + wall_cells_.bytes();
+  }
+  SimulationWorkspaceView view() noexcept {
+    return {fluid_.view(),           walls_.view(),       fluid_neighbors_.view(),
+            wall_neighbors_.view(), fluid_cells_.view(), wall_cells_.view()};
+  }
+
+ private:
+  WorkspaceSpec spec_{};
+  DeviceFluidParticles fluid_{};
+  DeviceWallParticles walls_{};
+  DeviceNeighborList fluid_neighbors_{};
+  DeviceNeighborList wall_neighbors_{};
+  DeviceCellList fluid_cells_{};
+  DeviceCellList wall_cells_{};
+};
+
 // ---- neighbor (neighbor_search.cuh:10-24) -------------------------------------
 
 inline void build_cell_list(const FluidParticleSoA& particles, const CellGrid& grid,
@@ -389,6 +659,21 @@ inline void classify_surface_particles(const FluidParticleSoA& fluid,
   detail::check(lsmps_classify_surface(detail::context(), &fluid, &walls, &fluid_neighbors,
                                        &wall_neighbors, &c, any ? &diagnostics : nullptr),
                 "classify_surface_particles");
+}
+
+// surface_detection.cuh:23-26,38-43
+using VirtualLightDiagnosticsView = lsmps_virtual_light_diagnostics;
+inline void compute_virtual_light_diagnostics(const FluidParticleSoA& fluid,
+                                              const WallParticleSoA& walls,
+                                              const NeighborListView& fluid_neighbors,
+                                              const NeighborListView& wall_neighbors,
+                                              const SimulationConfig& config,
+                                              VirtualLightDiagnosticsView diagnostics) {
+  const lsmps_config c = detail::to_c(config);
+  detail::check(lsmps_compute_virtual_light_diagnostics(detail::context(), &fluid, &walls,
+                                                        &fluid_neighbors, &wall_neighbors, &c,
+                                                        &diagnostics),
+                "compute_virtual_light_diagnostics");
 }
 
 // ---- moment matrices (moment_matrix.cuh:36-68) -------------------------------

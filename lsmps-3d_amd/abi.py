@@ -79,6 +79,10 @@ class SurfaceDiagnostics(C.Structure):
                 ("surface_normal_y", _dp), ("surface_normal_z", _dp)]
 
 
+class VirtualLightDiagnostics(C.Structure):
+    _fields_ = [("open_direction_count", _ip), ("open_fraction", _dp)]
+
+
 class Csr(C.Structure):
     _fields_ = [("rows", C.c_size_t), ("cols", C.c_size_t), ("nnz", C.c_size_t),
                 ("row_offsets", _ip), ("col_indices", _ip), ("values", _dp)]

@@ -261,6 +261,21 @@ int lsmps_classify_surface(lsmps_context* ctx, const lsmps_fluid_soa* fluid,
   API_END("lsmps_classify_surface")
 }
 
+int lsmps_compute_virtual_light_diagnostics(lsmps_context* ctx, const lsmps_fluid_soa* fluid,
+                                            const lsmps_wall_soa* walls,
+                                            const lsmps_neighbor_list* fn,
+                                            const lsmps_neighbor_list* wn,
+                                            const lsmps_config* config,
+                                            const lsmps_virtual_light_diagnostics* diagnostics) {
+  API_BEGIN
+  Context& c = C(ctx);
+  virtual_light(c, deref(fluid, "fluid"), deref(walls, "walls"), deref(fn, "fluid neighbors"),
+                deref(wn, "wall neighbors"), deref(config, "config"),
+                deref(diagnostics, "virtual light diagnostics"));
+  c.finish();
+  API_END("lsmps_compute_virtual_light_diagnostics")
+}
+
 // ---- moments ----------------------------------------------------------------
 int lsmps_moment_create(lsmps_context* ctx, size_t capacity, const lsmps_config* config,
                         lsmps_moment_matrix** out) {

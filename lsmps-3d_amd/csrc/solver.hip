@@ -1560,8 +1560,10 @@ void idr_solve_segmented(Context& ctx, PpeSolver& S, size_t n, const SellMatrix&
   const bool schwarz = S.schwarz.blocks > 0 && S.schwarz.n == n;
   const V* dia = sizeof(V) == sizeof(double) ? reinterpret_cast<const V*>(M.dia64.data())
                                              : reinterpret_cast<const V*>(M.dia32.data());
-  IdrVec<T> iv{n, ld, s, r, x, v, t, G, U, p0, S.scalars.data(), S.part_b.data(),
-               schwarz ? S.schwarz.pdiag.data() : dinv, g.row_seg};
+  // the segmented recurrence applies the Schwarz blocks with schwarz_apply, so
+  // the fused completion pointers stay null
+  IdrVec<T> iv{n,  ld, s, r, x, v, t, G, U, p0, S.scalars.data(), S.part_b.data(),
+               schwarz ? S.schwarz.pdiag.data() : dinv, g.row_seg, nullptr, nullptr, nullptr};
   const unsigned vb = vec_blocks(n);
   const unsigned sb = sell_blocks(M.chunks);
   const unsigned rb = static_cast<unsigned>(nseg * kSegBlocks);

@@ -44,6 +44,9 @@ double reference_number_density(double l0, double re, int kernel);
 void classify_surface(Context& ctx, const lsmps_fluid_soa& fluid, const lsmps_wall_soa& walls,
                       const lsmps_neighbor_list& fn, const lsmps_neighbor_list& wn,
                       const lsmps_config& config, const lsmps_surface_diagnostics* diagnostics);
+void virtual_light(Context& ctx, const lsmps_fluid_soa& fluid, const lsmps_wall_soa& walls,
+                   const lsmps_neighbor_list& fn, const lsmps_neighbor_list& wn,
+                   const lsmps_config& cfg, const lsmps_virtual_light_diagnostics& out);
 
 // ---- moment.hip
 struct MomentOutputs {

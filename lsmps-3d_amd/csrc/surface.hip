@@ -183,6 +183,76 @@ __global__ void __launch_bounds__(kBlock) expand_kernel(
   labels[i] = type;
 }
 
+// The 14 virtual-light directions in the reference's order
+// (virtual_light_direction, surface_detection.cu:350-383): the six axis
+// directions, then the diagonals of the xy, xz and yz planes it lists.
+constexpr int kLightDirections = 14;
+constexpr unsigned kAllBlocked = (1u << kLightDirections) - 1u;
+constexpr double kLight[kLightDirections][3] = {
+    {1.0, 0.0, 0.0},
+    {-1.0, 0.0, 0.0},
+    {0.0, 1.0, 0.0},
+    {0.0, -1.0, 0.0},
+    {0.0, 0.0, 1.0},
+    {0.0, 0.0, -1.0},
+    {0.7071067811865475, 0.7071067811865475, 0.0},
+    {0.7071067811865475, -0.7071067811865475, 0.0},
+    {-0.7071067811865475, 0.7071067811865475, 0.0},
+    {-0.7071067811865475, -0.7071067811865475, 0.0},
+    {0.7071067811865475, 0.0, 0.7071067811865475},
+    {0.7071067811865475, 0.0, -0.7071067811865475},
+    {0.0, 0.7071067811865475, 0.7071067811865475},
+    {0.0, 0.7071067811865475, -0.7071067811865475},
+};
+
+// Adds to `blocked` the directions whose cone holds the neighbor at (dx, dy, dz):
+// the projection (d . u) / |d| >= cone cosine, for 0 < |d|^2 <= re^2
+// (fluid_neighbor_blocks_light / wall_neighbor_blocks_light,
+// surface_detection.cu:385-447).  One pass over a row tests all 14 directions,
+// which gives the same set as the reference's one pass per direction.
+__device__ __forceinline__ unsigned light_blocked(double dx, double dy, double dz, double re2,
+                                                  double cone) {
+  const double d2 = dx * dx + dy * dy + dz * dz;
+  if (d2 <= 0.0 || d2 > re2) return 0u;
+  const double inv = safe_inv_sqrt(d2);
+  unsigned m = 0u;
+#pragma unroll
+  for (int k = 0; k < kLightDirections; ++k) {
+    const double proj = (dx * kLight[k][0] + dy * kLight[k][1] + dz * kLight[k][2]) * inv;
+    if (proj >= cone) m |= 1u << k;
+  }
+  return m;
+}
+
+// virtual_light_kernel (surface_detection.cu:449-475): one thread per fluid
+// particle; the wall row is read only while some direction is still open.
+__global__ void __launch_bounds__(kBlock) virtual_light_kernel(
+    size_t n, const double* __restrict__ x, const double* __restrict__ y,
+    const double* __restrict__ z, const double* __restrict__ wx, const double* __restrict__ wy,
+    const double* __restrict__ wz, const int32_t* __restrict__ f_off,
+    const int32_t* __restrict__ f_idx, const int32_t* __restrict__ w_off,
+    const int32_t* __restrict__ w_idx, double re2, double cone, int32_t include_walls,
+    int32_t* __restrict__ open_count, double* __restrict__ open_fraction) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double px = x[i], py = y[i], pz = z[i];
+  unsigned blocked = 0u;
+  for (int32_t c = f_off[i], e = f_off[i + 1]; c < e && blocked != kAllBlocked; ++c) {
+    const int32_t j = f_idx[c];
+    blocked |= light_blocked(x[j] - px, y[j] - py, z[j] - pz, re2, cone);
+  }
+  if (include_walls) {
+    for (int32_t c = w_off[i], e = w_off[i + 1]; c < e && blocked != kAllBlocked; ++c) {
+      const int32_t j = w_idx[c];
+      blocked |= light_blocked(wx[j] - px, wy[j] - py, wz[j] - pz, re2, cone);
+    }
+  }
+  const int open = kLightDirections - __popc(blocked);
+  if (open_count) open_count[i] = open;
+  if (open_fraction)
+    open_fraction[i] = static_cast<double>(open) / static_cast<double>(kLightDirections);
+}
+
 }  // namespace
 
 // compute_uniform_reference_number_density (surface_detection.cu:494-520), host FP64.
@@ -244,6 +314,32 @@ void classify_surface(Context& ctx, const lsmps_fluid_soa& fluid, const lsmps_wa
   expand_kernel<<<blocks_for(fluid.count), kBlock, 0, ctx.stream>>>(
       fluid.count, fluid.x, fluid.y, fluid.z, fn.offsets, fn.indices, primary, r2,
       fluid.surface_type);
+  LSMPS_LAUNCH_CHECK();
+}
+
+}  // namespace lsmps
+
+namespace lsmps {
+
+// compute_virtual_light_diagnostics (surface_detection.cu:555-578): the same
+// input checks and messages, then one launch.
+void virtual_light(Context& ctx, const lsmps_fluid_soa& fluid, const lsmps_wall_soa& walls,
+                   const lsmps_neighbor_list& fn, const lsmps_neighbor_list& wn,
+                   const lsmps_config& cfg, const lsmps_virtual_light_diagnostics& out) {
+  require(cfg.support_radius > 0.0, "Surface detection support radius must be positive");
+  require(fluid.count <= fn.particle_count && fluid.count <= wn.particle_count,
+          "Surface detection neighbor lists are smaller than fluid particle count");
+  // written as the reference's test, so a NaN cosine passes it as it does there
+  require(!(cfg.virtual_light_cone_cosine < -1.0 || cfg.virtual_light_cone_cosine > 1.0),
+          "Virtual light cone cosine must be in [-1, 1]");
+  require(out.open_direction_count != nullptr || out.open_fraction != nullptr,
+          "Virtual light diagnostics require at least one output buffer");
+  if (fluid.count == 0) return;
+  const double re2 = cfg.support_radius * cfg.support_radius;
+  virtual_light_kernel<<<blocks_for(fluid.count), kBlock, 0, ctx.stream>>>(
+      fluid.count, fluid.x, fluid.y, fluid.z, walls.x, walls.y, walls.z, fn.offsets, fn.indices,
+      wn.offsets, wn.indices, re2, cfg.virtual_light_cone_cosine,
+      cfg.include_wall_neighbors ? 1 : 0, out.open_direction_count, out.open_fraction);
   LSMPS_LAUNCH_CHECK();
 }
 

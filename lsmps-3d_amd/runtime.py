@@ -37,7 +37,7 @@ EXPORTS = [
     "lsmps_time_manager_mark_initial_output", "lsmps_max_velocity", "lsmps_stepper_run",
     "lsmps_stepper_refresh_geometry", "lsmps_default_config_files", "lsmps_config_validate",
     "lsmps_config_load", "lsmps_config_save", "lsmps_read_fluid_csv", "lsmps_read_wall_csv",
-    "lsmps_vtk_path", "lsmps_write_vtk",
+    "lsmps_vtk_path", "lsmps_write_vtk", "lsmps_compute_virtual_light_diagnostics",
 ]
 
 _LIB = None
@@ -506,6 +506,74 @@ def max_velocity(ctx, state):
     fs = state.fluid_soa()
     check(ctx.lib.lsmps_max_velocity(ctx.handle, C.byref(fs), C.byref(out)), "max_velocity")
     return out.value
+
+
+def surface_on_device(case, config=None, max_neighbors=256, virtual_light=True,
+                      device="cuda:0"):
+    """Neighbor lists, surface labels with every diagnostic and (optionally) the
+    virtual-light counts of `case` on the GPU through the module entries
+    (build_neighbor_lists, classify_surface_particles,
+    compute_virtual_light_diagnostics); numpy outputs."""
+    torch = _torch()
+    cfg = config or case.config
+    ctx = Context(abi.PRECISION_FP64)
+    state = DeviceState(case, device)
+    n, nw = case.n_fluid, case.n_wall
+    cells = int(cfg.cell_dims[0]) * int(cfg.cell_dims[1]) * int(cfg.cell_dims[2])
+
+    def i32(k):
+        return torch.zeros(max(k, 1), dtype=torch.int32, device=device)
+
+    def f64(k):
+        return torch.zeros(max(k, 1), dtype=torch.float64, device=device)
+
+    fc = [i32(n), i32(n), i32(cells), i32(cells)]
+    wc = [i32(nw), i32(nw), i32(cells), i32(cells)]
+    cap = n * max_neighbors
+    fo, fi, wo, wi = i32(n + 1), i32(cap), i32(n + 1), i32(cap)
+    fcl = abi.CellList(n, cells, *(abi.iptr(ptr(t)) for t in fc))
+    wcl = abi.CellList(nw, cells, *(abi.iptr(ptr(t)) for t in wc))
+    fnl = abi.NeighborList(n, cap, abi.iptr(ptr(fo)), abi.iptr(ptr(fi)))
+    wnl = abi.NeighborList(n, cap, abi.iptr(ptr(wo)), abi.iptr(ptr(wi)))
+    fs, ws = state.fluid_soa(), state.wall_soa()
+    lib = ctx.lib
+    check(lib.lsmps_build_neighbor_lists(ctx.handle, C.byref(fs), C.byref(ws), C.byref(cfg),
+                                         C.byref(fcl), C.byref(wcl), C.byref(fnl), C.byref(wnl)),
+          "build_neighbor_lists")
+    ints = {k: i32(n) for k in ("fluid_neighbor_count", "wall_neighbor_count")}
+    reals = {k: f64(n) for k in ("number_density", "number_density_ratio", "anisotropy",
+                                 "air_open_ratio", "air_anisotropy", "surface_normal_x",
+                                 "surface_normal_y", "surface_normal_z")}
+    diag = abi.SurfaceDiagnostics(abi.iptr(ptr(ints["fluid_neighbor_count"])),
+                                  abi.iptr(ptr(ints["wall_neighbor_count"])),
+                                  *(abi.dptr(ptr(reals[k])) for k in (
+                                      "number_density", "number_density_ratio", "anisotropy",
+                                      "air_open_ratio", "air_anisotropy", "surface_normal_x",
+                                      "surface_normal_y", "surface_normal_z")))
+    check(lib.lsmps_classify_surface(ctx.handle, C.byref(fs), C.byref(ws), C.byref(fnl),
+                                     C.byref(wnl), C.byref(cfg), C.byref(diag)),
+          "classify_surface")
+    out = {}
+    if virtual_light:
+        cnt, frac = i32(n), f64(n)
+        vl = abi.VirtualLightDiagnostics(abi.iptr(ptr(cnt)), abi.dptr(ptr(frac)))
+        check(lib.lsmps_compute_virtual_light_diagnostics(
+            ctx.handle, C.byref(fs), C.byref(ws), C.byref(fnl), C.byref(wnl), C.byref(cfg),
+            C.byref(vl)), "compute_virtual_light_diagnostics")
+        out["open_direction_count"] = cnt[:n].cpu().numpy()
+        out["open_fraction"] = frac[:n].cpu().numpy()
+    torch.cuda.synchronize()
+    f_off = fo[:n + 1].cpu().numpy()
+    w_off = wo[:n + 1].cpu().numpy()
+    out["fluid_offsets"] = f_off
+    out["fluid_indices"] = fi[:int(f_off[-1])].cpu().numpy()
+    out["wall_offsets"] = w_off
+    out["wall_indices"] = wi[:int(w_off[-1])].cpu().numpy()
+    out["surface_type"] = state.fluid["surface_type"][:n].cpu().numpy()
+    for k, t in list(ints.items()) + list(reals.items()):
+        out[k] = t[:n].cpu().numpy()
+    ctx.close()
+    return out
 
 
 def device_to_numpy(p, n, dtype):

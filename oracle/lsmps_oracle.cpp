@@ -21,8 +21,7 @@
 //    a relative true-residual tolerance, from p = 0, followed by the same clamp
 //    p = max(p, 0) (ppe_matrix.cu:662).
 //
-// Parity status per function is recorded in the oracle section of DESIGN.md and
-// in tests/test_oracle_*.py, which pin each function against the reference's own
+// tests/test_oracle_*.py pin each function against the reference's own
 // unit-test known answers, closed forms and brute force.
 
 #include <algorithm>
@@ -427,6 +426,69 @@ int classify_surface(const lsmps_fluid_soa& fluid, const lsmps_wall_soa& walls,
         }
       }
     }
+  }
+  return LSMPS_OK;
+}
+
+// virtual_light_direction (surface_detection.cu:350-383).
+void light_direction(int k, double& ux, double& uy, double& uz) {
+  const double h = 0.7071067811865475;
+  static const double table[14][3] = {{1, 0, 0},  {-1, 0, 0}, {0, 1, 0},  {0, -1, 0}, {0, 0, 1},
+                                      {0, 0, -1}, {h, h, 0},  {h, -h, 0}, {-h, h, 0}, {-h, -h, 0},
+                                      {h, 0, h},  {h, 0, -h}, {0, h, h},  {0, h, -h}};
+  ux = table[k][0];
+  uy = table[k][1];
+  uz = table[k][2];
+}
+
+// fluid_neighbor_blocks_light / wall_neighbor_blocks_light (surface_detection.cu:385-447):
+// does some neighbor of row i lie within the cone around direction k?
+bool row_blocks_light(size_t i, int k, const lsmps_fluid_soa& fluid, const double* nx,
+                      const double* ny, const double* nz, const lsmps_neighbor_list& nl,
+                      const lsmps_config& cfg) {
+  double ux, uy, uz;
+  light_direction(k, ux, uy, uz);
+  const double px = fluid.x[i], py = fluid.y[i], pz = fluid.z[i];
+  for (int32_t c = nl.offsets[i]; c < nl.offsets[i + 1]; ++c) {
+    const int32_t j = nl.indices[c];
+    const double dx = nx[j] - px, dy = ny[j] - py, dz = nz[j] - pz;
+    const double d2 = dx * dx + dy * dy + dz * dz;
+    if (d2 <= 0.0 || d2 > cfg.support_radius * cfg.support_radius) continue;
+    const double proj = (dx * ux + dy * uy + dz * uz) * (1.0 / std::sqrt(d2 > 1.0e-20 ? d2 : 1.0e-20));
+    if (proj >= cfg.virtual_light_cone_cosine) return true;
+  }
+  return false;
+}
+
+// compute_virtual_light_diagnostics + virtual_light_kernel
+// (surface_detection.cu:449-475,555-578): direction by direction, walls only for a
+// direction the fluid left open.
+int virtual_light(const lsmps_fluid_soa& fluid, const lsmps_wall_soa& walls,
+                  const lsmps_neighbor_list& fn, const lsmps_neighbor_list& wn,
+                  const lsmps_config& cfg, int32_t* open_count, double* open_fraction) {
+  if (cfg.support_radius <= 0.0)
+    return fail(LSMPS_ERR_INVALID_ARGUMENT, "Surface detection support radius must be positive");
+  if (fluid.count > fn.particle_count || fluid.count > wn.particle_count)
+    return fail(LSMPS_ERR_INVALID_ARGUMENT,
+                "Surface detection neighbor lists are smaller than fluid particle count");
+  if (cfg.virtual_light_cone_cosine < -1.0 || cfg.virtual_light_cone_cosine > 1.0)
+    return fail(LSMPS_ERR_INVALID_ARGUMENT, "Virtual light cone cosine must be in [-1, 1]");
+  if (open_count == nullptr && open_fraction == nullptr)
+    return fail(LSMPS_ERR_INVALID_ARGUMENT,
+                "Virtual light diagnostics require at least one output buffer");
+  const int64_t n = static_cast<int64_t>(fluid.count);
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int64_t s = 0; s < n; ++s) {
+    const size_t i = static_cast<size_t>(s);
+    int open = 0;
+    for (int k = 0; k < 14; ++k) {
+      bool blocked = row_blocks_light(i, k, fluid, fluid.x, fluid.y, fluid.z, fn, cfg);
+      if (!blocked && cfg.include_wall_neighbors)
+        blocked = row_blocks_light(i, k, fluid, walls.x, walls.y, walls.z, wn, cfg);
+      if (!blocked) ++open;
+    }
+    if (open_count) open_count[i] = open;
+    if (open_fraction) open_fraction[i] = static_cast<double>(open) / 14.0;
   }
   return LSMPS_OK;
 }
@@ -1506,6 +1568,12 @@ int orc_classify_surface(const lsmps_fluid_soa* fluid, const lsmps_wall_soa* wal
                          const lsmps_neighbor_list* fn, const lsmps_neighbor_list* wn,
                          const lsmps_config* cfg, const lsmps_surface_diagnostics* diag) {
   return classify_surface(*fluid, *walls, *fn, *wn, *cfg, diag);
+}
+
+int orc_virtual_light(const lsmps_fluid_soa* fluid, const lsmps_wall_soa* walls,
+                      const lsmps_neighbor_list* fn, const lsmps_neighbor_list* wn,
+                      const lsmps_config* cfg, int32_t* open_count, double* open_fraction) {
+  return virtual_light(*fluid, *walls, *fn, *wn, *cfg, open_count, open_fraction);
 }
 
 // One moment kind: matrices (may be NULL) and inverses are N*n*n column-major.

@@ -151,6 +151,22 @@ class Oracle:
         out["surface_type"] = fluid["surface_type"]
         return out
 
+    def virtual_light(self, case, nb, config=None):
+        """compute_virtual_light_diagnostics: open direction counts and fractions."""
+        a = self.abi
+        cfg = config or case.config
+        n = case.n_fluid
+        count = np.zeros(n, np.int32)
+        frac = np.zeros(n)
+        fs = self.fluid_soa(case.fluid)
+        ws = self.wall_soa(case.walls)
+        fn = self.nlist(nb["fluid_offsets"], nb["fluid_indices"])
+        wn = self.nlist(nb["wall_offsets"], nb["wall_indices"])
+        self._check(self.lib.orc_virtual_light(C.byref(fs), C.byref(ws), C.byref(fn), C.byref(wn),
+                                               C.byref(cfg), a.iptr(_p(count)), a.dptr(_p(frac))),
+                    "virtual_light")
+        return {"open_direction_count": count, "open_fraction": frac}
+
     def moments(self, case, nb, kind, config=None, want_matrices=False):
         a = self.abi
         cfg = config or case.config

@@ -60,3 +60,29 @@ def brute_force_neighbors(px, py, pz, sx, sy, sz, radius, exclude_self):
             mask[i] = False
         rows.append(np.nonzero(mask)[0])
     return rows
+
+
+def parse_vtk(path):
+    """Legacy ASCII VTK file -> points, vectors, real and int scalars (numpy)."""
+    toks = open(path).read().split("\n")
+    out = {"vectors": {}, "real": {}, "int": {}}
+    i = 0
+    while i < len(toks):
+        t = toks[i]
+        if t.startswith("POINTS"):
+            n = int(t.split()[1])
+            out["points"] = np.array([[float(v) for v in toks[i + 1 + k].split()] for k in range(n)])
+            i += n
+        elif t.startswith("VECTORS"):
+            name = t.split()[1]
+            out["vectors"][name] = np.array([[float(v) for v in toks[i + 1 + k].split()]
+                                             for k in range(n)])
+            i += n
+        elif t.startswith("SCALARS"):
+            _, name, typ, _ = t.split()
+            vals = [toks[i + 2 + k] for k in range(n)]
+            out["int" if typ == "int" else "real"][name] = np.array(
+                [int(v) for v in vals] if typ == "int" else [float(v) for v in vals])
+            i += n + 1
+        i += 1
+    return out

@@ -8,6 +8,8 @@ import os
 import numpy as np
 import pytest
 
+from helpers import parse_vtk
+
 
 def test_config_ini_known_answers(rt, abi, golden, tmp_path):
     fx = golden["config"]
@@ -128,31 +130,6 @@ def test_csv_round_trip_of_a_case(rt, cases, tmp_path):
         assert np.array_equal(w[k], wl[k]), k
 
 
-def _parse_vtk(path):
-    toks = open(path).read().split("\n")
-    out = {"vectors": {}, "real": {}, "int": {}}
-    i = 0
-    while i < len(toks):
-        t = toks[i]
-        if t.startswith("POINTS"):
-            n = int(t.split()[1])
-            out["points"] = np.array([[float(v) for v in toks[i + 1 + k].split()] for k in range(n)])
-            i += n
-        elif t.startswith("VECTORS"):
-            name = t.split()[1]
-            out["vectors"][name] = np.array([[float(v) for v in toks[i + 1 + k].split()]
-                                             for k in range(n)])
-            i += n
-        elif t.startswith("SCALARS"):
-            _, name, typ, _ = t.split()
-            vals = [toks[i + 2 + k] for k in range(n)]
-            out["int" if typ == "int" else "real"][name] = np.array(
-                [int(v) for v in vals] if typ == "int" else [float(v) for v in vals])
-            i += n + 1
-        i += 1
-    return out
-
-
 def test_vtk_writer_known_answers(rt, golden, tmp_path):
     fx = golden["vtk_writer"]
     p = fx["points"]
@@ -164,7 +141,7 @@ def test_vtk_writer_known_answers(rt, golden, tmp_path):
     text = open(path).read()
     for t in fx["expected_tokens"]:
         assert t in text, t
-    parsed = _parse_vtk(path)
+    parsed = parse_vtk(path)
     assert np.allclose(parsed["points"], np.array([p["x"], p["y"], p["z"]]).T)
     for k, v in fx["vectors"].items():
         assert np.allclose(parsed["vectors"][k], np.array(v).T)
