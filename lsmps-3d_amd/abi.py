@@ -249,7 +249,7 @@ def default_solver_options():
     o.inner_tolerance = 1.0e-3
     o.check_interval = 16
     o.clamp_negative = 1
-    o.shadow_dimension = 12
+    o.shadow_dimension = 16
     o.preconditioner = PRECOND_SCHWARZ
     return o
 

@@ -436,7 +436,7 @@ int lsmps_default_solver_options(lsmps_solver_options* out) {
   o.inner_tolerance = 1.0e-3;
   o.check_interval = 16;
   o.clamp_negative = 1;
-  o.shadow_dimension = 12;
+  o.shadow_dimension = 16;
   o.preconditioner = LSMPS_PRECOND_SCHWARZ;
   *out = o;
   API_END("lsmps_default_solver_options")

@@ -334,7 +334,8 @@ constexpr int32_t kDiaEmpty = INT32_MIN;
 // the atomics.
 __global__ void __launch_bounds__(kSellChunk) dia_select_kernel(
     size_t n, const int32_t* rows, const int32_t* cols, int32_t* dia_count,
-    int32_t* dia_offsets, int32_t* dia_slots, int32_t* rem_width, int32_t* rem_scaled,
+    int32_t* dia_offsets, int32_t* dia_slot, int32_t* dia_plan, int32_t* dia_pairs,
+    int32_t* dia_singles, int32_t* dia_slots, int32_t* rem_width, int32_t* rem_scaled,
     unsigned long long* rem_total) {
   __shared__ int32_t keys[kDiaTable];
   __shared__ int32_t counts[kDiaTable];
@@ -414,19 +415,55 @@ __global__ void __launch_bounds__(kSellChunk) dia_select_kernel(
     width = max(width, __shfl_xor(width, off, 64));
     total += __shfl_xor(total, off, 64);
   }
-  // Both parts are stored in quads (kQuad consecutive entries of one row per
-  // lane), so the diagonal count and the width are padded to multiples of
-  // kQuad.  Padding diagonals repeat the last offset (the list stays sorted for
-  // dia_find) and keep value 0.
-  const int nd4 = (nd + kQuad - 1) / kQuad * kQuad;
-  const int32_t w4 = (width + kQuad - 1) / kQuad * kQuad;
-  for (int i = nd + lane; i < nd4; i += kSellChunk)
-    dia_offsets[chunk * kMaxDia + i] = nd > 0 ? list[nd - 1] : 0;
+  // Storage plan (lane 0; nd <= kMaxDia): pairs (d, d + 1) greedily along the
+  // runs of the sorted list, kept only when every pair reads inside [0, n) for
+  // every real row of the chunk; then the singles.  Pair entries are padded to
+  // an even count and singles to a multiple of 4, so both parts fill whole
+  // quads; padding repeats an offset of its part (value 0).
+  const int32_t width4 = (width + kQuad - 1) / kQuad * kQuad;
   if (lane == 0) {
-    dia_count[chunk] = nd4;
-    dia_slots[chunk] = nd4 * kSellChunk;
-    rem_width[chunk] = w4;
-    rem_scaled[chunk] = w4 * kSellChunk;
+    const int64_t row0 = static_cast<int64_t>(chunk) * kSellChunk;
+    const int64_t rend = row0 + kSellChunk - 1;
+    const int64_t rlast = rend < static_cast<int64_t>(n) - 1 ? rend : static_cast<int64_t>(n) - 1;
+    bool safe = true;
+    int pairs = 0;
+    for (int i = 0; i < nd;) {
+      if (i + 1 < nd && list[i + 1] == list[i] + 1) {
+        if (row0 + list[i] < 0 || rlast + list[i] + 1 > static_cast<int64_t>(n) - 1) safe = false;
+        ++pairs;
+        i += 2;
+      } else {
+        ++i;
+      }
+    }
+    if (!safe) pairs = 0;
+    const int p2 = (pairs + 1) / 2 * 2;
+    int32_t* plan = dia_plan + chunk * kMaxPlan;
+    int32_t* slot_of = dia_slot + chunk * kMaxDia;
+    int np = 0, ns = 0;
+    for (int i = 0; i < nd;) {
+      if (pairs > 0 && i + 1 < nd && list[i + 1] == list[i] + 1) {
+        plan[np] = list[i];
+        slot_of[i] = 2 * np;
+        slot_of[i + 1] = 2 * np + 1;
+        ++np;
+        i += 2;
+      } else {
+        plan[p2 + ns] = list[i];
+        slot_of[i] = 2 * p2 + ns;
+        ++ns;
+        i += 1;
+      }
+    }
+    for (int q = np; q < p2; ++q) plan[q] = plan[0];
+    const int s4 = (ns + kQuad - 1) / kQuad * kQuad;
+    for (int q = ns; q < s4; ++q) plan[p2 + q] = ns > 0 ? plan[p2 + ns - 1] : 0;
+    dia_count[chunk] = nd;
+    dia_pairs[chunk] = p2;
+    dia_singles[chunk] = s4;
+    dia_slots[chunk] = (2 * p2 + s4) * kSellChunk;
+    rem_width[chunk] = width4;
+    rem_scaled[chunk] = width4 * kSellChunk;
     atomicAdd(rem_total, total);
   }
 }
@@ -510,7 +547,7 @@ template <typename V>
 __global__ void sell_fill_kernel(size_t n, size_t chunks, const int32_t* rows,
                                  const int32_t* cols, const double* vals,
                                  const int32_t* dia_count, const int32_t* dia_offsets,
-                                 const int32_t* dia_start, V* dia_vals,
+                                 const int32_t* dia_slot, const int32_t* dia_start, V* dia_vals,
                                  const int32_t* chunk_start, const int32_t* width,
                                  const int32_t* col_start, const int32_t* nwin,
                                  const int32_t* windows, uint16_t* out_cols, V* out_vals) {
@@ -545,7 +582,7 @@ __global__ void sell_fill_kernel(size_t n, size_t chunks, const int32_t* rows,
     const int32_t c = cols[b + q];
     const int j = dia_find(dl, nd, c - static_cast<int32_t>(r));
     if (j >= 0) {
-      dia_vals[dbase + quad_slot(j, lane)] = static_cast<V>(vals[b + q]);
+      dia_vals[dbase + quad_slot(dia_slot[chunk * kMaxDia + j], lane)] = static_cast<V>(vals[b + q]);
     } else {
       put(c, static_cast<V>(vals[b + q]));
     }
@@ -557,8 +594,9 @@ template <typename T, typename V>
 struct SellArgs {
   size_t n;
   size_t chunks;
-  const int32_t* dia_count;
-  const int32_t* dia_offsets;
+  const int32_t* dia_pairs;
+  const int32_t* dia_singles;
+  const int32_t* dia_plan;
   const int32_t* dia_start;
   const V* dia_vals;
   const int32_t* chunk_start;
@@ -586,60 +624,170 @@ __device__ __forceinline__ unsigned xcd_block() {
   return (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
 }
 
+// Gathers of x through a raw buffer resource: 32-bit byte offsets (no 64-bit
+// address arithmetic per entry) and hardware range checking, which returns 0
+// for any offset outside [0, n) -- negative ones wrap to huge unsigned values.
+// The padding slots of the diagonals (value 0) may therefore point anywhere.
+// Word 3 is the gfx9-family raw-buffer format (32-bit data, no swizzle).
+constexpr int kBufferWord3 = 0x00020000;
+template <typename T>
+struct XGather;
+template <>
+struct XGather<double> {
+  __amdgpu_buffer_rsrc_t rsrc;
+  __device__ XGather(const double* p, size_t n)
+      : rsrc(__builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(p), 0,
+                                               static_cast<int>(n * sizeof(double)), kBufferWord3)) {}
+  __device__ double operator()(uint32_t byte_offset) const {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rsrc, byte_offset, 0, 0));
+  }
+  // x[i] and x[i + 1] in one 16-byte load (8-byte aligned; the plan only pairs
+  // offsets whose two columns lie inside [0, n) for every row of the chunk).
+  __device__ void pair(uint32_t byte_offset, double& lo, double& hi) const {
+    typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+    const u32x4 q = __builtin_bit_cast(u32x4, __builtin_amdgcn_raw_buffer_load_b128(rsrc, byte_offset, 0, 0));
+    lo = __builtin_bit_cast(double, static_cast<unsigned long long>(q.x) |
+                                        (static_cast<unsigned long long>(q.y) << 32));
+    hi = __builtin_bit_cast(double, static_cast<unsigned long long>(q.z) |
+                                        (static_cast<unsigned long long>(q.w) << 32));
+  }
+};
+template <>
+struct XGather<float> {
+  __amdgpu_buffer_rsrc_t rsrc;
+  __device__ XGather(const float* p, size_t n)
+      : rsrc(__builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(p), 0,
+                                               static_cast<int>(n * sizeof(float)), kBufferWord3)) {}
+  __device__ float operator()(uint32_t byte_offset) const {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rsrc, byte_offset, 0, 0));
+  }
+  __device__ void pair(uint32_t byte_offset, float& lo, float& hi) const {
+    const unsigned long long q =
+        __builtin_bit_cast(unsigned long long, __builtin_amdgcn_raw_buffer_load_b64(rsrc, byte_offset, 0, 0));
+    lo = __builtin_bit_cast(float, static_cast<unsigned int>(q));
+    hi = __builtin_bit_cast(float, static_cast<unsigned int>(q >> 32));
+  }
+};
+// Largest system the 32-bit buffer offsets address.
+constexpr size_t kMaxGatherRows = (size_t{1} << 31) / sizeof(double) - 1;
+
 // One wavefront per 64-row chunk, lane = row.  Each workgroup owns a contiguous
 // range of chunks (XCD-aware order above); its kSellBlock/64 waves interleave
 // within it.  MODE 1: shadow dots P_q . y.  MODE 2: y.r and y.y.
+//
+// The chunk index is made wave-uniform (readfirstlane), so the chunk
+// descriptors and the diagonal offsets come through scalar loads and the loops
+// branch on scalars.  Each DIA step issues the gathers of 8 entries, then the
+// next step's value quads and offsets, then consumes the gathers: the
+// prefetches stay in flight across the wait.
 constexpr int kSellBlock = 512;
 template <typename T, typename V, int MODE>
 __global__ void __launch_bounds__(kSellBlock) sell_spmv_kernel(SellArgs<T, V> a) {
   const int lane = threadIdx.x & 63;
-  const unsigned wave_in_block = threadIdx.x >> 6;
+  const unsigned wave_in_block = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const size_t per_block = (a.chunks + gridDim.x - 1) / gridDim.x;
   const size_t first = static_cast<size_t>(xcd_block()) * per_block;
   const size_t last = first + per_block < a.chunks ? first + per_block : a.chunks;
   const int nq = MODE == 1 ? a.s : (MODE == 2 ? 2 : 0);
+  const XGather<T> xg(a.x, a.n);
   // Lane q keeps the wavefront's running dot product q (one register instead of
   // one per dot in every lane, which keeps occupancy up).
   double lane_acc = 0.0;
   for (size_t chunk = first + wave_in_block; chunk < last; chunk += kSellBlock / 64) {
     const size_t row = chunk * kSellChunk + lane;
-    const int32_t irow = static_cast<int32_t>(row);
-    const int32_t last_row = static_cast<int32_t>(a.n) - 1;
+    const uint32_t rowb = static_cast<uint32_t>(row) * static_cast<uint32_t>(sizeof(T));
     const int32_t w = a.width[chunk];
-    const int32_t nw = a.nwin[chunk];  // uniform per wavefront
+    const int32_t nw = a.nwin[chunk];
     const V* vp = a.vals + a.chunk_start[chunk] + lane * kQuad;
     const int64_t cbase = static_cast<int64_t>(a.col_start[chunk]) * kSellChunk;
     // FP64 accumulation whatever the storage types: the SpMV is bandwidth bound.
-    constexpr int U = 8;  // entries (two quads) in flight per lane
+    constexpr int U = 8;  // entries (two quads) per step
     double s[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) s[u] = 0.0;
-    // DIA part: column = row + offset (clamped into range for padding slots,
-    // whose value is 0), so the wavefront's gathers are contiguous.  nd and w
-    // are multiples of kQuad; each lane reads its row's values a quad at a time.
+    // DIA part, in the plan's order: pair entries (two slots, one 16-byte
+    // gather each), then single entries.  Both counts fill whole quads.
     {
-      const int nd = a.dia_count[chunk];
-      const int32_t* dl = a.dia_offsets + chunk * kMaxDia;
+      const int np = a.dia_pairs[chunk];
+      const int ns = a.dia_singles[chunk];
+      const int32_t* pl = a.dia_plan + chunk * kMaxPlan;
       const V* dp = a.dia_vals + a.dia_start[chunk] + lane * kQuad;
+      auto off = [&](int32_t d) {
+        return rowb + static_cast<uint32_t>(d) * static_cast<uint32_t>(sizeof(T));
+      };
+      // Pairs: a step is 4 pair entries = 8 slots = 2 value quads.  Two register
+      // sets alternate; each step's gathers are issued before the next step's
+      // value and offset loads, so the wait for the gathers leaves those in flight.
+      constexpr int P = 4;
+      auto load_pairs = [&](V* v, int32_t* o, int at) {  // at: pair index
+        load_quad(dp + (2 * at) * kSellChunk, v);
+        load_quad(dp + (2 * at + kQuad) * kSellChunk, v + kQuad);
+#pragma unroll
+        for (int u = 0; u < P; ++u) o[u] = pl[at + u];
+      };
+      auto gather_pairs = [&](const int32_t* o, T* xv) {
+#pragma unroll
+        for (int u = 0; u < P; ++u) xg.pair(off(o[u]), xv[2 * u], xv[2 * u + 1]);
+      };
+      auto fma8 = [&](const V* v, const T* xv) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) s[u] += static_cast<double>(v[u]) * static_cast<double>(xv[u]);
+      };
       int j = 0;
-      for (; j + U <= nd; j += U) {
-        V v[U];
-        int32_t c[U];
-        load_quad(dp + j * kSellChunk, v);
-        load_quad(dp + (j + kQuad) * kSellChunk, v + kQuad);
-#pragma unroll
-        for (int u = 0; u < U; ++u) c[u] = min(max(irow + dl[j + u], 0), last_row);
-#pragma unroll
-        for (int u = 0; u < U; ++u) s[u] += static_cast<double>(v[u]) * static_cast<double>(a.x[c[u]]);
-      }
-      for (; j < nd; j += kQuad) {
-        V v[kQuad];
-        load_quad(dp + j * kSellChunk, v);
-#pragma unroll
-        for (int u = 0; u < kQuad; ++u) {
-          const int32_t c = min(max(irow + dl[j + u], 0), last_row);
-          s[u] += static_cast<double>(v[u]) * static_cast<double>(a.x[c]);
+      if (np >= P) {
+        V va[U], vb[U];
+        int32_t oa[P], ob[P];
+        T xv[U];
+        load_pairs(va, oa, 0);
+        while (j + 2 * P <= np) {
+          gather_pairs(oa, xv);
+          load_pairs(vb, ob, j + P);
+          fma8(va, xv);
+          j += P;
+          if (j + 2 * P > np) {  // set B holds the last full step
+            gather_pairs(ob, xv);
+            fma8(vb, xv);
+            j += P;
+            goto pair_tail;
+          }
+          gather_pairs(ob, xv);
+          load_pairs(va, oa, j + P);
+          fma8(vb, xv);
+          j += P;
         }
+        gather_pairs(oa, xv);  // set A holds the last full step
+        fma8(va, xv);
+        j += P;
+      }
+    pair_tail:
+      if (j < np) {  // two pair entries left (np is even): one quad
+        V v[kQuad];
+        load_quad(dp + (2 * j) * kSellChunk, v);
+        T xv[kQuad];
+        xg.pair(off(pl[j]), xv[0], xv[1]);
+        xg.pair(off(pl[j + 1]), xv[2], xv[3]);
+#pragma unroll
+        for (int u = 0; u < kQuad; ++u) s[u] += static_cast<double>(v[u]) * static_cast<double>(xv[u]);
+      }
+      // Singles: 8 per step (2 quads), then at most one quad.
+      const V* sp = dp + (2 * np) * kSellChunk;
+      const int32_t* sl = pl + np;
+      int q = 0;
+      for (; q + U <= ns; q += U) {
+        V v[U];
+        load_quad(sp + q * kSellChunk, v);
+        load_quad(sp + (q + kQuad) * kSellChunk, v + kQuad);
+        T xv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) xv[u] = xg(off(sl[q + u]));
+        fma8(v, xv);
+      }
+      for (; q < ns; q += kQuad) {
+        V v[kQuad];
+        load_quad(sp + q * kSellChunk, v);
+#pragma unroll
+        for (int u = 0; u < kQuad; ++u)
+          s[u] += static_cast<double>(v[u]) * static_cast<double>(xg(off(sl[q + u])));
       }
     }
     int32_t k = 0;
@@ -658,7 +806,8 @@ __global__ void __launch_bounds__(kSellBlock) sell_spmv_kernel(SellArgs<T, V> a)
 #pragma unroll
         for (int u = 0; u < U; ++u) {
           const int32_t col = __shfl(wbase, c[u] >> kSellWindowBits, 64) + (c[u] & kMask);
-          s[u] += static_cast<double>(v[u]) * static_cast<double>(a.x[col]);
+          const T xv = xg(static_cast<uint32_t>(col) * static_cast<uint32_t>(sizeof(T)));
+          s[u] += static_cast<double>(v[u]) * static_cast<double>(xv);
         }
       }
       for (; k < w; k += kQuad) {
@@ -669,7 +818,8 @@ __global__ void __launch_bounds__(kSellBlock) sell_spmv_kernel(SellArgs<T, V> a)
 #pragma unroll
         for (int u = 0; u < kQuad; ++u) {
           const int32_t col = __shfl(wbase, c[u] >> kSellWindowBits, 64) + (c[u] & kMask);
-          s[u] += static_cast<double>(v[u]) * static_cast<double>(a.x[col]);
+          const T xv = xg(static_cast<uint32_t>(col) * static_cast<uint32_t>(sizeof(T)));
+          s[u] += static_cast<double>(v[u]) * static_cast<double>(xv);
         }
       }
     } else {
@@ -680,8 +830,10 @@ __global__ void __launch_bounds__(kSellBlock) sell_spmv_kernel(SellArgs<T, V> a)
         load_quad(cp + k * kSellChunk, c);
         load_quad(vp + k * kSellChunk, v);
 #pragma unroll
-        for (int u = 0; u < kQuad; ++u)
-          s[u] += static_cast<double>(v[u]) * static_cast<double>(a.x[c[u]]);
+        for (int u = 0; u < kQuad; ++u) {
+          const T xv = xg(static_cast<uint32_t>(c[u]) * static_cast<uint32_t>(sizeof(T)));
+          s[u] += static_cast<double>(v[u]) * static_cast<double>(xv);
+        }
       }
     }
     const double sum = ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
@@ -813,6 +965,66 @@ __device__ __forceinline__ const SolverScalars* row_sc(const IdrVec<T>& a, size_
   return a.sc;
 }
 
+// Linear combinations of basis columns for row i, accumulated in q order.  The
+// column loads are issued four at a time (the trip counts are run-time values,
+// so a plain loop would wait for each load before the next), which keeps four
+// (eight for the paired form) HBM reads in flight per lane.
+template <typename T>
+__device__ __forceinline__ double comb_sub(double acc, const double* c, const T* vec, size_t ld,
+                                           size_t i, int q0, int q1) {
+  int q = q0;
+  for (; q + 4 <= q1; q += 4) {
+    const double e0 = static_cast<double>(vec[static_cast<size_t>(q) * ld + i]);
+    const double e1 = static_cast<double>(vec[static_cast<size_t>(q + 1) * ld + i]);
+    const double e2 = static_cast<double>(vec[static_cast<size_t>(q + 2) * ld + i]);
+    const double e3 = static_cast<double>(vec[static_cast<size_t>(q + 3) * ld + i]);
+    acc -= c[q] * e0;
+    acc -= c[q + 1] * e1;
+    acc -= c[q + 2] * e2;
+    acc -= c[q + 3] * e3;
+  }
+  for (; q < q1; ++q) acc -= c[q] * static_cast<double>(vec[static_cast<size_t>(q) * ld + i]);
+  return acc;
+}
+template <typename T>
+__device__ __forceinline__ double comb_add(double acc, const double* c, const T* vec, size_t ld,
+                                           size_t i, int q0, int q1) {
+  int q = q0;
+  for (; q + 4 <= q1; q += 4) {
+    const double e0 = static_cast<double>(vec[static_cast<size_t>(q) * ld + i]);
+    const double e1 = static_cast<double>(vec[static_cast<size_t>(q + 1) * ld + i]);
+    const double e2 = static_cast<double>(vec[static_cast<size_t>(q + 2) * ld + i]);
+    const double e3 = static_cast<double>(vec[static_cast<size_t>(q + 3) * ld + i]);
+    acc += c[q] * e0;
+    acc += c[q + 1] * e1;
+    acc += c[q + 2] * e2;
+    acc += c[q + 3] * e3;
+  }
+  for (; q < q1; ++q) acc += c[q] * static_cast<double>(vec[static_cast<size_t>(q) * ld + i]);
+  return acc;
+}
+// Both updates of a biorthogonalisation over q in [0, k): g -= c_q G_q and
+// u -= c_q U_q, pairwise as before, loads issued two columns of each at a time.
+template <typename T>
+__device__ __forceinline__ void comb_sub2(double& g, double& u, const double* c, const T* G,
+                                          const T* U, size_t ld, size_t i, int k) {
+  int q = 0;
+  for (; q + 2 <= k; q += 2) {
+    const size_t o0 = static_cast<size_t>(q) * ld + i, o1 = o0 + ld;
+    const double g0 = static_cast<double>(G[o0]), g1 = static_cast<double>(G[o1]);
+    const double u0 = static_cast<double>(U[o0]), u1 = static_cast<double>(U[o1]);
+    g -= c[q] * g0;
+    u -= c[q] * u0;
+    g -= c[q + 1] * g1;
+    u -= c[q + 1] * u1;
+  }
+  for (; q < k; ++q) {
+    const size_t o = static_cast<size_t>(q) * ld + i;
+    g -= c[q] * static_cast<double>(G[o]);
+    u -= c[q] * static_cast<double>(U[o]);
+  }
+}
+
 // Schwarz path, first half of idr_a: v = pdiag (r - sum_{q>=k} c_q G_q): Jacobi
 // on uncovered rows, the raw value (the block application's input) on covered ones.
 template <typename T, bool kSeg = false>
@@ -820,8 +1032,7 @@ __global__ void __launch_bounds__(kBlock) idr_a1_kernel(IdrVec<T> a, int k) {
   for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < a.n;
        i += static_cast<size_t>(gridDim.x) * blockDim.x) {
     const SolverScalars* sc = row_sc<kSeg>(a, i);
-    double v = a.r[i];
-    for (int q = k; q < a.s; ++q) v -= sc->c[q] * static_cast<double>(a.G[q * a.ld + i]);
+    const double v = comb_sub(a.r[i], sc->c, a.G, a.ld, i, k, a.s);
     a.v[i] = static_cast<T>(a.pdiag[i] * v);
   }
 }
@@ -833,8 +1044,7 @@ __global__ void __launch_bounds__(kBlock) idr_a2_kernel(IdrVec<T> a, int k) {
        i += static_cast<size_t>(gridDim.x) * blockDim.x) {
     const SolverScalars* sc = row_sc<kSeg>(a, i);
     const double om = sc->om;
-    double u = 0.0;
-    for (int q = k; q < a.s; ++q) u += sc->c[q] * static_cast<double>(a.U[q * a.ld + i]);
+    const double u = comb_add(0.0, sc->c, a.U, a.ld, i, k, a.s);
     a.U[k * a.ld + i] = static_cast<T>(u + om * static_cast<double>(a.v[i]));
   }
 }
@@ -846,8 +1056,7 @@ __global__ void __launch_bounds__(kBlock) idr_a2g_kernel(IdrVec<T> a, int k) {
   const double om = a.sc->om;
   for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < a.n;
        i += static_cast<size_t>(gridDim.x) * blockDim.x) {
-    double u = 0.0;
-    for (int q = k; q < a.s; ++q) u += a.sc->c[q] * static_cast<double>(a.U[q * a.ld + i]);
+    const double u = comb_add(0.0, a.sc->c, a.U, a.ld, i, k, a.s);
     double v = static_cast<double>(a.v[i]);
     if (a.cover_start[i + 1] > a.cover_start[i])
       v = static_cast<double>(static_cast<T>(
@@ -1019,13 +1228,8 @@ __global__ void __launch_bounds__(kBlock) idr_a_kernel(IdrVec<T> a, int k) {
        i += static_cast<size_t>(gridDim.x) * blockDim.x) {
     const SolverScalars* sc = row_sc<kSeg>(a, i);
     const double om = sc->om;
-    double v = a.r[i];
-    double u = 0.0;
-    for (int q = k; q < a.s; ++q) {
-      const double c = sc->c[q];
-      v -= c * static_cast<double>(a.G[q * a.ld + i]);
-      u += c * static_cast<double>(a.U[q * a.ld + i]);
-    }
+    double v = comb_sub(a.r[i], sc->c, a.G, a.ld, i, k, a.s);
+    const double u = comb_add(0.0, sc->c, a.U, a.ld, i, k, a.s);
     v *= a.pdiag[i];
     a.U[k * a.ld + i] = static_cast<T>(u + om * v);
   }
@@ -1042,11 +1246,7 @@ __global__ void __launch_bounds__(kBlock) idr_b_kernel(IdrVec<T> a, int k, int w
     const double beta = sc->beta;
     double g = static_cast<double>(a.G[k * a.ld + i]);
     double u = static_cast<double>(a.U[k * a.ld + i]);
-    for (int q = 0; q < k; ++q) {
-      const double al = sc->alpha[q];
-      g -= al * static_cast<double>(a.G[q * a.ld + i]);
-      u -= al * static_cast<double>(a.U[q * a.ld + i]);
-    }
+    comb_sub2(g, u, sc->alpha, a.G, a.U, a.ld, i, k);
     a.G[k * a.ld + i] = static_cast<T>(g);
     a.U[k * a.ld + i] = static_cast<T>(u);
     const double r = a.r[i] - beta * g;
@@ -1072,25 +1272,19 @@ __global__ void __launch_bounds__(kBlock) idr_ba_kernel(IdrVec<T> a, int k) {
        i += static_cast<size_t>(gridDim.x) * blockDim.x) {
     double g = static_cast<double>(a.G[k * a.ld + i]);
     double u = static_cast<double>(a.U[k * a.ld + i]);
-    for (int q = 0; q < k; ++q) {
-      const double al = sc->alpha[q];
-      g -= al * static_cast<double>(a.G[q * a.ld + i]);
-      u -= al * static_cast<double>(a.U[q * a.ld + i]);
-    }
+    comb_sub2(g, u, sc->alpha, a.G, a.U, a.ld, i, k);
     a.G[k * a.ld + i] = static_cast<T>(g);
     a.U[k * a.ld + i] = static_cast<T>(u);
     const double r = a.r[i] - beta * g;
     a.r[i] = r;
     a.x[i] += beta * u;
     acc[0] += r * r;
-    double v = r;
-    for (int q = k + 1; q < a.s; ++q) v -= sc->c[q] * static_cast<double>(a.G[q * a.ld + i]);
+    double v = comb_sub(r, sc->c, a.G, a.ld, i, k + 1, a.s);
     v *= a.pdiag[i];
     if (kBlocks) {
       a.v[i] = static_cast<T>(v);
     } else {
-      double un = 0.0;
-      for (int q = k + 1; q < a.s; ++q) un += sc->c[q] * static_cast<double>(a.U[q * a.ld + i]);
+      const double un = comb_add(0.0, sc->c, a.U, a.ld, i, k + 1, a.s);
       a.U[(k + 1) * a.ld + i] = static_cast<T>(un + om * v);
     }
   }
@@ -1282,6 +1476,8 @@ template <typename V>
 void build_sell(Context& ctx, SellMatrix& M, const lsmps_csr& A_in, V* out_vals_dummy_type) {
   (void)out_vals_dummy_type;
   const size_t n = A_in.rows;
+  require(n <= kMaxGatherRows, "PPE solve: the system has more rows than the SpMV's 32-bit "
+                               "gather offsets address (2^28)");
   // row-sorted copy of the matrix (same rows and values, columns ascending)
   LSMPS_HIP(hipMemcpyAsync(ctx.host_i32, A_in.row_offsets + n, sizeof(int32_t),
                            hipMemcpyDeviceToHost, ctx.stream));
@@ -1305,6 +1501,10 @@ void build_sell(Context& ctx, SellMatrix& M, const lsmps_csr& A_in, V* out_vals_
   const size_t chunks = M.chunks;
   M.dia_count.reserve(chunks);
   M.dia_offsets.reserve(chunks * kMaxDia);
+  M.dia_slot.reserve(chunks * kMaxDia);
+  M.dia_plan.reserve(chunks * kMaxPlan);
+  M.dia_pairs.reserve(chunks);
+  M.dia_singles.reserve(chunks);
   M.dia_slots.reserve(chunks);
   M.dia_start.reserve(chunks + 1);
   M.chunk_width.reserve(chunks);
@@ -1319,6 +1519,7 @@ void build_sell(Context& ctx, SellMatrix& M, const lsmps_csr& A_in, V* out_vals_
   // diagonals and remainder widths
   dia_select_kernel<<<static_cast<unsigned>(chunks), kSellChunk, 0, ctx.stream>>>(
       n, A.row_offsets, A.col_indices, M.dia_count.data(), M.dia_offsets.data(),
+      M.dia_slot.data(), M.dia_plan.data(), M.dia_pairs.data(), M.dia_singles.data(),
       M.dia_slots.data(), M.chunk_width.data(), M.chunk_scaled.data(), M.rem_counter.data());
   LSMPS_LAUNCH_CHECK();
   const unsigned row_blocks = blocks_for(chunks * kSellChunk);
@@ -1367,7 +1568,7 @@ void build_sell(Context& ctx, SellMatrix& M, const lsmps_csr& A_in, V* out_vals_
       M.dia_entries, dia, V(0));
   sell_fill_kernel<V><<<row_blocks, kBlock, 0, ctx.stream>>>(
       n, chunks, A.row_offsets, A.col_indices, A.values, M.dia_count.data(),
-      M.dia_offsets.data(), M.dia_start.data(), dia, M.chunk_start.data(),
+      M.dia_offsets.data(), M.dia_slot.data(), M.dia_start.data(), dia, M.chunk_start.data(),
       M.chunk_width.data(), M.col_start.data(), M.nwin.data(), M.windows.data(), M.cols.data(),
       vals);
   LSMPS_LAUNCH_CHECK();
@@ -1466,7 +1667,7 @@ int idr_solve(Context& ctx, PpeSolver& S, size_t n, const SellMatrix& M, const V
         schwarz_products<T>(ctx, S.schwarz, v);
         idr_a2g_kernel<T><<<vb, kBlock, 0, ctx.stream>>>(iv, k);
       }
-      SellArgs<T, V> sa{n, M.chunks, M.dia_count.data(), M.dia_offsets.data(),
+      SellArgs<T, V> sa{n, M.chunks, M.dia_pairs.data(), M.dia_singles.data(), M.dia_plan.data(),
                         M.dia_start.data(), dia, M.chunk_start.data(), M.chunk_width.data(),
                         M.col_start.data(), M.nwin.data(), M.windows.data(), M.cols.data(),
                         vals, U + k * ld, G + k * ld, p0, s, nullptr, S.part_a.data()};
@@ -1488,7 +1689,7 @@ int idr_solve(Context& ctx, PpeSolver& S, size_t n, const SellMatrix& M, const V
     if (schwarz) {
       schwarz_apply<T>(ctx, S.schwarz, v);
     }
-    SellArgs<T, V> st{n, M.chunks, M.dia_count.data(), M.dia_offsets.data(),
+    SellArgs<T, V> st{n, M.chunks, M.dia_pairs.data(), M.dia_singles.data(), M.dia_plan.data(),
                       M.dia_start.data(), dia, M.chunk_start.data(), M.chunk_width.data(),
                       M.col_start.data(), M.nwin.data(), M.windows.data(), M.cols.data(), vals,
                       v, t, nullptr, s, r, S.part_a.data()};
@@ -1593,7 +1794,8 @@ void idr_solve_segmented(Context& ctx, PpeSolver& S, size_t n, const SellMatrix&
     LSMPS_LAUNCH_CHECK();
   };
   auto spmv = [&](const T* in, T* out) {
-    SellArgs<T, V> a{n, M.chunks, M.dia_count.data(), M.dia_offsets.data(), M.dia_start.data(),
+    SellArgs<T, V> a{n, M.chunks, M.dia_pairs.data(), M.dia_singles.data(), M.dia_plan.data(),
+                     M.dia_start.data(),
                      dia, M.chunk_start.data(), M.chunk_width.data(), M.col_start.data(),
                      M.nwin.data(), M.windows.data(), M.cols.data(), vals, in, out, p0, s,
                      nullptr, nullptr};
@@ -1785,7 +1987,7 @@ void ppe_solve(Context& ctx, PpeSolver& S, const lsmps_csr& A, const double* b, 
   const int max_it = opt.max_iterations > 0 ? opt.max_iterations : 20000;
   const size_t ld = (n + 63) / 64 * 64;
   int shadow_dim = opt.method == LSMPS_SOLVER_BICGSTAB ? 1 : opt.shadow_dimension;
-  if (shadow_dim <= 0) shadow_dim = 12;
+  if (shadow_dim <= 0) shadow_dim = 16;
   if (shadow_dim > kMaxShadow) shadow_dim = kMaxShadow;
   if (static_cast<size_t>(shadow_dim) > n) shadow_dim = static_cast<int>(n);
 

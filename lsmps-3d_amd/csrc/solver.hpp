@@ -9,7 +9,8 @@ namespace lsmps {
 constexpr int kMaxShadow = 16;           // largest IDR(s) shadow dimension
 constexpr int kPartialStride = kMaxShadow;  // doubles per block partial record
 constexpr int kMaxReduceBlocks = 2048;  // partial records per reduction (SpMV grid cap)
-constexpr int kMaxVectorBlocks = 1024;  // grid cap of the streaming vector kernels
+constexpr int kMaxVectorBlocks = 2048;  // grid cap of the streaming vector kernels (<= kMaxReduceBlocks)
+static_assert(kMaxVectorBlocks <= kMaxReduceBlocks, "vector kernels write one partial record per block");
 
 // Small device-resident state of one Krylov solve; kernels read and write it
 // directly so no host round trip happens per iteration.
@@ -34,10 +35,16 @@ struct SolverScalars {
 //
 // DIA part: the offsets d = column - row that at least kDiaMinRows of the
 // chunk's rows have (particles in lattice or cell order share the ~123 offsets
-// of the 7x7x7 neighbourhood) are listed per chunk, ascending, padded to a
-// multiple of 4 by repeating the last offset; zero where a row lacks offset j.
+// of the 7x7x7 neighbourhood) are listed per chunk, ascending (for lookups).
 // Columns are implicit, and the gathers x[row + d] of a wavefront are
-// contiguous.
+// contiguous.  The stored diagonals are ordered for the gathers: consecutive
+// offsets (d, d + 1) are paired greedily along each run of the sorted list and
+// stored first, two slots per pair, so one 16-byte load per lane fetches both
+// x values; the unpaired offsets follow.  The pair count is padded to even and
+// the single count to a multiple of 4 (value 0).  A chunk where some pair would
+// read x outside [0, n) for one of its rows keeps every offset single.  The
+// per-chunk plan lists the pair offsets d, then the single offsets, in slot
+// order; dia_slot maps a sorted offset to its storage slot.
 //
 // Remainder: every other entry, in row-sorted order, padded to the chunk's
 // widest remainder row rounded up to a multiple of 4 (value 0).
@@ -57,6 +64,7 @@ constexpr int kSellChunk = 64;
 constexpr int kSellWindows = 16;
 constexpr int kSellWindowBits = 12;
 constexpr int kMaxDia = 256;
+constexpr int kMaxPlan = kMaxDia + 8;  // plan entries incl. padding
 constexpr int kDiaMinRows = 16;
 struct SellMatrix {
   size_t n = 0;
@@ -66,9 +74,13 @@ struct SellMatrix {
   size_t col_units = 0;   // 16-bit column slots (entries + entries of 32-bit chunks)
   size_t nnz = 0;         // stored CSR nonzeros
   size_t rem_nnz = 0;     // of which in the remainder
-  DeviceArray<int32_t> dia_count;    // diagonals per chunk
+  DeviceArray<int32_t> dia_count;    // diagonals per chunk (sorted list length)
   DeviceArray<int32_t> dia_offsets;  // chunks x kMaxDia, ascending
-  DeviceArray<int32_t> dia_slots;    // dia_count * 64 (scan input)
+  DeviceArray<int32_t> dia_slot;     // chunks x kMaxDia: storage slot of sorted entry i
+  DeviceArray<int32_t> dia_plan;     // chunks x kMaxPlan: pair offsets, then single offsets
+  DeviceArray<int32_t> dia_pairs;    // plan pair entries per chunk (even)
+  DeviceArray<int32_t> dia_singles;  // plan single entries per chunk (multiple of 4)
+  DeviceArray<int32_t> dia_slots;    // stored slots * 64 (scan input)
   DeviceArray<int32_t> dia_start;    // scan of dia_slots
   DeviceArray<int32_t> chunk_start;  // remainder value slot offset (< 2^31 is checked)
   DeviceArray<int32_t> chunk_width;  // remainder width
@@ -84,7 +96,8 @@ struct SellMatrix {
   DeviceArray<int32_t> sorted_cols;  // the CSR with each row's columns ascending
   DeviceArray<double> sorted_vals;
   size_t bytes() const {
-    return dia_count.bytes() + dia_offsets.bytes() + dia_slots.bytes() + dia_start.bytes() +
+    return dia_count.bytes() + dia_offsets.bytes() + dia_slot.bytes() + dia_plan.bytes() +
+           dia_pairs.bytes() + dia_singles.bytes() + dia_slots.bytes() + dia_start.bytes() +
            chunk_start.bytes() + chunk_width.bytes() + chunk_scaled.bytes() + col_rows.bytes() +
            col_start.bytes() + nwin.bytes() + windows.bytes() + rem_counter.bytes() +
            cols.bytes() + vals64.bytes() + dia64.bytes() + vals32.bytes() + dia32.bytes() +

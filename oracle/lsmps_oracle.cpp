@@ -1076,7 +1076,7 @@ int ppe_solve(const lsmps_csr& a, const double* b, double* x, const lsmps_solver
   if (opt.method == LSMPS_SOLVER_IDRS) {
     // IDR(s) with biorthogonalisation (Sonneveld & van Gijzen 2008; van Gijzen &
     // Sonneveld 2011).  Shadow space: P_0 = r_0, P_q (q >= 1) = hashed signs.
-    int s = opt.shadow_dimension > 0 ? opt.shadow_dimension : 12;
+    int s = opt.shadow_dimension > 0 ? opt.shadow_dimension : 16;
     if (s > 16) s = 16;
     if (s > n) s = static_cast<int>(n);
     auto sign = [](uint32_t row, int q) {
