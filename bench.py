@@ -41,6 +41,9 @@ def parse():
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     p.add_argument("--shadow-dimension", type=int, default=0,
                    help="IDR(s) shadow dimension (0: the library default)")
+    p.add_argument("--coarse-cells", type=int, default=0,
+                   help="cells per side of the coarse-space aggregates (0: the library "
+                        "default, -1: no coarse space)")
     p.add_argument("--no-adjoint", action="store_true",
                    help="skip timing the adjoint of one step (after the timed steps)")
     p.add_argument("--cpu-threads", type=int, default=0)
@@ -65,7 +68,8 @@ def make_case(cases, name):
     raise SystemExit(f"unknown case {name}")
 
 
-def run_gpu(pkg, rt, case, precision, steps, warmup, timing=True, adjoint=False, shadow=0):
+def run_gpu(pkg, rt, case, precision, steps, warmup, timing=True, adjoint=False, shadow=0,
+            coarse=0):
     import numpy as np
     import torch
     abi = pkg.abi
@@ -75,6 +79,10 @@ def run_gpu(pkg, rt, case, precision, steps, warmup, timing=True, adjoint=False,
     opts = abi.default_solver_options()
     if shadow > 0:
         opts.shadow_dimension = shadow
+    if coarse > 0:
+        opts.coarse_cells = coarse
+    elif coarse < 0:
+        opts.preconditioner = abi.PRECOND_SCHWARZ
     st = rt.Stepper(ctx, state, case.config, 256, opts)
     if "fluid_offsets" in case.extra:  # batched tanks
         st.set_segments(case.extra["fluid_offsets"], case.extra["wall_offsets"],
@@ -237,14 +245,15 @@ def main():
     print(f"[bench] case {args.case}: {case.n_fluid} fluid, {case.n_wall} wall particles; "
           f"grid {list(case.config.cell_dims)}", file=sys.stderr, flush=True)
     main_run = run_gpu(pkg, rt, case, prec, args.steps, args.warmup,
-                       adjoint=not args.no_adjoint, shadow=args.shadow_dimension)
+                       adjoint=not args.no_adjoint, shadow=args.shadow_dimension,
+                       coarse=args.coarse_cells)
     iters = [r["iterations"] for r in main_run["reports"]]
     print(f"[bench] {args.precision}: {main_run['ms_per_step']:.1f} ms/step, iterations {iters}, "
           f"stages {np.round(main_run['stage_ms'], 2).tolist()}", file=sys.stderr, flush=True)
     fp64_run = None
     if not args.no_fp64 and prec != abi.PRECISION_FP64:
         fp64_run = run_gpu(pkg, rt, case, abi.PRECISION_FP64, max(1, min(args.steps, 2)), 0,
-                           shadow=args.shadow_dimension)
+                           shadow=args.shadow_dimension, coarse=args.coarse_cells)
         print(f"[bench] fp64: {fp64_run['ms_per_step']:.1f} ms/step, iterations "
               f"{[r['iterations'] for r in fp64_run['reports']]}", file=sys.stderr, flush=True)
     avg_ms = main_run["spmv_ms"] / max(main_run["spmv_launches"], 1)

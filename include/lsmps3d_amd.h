@@ -259,14 +259,19 @@ typedef struct lsmps_solver_options {
   int32_t check_interval;  /* PCG: iterations between host convergence checks */
   int32_t clamp_negative;  /* clamp p<0 after the solve, as ppe_matrix.cu:662 (default 1) */
   int32_t shadow_dimension; /* IDR(s): s, 1..16 (default 16) */
-  int32_t preconditioner;   /* LSMPS_PRECOND_* (default SCHWARZ) */
+  int32_t preconditioner;   /* LSMPS_PRECOND_* (default SCHWARZ_COARSE) */
+  int32_t coarse_cells;     /* SCHWARZ_COARSE: aggregates span coarse_cells^3 cells (default 4) */
 } lsmps_solver_options;
 
 /* Preconditioner of the Krylov PPE solve.  JACOBI: D^-1.  SCHWARZ: Jacobi plus
  * exact local solves (restricted additive Schwarz) on the 2-hop neighbourhoods
  * of the rows with a positive diagonal, which make the reference's PPE
- * indefinite.  Either way the matrix is the reference's. */
-enum { LSMPS_PRECOND_JACOBI = 0, LSMPS_PRECOND_SCHWARZ = 1 };
+ * indefinite.  SCHWARZ_COARSE: SCHWARZ plus an additive coarse-space
+ * correction P Ac^-1 P^T over aggregates of coarse_cells^3 cells of the fluid
+ * cell grid (Ac = P^T A P, inverted once per solve); it needs the geometry, so
+ * only a stepper applies it -- lsmps_ppe_solve on its own uses SCHWARZ.  The
+ * matrix is always the reference's. */
+enum { LSMPS_PRECOND_JACOBI = 0, LSMPS_PRECOND_SCHWARZ = 1, LSMPS_PRECOND_SCHWARZ_COARSE = 2 };
 
 typedef struct lsmps_solve_report {
   int32_t method;
@@ -277,6 +282,7 @@ typedef struct lsmps_solve_report {
   double rhs_norm;
   int32_t schwarz_blocks;     /* LSMPS_PRECOND_SCHWARZ: dense blocks (wrong-sign rows) */
   int32_t schwarz_rows;       /* LSMPS_PRECOND_SCHWARZ: rows summed over all blocks */
+  int32_t coarse_unknowns;    /* SCHWARZ_COARSE: aggregates of the applied coarse space (0: none) */
 } lsmps_solve_report;
 
 /* Opaque objects */

@@ -34,6 +34,7 @@ SOLVER_IDRS = 2
 
 PRECOND_JACOBI = 0
 PRECOND_SCHWARZ = 1
+PRECOND_SCHWARZ_COARSE = 2
 
 STEP_RECORD = 1
 STEP_FROZEN_GEOMETRY = 2
@@ -148,14 +149,15 @@ class SolverOptions(C.Structure):
     _fields_ = [("method", C.c_int32), ("max_iterations", C.c_int32),
                 ("tolerance", C.c_double), ("inner_tolerance", C.c_double),
                 ("check_interval", C.c_int32), ("clamp_negative", C.c_int32),
-                ("shadow_dimension", C.c_int32), ("preconditioner", C.c_int32)]
+                ("shadow_dimension", C.c_int32), ("preconditioner", C.c_int32),
+                ("coarse_cells", C.c_int32)]
 
 
 class SolveReport(C.Structure):
     _fields_ = [("method", C.c_int32), ("iterations", C.c_int32), ("refinements", C.c_int32),
                 ("converged", C.c_int32), ("relative_residual", C.c_double),
                 ("rhs_norm", C.c_double), ("schwarz_blocks", C.c_int32),
-                ("schwarz_rows", C.c_int32)]
+                ("schwarz_rows", C.c_int32), ("coarse_unknowns", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -250,7 +252,8 @@ def default_solver_options():
     o.check_interval = 16
     o.clamp_negative = 1
     o.shadow_dimension = 16
-    o.preconditioner = PRECOND_SCHWARZ
+    o.preconditioner = PRECOND_SCHWARZ_COARSE
+    o.coarse_cells = 4
     return o
 
 

@@ -27,6 +27,7 @@ SOURCES = {
     "solver.hip": ["-ffp-contract=fast"],
     "correction.hip": ["-ffp-contract=fast"],
     "schwarz.hip": ["-ffp-contract=fast"],
+    "coarse.hip": ["-ffp-contract=fast"],
     "adjoint.hip": ["-ffp-contract=fast"],
     "driver.hip": ["-ffp-contract=off"],
     "stepper.cpp": [],

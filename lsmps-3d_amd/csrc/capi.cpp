@@ -437,7 +437,8 @@ int lsmps_default_solver_options(lsmps_solver_options* out) {
   o.check_interval = 16;
   o.clamp_negative = 1;
   o.shadow_dimension = 16;
-  o.preconditioner = LSMPS_PRECOND_SCHWARZ;
+  o.preconditioner = LSMPS_PRECOND_SCHWARZ_COARSE;
+  o.coarse_cells = 4;
   *out = o;
   API_END("lsmps_default_solver_options")
 }

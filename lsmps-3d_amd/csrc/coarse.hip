@@ -1,0 +1,476 @@
+// Coarse-space correction of the PPE preconditioner: aggregation over blocks of
+// cells, Galerkin coarse matrix in exact fixed point, Gauss-Jordan inverse, and
+// the per-application restriction / coarse product / prolongation.
+//
+// Kernels that reduce over an aggregate or a coarse row give each one a
+// wavefront: lanes take the items in a fixed stride and the partial sums meet
+// in a fixed butterfly, so the sums are reproducible.
+#include <cmath>
+
+#include "coarse.hpp"
+#include "stages.hpp"
+
+namespace lsmps {
+namespace {
+
+constexpr int kWaves = kBlock / 64;
+
+__device__ __forceinline__ bool aggregated_label(int32_t label) {
+  return label == LSMPS_SURFACE_INNER || label == LSMPS_SURFACE_NEAR_SURFACE;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+__device__ __forceinline__ int wave_sum_i(int v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+struct CoarseGrid {
+  int32_t nx, ny, nz;     // fine cells
+  int32_t cx, cy, cz;     // coarse cells
+  int32_t f;
+};
+
+// Calls visit(row) for the members of coarse cell cc's fine cells in (z, y, x)
+// order, 64 candidates at a time (lane l gets candidate base + l, or -1).
+template <typename F>
+__device__ void for_each_candidate(const CoarseGrid& g, size_t cc, const int32_t* sorted,
+                                   const int32_t* begin, const int32_t* end, int lane, F visit) {
+  const int32_t ccx = static_cast<int32_t>(cc % g.cx);
+  const int32_t ccy = static_cast<int32_t>((cc / g.cx) % g.cy);
+  const int32_t ccz = static_cast<int32_t>(cc / (static_cast<size_t>(g.cx) * g.cy));
+  for (int32_t dz = 0; dz < g.f; ++dz) {
+    const int32_t z = ccz * g.f + dz;
+    if (z >= g.nz) break;
+    for (int32_t dy = 0; dy < g.f; ++dy) {
+      const int32_t y = ccy * g.f + dy;
+      if (y >= g.ny) break;
+      for (int32_t dx = 0; dx < g.f; ++dx) {
+        const int32_t x = ccx * g.f + dx;
+        if (x >= g.nx) break;
+        const size_t cell = (static_cast<size_t>(z) * g.ny + y) * g.nx + x;
+        const int32_t b = begin[cell];
+        if (b < 0) continue;
+        const int32_t e = end[cell];
+        for (int32_t base = b; base < e; base += 64) {
+          const int32_t k = base + lane;
+          visit(k < e ? sorted[k] : -1);
+        }
+      }
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) count_members_kernel(
+    CoarseGrid g, size_t ccells, const int32_t* labels, const int32_t* sorted,
+    const int32_t* begin, const int32_t* end, int32_t* count, int32_t* flag) {
+  const size_t cc = static_cast<size_t>(blockIdx.x) * kWaves + (threadIdx.x >> 6);
+  if (cc >= ccells) return;  // uniform per wavefront
+  const int lane = threadIdx.x & 63;
+  int c = 0;
+  for_each_candidate(g, cc, sorted, begin, end, lane, [&](int32_t row) {
+    if (row >= 0 && aggregated_label(labels[row])) ++c;
+  });
+  c = wave_sum_i(c);
+  if (lane == 0) {
+    count[cc] = c;
+    flag[cc] = c > 0 ? 1 : 0;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) fill_members_kernel(
+    CoarseGrid g, size_t ccells, const int32_t* labels, const int32_t* sorted,
+    const int32_t* begin, const int32_t* end, const int32_t* count, const int32_t* start,
+    const int32_t* id, int32_t* agg_start, int32_t* agg_rows, int32_t* agg_of_row) {
+  const size_t cc = static_cast<size_t>(blockIdx.x) * kWaves + (threadIdx.x >> 6);
+  if (cc >= ccells) return;
+  if (count[cc] == 0) return;  // uniform per wavefront
+  const int lane = threadIdx.x & 63;
+  const int32_t a = id[cc];
+  int32_t pos = start[cc];
+  if (lane == 0) agg_start[a] = pos;
+  for_each_candidate(g, cc, sorted, begin, end, lane, [&](int32_t row) {
+    const bool take = row >= 0 && aggregated_label(labels[row]);
+    const unsigned long long mask = __ballot(take);
+    if (take) {
+      const int before = __popcll(mask & ((1ull << lane) - 1ull));
+      agg_rows[pos + before] = row;
+      agg_of_row[row] = a;
+    }
+    pos += __popcll(mask);
+  });
+}
+
+__global__ void set_last_kernel(int32_t* agg_start, int count, const int32_t* counts,
+                                const int32_t* starts, size_t ccells) {
+  agg_start[count] = starts[ccells - 1] + counts[ccells - 1];
+}
+
+__global__ void fill_i32(int32_t* p, size_t n, int32_t v) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// Ac = P^T A P in exact 64-bit fixed point.  Row I's scale 2^s_I is set from a
+// bound B_I >= |Ac_IJ| (the sum over its members of sum_j |a_ij|) so that
+// 2^s_I B_I < 2^61: every a_ij enters as round(a_ij 2^s_I) and all additions
+// are exact integer additions, whose order does not matter.
+
+// |a_ij| summed over the aggregated columns of row i (rows outside: 0).
+__global__ void __launch_bounds__(kBlock) row_abs_kernel(size_t n, const int32_t* agg_of_row,
+                                                         const int32_t* rows, const int32_t* cols,
+                                                         const double* vals, double* out) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = 0.0;
+  if (agg_of_row[i] >= 0)
+    for (int32_t e = rows[i]; e < rows[i + 1]; ++e)
+      if (agg_of_row[cols[e]] >= 0) s += fabs(vals[e]);
+  out[i] = s;
+}
+
+__device__ __forceinline__ double row_scale(double B) {
+  return B > 0.0 ? ldexp(1.0, 60 - ilogb(B)) : 0.0;
+}
+
+// Block (I, part) adds the entries of its share of aggregate I's members into
+// an LDS row (consecutive entries mostly share their aggregate -- a row's
+// neighbors come cell by cell -- so each run is summed in a register first),
+// then the nonzero LDS entries into row I of the integer matrix.
+constexpr int kMatrixParts = 8;
+__global__ void __launch_bounds__(kBlock) coarse_matrix_kernel(
+    int count, const int32_t* agg_start, const int32_t* agg_rows, const int32_t* agg_of_row,
+    const int32_t* rows, const int32_t* cols, const double* vals, const double* bound,
+    unsigned long long* out) {
+  __shared__ unsigned long long acc[kMaxCoarse];
+  const int I = blockIdx.x / kMatrixParts;
+  const int part = blockIdx.x % kMatrixParts;
+  const double scale = row_scale(bound[I]);
+  for (int J = threadIdx.x; J < count; J += kBlock) acc[J] = 0ull;
+  __syncthreads();
+  if (scale > 0.0) {
+    const int32_t me = agg_start[I + 1];
+    for (int32_t m = agg_start[I] + part * kBlock + threadIdx.x; m < me;
+         m += kMatrixParts * kBlock) {
+      const int32_t i = agg_rows[m];
+      int32_t run = -1;
+      long long sum = 0;
+      for (int32_t e = rows[i]; e < rows[i + 1]; ++e) {
+        const int32_t J = agg_of_row[cols[e]];
+        if (J < 0) continue;
+        if (J != run) {
+          if (run >= 0) atomicAdd(&acc[run], static_cast<unsigned long long>(sum));
+          run = J;
+          sum = 0;
+        }
+        sum += llrint(vals[e] * scale);
+      }
+      if (run >= 0) atomicAdd(&acc[run], static_cast<unsigned long long>(sum));
+    }
+  }
+  __syncthreads();
+  unsigned long long* row = out + static_cast<size_t>(I) * count;
+  for (int J = threadIdx.x; J < count; J += kBlock)
+    if (acc[J] != 0ull) atomicAdd(&row[J], acc[J]);
+}
+
+__global__ void __launch_bounds__(kBlock) coarse_convert_kernel(int count, const double* bound,
+                                                                const unsigned long long* in,
+                                                                double* out) {
+  const size_t e = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= static_cast<size_t>(count) * count) return;
+  const double scale = row_scale(bound[e / count]);
+  out[e] = scale > 0.0 ? static_cast<double>(static_cast<long long>(in[e])) / scale : 0.0;
+}
+
+__global__ void __launch_bounds__(kBlock) fill_u64(unsigned long long* p, size_t n) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0ull;
+}
+
+// Gauss-Jordan step k, part 1 (one block): pivot row p >= k of largest |a_rk|
+// (lowest index on ties), swap rows k and p, scale row k by 1 / a_kk with
+// a_kk := 1 first (the in-place inverse), and save column k of the other rows.
+constexpr int kPivotThreads = 1024;
+__global__ void __launch_bounds__(kPivotThreads) gj_pivot_kernel(double* a, int count, int k,
+                                                                 double* colk, int32_t* perm,
+                                                                 int32_t* status) {
+  __shared__ double bv[kPivotThreads];
+  __shared__ int bi[kPivotThreads];
+  const int t = threadIdx.x;
+  double best = -1.0;
+  int bidx = count;
+  for (int r = k + t; r < count; r += kPivotThreads) {
+    const double v = fabs(a[static_cast<size_t>(r) * count + k]);
+    if (v > best) {
+      best = v;
+      bidx = r;
+    }
+  }
+  bv[t] = best;
+  bi[t] = bidx;
+  __syncthreads();
+  for (int w = kPivotThreads / 2; w > 0; w >>= 1) {
+    if (t < w) {
+      const double v = bv[t + w];
+      const int i = bi[t + w];
+      if (v > bv[t] || (v == bv[t] && i < bi[t])) {
+        bv[t] = v;
+        bi[t] = i;
+      }
+    }
+    __syncthreads();
+  }
+  const int p = bi[0];
+  const bool singular = !(bv[0] > 0.0) || !isfinite(bv[0]);
+  if (t == 0) {
+    perm[k] = p < count ? p : k;
+    if (singular) status[0] = 1;
+  }
+  if (singular) return;  // the result is discarded
+  if (p != k) {
+    double* rk = a + static_cast<size_t>(k) * count;
+    double* rp = a + static_cast<size_t>(p) * count;
+    for (int j = t; j < count; j += kPivotThreads) {
+      const double tmp = rk[j];
+      rk[j] = rp[j];
+      rp[j] = tmp;
+    }
+  }
+  __syncthreads();
+  double* rk = a + static_cast<size_t>(k) * count;
+  const double inv = 1.0 / rk[k];
+  __syncthreads();
+  for (int j = t; j < count; j += kPivotThreads) rk[j] = (j == k ? 1.0 : rk[j]) * inv;
+  for (int r = t; r < count; r += kPivotThreads)
+    colk[r] = r == k ? 0.0 : a[static_cast<size_t>(r) * count + k];
+}
+
+// Part 2 (whole grid): a_rj -= a_rk a_kj for r != k, with a_rk := -a_rk a_kk.
+__global__ void __launch_bounds__(kBlock) gj_eliminate_kernel(double* a, int count, int k,
+                                                              const double* colk,
+                                                              const int32_t* status) {
+  if (status[0]) return;
+  const size_t e = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const size_t total = static_cast<size_t>(count) * count;
+  if (e >= total) return;
+  const int r = static_cast<int>(e / count);
+  const int j = static_cast<int>(e % count);
+  if (r == k) return;
+  const double f = colk[r];
+  const double* rk = a + static_cast<size_t>(k) * count;
+  a[e] = j == k ? -f * rk[k] : a[e] - f * rk[j];
+}
+
+// The row swaps of the pivoting become column swaps of the inverse, applied in
+// reverse step order: out[:, j] = a[:, idx[j]] with idx the composed order.
+__global__ void gj_order_kernel(int count, const int32_t* perm, int32_t* idx) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  for (int j = 0; j < count; ++j) idx[j] = j;
+  for (int k = count - 1; k >= 0; --k) {
+    const int p = perm[k];
+    const int32_t tmp = idx[k];
+    idx[k] = idx[p];
+    idx[p] = tmp;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) gj_permute_kernel(const double* a, int count,
+                                                            const int32_t* idx, double* out) {
+  const size_t e = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const size_t total = static_cast<size_t>(count) * count;
+  if (e >= total) return;
+  const size_t r = e / count;
+  const int j = static_cast<int>(e % count);
+  out[e] = a[r * count + idx[j]];
+}
+
+__global__ void __launch_bounds__(kBlock) inverse_kernel(size_t n, const double* pdiag,
+                                                         double* wfac) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) wfac[i] = 1.0 / pdiag[i];
+}
+
+// Block-wide sum with a fixed assignment (thread t takes items t, t + 256, ...)
+// and a fixed tree, so the result does not depend on timing.
+__device__ __forceinline__ double block_sum(double s, double* red) {
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  double t = 0.0;
+#pragma unroll
+  for (int w = 0; w < kWaves; ++w) t += red[w];
+  return t;
+}
+
+// rc[I] = sum over the members i of aggregate I of w_i (times wfac_i); one
+// block per aggregate, two members per thread in flight.
+__global__ void __launch_bounds__(kBlock) restrict_kernel(int count, const int32_t* agg_start,
+                                                          const int32_t* agg_rows,
+                                                          const double* w, const double* wfac,
+                                                          double* rc) {
+  __shared__ double red[kWaves];
+  const int I = blockIdx.x;
+  const int32_t mb = agg_start[I], me = agg_start[I + 1];
+  double s = 0.0;
+  int32_t m = mb + threadIdx.x;
+  for (; m + kBlock < me; m += 2 * kBlock) {
+    const int32_t i0 = agg_rows[m], i1 = agg_rows[m + kBlock];
+    const double w0 = w[i0], w1 = w[i1];
+    if (wfac) {
+      s += w0 * wfac[i0];
+      s += w1 * wfac[i1];
+    } else {
+      s += w0;
+      s += w1;
+    }
+  }
+  for (; m < me; m += kBlock) {
+    const int32_t i = agg_rows[m];
+    s += wfac ? w[i] * wfac[i] : w[i];
+  }
+  s = block_sum(s, red);
+  if (threadIdx.x == 0) rc[I] = s;
+}
+
+// yc = Ainv rc, one block per row.
+__global__ void __launch_bounds__(kBlock) coarse_product_kernel(int count, const double* ainv,
+                                                                const double* rc, double* yc) {
+  __shared__ double red[kWaves];
+  const int I = blockIdx.x;
+  const double* row = ainv + static_cast<size_t>(I) * count;
+  double s = 0.0;
+  for (int J = threadIdx.x; J < count; J += kBlock) s += row[J] * rc[J];
+  s = block_sum(s, red);
+  if (threadIdx.x == 0) yc[I] = s;
+}
+
+__global__ void __launch_bounds__(kBlock) prolong_add_kernel(size_t n, const int32_t* agg_of_row,
+                                                             const double* yc, double* v) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int32_t a = agg_of_row[i];
+  if (a >= 0) v[i] += yc[a];
+}
+
+unsigned wave_blocks(size_t items) { return static_cast<unsigned>((items + kWaves - 1) / kWaves); }
+
+}  // namespace
+
+void coarse_aggregate(Context& ctx, CoarseSpace& C, size_t n, const int32_t* labels,
+                      const int32_t* cell_sorted, const int32_t* cell_begin,
+                      const int32_t* cell_end, const int32_t dims[3], int factor) {
+  C.count = 0;
+  C.n = n;
+  C.ready = false;
+  if (n == 0 || factor < 1 || dims[0] <= 0 || dims[1] <= 0 || dims[2] <= 0) return;
+  CoarseGrid g{dims[0], dims[1], dims[2], (dims[0] + factor - 1) / factor,
+               (dims[1] + factor - 1) / factor, (dims[2] + factor - 1) / factor, factor};
+  const size_t ccells = static_cast<size_t>(g.cx) * g.cy * g.cz;
+  C.cell_count.reserve(ccells);
+  C.cell_start.reserve(ccells);
+  C.cell_flag.reserve(ccells);
+  C.cell_id.reserve(ccells);
+  C.agg_of_row.reserve(n);
+  count_members_kernel<<<wave_blocks(ccells), kBlock, 0, ctx.stream>>>(
+      g, ccells, labels, cell_sorted, cell_begin, cell_end, C.cell_count.data(),
+      C.cell_flag.data());
+  LSMPS_LAUNCH_CHECK();
+  exclusive_scan_i32(ctx, C.cell_count.data(), C.cell_start.data(), ccells);
+  exclusive_scan_i32(ctx, C.cell_flag.data(), C.cell_id.data(), ccells);
+  int32_t* h = ctx.host_i32;
+  LSMPS_HIP(hipMemcpyAsync(h, C.cell_id.data() + ccells - 1, sizeof(int32_t),
+                           hipMemcpyDeviceToHost, ctx.stream));
+  LSMPS_HIP(hipMemcpyAsync(h + 1, C.cell_flag.data() + ccells - 1, sizeof(int32_t),
+                           hipMemcpyDeviceToHost, ctx.stream));
+  LSMPS_HIP(hipMemcpyAsync(h + 2, C.cell_start.data() + ccells - 1, sizeof(int32_t),
+                           hipMemcpyDeviceToHost, ctx.stream));
+  LSMPS_HIP(hipMemcpyAsync(h + 3, C.cell_count.data() + ccells - 1, sizeof(int32_t),
+                           hipMemcpyDeviceToHost, ctx.stream));
+  ctx.sync();
+  const int count = h[0] + h[1];
+  const int members = h[2] + h[3];
+  if (count <= 1 || count > kMaxCoarse || members <= 0) return;  // no useful coarse space
+  C.agg_start.reserve(static_cast<size_t>(count) + 1);
+  C.agg_rows.reserve(static_cast<size_t>(members));
+  fill_i32<<<blocks_for(n), kBlock, 0, ctx.stream>>>(C.agg_of_row.data(), n, -1);
+  fill_members_kernel<<<wave_blocks(ccells), kBlock, 0, ctx.stream>>>(
+      g, ccells, labels, cell_sorted, cell_begin, cell_end, C.cell_count.data(),
+      C.cell_start.data(), C.cell_id.data(), C.agg_start.data(), C.agg_rows.data(),
+      C.agg_of_row.data());
+  set_last_kernel<<<1, 1, 0, ctx.stream>>>(C.agg_start.data(), count, C.cell_count.data(),
+                                           C.cell_start.data(), ccells);
+  LSMPS_LAUNCH_CHECK();
+  C.count = count;
+}
+
+void coarse_setup(Context& ctx, CoarseSpace& C, const lsmps_csr& A, const double* pdiag) {
+  C.ready = false;
+  const int count = C.count;
+  if (count <= 1 || C.n != A.rows) return;
+  const size_t nn = static_cast<size_t>(count) * count;
+  C.work.reserve(nn);
+  C.ainv.reserve(nn);
+  C.colk.reserve(static_cast<size_t>(count));
+  C.perm.reserve(static_cast<size_t>(count));
+  C.status.reserve(1);
+  C.wfac.reserve(A.rows);
+  C.rc.reserve(static_cast<size_t>(count));
+  C.yc.reserve(static_cast<size_t>(count));
+  fill_i32<<<1, 1, 0, ctx.stream>>>(C.status.data(), 1, 0);
+  // integer matrix in ainv's storage (overwritten by the inverse at the end)
+  unsigned long long* acci = reinterpret_cast<unsigned long long*>(C.ainv.data());
+  C.rowabs.reserve(A.rows);
+  row_abs_kernel<<<blocks_for(A.rows), kBlock, 0, ctx.stream>>>(
+      A.rows, C.agg_of_row.data(), A.row_offsets, A.col_indices, A.values, C.rowabs.data());
+  restrict_kernel<<<static_cast<unsigned>(count), kBlock, 0, ctx.stream>>>(
+      count, C.agg_start.data(), C.agg_rows.data(), C.rowabs.data(), nullptr, C.rc.data());
+  fill_u64<<<blocks_for(nn), kBlock, 0, ctx.stream>>>(acci, nn);
+  coarse_matrix_kernel<<<static_cast<unsigned>(count) * kMatrixParts, kBlock, 0, ctx.stream>>>(
+      count, C.agg_start.data(), C.agg_rows.data(), C.agg_of_row.data(), A.row_offsets,
+      A.col_indices, A.values, C.rc.data(), acci);
+  coarse_convert_kernel<<<blocks_for(nn), kBlock, 0, ctx.stream>>>(count, C.rc.data(), acci,
+                                                                   C.work.data());
+  LSMPS_LAUNCH_CHECK();
+  const unsigned eb = blocks_for(nn);
+  for (int k = 0; k < count; ++k) {
+    gj_pivot_kernel<<<1, kPivotThreads, 0, ctx.stream>>>(C.work.data(), count, k, C.colk.data(),
+                                                          C.perm.data(), C.status.data());
+    gj_eliminate_kernel<<<eb, kBlock, 0, ctx.stream>>>(C.work.data(), count, k, C.colk.data(),
+                                                       C.status.data());
+  }
+  LSMPS_LAUNCH_CHECK();
+  // idx reuses colk's storage as int32 (count entries fit in count doubles)
+  int32_t* idx = reinterpret_cast<int32_t*>(C.colk.data());
+  gj_order_kernel<<<1, 1, 0, ctx.stream>>>(count, C.perm.data(), idx);
+  gj_permute_kernel<<<eb, kBlock, 0, ctx.stream>>>(C.work.data(), count, idx, C.ainv.data());
+  inverse_kernel<<<blocks_for(A.rows), kBlock, 0, ctx.stream>>>(A.rows, pdiag, C.wfac.data());
+  LSMPS_LAUNCH_CHECK();
+  LSMPS_HIP(hipMemcpyAsync(ctx.host_i32, C.status.data(), sizeof(int32_t), hipMemcpyDeviceToHost,
+                           ctx.stream));
+  ctx.sync();
+  C.ready = ctx.host_i32[0] == 0;
+}
+
+void coarse_solve(Context& ctx, CoarseSpace& C, const double* w, bool scale_by_wfac) {
+  restrict_kernel<<<static_cast<unsigned>(C.count), kBlock, 0, ctx.stream>>>(
+      C.count, C.agg_start.data(), C.agg_rows.data(), w, scale_by_wfac ? C.wfac.data() : nullptr,
+      C.rc.data());
+  coarse_product_kernel<<<static_cast<unsigned>(C.count), kBlock, 0, ctx.stream>>>(
+      C.count, C.ainv.data(), C.rc.data(), C.yc.data());
+  LSMPS_LAUNCH_CHECK();
+}
+
+void coarse_prolong_add(Context& ctx, const CoarseSpace& C, double* v) {
+  prolong_add_kernel<<<blocks_for(C.n), kBlock, 0, ctx.stream>>>(C.n, C.agg_of_row.data(),
+                                                                 C.yc.data(), v);
+  LSMPS_LAUNCH_CHECK();
+}
+
+}  // namespace lsmps

@@ -1,0 +1,70 @@
+// Coarse-space correction of the PPE preconditioner (no reference counterpart:
+// the reference hands the system to AMGX).
+//
+// Rows are grouped into aggregates (blocks of f x f x f cells of the fluid cell
+// grid; Dirichlet rows -- Surface and Splash particles -- belong to none).  With
+// P the piecewise-constant prolongation over the aggregates, the Galerkin coarse
+// matrix Ac = P^T A P is formed once per solve and inverted explicitly
+// (Gauss-Jordan with partial pivoting), and each preconditioner application
+// adds P Ac^-1 P^T w to the fine-level (Schwarz + Jacobi) result: an additive
+// two-level method.  It removes the smooth error components the local
+// preconditioner cannot reach, which is what dominates the Krylov iteration
+// count on large tanks.
+//
+// Determinism: Ac is accumulated in 64-bit fixed point (integer additions are
+// associative), and every per-aggregate or per-row sum runs in a fixed order, so
+// a solve is bit-reproducible run to run.
+#pragma once
+
+#include "common.hpp"
+
+namespace lsmps {
+
+constexpr int kMaxCoarse = 4096;  // LDS row of the coarse-matrix build (int64)
+
+struct CoarseSpace {
+  size_t n = 0;                      // fine rows the aggregation was built for
+  int count = 0;                     // aggregates (0: no correction)
+  DeviceArray<int32_t> agg_of_row;   // n: aggregate of a row, -1 outside
+  DeviceArray<int32_t> agg_start;    // count + 1
+  DeviceArray<int32_t> agg_rows;     // members, in cell order then index order
+  // setup scratch (per coarse cell of the grid)
+  DeviceArray<int32_t> cell_count, cell_start, cell_flag, cell_id;
+  // per solve
+  DeviceArray<double> ainv;          // count x count, row-major, Ac^-1
+  DeviceArray<double> work;          // count x count, Gauss-Jordan in place
+  DeviceArray<double> colk;          // count
+  DeviceArray<int32_t> perm;         // count: pivot row of step k
+  DeviceArray<int32_t> status;       // [0]: singular pivot met
+  DeviceArray<double> wfac;          // n: w = v * wfac (1 / pdiag)
+  DeviceArray<double> rowabs;        // n: setup scratch (row bounds)
+  DeviceArray<double> rc, yc;        // count
+  bool ready = false;                // ainv valid for the current solve
+  size_t bytes() const {
+    return agg_of_row.bytes() + agg_start.bytes() + agg_rows.bytes() + cell_count.bytes() +
+           cell_start.bytes() + cell_flag.bytes() + cell_id.bytes() + ainv.bytes() +
+           work.bytes() + colk.bytes() + perm.bytes() + status.bytes() + wfac.bytes() + rowabs.bytes() +
+           rc.bytes() + yc.bytes();
+  }
+};
+
+// Aggregation from a fluid cell list: cells (cx, cy, cz) of a dims grid grouped
+// as (cx / f, cy / f, cz / f); rows whose label is Inner or NearSurface join
+// their group.  Members are listed cell by cell in (z, y, x) order, each cell's
+// particles ascending (the cell list's sorted order).
+void coarse_aggregate(Context& ctx, CoarseSpace& C, size_t n, const int32_t* labels,
+                      const int32_t* cell_sorted, const int32_t* cell_begin,
+                      const int32_t* cell_end, const int32_t dims[3], int factor);
+
+// Ac = P^T A P and its inverse; wfac = 1 / pdiag.  Leaves C.ready false (and the
+// solve without the correction) when the aggregation is empty, too large, or Ac
+// has a zero pivot.
+void coarse_setup(Context& ctx, CoarseSpace& C, const lsmps_csr& A, const double* pdiag);
+
+// yc = Ac^-1 P^T (wfac * w) (wfac may be skipped: then yc = Ac^-1 P^T w).
+void coarse_solve(Context& ctx, CoarseSpace& C, const double* w, bool scale_by_wfac);
+
+// v[i] += yc[agg_of_row[i]] on the aggregated rows.
+void coarse_prolong_add(Context& ctx, const CoarseSpace& C, double* v);
+
+}  // namespace lsmps

@@ -30,6 +30,7 @@
 //    it falls by `inner_tolerance` (reliable updates), so the solve converges to
 //    the FP64 system at FP64 accuracy.
 #include <cmath>
+#include <type_traits>
 #include <cstring>
 #include <vector>
 
@@ -956,6 +957,9 @@ struct IdrVec {
   const int32_t* cover_start;
   const int32_t* cover_entries;
   const double* ybuf;
+  // coarse-space term added by idr_a2g_kernel (null without a coarse space)
+  const int32_t* agg_of_row;
+  const double* yc;
 };
 
 // Scalars that govern row i: the system's, or its segment's in a batched solve.
@@ -1061,6 +1065,10 @@ __global__ void __launch_bounds__(kBlock) idr_a2g_kernel(IdrVec<T> a, int k) {
     if (a.cover_start[i + 1] > a.cover_start[i])
       v = static_cast<double>(static_cast<T>(
           schwarz_row_mean(a.cover_start, a.cover_entries, a.ybuf, static_cast<int32_t>(i))));
+    if (a.yc) {
+      const int32_t g = a.agg_of_row[i];
+      if (g >= 0) v += a.yc[g];
+    }
     a.U[k * a.ld + i] = static_cast<T>(u + om * v);
   }
 }
@@ -1612,13 +1620,17 @@ int idr_solve(Context& ctx, PpeSolver& S, size_t n, const SellMatrix& M, const V
   T* G = work + 3 * ld;
   T* U = work + (3 + s) * ld;
   const bool schwarz = S.schwarz.blocks > 0 && S.schwarz.n == n;
+  static_assert(std::is_same<T, double>::value, "the IDR basis is FP64");
+  const bool coarse = schwarz && S.coarse.ready && S.coarse.n == n;
   const V* dia = sizeof(V) == sizeof(double) ? reinterpret_cast<const V*>(M.dia64.data())
                                              : reinterpret_cast<const V*>(M.dia32.data());
   IdrVec<T> iv{n, ld, s, r, x, v, t, G, U, p0, S.scalars.data(), S.part_b.data(),
                schwarz ? S.schwarz.pdiag.data() : dinv, nullptr,
                schwarz ? S.schwarz.cover_start.data() : nullptr,
                schwarz ? S.schwarz.cover_entries.data() : nullptr,
-               schwarz ? S.schwarz.ybuf.data() : nullptr};
+               schwarz ? S.schwarz.ybuf.data() : nullptr,
+               coarse ? S.coarse.agg_of_row.data() : nullptr,
+               coarse ? S.coarse.yc.data() : nullptr};
   const unsigned vb = vec_blocks(n);
   const unsigned sb = sell_blocks(M.chunks);
   fill_kernel<T><<<blocks_for(2 * s * ld), kBlock, 0, ctx.stream>>>(2 * s * ld, G, T(0));
@@ -1665,6 +1677,7 @@ int idr_solve(Context& ctx, PpeSolver& S, size_t n, const SellMatrix& M, const V
     for (int k = 0; k < s; ++k) {
       if (schwarz) {
         schwarz_products<T>(ctx, S.schwarz, v);
+        if (coarse) coarse_solve(ctx, S.coarse, v, true);
         idr_a2g_kernel<T><<<vb, kBlock, 0, ctx.stream>>>(iv, k);
       }
       SellArgs<T, V> sa{n, M.chunks, M.dia_pairs.data(), M.dia_singles.data(), M.dia_plan.data(),
@@ -1688,6 +1701,10 @@ int idr_solve(Context& ctx, PpeSolver& S, size_t n, const SellMatrix& M, const V
     }
     if (schwarz) {
       schwarz_apply<T>(ctx, S.schwarz, v);
+      if (coarse) {
+        coarse_solve(ctx, S.coarse, r, false);
+        coarse_prolong_add(ctx, S.coarse, v);
+      }
     }
     SellArgs<T, V> st{n, M.chunks, M.dia_pairs.data(), M.dia_singles.data(), M.dia_plan.data(),
                       M.dia_start.data(), dia, M.chunk_start.data(), M.chunk_width.data(),
@@ -1764,7 +1781,8 @@ void idr_solve_segmented(Context& ctx, PpeSolver& S, size_t n, const SellMatrix&
   // the segmented recurrence applies the Schwarz blocks with schwarz_apply, so
   // the fused completion pointers stay null
   IdrVec<T> iv{n,  ld, s, r, x, v, t, G, U, p0, S.scalars.data(), S.part_b.data(),
-               schwarz ? S.schwarz.pdiag.data() : dinv, g.row_seg, nullptr, nullptr, nullptr};
+               schwarz ? S.schwarz.pdiag.data() : dinv, g.row_seg, nullptr, nullptr, nullptr,
+               nullptr, nullptr};
   const unsigned vb = vec_blocks(n);
   const unsigned sb = sell_blocks(M.chunks);
   const unsigned rb = static_cast<unsigned>(nseg * kSegBlocks);
@@ -2005,12 +2023,20 @@ void ppe_solve(Context& ctx, PpeSolver& S, const lsmps_csr& A, const double* b, 
     int it = 0, replacements = 0;
     double rr = 0.0;
     S.schwarz.blocks = 0;
-    if (opt.preconditioner == LSMPS_PRECOND_SCHWARZ) {
+    S.coarse.ready = false;
+    if (opt.preconditioner == LSMPS_PRECOND_SCHWARZ ||
+        opt.preconditioner == LSMPS_PRECOND_SCHWARZ_COARSE) {
       schwarz_setup(ctx, S.schwarz, A, S.dinv64.data(), ctx.precision == LSMPS_PRECISION_MIXED);
       int64_t blocks = 0, rows = 0;
       schwarz_stats(ctx, S.schwarz, &blocks, &rows);
       rep.schwarz_blocks = static_cast<int32_t>(blocks);
       rep.schwarz_rows = static_cast<int32_t>(std::min<int64_t>(rows, INT32_MAX));
+      // the coarse space needs the caller's aggregation of these rows, one system
+      if (opt.preconditioner == LSMPS_PRECOND_SCHWARZ_COARSE && S.segments.count == 0 &&
+          S.coarse.count > 1 && S.coarse.n == n && S.schwarz.n == n) {
+        coarse_setup(ctx, S.coarse, A, S.schwarz.pdiag.data());
+        rep.coarse_unknowns = S.coarse.ready ? S.coarse.count : 0;
+      }
     }
     const bool mixed = ctx.precision == LSMPS_PRECISION_MIXED;
     // mixed: FP32 matrix values in the Krylov SpMVs; FP64 basis, iterate,

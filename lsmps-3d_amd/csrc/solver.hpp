@@ -2,6 +2,7 @@
 #pragma once
 
 #include "common.hpp"
+#include "coarse.hpp"
 #include "schwarz.hpp"
 
 namespace lsmps {
@@ -124,6 +125,7 @@ class PpeSolver {
   size_t seg_host_count = 0;
   SellMatrix sell;
   SchwarzPrec schwarz;
+  CoarseSpace coarse;          // aggregation set by the caller (count 0: none)
   DeviceArray<double> dinv64;  // Jacobi: 1 / diagonal
   DeviceArray<double> vec64;   // residual + Krylov basis (FP64)
   DeviceArray<double> part_a;
@@ -136,7 +138,7 @@ class PpeSolver {
   PpeSolver(const PpeSolver&) = delete;
   PpeSolver& operator=(const PpeSolver&) = delete;
   size_t bytes() const {
-    return sell.bytes() + schwarz.bytes() + dinv64.bytes() + vec64.bytes() +
+    return sell.bytes() + schwarz.bytes() + coarse.bytes() + dinv64.bytes() + vec64.bytes() +
            part_a.bytes() + part_b.bytes() + scalars.bytes();
   }
 };

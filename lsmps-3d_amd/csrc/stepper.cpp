@@ -290,6 +290,12 @@ void lsmps_stepper::step(const lsmps_fluid_soa& fluid, const lsmps_wall_soa& wal
   }
   lsmps_solver_options opts = solver_options;
   if (record) opts.clamp_negative = 0;  // the correction clamps the same way
+  // coarse space: aggregates of coarse_cells^3 fluid cells (one system only)
+  solver.coarse.count = 0;
+  if (segment_count == 0 && opts.preconditioner == LSMPS_PRECOND_SCHWARZ_COARSE && nf > 0)
+    coarse_aggregate(c, solver.coarse, nf, fluid.surface_type, fc_sorted.data(), fc_begin.data(),
+                     fc_end.data(), config.cell_dims,
+                     opts.coarse_cells > 0 ? opts.coarse_cells : 4);
   try {
     ppe_solve(c, solver, ws.matrix, ws.rhs, ws.pressure, opts, &last_report);
   } catch (const Error& e) {

@@ -1092,7 +1092,8 @@ int ppe_solve(const lsmps_csr& a, const double* b, double* x, const lsmps_solver
     std::vector<double> M(static_cast<size_t>(s) * s, 0.0), f(s), c(s), v(n), tv(n), d(s);
     // M^-1: Jacobi, or Schwarz blocks on the wrong-sign rows plus Jacobi elsewhere
     SchwarzBlocks blocks;
-    const bool use_schwarz = opt.preconditioner == LSMPS_PRECOND_SCHWARZ;
+    const bool use_schwarz = opt.preconditioner == LSMPS_PRECOND_SCHWARZ ||
+                             opt.preconditioner == LSMPS_PRECOND_SCHWARZ_COARSE;
     {
       const double t0 = omp_get_wtime();
       if (use_schwarz) schwarz_build(a, blocks);

@@ -109,7 +109,8 @@ def test_default_solver_options(rt, abi):
     for fname, _ in abi.SolverOptions._fields_:
         assert getattr(o, fname) == getattr(ref, fname), fname
     assert o.method == abi.SOLVER_IDRS and o.tolerance == 1e-10
-    assert o.preconditioner == abi.PRECOND_SCHWARZ and o.clamp_negative == 1
+    assert o.preconditioner == abi.PRECOND_SCHWARZ_COARSE and o.coarse_cells == 4
+    assert o.clamp_negative == 1
 
 
 def _lattice_density(l0, re, quadratic):

@@ -136,29 +136,41 @@ def test_mixed_vs_fp64_at_250k(rt, abi, cases, oracle):
 
 
 @pytest.mark.parametrize("precision", ["FP64", "MIXED"])
-def test_schwarz_matches_jacobi(rt, abi, cases, precision):
-    """The Schwarz blocks only change the Krylov path: both preconditioners solve
-    the same FP64 system to the 1e-10 residual, so the pressures agree to well
-    within the parity bar, and the Schwarz solve needs fewer SpMVs on a dam-break
-    with wrong-sign wall-corner rows."""
+def test_preconditioners_agree(rt, abi, cases, precision):
+    """The preconditioners only change the Krylov path: Jacobi, Schwarz blocks,
+    and Schwarz plus the coarse-space correction all solve the same FP64 system
+    to the 1e-10 residual, so the pressures agree to well within the parity
+    bar; each addition needs fewer SpMVs on a dam-break with wrong-sign
+    wall-corner rows; and every solve is bit-reproducible."""
     case = cases.dam_break(block=(40, 24, 24), tank=(80, 24, 48), l0=0.01, seed=1)
     prec = getattr(abi, "PRECISION_" + precision)
     jac = abi.default_solver_options()
     jac.preconditioner = abi.PRECOND_JACOBI
     sch = abi.default_solver_options()
-    assert sch.preconditioner == abi.PRECOND_SCHWARZ
+    sch.preconditioner = abi.PRECOND_SCHWARZ
+    crs = abi.default_solver_options()
+    assert crs.preconditioner == abi.PRECOND_SCHWARZ_COARSE
     gj = rt.run_step_on_device(case, precision=prec, solver=jac)
     gs = rt.run_step_on_device(case, precision=prec, solver=sch)
-    print(precision, "jacobi", gj["report"], "schwarz", gs["report"], flush=True)
-    assert gj["report"]["converged"] == 1 and gs["report"]["converged"] == 1
-    assert gj["report"]["schwarz_blocks"] == 0
-    assert gs["report"]["schwarz_blocks"] > 0
+    gc = rt.run_step_on_device(case, precision=prec, solver=crs)
+    print(precision, "jacobi", gj["report"], "schwarz", gs["report"], "coarse", gc["report"],
+          flush=True)
+    for g in (gj, gs, gc):
+        assert g["report"]["converged"] == 1
+    assert gj["report"]["schwarz_blocks"] == 0 and gj["report"]["coarse_unknowns"] == 0
+    assert gs["report"]["schwarz_blocks"] > 0 and gs["report"]["coarse_unknowns"] == 0
     assert gs["report"]["schwarz_rows"] >= gs["report"]["schwarz_blocks"]
+    assert gc["report"]["schwarz_blocks"] == gs["report"]["schwarz_blocks"]
+    assert gc["report"]["coarse_unknowns"] > 1
     assert rel_l2(gs["pressure"], gj["pressure"]) <= 1e-8
+    assert rel_l2(gc["pressure"], gj["pressure"]) <= 1e-8
     assert gs["report"]["iterations"] < gj["report"]["iterations"]
-    # deterministic: a second Schwarz run reproduces the pressure bit for bit
-    gs2 = rt.run_step_on_device(case, precision=prec, solver=sch)
-    assert np.array_equal(gs2["pressure"], gs["pressure"])
+    assert gc["report"]["iterations"] < gs["report"]["iterations"]
+    # deterministic: second runs reproduce the pressure bit for bit
+    assert np.array_equal(rt.run_step_on_device(case, precision=prec, solver=sch)["pressure"],
+                          gs["pressure"])
+    assert np.array_equal(rt.run_step_on_device(case, precision=prec, solver=crs)["pressure"],
+                          gc["pressure"])
 
 
 @pytest.mark.parametrize("precision", ["FP64", "MIXED"])
