@@ -1044,6 +1044,85 @@ void schwarz_apply(const SchwarzBlocks& S, const std::vector<double>& dinv, cons
 
 double g_solver_setup_seconds = 0.0;
 
+// Coarse space of the SCHWARZ_COARSE preconditioner: the aggregate of each row
+// (-1: none), set by orc_set_coarse before a solve.  Ac = P^T A P is summed row
+// by row in index order, inverted by Gauss-Jordan with partial pivoting (lowest
+// index on ties), and applied additively: out += P Ac^-1 P^T in.  (The library
+// forms the same operator with its own summation order.)
+struct CoarseOracle {
+  std::vector<int32_t> agg;
+  int count = 0;
+  std::vector<double> ainv;  // count x count, row-major
+  bool ready = false;
+};
+CoarseOracle g_coarse;
+
+bool coarse_build(const lsmps_csr& a, CoarseOracle& C) {
+  const int m = C.count;
+  const int64_t n = static_cast<int64_t>(a.rows);
+  std::vector<double> w(static_cast<size_t>(m) * m, 0.0);
+  for (int64_t i = 0; i < n; ++i) {
+    const int32_t I = C.agg[i];
+    if (I < 0) continue;
+    for (int32_t c = a.row_offsets[i]; c < a.row_offsets[i + 1]; ++c) {
+      const int32_t J = C.agg[a.col_indices[c]];
+      if (J >= 0) w[static_cast<size_t>(I) * m + J] += a.values[c];
+    }
+  }
+  std::vector<int32_t> perm(m);
+  std::vector<double> colk(m);
+  for (int k = 0; k < m; ++k) {
+    int p = k;
+    double best = std::fabs(w[static_cast<size_t>(k) * m + k]);
+    for (int r = k + 1; r < m; ++r) {
+      const double v = std::fabs(w[static_cast<size_t>(r) * m + k]);
+      if (v > best) {
+        best = v;
+        p = r;
+      }
+    }
+    if (!(best > 0.0) || !std::isfinite(best)) return false;
+    perm[k] = p;
+    if (p != k)
+      for (int j = 0; j < m; ++j)
+        std::swap(w[static_cast<size_t>(k) * m + j], w[static_cast<size_t>(p) * m + j]);
+    double* rk = w.data() + static_cast<size_t>(k) * m;
+    const double inv = 1.0 / rk[k];
+    rk[k] = 1.0;
+    for (int j = 0; j < m; ++j) rk[j] *= inv;
+    for (int r = 0; r < m; ++r) colk[r] = r == k ? 0.0 : w[static_cast<size_t>(r) * m + k];
+#pragma omp parallel for schedule(static)
+    for (int r = 0; r < m; ++r) {
+      if (r == k) continue;
+      double* rr = w.data() + static_cast<size_t>(r) * m;
+      const double f = colk[r];
+      for (int j = 0; j < m; ++j) rr[j] = j == k ? -f * rk[k] : rr[j] - f * rk[j];
+    }
+  }
+  for (int k = m - 1; k >= 0; --k)  // undo the pivoting as column swaps
+    if (perm[k] != k)
+      for (int r = 0; r < m; ++r)
+        std::swap(w[static_cast<size_t>(r) * m + k], w[static_cast<size_t>(r) * m + perm[k]]);
+  C.ainv.swap(w);
+  return true;
+}
+
+void coarse_apply_add(const CoarseOracle& C, const double* in, double* out, int64_t n) {
+  const int m = C.count;
+  std::vector<double> rc(m, 0.0), yc(m, 0.0);
+  for (int64_t i = 0; i < n; ++i)
+    if (C.agg[i] >= 0) rc[C.agg[i]] += in[i];
+#pragma omp parallel for schedule(static)
+  for (int I = 0; I < m; ++I) {
+    double t = 0.0;
+    for (int J = 0; J < m; ++J) t += C.ainv[static_cast<size_t>(I) * m + J] * rc[J];
+    yc[I] = t;
+  }
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; ++i)
+    if (C.agg[i] >= 0) out[i] += yc[C.agg[i]];
+}
+
 int ppe_solve(const lsmps_csr& a, const double* b, double* x, const lsmps_solver_options& opt,
               lsmps_solve_report* rep) {
   const int64_t n = static_cast<int64_t>(a.rows);
@@ -1097,6 +1176,11 @@ int ppe_solve(const lsmps_csr& a, const double* b, double* x, const lsmps_solver
     {
       const double t0 = omp_get_wtime();
       if (use_schwarz) schwarz_build(a, blocks);
+      g_coarse.ready = false;
+      if (opt.preconditioner == LSMPS_PRECOND_SCHWARZ_COARSE && g_coarse.count > 1 &&
+          static_cast<int64_t>(g_coarse.agg.size()) == n)
+        g_coarse.ready = coarse_build(a, g_coarse);
+      report.coarse_unknowns = g_coarse.ready ? g_coarse.count : 0;
       g_solver_setup_seconds = omp_get_wtime() - t0;
       report.schwarz_blocks = static_cast<int32_t>(blocks.members.size());
       for (const auto& m : blocks.members) report.schwarz_rows += static_cast<int32_t>(m.size());
@@ -1108,6 +1192,7 @@ int ppe_solve(const lsmps_csr& a, const double* b, double* x, const lsmps_solver
       } else {
         for (int64_t i = 0; i < n; ++i) out[i] = in[i] * dinv[i];
       }
+      if (g_coarse.ready) coarse_apply_add(g_coarse, in, out, n);
     };
     auto pdot = [&](int q, const double* w) {
       double acc = 0.0;
@@ -1616,6 +1701,27 @@ int orc_ppe_solve(const lsmps_csr* a, const double* b, double* x,
 
 // Seconds spent building the preconditioner in the last IDR(s) solve.
 double orc_solver_setup_seconds(void) { return g_solver_setup_seconds; }
+
+// Aggregation used by SCHWARZ_COARSE solves from now on (agg NULL or count < 2:
+// none).
+int orc_set_coarse(const int32_t* agg, int64_t n, int32_t count) {
+  g_coarse = CoarseOracle{};
+  if (!agg || n <= 0 || count < 2) return LSMPS_OK;
+  g_coarse.agg.assign(agg, agg + n);
+  g_coarse.count = count;
+  return LSMPS_OK;
+}
+
+// z = (P Ac^-1 P^T) v for the current aggregation and matrix `a` (tests).
+int orc_coarse_apply(const lsmps_csr* a, const double* v, double* z) {
+  const int64_t n = static_cast<int64_t>(a->rows);
+  if (g_coarse.count < 2 || static_cast<int64_t>(g_coarse.agg.size()) != n)
+    return fail(LSMPS_ERR_INVALID_ARGUMENT, "no coarse aggregation for this matrix");
+  if (!coarse_build(*a, g_coarse)) return fail(LSMPS_ERR_NUMERICAL, "singular coarse matrix");
+  for (int64_t i = 0; i < n; ++i) z[i] = 0.0;
+  coarse_apply_add(g_coarse, v, z, n);
+  return LSMPS_OK;
+}
 
 // z = M^-1 v with the Schwarz preconditioner of `a` (built on each call).
 // Returns the number of blocks.

@@ -261,7 +261,60 @@ class Oracle:
                 "values": vals[:used].copy(), "nnz": nnz, "rhs": rhs, "divergence": div,
                 "pressure_laplacian": lap}
 
-    def solve(self, row_offsets, col_indices, values, rhs, options=None):
+    def coarse_aggregation(self, case, nb, labels, factor=4, config=None):
+        """Aggregates of factor^3 fluid cells (the SCHWARZ_COARSE coarse space):
+        the aggregate of each fluid row, -1 for Surface / Splash rows; aggregates
+        are numbered in the linear order of their coarse cells."""
+        cfg = config or case.config
+        nx, ny, nz = (int(cfg.cell_dims[k]) for k in range(3))
+        # the cell list holds the ids in sorted order: cell_ids[k] is the cell of
+        # particle sorted_indices[k]
+        fc = nb["fluid_cells"]
+        cid = np.empty(case.n_fluid, np.int64)
+        cid[fc["sorted_indices"]] = fc["cell_ids"]
+        cx, cy, cz = cid % nx, (cid // nx) % ny, cid // (nx * ny)
+        gx, gy = -(-nx // factor), -(-ny // factor)
+        key = ((cz // factor) * gy + cy // factor) * gx + cx // factor
+        active = (labels == 0) | (labels == 1)
+        agg = np.full(case.n_fluid, -1, np.int32)
+        uniq, inv = np.unique(key[active], return_inverse=True)
+        agg[active] = inv.astype(np.int32)
+        return agg, int(uniq.shape[0])
+
+    def coarse_apply(self, row_offsets, col_indices, values, v, agg, count):
+        """z = P Ac^-1 P^T v with Ac = P^T A P over the aggregation."""
+        a = self.abi
+        n = v.shape[0]
+        rows = np.ascontiguousarray(row_offsets, np.int32)
+        cols = np.ascontiguousarray(col_indices, np.int32)
+        vals = np.ascontiguousarray(values, np.float64)
+        vv = np.ascontiguousarray(v, np.float64)
+        z = np.zeros(n)
+        csr = a.Csr(n, n, vals.shape[0], a.iptr(_p(rows)), a.iptr(_p(cols)), a.dptr(_p(vals)))
+        g = np.ascontiguousarray(agg, np.int32)
+        self.lib.orc_set_coarse(a.iptr(_p(g)), C.c_int64(n), C.c_int32(count))
+        try:
+            self._check(self.lib.orc_coarse_apply(C.byref(csr), a.dptr(_p(vv)), a.dptr(_p(z))),
+                        "coarse_apply")
+        finally:
+            self.lib.orc_set_coarse(None, C.c_int64(0), C.c_int32(0))
+        return z
+
+    def solve(self, row_offsets, col_indices, values, rhs, options=None, coarse=None):
+        """PPE solve; `coarse` = (aggregate per row, count) enables the coarse
+        space of a SCHWARZ_COARSE solve."""
+        a = self.abi
+        n = rhs.shape[0]
+        if coarse is not None:
+            g = np.ascontiguousarray(coarse[0], np.int32)
+            self.lib.orc_set_coarse(a.iptr(_p(g)), C.c_int64(n), C.c_int32(int(coarse[1])))
+        try:
+            return self._solve(row_offsets, col_indices, values, rhs, options)
+        finally:
+            if coarse is not None:
+                self.lib.orc_set_coarse(None, C.c_int64(0), C.c_int32(0))
+
+    def _solve(self, row_offsets, col_indices, values, rhs, options=None):
         a = self.abi
         n = rhs.shape[0]
         opt = options or a.default_solver_options()
@@ -334,8 +387,12 @@ class Oracle:
         tmp = self.provision(c2, nb, vel["inverse"])
         ppe = self.assemble(c2, nb, tmp, vel["inverse"], pre["inverse"],
                             neighbor_capacity=case.n_fluid * max_neighbors)
+        opt = options or a.default_solver_options()
+        coarse = None
+        if opt.preconditioner == a.PRECOND_SCHWARZ_COARSE:
+            coarse = self.coarse_aggregation(case, nb, st, opt.coarse_cells or 4)
         p, rep = self.solve(ppe["row_offsets"], ppe["col_indices"], ppe["values"], ppe["rhs"],
-                            options)
+                            opt, coarse=coarse)
         cor = self.correction(c2, nb, tmp, p, pre["inverse"])
         return {"neighbors": nb, "surface": cls, "velocity_moment": vel,
                 "pressure_moment": pre, "provision": tmp, "ppe": ppe, "pressure": p,
