@@ -103,9 +103,10 @@ def test_schwarz_reduces_iterations(oracle, abi, system):
 
 @pytest.fixture(scope="module")
 def coarse_system(oracle, cases, abi):
-    """A larger dam-break PPE (4,608 rows) with its fluid cells and labels, so
-    that 2x2x2-cell aggregates give a coarse space of a few dozen unknowns."""
-    case = cases.dam_break(block=(32, 12, 12), tank=(64, 12, 24), l0=0.01, seed=2)
+    """A larger dam-break PPE (23,040 rows) with its fluid cells and labels, so
+    that 2x2x2-cell aggregates give a coarse space of about a hundred unknowns
+    and the smooth error components it removes matter to the iteration count."""
+    case = cases.dam_break(block=(40, 24, 24), tank=(80, 24, 48), l0=0.01, seed=2)
     res = oracle.pipeline(case, abi.default_solver_options())
     ppe = res["ppe"]
     n = case.n_fluid
