@@ -122,17 +122,13 @@ __global__ void fill_i32(int32_t* p, size_t n, int32_t v) {
 // 2^s_I B_I < 2^61: every a_ij enters as round(a_ij 2^s_I) and all additions
 // are exact integer additions, whose order does not matter.
 
-// |a_ij| summed over the aggregated columns of row i (rows outside: 0).
-__global__ void __launch_bounds__(kBlock) row_abs_kernel(size_t n, const int32_t* agg_of_row,
-                                                         const int32_t* rows, const int32_t* cols,
-                                                         const double* vals, double* out) {
+// Row bound |a_ii| + sum |a_ij|, j != i, for the aggregated rows (others: 0).
+__global__ void __launch_bounds__(kBlock) row_bound_kernel(size_t n, const int32_t* agg_of_row,
+                                                           const double* diag,
+                                                           const double* offabs, double* out) {
   const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  double s = 0.0;
-  if (agg_of_row[i] >= 0)
-    for (int32_t e = rows[i]; e < rows[i + 1]; ++e)
-      if (agg_of_row[cols[e]] >= 0) s += fabs(vals[e]);
-  out[i] = s;
+  out[i] = agg_of_row[i] >= 0 ? fabs(diag[i]) + offabs[i] : 0.0;
 }
 
 __device__ __forceinline__ double row_scale(double B) {
@@ -140,38 +136,34 @@ __device__ __forceinline__ double row_scale(double B) {
 }
 
 // Block (I, part) adds the entries of its share of aggregate I's members into
-// an LDS row (consecutive entries mostly share their aggregate -- a row's
-// neighbors come cell by cell -- so each run is summed in a register first),
-// then the nonzero LDS entries into row I of the integer matrix.
+// an LDS row, then the nonzero LDS entries into row I of the integer matrix.
+// Sixteen lanes share a member row and stride over its entries (coalesced CSR
+// reads); the LDS atomics are integer additions, so their order is free.
 constexpr int kMatrixParts = 8;
+constexpr int kMatrixLanes = 16;
 __global__ void __launch_bounds__(kBlock) coarse_matrix_kernel(
     int count, const int32_t* agg_start, const int32_t* agg_rows, const int32_t* agg_of_row,
     const int32_t* rows, const int32_t* cols, const double* vals, const double* bound,
     unsigned long long* out) {
   __shared__ unsigned long long acc[kMaxCoarse];
+  constexpr int kGroups = kBlock / kMatrixLanes;
   const int I = blockIdx.x / kMatrixParts;
   const int part = blockIdx.x % kMatrixParts;
+  const int group = threadIdx.x / kMatrixLanes;
+  const int l = threadIdx.x % kMatrixLanes;
   const double scale = row_scale(bound[I]);
   for (int J = threadIdx.x; J < count; J += kBlock) acc[J] = 0ull;
   __syncthreads();
   if (scale > 0.0) {
     const int32_t me = agg_start[I + 1];
-    for (int32_t m = agg_start[I] + part * kBlock + threadIdx.x; m < me;
-         m += kMatrixParts * kBlock) {
+    for (int32_t m = agg_start[I] + part * kGroups + group; m < me;
+         m += kMatrixParts * kGroups) {
       const int32_t i = agg_rows[m];
-      int32_t run = -1;
-      long long sum = 0;
-      for (int32_t e = rows[i]; e < rows[i + 1]; ++e) {
+      for (int32_t e = rows[i] + l; e < rows[i + 1]; e += kMatrixLanes) {
         const int32_t J = agg_of_row[cols[e]];
-        if (J < 0) continue;
-        if (J != run) {
-          if (run >= 0) atomicAdd(&acc[run], static_cast<unsigned long long>(sum));
-          run = J;
-          sum = 0;
-        }
-        sum += llrint(vals[e] * scale);
+        if (J >= 0)
+          atomicAdd(&acc[J], static_cast<unsigned long long>(llrint(vals[e] * scale)));
       }
-      if (run >= 0) atomicAdd(&acc[run], static_cast<unsigned long long>(sum));
     }
   }
   __syncthreads();
@@ -341,9 +333,15 @@ __global__ void __launch_bounds__(kBlock) restrict_kernel(int count, const int32
 
 // yc = Ainv rc, one block per row.
 __global__ void __launch_bounds__(kBlock) coarse_product_kernel(int count, const double* ainv,
-                                                                const double* rc, double* yc) {
+                                                                const double* rc,
+                                                                const int32_t* status,
+                                                                double* yc) {
   __shared__ double red[kWaves];
   const int I = blockIdx.x;
+  if (status[0]) {  // zero pivot in the inverse: no correction
+    if (threadIdx.x == 0) yc[I] = 0.0;
+    return;
+  }
   const double* row = ainv + static_cast<size_t>(I) * count;
   double s = 0.0;
   for (int J = threadIdx.x; J < count; J += kBlock) s += row[J] * rc[J];
@@ -410,7 +408,9 @@ void coarse_aggregate(Context& ctx, CoarseSpace& C, size_t n, const int32_t* lab
   C.count = count;
 }
 
-void coarse_setup(Context& ctx, CoarseSpace& C, const lsmps_csr& A, const double* pdiag) {
+void coarse_setup(Context& ctx, CoarseSpace& C, const lsmps_csr& A, const double* diag,
+                  const double* offabs, hipStream_t st) {
+  (void)ctx;
   C.ready = false;
   const int count = C.count;
   if (count <= 1 || C.n != A.rows) return;
@@ -420,42 +420,51 @@ void coarse_setup(Context& ctx, CoarseSpace& C, const lsmps_csr& A, const double
   C.colk.reserve(static_cast<size_t>(count));
   C.perm.reserve(static_cast<size_t>(count));
   C.status.reserve(1);
-  C.wfac.reserve(A.rows);
   C.rc.reserve(static_cast<size_t>(count));
   C.yc.reserve(static_cast<size_t>(count));
-  fill_i32<<<1, 1, 0, ctx.stream>>>(C.status.data(), 1, 0);
+  fill_i32<<<1, 1, 0, st>>>(C.status.data(), 1, 0);
   // integer matrix in ainv's storage (overwritten by the inverse at the end)
   unsigned long long* acci = reinterpret_cast<unsigned long long*>(C.ainv.data());
   C.rowabs.reserve(A.rows);
-  row_abs_kernel<<<blocks_for(A.rows), kBlock, 0, ctx.stream>>>(
-      A.rows, C.agg_of_row.data(), A.row_offsets, A.col_indices, A.values, C.rowabs.data());
-  restrict_kernel<<<static_cast<unsigned>(count), kBlock, 0, ctx.stream>>>(
+  row_bound_kernel<<<blocks_for(A.rows), kBlock, 0, st>>>(
+      A.rows, C.agg_of_row.data(), diag, offabs, C.rowabs.data());
+  restrict_kernel<<<static_cast<unsigned>(count), kBlock, 0, st>>>(
       count, C.agg_start.data(), C.agg_rows.data(), C.rowabs.data(), nullptr, C.rc.data());
-  fill_u64<<<blocks_for(nn), kBlock, 0, ctx.stream>>>(acci, nn);
-  coarse_matrix_kernel<<<static_cast<unsigned>(count) * kMatrixParts, kBlock, 0, ctx.stream>>>(
+  fill_u64<<<blocks_for(nn), kBlock, 0, st>>>(acci, nn);
+  coarse_matrix_kernel<<<static_cast<unsigned>(count) * kMatrixParts, kBlock, 0, st>>>(
       count, C.agg_start.data(), C.agg_rows.data(), C.agg_of_row.data(), A.row_offsets,
       A.col_indices, A.values, C.rc.data(), acci);
-  coarse_convert_kernel<<<blocks_for(nn), kBlock, 0, ctx.stream>>>(count, C.rc.data(), acci,
+  coarse_convert_kernel<<<blocks_for(nn), kBlock, 0, st>>>(count, C.rc.data(), acci,
                                                                    C.work.data());
   LSMPS_LAUNCH_CHECK();
   const unsigned eb = blocks_for(nn);
   for (int k = 0; k < count; ++k) {
-    gj_pivot_kernel<<<1, kPivotThreads, 0, ctx.stream>>>(C.work.data(), count, k, C.colk.data(),
+    gj_pivot_kernel<<<1, kPivotThreads, 0, st>>>(C.work.data(), count, k, C.colk.data(),
                                                           C.perm.data(), C.status.data());
-    gj_eliminate_kernel<<<eb, kBlock, 0, ctx.stream>>>(C.work.data(), count, k, C.colk.data(),
+    gj_eliminate_kernel<<<eb, kBlock, 0, st>>>(C.work.data(), count, k, C.colk.data(),
                                                        C.status.data());
   }
   LSMPS_LAUNCH_CHECK();
   // idx reuses colk's storage as int32 (count entries fit in count doubles)
   int32_t* idx = reinterpret_cast<int32_t*>(C.colk.data());
-  gj_order_kernel<<<1, 1, 0, ctx.stream>>>(count, C.perm.data(), idx);
-  gj_permute_kernel<<<eb, kBlock, 0, ctx.stream>>>(C.work.data(), count, idx, C.ainv.data());
-  inverse_kernel<<<blocks_for(A.rows), kBlock, 0, ctx.stream>>>(A.rows, pdiag, C.wfac.data());
+  gj_order_kernel<<<1, 1, 0, st>>>(count, C.perm.data(), idx);
+  gj_permute_kernel<<<eb, kBlock, 0, st>>>(C.work.data(), count, idx, C.ainv.data());
   LSMPS_LAUNCH_CHECK();
+  C.ready = true;  // a zero pivot is reported in C.status (the product then yields 0)
+}
+
+void coarse_set_weights(Context& ctx, CoarseSpace& C, const double* pdiag) {
+  C.wfac.reserve(C.n);
+  inverse_kernel<<<blocks_for(C.n), kBlock, 0, ctx.stream>>>(C.n, pdiag, C.wfac.data());
+  LSMPS_LAUNCH_CHECK();
+}
+
+bool coarse_usable(Context& ctx, const CoarseSpace& C) {
+  if (!C.ready) return false;
   LSMPS_HIP(hipMemcpyAsync(ctx.host_i32, C.status.data(), sizeof(int32_t), hipMemcpyDeviceToHost,
                            ctx.stream));
   ctx.sync();
-  C.ready = ctx.host_i32[0] == 0;
+  return ctx.host_i32[0] == 0;
 }
 
 void coarse_solve(Context& ctx, CoarseSpace& C, const double* w, bool scale_by_wfac) {
@@ -463,7 +472,7 @@ void coarse_solve(Context& ctx, CoarseSpace& C, const double* w, bool scale_by_w
       C.count, C.agg_start.data(), C.agg_rows.data(), w, scale_by_wfac ? C.wfac.data() : nullptr,
       C.rc.data());
   coarse_product_kernel<<<static_cast<unsigned>(C.count), kBlock, 0, ctx.stream>>>(
-      C.count, C.ainv.data(), C.rc.data(), C.yc.data());
+      C.count, C.ainv.data(), C.rc.data(), C.status.data(), C.yc.data());
   LSMPS_LAUNCH_CHECK();
 }
 

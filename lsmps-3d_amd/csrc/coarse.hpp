@@ -56,10 +56,16 @@ void coarse_aggregate(Context& ctx, CoarseSpace& C, size_t n, const int32_t* lab
                       const int32_t* cell_sorted, const int32_t* cell_begin,
                       const int32_t* cell_end, const int32_t dims[3], int factor);
 
-// Ac = P^T A P and its inverse; wfac = 1 / pdiag.  Leaves C.ready false (and the
-// solve without the correction) when the aggregation is empty, too large, or Ac
-// has a zero pivot.
-void coarse_setup(Context& ctx, CoarseSpace& C, const lsmps_csr& A, const double* pdiag);
+// Ac = P^T A P and its inverse, launched on stream st (no host wait): diag and
+// offabs are A's row diagonals and off-diagonal magnitude sums (they bound the
+// fixed-point rows).  Leaves C.ready false when the aggregation is empty or too
+// large; a zero pivot sets C.status, after which every coarse product is 0.
+void coarse_setup(Context& ctx, CoarseSpace& C, const lsmps_csr& A, const double* diag,
+                  const double* offabs, hipStream_t st);
+// wfac = 1 / pdiag (the fine preconditioner's diagonal part), on ctx's stream.
+void coarse_set_weights(Context& ctx, CoarseSpace& C, const double* pdiag);
+// After the setup has completed: whether the inverse is valid (host wait).
+bool coarse_usable(Context& ctx, const CoarseSpace& C);
 
 // yc = Ac^-1 P^T (wfac * w) (wfac may be skipped: then yc = Ac^-1 P^T w).
 void coarse_solve(Context& ctx, CoarseSpace& C, const double* w, bool scale_by_wfac);

@@ -33,20 +33,13 @@ constexpr int kTable = 4096;  // LDS hash table entries (power of two)
 // Per row: 1 when eligible to join a block (has off-diagonal couplings),
 // 2 when it is also a seed (positive diagonal).  Dirichlet rows (identity rows
 // of Surface / Splash particles) have no off-diagonal values and stay Jacobi.
-__global__ void classify_rows_kernel(size_t n, const int32_t* rows, const int32_t* cols,
-                                     const double* vals, int32_t* eligible, int32_t* seed) {
+__global__ void classify_rows_kernel(size_t n, const double* diag, const double* offabs,
+                                     int32_t* eligible, int32_t* seed) {
   const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  double diag = 0.0, off = 0.0;
-  for (int32_t c = rows[i]; c < rows[i + 1]; ++c) {
-    if (cols[c] == static_cast<int32_t>(i)) {
-      diag += vals[c];
-    } else {
-      off += fabs(vals[c]);
-    }
-  }
+  const double off = offabs[i];
   eligible[i] = off > 0.0 ? 1 : 0;
-  seed[i] = (off > 0.0 && diag > 0.0) ? 1 : 0;
+  seed[i] = (off > 0.0 && diag[i] > 0.0) ? 1 : 0;
 }
 
 __global__ void compact_seeds_kernel(size_t n, const int32_t* flag, const int32_t* pos,
@@ -501,8 +494,8 @@ __global__ void compact_kernel(size_t n, const int32_t* flag, const int32_t* pos
 
 }  // namespace
 
-void schwarz_setup(Context& ctx, SchwarzPrec& P, const lsmps_csr& A, const double* dinv,
-                   bool fp32_inverse) {
+void schwarz_setup(Context& ctx, SchwarzPrec& P, const lsmps_csr& A, const double* diag,
+                   const double* dinv, const double* offabs, bool fp32_inverse) {
   const size_t n = A.rows;
   P.n = n;
   P.blocks = 0;
@@ -511,7 +504,7 @@ void schwarz_setup(Context& ctx, SchwarzPrec& P, const lsmps_csr& A, const doubl
   P.seed_flag.reserve(n);
   P.seed_pos.reserve(n + 1);
   classify_rows_kernel<<<blocks_for(n), kBlock, 0, ctx.stream>>>(
-      n, A.row_offsets, A.col_indices, A.values, P.eligible.data(), P.seed_flag.data());
+      n, diag, offabs, P.eligible.data(), P.seed_flag.data());
   LSMPS_LAUNCH_CHECK();
   exclusive_scan_i32(ctx, P.seed_flag.data(), P.seed_pos.data(), n);
   LSMPS_HIP(hipMemcpyAsync(ctx.host_i32, P.seed_pos.data() + n, sizeof(int32_t),

@@ -50,9 +50,10 @@ __device__ __forceinline__ double schwarz_row_mean(const int32_t* start, const i
   return z / static_cast<double>(hi - lo);
 }
 
-// Builds the blocks of matrix A; dinv is its Jacobi diagonal 1 / A_ii.
-void schwarz_setup(Context& ctx, SchwarzPrec& P, const lsmps_csr& A, const double* dinv,
-                   bool fp32_inverse);
+// Builds the blocks of matrix A from its per-row statistics: diagonal A_ii,
+// Jacobi diagonal 1 / A_ii and the sum of the off-diagonal magnitudes.
+void schwarz_setup(Context& ctx, SchwarzPrec& P, const lsmps_csr& A, const double* diag,
+                   const double* dinv, const double* offabs, bool fp32_inverse);
 // In-place completion of a preconditioned vector: on entry v = pdiag * w (w on
 // covered rows, Jacobi values elsewhere); on exit each covered row holds the mean
 // of its blocks' values (inv_b w_b), summed in block order.

@@ -41,6 +41,12 @@
 namespace lsmps {
 
 PpeSolver::~PpeSolver() {
+  if (aux) {
+    (void)hipStreamSynchronize(aux);
+    (void)hipEventDestroy(ev_fork);
+    (void)hipEventDestroy(ev_join);
+    (void)hipStreamDestroy(aux);
+  }
   if (host) (void)hipHostFree(host);
   if (seg_host) (void)hipHostFree(seg_host);
 }
@@ -552,18 +558,35 @@ __global__ void sell_fill_kernel(size_t n, size_t chunks, const int32_t* rows,
                                  const int32_t* chunk_start, const int32_t* width,
                                  const int32_t* col_start, const int32_t* nwin,
                                  const int32_t* windows, uint16_t* out_cols, V* out_vals) {
+  // The chunk's sorted offsets, their storage slots and its window bases are
+  // staged in LDS (each wave has its own region), so the per-entry binary
+  // search and window lookup read LDS rather than global memory.
+  constexpr int kWaves = kBlock / kSellChunk;
+  __shared__ int32_t s_off[kWaves][kMaxDia];
+  __shared__ int32_t s_slot[kWaves][kMaxDia];
+  __shared__ int32_t s_win[kWaves][kSellWindows];
   const size_t r = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (r >= chunks * kSellChunk) return;
+  const bool active = r < chunks * kSellChunk;  // uniform per wavefront
+  const int wv = threadIdx.x / kSellChunk;
   const size_t chunk = r / kSellChunk;
   const int lane = static_cast<int>(r % kSellChunk);
-  const int nd = dia_count[chunk];
-  const int32_t* dl = dia_offsets + chunk * kMaxDia;
+  const int nd = active ? dia_count[chunk] : 0;
+  if (active) {
+    for (int t = lane; t < nd; t += kSellChunk) {
+      s_off[wv][t] = dia_offsets[chunk * kMaxDia + t];
+      s_slot[wv][t] = dia_slot[chunk * kMaxDia + t];
+    }
+    if (lane < kSellWindows) s_win[wv][lane] = windows[chunk * kSellWindows + lane];
+  }
+  __syncthreads();
+  if (!active) return;
+  const int32_t* dl = s_off[wv];
   const int64_t dbase = dia_start[chunk];
   const int64_t vbase = chunk_start[chunk];
   const int64_t cbase = static_cast<int64_t>(col_start[chunk]) * kSellChunk;
   const int32_t w = width[chunk];
   const int32_t nw = nwin[chunk];
-  const int32_t* wb = windows + chunk * kSellWindows;
+  const int32_t* wb = s_win[wv];
   const int32_t b = r < n ? rows[r] : 0;
   const int32_t len = r < n ? rows[r + 1] - b : 0;
   int32_t k = 0;  // next remainder slot
@@ -583,7 +606,7 @@ __global__ void sell_fill_kernel(size_t n, size_t chunks, const int32_t* rows,
     const int32_t c = cols[b + q];
     const int j = dia_find(dl, nd, c - static_cast<int32_t>(r));
     if (j >= 0) {
-      dia_vals[dbase + quad_slot(dia_slot[chunk * kMaxDia + j], lane)] = static_cast<V>(vals[b + q]);
+      dia_vals[dbase + quad_slot(s_slot[wv][j], lane)] = static_cast<V>(vals[b + q]);
     } else {
       put(c, static_cast<V>(vals[b + q]));
     }
@@ -894,15 +917,34 @@ __global__ void __launch_bounds__(kBlock) residual_kernel(size_t n, const int32_
   block_reduce_store<1>(acc, 1, partials);
 }
 
-template <typename T>
-__global__ void jacobi_kernel(size_t n, const int32_t* rows, const int32_t* cols,
-                              const double* vals, T* dinv) {
-  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  double d = 0.0;
-  for (int32_t c = rows[i]; c < rows[i + 1]; ++c)
-    if (cols[c] == static_cast<int32_t>(i)) d += vals[c];
-  dinv[i] = static_cast<T>(d != 0.0 ? 1.0 / d : 1.0);
+// Per row: the diagonal (sum of the entries in column i), its inverse for
+// Jacobi (1 where it is 0), and the sum of the off-diagonal magnitudes.  Sixteen
+// lanes share a row and stride over its entries, so the CSR streams are read
+// in coalesced runs; the lane sums meet in a fixed butterfly.
+constexpr int kRowLanes = 16;
+__global__ void __launch_bounds__(kBlock) row_stats_kernel(size_t n, const int32_t* rows,
+                                                           const int32_t* cols,
+                                                           const double* vals, double* diag,
+                                                           double* dinv, double* offabs) {
+  const size_t i = (static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x) / kRowLanes;
+  const int l = threadIdx.x % kRowLanes;
+  double d = 0.0, o = 0.0;
+  if (i < n) {
+    for (int32_t c = rows[i] + l; c < rows[i + 1]; c += kRowLanes) {
+      const double v = vals[c];
+      if (cols[c] == static_cast<int32_t>(i)) d += v; else o += fabs(v);
+    }
+  }
+#pragma unroll
+  for (int off = kRowLanes / 2; off > 0; off >>= 1) {
+    d += __shfl_xor(d, off, kRowLanes);
+    o += __shfl_xor(o, off, kRowLanes);
+  }
+  if (i < n && l == 0) {
+    diag[i] = d;
+    dinv[i] = d != 0.0 ? 1.0 / d : 1.0;
+    offabs[i] = o;
+  }
 }
 
 template <typename T>
@@ -1983,8 +2025,11 @@ void ppe_solve(Context& ctx, PpeSolver& S, const lsmps_csr& A, const double* b, 
   S.part_a.reserve(static_cast<size_t>(kMaxReduceBlocks) * kPartialStride);
   S.part_b.reserve(static_cast<size_t>(kMaxReduceBlocks) * kPartialStride);
   S.dinv64.reserve(n);
-  jacobi_kernel<double><<<blocks_for(n), kBlock, 0, ctx.stream>>>(n, A.row_offsets, A.col_indices,
-                                                                   A.values, S.dinv64.data());
+  S.diag64.reserve(n);
+  S.offabs.reserve(n);
+  row_stats_kernel<<<blocks_for(n * kRowLanes), kBlock, 0, ctx.stream>>>(
+      n, A.row_offsets, A.col_indices, A.values, S.diag64.data(), S.dinv64.data(),
+      S.offabs.data());
   LSMPS_LAUNCH_CHECK();
   // ||b||
   dot_self_kernel<<<vec_blocks(n), kBlock, 0, ctx.stream>>>(n, b, S.part_b.data());
@@ -2024,18 +2069,34 @@ void ppe_solve(Context& ctx, PpeSolver& S, const lsmps_csr& A, const double* b, 
     double rr = 0.0;
     S.schwarz.blocks = 0;
     S.coarse.ready = false;
+    bool forked = false;  // the coarse setup runs on S.aux (joined before the loop)
     if (opt.preconditioner == LSMPS_PRECOND_SCHWARZ ||
         opt.preconditioner == LSMPS_PRECOND_SCHWARZ_COARSE) {
-      schwarz_setup(ctx, S.schwarz, A, S.dinv64.data(), ctx.precision == LSMPS_PRECISION_MIXED);
+      // The coarse space (it needs the caller's aggregation of these rows, one
+      // system) is set up on a second stream, overlapping the Schwarz setup and
+      // the SpMV layout build; the IDR loop waits for it.
+      const bool want_coarse = opt.preconditioner == LSMPS_PRECOND_SCHWARZ_COARSE &&
+                               S.segments.count == 0 && S.coarse.count > 1 && S.coarse.n == n;
+      if (want_coarse) {
+        S.ensure_aux();
+        LSMPS_HIP(hipEventRecord(S.ev_fork, ctx.stream));
+        LSMPS_HIP(hipStreamWaitEvent(S.aux, S.ev_fork, 0));
+        coarse_setup(ctx, S.coarse, A, S.diag64.data(), S.offabs.data(), S.aux);
+        LSMPS_HIP(hipEventRecord(S.ev_join, S.aux));
+        forked = true;
+      }
+      schwarz_setup(ctx, S.schwarz, A, S.diag64.data(), S.dinv64.data(), S.offabs.data(),
+                    ctx.precision == LSMPS_PRECISION_MIXED);
       int64_t blocks = 0, rows = 0;
       schwarz_stats(ctx, S.schwarz, &blocks, &rows);
       rep.schwarz_blocks = static_cast<int32_t>(blocks);
       rep.schwarz_rows = static_cast<int32_t>(std::min<int64_t>(rows, INT32_MAX));
-      // the coarse space needs the caller's aggregation of these rows, one system
-      if (opt.preconditioner == LSMPS_PRECOND_SCHWARZ_COARSE && S.segments.count == 0 &&
-          S.coarse.count > 1 && S.coarse.n == n && S.schwarz.n == n) {
-        coarse_setup(ctx, S.coarse, A, S.schwarz.pdiag.data());
-        rep.coarse_unknowns = S.coarse.ready ? S.coarse.count : 0;
+      if (want_coarse && S.coarse.ready) {
+        if (S.schwarz.n == n) {
+          coarse_set_weights(ctx, S.coarse, S.schwarz.pdiag.data());
+        } else {
+          S.coarse.ready = false;
+        }
       }
     }
     const bool mixed = ctx.precision == LSMPS_PRECISION_MIXED;
@@ -2049,6 +2110,7 @@ void ppe_solve(Context& ctx, PpeSolver& S, const lsmps_csr& A, const double* b, 
     } else {
       build_sell<double>(ctx, S.sell, A, nullptr);
     }
+    if (forked) LSMPS_HIP(hipStreamWaitEvent(ctx.stream, S.ev_join, 0));
     if (S.segments.count > 0) {
       // independent systems stacked in one matrix: per-segment recurrences
       if (!S.seg_host || S.seg_host_count < static_cast<size_t>(S.segments.count)) {
@@ -2081,6 +2143,7 @@ void ppe_solve(Context& ctx, PpeSolver& S, const lsmps_csr& A, const double* b, 
       }
       rep.iterations = it;
       rep.refinements = replacements;
+      if (S.coarse.ready && coarse_usable(ctx, S.coarse)) rep.coarse_unknowns = S.coarse.count;
       rep.relative_residual = std::sqrt(rr) / bnorm;
       rep.converged = rep.relative_residual <= opt.tolerance ? 1 : 0;
     }

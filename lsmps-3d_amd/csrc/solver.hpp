@@ -127,18 +127,31 @@ class PpeSolver {
   SchwarzPrec schwarz;
   CoarseSpace coarse;          // aggregation set by the caller (count 0: none)
   DeviceArray<double> dinv64;  // Jacobi: 1 / diagonal
+  DeviceArray<double> diag64;  // row diagonal
+  DeviceArray<double> offabs;  // row sum of |off-diagonal|
   DeviceArray<double> vec64;   // residual + Krylov basis (FP64)
   DeviceArray<double> part_a;
   DeviceArray<double> part_b;
   DeviceArray<SolverScalars> scalars;
   double* host = nullptr;      // pinned: [0] rr, [1] bnorm2, ...
+  // second stream for setup work that overlaps the main stream (default
+  // priority), with its fork / join events
+  hipStream_t aux = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  void ensure_aux() {
+    if (aux) return;
+    LSMPS_HIP(hipStreamCreateWithFlags(&aux, hipStreamNonBlocking));
+    LSMPS_HIP(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+    LSMPS_HIP(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+  }
 
   PpeSolver() = default;
   ~PpeSolver();
   PpeSolver(const PpeSolver&) = delete;
   PpeSolver& operator=(const PpeSolver&) = delete;
   size_t bytes() const {
-    return sell.bytes() + schwarz.bytes() + coarse.bytes() + dinv64.bytes() + vec64.bytes() +
+    return sell.bytes() + schwarz.bytes() + coarse.bytes() + dinv64.bytes() + diag64.bytes() +
+           offabs.bytes() + vec64.bytes() +
            part_a.bytes() + part_b.bytes() + scalars.bytes();
   }
 };
