@@ -186,78 +186,230 @@ __global__ void __launch_bounds__(kBlock) fill_u64(unsigned long long* p, size_t
   if (i < n) p[i] = 0ull;
 }
 
-// Gauss-Jordan step k, part 1 (one block): pivot row p >= k of largest |a_rk|
-// (lowest index on ties), swap rows k and p, scale row k by 1 / a_kk with
-// a_kk := 1 first (the in-place inverse), and save column k of the other rows.
-constexpr int kPivotThreads = 1024;
-__global__ void __launch_bounds__(kPivotThreads) gj_pivot_kernel(double* a, int count, int k,
-                                                                 double* colk, int32_t* perm,
-                                                                 int32_t* status) {
-  __shared__ double bv[kPivotThreads];
-  __shared__ int bi[kPivotThreads];
-  const int t = threadIdx.x;
-  double best = -1.0;
-  int bidx = count;
-  for (int r = k + t; r < count; r += kPivotThreads) {
-    const double v = fabs(a[static_cast<size_t>(r) * count + k]);
-    if (v > best) {
-      best = v;
-      bidx = r;
-    }
+// Blocked Gauss-Jordan inverse with partial pivoting.  Step k: pivot row p >= k
+// of largest |a_rk| (lowest index on ties), swap rows k and p, scale row k by
+// inv = 1 / a_kk with a_kk := 1 first (the in-place inverse), then for every
+// other row r, with f = a_rk before the step: a_rk := -f inv and
+// a_rj -= f a_kj (j != k).
+//
+// The steps run in panels of b columns [k0, k0 + b): gj_panel_kernel (one
+// block, a row's panel entries in its thread's registers) applies the b steps
+// to the panel's columns and records, per step, the pivot row, inv and the
+// column f; gj_update_kernel then applies the same swaps, scalings and
+// eliminations, in the same order and with the same arithmetic, to every other
+// column.  So the result equals the unblocked elimination's bit for bit, with
+// two launches per panel instead of two per column.
+constexpr int kPanelThreads = 1024;
+constexpr int kPanelWaves = kPanelThreads / 64;
+constexpr int kPanelLds = 7680;   // doubles of the update's f columns (b x count)
+constexpr int kMaxPanel = 8;
+constexpr int kUpdateRows = 32;   // column entries per thread in the update
+static_assert(kMaxCoarse <= 4 * kPanelThreads, "panel rows per thread");
+
+int panel_width(int count) { return std::max(1, std::min(kMaxPanel, kPanelLds / count)); }
+
+__device__ __forceinline__ void better_pivot(double& v, int& i, double v2, int i2) {
+  if (v2 > v || (v2 == v && i2 < i)) {
+    v = v2;
+    i = i2;
   }
-  bv[t] = best;
-  bi[t] = bidx;
-  __syncthreads();
-  for (int w = kPivotThreads / 2; w > 0; w >>= 1) {
-    if (t < w) {
-      const double v = bv[t + w];
-      const int i = bi[t + w];
-      if (v > bv[t] || (v == bv[t] && i < bi[t])) {
-        bv[t] = v;
-        bi[t] = i;
+}
+
+// Entry c of a register row (unrolled selects: no dynamic register indexing).
+__device__ __forceinline__ double pick(const double (&row)[kMaxPanel], int c) {
+  double x = row[0];
+#pragma unroll
+  for (int i = 1; i < kMaxPanel; ++i) x = c == i ? row[i] : x;
+  return x;
+}
+
+// panel: steps k0 .. k0 + b - 1 on columns [k0, k0 + b); fcol (b x count) and
+// pinv (b) receive f and inv of each step, perm[k] the pivot row.  Thread t
+// holds rows t, t + kPanelThreads, ... (kPanelRows of them, covering count).
+template <int kPanelRows>
+__global__ void __launch_bounds__(kPanelThreads) gj_panel_kernel(double* a, int count, int k0,
+                                                                 int b, double* fcol,
+                                                                 double* pinv, int32_t* perm,
+                                                                 int32_t* status) {
+  __shared__ double wv[kPanelWaves];
+  __shared__ int wi[kPanelWaves];
+  __shared__ double xk[kMaxPanel], xp[kMaxPanel], rk[kMaxPanel];
+  if (status[0]) return;  // an earlier panel met a zero pivot
+  const int t = threadIdx.x;
+  double P[kPanelRows][kMaxPanel];
+#pragma unroll
+  for (int q = 0; q < kPanelRows; ++q) {
+    const int r = t + q * kPanelThreads;
+#pragma unroll
+    for (int c = 0; c < kMaxPanel; ++c)
+      P[q][c] = r < count && c < b ? a[static_cast<size_t>(r) * count + k0 + c] : 0.0;
+  }
+  for (int s = 0; s < b; ++s) {
+    const int k = k0 + s;
+    double best = -1.0;
+    int bidx = count;
+#pragma unroll
+    for (int q = 0; q < kPanelRows; ++q) {
+      const int r = t + q * kPanelThreads;
+      const double v = fabs(pick(P[q], s));
+      if (r >= k && r < count && v > best) {
+        best = v;
+        bidx = r;
+      }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const double v2 = __shfl_xor(best, off, 64);
+      const int i2 = __shfl_xor(bidx, off, 64);
+      better_pivot(best, bidx, v2, i2);
+    }
+    if ((t & 63) == 0) {
+      wv[t >> 6] = best;
+      wi[t >> 6] = bidx;
+    }
+    __syncthreads();
+    best = wv[0];
+    bidx = wi[0];
+    for (int w = 1; w < kPanelWaves; ++w) better_pivot(best, bidx, wv[w], wi[w]);
+    const int p = bidx;
+    const bool singular = !(best > 0.0) || !isfinite(best);
+    if (t == 0) {
+      perm[k] = p < count ? p : k;
+      if (singular) status[0] = 1;
+    }
+    if (singular) return;  // uniform: every thread read the same reduction
+    // rows k and p through LDS
+#pragma unroll
+    for (int q = 0; q < kPanelRows; ++q) {
+      const int r = t + q * kPanelThreads;
+      if (r == k) {
+#pragma unroll
+        for (int c = 0; c < kMaxPanel; ++c) xk[c] = P[q][c];
+      }
+      if (r == p) {
+#pragma unroll
+        for (int c = 0; c < kMaxPanel; ++c) xp[c] = P[q][c];
       }
     }
     __syncthreads();
-  }
-  const int p = bi[0];
-  const bool singular = !(bv[0] > 0.0) || !isfinite(bv[0]);
-  if (t == 0) {
-    perm[k] = p < count ? p : k;
-    if (singular) status[0] = 1;
-  }
-  if (singular) return;  // the result is discarded
-  if (p != k) {
-    double* rk = a + static_cast<size_t>(k) * count;
-    double* rp = a + static_cast<size_t>(p) * count;
-    for (int j = t; j < count; j += kPivotThreads) {
-      const double tmp = rk[j];
-      rk[j] = rp[j];
-      rp[j] = tmp;
+    const double inv = 1.0 / xp[s];  // the pivot, now at row k
+#pragma unroll
+    for (int q = 0; q < kPanelRows; ++q) {
+      const int r = t + q * kPanelThreads;
+      if (r == p) {
+#pragma unroll
+        for (int c = 0; c < kMaxPanel; ++c) P[q][c] = xk[c];
+      }
+      if (r == k) {
+#pragma unroll
+        for (int c = 0; c < kMaxPanel; ++c) {
+          P[q][c] = (c == s ? 1.0 : xp[c]) * inv;
+          rk[c] = P[q][c];
+        }
+      }
     }
+    if (t == 0) pinv[s] = inv;
+    __syncthreads();
+    double kr[kMaxPanel];
+#pragma unroll
+    for (int c = 0; c < kMaxPanel; ++c) kr[c] = rk[c];
+    const double ks = rk[s];
+#pragma unroll
+    for (int q = 0; q < kPanelRows; ++q) {
+      const int r = t + q * kPanelThreads;
+      if (r >= count) continue;
+      if (r == k) {
+        fcol[static_cast<size_t>(s) * count + r] = 0.0;
+        continue;
+      }
+      const double f = pick(P[q], s);
+      fcol[static_cast<size_t>(s) * count + r] = f;
+#pragma unroll
+      for (int c = 0; c < kMaxPanel; ++c) P[q][c] = c == s ? -f * ks : P[q][c] - f * kr[c];
+    }
+    __syncthreads();  // xk / xp / rk / wv are rewritten by the next step
   }
-  __syncthreads();
-  double* rk = a + static_cast<size_t>(k) * count;
-  const double inv = 1.0 / rk[k];
-  __syncthreads();
-  for (int j = t; j < count; j += kPivotThreads) rk[j] = (j == k ? 1.0 : rk[j]) * inv;
-  for (int r = t; r < count; r += kPivotThreads)
-    colk[r] = r == k ? 0.0 : a[static_cast<size_t>(r) * count + k];
+#pragma unroll
+  for (int q = 0; q < kPanelRows; ++q) {
+    const int r = t + q * kPanelThreads;
+    if (r >= count) continue;
+#pragma unroll
+    for (int c = 0; c < kMaxPanel; ++c)
+      if (c < b) a[static_cast<size_t>(r) * count + k0 + c] = P[q][c];
+  }
 }
 
-// Part 2 (whole grid): a_rj -= a_rk a_kj for r != k, with a_rk := -a_rk a_kk.
-__global__ void __launch_bounds__(kBlock) gj_eliminate_kernel(double* a, int count, int k,
-                                                              const double* colk,
-                                                              const int32_t* status) {
-  if (status[0]) return;
-  const size_t e = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  const size_t total = static_cast<size_t>(count) * count;
-  if (e >= total) return;
-  const int r = static_cast<int>(e / count);
-  const int j = static_cast<int>(e % count);
-  if (r == k) return;
-  const double f = colk[r];
-  const double* rk = a + static_cast<size_t>(k) * count;
-  a[e] = j == k ? -f * rk[k] : a[e] - f * rk[j];
+// The panel's steps on every column outside it.  A block takes tj adjacent
+// columns; its threads hold rows g, g + G, ... (G = kBlock / tj) of one column
+// in registers for all b steps, with the panel's f columns in LDS.
+__global__ void __launch_bounds__(kBlock) gj_update_kernel(double* a, int count, int k0, int b,
+                                                           int tj, const double* fcol,
+                                                           const double* pinv,
+                                                           const int32_t* perm,
+                                                           const int32_t* status) {
+  __shared__ double F[kPanelLds];
+  __shared__ double xk[64], xp[64], rk[64];
+  if (status[0]) return;  // uniform: written before this launch
+  const int t = threadIdx.x;
+  for (int e = t; e < b * count; e += kBlock) F[e] = fcol[e];
+  const int cj = t % tj;
+  const int g = t / tj;
+  const int G = kBlock / tj;
+  const int u = static_cast<int>(blockIdx.x) * tj + cj;  // column among the count - b others
+  const bool live = u < count - b;
+  const int j = u < k0 ? u : u + b;
+  double v[kUpdateRows];
+#pragma unroll
+  for (int q = 0; q < kUpdateRows; ++q) {
+    const int r = g + q * G;
+    v[q] = live && r < count ? a[static_cast<size_t>(r) * count + j] : 0.0;
+  }
+  __syncthreads();
+  for (int s = 0; s < b; ++s) {
+    const int k = k0 + s;
+    const int p = perm[k];
+    const double inv = pinv[s];
+    // swap rows k and p of this column, then scale row k
+#pragma unroll
+    for (int q = 0; q < kUpdateRows; ++q) {
+      const int r = g + q * G;
+      if (r == k) xk[cj] = v[q];
+      if (r == p) xp[cj] = v[q];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kUpdateRows; ++q) {
+      const int r = g + q * G;
+      if (r == p) v[q] = xk[cj];
+      if (r == k) {
+        v[q] = xp[cj] * inv;
+        rk[cj] = v[q];
+      }
+    }
+    __syncthreads();
+    const double pk = rk[cj];
+    const double* f = F + s * count;
+#pragma unroll
+    for (int q = 0; q < kUpdateRows; ++q) {
+      const int r = g + q * G;
+      if (r < count && r != k) v[q] = v[q] - f[r] * pk;
+    }
+    __syncthreads();  // xk / xp / rk are rewritten by the next step
+  }
+  if (!live) return;
+#pragma unroll
+  for (int q = 0; q < kUpdateRows; ++q) {
+    const int r = g + q * G;
+    if (r < count) a[static_cast<size_t>(r) * count + j] = v[q];
+  }
+}
+
+// Columns per update block: the largest power of two tj <= 64 whose rows fit
+// kUpdateRows per thread.
+int update_columns(int count) {
+  int tj = 64;
+  while (tj > 1 && (count + kBlock / tj - 1) / (kBlock / tj) > kUpdateRows) tj >>= 1;
+  return tj;
 }
 
 // The row swaps of the pivoting become column swaps of the inverse, applied in
@@ -417,7 +569,7 @@ void coarse_setup(Context& ctx, CoarseSpace& C, const lsmps_csr& A, const double
   const size_t nn = static_cast<size_t>(count) * count;
   C.work.reserve(nn);
   C.ainv.reserve(nn);
-  C.colk.reserve(static_cast<size_t>(count));
+  C.colk.reserve(static_cast<size_t>(kMaxPanel) * count + kMaxPanel);
   C.perm.reserve(static_cast<size_t>(count));
   C.status.reserve(1);
   C.rc.reserve(static_cast<size_t>(count));
@@ -438,11 +590,25 @@ void coarse_setup(Context& ctx, CoarseSpace& C, const lsmps_csr& A, const double
                                                                    C.work.data());
   LSMPS_LAUNCH_CHECK();
   const unsigned eb = blocks_for(nn);
-  for (int k = 0; k < count; ++k) {
-    gj_pivot_kernel<<<1, kPivotThreads, 0, st>>>(C.work.data(), count, k, C.colk.data(),
-                                                          C.perm.data(), C.status.data());
-    gj_eliminate_kernel<<<eb, kBlock, 0, st>>>(C.work.data(), count, k, C.colk.data(),
-                                                       C.status.data());
+  const int b = panel_width(count);
+  const int tj = update_columns(count);
+  double* pinv = C.colk.data() + static_cast<size_t>(kMaxPanel) * count;
+  for (int k0 = 0; k0 < count; k0 += b) {
+    const int w = std::min(b, count - k0);
+    if (count <= kPanelThreads) {
+      gj_panel_kernel<1><<<1, kPanelThreads, 0, st>>>(C.work.data(), count, k0, w, C.colk.data(),
+                                                      pinv, C.perm.data(), C.status.data());
+    } else if (count <= 2 * kPanelThreads) {
+      gj_panel_kernel<2><<<1, kPanelThreads, 0, st>>>(C.work.data(), count, k0, w, C.colk.data(),
+                                                      pinv, C.perm.data(), C.status.data());
+    } else {
+      gj_panel_kernel<4><<<1, kPanelThreads, 0, st>>>(C.work.data(), count, k0, w, C.colk.data(),
+                                                      pinv, C.perm.data(), C.status.data());
+    }
+    if (count > w) {
+      gj_update_kernel<<<static_cast<unsigned>((count - w + tj - 1) / tj), kBlock, 0, st>>>(
+          C.work.data(), count, k0, w, tj, C.colk.data(), pinv, C.perm.data(), C.status.data());
+    }
   }
   LSMPS_LAUNCH_CHECK();
   // idx reuses colk's storage as int32 (count entries fit in count doubles)

@@ -33,7 +33,7 @@ struct CoarseSpace {
   // per solve
   DeviceArray<double> ainv;          // count x count, row-major, Ac^-1
   DeviceArray<double> work;          // count x count, Gauss-Jordan in place
-  DeviceArray<double> colk;          // count
+  DeviceArray<double> colk;          // Gauss-Jordan panel columns f (b x count) and 1 / pivot (b)
   DeviceArray<int32_t> perm;         // count: pivot row of step k
   DeviceArray<int32_t> status;       // [0]: singular pivot met
   DeviceArray<double> wfac;          // n: w = v * wfac (1 / pdiag)

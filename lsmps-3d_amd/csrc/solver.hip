@@ -34,6 +34,9 @@
 #include <cstring>
 #include <vector>
 
+#include <exception>
+#include <thread>
+
 #include "common.hpp"
 #include "solver.hpp"
 #include "stages.hpp"
@@ -332,28 +335,76 @@ __device__ __forceinline__ void load_quad(const int32_t* p, int32_t v[kQuad]) {
   v[3] = q.w;
 }
 constexpr int32_t kDiaEmpty = INT32_MIN;
+constexpr int kBucketBits = kSellWindowBits - 1;  // 2048-column buckets
 
-// One 64-thread block per chunk, lane = row: counts the offsets column - row
-// of the chunk's entries in an LDS hash table, keeps those present in at least
-// kDiaMinRows rows as the chunk's diagonals (ascending), and sizes the
-// remainder.  A table overflow or more than kMaxDia qualifying offsets leaves
+// The build kernels walk a chunk's entries (row-sorted, contiguous in the CSR)
+// 64 at a time, lane i taking entry base + i, so every load is coalesced.  The
+// entry's row within the chunk comes from the chunk's 65 row starts in LDS: the
+// last row whose start is <= e (empty rows share a start with the next row).
+__device__ __forceinline__ int chunk_row(const int32_t* rs, int32_t e) {
+  int lo = 0, hi = kSellChunk - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (rs[mid] <= e) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Loads the chunk's row starts (rows past n start at the CSR's end).
+__device__ __forceinline__ void load_row_starts(size_t n, size_t chunk, const int32_t* rows,
+                                                int32_t* rs, int lane) {
+  const size_t r = chunk * kSellChunk + lane;
+  rs[lane] = rows[r < n ? r : n];
+  if (lane == 0) {
+    const size_t end = chunk * kSellChunk + kSellChunk;
+    rs[kSellChunk] = rows[end < n ? end : n];
+  }
+}
+
+// Rank of a flagged entry among the flagged entries of its row that lie in
+// earlier lanes of the same 64-entry group (the row's entries are contiguous
+// lanes), and whether this lane is the row's last lane in the group.
+struct SegmentRank {
+  int before;
+  bool tail;
+};
+__device__ __forceinline__ SegmentRank segment_rank(int lane, int row, bool flag) {
+  const int prev = __shfl(row, lane > 0 ? lane - 1 : 0, 64);
+  const int next = __shfl(row, lane < 63 ? lane + 1 : 63, 64);
+  const bool head = lane == 0 || prev != row;
+  const bool tail = lane == 63 || next != row;
+  const unsigned long long heads = __ballot(head);
+  const unsigned long long flags = __ballot(flag);
+  const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  const int start = 63 - __clzll(heads & (below | (1ull << lane)));
+  const unsigned long long seg = below & ~((1ull << start) - 1ull);
+  return {static_cast<int>(__popcll(flags & seg)), tail};
+}
+
+// One 64-thread block per chunk: counts the offsets column - row of the
+// chunk's entries in an LDS hash table, keeps those present in at least
+// kDiaMinRows rows as the chunk's diagonals (ascending), sizes the remainder,
+// plans the diagonals' storage and lays the remainder's column windows.  A
+// table overflow or more than kMaxDia qualifying offsets leaves
 // the chunk without diagonals, so the selection never depends on the order of
 // the atomics.
 __global__ void __launch_bounds__(kSellChunk) dia_select_kernel(
     size_t n, const int32_t* rows, const int32_t* cols, int32_t* dia_count,
     int32_t* dia_offsets, int32_t* dia_slot, int32_t* dia_plan, int32_t* dia_pairs,
     int32_t* dia_singles, int32_t* dia_slots, int32_t* rem_width, int32_t* rem_scaled,
-    unsigned long long* rem_total) {
+    unsigned long long* rem_total, int32_t* nwin, int32_t* windows, int32_t* col_rows) {
   __shared__ int32_t keys[kDiaTable];
   __shared__ int32_t counts[kDiaTable];
   __shared__ int32_t list[kMaxDia];
   __shared__ int32_t nlist;
   __shared__ int32_t failed;
+  __shared__ int32_t rs[kSellChunk + 1];
+  __shared__ int32_t rem_count[kSellChunk + 1];
   const size_t chunk = blockIdx.x;
   const int lane = threadIdx.x;
-  const size_t r = chunk * kSellChunk + lane;
-  const int32_t b = r < n ? rows[r] : 0;
-  const int32_t e = r < n ? rows[r + 1] : 0;
+  const int32_t row0 = static_cast<int32_t>(chunk * kSellChunk);
+  load_row_starts(n, chunk, rows, rs, lane);
+  rem_count[lane] = 0;
   for (int t = lane; t < kDiaTable; t += kSellChunk) {
     keys[t] = kDiaEmpty;
     counts[t] = 0;
@@ -363,8 +414,11 @@ __global__ void __launch_bounds__(kSellChunk) dia_select_kernel(
     failed = 0;
   }
   __syncthreads();
-  for (int32_t c = b; c < e; ++c) {
-    const int32_t d = cols[c] - static_cast<int32_t>(r);
+  const int32_t e0 = rs[0], e1 = rs[kSellChunk];
+  for (int32_t base = e0; base < e1; base += kSellChunk) {
+    const int32_t c = base + lane;
+    if (c >= e1) break;
+    const int32_t d = cols[c] - (row0 + chunk_row(rs, c));
     uint32_t h = (static_cast<uint32_t>(d) * 2654435761u) & (kDiaTable - 1);
     bool placed = false;
     for (int probe = 0; probe < kDiaTable; ++probe) {
@@ -411,16 +465,75 @@ __global__ void __launch_bounds__(kSellChunk) dia_select_kernel(
     }
   }
   __syncthreads();
-  // remainder entries of this row, and the chunk's widest remainder row
-  int32_t rem = 0;
-  for (int32_t c = b; c < e; ++c)
-    if (dia_find(list, nd, cols[c] - static_cast<int32_t>(r)) < 0) ++rem;
-  int32_t width = rem;
-  unsigned long long total = static_cast<unsigned long long>(rem);
+  // Remainder entries per row (the chunk's widest remainder row sizes the
+  // remainder), and the 2048-wide column buckets the remainder touches, as a
+  // 128-bit mask relative to the chunk's first row.
+  const int64_t first_bucket = static_cast<int64_t>(row0 >> kBucketBits) - 64;
+  uint64_t lo = 0, hi = 0;
+  int outside = 0;
+  for (int32_t base = e0; base < e1; base += kSellChunk) {
+    const int32_t c = base + lane;
+    const bool in = c < e1;
+    const int lr = in ? chunk_row(rs, c) : kSellChunk;
+    const int32_t col = in ? cols[c] : 0;
+    const bool rem = in && dia_find(list, nd, col - (row0 + lr)) < 0;
+    if (rem) {
+      const int64_t rel = (static_cast<int64_t>(col) >> kBucketBits) - first_bucket;
+      if (rel < 0 || rel >= 128) {
+        outside = 1;
+      } else if (rel < 64) {
+        lo |= 1ull << rel;
+      } else {
+        hi |= 1ull << (rel - 64);
+      }
+    }
+    const SegmentRank sr = segment_rank(lane, lr, rem);
+    if (in && sr.tail) rem_count[lr] += sr.before + (rem ? 1 : 0);
+    __syncthreads();
+  }
+  const int32_t rem_row = rem_count[lane];
+  int32_t width = rem_row;
+  unsigned long long total = static_cast<unsigned long long>(rem_row);
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
     width = max(width, __shfl_xor(width, off, 64));
     total += __shfl_xor(total, off, 64);
+    lo |= __shfl_xor(lo, off, 64);
+    hi |= __shfl_xor(hi, off, 64);
+    outside |= __shfl_xor(outside, off, 64);
+  }
+  // Windows of two buckets (4096 columns), laid greedily from the lowest
+  // uncovered bucket; columns outside the 128-bucket span, or more than
+  // kSellWindows windows, give the chunk 32-bit columns.
+  {
+    int32_t* wb = windows + chunk * kSellWindows;
+    int count = 0;
+    bool wide = outside != 0;
+    for (int bit = 0; bit < 128 && !wide; ++bit) {
+      const bool set = bit < 64 ? ((lo >> bit) & 1ull) : ((hi >> (bit - 64)) & 1ull);
+      if (!set) continue;
+      if (count == kSellWindows) {
+        wide = true;
+        break;
+      }
+      if (lane == 0) wb[count] = static_cast<int32_t>((first_bucket + bit) << kBucketBits);
+      ++count;
+      ++bit;  // the window also covers the next bucket
+    }
+    if (!wide && count == 0) {  // a chunk without remainder: one window at 0 for the padding
+      if (lane == 0) wb[0] = 0;
+      count = 1;
+    }
+    if (wide) {  // 32-bit columns; padding points at the chunk's first row
+      if (lane == 0) wb[0] = row0;
+      count = 1;
+    }
+    if (lane == 0) {
+      for (int j = count; j < kSellWindows; ++j) wb[j] = wb[count - 1];
+      nwin[chunk] = wide ? -1 : count;
+      const int32_t w4 = (width + kQuad - 1) / kQuad * kQuad;
+      col_rows[chunk] = w4 * (wide ? 2 : 1);
+    }
   }
   // Storage plan (lane 0; nd <= kMaxDia): pairs (d, d + 1) greedily along the
   // runs of the sorted list, kept only when every pair reads inside [0, n) for
@@ -429,14 +542,15 @@ __global__ void __launch_bounds__(kSellChunk) dia_select_kernel(
   // quads; padding repeats an offset of its part (value 0).
   const int32_t width4 = (width + kQuad - 1) / kQuad * kQuad;
   if (lane == 0) {
-    const int64_t row0 = static_cast<int64_t>(chunk) * kSellChunk;
-    const int64_t rend = row0 + kSellChunk - 1;
+    const int64_t rend = static_cast<int64_t>(row0) + kSellChunk - 1;
     const int64_t rlast = rend < static_cast<int64_t>(n) - 1 ? rend : static_cast<int64_t>(n) - 1;
     bool safe = true;
     int pairs = 0;
     for (int i = 0; i < nd;) {
       if (i + 1 < nd && list[i + 1] == list[i] + 1) {
-        if (row0 + list[i] < 0 || rlast + list[i] + 1 > static_cast<int64_t>(n) - 1) safe = false;
+        if (static_cast<int64_t>(row0) + list[i] < 0 ||
+            rlast + list[i] + 1 > static_cast<int64_t>(n) - 1)
+          safe = false;
         ++pairs;
         i += 2;
       } else {
@@ -475,143 +589,77 @@ __global__ void __launch_bounds__(kSellChunk) dia_select_kernel(
   }
 }
 
-// Windows of one chunk (one wavefront per chunk, lane = row), in one pass over
-// the rows: each lane marks the 2048-wide column buckets its row touches in a
-// 128-bit mask relative to the chunk's first row, the masks are OR-reduced
-// over the wavefront, and windows of two buckets (4096 columns) are laid
-// greedily from the lowest uncovered bucket.  Columns outside the 128-bucket
-// span, or more than kSellWindows windows, give the chunk 32-bit columns.
-constexpr int kBucketBits = kSellWindowBits - 1;
-__global__ void sell_window_kernel(size_t n, size_t chunks, const int32_t* rows,
-                                   const int32_t* cols, const int32_t* dia_count,
-                                   const int32_t* dia_offsets, const int32_t* width,
-                                   int32_t* nwin, int32_t* windows, int32_t* col_rows) {
-  const size_t chunk = (static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x) / kSellChunk;
-  if (chunk >= chunks) return;  // uniform per wavefront
-  const int lane = threadIdx.x & 63;
-  const size_t r = chunk * kSellChunk + lane;
-  const int32_t b = r < n ? rows[r] : 0;
-  const int32_t e = r < n ? rows[r + 1] : 0;
-  const int64_t first_bucket = static_cast<int64_t>(chunk * kSellChunk >> kBucketBits) - 64;
-  const int nd = dia_count[chunk];
-  const int32_t* dl = dia_offsets + chunk * kMaxDia;
-  uint64_t lo = 0, hi = 0;
-  int outside = 0;
-  for (int32_t c = b; c < e; ++c) {
-    if (dia_find(dl, nd, cols[c] - static_cast<int32_t>(r)) >= 0) continue;  // a DIA entry
-    const int64_t rel = (static_cast<int64_t>(cols[c]) >> kBucketBits) - first_bucket;
-    if (rel < 0 || rel >= 128) {
-      outside = 1;
-    } else if (rel < 64) {
-      lo |= 1ull << rel;
-    } else {
-      hi |= 1ull << (rel - 64);
-    }
-  }
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    lo |= __shfl_xor(lo, off, 64);
-    hi |= __shfl_xor(hi, off, 64);
-    outside |= __shfl_xor(outside, off, 64);
-  }
-  // every lane now holds the chunk's masks; lay the windows
-  int32_t* wb = windows + chunk * kSellWindows;
-  int count = 0;
-  bool wide = outside != 0;
-  for (int bit = 0; bit < 128 && !wide; ++bit) {
-    const bool set = bit < 64 ? ((lo >> bit) & 1ull) : ((hi >> (bit - 64)) & 1ull);
-    if (!set) continue;
-    if (count == kSellWindows) {
-      wide = true;
-      break;
-    }
-    if (lane == 0) wb[count] = static_cast<int32_t>((first_bucket + bit) << kBucketBits);
-    ++count;
-    ++bit;  // the window also covers the next bucket
-  }
-  if (!wide && count == 0) {  // a chunk without entries: one window at 0 for the padding
-    if (lane == 0) wb[0] = 0;
-    count = 1;
-  }
-  if (wide) {  // 32-bit columns; padding points at the chunk's first row
-    if (lane == 0) wb[0] = static_cast<int32_t>(chunk * kSellChunk);
-    count = 1;
-  }
-  if (lane == 0) {
-    for (int j = count; j < kSellWindows; ++j) wb[j] = wb[count - 1];
-    nwin[chunk] = wide ? -1 : count;
-    col_rows[chunk] = width[chunk] * (wide ? 2 : 1);
-  }
-}
-
-// One thread per (padded) row writes its entries: values, and columns encoded
-// against the chunk's windows (or as 32-bit indices).  Padding points at base_0.
-// One thread per (padded) row: DIA values into their diagonal slots (the DIA
-// array is zeroed beforehand), every other entry into the remainder with its
-// column encoded against the chunk's windows (or as a 32-bit index); remainder
-// padding points at base_0 with value 0.
+// One 64-thread block per chunk writes its entries, 64 at a time in CSR order
+// (coalesced loads): DIA values into their diagonal slots (the DIA array is
+// zeroed beforehand), every other entry into the next remainder slot of its row
+// with its column encoded against the chunk's windows (or as a 32-bit index);
+// then each lane pads its row's remainder to the chunk's width (base_0, value 0).
 template <typename V>
-__global__ void sell_fill_kernel(size_t n, size_t chunks, const int32_t* rows,
-                                 const int32_t* cols, const double* vals,
-                                 const int32_t* dia_count, const int32_t* dia_offsets,
-                                 const int32_t* dia_slot, const int32_t* dia_start, V* dia_vals,
-                                 const int32_t* chunk_start, const int32_t* width,
-                                 const int32_t* col_start, const int32_t* nwin,
-                                 const int32_t* windows, uint16_t* out_cols, V* out_vals) {
-  // The chunk's sorted offsets, their storage slots and its window bases are
-  // staged in LDS (each wave has its own region), so the per-entry binary
-  // search and window lookup read LDS rather than global memory.
-  constexpr int kWaves = kBlock / kSellChunk;
-  __shared__ int32_t s_off[kWaves][kMaxDia];
-  __shared__ int32_t s_slot[kWaves][kMaxDia];
-  __shared__ int32_t s_win[kWaves][kSellWindows];
-  const size_t r = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  const bool active = r < chunks * kSellChunk;  // uniform per wavefront
-  const int wv = threadIdx.x / kSellChunk;
-  const size_t chunk = r / kSellChunk;
-  const int lane = static_cast<int>(r % kSellChunk);
-  const int nd = active ? dia_count[chunk] : 0;
-  if (active) {
-    for (int t = lane; t < nd; t += kSellChunk) {
-      s_off[wv][t] = dia_offsets[chunk * kMaxDia + t];
-      s_slot[wv][t] = dia_slot[chunk * kMaxDia + t];
-    }
-    if (lane < kSellWindows) s_win[wv][lane] = windows[chunk * kSellWindows + lane];
+__global__ void __launch_bounds__(kSellChunk) sell_fill_kernel(
+    size_t n, const int32_t* __restrict__ rows, const int32_t* __restrict__ cols,
+    const double* __restrict__ vals, const int32_t* __restrict__ dia_count,
+    const int32_t* __restrict__ dia_offsets, const int32_t* __restrict__ dia_slot,
+    const int32_t* __restrict__ dia_start, V* __restrict__ dia_vals,
+    const int32_t* __restrict__ chunk_start, const int32_t* __restrict__ width,
+    const int32_t* __restrict__ col_start, const int32_t* __restrict__ nwin,
+    const int32_t* __restrict__ windows, uint16_t* __restrict__ out_cols,
+    V* __restrict__ out_vals) {
+  __shared__ int32_t s_off[kMaxDia];
+  __shared__ int32_t s_slot[kMaxDia];
+  __shared__ int32_t s_win[kSellWindows];
+  __shared__ int32_t rs[kSellChunk + 1];
+  __shared__ int32_t rem_count[kSellChunk + 1];
+  const size_t chunk = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int32_t row0 = static_cast<int32_t>(chunk * kSellChunk);
+  const int nd = dia_count[chunk];
+  for (int t = lane; t < nd; t += kSellChunk) {
+    s_off[t] = dia_offsets[chunk * kMaxDia + t];
+    s_slot[t] = dia_slot[chunk * kMaxDia + t];
   }
+  if (lane < kSellWindows) s_win[lane] = windows[chunk * kSellWindows + lane];
+  load_row_starts(n, chunk, rows, rs, lane);
+  rem_count[lane] = 0;
   __syncthreads();
-  if (!active) return;
-  const int32_t* dl = s_off[wv];
   const int64_t dbase = dia_start[chunk];
   const int64_t vbase = chunk_start[chunk];
   const int64_t cbase = static_cast<int64_t>(col_start[chunk]) * kSellChunk;
   const int32_t w = width[chunk];
   const int32_t nw = nwin[chunk];
-  const int32_t* wb = s_win[wv];
-  const int32_t b = r < n ? rows[r] : 0;
-  const int32_t len = r < n ? rows[r + 1] - b : 0;
-  int32_t k = 0;  // next remainder slot
-  auto put = [&](int32_t c, V v) {
-    const int64_t slot = quad_slot(k, lane);
+  auto put = [&](int32_t k, int row, int32_t c, V v) {
+    const int64_t slot = quad_slot(k, row);
     out_vals[vbase + slot] = v;
     if (nw < 0) {
       reinterpret_cast<int32_t*>(out_cols + cbase)[slot] = c;
     } else {
       int j = 0;
-      while (j + 1 < nw && wb[j + 1] <= c) ++j;
-      out_cols[cbase + slot] = static_cast<uint16_t>((j << kSellWindowBits) | (c - wb[j]));
+      while (j + 1 < nw && s_win[j + 1] <= c) ++j;
+      out_cols[cbase + slot] = static_cast<uint16_t>((j << kSellWindowBits) | (c - s_win[j]));
     }
-    ++k;
   };
-  for (int32_t q = 0; q < len; ++q) {
-    const int32_t c = cols[b + q];
-    const int j = dia_find(dl, nd, c - static_cast<int32_t>(r));
-    if (j >= 0) {
-      dia_vals[dbase + quad_slot(s_slot[wv][j], lane)] = static_cast<V>(vals[b + q]);
-    } else {
-      put(c, static_cast<V>(vals[b + q]));
+  const int32_t e0 = rs[0], e1 = rs[kSellChunk];
+  int32_t c_next = e0 + lane < e1 ? cols[e0 + lane] : 0;
+  double v_next = e0 + lane < e1 ? vals[e0 + lane] : 0.0;
+  for (int32_t base = e0; base < e1; base += kSellChunk) {
+    const int32_t e = base + lane;
+    const bool in = e < e1;
+    const int32_t c = c_next;
+    const double v = v_next;
+    if (e + kSellChunk < e1) {  // next group's entry, in flight during this one
+      c_next = cols[e + kSellChunk];
+      v_next = vals[e + kSellChunk];
     }
+    const int lr = in ? chunk_row(rs, e) : kSellChunk;
+    const int j = in ? dia_find(s_off, nd, c - (row0 + lr)) : -1;
+    const bool rem = in && j < 0;
+    const SegmentRank sr = segment_rank(lane, lr, rem);
+    if (in && j >= 0) dia_vals[dbase + quad_slot(s_slot[j], lr)] = static_cast<V>(v);
+    if (rem) put(rem_count[lr] + sr.before, lr, c, static_cast<V>(v));
+    __syncthreads();  // every lane has read rem_count[lr]
+    if (in && sr.tail) rem_count[lr] += sr.before + (rem ? 1 : 0);
+    __syncthreads();
   }
-  while (k < w) put(wb[0], V(0));
+  for (int32_t k = rem_count[lane]; k < w; ++k) put(k, lane, s_win[0], V(0));
 }
 
 template <typename T, typename V>
@@ -1566,16 +1614,12 @@ void build_sell(Context& ctx, SellMatrix& M, const lsmps_csr& A_in, V* out_vals_
   M.windows.reserve(chunks * kSellWindows);
   M.rem_counter.reserve(1);
   fill_kernel<unsigned long long><<<1, kBlock, 0, ctx.stream>>>(1, M.rem_counter.data(), 0ull);
-  // diagonals and remainder widths
+  // diagonals, remainder widths and column windows
   dia_select_kernel<<<static_cast<unsigned>(chunks), kSellChunk, 0, ctx.stream>>>(
       n, A.row_offsets, A.col_indices, M.dia_count.data(), M.dia_offsets.data(),
       M.dia_slot.data(), M.dia_plan.data(), M.dia_pairs.data(), M.dia_singles.data(),
-      M.dia_slots.data(), M.chunk_width.data(), M.chunk_scaled.data(), M.rem_counter.data());
-  LSMPS_LAUNCH_CHECK();
-  const unsigned row_blocks = blocks_for(chunks * kSellChunk);
-  sell_window_kernel<<<row_blocks, kBlock, 0, ctx.stream>>>(
-      n, chunks, A.row_offsets, A.col_indices, M.dia_count.data(), M.dia_offsets.data(),
-      M.chunk_width.data(), M.nwin.data(), M.windows.data(), M.col_rows.data());
+      M.dia_slots.data(), M.chunk_width.data(), M.chunk_scaled.data(), M.rem_counter.data(),
+      M.nwin.data(), M.windows.data(), M.col_rows.data());
   LSMPS_LAUNCH_CHECK();
   exclusive_scan_i32(ctx, M.dia_slots.data(), M.dia_start.data(), chunks);
   exclusive_scan_i32(ctx, M.chunk_scaled.data(), M.chunk_start.data(), chunks);
@@ -1616,8 +1660,8 @@ void build_sell(Context& ctx, SellMatrix& M, const lsmps_csr& A_in, V* out_vals_
   }
   fill_kernel<V><<<blocks_for(std::max<size_t>(M.dia_entries, 1)), kBlock, 0, ctx.stream>>>(
       M.dia_entries, dia, V(0));
-  sell_fill_kernel<V><<<row_blocks, kBlock, 0, ctx.stream>>>(
-      n, chunks, A.row_offsets, A.col_indices, A.values, M.dia_count.data(),
+  sell_fill_kernel<V><<<static_cast<unsigned>(chunks), kSellChunk, 0, ctx.stream>>>(
+      n, A.row_offsets, A.col_indices, A.values, M.dia_count.data(),
       M.dia_offsets.data(), M.dia_slot.data(), M.dia_start.data(), dia, M.chunk_start.data(),
       M.chunk_width.data(), M.col_start.data(), M.nwin.data(), M.windows.data(), M.cols.data(),
       vals);
@@ -2070,19 +2114,37 @@ void ppe_solve(Context& ctx, PpeSolver& S, const lsmps_csr& A, const double* b, 
     S.schwarz.blocks = 0;
     S.coarse.ready = false;
     bool forked = false;  // the coarse setup runs on S.aux (joined before the loop)
+    std::thread coarse_thread;
+    std::exception_ptr coarse_error;
+    struct Joiner {
+      std::thread& t;
+      ~Joiner() {
+        if (t.joinable()) t.join();
+      }
+    } joiner{coarse_thread};
     if (opt.preconditioner == LSMPS_PRECOND_SCHWARZ ||
         opt.preconditioner == LSMPS_PRECOND_SCHWARZ_COARSE) {
       // The coarse space (it needs the caller's aggregation of these rows, one
       // system) is set up on a second stream, overlapping the Schwarz setup and
-      // the SpMV layout build; the IDR loop waits for it.
+      // the SpMV layout build; the IDR loop waits for it.  A second host thread
+      // enqueues it: the runtime holds an enqueueing thread back while many
+      // launches are outstanding on a stream, and the Gauss-Jordan chain has
+      // two launches per panel.
       const bool want_coarse = opt.preconditioner == LSMPS_PRECOND_SCHWARZ_COARSE &&
                                S.segments.count == 0 && S.coarse.count > 1 && S.coarse.n == n;
       if (want_coarse) {
         S.ensure_aux();
         LSMPS_HIP(hipEventRecord(S.ev_fork, ctx.stream));
         LSMPS_HIP(hipStreamWaitEvent(S.aux, S.ev_fork, 0));
-        coarse_setup(ctx, S.coarse, A, S.diag64.data(), S.offabs.data(), S.aux);
-        LSMPS_HIP(hipEventRecord(S.ev_join, S.aux));
+        coarse_thread = std::thread([&ctx, &S, &A, &coarse_error] {
+          try {
+            ctx.activate();
+            coarse_setup(ctx, S.coarse, A, S.diag64.data(), S.offabs.data(), S.aux);
+            LSMPS_HIP(hipEventRecord(S.ev_join, S.aux));
+          } catch (...) {
+            coarse_error = std::current_exception();
+          }
+        });
         forked = true;
       }
       schwarz_setup(ctx, S.schwarz, A, S.diag64.data(), S.dinv64.data(), S.offabs.data(),
@@ -2091,13 +2153,6 @@ void ppe_solve(Context& ctx, PpeSolver& S, const lsmps_csr& A, const double* b, 
       schwarz_stats(ctx, S.schwarz, &blocks, &rows);
       rep.schwarz_blocks = static_cast<int32_t>(blocks);
       rep.schwarz_rows = static_cast<int32_t>(std::min<int64_t>(rows, INT32_MAX));
-      if (want_coarse && S.coarse.ready) {
-        if (S.schwarz.n == n) {
-          coarse_set_weights(ctx, S.coarse, S.schwarz.pdiag.data());
-        } else {
-          S.coarse.ready = false;
-        }
-      }
     }
     const bool mixed = ctx.precision == LSMPS_PRECISION_MIXED;
     // mixed: FP32 matrix values in the Krylov SpMVs; FP64 basis, iterate,
@@ -2110,7 +2165,16 @@ void ppe_solve(Context& ctx, PpeSolver& S, const lsmps_csr& A, const double* b, 
     } else {
       build_sell<double>(ctx, S.sell, A, nullptr);
     }
-    if (forked) LSMPS_HIP(hipStreamWaitEvent(ctx.stream, S.ev_join, 0));
+    if (forked) {
+      coarse_thread.join();
+      if (coarse_error) std::rethrow_exception(coarse_error);
+      if (S.coarse.ready && S.schwarz.n == n) {
+        coarse_set_weights(ctx, S.coarse, S.schwarz.pdiag.data());
+      } else {
+        S.coarse.ready = false;
+      }
+      LSMPS_HIP(hipStreamWaitEvent(ctx.stream, S.ev_join, 0));
+    }
     if (S.segments.count > 0) {
       // independent systems stacked in one matrix: per-segment recurrences
       if (!S.seg_host || S.seg_host_count < static_cast<size_t>(S.segments.count)) {
