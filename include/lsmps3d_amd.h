@@ -258,7 +258,8 @@ typedef struct lsmps_solver_options {
                               between replacements by the true FP64 residual (1e-3) */
   int32_t check_interval;  /* PCG: iterations between host convergence checks */
   int32_t clamp_negative;  /* clamp p<0 after the solve, as ppe_matrix.cu:662 (default 1) */
-  int32_t shadow_dimension; /* IDR(s): s, 1..16 (default 16) */
+  int32_t shadow_dimension; /* IDR(s): s, 1..16; 0 (default): 12 with the coarse
+                               space, 16 without */
   int32_t preconditioner;   /* LSMPS_PRECOND_* (default SCHWARZ_COARSE) */
   int32_t coarse_cells;     /* SCHWARZ_COARSE: aggregates span coarse_cells^3 cells (default 4) */
 } lsmps_solver_options;
@@ -411,7 +412,7 @@ int lsmps_ppe_assemble(lsmps_context* ctx, const lsmps_config* config,
 /* AmgxPpeSolver::solve(const CsrMatrixView&, const real* rhs, real* pressure) —
  * ppe_matrix.cuh:112.  Hand-written IDR(s) / BiCGStab / PCG from p = 0, then the
  * reference's clamp p = max(p, 0) (ppe_matrix.cu:662).  options may be NULL
- * (lsmps_default_solver_options: IDR(16), Schwarz preconditioner, relative true
+ * (lsmps_default_solver_options: IDR(s), Schwarz preconditioner, relative true
  * residual 1e-10); report may be NULL.  Returns LSMPS_ERR_NOT_CONVERGED (with the
  * iterate written and the report filled) when max_iterations is reached. */
 int lsmps_ppe_solve(lsmps_context* ctx, const lsmps_csr* matrix, const double* rhs,

@@ -11,7 +11,7 @@
 //  * all calls run on one process-wide context (device and precision chosen by
 //    set_default_device / set_precision before first use) and synchronise its
 //    stream before returning, like the reference's legacy-default-stream calls;
-//  * AmgxPpeSolver is the library's own Krylov solver (IDR(16) with Schwarz +
+//  * AmgxPpeSolver is the library's own Krylov solver (IDR(s) with Schwarz +
 //    Jacobi preconditioning to a 1e-10 relative true residual by default); the
 //    AMGX config path is accepted and ignored;
 //  * the moment-view free functions (assemble_ppe_matrix_and_rhs,
