@@ -282,7 +282,7 @@ def main():
                                 "C3": "64 batched tanks, 2k-60k fluid each, one launch per stage",
                                 "C5": "splash 160^3 (4,096,000 fluid), top 8 layers lifted"}[args.case],
                    "n_fluid": case.n_fluid, "n_wall": case.n_wall, "l0_m": case.config.particle_spacing,
-                   "re_over_l0": 3.1, "dt_s": case.config.time_step, "solver": f"IDR({args.shadow_dimension or (16 if args.coarse_cells < 0 else 12)}); Schwarz blocks on wrong-sign rows, Jacobi elsewhere" + ("" if args.coarse_cells < 0 else f", coarse space of {args.coarse_cells or pkg.abi.default_solver_options().coarse_cells}^3-cell aggregates"),
+                   "re_over_l0": 3.1, "dt_s": case.config.time_step, "solver": f"IDR({args.shadow_dimension or (16 if args.coarse_cells < 0 else 14)}); Schwarz blocks on wrong-sign rows, Jacobi elsewhere" + ("" if args.coarse_cells < 0 else f", coarse space of {args.coarse_cells or pkg.abi.default_solver_options().coarse_cells}^3-cell aggregates"),
                    "tolerance": 1e-10, "precision": args.precision},
         "solver_iterations": iters,
         "schwarz_blocks": main_run["reports"][-1]["schwarz_blocks"],

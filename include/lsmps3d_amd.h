@@ -258,7 +258,7 @@ typedef struct lsmps_solver_options {
                               between replacements by the true FP64 residual (1e-3) */
   int32_t check_interval;  /* PCG: iterations between host convergence checks */
   int32_t clamp_negative;  /* clamp p<0 after the solve, as ppe_matrix.cu:662 (default 1) */
-  int32_t shadow_dimension; /* IDR(s): s, 1..16; 0 (default): 12 with the coarse
+  int32_t shadow_dimension; /* IDR(s): s, 1..16; 0 (default): 14 with the coarse
                                space, 16 without */
   int32_t preconditioner;   /* LSMPS_PRECOND_* (default SCHWARZ_COARSE) */
   int32_t coarse_cells;     /* SCHWARZ_COARSE: aggregates span coarse_cells^3 cells (default 4) */

@@ -251,7 +251,7 @@ def default_solver_options():
     o.inner_tolerance = 1.0e-3
     o.check_interval = 16
     o.clamp_negative = 1
-    o.shadow_dimension = 0  # 12 with the coarse space, 16 without
+    o.shadow_dimension = 0  # 14 with the coarse space, 16 without
     o.preconditioner = PRECOND_SCHWARZ_COARSE
     o.coarse_cells = 4
     return o

@@ -436,7 +436,7 @@ int lsmps_default_solver_options(lsmps_solver_options* out) {
   o.inner_tolerance = 1.0e-3;
   o.check_interval = 16;
   o.clamp_negative = 1;
-  o.shadow_dimension = 0;  // 12 with the coarse space, 16 without
+  o.shadow_dimension = 0;  // 14 with the coarse space, 16 without
   o.preconditioner = LSMPS_PRECOND_SCHWARZ_COARSE;
   o.coarse_cells = 4;
   *out = o;

@@ -53,10 +53,25 @@ struct GradArgs {
 };
 
 // compute_pressure_gradient_kernel (pressure_correction.cu:200-276)
+// The neighbour-sum kernels give each particle kCorrLanes adjacent lanes: lane
+// l takes neighbours l, l + kCorrLanes, ... (contiguous index loads per
+// wavefront, a particle's gathers in flight together), and the sums meet in a
+// fixed xor butterfly before lane 0 finishes the particle.
+constexpr int kCorrLanes = 8;
+
+__device__ __forceinline__ double group_sum(double v) {
+#pragma unroll
+  for (int off = kCorrLanes / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
 __global__ void __launch_bounds__(kBlock) gradient_kernel(GradArgs a) {
-  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= a.n) return;
+  const size_t t = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const size_t i = t / kCorrLanes;
+  const int sub = static_cast<int>(t % kCorrLanes);
+  if (i >= a.n) return;  // whole groups
   if (is_splash(a.st, i)) {
+    if (sub != 0) return;
     a.gx[i] = 0.0;
     a.gy[i] = 0.0;
     a.gz[i] = 0.0;
@@ -68,7 +83,7 @@ __global__ void __launch_bounds__(kBlock) gradient_kernel(GradArgs a) {
 #pragma unroll
   for (int k = 0; k < 9; ++k) rhs[k] = 0.0;
   double b[9];
-  for (int32_t c = a.f_off[i]; c < a.f_off[i + 1]; ++c) {
+  for (int32_t c = a.f_off[i] + sub; c < a.f_off[i + 1]; c += kCorrLanes) {
     const int32_t j = a.f_idx[c];
     const double dx = a.x[j] - xi, dy = a.y[j] - yi, dz = a.z[j] - zi;
     const double w = correction_weight_of(sqrt(dx * dx + dy * dy + dz * dz), a.re, a.kernel);
@@ -78,7 +93,7 @@ __global__ void __launch_bounds__(kBlock) gradient_kernel(GradArgs a) {
 #pragma unroll
     for (int k = 0; k < 9; ++k) rhs[k] += w * b[k] * dp;
   }
-  for (int32_t c = a.w_off[i]; c < a.w_off[i + 1]; ++c) {
+  for (int32_t c = a.w_off[i] + sub; c < a.w_off[i + 1]; c += kCorrLanes) {
     const int32_t j = a.w_idx[c];
     const double dx = a.wx[j] - xi, dy = a.wy[j] - yi, dz = a.wz[j] - zi;
     const double w =
@@ -91,6 +106,9 @@ __global__ void __launch_bounds__(kBlock) gradient_kernel(GradArgs a) {
 #pragma unroll
     for (int k = 0; k < 9; ++k) rhs[k] += w * b[k] * wall_rhs;
   }
+#pragma unroll
+  for (int k = 0; k < 9; ++k) rhs[k] = group_sum(rhs[k]);
+  if (sub != 0) return;
   double r0 = 0.0, r1 = 0.0, r2 = 0.0;
 #pragma unroll
   for (int k = 0; k < 9; ++k) {
@@ -129,10 +147,13 @@ struct ShiftArgs {
 
 // compute_particle_shifting_kernel (pressure_correction.cu:278-367)
 __global__ void __launch_bounds__(kBlock) shifting_kernel(ShiftArgs a) {
-  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= a.n) return;
+  const size_t t = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const size_t i = t / kCorrLanes;
+  const int sub = static_cast<int>(t % kCorrLanes);
+  if (i >= a.n) return;  // whole groups
   const int32_t st = a.st ? a.st[i] : LSMPS_SURFACE_INNER;
   if (a.scale <= 0.0 || st == LSMPS_SURFACE_SPLASH) {
+    if (sub != 0) return;
     a.sx[i] = 0.0;
     a.sy[i] = 0.0;
     a.sz[i] = 0.0;
@@ -140,7 +161,7 @@ __global__ void __launch_bounds__(kBlock) shifting_kernel(ShiftArgs a) {
   }
   const double xi = a.x[i], yi = a.y[i], zi = a.z[i];
   double shx = 0.0, shy = 0.0, shz = 0.0, nx = 0.0, ny = 0.0, nz = 0.0;
-  for (int32_t c = a.f_off[i]; c < a.f_off[i + 1]; ++c) {
+  for (int32_t c = a.f_off[i] + sub; c < a.f_off[i + 1]; c += kCorrLanes) {
     const int32_t j = a.f_idx[c];
     const double dx = xi - a.x[j], dy = yi - a.y[j], dz = zi - a.z[j];
     const double d = sqrt(dx * dx + dy * dy + dz * dz);
@@ -153,7 +174,7 @@ __global__ void __launch_bounds__(kBlock) shifting_kernel(ShiftArgs a) {
     ny += dy / d;
     nz += dz / d;
   }
-  for (int32_t c = a.w_off[i]; c < a.w_off[i + 1]; ++c) {
+  for (int32_t c = a.w_off[i] + sub; c < a.w_off[i + 1]; c += kCorrLanes) {
     const int32_t j = a.w_idx[c];
     const double dx = xi - a.wx[j], dy = yi - a.wy[j], dz = zi - a.wz[j];
     const double d = sqrt(dx * dx + dy * dy + dz * dz);
@@ -163,6 +184,13 @@ __global__ void __launch_bounds__(kBlock) shifting_kernel(ShiftArgs a) {
     shy += push * a.wny[j];
     shz += push * a.wnz[j];
   }
+  shx = group_sum(shx);
+  shy = group_sum(shy);
+  shz = group_sum(shz);
+  nx = group_sum(nx);
+  ny = group_sum(ny);
+  nz = group_sum(nz);
+  if (sub != 0) return;
   if (st == LSMPS_SURFACE_SURFACE || st == LSMPS_SURFACE_SPLASH) {
     const double nn = sqrt(nx * nx + ny * ny + nz * nz);
     if (nn > 0.0) {
@@ -255,9 +283,12 @@ __global__ void __launch_bounds__(kBlock) smooth_kernel(
     const double* vy, const double* vz, const int32_t* st, const int32_t* f_off,
     const int32_t* f_idx, double re, double strength, int32_t kernel, double* ox, double* oy,
     double* oz) {
-  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  const size_t t = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const size_t i = t / kCorrLanes;
+  const int sub = static_cast<int>(t % kCorrLanes);
+  if (i >= n) return;  // whole groups
   if (strength <= 0.0 || is_splash(st, i)) {
+    if (sub != 0) return;
     ox[i] = vx[i];
     oy[i] = vy[i];
     oz[i] = vz[i];
@@ -265,7 +296,7 @@ __global__ void __launch_bounds__(kBlock) smooth_kernel(
   }
   double ws = 0.0, ax = 0.0, ay = 0.0, az = 0.0;
   const double xi = x[i], yi = y[i], zi = z[i];
-  for (int32_t c = f_off[i]; c < f_off[i + 1]; ++c) {
+  for (int32_t c = f_off[i] + sub; c < f_off[i + 1]; c += kCorrLanes) {
     const int32_t j = f_idx[c];
     if (is_splash(st, j)) continue;
     const double dx = x[j] - xi, dy = y[j] - yi, dz = z[j] - zi;
@@ -276,6 +307,11 @@ __global__ void __launch_bounds__(kBlock) smooth_kernel(
     ay += w * vy[j];
     az += w * vz[j];
   }
+  ws = group_sum(ws);
+  ax = group_sum(ax);
+  ay = group_sum(ay);
+  az = group_sum(az);
+  if (sub != 0) return;
   if (ws <= 0.0) {
     ox[i] = vx[i];
     oy[i] = vy[i];
@@ -355,7 +391,7 @@ void pressure_gradient(Context& ctx, const lsmps_config& cfg, const lsmps_fluid_
   a.gx = grad.x;
   a.gy = grad.y;
   a.gz = grad.z;
-  gradient_kernel<<<blocks_for(fluid.count), kBlock, 0, ctx.stream>>>(a);
+  gradient_kernel<<<blocks_for(fluid.count * kCorrLanes), kBlock, 0, ctx.stream>>>(a);
   LSMPS_LAUNCH_CHECK();
 }
 
@@ -389,7 +425,7 @@ void particle_shifting(Context& ctx, const lsmps_config& cfg, const lsmps_fluid_
   a.sx = ps.x;
   a.sy = ps.y;
   a.sz = ps.z;
-  shifting_kernel<<<blocks_for(fluid.count), kBlock, 0, ctx.stream>>>(a);
+  shifting_kernel<<<blocks_for(fluid.count * kCorrLanes), kBlock, 0, ctx.stream>>>(a);
   LSMPS_LAUNCH_CHECK();
 }
 
@@ -427,7 +463,7 @@ void smooth_velocity(Context& ctx, const lsmps_config& cfg, const lsmps_fluid_so
           "Pressure correction requires a valid fluid-neighbor CSR table");
   require_vector(fluid, smoothed, "Velocity smoothing");
   if (fluid.count == 0) return;
-  smooth_kernel<<<blocks_for(fluid.count), kBlock, 0, ctx.stream>>>(
+  smooth_kernel<<<blocks_for(fluid.count * kCorrLanes), kBlock, 0, ctx.stream>>>(
       fluid.count, fluid.x, fluid.y, fluid.z, fluid.vx, fluid.vy, fluid.vz, fluid.surface_type,
       fn.offsets, fn.indices, cfg.support_radius, cfg.velocity_smoothing_strength,
       cfg.weight_kernel, smoothed.x, smoothed.y, smoothed.z);

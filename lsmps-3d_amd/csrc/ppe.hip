@@ -59,23 +59,39 @@ __global__ void row_offsets_kernel(size_t n, const int32_t* f_off, int32_t* row_
   row_offsets[i] = static_cast<int32_t>(i) + f_off[i];
 }
 
+// kAsmLanes adjacent lanes per row: lane l of a row's group takes neighbours
+// l, l + kAsmLanes, ..., so the index loads and the CSR writes of a wavefront
+// are contiguous runs and the neighbour gathers of a row are in flight
+// together; the per-row sums meet in a fixed xor butterfly.
+constexpr int kAsmLanes = 8;
+
+__device__ __forceinline__ double group_sum(double v) {
+#pragma unroll
+  for (int off = kAsmLanes / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
 __global__ void __launch_bounds__(kBlock) assemble_kernel(PpeArgs a) {
-  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= a.n) return;
+  const size_t t = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const size_t i = t / kAsmLanes;
+  const int sub = static_cast<int>(t % kAsmLanes);
+  if (i >= a.n) return;  // whole groups
   const int32_t row_begin = static_cast<int32_t>(i) + a.f_off[i];
   const int32_t nb = a.f_off[i], ne = a.f_off[i + 1];
   const int32_t st = a.surface_type ? a.surface_type[i] : LSMPS_SURFACE_INNER;
   if (st == LSMPS_SURFACE_SURFACE || st == LSMPS_SURFACE_SPLASH) {
-    a.cols[row_begin] = static_cast<int32_t>(i);
-    a.vals[row_begin] = 1.0;
-    for (int32_t c = nb; c < ne; ++c) {
+    for (int32_t c = nb + sub; c < ne; c += kAsmLanes) {
       const int32_t out = row_begin + 1 + c - nb;
       a.cols[out] = a.f_idx[c];
       a.vals[out] = 0.0;
     }
-    a.rhs[i] = 0.0;
-    a.divergence[i] = 0.0;
-    if (a.wall_term) a.wall_term[i] = 0.0;
+    if (sub == 0) {
+      a.cols[row_begin] = static_cast<int32_t>(i);
+      a.vals[row_begin] = 1.0;
+      a.rhs[i] = 0.0;
+      a.divergence[i] = 0.0;
+      if (a.wall_term) a.wall_term[i] = 0.0;
+    }
     return;
   }
   double L[9];
@@ -94,7 +110,7 @@ __global__ void __launch_bounds__(kBlock) assemble_kernel(PpeArgs a) {
   }
   double diagonal = 0.0;
   double b[9];
-  for (int32_t c = nb; c < ne; ++c) {
+  for (int32_t c = nb + sub; c < ne; c += kAsmLanes) {
     const int32_t j = a.f_idx[c];
     const double dx = a.x[j] - xi, dy = a.y[j] - yi, dz = a.z[j] - zi;
     const double w = weight_of(sqrt(dx * dx + dy * dy + dz * dz), a.re, a.kernel);
@@ -121,7 +137,7 @@ __global__ void __launch_bounds__(kBlock) assemble_kernel(PpeArgs a) {
   }
   double wall_rhs = 0.0;
   double q[9];
-  for (int32_t c = a.w_off[i]; c < a.w_off[i + 1]; ++c) {
+  for (int32_t c = a.w_off[i] + sub; c < a.w_off[i + 1]; c += kAsmLanes) {
     const int32_t j = a.w_idx[c];
     const double dx = a.wx[j] - xi, dy = a.wy[j] - yi, dz = a.wz[j] - zi;
     const double bw = weight_of(sqrt(dx * dx + dy * dy + dz * dz), a.re, a.kernel);
@@ -144,6 +160,15 @@ __global__ void __launch_bounds__(kBlock) assemble_kernel(PpeArgs a) {
     for (int k = 0; k < 9; ++k) lq += L[k] * q[k];
     wall_rhs += 2.0 * a.wall_scale * bw * ng * lq / a.re;
   }
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    rx[k] = group_sum(rx[k]);
+    ry[k] = group_sum(ry[k]);
+    rz[k] = group_sum(rz[k]);
+  }
+  diagonal = group_sum(diagonal);
+  wall_rhs = group_sum(wall_rhs);
+  if (sub != 0) return;
   // divergence from velocity rows 0..2 (first_coefficient_from_inverse_rhs, ppe_matrix.cu:233-241)
   double d0 = 0.0, d1 = 0.0, d2 = 0.0;
 #pragma unroll
@@ -220,7 +245,7 @@ void ppe_assemble(Context& ctx, const lsmps_config& cfg, const lsmps_fluid_soa& 
   a.rhs = ws.rhs;
   a.divergence = ws.divergence;
   a.wall_term = ws.pressure_laplacian;
-  assemble_kernel<<<blocks_for(fluid.count), kBlock, 0, ctx.stream>>>(a);
+  assemble_kernel<<<blocks_for(fluid.count * kAsmLanes), kBlock, 0, ctx.stream>>>(a);
   LSMPS_LAUNCH_CHECK();
 }
 

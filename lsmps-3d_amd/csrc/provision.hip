@@ -48,9 +48,22 @@ struct ProvisionArgs {
   double* lap_z;
 };
 
+// kProvLanes adjacent lanes per particle: lane l takes neighbours l,
+// l + kProvLanes, ... (contiguous index loads per wavefront, a particle's
+// gathers in flight together); the sums meet in a fixed xor butterfly.
+constexpr int kProvLanes = 8;
+
+__device__ __forceinline__ double group_sum(double v) {
+#pragma unroll
+  for (int off = kProvLanes / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
 __global__ void __launch_bounds__(kBlock) provision_kernel(ProvisionArgs a) {
-  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= a.n) return;
+  const size_t t = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const size_t i = t / kProvLanes;
+  const int sub = static_cast<int>(t % kProvLanes);
+  if (i >= a.n) return;  // whole groups
   double lap_row[9];
 #pragma unroll
   for (int k = 0; k < 9; ++k) lap_row[k] = a.ops[static_cast<size_t>(kOpLap + k) * a.n + i];
@@ -61,7 +74,7 @@ __global__ void __launch_bounds__(kBlock) provision_kernel(ProvisionArgs a) {
   // Laplacian row on the fly: lap = 2 (L . rhs) / re^2 and L . rhs = sum_j w (L . b_j) dv_j.
   double sx = 0.0, sy = 0.0, sz = 0.0;
   double b[9];
-  for (int32_t c = a.f_off[i]; c < a.f_off[i + 1]; ++c) {
+  for (int32_t c = a.f_off[i] + sub; c < a.f_off[i + 1]; c += kProvLanes) {
     const int32_t j = a.f_idx[c];
     const double dx = a.x[j] - xi, dy = a.y[j] - yi, dz = a.z[j] - zi;
     const double w = weight_of(sqrt(dx * dx + dy * dy + dz * dz), a.re, a.kernel);
@@ -75,7 +88,7 @@ __global__ void __launch_bounds__(kBlock) provision_kernel(ProvisionArgs a) {
     sy += wl * (a.vy[j] - vyi);
     sz += wl * (a.vz[j] - vzi);
   }
-  for (int32_t c = a.w_off[i]; c < a.w_off[i + 1]; ++c) {
+  for (int32_t c = a.w_off[i] + sub; c < a.w_off[i + 1]; c += kProvLanes) {
     const int32_t j = a.w_idx[c];
     const double dx = a.wx[j] - xi, dy = a.wy[j] - yi, dz = a.wz[j] - zi;
     const double w = a.wall_scale * weight_of(sqrt(dx * dx + dy * dy + dz * dz), a.re, a.kernel);
@@ -89,6 +102,10 @@ __global__ void __launch_bounds__(kBlock) provision_kernel(ProvisionArgs a) {
     sy += wl * (a.wvy[j] - vyi);
     sz += wl * (a.wvz[j] - vzi);
   }
+  sx = group_sum(sx);
+  sy = group_sum(sy);
+  sz = group_sum(sz);
+  if (sub != 0) return;
   // laplacian_from_inverse_rhs (explicit_update.cu:164-176)
   const double scale = 2.0 / (a.re * a.re);
   const double lx = scale * sx, ly = scale * sy, lz = scale * sz;
@@ -173,7 +190,7 @@ void provision_fluid(Context& ctx, const lsmps_config& cfg, const lsmps_fluid_so
   a.lap_x = lap ? lap->x : nullptr;
   a.lap_y = lap ? lap->y : nullptr;
   a.lap_z = lap ? lap->z : nullptr;
-  provision_kernel<<<blocks_for(fluid.count), kBlock, 0, ctx.stream>>>(a);
+  provision_kernel<<<blocks_for(fluid.count * kProvLanes), kBlock, 0, ctx.stream>>>(a);
   LSMPS_LAUNCH_CHECK();
 }
 

@@ -2094,12 +2094,12 @@ void ppe_solve(Context& ctx, PpeSolver& S, const lsmps_csr& A, const double* b, 
   const int max_it = opt.max_iterations > 0 ? opt.max_iterations : 20000;
   const size_t ld = (n + 63) / 64 * 64;
   int shadow_dim = opt.method == LSMPS_SOLVER_BICGSTAB ? 1 : opt.shadow_dimension;
-  // s = 0 picks 12 when the coarse space is requested and 16 otherwise: with
-  // the smooth modes removed by the coarse term a larger shadow space saves
-  // fewer iterations than its extra vector traffic per step costs.
+  // s = 0 picks 14 when the coarse space is requested and 16 otherwise: with
+  // the smooth modes removed by the coarse term, s = 12..16 cost about the same
+  // (fewer iterations against more vector traffic per step).
   const bool coarse_requested = opt.preconditioner == LSMPS_PRECOND_SCHWARZ_COARSE &&
                                 S.segments.count == 0 && S.coarse.count > 1 && S.coarse.n == n;
-  if (shadow_dim <= 0) shadow_dim = coarse_requested ? 12 : 16;
+  if (shadow_dim <= 0) shadow_dim = coarse_requested ? 14 : 16;
   if (shadow_dim > kMaxShadow) shadow_dim = kMaxShadow;
   if (static_cast<size_t>(shadow_dim) > n) shadow_dim = static_cast<int>(n);
 

@@ -1155,11 +1155,11 @@ int ppe_solve(const lsmps_csr& a, const double* b, double* x, const lsmps_solver
   if (opt.method == LSMPS_SOLVER_IDRS) {
     // IDR(s) with biorthogonalisation (Sonneveld & van Gijzen 2008; van Gijzen &
     // Sonneveld 2011).  Shadow space: P_0 = r_0, P_q (q >= 1) = hashed signs.
-    // s = 0: 12 when the coarse space is requested, 16 otherwise
+    // s = 0: 14 when the coarse space is requested, 16 otherwise
     const bool coarse_requested = opt.preconditioner == LSMPS_PRECOND_SCHWARZ_COARSE &&
                                   g_coarse.count > 1 &&
                                   static_cast<int64_t>(g_coarse.agg.size()) == n;
-    int s = opt.shadow_dimension > 0 ? opt.shadow_dimension : (coarse_requested ? 12 : 16);
+    int s = opt.shadow_dimension > 0 ? opt.shadow_dimension : (coarse_requested ? 14 : 16);
     if (s > 16) s = 16;
     if (s > n) s = static_cast<int>(n);
     auto sign = [](uint32_t row, int q) {
