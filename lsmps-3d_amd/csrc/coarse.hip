@@ -462,22 +462,32 @@ __global__ void __launch_bounds__(kBlock) restrict_kernel(int count, const int32
   __shared__ double red[kWaves];
   const int I = blockIdx.x;
   const int32_t mb = agg_start[I], me = agg_start[I + 1];
+  // up to kInFlight members per thread in flight (an aggregate has ~1400 rows,
+  // so usually one round), summed in member order
+  constexpr int kInFlight = 8;
   double s = 0.0;
-  int32_t m = mb + threadIdx.x;
-  for (; m + kBlock < me; m += 2 * kBlock) {
-    const int32_t i0 = agg_rows[m], i1 = agg_rows[m + kBlock];
-    const double w0 = w[i0], w1 = w[i1];
-    if (wfac) {
-      s += w0 * wfac[i0];
-      s += w1 * wfac[i1];
-    } else {
-      s += w0;
-      s += w1;
+  for (int32_t m = mb + threadIdx.x; m < me; m += kInFlight * kBlock) {
+    int32_t idx[kInFlight];
+#pragma unroll
+    for (int u = 0; u < kInFlight; ++u) {
+      const int32_t mm = m + u * kBlock;
+      idx[u] = mm < me ? agg_rows[mm] : -1;
     }
-  }
-  for (; m < me; m += kBlock) {
-    const int32_t i = agg_rows[m];
-    s += wfac ? w[i] * wfac[i] : w[i];
+    double v[kInFlight], f[kInFlight];
+#pragma unroll
+    for (int u = 0; u < kInFlight; ++u) {
+      v[u] = idx[u] >= 0 ? w[idx[u]] : 0.0;
+      f[u] = idx[u] >= 0 && wfac ? wfac[idx[u]] : 1.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kInFlight; ++u) {
+      if (idx[u] < 0) continue;
+      if (wfac) {
+        s += v[u] * f[u];
+      } else {
+        s += v[u];
+      }
+    }
   }
   s = block_sum(s, red);
   if (threadIdx.x == 0) rc[I] = s;
