@@ -170,21 +170,31 @@ __device__ __forceinline__ void sym_matvec(const double m[Packed<N>::kSize], con
 
 template <int N>
 __device__ __forceinline__ void rhs_vector(int which, double e[N]) {
+  // selects, not e[which]: `which` may differ between lanes
 #pragma unroll
-  for (int k = 0; k < N; ++k) e[k] = 0.0;
-  if (which < 3) {
-    e[which] = 1.0;
-  } else {
-    e[3] = 1.0;
-    e[4] = 1.0;
-    e[5] = 1.0;
-  }
+  for (int k = 0; k < N; ++k)
+    e[k] = which < 3 ? (k == which ? 1.0 : 0.0) : (k >= 3 && k <= 5 ? 1.0 : 0.0);
+}
+
+// kMomLanes adjacent lanes per particle: lane l accumulates neighbours l,
+// l + kMomLanes, ... (contiguous index loads per wavefront, a particle's gathers
+// in flight together); the moment sums meet in a xor butterfly, which leaves
+// the same total in every lane of the group (each pairwise sum is commutative),
+// so every lane factors the same matrix and the lanes share the solves.
+constexpr int kMomLanes = 4;
+
+__device__ __forceinline__ double group_sum(double v) {
+#pragma unroll
+  for (int off = kMomLanes / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
 }
 
 template <int N, bool MIXED>
 __global__ void __launch_bounds__(kBlock) moment_kernel(MomentArgs a) {
-  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= a.n) return;
+  const size_t t = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const size_t i = t / kMomLanes;
+  const int sub = static_cast<int>(t % kMomLanes);
+  if (i >= a.n) return;  // whole groups
   constexpr int P = Packed<N>::kSize;
   double m[P];
 #pragma unroll
@@ -193,7 +203,7 @@ __global__ void __launch_bounds__(kBlock) moment_kernel(MomentArgs a) {
   const double inv_re = 1.0 / a.re;
   double v[N];
   // fluid samples: add_sample_to_matrix (moment_matrix.cu:211-229)
-  for (int32_t c = a.f_off[i]; c < a.f_off[i + 1]; ++c) {
+  for (int32_t c = a.f_off[i] + sub; c < a.f_off[i + 1]; c += kMomLanes) {
     const int32_t j = a.f_idx[c];
     const double dx = a.x[j] - xi, dy = a.y[j] - yi, dz = a.z[j] - zi;
     const double d2 = dx * dx + dy * dy + dz * dz;
@@ -204,7 +214,7 @@ __global__ void __launch_bounds__(kBlock) moment_kernel(MomentArgs a) {
     add_outer<N>(m, v, w);
   }
   if (a.boundary != 0) {
-    for (int32_t c = a.w_off[i]; c < a.w_off[i + 1]; ++c) {
+    for (int32_t c = a.w_off[i] + sub; c < a.w_off[i + 1]; c += kMomLanes) {
       const int32_t j = a.w_idx[c];
       const double dx = a.wx[j] - xi, dy = a.wy[j] - yi, dz = a.wz[j] - zi;
       const double d2 = dx * dx + dy * dy + dz * dz;
@@ -220,15 +230,18 @@ __global__ void __launch_bounds__(kBlock) moment_kernel(MomentArgs a) {
       add_outer<N>(m, v, w);
     }
   }
+#pragma unroll
+  for (int k = 0; k < P; ++k) m[k] = group_sum(m[k]);
+  const bool lead = sub == 0;
   double trace = 0.0;
 #pragma unroll
   for (int r = 0; r < N; ++r) trace += m[Packed<N>::at(r, r)];
   const double reg = a.lambda * (trace > 0.0 ? trace / static_cast<double>(N) : 1.0);
 #pragma unroll
   for (int r = 0; r < N; ++r) m[Packed<N>::at(r, r)] += reg;
-  if (a.trace) a.trace[i] = trace;
-  if (a.regularization) a.regularization[i] = reg;
-  if (a.matrices) {
+  if (lead && a.trace) a.trace[i] = trace;
+  if (lead && a.regularization) a.regularization[i] = reg;
+  if (lead && a.matrices) {
     double* out = a.matrices + i * N * N;
     for (int r = 0; r < N; ++r)
       for (int c = 0; c < N; ++c) out[r + c * N] = c <= r ? m[Packed<N>::at(r, c)] : m[Packed<N>::at(c, r)];
@@ -246,7 +259,7 @@ __global__ void __launch_bounds__(kBlock) moment_kernel(MomentArgs a) {
     fp32_factor = cholesky<N, float>(lf) == 0;
   }
   if (!fp32_factor) info = cholesky<N, double>(m);
-  if (a.info) a.info[i] = info;
+  if (lead && a.info) a.info[i] = info;
   const double nan = __longlong_as_double(0x7ff8000000000000LL);
 
   auto solve = [&](double e[N], double y[N]) {
@@ -275,20 +288,19 @@ __global__ void __launch_bounds__(kBlock) moment_kernel(MomentArgs a) {
     }
   };
 
+  static_assert(kMomLanes == 4, "one operator row per lane");
   if (a.ops) {
-    // N == 9 here (Type-A kinds only feed the operators)
+    // N == 9 here (Type-A kinds only feed the operators); lane `sub` solves row `sub`
+    const int which = sub;
+    double e[N], y[N];
+    rhs_vector<N>(which, e);
+    solve(e, y);
 #pragma unroll
-    for (int which = 0; which < 4; ++which) {
-      double e[N], y[N];
-      rhs_vector<N>(which, e);
-      solve(e, y);
-#pragma unroll
-      for (int k = 0; k < N; ++k) a.ops[static_cast<size_t>(which * 9 + k) * a.n + i] = y[k];
-    }
+    for (int k = 0; k < N; ++k) a.ops[static_cast<size_t>(which * 9 + k) * a.n + i] = y[k];
   }
   if (a.inverse) {
     double* out = a.inverse + i * N * N;
-    for (int col = 0; col < N; ++col) {
+    for (int col = sub; col < N; col += kMomLanes) {
       double e[N], y[N];
 #pragma unroll
       for (int k = 0; k < N; ++k) e[k] = (k == col) ? 1.0 : 0.0;
@@ -495,7 +507,7 @@ void compute_moments(Context& ctx, int kind, const lsmps_config& cfg, const lsmp
   a.trace = out.trace;
   a.regularization = out.regularization;
   a.info = out.info;
-  const unsigned grid = blocks_for(fluid.count);
+  const unsigned grid = blocks_for(fluid.count * kMomLanes);
   if (size == 10) {
     if (a.mixed) moment_kernel<10, true><<<grid, kBlock, 0, ctx.stream>>>(a);
     else moment_kernel<10, false><<<grid, kBlock, 0, ctx.stream>>>(a);
