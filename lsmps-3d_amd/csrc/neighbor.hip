@@ -247,6 +247,10 @@ __global__ void gather_points_kernel(size_t n, const int32_t* __restrict__ sorte
 // count_neighbors_kernel / write_neighbors_kernel (neighbor_search.cu:221-253) with
 // count/write_source_neighbors_for_fluid (:100-219) inlined.  Row t of the launch
 // handles fluid particle order[t] (cell order) so neighboring rows share cells.
+// kScanLanes adjacent lanes per particle (a power of two dividing 64).
+constexpr int kScanLanes = 8;
+constexpr uint32_t kScanMask = (1u << kScanLanes) - 1u;
+
 template <bool kWrite, bool kSeg>
 __global__ void __launch_bounds__(kBlock) neighbor_scan_kernel(
     size_t n_fluid, const int32_t* __restrict__ order, const double* __restrict__ fx,
@@ -255,8 +259,11 @@ __global__ void __launch_bounds__(kBlock) neighbor_scan_kernel(
     const int32_t* __restrict__ src_begin, const int32_t* __restrict__ src_end,
     const SortedPoint* __restrict__ src, double radius_squared, int exclude_self,
     int32_t* __restrict__ offsets, int32_t* __restrict__ indices) {
-  const size_t t = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (t >= n_fluid) return;
+  const size_t gt = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const size_t t = gt / kScanLanes;
+  const int sub = static_cast<int>(gt % kScanLanes);
+  if (t >= n_fluid) return;  // whole groups
+  const int group_lane = static_cast<int>(threadIdx.x & 63) - sub;  // first lane of the group
   const int32_t i = order ? order[t] : static_cast<int32_t>(t);
   GridDev g = g0;
   int32_t base = 0;
@@ -286,25 +293,33 @@ __global__ void __launch_bounds__(kBlock) neighbor_scan_kernel(
         const int32_t b = src_begin[cell];
         const int32_t e = src_end[cell];
         if (b < 0 || e < 0) continue;
-        for (int32_t p = b; p < e; ++p) {
-          const SortedPoint q = src[p];
-          if (exclude_self && q.index == i) continue;
-          // within_radius (neighbor_search.cu:48-59)
-          const double ddx = px - q.x;
-          const double ddy = py - q.y;
-          const double ddz = pz - q.z;
-          if (ddx * ddx + ddy * ddy + ddz * ddz <= radius_squared) {
-            if (kWrite) {
-              indices[out++] = q.index;
-            } else {
-              ++count;
-            }
+        // the group's lanes test points p0 + sub; hits are compacted in point
+        // order, so the list is the one a sequential scan writes
+        for (int32_t p0 = b; p0 < e; p0 += kScanLanes) {
+          const int32_t p = p0 + sub;
+          bool hit = false;
+          int32_t qi = 0;
+          if (p < e) {
+            const SortedPoint q = src[p];
+            qi = q.index;
+            // within_radius (neighbor_search.cu:48-59)
+            const double ddx = px - q.x;
+            const double ddy = py - q.y;
+            const double ddz = pz - q.z;
+            hit = !(exclude_self && q.index == i) &&
+                  ddx * ddx + ddy * ddy + ddz * ddz <= radius_squared;
           }
+          const unsigned long long bits = __ballot(hit);
+          const uint32_t mine = static_cast<uint32_t>(bits >> group_lane) & kScanMask;
+          if (kWrite && hit) indices[out + __popc(mine & ((1u << sub) - 1u))] = qi;
+          const int32_t found = __popc(mine);
+          out += found;
+          count += found;
         }
       }
     }
   }
-  if (!kWrite) offsets[i] = count;
+  if (!kWrite && sub == 0) offsets[i] = count;
 }
 
 GridDev to_dev(const lsmps_cell_grid& g) {
@@ -458,12 +473,12 @@ void one_neighbor_list(Context& ctx, const lsmps_fluid_soa& fluid, size_t source
   }
   const int32_t* order = fluid_cells.sorted_indices;
   if (grids) {
-    neighbor_scan_kernel<false, true><<<blocks_for(fluid.count), kBlock, 0, ctx.stream>>>(
+    neighbor_scan_kernel<false, true><<<blocks_for(fluid.count * kScanLanes), kBlock, 0, ctx.stream>>>(
         fluid.count, order, fluid.x, fluid.y, fluid.z, g, grids, fluid_seg,
         source_cells.cell_begin, source_cells.cell_end, points, radius_squared,
         exclude_self ? 1 : 0, nbrs.offsets, nullptr);
   } else {
-    neighbor_scan_kernel<false, false><<<blocks_for(fluid.count), kBlock, 0, ctx.stream>>>(
+    neighbor_scan_kernel<false, false><<<blocks_for(fluid.count * kScanLanes), kBlock, 0, ctx.stream>>>(
         fluid.count, order, fluid.x, fluid.y, fluid.z, g, nullptr, nullptr,
         source_cells.cell_begin, source_cells.cell_end, points, radius_squared,
         exclude_self ? 1 : 0, nbrs.offsets, nullptr);
@@ -482,12 +497,12 @@ void one_neighbor_list(Context& ctx, const lsmps_fluid_soa& fluid, size_t source
                                   std::to_string(nbrs.neighbor_count));
   require(required == 0 || nbrs.indices != nullptr, "Neighbor list requires an indices buffer");
   if (grids) {
-    neighbor_scan_kernel<true, true><<<blocks_for(fluid.count), kBlock, 0, ctx.stream>>>(
+    neighbor_scan_kernel<true, true><<<blocks_for(fluid.count * kScanLanes), kBlock, 0, ctx.stream>>>(
         fluid.count, order, fluid.x, fluid.y, fluid.z, g, grids, fluid_seg,
         source_cells.cell_begin, source_cells.cell_end, points, radius_squared,
         exclude_self ? 1 : 0, nbrs.offsets, nbrs.indices);
   } else {
-    neighbor_scan_kernel<true, false><<<blocks_for(fluid.count), kBlock, 0, ctx.stream>>>(
+    neighbor_scan_kernel<true, false><<<blocks_for(fluid.count * kScanLanes), kBlock, 0, ctx.stream>>>(
         fluid.count, order, fluid.x, fluid.y, fluid.z, g, nullptr, nullptr,
         source_cells.cell_begin, source_cells.cell_end, points, radius_squared,
         exclude_self ? 1 : 0, nbrs.offsets, nbrs.indices);

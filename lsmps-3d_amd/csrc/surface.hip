@@ -72,7 +72,24 @@ __global__ void __launch_bounds__(kBlock) classify_kernel(
   double nd = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
   int wc = 0;
   const double px = x[i], py = y[i], pz = z[i];
-  for (int32_t c = fb; c < fe; ++c) {
+  // Four neighbours' loads are issued before their sums, which still run in
+  // neighbour order (the labels depend on the exact rounding of these sums).
+  int32_t c = fb;
+  for (; c + 4 <= fe; c += 4) {
+    int32_t j[4];
+    double dx[4], dy[4], dz[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) j[u] = f_idx[c + u];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      dx[u] = x[j[u]] - px;
+      dy[u] = y[j[u]] - py;
+      dz[u] = z[j[u]] - pz;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) accumulate(dx[u], dy[u], dz[u], p, nd, sx, sy, sz, wc);
+  }
+  for (; c < fe; ++c) {
     const int32_t j = f_idx[c];
     accumulate(x[j] - px, y[j] - py, z[j] - pz, p, nd, sx, sy, sz, wc);
   }
@@ -160,27 +177,38 @@ __global__ void __launch_bounds__(kBlock) classify_kernel(
 }
 
 // expand_near_surface_kernel (surface_detection.cu:322-348), reading primary labels only.
+// kExpandLanes adjacent lanes per particle, each testing neighbours l,
+// l + kExpandLanes, ...: the result is an "any" over the neighbours, so the
+// split does not change it.
+constexpr int kExpandLanes = 8;
+
 __global__ void __launch_bounds__(kBlock) expand_kernel(
     size_t n, const double* __restrict__ x, const double* __restrict__ y,
     const double* __restrict__ z, const int32_t* __restrict__ f_off,
     const int32_t* __restrict__ f_idx, const int32_t* __restrict__ primary, double r2,
     int32_t* __restrict__ labels) {
-  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  const size_t t = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const size_t i = t / kExpandLanes;
+  const int sub = static_cast<int>(t % kExpandLanes);
+  if (i >= n) return;  // whole groups
   int32_t type = primary[i];
   if (type == LSMPS_SURFACE_INNER && r2 > 0.0) {
     const double px = x[i], py = y[i], pz = z[i];
-    for (int32_t c = f_off[i]; c < f_off[i + 1]; ++c) {
+    int near = 0;
+    for (int32_t c = f_off[i] + sub; c < f_off[i + 1]; c += kExpandLanes) {
       const int32_t j = f_idx[c];
       if (primary[j] != LSMPS_SURFACE_SURFACE) continue;
       const double dx = x[j] - px, dy = y[j] - py, dz = z[j] - pz;
       if (dx * dx + dy * dy + dz * dz <= r2) {
-        type = LSMPS_SURFACE_NEAR_SURFACE;
+        near = 1;
         break;
       }
     }
+#pragma unroll
+    for (int off = kExpandLanes / 2; off > 0; off >>= 1) near |= __shfl_xor(near, off, 64);
+    if (near) type = LSMPS_SURFACE_NEAR_SURFACE;
   }
-  labels[i] = type;
+  if (sub == 0) labels[i] = type;
 }
 
 // The 14 virtual-light directions in the reference's order
@@ -311,7 +339,7 @@ void classify_surface(Context& ctx, const lsmps_fluid_soa& fluid, const lsmps_wa
   const double r2 = cfg.near_surface_radius > 0.0
                         ? cfg.near_surface_radius * cfg.near_surface_radius
                         : 0.0;
-  expand_kernel<<<blocks_for(fluid.count), kBlock, 0, ctx.stream>>>(
+  expand_kernel<<<blocks_for(fluid.count * kExpandLanes), kBlock, 0, ctx.stream>>>(
       fluid.count, fluid.x, fluid.y, fluid.z, fn.offsets, fn.indices, primary, r2,
       fluid.surface_type);
   LSMPS_LAUNCH_CHECK();
