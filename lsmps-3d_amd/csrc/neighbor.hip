@@ -247,9 +247,12 @@ __global__ void gather_points_kernel(size_t n, const int32_t* __restrict__ sorte
 // count_neighbors_kernel / write_neighbors_kernel (neighbor_search.cu:221-253) with
 // count/write_source_neighbors_for_fluid (:100-219) inlined.  Row t of the launch
 // handles fluid particle order[t] (cell order) so neighboring rows share cells.
-// kScanLanes adjacent lanes per particle (a power of two dividing 64).
-constexpr int kScanLanes = 8;
-constexpr uint32_t kScanMask = (1u << kScanLanes) - 1u;
+// Lanes per particle (a power of two dividing 64): the fill pass uses 8; the
+// count pass, which writes nothing per hit, is fastest with one.
+template <bool kWrite>
+constexpr int scan_lanes() {
+  return kWrite ? 8 : 1;
+}
 
 template <bool kWrite, bool kSeg>
 __global__ void __launch_bounds__(kBlock) neighbor_scan_kernel(
@@ -259,6 +262,8 @@ __global__ void __launch_bounds__(kBlock) neighbor_scan_kernel(
     const int32_t* __restrict__ src_begin, const int32_t* __restrict__ src_end,
     const SortedPoint* __restrict__ src, double radius_squared, int exclude_self,
     int32_t* __restrict__ offsets, int32_t* __restrict__ indices) {
+  constexpr int kScanLanes = scan_lanes<kWrite>();
+  constexpr uint32_t kScanMask = (1u << kScanLanes) - 1u;
   const size_t gt = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   const size_t t = gt / kScanLanes;
   const int sub = static_cast<int>(gt % kScanLanes);
@@ -473,12 +478,12 @@ void one_neighbor_list(Context& ctx, const lsmps_fluid_soa& fluid, size_t source
   }
   const int32_t* order = fluid_cells.sorted_indices;
   if (grids) {
-    neighbor_scan_kernel<false, true><<<blocks_for(fluid.count * kScanLanes), kBlock, 0, ctx.stream>>>(
+    neighbor_scan_kernel<false, true><<<blocks_for(fluid.count * scan_lanes<false>()), kBlock, 0, ctx.stream>>>(
         fluid.count, order, fluid.x, fluid.y, fluid.z, g, grids, fluid_seg,
         source_cells.cell_begin, source_cells.cell_end, points, radius_squared,
         exclude_self ? 1 : 0, nbrs.offsets, nullptr);
   } else {
-    neighbor_scan_kernel<false, false><<<blocks_for(fluid.count * kScanLanes), kBlock, 0, ctx.stream>>>(
+    neighbor_scan_kernel<false, false><<<blocks_for(fluid.count * scan_lanes<false>()), kBlock, 0, ctx.stream>>>(
         fluid.count, order, fluid.x, fluid.y, fluid.z, g, nullptr, nullptr,
         source_cells.cell_begin, source_cells.cell_end, points, radius_squared,
         exclude_self ? 1 : 0, nbrs.offsets, nullptr);
@@ -497,12 +502,12 @@ void one_neighbor_list(Context& ctx, const lsmps_fluid_soa& fluid, size_t source
                                   std::to_string(nbrs.neighbor_count));
   require(required == 0 || nbrs.indices != nullptr, "Neighbor list requires an indices buffer");
   if (grids) {
-    neighbor_scan_kernel<true, true><<<blocks_for(fluid.count * kScanLanes), kBlock, 0, ctx.stream>>>(
+    neighbor_scan_kernel<true, true><<<blocks_for(fluid.count * scan_lanes<true>()), kBlock, 0, ctx.stream>>>(
         fluid.count, order, fluid.x, fluid.y, fluid.z, g, grids, fluid_seg,
         source_cells.cell_begin, source_cells.cell_end, points, radius_squared,
         exclude_self ? 1 : 0, nbrs.offsets, nbrs.indices);
   } else {
-    neighbor_scan_kernel<true, false><<<blocks_for(fluid.count * kScanLanes), kBlock, 0, ctx.stream>>>(
+    neighbor_scan_kernel<true, false><<<blocks_for(fluid.count * scan_lanes<true>()), kBlock, 0, ctx.stream>>>(
         fluid.count, order, fluid.x, fluid.y, fluid.z, g, nullptr, nullptr,
         source_cells.cell_begin, source_cells.cell_end, points, radius_squared,
         exclude_self ? 1 : 0, nbrs.offsets, nbrs.indices);
