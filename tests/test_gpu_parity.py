@@ -173,6 +173,32 @@ def test_preconditioners_agree(rt, abi, cases, precision):
                           gc["pressure"])
 
 
+@pytest.mark.parametrize("block,tank,cells,lo,hi", [
+    ((60, 40, 40), (120, 40, 80), 1, 2048, 4096),  # 4 rows per panel thread, 1-column panels
+    ((44, 28, 28), (88, 28, 56), 1, 1024, 2048),   # 2 rows per panel thread
+    ((60, 40, 40), (120, 40, 80), 2, 64, 1024),    # 1 row per panel thread, 8-column panels
+])
+def test_coarse_space_sizes(rt, abi, cases, block, tank, cells, lo, hi):
+    """The coarse inverse runs in Gauss-Jordan panels whose width and rows per
+    thread follow the number of coarse unknowns.  At each size the coarse
+    space must leave the solution unchanged (same FP64 system, same 1e-10
+    residual) and cut the iteration count against the Schwarz blocks alone,
+    which a wrong inverse would not do."""
+    case = cases.dam_break(block=block, tank=tank, l0=0.01, seed=2)
+    prec = abi.PRECISION_FP64
+    sch = abi.default_solver_options()
+    sch.preconditioner = abi.PRECOND_SCHWARZ
+    crs = abi.default_solver_options()
+    crs.coarse_cells = cells
+    gs = rt.run_step_on_device(case, precision=prec, solver=sch)
+    gc = rt.run_step_on_device(case, precision=prec, solver=crs)
+    print(block, cells, "schwarz", gs["report"], "coarse", gc["report"], flush=True)
+    assert gs["report"]["converged"] == 1 and gc["report"]["converged"] == 1
+    assert lo < gc["report"]["coarse_unknowns"] <= hi
+    assert rel_l2(gc["pressure"], gs["pressure"]) <= 1e-8
+    assert gc["report"]["iterations"] < gs["report"]["iterations"]
+
+
 @pytest.mark.parametrize("precision", ["FP64", "MIXED"])
 def test_batched_tanks_match_single_tanks(rt, abi, cases, oracle, precision):
     """Four independent tanks (hydrostatic and dam-break, uneven sizes) stepped
