@@ -1193,14 +1193,27 @@ __device__ void freeze(SolverScalars* sc) {
 }
 
 // Lower-triangular solve M[k:, k:] c[k:] = f[k:] (single thread).
+// The s x s recurrences run on the first wavefront of the block, lane i owning
+// row i: forward substitutions go column by column (lane l finishes its row,
+// broadcasts the result, and the rows below subtract its term), so every row
+// takes the same operations in the same order as a row-by-row substitution.
+// c(k:s) = M(k:s, k:s)^-1 f(k:s), lower triangular.
 __device__ void solve_c(SolverScalars* sc, int s, int k) {
-  for (int i = k; i < s; ++i) {
-    double t = sc->f[i];
-    for (int l = k; l < i; ++l) t -= sc->M[i * kMaxShadow + l] * sc->c[l];
-    const double d = sc->M[i * kMaxShadow + i];
-    sc->c[i] = d != 0.0 ? t / d : 0.0;
-    if (d == 0.0) sc->breakdown = 1;
+  const int i = threadIdx.x;  // lane, < 64
+  double t = i >= k && i < s ? sc->f[i] : 0.0;
+  int singular = 0;
+  for (int l = k; l < s; ++l) {
+    double cl = 0.0;
+    if (i == l) {
+      const double d = sc->M[l * kMaxShadow + l];
+      cl = d != 0.0 ? t / d : 0.0;
+      sc->c[l] = cl;
+      singular = d == 0.0;
+    }
+    cl = __shfl(cl, l, 64);
+    if (i > l && i < s) t -= sc->M[i * kMaxShadow + l] * cl;
   }
+  if (__any(singular) && i == 0) sc->breakdown = 1;
 }
 
 // The single-thread recurrences below work on an LDS copy of the system's
@@ -1225,11 +1238,11 @@ __global__ void __launch_bounds__(kReduceThreads)
   reduce_partials(partials, nb, d);
   scalars_copy(sc_g, &lds);
   SolverScalars* sc = &lds;
-  if (threadIdx.x == 0) {
+  if (threadIdx.x < 64) {  // the first wavefront
     if (sc->done) {
-      freeze(sc);
+      if (threadIdx.x == 0) freeze(sc);
     } else {
-      for (int q = 0; q < s; ++q) sc->f[q] = d[q];
+      if (static_cast<int>(threadIdx.x) < s) sc->f[threadIdx.x] = d[threadIdx.x];
       solve_c(sc, s, 0);
     }
   }
@@ -1249,35 +1262,49 @@ __global__ void __launch_bounds__(kReduceThreads)
   __shared__ SolverScalars lds;
   reduce_partials(partials, nb, d);
   scalars_copy(sc_g, &lds);
-  if (threadIdx.x == 0) {
-    if (lds.done) freeze(&lds);
-    else scalar_after_body(&lds, d, s, k);
+  if (threadIdx.x < 64) {  // the first wavefront
+    if (lds.done) {
+      if (threadIdx.x == 0) freeze(&lds);
+    } else {
+      scalar_after_body(&lds, d, s, k);
+    }
   }
   __syncthreads();
   scalars_copy(&lds, sc_g);
 }
 
 __device__ void scalar_after_body(SolverScalars* sc, const double* d, int s, int k) {
-  double* alpha = sc->alpha;
-  for (int i = 0; i < k; ++i) {
-    double t = d[i];
-    for (int l = 0; l < i; ++l) t -= alpha[l] * sc->M[i * kMaxShadow + l];
-    const double mii = sc->M[i * kMaxShadow + i];
-    alpha[i] = mii != 0.0 ? t / mii : 0.0;
+  const int i = threadIdx.x;  // lane, < 64
+  // alpha(0:k) = M(0:k, 0:k)^-1 d(0:k)
+  double t = i < k ? d[i] : 0.0;
+  double my_alpha = 0.0;
+  for (int l = 0; l < k; ++l) {
+    double al = 0.0;
+    if (i == l) {
+      const double mii = sc->M[l * kMaxShadow + l];
+      al = mii != 0.0 ? t / mii : 0.0;
+      sc->alpha[l] = al;
+      my_alpha = al;
+    }
+    al = __shfl(al, l, 64);
+    if (i > l && i < k) t -= al * sc->M[i * kMaxShadow + l];
   }
-  for (int i = k; i < s; ++i) {
-    double t = d[i];
-    for (int l = 0; l < k; ++l) t -= alpha[l] * sc->M[i * kMaxShadow + l];
-    sc->M[i * kMaxShadow + k] = t;
+  // new column k of M: d(k:s) minus the projections on the earlier columns
+  double mk = i >= k && i < s ? d[i] : 0.0;
+  for (int l = 0; l < k; ++l) {
+    const double al = __shfl(my_alpha, l, 64);
+    if (i >= k && i < s) mk -= al * sc->M[i * kMaxShadow + l];
   }
-  const double mkk = sc->M[k * kMaxShadow + k];
+  if (i >= k && i < s) sc->M[i * kMaxShadow + k] = mk;
+  const double mkk = __shfl(mk, k, 64);
+  double beta = 0.0;
   if (mkk == 0.0 || !isfinite(mkk)) {
-    sc->breakdown = 1;
-    sc->beta = 0.0;
+    if (i == 0) sc->breakdown = 1;
   } else {
-    sc->beta = sc->f[k] / mkk;
+    beta = sc->f[k] / mkk;
   }
-  for (int i = k + 1; i < s; ++i) sc->f[i] -= sc->beta * sc->M[i * kMaxShadow + k];
+  if (i == 0) sc->beta = beta;
+  if (i > k && i < s) sc->f[i] -= beta * mk;
   if (k + 1 < s) solve_c(sc, s, k + 1);
 }
 
