@@ -201,6 +201,7 @@ __global__ void __launch_bounds__(kBlock) fill_u64(unsigned long long* p, size_t
 // two launches per panel instead of two per column.
 constexpr int kPanelThreads = 1024;
 constexpr int kPanelWaves = kPanelThreads / 64;
+static_assert(kPanelWaves <= 64 && (kPanelWaves & (kPanelWaves - 1)) == 0, "one partial per lane");
 constexpr int kPanelLds = 7680;   // doubles of the update's f columns (b x count)
 constexpr int kMaxPanel = 8;
 constexpr int kUpdateRows = 32;   // column entries per thread in the update
@@ -268,9 +269,19 @@ __global__ void __launch_bounds__(kPanelThreads) gj_panel_kernel(double* a, int 
       wi[t >> 6] = bidx;
     }
     __syncthreads();
-    best = wv[0];
-    bidx = wi[0];
-    for (int w = 1; w < kPanelWaves; ++w) better_pivot(best, bidx, wv[w], wi[w]);
+    {  // every wavefront reduces the kPanelWaves partials the same way
+      const int l = t & 63;
+      best = l < kPanelWaves ? wv[l] : -1.0;
+      bidx = l < kPanelWaves ? wi[l] : count;
+#pragma unroll
+      for (int off = kPanelWaves / 2; off > 0; off >>= 1) {
+        const double v2 = __shfl_xor(best, off, 64);
+        const int i2 = __shfl_xor(bidx, off, 64);
+        better_pivot(best, bidx, v2, i2);
+      }
+      best = __shfl(best, 0, 64);
+      bidx = __shfl(bidx, 0, 64);
+    }
     const int p = bidx;
     const bool singular = !(best > 0.0) || !isfinite(best);
     if (t == 0) {
