@@ -415,10 +415,14 @@ __global__ void __launch_bounds__(kSellChunk) dia_select_kernel(
   }
   __syncthreads();
   const int32_t e0 = rs[0], e1 = rs[kSellChunk];
+  // the next group's column is loaded while this group's entry is hashed
+  int32_t col_next = e0 + lane < e1 ? cols[e0 + lane] : 0;
   for (int32_t base = e0; base < e1; base += kSellChunk) {
     const int32_t c = base + lane;
     if (c >= e1) break;
-    const int32_t d = cols[c] - (row0 + chunk_row(rs, c));
+    const int32_t col = col_next;
+    if (c + kSellChunk < e1) col_next = cols[c + kSellChunk];
+    const int32_t d = col - (row0 + chunk_row(rs, c));
     uint32_t h = (static_cast<uint32_t>(d) * 2654435761u) & (kDiaTable - 1);
     bool placed = false;
     for (int probe = 0; probe < kDiaTable; ++probe) {
@@ -471,11 +475,13 @@ __global__ void __launch_bounds__(kSellChunk) dia_select_kernel(
   const int64_t first_bucket = static_cast<int64_t>(row0 >> kBucketBits) - 64;
   uint64_t lo = 0, hi = 0;
   int outside = 0;
+  col_next = e0 + lane < e1 ? cols[e0 + lane] : 0;
   for (int32_t base = e0; base < e1; base += kSellChunk) {
     const int32_t c = base + lane;
     const bool in = c < e1;
     const int lr = in ? chunk_row(rs, c) : kSellChunk;
-    const int32_t col = in ? cols[c] : 0;
+    const int32_t col = in ? col_next : 0;
+    if (c + kSellChunk < e1) col_next = cols[c + kSellChunk];
     const bool rem = in && dia_find(list, nd, col - (row0 + lr)) < 0;
     if (rem) {
       const int64_t rel = (static_cast<int64_t>(col) >> kBucketBits) - first_bucket;
