@@ -43,7 +43,8 @@ enum {
   LSMPS_ERR_NOT_CONVERGED = 5,    /* PPE solver hit max_iterations */
   LSMPS_ERR_OUT_OF_MEMORY = 6,
   LSMPS_ERR_NUMERICAL = 7,        /* non-finite values / breakdown */
-  LSMPS_ERR_IO = 8                /* a file could not be opened or written (std::runtime_error paths) */
+  LSMPS_ERR_IO = 8,               /* a file could not be opened or written (std::runtime_error paths) */
+  LSMPS_ERR_INTERNAL = 9          /* a library invariant failed (e.g. a device allocation where none may happen) */
 };
 
 /* SurfaceType (include/lsmps3d/surface/surface_type.cuh:5-10) */
@@ -409,6 +410,22 @@ int lsmps_ppe_assemble(lsmps_context* ctx, const lsmps_config* config,
                        const lsmps_fluid_soa* temporary_velocity,
                        const lsmps_wall_soa* temporary_wall_velocity, lsmps_moment_matrix* moments,
                        uint64_t geometry_generation, const lsmps_ppe_workspace* workspace);
+/* assemble_ppe_matrix_and_rhs(fluid, walls, config, fluid_neighbors, wall_neighbors,
+ * temporary_velocity, temporary_wall_velocity, const MomentMatrixView& velocity_moment,
+ * const MomentMatrixView& pressure_moment, const PpeWorkspaceView&) — ppe_matrix.cuh:120-129,
+ * ppe_matrix.cu:517-551.  The views' explicit inverses are read (rows 0-2 of the
+ * velocity kind, rows 3+4+5 of the pressure kind); a view that is not a prepared
+ * Type-A matrix of the expected kind gives LSMPS_ERR_NOT_PREPARED with the
+ * reference's message (validate_ppe_moment_matrix, ppe_matrix.cu:108-122). */
+int lsmps_ppe_assemble_views(lsmps_context* ctx, const lsmps_config* config,
+                             const lsmps_fluid_soa* fluid, const lsmps_wall_soa* walls,
+                             const lsmps_neighbor_list* fluid_neighbors,
+                             const lsmps_neighbor_list* wall_neighbors,
+                             const lsmps_fluid_soa* temporary_velocity,
+                             const lsmps_wall_soa* temporary_wall_velocity,
+                             const lsmps_moment_view* velocity_moment,
+                             const lsmps_moment_view* pressure_moment,
+                             const lsmps_ppe_workspace* workspace);
 /* AmgxPpeSolver::solve(const CsrMatrixView&, const real* rhs, real* pressure) —
  * ppe_matrix.cuh:112.  Hand-written IDR(s) / BiCGStab / PCG from p = 0, then the
  * reference's clamp p = max(p, 0) (ppe_matrix.cu:662).  options may be NULL
@@ -443,6 +460,17 @@ int lsmps_compute_pressure_gradient(lsmps_context* ctx, const lsmps_fluid_soa* f
                                     const double* pressure, const lsmps_config* config,
                                     lsmps_moment_matrix* moments,
                                     const lsmps_vector3* pressure_gradient);
+/* compute_pressure_gradient(..., const MomentMatrixView& pressure_moment,
+ * CorrectionVectorView) — pressure_correction.cuh:93-100, pressure_correction.cu:657-682:
+ * rows 0-2 of the view's explicit inverses; validation as
+ * validate_pressure_moment_matrix (pressure_correction.cu:96-111). */
+int lsmps_compute_pressure_gradient_view(lsmps_context* ctx, const lsmps_fluid_soa* fluid,
+                                         const lsmps_wall_soa* walls,
+                                         const lsmps_neighbor_list* fluid_neighbors,
+                                         const lsmps_neighbor_list* wall_neighbors,
+                                         const double* pressure, const lsmps_config* config,
+                                         const lsmps_moment_view* pressure_moment,
+                                         const lsmps_vector3* pressure_gradient);
 /* compute_particle_shifting_displacement — pressure_correction.cuh:102-107 */
 int lsmps_compute_particle_shifting(lsmps_context* ctx, const lsmps_fluid_soa* fluid,
                                     const lsmps_wall_soa* walls,

@@ -15,9 +15,10 @@
 //    Jacobi preconditioning to a 1e-10 relative true residual by default); the
 //    AMGX config path is accepted and ignored;
 //  * the moment-view free functions (assemble_ppe_matrix_and_rhs,
-//    compute_pressure_gradient) take the DeviceMomentMatrix that produced the
-//    view, because the library keeps LSMPS operator rows rather than the
-//    explicit inverses those functions read.
+//    compute_pressure_gradient) have the reference's MomentMatrixView
+//    signatures, which read the views' explicit inverses, and a second form
+//    taking the DeviceMomentMatrix that produced them, which reads the operator
+//    rows the library keeps instead (no inverse is materialised).
 #pragma once
 
 #include <hip/hip_runtime_api.h>
@@ -941,7 +942,48 @@ class DevicePpeMatrixAssembler {
   PpeWorkspaceView active_{};
 };
 
-// Free assembly (ppe_matrix.cuh:120-129), reading the operators of `moments`.
+namespace detail {
+inline lsmps_moment_view to_c(const MomentMatrixView& v) {
+  lsmps_moment_view o{};
+  o.particle_count = v.particle_count;
+  o.matrix_size = v.matrix_size;
+  o.basis_kind = static_cast<int32_t>(v.basis_kind);
+  o.kind = static_cast<int32_t>(v.kind);
+  o.is_ready = v.is_ready ? 1 : 0;
+  o.support_radius = v.support_radius;
+  o.wall_weight_scale = v.wall_weight_scale;
+  o.geometry_generation = v.geometry_generation;
+  o.inverse_matrices = v.inverse_matrices;
+  o.moment_trace = v.moment_trace;
+  o.regularization_added = v.regularization_added;
+  o.info = v.info;
+  o.inversion_count = v.inversion_count;
+  return o;
+}
+}  // namespace detail
+
+// Free assembly (ppe_matrix.cuh:120-129): the reference's signature, reading
+// rows 0-2 of the velocity view's inverses and rows 3+4+5 of the pressure view's.
+inline void assemble_ppe_matrix_and_rhs(const FluidParticleSoA& fluid,
+                                        const WallParticleSoA& walls,
+                                        const SimulationConfig& config,
+                                        const NeighborListView& fluid_neighbors,
+                                        const NeighborListView& wall_neighbors,
+                                        const FluidParticleSoA& temporary_velocity,
+                                        const WallParticleSoA& temporary_wall_velocity,
+                                        const MomentMatrixView& velocity_moment,
+                                        const MomentMatrixView& pressure_moment,
+                                        const PpeWorkspaceView& workspace) {
+  const lsmps_config c = detail::to_c(config);
+  const lsmps_moment_view vm = detail::to_c(velocity_moment);
+  const lsmps_moment_view pm = detail::to_c(pressure_moment);
+  detail::check(lsmps_ppe_assemble_views(detail::context(), &c, &fluid, &walls, &fluid_neighbors,
+                                         &wall_neighbors, &temporary_velocity,
+                                         &temporary_wall_velocity, &vm, &pm, &workspace),
+                "assemble_ppe_matrix_and_rhs");
+}
+
+// Free assembly reading the operator rows `moments` keeps (no inverse needed).
 inline void assemble_ppe_matrix_and_rhs(const FluidParticleSoA& fluid,
                                         const WallParticleSoA& walls,
                                         const SimulationConfig& config,
@@ -1042,6 +1084,21 @@ inline void clamp_particle_pressure(const FluidParticleSoA& fluid, const real* p
   detail::check(lsmps_clamp_particle_pressure(detail::context(), &fluid, const_cast<real*>(pressure)),
                 "clamp_particle_pressure");
 }
+// pressure_correction.cuh:93-100: rows 0-2 of the pressure view's inverses.
+inline void compute_pressure_gradient(const FluidParticleSoA& fluid, const WallParticleSoA& walls,
+                                      const NeighborListView& fluid_neighbors,
+                                      const NeighborListView& wall_neighbors,
+                                      const real* pressure, const SimulationConfig& config,
+                                      const MomentMatrixView& pressure_moment,
+                                      CorrectionVectorView pressure_gradient) {
+  const lsmps_config c = detail::to_c(config);
+  const lsmps_moment_view pm = detail::to_c(pressure_moment);
+  detail::check(lsmps_compute_pressure_gradient_view(detail::context(), &fluid, &walls,
+                                                     &fluid_neighbors, &wall_neighbors, pressure,
+                                                     &c, &pm, &pressure_gradient),
+                "compute_pressure_gradient");
+}
+// The same, reading the operator rows `moments` keeps.
 inline void compute_pressure_gradient(const FluidParticleSoA& fluid, const WallParticleSoA& walls,
                                       const NeighborListView& fluid_neighbors,
                                       const NeighborListView& wall_neighbors,

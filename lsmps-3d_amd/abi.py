@@ -14,6 +14,7 @@ LSMPS_ERR_NOT_CONVERGED = 5
 LSMPS_ERR_OUT_OF_MEMORY = 6
 LSMPS_ERR_NUMERICAL = 7
 LSMPS_ERR_IO = 8
+LSMPS_ERR_INTERNAL = 9
 
 INNER, NEAR_SURFACE, SURFACE, SPLASH = 0, 1, 2, 3
 

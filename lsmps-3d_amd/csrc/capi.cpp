@@ -426,6 +426,50 @@ int lsmps_ppe_assemble(lsmps_context* ctx, const lsmps_config* config,
   API_END("lsmps_ppe_assemble")
 }
 
+namespace {
+// validate_ppe_moment_matrix (ppe_matrix.cu:108-122) and
+// validate_pressure_moment_matrix (pressure_correction.cu:96-111).
+void require_type_a_view(const lsmps_moment_view& v, int kind, size_t count,
+                         const std::string& message, const std::string& missing) {
+  require(v.is_ready != 0 && v.kind == kind && v.basis_kind == LSMPS_BASIS_TYPE_A &&
+              v.matrix_size == 9 && v.particle_count >= count,
+          message.c_str(), LSMPS_ERR_NOT_PREPARED);
+  require(count == 0 || v.inverse_matrices != nullptr, missing.c_str(), LSMPS_ERR_NOT_PREPARED);
+}
+}  // namespace
+
+int lsmps_ppe_assemble_views(lsmps_context* ctx, const lsmps_config* config,
+                             const lsmps_fluid_soa* fluid, const lsmps_wall_soa* walls,
+                             const lsmps_neighbor_list* fn, const lsmps_neighbor_list* wn,
+                             const lsmps_fluid_soa* tmp, const lsmps_wall_soa* tmpw,
+                             const lsmps_moment_view* velocity_moment,
+                             const lsmps_moment_view* pressure_moment,
+                             const lsmps_ppe_workspace* ws) {
+  API_BEGIN
+  Context& c = C(ctx);
+  const auto& cfg = deref(config, "config");
+  const auto& f = deref(fluid, "fluid");
+  const auto& vm = deref(velocity_moment, "velocity moment view");
+  const auto& pm = deref(pressure_moment, "pressure moment view");
+  require_type_a_view(vm, LSMPS_MOMENT_VELOCITY_WALL_DIRICHLET_A, f.count,
+                      "PPE requires a prepared velocity Type-A moment matrix",
+                      "PPE moment matrix inverse buffer is missing for velocity");
+  require_type_a_view(pm, LSMPS_MOMENT_PRESSURE_WALL_NEUMANN_A, f.count,
+                      "PPE requires a prepared pressure Type-A moment matrix",
+                      "PPE moment matrix inverse buffer is missing for pressure");
+  const size_t n = f.count;
+  c.view_ops.reserve(std::max<size_t>(2 * kOpRows * n, 1));
+  double* vel = c.view_ops.data();
+  double* pre = vel + kOpRows * n;
+  operator_rows_from_view(c, vm, n, vel);
+  operator_rows_from_view(c, pm, n, pre);
+  ppe_assemble(c, cfg, f, deref(walls, "walls"), deref(fn, "fluid neighbors"),
+               deref(wn, "wall neighbors"), deref(tmp, "temporary velocity"),
+               deref(tmpw, "temporary wall velocity"), vel, pre, deref(ws, "workspace"));
+  c.finish();
+  API_END("lsmps_ppe_assemble_views")
+}
+
 int lsmps_default_solver_options(lsmps_solver_options* out) {
   API_BEGIN
   require(out != nullptr, "null options");
@@ -514,6 +558,30 @@ int lsmps_compute_pressure_gradient(lsmps_context* ctx, const lsmps_fluid_soa* f
                     m->pressure_ops(), deref(grad, "gradient"));
   c.finish();
   API_END("lsmps_compute_pressure_gradient")
+}
+
+int lsmps_compute_pressure_gradient_view(lsmps_context* ctx, const lsmps_fluid_soa* fluid,
+                                         const lsmps_wall_soa* walls,
+                                         const lsmps_neighbor_list* fn,
+                                         const lsmps_neighbor_list* wn, const double* pressure,
+                                         const lsmps_config* config,
+                                         const lsmps_moment_view* pressure_moment,
+                                         const lsmps_vector3* grad) {
+  API_BEGIN
+  Context& c = C(ctx);
+  const auto& f = deref(fluid, "fluid");
+  const auto& pm = deref(pressure_moment, "pressure moment view");
+  require_type_a_view(pm, LSMPS_MOMENT_PRESSURE_WALL_NEUMANN_A, f.count,
+                      "Pressure correction requires a prepared pressure Type-A moment matrix",
+                      "Pressure correction pressure moment inverse buffer is missing");
+  const size_t n = f.count;
+  c.view_ops.reserve(std::max<size_t>(kOpRows * n, 1));
+  operator_rows_from_view(c, pm, n, c.view_ops.data());
+  pressure_gradient(c, deref(config, "config"), f, deref(walls, "walls"),
+                    deref(fn, "fluid neighbors"), deref(wn, "wall neighbors"), pressure,
+                    c.view_ops.data(), deref(grad, "gradient"));
+  c.finish();
+  API_END("lsmps_compute_pressure_gradient_view")
 }
 
 int lsmps_compute_particle_shifting(lsmps_context* ctx, const lsmps_fluid_soa* fluid,

@@ -5,6 +5,7 @@
 // Kernels that reduce over an aggregate or a coarse row give each one a
 // wavefront: lanes take the items in a fixed stride and the partial sums meet
 // in a fixed butterfly, so the sums are reproducible.
+#include <algorithm>
 #include <cmath>
 
 #include "coarse.hpp"
@@ -538,47 +539,69 @@ void coarse_aggregate(Context& ctx, CoarseSpace& C, size_t n, const int32_t* lab
                       const int32_t* cell_sorted, const int32_t* cell_begin,
                       const int32_t* cell_end, const int32_t dims[3], int factor) {
   C.count = 0;
+  C.factor = 0;
   C.n = n;
   C.ready = false;
   if (n == 0 || factor < 1 || dims[0] <= 0 || dims[1] <= 0 || dims[2] <= 0) return;
-  CoarseGrid g{dims[0], dims[1], dims[2], (dims[0] + factor - 1) / factor,
-               (dims[1] + factor - 1) / factor, (dims[2] + factor - 1) / factor, factor};
-  const size_t ccells = static_cast<size_t>(g.cx) * g.cy * g.cz;
-  C.cell_count.reserve(ccells);
-  C.cell_start.reserve(ccells);
-  C.cell_flag.reserve(ccells);
-  C.cell_id.reserve(ccells);
   C.agg_of_row.reserve(n);
-  count_members_kernel<<<wave_blocks(ccells), kBlock, 0, ctx.stream>>>(
-      g, ccells, labels, cell_sorted, cell_begin, cell_end, C.cell_count.data(),
-      C.cell_flag.data());
-  LSMPS_LAUNCH_CHECK();
-  exclusive_scan_i32(ctx, C.cell_count.data(), C.cell_start.data(), ccells);
-  exclusive_scan_i32(ctx, C.cell_flag.data(), C.cell_id.data(), ccells);
-  int32_t* h = ctx.host_i32;
-  LSMPS_HIP(hipMemcpyAsync(h, C.cell_id.data() + ccells - 1, sizeof(int32_t),
-                           hipMemcpyDeviceToHost, ctx.stream));
-  LSMPS_HIP(hipMemcpyAsync(h + 1, C.cell_flag.data() + ccells - 1, sizeof(int32_t),
-                           hipMemcpyDeviceToHost, ctx.stream));
-  LSMPS_HIP(hipMemcpyAsync(h + 2, C.cell_start.data() + ccells - 1, sizeof(int32_t),
-                           hipMemcpyDeviceToHost, ctx.stream));
-  LSMPS_HIP(hipMemcpyAsync(h + 3, C.cell_count.data() + ccells - 1, sizeof(int32_t),
-                           hipMemcpyDeviceToHost, ctx.stream));
-  ctx.sync();
-  const int count = h[0] + h[1];
-  const int members = h[2] + h[3];
-  if (count <= 1 || count > kMaxCoarse || members <= 0) return;  // no useful coarse space
-  C.agg_start.reserve(static_cast<size_t>(count) + 1);
-  C.agg_rows.reserve(static_cast<size_t>(members));
-  fill_i32<<<blocks_for(n), kBlock, 0, ctx.stream>>>(C.agg_of_row.data(), n, -1);
-  fill_members_kernel<<<wave_blocks(ccells), kBlock, 0, ctx.stream>>>(
-      g, ccells, labels, cell_sorted, cell_begin, cell_end, C.cell_count.data(),
-      C.cell_start.data(), C.cell_id.data(), C.agg_start.data(), C.agg_rows.data(),
-      C.agg_of_row.data());
-  set_last_kernel<<<1, 1, 0, ctx.stream>>>(C.agg_start.data(), count, C.cell_count.data(),
-                                           C.cell_start.data(), ccells);
-  LSMPS_LAUNCH_CHECK();
-  C.count = count;
+  const int32_t widest = std::max(dims[0], std::max(dims[1], dims[2]));
+  for (int f = factor; f <= widest; ++f) {
+    CoarseGrid g{dims[0], dims[1], dims[2], (dims[0] + f - 1) / f, (dims[1] + f - 1) / f,
+                 (dims[2] + f - 1) / f, f};
+    const size_t ccells = static_cast<size_t>(g.cx) * g.cy * g.cz;
+    C.cell_count.reserve(ccells);
+    C.cell_start.reserve(ccells);
+    C.cell_flag.reserve(ccells);
+    C.cell_id.reserve(ccells);
+    count_members_kernel<<<wave_blocks(ccells), kBlock, 0, ctx.stream>>>(
+        g, ccells, labels, cell_sorted, cell_begin, cell_end, C.cell_count.data(),
+        C.cell_flag.data());
+    LSMPS_LAUNCH_CHECK();
+    exclusive_scan_i32(ctx, C.cell_count.data(), C.cell_start.data(), ccells);
+    exclusive_scan_i32(ctx, C.cell_flag.data(), C.cell_id.data(), ccells);
+    int32_t* h = ctx.host_i32;
+    LSMPS_HIP(hipMemcpyAsync(h, C.cell_id.data() + ccells - 1, sizeof(int32_t),
+                             hipMemcpyDeviceToHost, ctx.stream));
+    LSMPS_HIP(hipMemcpyAsync(h + 1, C.cell_flag.data() + ccells - 1, sizeof(int32_t),
+                             hipMemcpyDeviceToHost, ctx.stream));
+    LSMPS_HIP(hipMemcpyAsync(h + 2, C.cell_start.data() + ccells - 1, sizeof(int32_t),
+                             hipMemcpyDeviceToHost, ctx.stream));
+    LSMPS_HIP(hipMemcpyAsync(h + 3, C.cell_count.data() + ccells - 1, sizeof(int32_t),
+                             hipMemcpyDeviceToHost, ctx.stream));
+    ctx.sync();
+    const int count = h[0] + h[1];
+    const int members = h[2] + h[3];
+    if (count <= 1 || members <= 0) return;  // no useful coarse space
+    if (count > kMaxCoarse) continue;        // too many aggregates: wider ones
+    C.agg_start.reserve(static_cast<size_t>(count) + 1);
+    C.agg_rows.reserve(static_cast<size_t>(members));
+    fill_i32<<<blocks_for(n), kBlock, 0, ctx.stream>>>(C.agg_of_row.data(), n, -1);
+    fill_members_kernel<<<wave_blocks(ccells), kBlock, 0, ctx.stream>>>(
+        g, ccells, labels, cell_sorted, cell_begin, cell_end, C.cell_count.data(),
+        C.cell_start.data(), C.cell_id.data(), C.agg_start.data(), C.agg_rows.data(),
+        C.agg_of_row.data());
+    set_last_kernel<<<1, 1, 0, ctx.stream>>>(C.agg_start.data(), count, C.cell_count.data(),
+                                             C.cell_start.data(), ccells);
+    LSMPS_LAUNCH_CHECK();
+    C.count = count;
+    C.factor = f;
+    return;
+  }
+}
+
+void coarse_reserve(CoarseSpace& C) {
+  const int count = C.count;
+  if (count <= 1) return;
+  const size_t nn = static_cast<size_t>(count) * count;
+  C.work.reserve(nn);
+  C.ainv.reserve(nn);
+  C.colk.reserve(static_cast<size_t>(kMaxPanel) * count + kMaxPanel);
+  C.perm.reserve(static_cast<size_t>(count));
+  C.status.reserve(1);
+  C.rc.reserve(static_cast<size_t>(count));
+  C.yc.reserve(static_cast<size_t>(count));
+  C.rowabs.reserve(C.n);
+  C.wfac.reserve(C.n);
 }
 
 void coarse_setup(Context& ctx, CoarseSpace& C, const lsmps_csr& A, const double* diag,
@@ -588,17 +611,12 @@ void coarse_setup(Context& ctx, CoarseSpace& C, const lsmps_csr& A, const double
   const int count = C.count;
   if (count <= 1 || C.n != A.rows) return;
   const size_t nn = static_cast<size_t>(count) * count;
-  C.work.reserve(nn);
-  C.ainv.reserve(nn);
-  C.colk.reserve(static_cast<size_t>(kMaxPanel) * count + kMaxPanel);
-  C.perm.reserve(static_cast<size_t>(count));
-  C.status.reserve(1);
-  C.rc.reserve(static_cast<size_t>(count));
-  C.yc.reserve(static_cast<size_t>(count));
+  require(C.work.size() >= nn && C.ainv.size() >= nn && C.rc.size() >= static_cast<size_t>(count) &&
+              C.rowabs.size() >= A.rows,
+          "coarse setup before coarse_reserve", LSMPS_ERR_INTERNAL);
   fill_i32<<<1, 1, 0, st>>>(C.status.data(), 1, 0);
   // integer matrix in ainv's storage (overwritten by the inverse at the end)
   unsigned long long* acci = reinterpret_cast<unsigned long long*>(C.ainv.data());
-  C.rowabs.reserve(A.rows);
   row_bound_kernel<<<blocks_for(A.rows), kBlock, 0, st>>>(
       A.rows, C.agg_of_row.data(), diag, offabs, C.rowabs.data());
   restrict_kernel<<<static_cast<unsigned>(count), kBlock, 0, st>>>(
@@ -641,7 +659,7 @@ void coarse_setup(Context& ctx, CoarseSpace& C, const lsmps_csr& A, const double
 }
 
 void coarse_set_weights(Context& ctx, CoarseSpace& C, const double* pdiag) {
-  C.wfac.reserve(C.n);
+  require(C.wfac.size() >= C.n, "coarse weights before coarse_reserve", LSMPS_ERR_INTERNAL);
   inverse_kernel<<<blocks_for(C.n), kBlock, 0, ctx.stream>>>(C.n, pdiag, C.wfac.data());
   LSMPS_LAUNCH_CHECK();
 }

@@ -25,6 +25,7 @@ constexpr int kMaxCoarse = 4096;  // LDS row of the coarse-matrix build (int64)
 struct CoarseSpace {
   size_t n = 0;                      // fine rows the aggregation was built for
   int count = 0;                     // aggregates (0: no correction)
+  int factor = 0;                    // cells per aggregate side actually used
   DeviceArray<int32_t> agg_of_row;   // n: aggregate of a row, -1 outside
   DeviceArray<int32_t> agg_start;    // count + 1
   DeviceArray<int32_t> agg_rows;     // members, in cell order then index order
@@ -51,15 +52,22 @@ struct CoarseSpace {
 // Aggregation from a fluid cell list: cells (cx, cy, cz) of a dims grid grouped
 // as (cx / f, cy / f, cz / f); rows whose label is Inner or NearSurface join
 // their group.  Members are listed cell by cell in (z, y, x) order, each cell's
-// particles ascending (the cell list's sorted order).
+// particles ascending (the cell list's sorted order).  When f gives more than
+// kMaxCoarse non-empty aggregates, f grows by one until the count fits
+// (C.factor records the value used), so a large tank keeps a coarser space
+// instead of none.
 void coarse_aggregate(Context& ctx, CoarseSpace& C, size_t n, const int32_t* labels,
                       const int32_t* cell_sorted, const int32_t* cell_begin,
                       const int32_t* cell_end, const int32_t dims[3], int factor);
 
-// Ac = P^T A P and its inverse, launched on stream st (no host wait): diag and
-// offabs are A's row diagonals and off-diagonal magnitude sums (they bound the
-// fixed-point rows).  Leaves C.ready false when the aggregation is empty or too
-// large; a zero pivot sets C.status, after which every coarse product is 0.
+// Reserves every per-solve buffer of coarse_setup / coarse_set_weights for the
+// current aggregation (the only device allocations of the coarse solve).
+void coarse_reserve(CoarseSpace& C);
+// Ac = P^T A P and its inverse, launched on stream st (no host wait, no device
+// allocation: coarse_reserve runs first): diag and offabs are A's row
+// diagonals and off-diagonal magnitude sums (they bound the fixed-point rows).
+// Leaves C.ready false when the aggregation is empty; a zero pivot sets
+// C.status, after which every coarse product is 0.
 void coarse_setup(Context& ctx, CoarseSpace& C, const lsmps_csr& A, const double* diag,
                   const double* offabs, hipStream_t st);
 // wfac = 1 / pdiag (the fine preconditioner's diagonal part), on ctx's stream.

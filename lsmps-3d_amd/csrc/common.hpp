@@ -38,6 +38,21 @@ inline void require(bool ok, const char* message, int code = LSMPS_ERR_INVALID_A
 
 #define LSMPS_LAUNCH_CHECK() LSMPS_HIP(hipGetLastError())
 
+// Regions of host code that must not allocate or free device memory (work
+// enqueued there may still read every buffer it was given, and a stream being
+// captured into a graph cannot allocate): while an AllocationFreeze lives on
+// this thread, any DeviceArray reallocation raises LSMPS_ERR_INTERNAL instead.
+inline int& allocation_freeze_depth() {
+  thread_local int depth = 0;
+  return depth;
+}
+struct AllocationFreeze {
+  AllocationFreeze() { ++allocation_freeze_depth(); }
+  ~AllocationFreeze() { --allocation_freeze_depth(); }
+  AllocationFreeze(const AllocationFreeze&) = delete;
+  AllocationFreeze& operator=(const AllocationFreeze&) = delete;
+};
+
 // Move-only owner of one device allocation.  Reallocates only when the
 // requested element count grows (reserve) or changes (resize).
 template <typename T>
@@ -62,6 +77,8 @@ class DeviceArray {
   }
   void resize(size_t n) {
     if (n == n_) return;
+    if (allocation_freeze_depth() > 0)
+      raise(LSMPS_ERR_INTERNAL, "device allocation inside an allocation-free region");
     release();
     if (n > 0) LSMPS_HIP(hipMalloc(reinterpret_cast<void**>(&ptr_), n * sizeof(T)));
     n_ = n;
@@ -115,6 +132,7 @@ struct Context {
   DeviceArray<int32_t> tmp_primary;  // primary surface labels before the near-surface pass
   DeviceArray<char> tmp_points;      // sorted source points for the neighbor scan
   DeviceArray<double> reduce_scratch;  // block maxima of lsmps_max_velocity
+  DeviceArray<double> view_ops;        // operator rows read from caller moment views
   int32_t* host_i32 = nullptr;   // pinned host scratch
   double* host_f64 = nullptr;
   // PPE SpMV timing (LSMPS_FLAG_KERNEL_TIMING)

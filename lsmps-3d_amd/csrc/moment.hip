@@ -534,4 +534,33 @@ void moment_adjoint(Context& ctx, int kind, const lsmps_config& cfg, const lsmps
   LSMPS_LAUNCH_CHECK();
 }
 
+
+namespace {
+// Operator rows from an explicit inverse (MomentMatrixView::inverse_matrices,
+// column-major per particle): rows 0, 1, 2 and the row sum 3 + 4 + 5, in the
+// order compute_pressure_laplacian_row adds them (ppe_matrix.cu:203-215).
+__global__ void __launch_bounds__(kBlock) view_rows_kernel(size_t n, int size,
+                                                           const double* __restrict__ inverse,
+                                                           double* __restrict__ ops) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double* m = inverse + i * static_cast<size_t>(size) * size;
+  for (int k = 0; k < 9; ++k) {
+    const double* col = m + static_cast<size_t>(k) * size;
+    ops[static_cast<size_t>(k) * n + i] = col[0];
+    ops[static_cast<size_t>(9 + k) * n + i] = col[1];
+    ops[static_cast<size_t>(18 + k) * n + i] = col[2];
+    ops[static_cast<size_t>(kOpLap + k) * n + i] = col[3] + col[4] + col[5];
+  }
+}
+}  // namespace
+
+void operator_rows_from_view(Context& ctx, const lsmps_moment_view& view, size_t n,
+                             double* ops) {
+  if (n == 0) return;
+  view_rows_kernel<<<blocks_for(n), kBlock, 0, ctx.stream>>>(n, view.matrix_size,
+                                                              view.inverse_matrices, ops);
+  LSMPS_LAUNCH_CHECK();
+}
+
 }  // namespace lsmps

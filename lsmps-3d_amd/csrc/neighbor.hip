@@ -395,7 +395,9 @@ void exclusive_scan_i32(Context& ctx, const int32_t* in, int32_t* out, size_t n)
     if (tiles <= 1) break;
     m = tiles;
   }
-  ctx.scan_partials.reserve(need + 16);
+  // one allocation of 2^20 partials covers every n below 2^31 (need ~ n / 2047),
+  // so the buffer never grows between scans already enqueued
+  ctx.scan_partials.reserve(std::max<size_t>(need + 16, size_t(1) << 20));
   scan_level(ctx, in, out, n, ctx.scan_partials.data(), true);
 }
 
@@ -453,6 +455,15 @@ void build_cell_list(Context& ctx, size_t count, const double* x, const double* 
 }
 
 namespace {
+// The sorted source points of both tables share one context buffer: it is
+// sized for the larger source before the fluid table is built, so it never
+// grows (and frees the fluid points) while the fluid write pass may still be
+// reading them.
+void reserve_points(Context& ctx, size_t fluid_count, size_t wall_count) {
+  ctx.tmp_points.reserve(std::max<size_t>(std::max(fluid_count, wall_count), 1) *
+                         sizeof(SortedPoint));
+}
+
 // build_one_neighbor_list (neighbor_search.cu:294-333)
 void one_neighbor_list(Context& ctx, const lsmps_fluid_soa& fluid, size_t source_count,
                        const double* sx, const double* sy, const double* sz,
@@ -469,7 +480,8 @@ void one_neighbor_list(Context& ctx, const lsmps_fluid_soa& fluid, size_t source
     return;
   }
   const GridDev g = to_dev(grid);
-  ctx.tmp_points.reserve(std::max<size_t>(source_count, 1) * sizeof(SortedPoint));
+  require(ctx.tmp_points.size() >= std::max<size_t>(source_count, 1) * sizeof(SortedPoint),
+          "neighbor scan before reserve_points", LSMPS_ERR_INTERNAL);
   SortedPoint* points = reinterpret_cast<SortedPoint*>(ctx.tmp_points.data());
   if (source_count > 0) {
     gather_points_kernel<<<blocks_for(source_count), kBlock, 0, ctx.stream>>>(
@@ -533,6 +545,7 @@ void build_neighbor_lists(Context& ctx, const lsmps_fluid_soa& fluid, const lsmp
   require(radius <= grid.cell_size, "Neighbor search currently requires radius <= cell_size");
   build_cell_list(ctx, fluid.count, fluid.x, fluid.y, fluid.z, grid, fluid_cells);
   build_cell_list(ctx, walls.count, walls.x, walls.y, walls.z, grid, wall_cells);
+  reserve_points(ctx, fluid.count, walls.count);
   one_neighbor_list(ctx, fluid, fluid.count, fluid.x, fluid.y, fluid.z, fluid_cells, fluid_cells,
                     grid, nullptr, nullptr, radius, true, fluid_neighbors, fluid_total);
   one_neighbor_list(ctx, fluid, walls.count, walls.x, walls.y, walls.z, wall_cells, fluid_cells,
@@ -560,6 +573,7 @@ void build_neighbor_lists_segmented(Context& ctx, const lsmps_fluid_soa& fluid,
                  segments.fluid_seg, segments.cells, fluid_cells);
   cell_list_impl(ctx, walls.count, walls.x, walls.y, walls.z, none, segments.grids,
                  segments.wall_seg, segments.cells, wall_cells);
+  reserve_points(ctx, fluid.count, walls.count);
   lsmps_cell_grid unused{};
   unused.cell_size = 1.0;
   unused.dims[0] = unused.dims[1] = unused.dims[2] = 1;

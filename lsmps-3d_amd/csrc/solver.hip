@@ -35,7 +35,6 @@
 #include <vector>
 
 #include <exception>
-#include <thread>
 
 #include "common.hpp"
 #include "solver.hpp"
@@ -44,12 +43,6 @@
 namespace lsmps {
 
 PpeSolver::~PpeSolver() {
-  if (aux) {
-    (void)hipStreamSynchronize(aux);
-    (void)hipEventDestroy(ev_fork);
-    (void)hipEventDestroy(ev_join);
-    (void)hipStreamDestroy(aux);
-  }
   if (host) (void)hipHostFree(host);
   if (seg_host) (void)hipHostFree(seg_host);
 }
@@ -2151,40 +2144,8 @@ void ppe_solve(Context& ctx, PpeSolver& S, const lsmps_csr& A, const double* b, 
     double rr = 0.0;
     S.schwarz.blocks = 0;
     S.coarse.ready = false;
-    bool forked = false;  // the coarse setup runs on S.aux (joined before the loop)
-    std::thread coarse_thread;
-    std::exception_ptr coarse_error;
-    struct Joiner {
-      std::thread& t;
-      ~Joiner() {
-        if (t.joinable()) t.join();
-      }
-    } joiner{coarse_thread};
     if (opt.preconditioner == LSMPS_PRECOND_SCHWARZ ||
         opt.preconditioner == LSMPS_PRECOND_SCHWARZ_COARSE) {
-      // The coarse space (it needs the caller's aggregation of these rows, one
-      // system) is set up on a second stream, overlapping the Schwarz setup and
-      // the SpMV layout build; the IDR loop waits for it.  A second host thread
-      // enqueues it: the runtime holds an enqueueing thread back while many
-      // launches are outstanding on a stream, and the Gauss-Jordan chain has
-      // two launches per panel.
-      const bool want_coarse = opt.preconditioner == LSMPS_PRECOND_SCHWARZ_COARSE &&
-                               S.segments.count == 0 && S.coarse.count > 1 && S.coarse.n == n;
-      if (want_coarse) {
-        S.ensure_aux();
-        LSMPS_HIP(hipEventRecord(S.ev_fork, ctx.stream));
-        LSMPS_HIP(hipStreamWaitEvent(S.aux, S.ev_fork, 0));
-        coarse_thread = std::thread([&ctx, &S, &A, &coarse_error] {
-          try {
-            ctx.activate();
-            coarse_setup(ctx, S.coarse, A, S.diag64.data(), S.offabs.data(), S.aux);
-            LSMPS_HIP(hipEventRecord(S.ev_join, S.aux));
-          } catch (...) {
-            coarse_error = std::current_exception();
-          }
-        });
-        forked = true;
-      }
       schwarz_setup(ctx, S.schwarz, A, S.diag64.data(), S.dinv64.data(), S.offabs.data(),
                     ctx.precision == LSMPS_PRECISION_MIXED);
       int64_t blocks = 0, rows = 0;
@@ -2203,15 +2164,15 @@ void ppe_solve(Context& ctx, PpeSolver& S, const lsmps_csr& A, const double* b, 
     } else {
       build_sell<double>(ctx, S.sell, A, nullptr);
     }
-    if (forked) {
-      coarse_thread.join();
-      if (coarse_error) std::rethrow_exception(coarse_error);
-      if (S.coarse.ready && S.schwarz.n == n) {
-        coarse_set_weights(ctx, S.coarse, S.schwarz.pdiag.data());
-      } else {
-        S.coarse.ready = false;
-      }
-      LSMPS_HIP(hipStreamWaitEvent(ctx.stream, S.ev_join, 0));
+    // The coarse space (it needs the caller's aggregation of these rows, one
+    // system) is set up on the library stream after the Schwarz blocks and the
+    // SpMV layout: every buffer it touches is reserved first, on this thread,
+    // and nothing is enqueued on another stream.
+    if (coarse_requested && S.schwarz.n == n) {
+      coarse_reserve(S.coarse);
+      const AllocationFreeze frozen;  // no device allocation until the loop starts
+      coarse_setup(ctx, S.coarse, A, S.diag64.data(), S.offabs.data(), ctx.stream);
+      if (S.coarse.ready) coarse_set_weights(ctx, S.coarse, S.schwarz.pdiag.data());
     }
     if (S.segments.count > 0) {
       // independent systems stacked in one matrix: per-segment recurrences

@@ -134,17 +134,6 @@ class PpeSolver {
   DeviceArray<double> part_b;
   DeviceArray<SolverScalars> scalars;
   double* host = nullptr;      // pinned: [0] rr, [1] bnorm2, ...
-  // second stream for setup work that overlaps the main stream (default
-  // priority), with its fork / join events
-  hipStream_t aux = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-  void ensure_aux() {
-    if (aux) return;
-    LSMPS_HIP(hipStreamCreateWithFlags(&aux, hipStreamNonBlocking));
-    LSMPS_HIP(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
-    LSMPS_HIP(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
-  }
-
   PpeSolver() = default;
   ~PpeSolver();
   PpeSolver(const PpeSolver&) = delete;

@@ -70,6 +70,11 @@ void moment_adjoint(Context& ctx, int kind, const lsmps_config& cfg, const lsmps
                     const lsmps_neighbor_list& wn, const double* ops_bar, double* pair_grad,
                     size_t pair_stride, double* xbar);
 
+// Operator rows (kOpRows x n, the consumers' layout) read from the explicit
+// inverses of a Type-A MomentMatrixView.
+void operator_rows_from_view(Context& ctx, const lsmps_moment_view& view, size_t n,
+                             double* ops);
+
 // ---- provision.hip
 // velocity Laplacian (rows of the velocity Type-A operator) fused with the
 // explicit update u* = u + dt (nu_eff lap + g); `lap` may be null.

@@ -38,6 +38,7 @@ EXPORTS = [
     "lsmps_stepper_refresh_geometry", "lsmps_default_config_files", "lsmps_config_validate",
     "lsmps_config_load", "lsmps_config_save", "lsmps_read_fluid_csv", "lsmps_read_wall_csv",
     "lsmps_vtk_path", "lsmps_write_vtk", "lsmps_compute_virtual_light_diagnostics",
+    "lsmps_ppe_assemble_views", "lsmps_compute_pressure_gradient_view",
 ]
 
 _LIB = None

@@ -1779,6 +1779,18 @@ int orc_particle_shifting(const lsmps_config* cfg, const lsmps_fluid_soa* fluid,
   return LSMPS_OK;
 }
 
+int orc_anti_penetration(const lsmps_config* cfg, const lsmps_fluid_soa* fluid,
+                         const lsmps_wall_soa* walls, const lsmps_neighbor_list* wn) {
+  anti_penetration(*cfg, *fluid, *walls, *wn);
+  return LSMPS_OK;
+}
+
+int orc_smooth_velocity(const lsmps_config* cfg, const lsmps_fluid_soa* fluid,
+                        const lsmps_neighbor_list* fn, const lsmps_vector3* smoothed) {
+  smooth_velocity(*cfg, *fluid, *fn, *smoothed);
+  return LSMPS_OK;
+}
+
 // Stage timings of the last orc_step call (seconds): geometry, moments,
 // provision, assembly, solve, correction.
 static double g_stage_seconds[6];

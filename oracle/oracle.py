@@ -349,6 +349,47 @@ class Oracle:
                                                C.byref(nb), C.byref(nr)), "schwarz_apply")
         return z, nb.value, nr.value
 
+    def particle_shifting(self, case, nb, config=None):
+        """compute_particle_shifting_kernel alone; returns [ps_x, ps_y, ps_z]."""
+        a = self.abi
+        cfg = config or case.config
+        n = case.n_fluid
+        ps = [np.zeros(n) for _ in range(3)]
+        v3 = a.Vector3(n, *(a.dptr(_p(x)) for x in ps))
+        fs = self.fluid_soa(case.fluid)
+        ws = self.wall_soa(case.walls)
+        fn = self.nlist(nb["fluid_offsets"], nb["fluid_indices"])
+        wn = self.nlist(nb["wall_offsets"], nb["wall_indices"])
+        self._check(self.lib.orc_particle_shifting(C.byref(cfg), C.byref(fs), C.byref(ws),
+                                                   C.byref(fn), C.byref(wn), C.byref(v3)),
+                    "particle_shifting")
+        return ps
+
+    def anti_penetration(self, case, nb, config=None):
+        """anti_penetration_kernel alone on a copy of the fluid; returns the new fluid dict."""
+        cfg = config or case.config
+        fluid = {k: v.copy() for k, v in case.fluid.items()}
+        fs = self.fluid_soa(fluid)
+        ws = self.wall_soa(case.walls)
+        wn = self.nlist(nb["wall_offsets"], nb["wall_indices"])
+        self._check(self.lib.orc_anti_penetration(C.byref(cfg), C.byref(fs), C.byref(ws),
+                                                  C.byref(wn)), "anti_penetration")
+        return fluid
+
+    def smooth_velocity(self, case, nb, config=None):
+        """smooth_velocity_kernel + copy on a copy of the fluid; returns (vx, vy, vz)."""
+        a = self.abi
+        cfg = config or case.config
+        n = case.n_fluid
+        fluid = {k: v.copy() for k, v in case.fluid.items()}
+        sm = [np.zeros(n) for _ in range(3)]
+        v3 = a.Vector3(n, *(a.dptr(_p(x)) for x in sm))
+        fs = self.fluid_soa(fluid)
+        fn = self.nlist(nb["fluid_offsets"], nb["fluid_indices"])
+        self._check(self.lib.orc_smooth_velocity(C.byref(cfg), C.byref(fs), C.byref(fn),
+                                                 C.byref(v3)), "smooth_velocity")
+        return fluid["vx"], fluid["vy"], fluid["vz"]
+
     def correction(self, case, nb, tmp, pressure, pre_inverse, config=None, surface_type=None):
         """DevicePressureCorrection::apply on a copy of the case's fluid; returns the new state."""
         a = self.abi

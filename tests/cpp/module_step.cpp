@@ -5,6 +5,9 @@
 // correction through the free functions (clamp, gradient, shifting, update,
 // anti-penetration, smoothing) on a second copy of the state, as
 // DevicePressureCorrection::apply sequences them (pressure_correction.cu:598-645).
+// The reference-signature free functions that read MomentMatrixViews
+// (assemble_ppe_matrix_and_rhs, compute_pressure_gradient) run beside the
+// module classes, and all four moment views are written out.
 //
 // Usage: module_step <dir>.  <dir>/sizes.txt holds "n_fluid n_wall
 // max_neighbors precision"; <dir>/config.bin the raw lsmps_config; <dir>/<name>.bin
@@ -155,6 +158,26 @@ int run() {
   const MomentMatrixView vview = moments.velocity_type_a();
   if (!vview.is_ready || vview.matrix_size != 9 || vview.particle_count < n)
     throw std::runtime_error("velocity view not ready");
+  // all four views (moment_matrix.cuh:60-63), taken before anything moves the
+  // particles: explicit inverses, traces, regularisations and factor status
+  const MomentMatrixView pview = moments.pressure_type_a();
+  const MomentMatrixView views[4] = {vview, pview, moments.pressure_type_b(),
+                                     moments.fluid_only_type_a()};
+  const char* view_names[4] = {"velocity_a", "pressure_a", "pressure_b", "fluid_only_a"};
+  for (int k = 0; k < 4; ++k) {
+    const MomentMatrixView& v = views[k];
+    if (!v.is_ready || v.particle_count < n || static_cast<int>(v.kind) != k)
+      throw std::runtime_error(std::string("view not ready: ") + view_names[k]);
+    const size_t nn = static_cast<size_t>(v.matrix_size) * v.matrix_size;
+    write_device(std::string("view_") + view_names[k] + "_inverse", v.inverse_matrices, n * nn);
+    write_device(std::string("view_") + view_names[k] + "_trace", v.moment_trace, n);
+    write_device(std::string("view_") + view_names[k] + "_regularization",
+                 v.regularization_added, n);
+    write_device(std::string("view_") + view_names[k] + "_info", v.info, n);
+    const int32_t sizes[2] = {v.matrix_size, static_cast<int32_t>(v.basis_kind)};
+    std::ofstream f(g_dir + "/out_view_" + view_names[k] + "_shape.bin", std::ios::binary);
+    f.write(reinterpret_cast<const char*>(sizes), sizeof(sizes));
+  }
 
   // provision (explicit_update.cuh:58-82)
   double* tmp = device_zeros<double>(3 * n);
@@ -170,6 +193,10 @@ int run() {
   DevicePpeMatrixAssembler assembler(n, n + cap, config);
   assembler.assemble(fluid, walls, fn, wn, temporary, temporary_wall, moments, generation);
   const PpeWorkspaceView ws = assembler.workspace();
+  // the reference's free assembly (ppe_matrix.cuh:120-129), reading the views
+  DevicePpeWorkspace view_ws(n, n + cap);
+  assemble_ppe_matrix_and_rhs(fluid, walls, config, fn, wn, temporary, temporary_wall, vview,
+                              pview, view_ws.view());
   AmgxPpeSolver solver;
   solver.solve(ws.matrix, ws.rhs, ws.pressure);
   std::printf("solve: %d iterations, relative residual %.3e, %d Schwarz blocks\n",
@@ -198,16 +225,28 @@ int run() {
   write_device("ppe_values", ws.matrix.values, static_cast<size_t>(used));
   write_device("ppe_rhs", ws.rhs, n);
   write_device("pressure", ws.pressure, n);
+  {
+    int32_t vused = 0;
+    detail::hip_check(hipMemcpy(&vused, view_ws.view().matrix.row_offsets + n, sizeof(int32_t),
+                                hipMemcpyDeviceToHost), "hipMemcpy");
+    write_device("view_ppe_row_offsets", view_ws.view().matrix.row_offsets, n + 1);
+    write_device("view_ppe_col_indices", view_ws.view().matrix.col_indices, static_cast<size_t>(vused));
+    write_device("view_ppe_values", view_ws.view().matrix.values, static_cast<size_t>(vused));
+    write_device("view_ppe_rhs", view_ws.view().rhs, n);
+  }
 
   // free-function replay of the correction on the second copy, from the same pressure
   double* p2 = device_zeros<double>(n);
   detail::hip_check(hipMemcpy(p2, ws.pressure, n * sizeof(double), hipMemcpyDeviceToDevice), "hipMemcpy");
-  double* vec2 = device_zeros<double>(9 * n);
+  double* vec2 = device_zeros<double>(12 * n);
   CorrectionVectorView grad2{n, vec2, vec2 + n, vec2 + 2 * n};
   CorrectionVectorView ps2{n, vec2 + 3 * n, vec2 + 4 * n, vec2 + 5 * n};
   CorrectionVectorView smooth2{n, vec2 + 6 * n, vec2 + 7 * n, vec2 + 8 * n};
   clamp_particle_pressure(fluid2, p2);
   compute_pressure_gradient(fluid2, walls, fn, wn, p2, config, moments, grad2);
+  // the reference's signature (pressure_correction.cuh:93-100), reading the view
+  CorrectionVectorView grad_view{n, vec2 + 9 * n, vec2 + 10 * n, vec2 + 11 * n};
+  compute_pressure_gradient(fluid2, walls, fn, wn, p2, config, pview, grad_view);
   compute_particle_shifting_displacement(fluid2, walls, fn, wn, config, ps2);
   update_velocity_and_position(fluid2, temporary, config, grad2, ps2);
   const lsmps_config cc = detail::to_c(config);
@@ -228,6 +267,7 @@ int run() {
   }
   write_device("replay_grad", vec2, 3 * n);
   write_device("replay_ps", vec2 + 3 * n, 3 * n);
+  write_device("view_grad", vec2 + 9 * n, 3 * n);
   return 0;
 }
 

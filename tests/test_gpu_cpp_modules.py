@@ -3,7 +3,10 @@ function names) driven by a compiled program, tests/cpp/module_step.cpp, that
 steps a case module by module as the reference driver does.  Its outputs are
 compared with the FP64 oracle (same bars as the fused-step parity tests), and
 the correction replayed through the free functions must equal
-DevicePressureCorrection::apply bit for bit."""
+DevicePressureCorrection::apply bit for bit.  The reference-signature free
+functions that read MomentMatrixViews, and the explicit inverses of all four
+moment kinds (velocity/pressure Type-A, pressure Type-B, fluid-only Type-A),
+are checked against the oracle as well."""
 import ctypes as C
 import os
 import subprocess
@@ -87,6 +90,33 @@ def test_cpp_module_api_fp64(abi, cases, oracle, tmp_path, idx):
     scale = max(np.linalg.norm(pv["vz"]), 1e-300)
     for k in ("vx", "vy", "vz"):
         assert np.linalg.norm(out(f"fluid_{k}") - cg["fluid"][k]) / scale <= 1e-8, k
+    # the reference-signature free functions reading MomentMatrixViews
+    # (ppe_matrix.cuh:120-129, pressure_correction.cuh:93-100)
+    assert np.array_equal(out("view_ppe_row_offsets", i32), pp["row_offsets"])
+    assert np.array_equal(out("view_ppe_col_indices", i32), pp["col_indices"])
+    assert rel_l2(out("view_ppe_values"), pp["values"]) <= 1e-9
+    assert rel_l2(out("view_ppe_rhs"), pp["rhs"]) <= 1e-9
+    vgrad = out("view_grad").reshape(3, n)
+    num = sum(np.sum((vgrad[k] - cg["grad"][k]) ** 2) for k in range(3))
+    assert np.sqrt(num / den) <= 1e-8
+    # all four moment views against the oracle's explicit inverses
+    c2 = case.copy()
+    c2.fluid["surface_type"] = ref["surface"]["surface_type"]
+    kinds = {"velocity_a": abi.MOMENT_VELOCITY_WALL_DIRICHLET_A,
+             "pressure_a": abi.MOMENT_PRESSURE_WALL_NEUMANN_A,
+             "pressure_b": abi.MOMENT_PRESSURE_WALL_NEUMANN_B,
+             "fluid_only_a": abi.MOMENT_FLUID_ONLY_A}
+    for name, kind in kinds.items():
+        mo = oracle.moments(c2, nb, kind)
+        size, basis = out(f"view_{name}_shape", i32)
+        assert size == mo["size"] and basis == (1 if name == "pressure_b" else 0), name
+        inv = out(f"view_{name}_inverse").reshape(n, size, size)
+        err = rel_l2(inv, mo["inverse"])
+        print(name, "inverse rel L2", err, flush=True)
+        assert err <= 1e-7, name
+        assert rel_l2(out(f"view_{name}_trace"), mo["trace"]) <= 1e-12, name
+        assert rel_l2(out(f"view_{name}_regularization"), mo["regularization"]) <= 1e-12, name
+        assert np.array_equal(out(f"view_{name}_info", i32), mo["info"]), name
     # the free functions in apply's order reproduce apply exactly
     for k in FLUID:
         assert np.array_equal(out(f"replay_fluid_{k}"), out(f"fluid_{k}")), k

@@ -217,6 +217,17 @@ def test_hydrostatic_program(rt, oracle, cases, abi, tmp_path):
     assert abs(_stat(out, "reference number density n0") - n0) <= 1e-5 * n0
     print("labels", counts.tolist(), "max |p - p_hydro|", float(np.max(np.abs(pressure - p_true))),
           flush=True)
+    # closed form: the hydrostatic field p = rho g (H - l0/2 - z) is linear in z,
+    # zero on the top (Surface, Dirichlet) layer and has dp/dn = rho g.n on the
+    # walls, so LSMPS reproduces it up to the lambda = 1e-8 regularisation
+    rho_g = 1000.0 * 9.81
+    p_scale = rho_g * surface_z
+    assert float(np.max(np.abs(pressure - p_true))) <= 1e-7 * p_scale
+    assert float(np.max(np.abs(R["ppe_pressure"] - p_true))) <= 1e-7 * p_scale
+    inner = (st == 0) | (st == 1)
+    assert float(np.max(np.abs(R["ppe_pressure_gradient_z"][inner] + rho_g))) <= 1e-5 * rho_g
+    for k in ("ppe_pressure_gradient_x", "ppe_pressure_gradient_y"):
+        assert float(np.max(np.abs(R[k][inner]))) <= 1e-5 * rho_g, k
 
 
 def test_pipe_flow_program(tmp_path):
