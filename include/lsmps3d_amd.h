@@ -296,6 +296,9 @@ typedef struct lsmps_stepper lsmps_stepper;
 /* No reference counterpart: the reference used the legacy default stream of the
  * current device implicitly.  `precision` is LSMPS_PRECISION_*. */
 int lsmps_context_create(int device, int precision, uint32_t flags, lsmps_context** out);
+/* Drops the caller's handle.  Moment matrices and steppers created on the
+ * context keep it alive, so objects and their context may be destroyed in any
+ * order; the context is freed with the last of them. */
 int lsmps_context_destroy(lsmps_context* ctx);
 int lsmps_context_set_stream(lsmps_context* ctx, void* hip_stream);
 int lsmps_context_synchronize(lsmps_context* ctx);

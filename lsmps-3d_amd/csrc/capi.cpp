@@ -34,7 +34,12 @@ int handle(const char* where) {
   }
 }
 
-#define API_BEGIN try {
+// Every entry starts by clearing the thread's last HIP error, so a launch check
+// inside it reports this call's work and not an error some other HIP user of
+// the thread (the caller, torch) left behind.
+#define API_BEGIN       \
+  (void)hipGetLastError(); \
+  try {
 #define API_END(name)      \
   }                        \
   catch (...) {            \
@@ -54,6 +59,24 @@ const T& deref(const T* p, const char* what) {
   return *p;
 }
 
+}  // namespace
+
+namespace {
+lsmps_context* retain_context(lsmps_context* ctx) {
+  ctx->refs.fetch_add(1);
+  return ctx;
+}
+
+void release_context(lsmps_context* ctx) {
+  if (!ctx || ctx->refs.fetch_sub(1) != 1) return;
+  (void)hipSetDevice(ctx->impl.device);
+  (void)hipStreamSynchronize(ctx->impl.stream);
+  if (ctx->impl.timing_ready)
+    for (auto& e : ctx->impl.timing_events) (void)hipEventDestroy(e);
+  if (ctx->impl.host_i32) (void)hipHostFree(ctx->impl.host_i32);
+  if (ctx->impl.host_f64) (void)hipHostFree(ctx->impl.host_f64);
+  delete ctx;
+}
 }  // namespace
 
 extern "C" {
@@ -89,13 +112,7 @@ int lsmps_context_create(int device, int precision, uint32_t flags, lsmps_contex
 int lsmps_context_destroy(lsmps_context* ctx) {
   API_BEGIN
   if (!ctx) return LSMPS_OK;
-  (void)hipSetDevice(ctx->impl.device);
-  (void)hipStreamSynchronize(ctx->impl.stream);
-  if (ctx->impl.timing_ready)
-    for (auto& e : ctx->impl.timing_events) (void)hipEventDestroy(e);
-  if (ctx->impl.host_i32) (void)hipHostFree(ctx->impl.host_i32);
-  if (ctx->impl.host_f64) (void)hipHostFree(ctx->impl.host_f64);
-  delete ctx;
+  release_context(ctx);
   API_END("lsmps_context_destroy")
 }
 
@@ -282,10 +299,12 @@ int lsmps_moment_create(lsmps_context* ctx, size_t capacity, const lsmps_config*
   API_BEGIN
   require(out != nullptr, "null output pointer");
   Context& c = C(ctx);
+  const auto& cfg = deref(config, "config");
   auto* m = new lsmps_moment_matrix();
   m->ctx = &c;
-  m->config = deref(config, "config");
-  m->resize(capacity);
+  m->config = cfg;
+  m->resize(capacity);  // records the capacity; allocates on first prepare
+  m->owner = retain_context(ctx);
   *out = m;
   API_END("lsmps_moment_create")
 }
@@ -295,7 +314,9 @@ int lsmps_moment_destroy(lsmps_moment_matrix* m) {
   if (m) {
     (void)hipSetDevice(m->ctx->device);
     (void)hipStreamSynchronize(m->ctx->stream);
+    lsmps_context* owner = m->owner;
     delete m;
+    release_context(owner);
   }
   API_END("lsmps_moment_destroy")
 }
@@ -653,6 +674,7 @@ int lsmps_stepper_create(lsmps_context* ctx, size_t fluid_capacity, size_t wall_
     delete s;
     throw;
   }
+  s->owner = retain_context(ctx);
   *out = s;
   API_END("lsmps_stepper_create")
 }
@@ -662,7 +684,9 @@ int lsmps_stepper_destroy(lsmps_stepper* s) {
   if (s) {
     (void)hipSetDevice(s->ctx->device);
     (void)hipStreamSynchronize(s->ctx->stream);
+    lsmps_context* owner = s->owner;
     delete s;
+    release_context(owner);
   }
   API_END("lsmps_stepper_destroy")
 }

@@ -7,6 +7,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <atomic>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -163,4 +164,9 @@ struct lsmps_context {
   // Workspace of the free lsmps_ppe_solve entry (created on first use; the
   // complete type is visible where contexts are destroyed).
   std::unique_ptr<lsmps::PpeSolver> solver;
+  // Owners: the caller's handle plus every moment matrix and stepper created
+  // on this context.  lsmps_context_destroy drops the caller's reference; the
+  // context is freed when the last object using it is destroyed, in whatever
+  // order the caller destroys them.
+  std::atomic<int> refs{1};
 };

@@ -13,6 +13,7 @@
 // from the inputs of the last prepare (which must not have changed since).
 struct lsmps_moment_matrix {
   lsmps::Context* ctx = nullptr;
+  lsmps_context* owner = nullptr;  // holds a reference (standalone objects only)
   lsmps_config config{};
   size_t capacity = 0;
   size_t count = 0;
@@ -67,6 +68,7 @@ struct lsmps_moment_matrix {
 // Owns every intermediate buffer of one time step.
 struct lsmps_stepper {
   lsmps::Context* ctx = nullptr;
+  lsmps_context* owner = nullptr;  // holds a reference
   lsmps_config config{};
   lsmps_solver_options solver_options{};
   size_t fluid_capacity = 0;

@@ -185,3 +185,23 @@ def test_unprepared_views_rejected(rt, abi, cases):
         assert rc == abi.LSMPS_ERR_NOT_PREPARED, (which, rc, msg)
         assert message in msg, (message, msg)
     ctx.close()
+
+
+def test_context_destroyed_before_its_objects(rt, abi, cases):
+    """A context destroyed before the stepper and moment matrix created on it
+    stays alive until they are destroyed; the next calls see no stale error."""
+    case = cases.hydrostatic(n=6, l0=0.01)
+    ctx = rt.Context(abi.PRECISION_FP64)
+    state = rt.DeviceState(case)
+    st = rt.Stepper(ctx, state, case.config, 256, abi.default_solver_options())
+    m = C.c_void_p()
+    rt.check(ctx.lib.lsmps_moment_create(ctx.handle, C.c_size_t(case.n_fluid),
+                                         C.byref(case.config), C.byref(m)), "moment_create")
+    lib = ctx.lib
+    ctx.close()  # the caller's handle first
+    rep = st.step()
+    assert rep["converged"] == 1
+    rt.check(lib.lsmps_moment_destroy(m), "moment_destroy")
+    st.close()
+    g = rt.run_step_on_device(case, precision=abi.PRECISION_FP64)
+    assert g["report"]["converged"] == 1
