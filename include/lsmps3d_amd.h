@@ -333,6 +333,20 @@ int lsmps_build_neighbor_lists(lsmps_context* ctx, const lsmps_fluid_soa* fluid,
                                const lsmps_cell_list* fluid_cells, const lsmps_cell_list* wall_cells,
                                const lsmps_neighbor_list* fluid_neighbors,
                                const lsmps_neighbor_list* wall_neighbors);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the same tables as lsmps_build_neighbor_lists, bit for bit,
+ * with every scratch buffer in a static per-GPU device array instead of the
+ * context's buffers; it calls no hipMalloc / hipFree.  Limits: at most 2^20
+ * fluid and 2^20 wall particles and 2^20 cells (LSMPS_ERR_CAPACITY beyond, before
+ * any kernel runs).  All calls on one GPU share that GPU's array: calls that
+ * overlap on one GPU race. */
+int lsmps_build_neighbor_lists_noalloc(lsmps_context* ctx, const lsmps_fluid_soa* fluid,
+                                       const lsmps_wall_soa* walls, const lsmps_config* config,
+                                       const lsmps_cell_list* fluid_cells,
+                                       const lsmps_cell_list* wall_cells,
+                                       const lsmps_neighbor_list* fluid_neighbors,
+                                       const lsmps_neighbor_list* wall_neighbors);
 
 /* ---- surface (include/lsmps3d/surface/surface_detection.cuh) ------------ */
 /* compute_uniform_reference_number_density(l0, re) — surface_detection.cuh:28-29 (host) */

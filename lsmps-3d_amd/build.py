@@ -20,6 +20,7 @@ ARCH = os.environ.get("LSMPS_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = {
     "neighbor.hip": ["-ffp-contract=off"],
+    "noalloc.hip": ["-ffp-contract=off"],
     "surface.hip": ["-ffp-contract=off"],
     "moment.hip": ["-ffp-contract=fast"],
     "provision.hip": ["-ffp-contract=fast"],

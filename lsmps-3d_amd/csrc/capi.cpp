@@ -101,6 +101,7 @@ int lsmps_context_create(int device, int precision, uint32_t flags, lsmps_contex
     LSMPS_HIP(hipSetDevice(device));
     LSMPS_HIP(hipHostMalloc(reinterpret_cast<void**>(&ctx->impl.host_i32), 64));
     LSMPS_HIP(hipHostMalloc(reinterpret_cast<void**>(&ctx->impl.host_f64), 64));
+    noalloc_prepare_device(device);
   } catch (...) {
     delete ctx;
     throw;
@@ -256,6 +257,21 @@ int lsmps_build_neighbor_lists(lsmps_context* ctx, const lsmps_fluid_soa* fluid,
                        nullptr);
   c.finish();
   API_END("lsmps_build_neighbor_lists")
+}
+
+int lsmps_build_neighbor_lists_noalloc(lsmps_context* ctx, const lsmps_fluid_soa* fluid,
+                                       const lsmps_wall_soa* walls, const lsmps_config* config,
+                                       const lsmps_cell_list* fc, const lsmps_cell_list* wc,
+                                       const lsmps_neighbor_list* fn,
+                                       const lsmps_neighbor_list* wn) {
+  API_BEGIN
+  Context& c = C(ctx);
+  build_neighbor_lists_noalloc(c, deref(fluid, "fluid"), deref(walls, "walls"),
+                               deref(config, "config"), deref(fc, "fluid cells"),
+                               deref(wc, "wall cells"), deref(fn, "fluid neighbors"),
+                               deref(wn, "wall neighbors"));
+  c.finish();
+  API_END("lsmps_build_neighbor_lists_noalloc")
 }
 
 // ---- surface ----------------------------------------------------------------

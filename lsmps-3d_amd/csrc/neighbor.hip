@@ -29,6 +29,7 @@ struct alignas(32) SortedPoint {
   int32_t index;
   int32_t pad;
 };
+static_assert(sizeof(SortedPoint) == kSortedPointBytes, "sorted point record size");
 
 struct GridDev {
   double ox, oy, oz, h;
@@ -383,41 +384,73 @@ void scan_level(Context& ctx, const int32_t* in, int32_t* out, size_t n, int32_t
 }
 }  // namespace
 
-void exclusive_scan_i32(Context& ctx, const int32_t* in, int32_t* out, size_t n) {
-  if (n == 0) {
-    fill_i32(ctx.stream, out, 1, 0);
-    return;
-  }
-  size_t need = 0;
+size_t scan_partials_needed(size_t n) {
+  size_t need = 16;
   for (size_t m = n; m > 1;) {
     const size_t tiles = (m + kScanTile - 1) / kScanTile;
     need += tiles + 1;
     if (tiles <= 1) break;
     m = tiles;
   }
+  return need;
+}
+
+void exclusive_scan_i32_with(Context& ctx, const int32_t* in, int32_t* out, size_t n,
+                             int32_t* partials, size_t partials_capacity) {
+  if (n == 0) {
+    fill_i32(ctx.stream, out, 1, 0);
+    return;
+  }
+  require(scan_partials_needed(n) <= partials_capacity, "scan partials buffer too small",
+          LSMPS_ERR_INTERNAL);
+  scan_level(ctx, in, out, n, partials, true);
+}
+
+void exclusive_scan_i32(Context& ctx, const int32_t* in, int32_t* out, size_t n) {
   // one allocation of 2^20 partials covers every n below 2^31 (need ~ n / 2047),
   // so the buffer never grows between scans already enqueued
-  ctx.scan_partials.reserve(std::max<size_t>(need + 16, size_t(1) << 20));
-  scan_level(ctx, in, out, n, ctx.scan_partials.data(), true);
+  ctx.scan_partials.reserve(std::max<size_t>(scan_partials_needed(n), size_t(1) << 20));
+  exclusive_scan_i32_with(ctx, in, out, n, ctx.scan_partials.data(), ctx.scan_partials.size());
+}
+
+GeometryScratch context_geometry_scratch(Context& ctx, size_t cells, size_t fluid_count,
+                                         size_t wall_count) {
+  const size_t most = std::max<size_t>(std::max(fluid_count, wall_count), 1);
+  ctx.tmp_counts.reserve(cells + 1);
+  ctx.tmp_cursor.reserve(cells + 1);
+  ctx.tmp_labels.reserve(most);
+  ctx.tmp_points.reserve(most * kSortedPointBytes);
+  ctx.scan_partials.reserve(std::max<size_t>(scan_partials_needed(std::max(most, cells)),
+                                             size_t(1) << 20));
+  GeometryScratch g;
+  g.counts = ctx.tmp_counts.data();
+  g.cursor = ctx.tmp_cursor.data();
+  g.cells_capacity = ctx.tmp_counts.size();
+  g.labels = ctx.tmp_labels.data();
+  g.particles_capacity = std::min(ctx.tmp_labels.size(), ctx.tmp_points.size() / kSortedPointBytes);
+  g.points = ctx.tmp_points.data();
+  g.partials = ctx.scan_partials.data();
+  g.partials_capacity = ctx.scan_partials.size();
+  return g;
 }
 
 namespace {
 // Counting sort of `count` particles into `cc` cells (one grid, or per-particle
 // segment grids when `grids` is set).
-void cell_list_impl(Context& ctx, size_t count, const double* x, const double* y,
-                    const double* z, const GridDev& g, const SegmentGrid* grids,
-                    const int32_t* seg, size_t cc, const lsmps_cell_list& cells) {
+void cell_list_impl(Context& ctx, const GeometryScratch& sc, size_t count, const double* x,
+                    const double* y, const double* z, const GridDev& g,
+                    const SegmentGrid* grids, const int32_t* seg, size_t cc,
+                    const lsmps_cell_list& cells) {
   fill_i32(ctx.stream, cells.cell_begin, cc, -1);
   fill_i32(ctx.stream, cells.cell_end, cc, -1);
   if (count == 0) return;
   require(x && y && z && cells.cell_ids && cells.sorted_indices && cells.cell_begin && cells.cell_end,
           "Cell list requires position and cell buffers");
-  ctx.tmp_counts.reserve(cc + 1);
-  ctx.tmp_cursor.reserve(cc + 1);
-  ctx.tmp_labels.reserve(count);
-  int32_t* counts = ctx.tmp_counts.data();
-  int32_t* starts = ctx.tmp_cursor.data();
-  int32_t* cell_of = ctx.tmp_labels.data();
+  require(sc.cells_capacity >= cc + 1 && sc.particles_capacity >= count,
+          "cell list scratch too small", LSMPS_ERR_INTERNAL);
+  int32_t* counts = sc.counts;
+  int32_t* starts = sc.cursor;
+  int32_t* cell_of = sc.labels;
   fill_i32(ctx.stream, counts, cc, 0);
   if (grids) {
     assign_cells_kernel<true><<<blocks_for(count), kBlock, 0, ctx.stream>>>(
@@ -427,7 +460,7 @@ void cell_list_impl(Context& ctx, size_t count, const double* x, const double* y
         count, x, y, z, g, nullptr, nullptr, cell_of, counts);
   }
   LSMPS_LAUNCH_CHECK();
-  exclusive_scan_i32(ctx, counts, starts, cc);
+  exclusive_scan_i32_with(ctx, counts, starts, cc, sc.partials, sc.partials_capacity);
   fill_i32(ctx.stream, counts, cc, 0);
   scatter_cells_kernel<<<blocks_for(count), kBlock, 0, ctx.stream>>>(count, cell_of, starts,
                                                                      counts, cells.sorted_indices);
@@ -438,8 +471,8 @@ void cell_list_impl(Context& ctx, size_t count, const double* x, const double* y
 }
 }  // namespace
 
-void build_cell_list(Context& ctx, size_t count, const double* x, const double* y,
-                     const double* z, const lsmps_cell_grid& grid, const lsmps_cell_list& cells) {
+namespace {
+void check_cell_list(size_t count, const lsmps_cell_grid& grid, const lsmps_cell_list& cells) {
   // validation messages follow build_cell_list_impl (neighbor_search.cu:257-273)
   require(grid.cell_size > 0.0 && grid.dims[0] > 0 && grid.dims[1] > 0 && grid.dims[2] > 0,
           "Invalid cell grid configuration");
@@ -450,22 +483,22 @@ void build_cell_list(Context& ctx, size_t count, const double* x, const double* 
           "Cell list particle capacity is smaller than particle count");
   require(count <= static_cast<size_t>(INT32_MAX), "Particle count exceeds 32-bit index capacity");
   require(cell_count < static_cast<int64_t>(INT32_MAX), "Cell count exceeds 32-bit index capacity");
-  cell_list_impl(ctx, count, x, y, z, to_dev(grid), nullptr, nullptr,
-                 static_cast<size_t>(cell_count), cells);
+}
+
+}  // namespace
+
+void build_cell_list(Context& ctx, size_t count, const double* x, const double* y,
+                     const double* z, const lsmps_cell_grid& grid, const lsmps_cell_list& cells) {
+  check_cell_list(count, grid, cells);
+  const size_t cc = static_cast<size_t>(grid.dims[0]) * grid.dims[1] * grid.dims[2];
+  const GeometryScratch sc = context_geometry_scratch(ctx, cc, count, 0);
+  cell_list_impl(ctx, sc, count, x, y, z, to_dev(grid), nullptr, nullptr, cc, cells);
 }
 
 namespace {
-// The sorted source points of both tables share one context buffer: it is
-// sized for the larger source before the fluid table is built, so it never
-// grows (and frees the fluid points) while the fluid write pass may still be
-// reading them.
-void reserve_points(Context& ctx, size_t fluid_count, size_t wall_count) {
-  ctx.tmp_points.reserve(std::max<size_t>(std::max(fluid_count, wall_count), 1) *
-                         sizeof(SortedPoint));
-}
-
 // build_one_neighbor_list (neighbor_search.cu:294-333)
-void one_neighbor_list(Context& ctx, const lsmps_fluid_soa& fluid, size_t source_count,
+void one_neighbor_list(Context& ctx, const GeometryScratch& sc, const lsmps_fluid_soa& fluid,
+                       size_t source_count,
                        const double* sx, const double* sy, const double* sz,
                        const lsmps_cell_list& source_cells, const lsmps_cell_list& fluid_cells,
                        const lsmps_cell_grid& grid, const SegmentGrid* grids,
@@ -480,9 +513,9 @@ void one_neighbor_list(Context& ctx, const lsmps_fluid_soa& fluid, size_t source
     return;
   }
   const GridDev g = to_dev(grid);
-  require(ctx.tmp_points.size() >= std::max<size_t>(source_count, 1) * sizeof(SortedPoint),
-          "neighbor scan before reserve_points", LSMPS_ERR_INTERNAL);
-  SortedPoint* points = reinterpret_cast<SortedPoint*>(ctx.tmp_points.data());
+  require(sc.particles_capacity >= source_count, "neighbor scan scratch too small",
+          LSMPS_ERR_INTERNAL);
+  SortedPoint* points = reinterpret_cast<SortedPoint*>(sc.points);
   if (source_count > 0) {
     gather_points_kernel<<<blocks_for(source_count), kBlock, 0, ctx.stream>>>(
         source_count, source_cells.sorted_indices, sx, sy, sz, points);
@@ -501,7 +534,8 @@ void one_neighbor_list(Context& ctx, const lsmps_fluid_soa& fluid, size_t source
         exclude_self ? 1 : 0, nbrs.offsets, nullptr);
   }
   LSMPS_LAUNCH_CHECK();
-  exclusive_scan_i32(ctx, nbrs.offsets, nbrs.offsets, fluid.count);
+  exclusive_scan_i32_with(ctx, nbrs.offsets, nbrs.offsets, fluid.count, sc.partials,
+                          sc.partials_capacity);
   // Capacity check (require_neighbor_capacity, neighbor_search.cu:22-30,323-328).
   LSMPS_HIP(hipMemcpyAsync(ctx.host_i32, nbrs.offsets + fluid.count, sizeof(int32_t),
                            hipMemcpyDeviceToHost, ctx.stream));
@@ -528,28 +562,54 @@ void one_neighbor_list(Context& ctx, const lsmps_fluid_soa& fluid, size_t source
 }
 }  // namespace
 
-void build_neighbor_lists(Context& ctx, const lsmps_fluid_soa& fluid, const lsmps_wall_soa& walls,
-                          const lsmps_config& config, const lsmps_cell_list& fluid_cells,
-                          const lsmps_cell_list& wall_cells,
-                          const lsmps_neighbor_list& fluid_neighbors,
-                          const lsmps_neighbor_list& wall_neighbors, int32_t* fluid_total,
-                          int32_t* wall_total) {
+lsmps_cell_grid neighbor_grid(const lsmps_config& config) {
   lsmps_cell_grid grid;
   for (int k = 0; k < 3; ++k) {
     grid.origin[k] = config.cell_origin[k];
     grid.dims[k] = config.cell_dims[k];
   }
   grid.cell_size = config.cell_size;
+  return grid;
+}
+
+void build_neighbor_lists(Context& ctx, const lsmps_fluid_soa& fluid, const lsmps_wall_soa& walls,
+                          const lsmps_config& config, const lsmps_cell_list& fluid_cells,
+                          const lsmps_cell_list& wall_cells,
+                          const lsmps_neighbor_list& fluid_neighbors,
+                          const lsmps_neighbor_list& wall_neighbors, int32_t* fluid_total,
+                          int32_t* wall_total) {
+  const lsmps_cell_grid grid = neighbor_grid(config);
+  const size_t cc = static_cast<size_t>(std::max(grid.dims[0], 1)) * std::max(grid.dims[1], 1) *
+                    std::max(grid.dims[2], 1);
+  const GeometryScratch sc = context_geometry_scratch(ctx, cc, fluid.count, walls.count);
+  build_neighbor_lists_with(ctx, sc, fluid, walls, config, fluid_cells, wall_cells,
+                            fluid_neighbors, wall_neighbors, fluid_total, wall_total);
+}
+
+void build_neighbor_lists_with(Context& ctx, const GeometryScratch& sc,
+                               const lsmps_fluid_soa& fluid, const lsmps_wall_soa& walls,
+                               const lsmps_config& config, const lsmps_cell_list& fluid_cells,
+                               const lsmps_cell_list& wall_cells,
+                               const lsmps_neighbor_list& fluid_neighbors,
+                               const lsmps_neighbor_list& wall_neighbors, int32_t* fluid_total,
+                               int32_t* wall_total) {
+  const lsmps_cell_grid grid = neighbor_grid(config);
   const double radius = config.support_radius;
   require(radius > 0.0, "Neighbor search radius must be positive");
   require(radius <= grid.cell_size, "Neighbor search currently requires radius <= cell_size");
-  build_cell_list(ctx, fluid.count, fluid.x, fluid.y, fluid.z, grid, fluid_cells);
-  build_cell_list(ctx, walls.count, walls.x, walls.y, walls.z, grid, wall_cells);
-  reserve_points(ctx, fluid.count, walls.count);
-  one_neighbor_list(ctx, fluid, fluid.count, fluid.x, fluid.y, fluid.z, fluid_cells, fluid_cells,
-                    grid, nullptr, nullptr, radius, true, fluid_neighbors, fluid_total);
-  one_neighbor_list(ctx, fluid, walls.count, walls.x, walls.y, walls.z, wall_cells, fluid_cells,
-                    grid, nullptr, nullptr, radius, false, wall_neighbors, wall_total);
+  check_cell_list(fluid.count, grid, fluid_cells);
+  check_cell_list(walls.count, grid, wall_cells);
+  const size_t cc = static_cast<size_t>(grid.dims[0]) * grid.dims[1] * grid.dims[2];
+  cell_list_impl(ctx, sc, fluid.count, fluid.x, fluid.y, fluid.z, to_dev(grid), nullptr, nullptr,
+                 cc, fluid_cells);
+  cell_list_impl(ctx, sc, walls.count, walls.x, walls.y, walls.z, to_dev(grid), nullptr, nullptr,
+                 cc, wall_cells);
+  one_neighbor_list(ctx, sc, fluid, fluid.count, fluid.x, fluid.y, fluid.z, fluid_cells,
+                    fluid_cells, grid, nullptr, nullptr, radius, true, fluid_neighbors,
+                    fluid_total);
+  one_neighbor_list(ctx, sc, fluid, walls.count, walls.x, walls.y, walls.z, wall_cells,
+                    fluid_cells, grid, nullptr, nullptr, radius, false, wall_neighbors,
+                    wall_total);
 }
 
 void build_neighbor_lists_segmented(Context& ctx, const lsmps_fluid_soa& fluid,
@@ -569,20 +629,21 @@ void build_neighbor_lists_segmented(Context& ctx, const lsmps_fluid_soa& fluid,
   require(fluid.count <= fluid_cells.particle_count && walls.count <= wall_cells.particle_count,
           "Cell list particle capacity is smaller than particle count");
   const GridDev none{};
-  cell_list_impl(ctx, fluid.count, fluid.x, fluid.y, fluid.z, none, segments.grids,
+  const GeometryScratch sc = context_geometry_scratch(ctx, segments.cells, fluid.count,
+                                                      walls.count);
+  cell_list_impl(ctx, sc, fluid.count, fluid.x, fluid.y, fluid.z, none, segments.grids,
                  segments.fluid_seg, segments.cells, fluid_cells);
-  cell_list_impl(ctx, walls.count, walls.x, walls.y, walls.z, none, segments.grids,
+  cell_list_impl(ctx, sc, walls.count, walls.x, walls.y, walls.z, none, segments.grids,
                  segments.wall_seg, segments.cells, wall_cells);
-  reserve_points(ctx, fluid.count, walls.count);
   lsmps_cell_grid unused{};
   unused.cell_size = 1.0;
   unused.dims[0] = unused.dims[1] = unused.dims[2] = 1;
-  one_neighbor_list(ctx, fluid, fluid.count, fluid.x, fluid.y, fluid.z, fluid_cells, fluid_cells,
-                    unused, segments.grids, segments.fluid_seg, radius, true, fluid_neighbors,
-                    nullptr);
-  one_neighbor_list(ctx, fluid, walls.count, walls.x, walls.y, walls.z, wall_cells, fluid_cells,
-                    unused, segments.grids, segments.fluid_seg, radius, false, wall_neighbors,
-                    nullptr);
+  one_neighbor_list(ctx, sc, fluid, fluid.count, fluid.x, fluid.y, fluid.z, fluid_cells,
+                    fluid_cells, unused, segments.grids, segments.fluid_seg, radius, true,
+                    fluid_neighbors, nullptr);
+  one_neighbor_list(ctx, sc, fluid, walls.count, walls.x, walls.y, walls.z, wall_cells,
+                    fluid_cells, unused, segments.grids, segments.fluid_seg, radius, false,
+                    wall_neighbors, nullptr);
 }
 
 }  // namespace lsmps

@@ -9,6 +9,31 @@
 namespace lsmps {
 
 // ---- neighbor.hip
+// Scratch of the cell-list / neighbor-table build, as raw device pointers:
+// per-cell counts and cursors (cells_capacity entries each, >= cells + 1),
+// per-particle cell labels and 32-byte sorted point records
+// (particles_capacity, >= the larger of the fluid and wall counts), and the
+// partial sums of the device-wide scans.
+constexpr size_t kSortedPointBytes = 32;
+struct GeometryScratch {
+  int32_t* counts = nullptr;
+  int32_t* cursor = nullptr;
+  size_t cells_capacity = 0;
+  int32_t* labels = nullptr;
+  char* points = nullptr;
+  size_t particles_capacity = 0;
+  int32_t* partials = nullptr;
+  size_t partials_capacity = 0;
+};
+// Partial-sum entries the device-wide scan of n values needs.
+size_t scan_partials_needed(size_t n);
+void exclusive_scan_i32_with(Context& ctx, const int32_t* in, int32_t* out, size_t n,
+                             int32_t* partials, size_t partials_capacity);
+// The context's own scratch, grown to the given sizes (the only allocation of
+// the geometry build).
+GeometryScratch context_geometry_scratch(Context& ctx, size_t cells, size_t fluid_count,
+                                         size_t wall_count);
+lsmps_cell_grid neighbor_grid(const lsmps_config& config);
 void build_cell_list(Context& ctx, size_t count, const double* x, const double* y,
                      const double* z, const lsmps_cell_grid& grid, const lsmps_cell_list& cells);
 void build_neighbor_lists(Context& ctx, const lsmps_fluid_soa& fluid, const lsmps_wall_soa& walls,
@@ -17,6 +42,27 @@ void build_neighbor_lists(Context& ctx, const lsmps_fluid_soa& fluid, const lsmp
                           const lsmps_neighbor_list& fluid_neighbors,
                           const lsmps_neighbor_list& wall_neighbors, int32_t* fluid_total,
                           int32_t* wall_total);
+// The same with caller-provided scratch (no device allocation).
+void build_neighbor_lists_with(Context& ctx, const GeometryScratch& scratch,
+                               const lsmps_fluid_soa& fluid, const lsmps_wall_soa& walls,
+                               const lsmps_config& config, const lsmps_cell_list& fluid_cells,
+                               const lsmps_cell_list& wall_cells,
+                               const lsmps_neighbor_list& fluid_neighbors,
+                               const lsmps_neighbor_list& wall_neighbors, int32_t* fluid_total,
+                               int32_t* wall_total);
+
+// ---- noalloc.hip
+// Looks up this device's copy of the allocation-free build's static scratch
+// (once per device; the device must be current).
+void noalloc_prepare_device(int device);
+// build_neighbor_lists with the static scratch: no device allocation, at most
+// 2^20 particles per set and 2^20 cells (LSMPS_ERR_CAPACITY beyond).
+void build_neighbor_lists_noalloc(Context& ctx, const lsmps_fluid_soa& fluid,
+                                  const lsmps_wall_soa& walls, const lsmps_config& config,
+                                  const lsmps_cell_list& fluid_cells,
+                                  const lsmps_cell_list& wall_cells,
+                                  const lsmps_neighbor_list& fluid_neighbors,
+                                  const lsmps_neighbor_list& wall_neighbors);
 
 // Batched independent tanks: tank k's cells occupy [base, base + nx ny nz) of a
 // combined cell numbering; fluid_seg / wall_seg give each particle's tank.  All

@@ -39,6 +39,7 @@ EXPORTS = [
     "lsmps_config_load", "lsmps_config_save", "lsmps_read_fluid_csv", "lsmps_read_wall_csv",
     "lsmps_vtk_path", "lsmps_write_vtk", "lsmps_compute_virtual_light_diagnostics",
     "lsmps_ppe_assemble_views", "lsmps_compute_pressure_gradient_view",
+    "lsmps_build_neighbor_lists_noalloc",
 ]
 
 _LIB = None
