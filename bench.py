@@ -10,11 +10,14 @@ timed loop).  Inputs are resident in HBM before timing starts.
 
 Prints one JSON line.  `value` is particle-steps/s of the mixed-precision step
 (FP32 SpMV values, FP64 vectors/dots/residual); the plain FP64 step is timed
-beside it.  `roofline` is the PPE SpMV (the dominant kernel), measured with
-HIP events on the library stream during the timed steps.  `cpu_baseline` is
-the FP64 C++/OpenMP oracle on the host: its fixed stages are timed in full on
-the same 1M workload and its solver for a bounded number of iterations, scaled
-to the GPU's iteration count.
+beside it.  Each step is timed on its own (host clock around a synchronised
+step); `ms_per_step` is the median of the timed steps.  `roofline` is the PPE
+SpMV (the dominant kernel), measured with HIP events on the library stream
+during the timed steps.  `cpu_baseline` is the FP64 C++/OpenMP oracle on the
+host: the whole step (every stage, the PPE solved to the same 1e-10 relative
+true residual with the same preconditioner) on the same workload, the median
+of --cpu-repeats runs.  The last oracle step is also the reference for the
+per-field errors of the GPU step (`errors_vs_cpu_oracle`).
 """
 import argparse
 import json
@@ -31,7 +34,7 @@ HBM_PEAK_GBS = 8000.0
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=3)
+    p.add_argument("--steps", type=int, default=5)
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--case", default="M1",
                    help="M1 (1M dam-break), C2 (250k dam-break), C1 (8k hydrostatic), "
@@ -47,7 +50,8 @@ def parse():
     p.add_argument("--no-adjoint", action="store_true",
                    help="skip timing the adjoint of one step (after the timed steps)")
     p.add_argument("--cpu-threads", type=int, default=0)
-    p.add_argument("--cpu-solver-iterations", type=int, default=40)
+    p.add_argument("--cpu-repeats", type=int, default=5,
+                   help="full oracle steps timed on the host (the median is reported)")
     p.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
                    help="only run a plain device-to-device copy of GIB GiB (5 launches) and "
                         "print its rate: the reference kernel for scaling HBM byte counters")
@@ -96,12 +100,20 @@ def run_gpu(pkg, rt, case, precision, steps, warmup, timing=True, adjoint=False,
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     stage = np.zeros(7)
+    step_ms = []
     for _ in range(steps):
+        ts = time.perf_counter()
         state.restore(snap)
         reports.append(st.step(raise_on_nonconvergence=False))
+        torch.cuda.synchronize()
+        step_ms.append((time.perf_counter() - ts) * 1e3)
         stage += np.array(st.stage_times())
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    # outputs of the last timed step (every step starts from the same state)
+    grad, _ = st.correction_vectors()
+    fields = {"pressure": state.fluid["pressure"].cpu().numpy(), "grad": grad,
+              "velocity": [state.fluid[k].cpu().numpy() for k in ("vx", "vy", "vz")]}
     import ctypes as C
     launches, ms, nbytes = C.c_int64(), C.c_double(), C.c_double()
     rt.check(rt.load().lsmps_context_kernel_stats(ctx.handle, C.byref(launches), C.byref(ms),
@@ -124,7 +136,10 @@ def run_gpu(pkg, rt, case, precision, steps, warmup, timing=True, adjoint=False,
                "transpose_iterations": res["report"]["iterations"],
                "transpose_relative_residual": res["report"]["relative_residual"]}
     out = {
-        "ms_per_step": (t1 - t0) * 1e3 / steps,
+        "ms_per_step": float(np.median(step_ms)),
+        "ms_per_step_mean": (t1 - t0) * 1e3 / steps,
+        "step_ms": step_ms,
+        "fields": fields,
         "stage_ms": (stage / steps).tolist(),
         "reports": reports,
         "spmv_launches": launches.value,
@@ -139,60 +154,94 @@ def run_gpu(pkg, rt, case, precision, steps, warmup, timing=True, adjoint=False,
     return out
 
 
-def cpu_baseline(pkg, case, gpu_iterations, threads, solver_iterations):
-    """FP64 oracle on the host: fixed stages in full, solver on a bounded sample."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(pkg, case, threads, repeats):
+    """FP64 oracle on the host: the whole step, `repeats` times from the same
+    state (median reported); returns the timing and the last step's outputs."""
     import numpy as np
     from oracle.oracle import Oracle
     abi = pkg.abi
     orc = Oracle(abi, threads=threads or None)
-    t = {}
-    t0 = time.perf_counter()
-    nb = orc.neighbors(case)
-    cls = orc.classify(case, nb, diagnostics=False)
-    t["geometry"] = time.perf_counter() - t0
-    c2 = case.copy()
-    c2.fluid["surface_type"] = cls["surface_type"]
-    t0 = time.perf_counter()
-    vel = orc.moments(c2, nb, abi.MOMENT_VELOCITY_WALL_DIRICHLET_A)
-    pre = orc.moments(c2, nb, abi.MOMENT_PRESSURE_WALL_NEUMANN_A)
-    t["moments"] = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    tmp = orc.provision(c2, nb, vel["inverse"])
-    t["provision"] = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    ppe = orc.assemble(c2, nb, tmp, vel["inverse"], pre["inverse"],
-                       neighbor_capacity=case.n_fluid * 256)
-    t["assembly"] = time.perf_counter() - t0
-    opt = abi.default_solver_options()
-    opt.max_iterations = solver_iterations
-    opt.clamp_negative = 0
-    coarse = orc.coarse_aggregation(case, nb, cls["surface_type"], opt.coarse_cells)
-    t0 = time.perf_counter()
-    p, rep = orc.solve(ppe["row_offsets"], ppe["col_indices"], ppe["values"], ppe["rhs"], opt,
-                       coarse=coarse)
-    t_solve = time.perf_counter() - t0
-    t["solver_setup"] = rep["setup_seconds"]
-    per_iter = (t_solve - rep["setup_seconds"]) / max(rep["iterations"], 1)
-    t0 = time.perf_counter()
-    orc.correction(c2, nb, tmp, np.maximum(p, 0.0), pre["inverse"])
-    t["correction"] = time.perf_counter() - t0
-    fixed = sum(t.values())
-    est = fixed + per_iter * gpu_iterations
+    runs = []
+    for _ in range(max(1, repeats)):
+        t = {}
+        t0 = time.perf_counter()
+        nb = orc.neighbors(case)
+        cls = orc.classify(case, nb, diagnostics=False)
+        t["geometry"] = time.perf_counter() - t0
+        c2 = case.copy()
+        c2.fluid["surface_type"] = cls["surface_type"]
+        t0 = time.perf_counter()
+        vel = orc.moments(c2, nb, abi.MOMENT_VELOCITY_WALL_DIRICHLET_A)
+        pre = orc.moments(c2, nb, abi.MOMENT_PRESSURE_WALL_NEUMANN_A)
+        t["moments"] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        tmp = orc.provision(c2, nb, vel["inverse"])
+        t["provision"] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        ppe = orc.assemble(c2, nb, tmp, vel["inverse"], pre["inverse"],
+                           neighbor_capacity=case.n_fluid * 256)
+        t["assembly"] = time.perf_counter() - t0
+        opt = abi.default_solver_options()
+        opt.clamp_negative = 0
+        t0 = time.perf_counter()
+        coarse = orc.coarse_aggregation(case, nb, cls["surface_type"], opt.coarse_cells)
+        p, rep = orc.solve(ppe["row_offsets"], ppe["col_indices"], ppe["values"], ppe["rhs"],
+                           opt, coarse=coarse)
+        t["solve"] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        cor = orc.correction(c2, nb, tmp, np.maximum(p, 0.0), pre["inverse"])
+        t["correction"] = time.perf_counter() - t0
+        runs.append((sum(t.values()), t, rep))
+    totals = [r[0] for r in runs]
+    med = float(np.median(totals))
+    _, t, rep = runs[-1]
     return {
-        "value": case.n_fluid / est,
+        "value": case.n_fluid / med,
         "unit": "particle-steps/s",
         "cores": orc.threads,
         "kind": "port",
-        "sample": (f"FP64 oracle on the same {case.n_fluid}-particle workload: geometry, moments, "
-                   f"provision, assembly, correction and the solver's setup (Schwarz blocks, "
-                   f"coarse matrix and its inverse) timed in full ({fixed:.2f} s); the same "
-                   f"IDR(s) solver with the same preconditioner timed for {rep['iterations']} "
-                   f"iterations ({per_iter * 1e3:.1f} ms each) and scaled to the GPU's "
-                   f"{gpu_iterations} iterations"),
+        "sample": (f"FP64 oracle, the whole step on the same {case.n_fluid}-particle workload "
+                   f"(PPE solved to the 1e-10 relative true residual with the same IDR(s) and "
+                   f"preconditioner: {rep['iterations']} iterations), median of {len(runs)} "
+                   f"steps ({', '.join(f'{x:.2f}' for x in totals)} s) on {orc.threads} threads "
+                   f"of {cpu_model()}"),
+        "cpu_model": cpu_model(),
+        "step_seconds": totals,
         "stage_seconds": {k: round(v, 3) for k, v in t.items()},
-        "solver_ms_per_iteration": per_iter * 1e3,
-        "est_step_seconds": est,
+        "iterations": rep["iterations"],
+        "relative_residual": rep["relative_residual"],
+        "reference": {"pressure": np.maximum(p, 0.0), "grad": cor["grad"],
+                      "velocity": [cor["fluid"][k] for k in ("vx", "vy", "vz")],
+                      "u_star": [tmp[k] for k in ("vx", "vy", "vz")]},
     }
+
+
+def field_errors(got, ref):
+    """Max / RMS absolute and relative-L2 errors of pressure, grad p and the
+    corrected velocity (velocity relative to the predictor u*, whose magnitude
+    the correction nearly cancels)."""
+    import numpy as np
+    out = {}
+    for key, scale_key in (("pressure", None), ("grad", None), ("velocity", "u_star")):
+        g = np.concatenate([np.ravel(x) for x in (got[key] if key != "pressure" else [got[key]])])
+        r = np.concatenate([np.ravel(x) for x in (ref[key] if key != "pressure" else [ref[key]])])
+        scale = np.linalg.norm(np.concatenate([np.ravel(x) for x in ref[scale_key]])) \
+            if scale_key else np.linalg.norm(r)
+        d = g - r
+        out[key] = {"max_abs": float(np.max(np.abs(d))), "rms": float(np.sqrt(np.mean(d * d))),
+                    "rel_l2": float(np.linalg.norm(d) / max(scale, 1e-300))}
+    return out
 
 
 def copy_calibration(gib):
@@ -217,15 +266,20 @@ def copy_calibration(gib):
 
 def measured_traffic(case, precision):
     """HBM bytes per SpMV launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
-    summary of this workload (scaled by a plain-copy calibration), or None."""
+    summary of this workload (scaled by a plain-copy calibration), with the
+    profile it comes from and the commit it was measured at; (None, None) when
+    no profile covers the workload.  The counters cannot be read during the
+    timed run itself, so this is a recorded measurement, not a live one."""
     path = os.path.join(ROOT, "profiles", "spmv_dia_sell_m1_mixed_traffic.json")
     if case != "M1" or precision != "mixed" or not os.path.exists(path):
-        return None
+        return None, None
     with open(path) as f:
         d = json.load(f)
     fetch = d["fetch_kib"] * 1024 * d["copy_bytes"] / (d["copy_fetch_kib"] * 1024)
     write = d["write_kib"] * 1024 * d["copy_bytes"] / (d["copy_write_kib"] * 1024)
-    return fetch + write
+    source = {"profile": "profiles/" + os.path.basename(path), "kernel": d["kernel"],
+              "measured_at_commit": d.get("measured_at_commit")}
+    return fetch + write, source
 
 
 def main():
@@ -255,10 +309,11 @@ def main():
           f"stages {np.round(main_run['stage_ms'], 2).tolist()}", file=sys.stderr, flush=True)
     fp64_run = None
     if not args.no_fp64 and prec != abi.PRECISION_FP64:
-        fp64_run = run_gpu(pkg, rt, case, abi.PRECISION_FP64, max(1, min(args.steps, 2)), 0,
+        fp64_run = run_gpu(pkg, rt, case, abi.PRECISION_FP64, 5, 1,
                            shadow=args.shadow_dimension, coarse=args.coarse_cells)
         print(f"[bench] fp64: {fp64_run['ms_per_step']:.1f} ms/step, iterations "
               f"{[r['iterations'] for r in fp64_run['reports']]}", file=sys.stderr, flush=True)
+    traffic, traffic_source = measured_traffic(args.case, args.precision)
     avg_ms = main_run["spmv_ms"] / max(main_run["spmv_launches"], 1)
     avg_bytes = main_run["spmv_bytes"] / max(main_run["spmv_launches"], 1)
     achieved = avg_bytes / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
@@ -271,6 +326,8 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": main_run["ms_per_step"],
+        "ms_per_step_mean": main_run["ms_per_step_mean"],
+        "step_ms": [round(x, 3) for x in main_run["step_ms"]],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -293,7 +350,7 @@ def main():
                               "correction", "total"], np.round(main_run["stage_ms"], 3).tolist())),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(args.case, args.precision),
+                     "traffic": traffic, "traffic_source": traffic_source,
                      "kernel": "PPE SpMV (IDR shadow-dot epilogue)",
                      "avg_launch_ms": avg_ms, "algorithmic_bytes_per_launch": avg_bytes,
                      "launches": main_run["spmv_launches"]},
@@ -309,11 +366,16 @@ def main():
         line["pressure_rel_l2_vs_fp64_gpu"] = float(
             np.linalg.norm(main_run["pressure"] - p64) / max(np.linalg.norm(p64), 1e-300))
     if not args.no_cpu:
-        cb = cpu_baseline(pkg, case, int(np.mean(iters)), args.cpu_threads,
-                          args.cpu_solver_iterations)
+        cb = cpu_baseline(pkg, case, args.cpu_threads, args.cpu_repeats)
         print(f"[bench] cpu oracle: {cb['sample']}", file=sys.stderr, flush=True)
         line["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
+        line["cpu_baseline"]["cpu_model"] = cb["cpu_model"]
+        line["cpu_baseline"]["step_seconds"] = [round(x, 3) for x in cb["step_seconds"]]
+        line["cpu_baseline"]["iterations"] = cb["iterations"]
         line["speedup_vs_cpu"] = value / cb["value"]
+        errs = field_errors(main_run["fields"], cb["reference"])
+        line["errors_vs_cpu_oracle"] = {"precision": args.precision, **errs}
+        print(f"[bench] errors vs oracle: {errs}", file=sys.stderr, flush=True)
     print(json.dumps(line), flush=True)
 
 
