@@ -47,6 +47,10 @@ def parse():
     p.add_argument("--coarse-cells", type=int, default=0,
                    help="cells per side of the coarse-space aggregates (0: the library "
                         "default, -1: no coarse space)")
+    p.add_argument("--no-graph", action="store_true",
+                   help="skip the extra run that replays the captured IDR cycles "
+                        "(LSMPS_FLAG_GRAPH); the timed steps always launch directly, "
+                        "which is what the HIP-event SpMV timing needs")
     p.add_argument("--no-adjoint", action="store_true",
                    help="skip timing the adjoint of one step (after the timed steps)")
     p.add_argument("--cpu-threads", type=int, default=0)
@@ -73,11 +77,11 @@ def make_case(cases, name):
 
 
 def run_gpu(pkg, rt, case, precision, steps, warmup, timing=True, adjoint=False, shadow=0,
-            coarse=0):
+            coarse=0, graph=True):
     import numpy as np
     import torch
     abi = pkg.abi
-    flags = abi.FLAG_KERNEL_TIMING if timing else 0
+    flags = (abi.FLAG_KERNEL_TIMING if timing else 0) | (abi.FLAG_GRAPH if graph else 0)
     ctx = rt.Context(precision, flags=flags)
     state = rt.DeviceState(case)
     opts = abi.default_solver_options()
@@ -303,14 +307,14 @@ def main():
           f"grid {list(case.config.cell_dims)}", file=sys.stderr, flush=True)
     main_run = run_gpu(pkg, rt, case, prec, args.steps, args.warmup,
                        adjoint=not args.no_adjoint, shadow=args.shadow_dimension,
-                       coarse=args.coarse_cells)
+                       coarse=args.coarse_cells, graph=False)
     iters = [r["iterations"] for r in main_run["reports"]]
     print(f"[bench] {args.precision}: {main_run['ms_per_step']:.1f} ms/step, iterations {iters}, "
           f"stages {np.round(main_run['stage_ms'], 2).tolist()}", file=sys.stderr, flush=True)
     fp64_run = None
     if not args.no_fp64 and prec != abi.PRECISION_FP64:
         fp64_run = run_gpu(pkg, rt, case, abi.PRECISION_FP64, 5, 1,
-                           shadow=args.shadow_dimension, coarse=args.coarse_cells)
+                           shadow=args.shadow_dimension, coarse=args.coarse_cells, graph=False)
         print(f"[bench] fp64: {fp64_run['ms_per_step']:.1f} ms/step, iterations "
               f"{[r['iterations'] for r in fp64_run['reports']]}", file=sys.stderr, flush=True)
     traffic, traffic_source = measured_traffic(args.case, args.precision)
@@ -342,6 +346,7 @@ def main():
                    "re_over_l0": 3.1, "dt_s": case.config.time_step, "solver": f"IDR({args.shadow_dimension or (16 if args.coarse_cells < 0 else 14)}); Schwarz blocks on wrong-sign rows, Jacobi elsewhere" + ("" if args.coarse_cells < 0 else f", coarse space of {args.coarse_cells or pkg.abi.default_solver_options().coarse_cells}^3-cell aggregates"),
                    "tolerance": 1e-10, "precision": args.precision},
         "solver_iterations": iters,
+        "graph_cycles": [r["graph_cycles"] for r in main_run["reports"]],
         "schwarz_blocks": main_run["reports"][-1]["schwarz_blocks"],
         "schwarz_rows": main_run["reports"][-1]["schwarz_rows"],
         "coarse_unknowns": main_run["reports"][-1]["coarse_unknowns"],
@@ -365,6 +370,18 @@ def main():
         p64 = fp64_run["pressure"]
         line["pressure_rel_l2_vs_fp64_gpu"] = float(
             np.linalg.norm(main_run["pressure"] - p64) / max(np.linalg.norm(p64), 1e-300))
+    if not args.no_graph:
+        graph_run = run_gpu(pkg, rt, case, prec, 5, 1, timing=False,
+                            shadow=args.shadow_dimension, coarse=args.coarse_cells, graph=True)
+        line["graph"] = {
+            "ms_per_step": graph_run["ms_per_step"],
+            "graph_cycles": [r["graph_cycles"] for r in graph_run["reports"]],
+            "pressure_bit_identical": bool(np.array_equal(graph_run["pressure"],
+                                                          main_run["pressure"])),
+            "note": "the same step with the IDR cycles replayed from captured HIP graphs "
+                    "(device-side loop control, two cycles in flight), no kernel timing"}
+        print(f"[bench] graph replays: {graph_run['ms_per_step']:.1f} ms/step", file=sys.stderr,
+              flush=True)
     if not args.no_cpu:
         cb = cpu_baseline(pkg, case, args.cpu_threads, args.cpu_repeats)
         print(f"[bench] cpu oracle: {cb['sample']}", file=sys.stderr, flush=True)

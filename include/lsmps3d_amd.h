@@ -86,7 +86,14 @@ enum { LSMPS_SOLVER_PCG = 0, LSMPS_SOLVER_BICGSTAB = 1, LSMPS_SOLVER_IDRS = 2 };
 /* Context flags.  KERNEL_TIMING brackets every PPE SpMV launch with hipEvents on
  * the context stream and accumulates durations and algorithmic bytes
  * (lsmps_context_kernel_stats). */
-enum { LSMPS_FLAG_ASYNC = 1, LSMPS_FLAG_KERNEL_TIMING = 2 };
+/* LSMPS_FLAG_GRAPH: the solver captures one IDR(s) cycle (its vector kernels,
+ * SpMVs, preconditioner, scalar recurrences and the loop's convergence,
+ * reliable-update and restart decisions, made on the device) in a HIP graph and
+ * replays it, two cycles in flight, instead of launching every kernel and
+ * reading the residual on the host after each cycle; results are bit-identical
+ * to the launch path.  With LSMPS_FLAG_KERNEL_TIMING the cycles are launched
+ * directly (events recorded inside a graph cannot be timed). */
+enum { LSMPS_FLAG_ASYNC = 1, LSMPS_FLAG_KERNEL_TIMING = 2, LSMPS_FLAG_GRAPH = 4 };
 
 /* ---- data contract (POD mirrors of the reference views) ------------------ */
 
@@ -285,6 +292,7 @@ typedef struct lsmps_solve_report {
   int32_t schwarz_blocks;     /* LSMPS_PRECOND_SCHWARZ: dense blocks (wrong-sign rows) */
   int32_t schwarz_rows;       /* LSMPS_PRECOND_SCHWARZ: rows summed over all blocks */
   int32_t coarse_unknowns;    /* SCHWARZ_COARSE: aggregates of the applied coarse space (0: none) */
+  int32_t graph_cycles;       /* LSMPS_FLAG_GRAPH: IDR cycles replayed from a captured HIP graph */
 } lsmps_solve_report;
 
 /* Opaque objects */

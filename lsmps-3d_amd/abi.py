@@ -42,6 +42,7 @@ STEP_FROZEN_GEOMETRY = 2
 
 FLAG_ASYNC = 1
 FLAG_KERNEL_TIMING = 2
+FLAG_GRAPH = 4
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
@@ -158,7 +159,8 @@ class SolveReport(C.Structure):
     _fields_ = [("method", C.c_int32), ("iterations", C.c_int32), ("refinements", C.c_int32),
                 ("converged", C.c_int32), ("relative_residual", C.c_double),
                 ("rhs_norm", C.c_double), ("schwarz_blocks", C.c_int32),
-                ("schwarz_rows", C.c_int32), ("coarse_unknowns", C.c_int32)]
+                ("schwarz_rows", C.c_int32), ("coarse_unknowns", C.c_int32),
+                ("graph_cycles", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -470,7 +470,9 @@ __device__ __forceinline__ double block_sum(double s, double* red) {
 __global__ void __launch_bounds__(kBlock) restrict_kernel(int count, const int32_t* agg_start,
                                                           const int32_t* agg_rows,
                                                           const double* w, const double* wfac,
-                                                          double* rc) {
+                                                          double* rc,
+                                                          const int32_t* skip = nullptr) {
+  if (launch_skipped(skip)) return;
   __shared__ double red[kWaves];
   const int I = blockIdx.x;
   const int32_t mb = agg_start[I], me = agg_start[I + 1];
@@ -509,7 +511,9 @@ __global__ void __launch_bounds__(kBlock) restrict_kernel(int count, const int32
 __global__ void __launch_bounds__(kBlock) coarse_product_kernel(int count, const double* ainv,
                                                                 const double* rc,
                                                                 const int32_t* status,
-                                                                double* yc) {
+                                                                double* yc,
+                                                                const int32_t* skip = nullptr) {
+  if (launch_skipped(skip)) return;
   __shared__ double red[kWaves];
   const int I = blockIdx.x;
   if (status[0]) {  // zero pivot in the inverse: no correction
@@ -524,7 +528,9 @@ __global__ void __launch_bounds__(kBlock) coarse_product_kernel(int count, const
 }
 
 __global__ void __launch_bounds__(kBlock) prolong_add_kernel(size_t n, const int32_t* agg_of_row,
-                                                             const double* yc, double* v) {
+                                                             const double* yc, double* v,
+                                                             const int32_t* skip = nullptr) {
+  if (launch_skipped(skip)) return;
   const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int32_t a = agg_of_row[i];
@@ -672,18 +678,19 @@ bool coarse_usable(Context& ctx, const CoarseSpace& C) {
   return ctx.host_i32[0] == 0;
 }
 
-void coarse_solve(Context& ctx, CoarseSpace& C, const double* w, bool scale_by_wfac) {
+void coarse_solve(Context& ctx, CoarseSpace& C, const double* w, bool scale_by_wfac,
+                  const int32_t* skip) {
   restrict_kernel<<<static_cast<unsigned>(C.count), kBlock, 0, ctx.stream>>>(
       C.count, C.agg_start.data(), C.agg_rows.data(), w, scale_by_wfac ? C.wfac.data() : nullptr,
-      C.rc.data());
+      C.rc.data(), skip);
   coarse_product_kernel<<<static_cast<unsigned>(C.count), kBlock, 0, ctx.stream>>>(
-      C.count, C.ainv.data(), C.rc.data(), C.status.data(), C.yc.data());
+      C.count, C.ainv.data(), C.rc.data(), C.status.data(), C.yc.data(), skip);
   LSMPS_LAUNCH_CHECK();
 }
 
-void coarse_prolong_add(Context& ctx, const CoarseSpace& C, double* v) {
+void coarse_prolong_add(Context& ctx, const CoarseSpace& C, double* v, const int32_t* skip) {
   prolong_add_kernel<<<blocks_for(C.n), kBlock, 0, ctx.stream>>>(C.n, C.agg_of_row.data(),
-                                                                 C.yc.data(), v);
+                                                                 C.yc.data(), v, skip);
   LSMPS_LAUNCH_CHECK();
 }
 

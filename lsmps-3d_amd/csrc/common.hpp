@@ -108,6 +108,13 @@ inline unsigned blocks_for(size_t n, int block = kBlock) {
   return static_cast<unsigned>(b > 0 ? b : 1);
 }
 
+// Kernels that a captured solver cycle replays take an optional `skip` word:
+// when it points at a non-zero value the whole launch returns at once (the
+// value is uniform, so no barrier is split).  Eager launches pass null.
+__device__ __forceinline__ bool launch_skipped(const int32_t* skip) {
+  return skip != nullptr && *skip != 0;
+}
+
 // Fill kernels replace hipMemsetAsync so that every clear is an ordinary
 // kernel node if the stream is captured into a graph.
 void fill_i32(hipStream_t s, int32_t* p, size_t n, int32_t v);
@@ -137,13 +144,17 @@ struct Context {
   int32_t* host_i32 = nullptr;   // pinned host scratch
   double* host_f64 = nullptr;
   // PPE SpMV timing (LSMPS_FLAG_KERNEL_TIMING)
-  hipEvent_t timing_events[2 * 17] = {};
+  // two sets of 2 (s + 1) SpMV events: eager cycles use set 0, the two
+  // captured cycle graphs one set each
+  static constexpr int kTimingSet = 2 * 17;
+  hipEvent_t timing_events[2 * kTimingSet] = {};
   bool timing_ready = false;
   int64_t spmv_launches = 0;
   double spmv_ms = 0.0;
   double spmv_bytes = 0.0;
 
   bool timing() const { return (flags & LSMPS_FLAG_KERNEL_TIMING) != 0; }
+  bool graph() const { return (flags & LSMPS_FLAG_GRAPH) != 0; }
 
   void activate() const { LSMPS_HIP(hipSetDevice(device)); }
   void finish() const {

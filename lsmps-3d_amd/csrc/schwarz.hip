@@ -441,7 +441,9 @@ __global__ void cover_sort_kernel(size_t n, const int32_t* start, int32_t* entri
 template <typename T, typename V>
 __global__ void __launch_bounds__(kMaxBlock) apply_kernel(const int32_t* members,
                                                           const int32_t* sizes, const V* inverse,
-                                                          const T* w, double* ybuf) {
+                                                          const T* w, double* ybuf,
+                                                          const int32_t* skip) {
+  if (launch_skipped(skip)) return;
   __shared__ double wb[kApplyCols];
   const int t = threadIdx.x;
   const size_t b = blockIdx.x / kApplyParts;
@@ -470,7 +472,9 @@ __global__ void __launch_bounds__(kMaxBlock) apply_kernel(const int32_t* members
 // kApplyParts partials), in ascending block order.
 template <typename T>
 __global__ void gather_kernel(size_t count, const int32_t* covered, const int32_t* start,
-                              const int32_t* entries, const double* ybuf, T* v) {
+                              const int32_t* entries, const double* ybuf, T* v,
+                              const int32_t* skip) {
+  if (launch_skipped(skip)) return;
   const size_t k = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (k >= count) return;
   const int32_t i = covered[k];
@@ -571,33 +575,37 @@ void schwarz_setup(Context& ctx, SchwarzPrec& P, const lsmps_csr& A, const doubl
 }
 
 template <typename T>
-void schwarz_products(Context& ctx, const SchwarzPrec& P, const T* v) {
+void schwarz_products(Context& ctx, const SchwarzPrec& P, const T* v, const int32_t* skip) {
   if (P.blocks == 0) return;
   const unsigned grid = static_cast<unsigned>(P.blocks * kApplyParts);
   if (P.fp32) {
     apply_kernel<T, float><<<grid, kMaxBlock, 0, ctx.stream>>>(P.members.data(), P.sizes.data(),
-                                                               P.inv32.data(), v, P.ybuf.data());
+                                                               P.inv32.data(), v, P.ybuf.data(),
+                                                               skip);
   } else {
     apply_kernel<T, double><<<grid, kMaxBlock, 0, ctx.stream>>>(P.members.data(), P.sizes.data(),
-                                                                P.inv64.data(), v, P.ybuf.data());
+                                                                P.inv64.data(), v, P.ybuf.data(),
+                                                                skip);
   }
   LSMPS_LAUNCH_CHECK();
 }
 
 template <typename T>
-void schwarz_apply(Context& ctx, const SchwarzPrec& P, T* v) {
+void schwarz_apply(Context& ctx, const SchwarzPrec& P, T* v, const int32_t* skip) {
   if (P.blocks == 0) return;
-  schwarz_products<T>(ctx, P, v);
+  schwarz_products<T>(ctx, P, v, skip);
   gather_kernel<T><<<blocks_for(std::max<size_t>(P.covered_count, 1)), kBlock, 0, ctx.stream>>>(
       P.covered_count, P.covered.data(), P.cover_start.data(), P.cover_entries.data(),
-      P.ybuf.data(), v);
+      P.ybuf.data(), v, skip);
   LSMPS_LAUNCH_CHECK();
 }
 
-template void schwarz_apply<double>(Context&, const SchwarzPrec&, double*);
-template void schwarz_apply<float>(Context&, const SchwarzPrec&, float*);
-template void schwarz_products<double>(Context&, const SchwarzPrec&, const double*);
-template void schwarz_products<float>(Context&, const SchwarzPrec&, const float*);
+template void schwarz_apply<double>(Context&, const SchwarzPrec&, double*, const int32_t*);
+template void schwarz_apply<float>(Context&, const SchwarzPrec&, float*, const int32_t*);
+template void schwarz_products<double>(Context&, const SchwarzPrec&, const double*,
+                                       const int32_t*);
+template void schwarz_products<float>(Context&, const SchwarzPrec&, const float*,
+                                      const int32_t*);
 
 void schwarz_stats(Context& ctx, const SchwarzPrec& P, int64_t* blocks, int64_t* members) {
   *blocks = static_cast<int64_t>(P.blocks);

@@ -58,11 +58,12 @@ void schwarz_setup(Context& ctx, SchwarzPrec& P, const lsmps_csr& A, const doubl
 // covered rows, Jacobi values elsewhere); on exit each covered row holds the mean
 // of its blocks' values (inv_b w_b), summed in block order.
 template <typename T>
-void schwarz_apply(Context& ctx, const SchwarzPrec& P, T* v);
+void schwarz_apply(Context& ctx, const SchwarzPrec& P, T* v, const int32_t* skip = nullptr);
 // The block products alone (into P.ybuf); a consumer that reads v row by row
 // can then take covered rows from schwarz_row_mean instead of the completion.
 template <typename T>
-void schwarz_products(Context& ctx, const SchwarzPrec& P, const T* v);
+void schwarz_products(Context& ctx, const SchwarzPrec& P, const T* v,
+                      const int32_t* skip = nullptr);
 void schwarz_stats(Context& ctx, const SchwarzPrec& P, int64_t* blocks, int64_t* members);
 
 }  // namespace lsmps

@@ -42,7 +42,23 @@
 
 namespace lsmps {
 
+void PpeSolver::release_graphs() noexcept {
+  for (auto& e : cycle_exec) {
+    if (e) (void)hipGraphExecDestroy(e);
+    e = nullptr;
+  }
+  cycle_signature.clear();
+}
+
 PpeSolver::~PpeSolver() {
+  release_graphs();
+  for (auto& e : cycle_done)
+    if (e) (void)hipEventDestroy(e);
+  if (capture_stream) {
+    (void)hipStreamSynchronize(capture_stream);
+    (void)hipStreamDestroy(capture_stream);
+  }
+  if (host_control) (void)hipHostFree(host_control);
   if (host) (void)hipHostFree(host);
   if (seg_host) (void)hipHostFree(seg_host);
 }
@@ -683,6 +699,7 @@ struct SellArgs {
   int s;             // mode 1
   const double* r;   // mode 2: the FP64 residual
   double* partials;
+  const int32_t* skip = nullptr;  // captured cycles: the launch returns when *skip != 0
 };
 
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.
@@ -754,6 +771,7 @@ constexpr size_t kMaxGatherRows = (size_t{1} << 31) / sizeof(double) - 1;
 constexpr int kSellBlock = 512;
 template <typename T, typename V, int MODE>
 __global__ void __launch_bounds__(kSellBlock) sell_spmv_kernel(SellArgs<T, V> a) {
+  if (launch_skipped(a.skip)) return;
   const int lane = threadIdx.x & 63;
   const unsigned wave_in_block = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const size_t per_block = (a.chunks + gridDim.x - 1) / gridDim.x;
@@ -944,7 +962,9 @@ __global__ void __launch_bounds__(kBlock) residual_kernel(size_t n, const int32_
                                                           const int32_t* cols, const double* vals,
                                                           const double* x, const double* b,
                                                           double* r, float* r32,
-                                                          double* partials) {
+                                                          double* partials,
+                                                          const int32_t* skip = nullptr) {
+  if (launch_skipped(skip)) return;
   const int lane = threadIdx.x & (kLanes - 1);
   const size_t group = (static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x) / kLanes;
   const size_t ngroups = static_cast<size_t>(gridDim.x) * blockDim.x / kLanes;
@@ -995,13 +1015,15 @@ __global__ void __launch_bounds__(kBlock) row_stats_kernel(size_t n, const int32
 }
 
 template <typename T>
-__global__ void fill_kernel(size_t n, T* p, T v) {
+__global__ void fill_kernel(size_t n, T* p, T v, const int32_t* skip = nullptr) {
+  if (launch_skipped(skip)) return;
   const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
 }
 
 template <typename S, typename D>
-__global__ void copy_kernel(size_t n, const S* src, D* dst) {
+__global__ void copy_kernel(size_t n, const S* src, D* dst, const int32_t* skip = nullptr) {
+  if (launch_skipped(skip)) return;
   const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i < n) dst[i] = static_cast<D>(src[i]);
 }
@@ -1049,6 +1071,7 @@ struct IdrVec {
   // coarse-space term added by idr_a2g_kernel (null without a coarse space)
   const int32_t* agg_of_row;
   const double* yc;
+  const int32_t* skip = nullptr;  // captured cycles: the launch returns when *skip != 0
 };
 
 // Scalars that govern row i: the system's, or its segment's in a batched solve.
@@ -1122,6 +1145,7 @@ __device__ __forceinline__ void comb_sub2(double& g, double& u, const double* c,
 // on uncovered rows, the raw value (the block application's input) on covered ones.
 template <typename T, bool kSeg = false>
 __global__ void __launch_bounds__(kBlock) idr_a1_kernel(IdrVec<T> a, int k) {
+  if (launch_skipped(a.skip)) return;
   for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < a.n;
        i += static_cast<size_t>(gridDim.x) * blockDim.x) {
     const SolverScalars* sc = row_sc<kSeg>(a, i);
@@ -1133,6 +1157,7 @@ __global__ void __launch_bounds__(kBlock) idr_a1_kernel(IdrVec<T> a, int k) {
 // Schwarz path, second half (v already preconditioned): U_k = sum_{q>=k} c_q U_q + om v.
 template <typename T, bool kSeg = false>
 __global__ void __launch_bounds__(kBlock) idr_a2_kernel(IdrVec<T> a, int k) {
+  if (launch_skipped(a.skip)) return;
   for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < a.n;
        i += static_cast<size_t>(gridDim.x) * blockDim.x) {
     const SolverScalars* sc = row_sc<kSeg>(a, i);
@@ -1146,6 +1171,7 @@ __global__ void __launch_bounds__(kBlock) idr_a2_kernel(IdrVec<T> a, int k) {
 // blocks' mean (schwarz_row_mean) in place of v, which is not written back.
 template <typename T>
 __global__ void __launch_bounds__(kBlock) idr_a2g_kernel(IdrVec<T> a, int k) {
+  if (launch_skipped(a.skip)) return;
   const double om = a.sc->om;
   for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < a.n;
        i += static_cast<size_t>(gridDim.x) * blockDim.x) {
@@ -1164,7 +1190,9 @@ __global__ void __launch_bounds__(kBlock) idr_a2g_kernel(IdrVec<T> a, int k) {
 
 // One block per system (segment); the grid is 1 for a single system.  `only`
 // (optional, per segment) restricts the reset to the flagged segments.
-__global__ void idr_init_scalars(SolverScalars* sc, int s, const int32_t* only = nullptr) {
+__global__ void idr_init_scalars(SolverScalars* sc, int s, const int32_t* only = nullptr,
+                                 const int32_t* skip = nullptr) {
+  if (launch_skipped(skip)) return;
   if (only && !only[blockIdx.x]) return;
   sc += blockIdx.x;
   const int t = threadIdx.x;
@@ -1229,7 +1257,9 @@ __device__ void scalars_copy(const SolverScalars* from, SolverScalars* to) {
 
 // Cycle start: f = P^T r (from partials), c = M^-1 f.
 __global__ void __launch_bounds__(kReduceThreads)
-    idr_scalar_f(SolverScalars* sc_g, const double* partials, int nb, int s) {
+    idr_scalar_f(SolverScalars* sc_g, const double* partials, int nb, int s,
+                 const int32_t* skip = nullptr) {
+  if (launch_skipped(skip)) return;
   sc_g += blockIdx.x;
   partials += static_cast<size_t>(blockIdx.x) * nb * kPartialStride;
   __shared__ double d[kPartialStride];
@@ -1254,7 +1284,9 @@ __global__ void __launch_bounds__(kReduceThreads)
 __device__ void scalar_after_body(SolverScalars* sc, const double* d, int s, int k);
 
 __global__ void __launch_bounds__(kReduceThreads)
-    idr_scalar_after(SolverScalars* sc_g, const double* partials, int nb, int s, int k) {
+    idr_scalar_after(SolverScalars* sc_g, const double* partials, int nb, int s, int k,
+                     const int32_t* skip = nullptr) {
+  if (launch_skipped(skip)) return;
   sc_g += blockIdx.x;
   partials += static_cast<size_t>(blockIdx.x) * nb * kPartialStride;
   __shared__ double d[kPartialStride];
@@ -1310,7 +1342,8 @@ __device__ void scalar_after_body(SolverScalars* sc, const double* d, int s, int
 // omega = (t.r)/(t.t) with the 0.7 "maintaining the convergence" safeguard.
 __global__ void __launch_bounds__(kReduceThreads)
     idr_scalar_omega(SolverScalars* sc, const double* part_t, const double* part_r, int nb_t,
-                     int nb_r, int rr_index = 0) {
+                     int nb_r, int rr_index = 0, const int32_t* skip = nullptr) {
+  if (launch_skipped(skip)) return;
   sc += blockIdx.x;
   part_t += static_cast<size_t>(blockIdx.x) * nb_t * kPartialStride;
   part_r += static_cast<size_t>(blockIdx.x) * nb_r * kPartialStride;
@@ -1337,7 +1370,9 @@ __global__ void __launch_bounds__(kReduceThreads)
 }
 
 __global__ void __launch_bounds__(kReduceThreads)
-    scalar_norm(SolverScalars* sc, const double* partials, int nb) {
+    scalar_norm(SolverScalars* sc, const double* partials, int nb,
+                const int32_t* skip = nullptr) {
+  if (launch_skipped(skip)) return;
   sc += blockIdx.x;
   partials += static_cast<size_t>(blockIdx.x) * nb * kPartialStride;
   __shared__ double d[kPartialStride];
@@ -1348,6 +1383,7 @@ __global__ void __launch_bounds__(kReduceThreads)
 // v = D^-1 (r - sum_{q>=k} c_q G_q);  U_k = sum_{q>=k} c_q U_q + om v
 template <typename T, bool kSeg = false>
 __global__ void __launch_bounds__(kBlock) idr_a_kernel(IdrVec<T> a, int k) {
+  if (launch_skipped(a.skip)) return;
   for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < a.n;
        i += static_cast<size_t>(gridDim.x) * blockDim.x) {
     const SolverScalars* sc = row_sc<kSeg>(a, i);
@@ -1363,6 +1399,7 @@ __global__ void __launch_bounds__(kBlock) idr_a_kernel(IdrVec<T> a, int k) {
 // optionally v = D^-1 r for the dimension-reduction step.
 template <typename T, bool kSeg = false>
 __global__ void __launch_bounds__(kBlock) idr_b_kernel(IdrVec<T> a, int k, int write_v) {
+  if (launch_skipped(a.skip)) return;
   double acc[1] = {0.0};
   for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < a.n;
        i += static_cast<size_t>(gridDim.x) * blockDim.x) {
@@ -1389,6 +1426,7 @@ __global__ void __launch_bounds__(kBlock) idr_b_kernel(IdrVec<T> a, int k, int w
 // U_{k+1} = sum_{q>k} c_q U_q + om v (idr_a_kernel).
 template <typename T, bool kBlocks>
 __global__ void __launch_bounds__(kBlock) idr_ba_kernel(IdrVec<T> a, int k) {
+  if (launch_skipped(a.skip)) return;
   double acc[1] = {0.0};
   const SolverScalars* sc = a.sc;
   const double beta = sc->beta, om = sc->om;
@@ -1418,6 +1456,7 @@ __global__ void __launch_bounds__(kBlock) idr_ba_kernel(IdrVec<T> a, int k) {
 // x += om v; r -= om t; partial r.r
 template <typename T, bool kSeg = false>
 __global__ void __launch_bounds__(kBlock) idr_c_kernel(IdrVec<T> a) {
+  if (launch_skipped(a.skip)) return;
   double acc[1] = {0.0};
   for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < a.n;
        i += static_cast<size_t>(gridDim.x) * blockDim.x) {
@@ -1433,6 +1472,7 @@ __global__ void __launch_bounds__(kBlock) idr_c_kernel(IdrVec<T> a) {
 // f partials: P_q . r
 template <typename T>
 __global__ void __launch_bounds__(kBlock) shadow_dots_kernel(IdrVec<T> a) {
+  if (launch_skipped(a.skip)) return;
   double acc[kMaxShadow];
 #pragma unroll
   for (int q = 0; q < kMaxShadow; ++q) acc[q] = 0.0;
@@ -1714,6 +1754,145 @@ double true_residual(Context& ctx, PpeSolver& S, const lsmps_csr& A, const doubl
   return read_rr(ctx, S);
 }
 
+// ---- captured IDR cycles ----------------------------------------------------
+
+__global__ void idr_control_init(IdrControl* c, double rr0, double target2, double factor2,
+                                 int max_iterations, int s) {
+  c->halt = rr0 > target2 && max_iterations > 0 ? 0 : 1;
+  c->skip_replace = 1;
+  c->skip_reinit = 1;
+  c->iterations = 0;
+  c->replacements = 0;
+  c->max_iterations = max_iterations;
+  c->s = s;
+  c->pad = 0;
+  c->rr = rr0;
+  c->rr_replaced = rr0;
+  c->target2 = target2;
+  c->factor2 = factor2;
+}
+
+// First launch of a cycle: once halted, the gated launches are skipped too.
+__global__ void idr_control_begin(IdrControl* c) {
+  if (c->halt) {
+    c->skip_replace = 1;
+    c->skip_reinit = 1;
+  }
+}
+
+// After the cycle's residual norm: the tests the host loop makes on the value
+// it reads back (non-finite stop, reliable update, restart after a breakdown).
+__global__ void idr_control_cycle(IdrControl* c, const SolverScalars* sc) {
+  if (c->halt) return;
+  c->iterations += c->s + 1;
+  const double rr = sc->rr;
+  c->rr = rr;
+  if (!isfinite(rr)) {
+    c->halt = 1;
+    c->skip_replace = 1;
+    c->skip_reinit = 1;
+    return;
+  }
+  c->skip_replace = rr <= c->target2 || rr <= c->factor2 * c->rr_replaced ? 0 : 1;
+  c->skip_reinit = sc->breakdown ? 0 : 1;
+}
+
+// After the gated true-residual launches: take their norm, then the loop test.
+__global__ void idr_control_after_replace(IdrControl* c, const SolverScalars* sc) {
+  if (c->halt) return;
+  if (!c->skip_replace) {
+    c->rr = sc->rr;
+    c->rr_replaced = c->rr;
+    c->replacements += 1;
+  }
+  if (!(c->rr > c->target2 && c->iterations < c->max_iterations)) c->halt = 1;
+}
+
+void append_sell_signature(std::vector<uintptr_t>& sig, const SellMatrix& M) {
+  const void* p[] = {M.dia_pairs.data(), M.dia_singles.data(), M.dia_plan.data(),
+                     M.dia_start.data(), M.chunk_start.data(), M.chunk_width.data(),
+                     M.col_start.data(), M.nwin.data(), M.windows.data(), M.cols.data()};
+  for (const void* q : p) sig.push_back(reinterpret_cast<uintptr_t>(q));
+  sig.push_back(M.chunks);
+}
+
+void append_schwarz_signature(std::vector<uintptr_t>& sig, const SchwarzPrec& P) {
+  const void* p[] = {P.members.data(), P.sizes.data(), P.inv32.data(), P.inv64.data(),
+                     P.ybuf.data(), P.covered.data(), P.cover_start.data(),
+                     P.cover_entries.data(), P.pdiag.data()};
+  for (const void* q : p) sig.push_back(reinterpret_cast<uintptr_t>(q));
+  sig.push_back(P.blocks);
+  sig.push_back(P.covered_count);
+  sig.push_back(P.fp32 ? 1 : 0);
+}
+
+void append_coarse_signature(std::vector<uintptr_t>& sig, const CoarseSpace& C) {
+  const void* p[] = {C.agg_start.data(), C.agg_rows.data(), C.agg_of_row.data(), C.wfac.data(),
+                     C.rc.data(), C.yc.data(), C.ainv.data(), C.status.data()};
+  for (const void* q : p) sig.push_back(reinterpret_cast<uintptr_t>(q));
+  sig.push_back(static_cast<uintptr_t>(C.count));
+  sig.push_back(C.n);
+}
+
+// Makes S.cycle_exec[0..1] the captured cycle for `sig`, capturing it (twice:
+// SpMV event sets 0 and 1) when the signature changed.  enqueue(halt,
+// skip_replace, skip_reinit, set) launches one cycle on ctx.stream.  Returns
+// false when the graph path cannot be used (then nothing was left captured).
+template <typename F>
+bool idr_cycles_captured(Context& ctx, PpeSolver& S, F enqueue, std::vector<uintptr_t> sig) {
+  S.control.reserve(1);
+  if (!S.host_control)
+    LSMPS_HIP(hipHostMalloc(reinterpret_cast<void**>(&S.host_control), 2 * sizeof(IdrControl)));
+  for (auto& e : S.cycle_done)
+    if (!e) LSMPS_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  if (!S.capture_stream) LSMPS_HIP(hipStreamCreateWithFlags(&S.capture_stream, hipStreamNonBlocking));
+  sig.push_back(reinterpret_cast<uintptr_t>(S.control.data()));
+  sig.push_back(reinterpret_cast<uintptr_t>(S.scalars.data()));
+  sig.push_back(reinterpret_cast<uintptr_t>(S.part_a.data()));
+  sig.push_back(reinterpret_cast<uintptr_t>(S.part_b.data()));
+  if (S.cycle_exec[0] && S.cycle_exec[1] && sig == S.cycle_signature) return true;
+  S.release_graphs();
+  IdrControl* ctl = S.control.data();
+  const int32_t* halt = &ctl->halt;
+  const int32_t* skip_replace = &ctl->skip_replace;
+  const int32_t* skip_reinit = &ctl->skip_reinit;
+  // the capture stream stands in for the library stream while the cycle is
+  // recorded (the legacy default stream cannot be captured); nothing may
+  // allocate or synchronise in between
+  const hipStream_t library_stream = ctx.stream;
+  for (int set = 0; set < 2; ++set) {
+    hipGraph_t graph = nullptr;
+    ctx.stream = S.capture_stream;
+    try {
+      const AllocationFreeze frozen;
+      LSMPS_HIP(hipStreamBeginCapture(S.capture_stream, hipStreamCaptureModeThreadLocal));
+      try {
+        enqueue(halt, skip_replace, skip_reinit, set);
+      } catch (...) {
+        (void)hipStreamEndCapture(S.capture_stream, &graph);
+        if (graph) (void)hipGraphDestroy(graph);
+        throw;
+      }
+      LSMPS_HIP(hipStreamEndCapture(S.capture_stream, &graph));
+      ctx.stream = library_stream;
+      const hipError_t e = hipGraphInstantiate(&S.cycle_exec[set], graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      LSMPS_HIP(e);
+    } catch (const Error& e) {
+      // the runtime could not record or instantiate the cycle: the solve runs
+      // its launches directly (graph_cycles = 0 in the report says so)
+      ctx.stream = library_stream;
+      S.release_graphs();
+      S.graph_error = e.what();
+      (void)hipGetLastError();
+      return false;
+    }
+  }
+  S.cycle_signature = std::move(sig);
+  S.graph_error.clear();
+  return true;
+}
+
 // IDR(s) with an FP64 iterate and residual, a Krylov basis of type T and matrix
 // values of type V.  x and r hold the initial iterate and its residual.  Each time the recursive residual has
 // dropped by `replace_factor` since the last replacement (and when it meets the
@@ -1772,14 +1951,21 @@ int idr_solve(Context& ctx, PpeSolver& S, size_t n, const SellMatrix& M, const V
                             M.chunks * (6.0 + kSellWindows) * sizeof(int32_t) +
                             static_cast<double>(M.dia_entries) / kSellChunk * sizeof(int32_t) +
                             2.0 * n * sizeof(T);
-  auto mark = [&](int slot) {
-    if (timing) LSMPS_HIP(hipEventRecord(ctx.timing_events[slot], ctx.stream));
-  };
-  while (rr > target2 && it < max_iterations) {
+  // One IDR(s) cycle: s steps, each with one SpMV G_k = A U_k (and, with the
+  // Schwarz blocks, their products and the coarse term before it), then the
+  // omega step's SpMV and the residual norm.  `skip` (null when launched
+  // directly) is the captured graphs' halt word; `set` picks the SpMV events.
+  auto cycle_body = [&](const int32_t* skip, int set) {
+    // SpMV timing: HIP events around each launch (direct launches only)
+    auto mark = [&](int slot) {
+      if (timing)
+        LSMPS_HIP(hipEventRecord(ctx.timing_events[set * Context::kTimingSet + slot], ctx.stream));
+    };
+    iv.skip = skip;
     iv.partials = S.part_a.data();
     shadow_dots_kernel<T><<<vb, kBlock, 0, ctx.stream>>>(iv);
     idr_scalar_f<<<1, kReduceThreads, 0, ctx.stream>>>(S.scalars.data(), S.part_a.data(),
-                                               static_cast<int>(vb), s);
+                                                       static_cast<int>(vb), s, skip);
     // step 0's first half; later steps get it fused into the previous step's b
     if (schwarz) {
       idr_a1_kernel<T><<<vb, kBlock, 0, ctx.stream>>>(iv, 0);
@@ -1788,19 +1974,19 @@ int idr_solve(Context& ctx, PpeSolver& S, size_t n, const SellMatrix& M, const V
     }
     for (int k = 0; k < s; ++k) {
       if (schwarz) {
-        schwarz_products<T>(ctx, S.schwarz, v);
-        if (coarse) coarse_solve(ctx, S.coarse, v, true);
+        schwarz_products<T>(ctx, S.schwarz, v, skip);
+        if (coarse) coarse_solve(ctx, S.coarse, v, true, skip);
         idr_a2g_kernel<T><<<vb, kBlock, 0, ctx.stream>>>(iv, k);
       }
       SellArgs<T, V> sa{n, M.chunks, M.dia_pairs.data(), M.dia_singles.data(), M.dia_plan.data(),
                         M.dia_start.data(), dia, M.chunk_start.data(), M.chunk_width.data(),
                         M.col_start.data(), M.nwin.data(), M.windows.data(), M.cols.data(),
-                        vals, U + k * ld, G + k * ld, p0, s, nullptr, S.part_a.data()};
+                        vals, U + k * ld, G + k * ld, p0, s, nullptr, S.part_a.data(), skip};
       mark(2 * k);
       sell_spmv_kernel<T, V, 1><<<sb, kSellBlock, 0, ctx.stream>>>(sa);
       mark(2 * k + 1);
       idr_scalar_after<<<1, kReduceThreads, 0, ctx.stream>>>(S.scalars.data(), S.part_a.data(),
-                                                     static_cast<int>(sb), s, k);
+                                                             static_cast<int>(sb), s, k, skip);
       iv.partials = S.part_b.data();
       if (k == s - 1) {
         idr_b_kernel<T><<<vb, kBlock, 0, ctx.stream>>>(iv, k, 1);
@@ -1809,59 +1995,139 @@ int idr_solve(Context& ctx, PpeSolver& S, size_t n, const SellMatrix& M, const V
       } else {
         idr_ba_kernel<T, false><<<vb, kBlock, 0, ctx.stream>>>(iv, k);
       }
-      ++it;
     }
     if (schwarz) {
-      schwarz_apply<T>(ctx, S.schwarz, v);
+      schwarz_apply<T>(ctx, S.schwarz, v, skip);
       if (coarse) {
-        coarse_solve(ctx, S.coarse, r, false);
-        coarse_prolong_add(ctx, S.coarse, v);
+        coarse_solve(ctx, S.coarse, r, false, skip);
+        coarse_prolong_add(ctx, S.coarse, v, skip);
       }
     }
     SellArgs<T, V> st{n, M.chunks, M.dia_pairs.data(), M.dia_singles.data(), M.dia_plan.data(),
                       M.dia_start.data(), dia, M.chunk_start.data(), M.chunk_width.data(),
                       M.col_start.data(), M.nwin.data(), M.windows.data(), M.cols.data(), vals,
-                      v, t, nullptr, s, r, S.part_a.data()};
+                      v, t, nullptr, s, r, S.part_a.data(), skip};
     mark(2 * s);
     sell_spmv_kernel<T, V, 2><<<sb, kSellBlock, 0, ctx.stream>>>(st);
     mark(2 * s + 1);
     idr_scalar_omega<<<1, kReduceThreads, 0, ctx.stream>>>(S.scalars.data(), S.part_a.data(),
-                                                   S.part_b.data(), static_cast<int>(sb),
-                                                   static_cast<int>(vb));
+                                                           S.part_b.data(), static_cast<int>(sb),
+                                                           static_cast<int>(vb), 0, skip);
     iv.partials = S.part_b.data();
     idr_c_kernel<T><<<vb, kBlock, 0, ctx.stream>>>(iv);
     scalar_norm<<<1, kReduceThreads, 0, ctx.stream>>>(S.scalars.data(), S.part_b.data(),
-                                              static_cast<int>(vb));
+                                                      static_cast<int>(vb), skip);
     LSMPS_LAUNCH_CHECK();
-    ++it;
-    LSMPS_HIP(hipMemcpyAsync(S.host, &S.scalars.data()->rr, sizeof(double), hipMemcpyDeviceToHost,
-                             ctx.stream));
-    LSMPS_HIP(hipMemcpyAsync(S.host + 2, &S.scalars.data()->breakdown, sizeof(int32_t),
-                             hipMemcpyDeviceToHost, ctx.stream));
-    ctx.sync();
-    if (timing) {
-      for (int k = 0; k <= s; ++k) {
-        float ms = 0.0f;
-        LSMPS_HIP(hipEventElapsedTime(&ms, ctx.timing_events[2 * k], ctx.timing_events[2 * k + 1]));
-        ctx.spmv_ms += ms;
-        ctx.spmv_bytes += spmv_bytes + (k == s ? n * sizeof(double) : 0.0);  // mode 2 also reads r
-        ++ctx.spmv_launches;
+    iv.skip = nullptr;
+  };
+  // reliable update: r = b - A64 x in FP64 and its norm
+  auto replace_body = [&](const int32_t* skip) {
+    const unsigned nb = spmv_blocks(n);
+    residual_kernel<<<nb, kBlock, 0, ctx.stream>>>(n, A64.row_offsets, A64.col_indices,
+                                                   A64.values, x, b64, r, nullptr,
+                                                   S.part_b.data(), skip);
+    scalar_norm<<<1, kReduceThreads, 0, ctx.stream>>>(S.scalars.data(), S.part_b.data(),
+                                                      static_cast<int>(nb), skip);
+    LSMPS_LAUNCH_CHECK();
+  };
+  // restart after a breakdown: fresh basis, shadow space and scalars
+  auto reinit_body = [&](const int32_t* skip) {
+    fill_kernel<T><<<blocks_for(2 * s * ld), kBlock, 0, ctx.stream>>>(2 * s * ld, G, T(0), skip);
+    copy_kernel<double, T><<<blocks_for(n), kBlock, 0, ctx.stream>>>(n, r, p0, skip);
+    idr_init_scalars<<<1, kBlock, 0, ctx.stream>>>(S.scalars.data(), s, nullptr, skip);
+    LSMPS_LAUNCH_CHECK();
+  };
+  auto add_timing = [&](int set) {
+    for (int k = 0; k <= s; ++k) {
+      float ms = 0.0f;
+      LSMPS_HIP(hipEventElapsedTime(&ms, ctx.timing_events[set * Context::kTimingSet + 2 * k],
+                                    ctx.timing_events[set * Context::kTimingSet + 2 * k + 1]));
+      ctx.spmv_ms += ms;
+      ctx.spmv_bytes += spmv_bytes + (k == s ? n * sizeof(double) : 0.0);  // mode 2 also reads r
+      ++ctx.spmv_launches;
+    }
+  };
+  bool use_graph = false;
+  // HIP events recorded inside a graph cannot be timed on this runtime, so
+  // kernel timing keeps the direct launches
+  if (S.graph_requested && !timing && rr > target2 && max_iterations > 0) {
+    std::vector<uintptr_t> sig{
+        reinterpret_cast<uintptr_t>(x), reinterpret_cast<uintptr_t>(r),
+        reinterpret_cast<uintptr_t>(work), ld, n, static_cast<uintptr_t>(s),
+        static_cast<uintptr_t>(vb), static_cast<uintptr_t>(sb), reinterpret_cast<uintptr_t>(vals),
+        reinterpret_cast<uintptr_t>(dia), reinterpret_cast<uintptr_t>(dinv),
+        reinterpret_cast<uintptr_t>(A64.row_offsets), reinterpret_cast<uintptr_t>(A64.col_indices),
+        reinterpret_cast<uintptr_t>(A64.values), reinterpret_cast<uintptr_t>(b64),
+        schwarz ? 1u : 0u, coarse ? 1u : 0u, sizeof(V)};
+    append_sell_signature(sig, M);
+    if (schwarz) append_schwarz_signature(sig, S.schwarz);
+    if (coarse) append_coarse_signature(sig, S.coarse);
+    use_graph = idr_cycles_captured(
+        ctx, S,
+        [&](const int32_t* halt, const int32_t* skip_replace, const int32_t* skip_reinit,
+            int set) {
+          IdrControl* ctl = S.control.data();
+          idr_control_begin<<<1, 1, 0, ctx.stream>>>(ctl);
+          cycle_body(halt, set);
+          idr_control_cycle<<<1, 1, 0, ctx.stream>>>(ctl, S.scalars.data());
+          replace_body(skip_replace);
+          idr_control_after_replace<<<1, 1, 0, ctx.stream>>>(ctl, S.scalars.data());
+          reinit_body(skip_reinit);
+          LSMPS_LAUNCH_CHECK();
+        },
+        std::move(sig));
+  }
+  if (use_graph) {
+    // two cycles in flight: while cycle c runs, the host waits for cycle c - 1's
+    // control record and enqueues cycle c + 1; the cycle queued after the one
+    // that halts returns at once from every launch
+    idr_control_init<<<1, 1, 0, ctx.stream>>>(S.control.data(), rr, target2, factor2,
+                                              max_iterations, s);
+    LSMPS_LAUNCH_CHECK();
+    auto launch = [&](int slot) {
+      LSMPS_HIP(hipGraphLaunch(S.cycle_exec[slot], ctx.stream));
+      LSMPS_HIP(hipMemcpyAsync(S.host_control + slot, S.control.data(), sizeof(IdrControl),
+                               hipMemcpyDeviceToHost, ctx.stream));
+      LSMPS_HIP(hipEventRecord(S.cycle_done[slot], ctx.stream));
+    };
+    launch(0);
+    launch(1);
+    int replays = 2;
+    IdrControl last{};
+    for (int c = 0;; ++c) {
+      const int slot = c & 1;
+      LSMPS_HIP(hipEventSynchronize(S.cycle_done[slot]));
+      last = S.host_control[slot];
+      if (last.halt) break;
+      launch(slot);
+      ++replays;
+    }
+    it = last.iterations;
+    replacements = last.replacements;
+    rr = last.rr;
+    S.last_graph_cycles = replays;
+  } else {
+    S.last_graph_cycles = 0;
+    while (rr > target2 && it < max_iterations) {
+      cycle_body(nullptr, 0);
+      it += s + 1;
+      LSMPS_HIP(hipMemcpyAsync(S.host, &S.scalars.data()->rr, sizeof(double),
+                               hipMemcpyDeviceToHost, ctx.stream));
+      LSMPS_HIP(hipMemcpyAsync(S.host + 2, &S.scalars.data()->breakdown, sizeof(int32_t),
+                               hipMemcpyDeviceToHost, ctx.stream));
+      ctx.sync();
+      if (timing) add_timing(0);
+      rr = S.host[0];
+      int32_t breakdown = 0;
+      std::memcpy(&breakdown, S.host + 2, sizeof(int32_t));
+      if (!std::isfinite(rr)) break;
+      if (rr <= target2 || rr <= factor2 * rr_replaced) {
+        replace_body(nullptr);
+        rr = read_rr(ctx, S);
+        rr_replaced = rr;
+        ++replacements;
       }
-    }
-    rr = S.host[0];
-    int32_t breakdown = 0;
-    std::memcpy(&breakdown, S.host + 2, sizeof(int32_t));
-    if (!std::isfinite(rr)) break;
-    if (rr <= target2 || rr <= factor2 * rr_replaced) {
-      rr = true_residual(ctx, S, A64, x, b64, r, nullptr);
-      rr_replaced = rr;
-      ++replacements;
-    }
-    if (breakdown) {
-      fill_kernel<T><<<blocks_for(2 * s * ld), kBlock, 0, ctx.stream>>>(2 * s * ld, G, T(0));
-      copy_kernel<double, T><<<blocks_for(n), kBlock, 0, ctx.stream>>>(n, r, p0);
-      idr_init_scalars<<<1, kBlock, 0, ctx.stream>>>(S.scalars.data(), s);
-      LSMPS_LAUNCH_CHECK();
+      if (breakdown) reinit_body(nullptr);
     }
   }
   *iterations_out = it;
@@ -2195,6 +2461,7 @@ void ppe_solve(Context& ctx, PpeSolver& S, const lsmps_csr& A, const double* b, 
                                             opt.tolerance, max_it, A, b, factor, rep);
       }
     } else {
+      S.graph_requested = ctx.graph();
       if (mixed) {
         idr_solve<double, float>(ctx, S, n, S.sell, S.sell.vals32.data(), S.dinv64.data(), x,
                                  r, r + ld, ld, shadow_dim, bnorm * bnorm, target, max_it, A, b,
@@ -2206,6 +2473,7 @@ void ppe_solve(Context& ctx, PpeSolver& S, const lsmps_csr& A, const double* b, 
       }
       rep.iterations = it;
       rep.refinements = replacements;
+      rep.graph_cycles = S.last_graph_cycles;
       if (S.coarse.ready && coarse_usable(ctx, S.coarse)) rep.coarse_unknowns = S.coarse.count;
       rep.relative_residual = std::sqrt(rr) / bnorm;
       rep.converged = rep.relative_residual <= opt.tolerance ? 1 : 0;

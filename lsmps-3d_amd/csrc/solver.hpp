@@ -1,6 +1,9 @@
 // PPE Krylov solver workspaces (see solver.hip).
 #pragma once
 
+#include <string>
+#include <vector>
+
 #include "common.hpp"
 #include "coarse.hpp"
 #include "schwarz.hpp"
@@ -28,6 +31,27 @@ struct SolverScalars {
   double cg_beta;
   int32_t breakdown;
   int32_t done;     // batched solves: this segment has converged and is frozen
+};
+
+// Device side of the IDR loop's per-cycle decisions, for cycles replayed from
+// a captured graph: the same tests the host loop makes after each cycle
+// (non-finite residual, reliable update, restart after a breakdown, tolerance
+// and iteration limit), made by one-thread kernels at the end of the cycle.
+// `halt` is the skip word of every captured launch; skip_replace and
+// skip_reinit gate the true-residual and restart launches of the cycle.
+struct IdrControl {
+  int32_t halt;
+  int32_t skip_replace;
+  int32_t skip_reinit;
+  int32_t iterations;
+  int32_t replacements;
+  int32_t max_iterations;
+  int32_t s;
+  int32_t pad;
+  double rr;           // current squared residual norm (true after a replacement)
+  double rr_replaced;  // at the last replacement
+  double target2;
+  double factor2;
 };
 
 // PPE matrix for the Krylov SpMVs: a hybrid of a per-chunk diagonal (DIA)
@@ -134,6 +158,19 @@ class PpeSolver {
   DeviceArray<double> part_b;
   DeviceArray<SolverScalars> scalars;
   double* host = nullptr;      // pinned: [0] rr, [1] bnorm2, ...
+  // captured IDR cycles (LSMPS_FLAG_GRAPH): two executable graphs of the same
+  // cycle, replayed alternately so that each in-flight cycle has its own
+  // control record slot; the signature lists every pointer and size baked in
+  DeviceArray<IdrControl> control;
+  IdrControl* host_control = nullptr;  // pinned, one slot per graph
+  hipGraphExec_t cycle_exec[2] = {nullptr, nullptr};
+  hipEvent_t cycle_done[2] = {nullptr, nullptr};
+  hipStream_t capture_stream = nullptr;
+  std::vector<uintptr_t> cycle_signature;
+  bool graph_requested = false;  // set per solve from the context's LSMPS_FLAG_GRAPH
+  int last_graph_cycles = 0;     // cycles the last solve replayed from the graphs
+  std::string graph_error;       // why the last capture failed (empty: none)
+  void release_graphs() noexcept;
   PpeSolver() = default;
   ~PpeSolver();
   PpeSolver(const PpeSolver&) = delete;

@@ -619,9 +619,9 @@ def default_solver(precision=abi.PRECISION_FP64):
 
 
 def run_step_on_device(case, precision=abi.PRECISION_FP64, solver=None, max_neighbors=256,
-                       device="cuda:0"):
+                       device="cuda:0", flags=0):
     """One fused step of `case` on the GPU; returns every inspectable output as numpy."""
-    ctx = Context(precision)
+    ctx = Context(precision, flags=flags)
     state = DeviceState(case, device)
     st = Stepper(ctx, state, case.config, max_neighbors, solver or abi.default_solver_options())
     if "fluid_offsets" in case.extra:  # batched tanks
