@@ -53,6 +53,11 @@ def parse():
                         "which is what the HIP-event SpMV timing needs")
     p.add_argument("--no-adjoint", action="store_true",
                    help="skip timing the adjoint of one step (after the timed steps)")
+    p.add_argument("--no-autotune", action="store_true",
+                   help="fixed lanes per particle instead of the autotuner's choice "
+                        "(LSMPS_FLAG_AUTOTUNE, tuned in the warm-up steps)")
+    p.add_argument("--inner-tolerance", type=float, default=0.0,
+                   help="mixed mode: residual drop between reliable updates (0: the default)")
     p.add_argument("--cpu-threads", type=int, default=0)
     p.add_argument("--cpu-repeats", type=int, default=5,
                    help="full oracle steps timed on the host (the median is reported)")
@@ -77,11 +82,12 @@ def make_case(cases, name):
 
 
 def run_gpu(pkg, rt, case, precision, steps, warmup, timing=True, adjoint=False, shadow=0,
-            coarse=0, graph=True):
+            coarse=0, graph=True, inner_tolerance=0.0, autotune=True):
     import numpy as np
     import torch
     abi = pkg.abi
-    flags = (abi.FLAG_KERNEL_TIMING if timing else 0) | (abi.FLAG_GRAPH if graph else 0)
+    flags = ((abi.FLAG_KERNEL_TIMING if timing else 0) | (abi.FLAG_GRAPH if graph else 0) |
+             (abi.FLAG_AUTOTUNE if autotune else 0))
     ctx = rt.Context(precision, flags=flags)
     state = rt.DeviceState(case)
     opts = abi.default_solver_options()
@@ -91,6 +97,8 @@ def run_gpu(pkg, rt, case, precision, steps, warmup, timing=True, adjoint=False,
         opts.coarse_cells = coarse
     elif coarse < 0:
         opts.preconditioner = abi.PRECOND_SCHWARZ
+    if inner_tolerance > 0:
+        opts.inner_tolerance = inner_tolerance
     st = rt.Stepper(ctx, state, case.config, 256, opts)
     if "fluid_offsets" in case.extra:  # batched tanks
         st.set_segments(case.extra["fluid_offsets"], case.extra["wall_offsets"],
@@ -123,6 +131,7 @@ def run_gpu(pkg, rt, case, precision, steps, warmup, timing=True, adjoint=False,
     rt.check(rt.load().lsmps_context_kernel_stats(ctx.handle, C.byref(launches), C.byref(ms),
                                                   C.byref(nbytes), 0))
     pressure = state.fluid["pressure"].cpu().numpy()
+    lanes, tuned_keys = ctx.lanes()
     adj = None
     if adjoint:
         # one recorded step, then the reverse mode of it (outside the timed region)
@@ -152,6 +161,8 @@ def run_gpu(pkg, rt, case, precision, steps, warmup, timing=True, adjoint=False,
         "pressure": pressure,
         "bytes": st.bytes(),
         "adjoint": adj,
+        "lanes": lanes,
+        "tuned_keys": tuned_keys,
     }
     st.close()
     ctx.close()
@@ -307,14 +318,16 @@ def main():
           f"grid {list(case.config.cell_dims)}", file=sys.stderr, flush=True)
     main_run = run_gpu(pkg, rt, case, prec, args.steps, args.warmup,
                        adjoint=not args.no_adjoint, shadow=args.shadow_dimension,
-                       coarse=args.coarse_cells, graph=False)
+                       coarse=args.coarse_cells, graph=False,
+                       inner_tolerance=args.inner_tolerance, autotune=not args.no_autotune)
     iters = [r["iterations"] for r in main_run["reports"]]
     print(f"[bench] {args.precision}: {main_run['ms_per_step']:.1f} ms/step, iterations {iters}, "
           f"stages {np.round(main_run['stage_ms'], 2).tolist()}", file=sys.stderr, flush=True)
     fp64_run = None
     if not args.no_fp64 and prec != abi.PRECISION_FP64:
         fp64_run = run_gpu(pkg, rt, case, abi.PRECISION_FP64, 5, 1,
-                           shadow=args.shadow_dimension, coarse=args.coarse_cells, graph=False)
+                           shadow=args.shadow_dimension, coarse=args.coarse_cells, graph=False,
+                           autotune=not args.no_autotune)
         print(f"[bench] fp64: {fp64_run['ms_per_step']:.1f} ms/step, iterations "
               f"{[r['iterations'] for r in fp64_run['reports']]}", file=sys.stderr, flush=True)
     traffic, traffic_source = measured_traffic(args.case, args.precision)
@@ -346,7 +359,8 @@ def main():
                    "re_over_l0": 3.1, "dt_s": case.config.time_step, "solver": f"IDR({args.shadow_dimension or (16 if args.coarse_cells < 0 else 14)}); Schwarz blocks on wrong-sign rows, Jacobi elsewhere" + ("" if args.coarse_cells < 0 else f", coarse space of {args.coarse_cells or pkg.abi.default_solver_options().coarse_cells}^3-cell aggregates"),
                    "tolerance": 1e-10, "precision": args.precision},
         "solver_iterations": iters,
-        "graph_cycles": [r["graph_cycles"] for r in main_run["reports"]],
+        "lanes_per_particle": main_run["lanes"],
+        "autotuned_keys": main_run["tuned_keys"] if not args.no_autotune else 0,
         "schwarz_blocks": main_run["reports"][-1]["schwarz_blocks"],
         "schwarz_rows": main_run["reports"][-1]["schwarz_rows"],
         "coarse_unknowns": main_run["reports"][-1]["coarse_unknowns"],
@@ -372,7 +386,9 @@ def main():
             np.linalg.norm(main_run["pressure"] - p64) / max(np.linalg.norm(p64), 1e-300))
     if not args.no_graph:
         graph_run = run_gpu(pkg, rt, case, prec, 5, 1, timing=False,
-                            shadow=args.shadow_dimension, coarse=args.coarse_cells, graph=True)
+                            shadow=args.shadow_dimension, coarse=args.coarse_cells, graph=True,
+                            inner_tolerance=args.inner_tolerance,
+                            autotune=not args.no_autotune)
         line["graph"] = {
             "ms_per_step": graph_run["ms_per_step"],
             "graph_cycles": [r["graph_cycles"] for r in graph_run["reports"]],

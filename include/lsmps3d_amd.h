@@ -93,7 +93,28 @@ enum { LSMPS_SOLVER_PCG = 0, LSMPS_SOLVER_BICGSTAB = 1, LSMPS_SOLVER_IDRS = 2 };
  * reading the residual on the host after each cycle; results are bit-identical
  * to the launch path.  With LSMPS_FLAG_KERNEL_TIMING the cycles are launched
  * directly (events recorded inside a graph cannot be timed). */
-enum { LSMPS_FLAG_ASYNC = 1, LSMPS_FLAG_KERNEL_TIMING = 2, LSMPS_FLAG_GRAPH = 4 };
+/* LSMPS_FLAG_AUTOTUNE: the neighbour-list kernels (neighbour count and fill
+ * passes, provision, PPE assembly, pressure gradient, shifting, smoothing) pick
+ * their lanes per particle by timing each candidate once per neighbour-count
+ * histogram of the list they read; every choice computes the same result. */
+enum {
+  LSMPS_FLAG_ASYNC = 1,
+  LSMPS_FLAG_KERNEL_TIMING = 2,
+  LSMPS_FLAG_GRAPH = 4,
+  LSMPS_FLAG_AUTOTUNE = 8
+};
+
+/* Kernels the autotuner shapes (lsmps_context_lanes / lsmps_context_force_lanes). */
+enum {
+  LSMPS_TUNE_SCAN_COUNT = 0,
+  LSMPS_TUNE_SCAN_WRITE = 1,
+  LSMPS_TUNE_PROVISION = 2,
+  LSMPS_TUNE_ASSEMBLY = 3,
+  LSMPS_TUNE_GRADIENT = 4,
+  LSMPS_TUNE_SHIFTING = 5,
+  LSMPS_TUNE_SMOOTHING = 6,
+  LSMPS_TUNE_KERNELS = 7
+};
 
 /* ---- data contract (POD mirrors of the reference views) ------------------ */
 
@@ -311,6 +332,16 @@ int lsmps_context_destroy(lsmps_context* ctx);
 int lsmps_context_set_stream(lsmps_context* ctx, void* hip_stream);
 int lsmps_context_synchronize(lsmps_context* ctx);
 int lsmps_context_precision(const lsmps_context* ctx);
+/* No reference counterpart (the reference launches 128 threads, one particle
+ * each, neighbor_search.cu:16).  Lanes per particle of the latest launch of a
+ * LSMPS_TUNE_* kernel on this context (0 before its first launch), and how many
+ * neighbour-count histograms the autotuner has timed so far. */
+int lsmps_context_lanes(lsmps_context* ctx, int32_t kernel, int32_t* lanes,
+                        int64_t* tuned_keys);
+/* Pins a LSMPS_TUNE_* kernel to `lanes` per particle (one of its candidates:
+ * count pass 1/2/4, fill pass 4/8/16, assembly 4/8, the others 4/8/16); 0
+ * returns it to the default or the autotuner.  Results do not depend on it. */
+int lsmps_context_force_lanes(lsmps_context* ctx, int32_t kernel, int32_t lanes);
 /* PPE SpMV statistics gathered under LSMPS_FLAG_KERNEL_TIMING since the last
  * reset: launches, summed duration (ms) and summed algorithmic bytes
  * (values + column indices + row pointers, each vector element read or written

@@ -43,6 +43,14 @@ STEP_FROZEN_GEOMETRY = 2
 FLAG_ASYNC = 1
 FLAG_KERNEL_TIMING = 2
 FLAG_GRAPH = 4
+FLAG_AUTOTUNE = 8
+
+# kernels the autotuner shapes (LSMPS_TUNE_*) and their lanes-per-particle candidates
+TUNED_KERNELS = {"scan_count": 0, "scan_write": 1, "provision": 2, "assembly": 3,
+                 "gradient": 4, "shifting": 5, "smoothing": 6}
+TUNED_CANDIDATES = {"scan_count": (1, 2, 4), "scan_write": (4, 8, 16), "provision": (4, 8, 16),
+                    "assembly": (4, 8), "gradient": (4, 8, 16), "shifting": (4, 8, 16),
+                    "smoothing": (4, 8, 16)}
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)

@@ -31,6 +31,7 @@ SOURCES = {
     "coarse.hip": ["-ffp-contract=fast"],
     "adjoint.hip": ["-ffp-contract=fast"],
     "driver.hip": ["-ffp-contract=off"],
+    "tune.hip": ["-ffp-contract=fast"],
     "stepper.cpp": [],
     "io.cpp": [],
     "capi.cpp": [],

@@ -9,6 +9,7 @@
 #include <string>
 
 #include "objects.hpp"
+#include "tune.hpp"
 
 using namespace lsmps;
 
@@ -75,6 +76,8 @@ void release_context(lsmps_context* ctx) {
     for (auto& e : ctx->impl.timing_events) (void)hipEventDestroy(e);
   if (ctx->impl.host_i32) (void)hipHostFree(ctx->impl.host_i32);
   if (ctx->impl.host_f64) (void)hipHostFree(ctx->impl.host_f64);
+  for (auto& e : ctx->impl.tune.ev)
+    if (e) (void)hipEventDestroy(e);
   delete ctx;
 }
 }  // namespace
@@ -102,6 +105,7 @@ int lsmps_context_create(int device, int precision, uint32_t flags, lsmps_contex
     LSMPS_HIP(hipHostMalloc(reinterpret_cast<void**>(&ctx->impl.host_i32), 64));
     LSMPS_HIP(hipHostMalloc(reinterpret_cast<void**>(&ctx->impl.host_f64), 64));
     noalloc_prepare_device(device);
+    tune_prepare(ctx->impl);
   } catch (...) {
     delete ctx;
     throw;
@@ -121,6 +125,26 @@ int lsmps_context_set_stream(lsmps_context* ctx, void* hip_stream) {
   API_BEGIN
   C(ctx).stream = static_cast<hipStream_t>(hip_stream);
   API_END("lsmps_context_set_stream")
+}
+
+int lsmps_context_lanes(lsmps_context* ctx, int32_t kernel, int32_t* lanes,
+                        int64_t* tuned_keys) {
+  API_BEGIN
+  Context& c = C(ctx);
+  require(kernel >= 0 && kernel < kTunedKernels, "unknown tuned kernel");
+  if (lanes) *lanes = c.tune.last[kernel];
+  if (tuned_keys) *tuned_keys = c.tune.tuned_keys;
+  API_END("lsmps_context_lanes")
+}
+
+int lsmps_context_force_lanes(lsmps_context* ctx, int32_t kernel, int32_t lanes) {
+  API_BEGIN
+  Context& c = C(ctx);
+  require(kernel >= 0 && kernel < kTunedKernels, "unknown tuned kernel");
+  require(lanes >= 0 && lanes <= 16 && (lanes & (lanes - 1)) == 0,
+          "lanes per particle must be 0 or a power of two up to 16");
+  c.tune.forced[kernel] = lanes;
+  API_END("lsmps_context_force_lanes")
 }
 
 int lsmps_context_synchronize(lsmps_context* ctx) {

@@ -467,9 +467,10 @@ __device__ __forceinline__ double block_sum(double s, double* red) {
 
 // rc[I] = sum over the members i of aggregate I of w_i (times wfac_i); one
 // block per aggregate, two members per thread in flight.
+template <typename W>
 __global__ void __launch_bounds__(kBlock) restrict_kernel(int count, const int32_t* agg_start,
                                                           const int32_t* agg_rows,
-                                                          const double* w, const double* wfac,
+                                                          const W* w, const double* wfac,
                                                           double* rc,
                                                           const int32_t* skip = nullptr) {
   if (launch_skipped(skip)) return;
@@ -490,7 +491,7 @@ __global__ void __launch_bounds__(kBlock) restrict_kernel(int count, const int32
     double v[kInFlight], f[kInFlight];
 #pragma unroll
     for (int u = 0; u < kInFlight; ++u) {
-      v[u] = idx[u] >= 0 ? w[idx[u]] : 0.0;
+      v[u] = idx[u] >= 0 ? static_cast<double>(w[idx[u]]) : 0.0;
       f[u] = idx[u] >= 0 && wfac ? wfac[idx[u]] : 1.0;
     }
 #pragma unroll
@@ -527,14 +528,15 @@ __global__ void __launch_bounds__(kBlock) coarse_product_kernel(int count, const
   if (threadIdx.x == 0) yc[I] = s;
 }
 
+template <typename W>
 __global__ void __launch_bounds__(kBlock) prolong_add_kernel(size_t n, const int32_t* agg_of_row,
-                                                             const double* yc, double* v,
+                                                             const double* yc, W* v,
                                                              const int32_t* skip = nullptr) {
   if (launch_skipped(skip)) return;
   const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int32_t a = agg_of_row[i];
-  if (a >= 0) v[i] += yc[a];
+  if (a >= 0) v[i] = static_cast<W>(static_cast<double>(v[i]) + yc[a]);
 }
 
 unsigned wave_blocks(size_t items) { return static_cast<unsigned>((items + kWaves - 1) / kWaves); }
@@ -625,7 +627,7 @@ void coarse_setup(Context& ctx, CoarseSpace& C, const lsmps_csr& A, const double
   unsigned long long* acci = reinterpret_cast<unsigned long long*>(C.ainv.data());
   row_bound_kernel<<<blocks_for(A.rows), kBlock, 0, st>>>(
       A.rows, C.agg_of_row.data(), diag, offabs, C.rowabs.data());
-  restrict_kernel<<<static_cast<unsigned>(count), kBlock, 0, st>>>(
+  restrict_kernel<double><<<static_cast<unsigned>(count), kBlock, 0, st>>>(
       count, C.agg_start.data(), C.agg_rows.data(), C.rowabs.data(), nullptr, C.rc.data());
   fill_u64<<<blocks_for(nn), kBlock, 0, st>>>(acci, nn);
   coarse_matrix_kernel<<<static_cast<unsigned>(count) * kMatrixParts, kBlock, 0, st>>>(
@@ -678,9 +680,10 @@ bool coarse_usable(Context& ctx, const CoarseSpace& C) {
   return ctx.host_i32[0] == 0;
 }
 
-void coarse_solve(Context& ctx, CoarseSpace& C, const double* w, bool scale_by_wfac,
+template <typename W>
+void coarse_solve(Context& ctx, CoarseSpace& C, const W* w, bool scale_by_wfac,
                   const int32_t* skip) {
-  restrict_kernel<<<static_cast<unsigned>(C.count), kBlock, 0, ctx.stream>>>(
+  restrict_kernel<W><<<static_cast<unsigned>(C.count), kBlock, 0, ctx.stream>>>(
       C.count, C.agg_start.data(), C.agg_rows.data(), w, scale_by_wfac ? C.wfac.data() : nullptr,
       C.rc.data(), skip);
   coarse_product_kernel<<<static_cast<unsigned>(C.count), kBlock, 0, ctx.stream>>>(
@@ -688,10 +691,16 @@ void coarse_solve(Context& ctx, CoarseSpace& C, const double* w, bool scale_by_w
   LSMPS_LAUNCH_CHECK();
 }
 
-void coarse_prolong_add(Context& ctx, const CoarseSpace& C, double* v, const int32_t* skip) {
-  prolong_add_kernel<<<blocks_for(C.n), kBlock, 0, ctx.stream>>>(C.n, C.agg_of_row.data(),
-                                                                 C.yc.data(), v, skip);
+template <typename W>
+void coarse_prolong_add(Context& ctx, const CoarseSpace& C, W* v, const int32_t* skip) {
+  prolong_add_kernel<W><<<blocks_for(C.n), kBlock, 0, ctx.stream>>>(C.n, C.agg_of_row.data(),
+                                                                    C.yc.data(), v, skip);
   LSMPS_LAUNCH_CHECK();
 }
+
+template void coarse_solve<double>(Context&, CoarseSpace&, const double*, bool, const int32_t*);
+template void coarse_solve<float>(Context&, CoarseSpace&, const float*, bool, const int32_t*);
+template void coarse_prolong_add<double>(Context&, const CoarseSpace&, double*, const int32_t*);
+template void coarse_prolong_add<float>(Context&, const CoarseSpace&, float*, const int32_t*);
 
 }  // namespace lsmps

@@ -76,11 +76,12 @@ void coarse_set_weights(Context& ctx, CoarseSpace& C, const double* pdiag);
 bool coarse_usable(Context& ctx, const CoarseSpace& C);
 
 // yc = Ac^-1 P^T (wfac * w) (wfac may be skipped: then yc = Ac^-1 P^T w).
-void coarse_solve(Context& ctx, CoarseSpace& C, const double* w, bool scale_by_wfac,
+template <typename W>
+void coarse_solve(Context& ctx, CoarseSpace& C, const W* w, bool scale_by_wfac,
                   const int32_t* skip = nullptr);
 
 // v[i] += yc[agg_of_row[i]] on the aggregated rows.
-void coarse_prolong_add(Context& ctx, const CoarseSpace& C, double* v,
-                        const int32_t* skip = nullptr);
+template <typename W>
+void coarse_prolong_add(Context& ctx, const CoarseSpace& C, W* v, const int32_t* skip = nullptr);
 
 }  // namespace lsmps

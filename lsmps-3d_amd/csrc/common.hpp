@@ -7,8 +7,11 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <array>
 #include <atomic>
+#include <map>
 #include <memory>
+#include <vector>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -127,6 +130,27 @@ void copy_f64(hipStream_t s, double* dst, const double* src, size_t n);
 struct Context;
 void exclusive_scan_i32(Context& ctx, const int32_t* in, int32_t* out, size_t n);
 
+// Lanes-per-particle choices of the neighbour-list kernels (tune.hpp).
+enum TunedKernel : int {
+  kTuneScanCount = 0,  // neighbour count pass (one table)
+  kTuneScanWrite,      // neighbour fill pass
+  kTuneProvision,
+  kTuneAssembly,
+  kTuneGradient,
+  kTuneShifting,
+  kTuneSmoothing,
+  kTunedKernels
+};
+
+struct TuneTable {
+  std::map<std::vector<int32_t>, int> best;  // key -> lanes
+  std::array<int, kTunedKernels> forced{};   // > 0: every launch uses this L
+  std::array<int, kTunedKernels> last{};     // L of the latest launch
+  int64_t tuned_keys = 0;                    // keys timed so far
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  DeviceArray<int32_t> hist;                 // row-length histogram of the latest key
+};
+
 struct Context {
   int device = 0;
   int precision = LSMPS_PRECISION_FP64;
@@ -155,6 +179,8 @@ struct Context {
 
   bool timing() const { return (flags & LSMPS_FLAG_KERNEL_TIMING) != 0; }
   bool graph() const { return (flags & LSMPS_FLAG_GRAPH) != 0; }
+  bool autotune() const { return (flags & LSMPS_FLAG_AUTOTUNE) != 0; }
+  TuneTable tune;
 
   void activate() const { LSMPS_HIP(hipSetDevice(device)); }
   void finish() const {

@@ -4,9 +4,12 @@
 // and its free functions (:647-734).  Kernel order is the reference's:
 // clamp -> pressure gradient -> particle shifting -> velocity/position update ->
 // anti-penetration -> velocity smoothing (+ copy back).
+#include <type_traits>
+
 #include "common.hpp"
 #include "device_math.hpp"
 #include "stages.hpp"
+#include "tune.hpp"
 
 namespace lsmps {
 namespace {
@@ -53,22 +56,18 @@ struct GradArgs {
 };
 
 // compute_pressure_gradient_kernel (pressure_correction.cu:200-276)
-// The neighbour-sum kernels give each particle kCorrLanes adjacent lanes: lane
-// l takes neighbours l, l + kCorrLanes, ... (contiguous index loads per
-// wavefront, a particle's gathers in flight together), and the sums meet in a
-// fixed xor butterfly before lane 0 finishes the particle.
-constexpr int kCorrLanes = 8;
+// The neighbour-sum kernels give each particle L adjacent lanes that take its
+// neighbours round-robin (contiguous index loads per wavefront, a particle's
+// gathers in flight together); the sums follow the 16-slot tree of SlotSums, so
+// every L gives the same result, and lane 0 finishes the particle.
+constexpr int kCorrSlots = 16;
 
-__device__ __forceinline__ double group_sum(double v) {
-#pragma unroll
-  for (int off = kCorrLanes / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
-}
-
+template <int L>
 __global__ void __launch_bounds__(kBlock) gradient_kernel(GradArgs a) {
+  using Sums = SlotSums<L, kCorrSlots>;
   const size_t t = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  const size_t i = t / kCorrLanes;
-  const int sub = static_cast<int>(t % kCorrLanes);
+  const size_t i = t / L;
+  const int sub = static_cast<int>(t % L);
   if (i >= a.n) return;  // whole groups
   if (is_splash(a.st, i)) {
     if (sub != 0) return;
@@ -79,42 +78,44 @@ __global__ void __launch_bounds__(kBlock) gradient_kernel(GradArgs a) {
   }
   const double xi = a.x[i], yi = a.y[i], zi = a.z[i], pi = a.p[i];
   const double inv_re = 1.0 / a.re;
-  double rhs[9];
+  double rhs[Sums::M][9];
 #pragma unroll
-  for (int k = 0; k < 9; ++k) rhs[k] = 0.0;
-  double b[9];
-  for (int32_t c = a.f_off[i] + sub; c < a.f_off[i + 1]; c += kCorrLanes) {
+  for (int m = 0; m < Sums::M; ++m)
+#pragma unroll
+    for (int k = 0; k < 9; ++k) rhs[m][k] = 0.0;
+  Sums::for_each(a.f_off[i], a.f_off[i + 1], sub, [&](int32_t c, int m) {
     const int32_t j = a.f_idx[c];
     const double dx = a.x[j] - xi, dy = a.y[j] - yi, dz = a.z[j] - zi;
     const double w = correction_weight_of(sqrt(dx * dx + dy * dy + dz * dz), a.re, a.kernel);
-    if (w <= 0.0) continue;
+    if (w <= 0.0) return;
+    double b[9];
     type_a_basis(dx, dy, dz, inv_re, b);
     const double dp = a.p[j] - pi;
 #pragma unroll
-    for (int k = 0; k < 9; ++k) rhs[k] += w * b[k] * dp;
-  }
-  for (int32_t c = a.w_off[i] + sub; c < a.w_off[i + 1]; c += kCorrLanes) {
+    for (int k = 0; k < 9; ++k) rhs[m][k] += w * b[k] * dp;
+  });
+  Sums::for_each(a.w_off[i], a.w_off[i + 1], sub, [&](int32_t c, int m) {
     const int32_t j = a.w_idx[c];
     const double dx = a.wx[j] - xi, dy = a.wy[j] - yi, dz = a.wz[j] - zi;
     const double w =
         a.wall_scale * correction_weight_of(sqrt(dx * dx + dy * dy + dz * dz), a.re, a.kernel);
-    if (w <= 0.0) continue;
+    if (w <= 0.0) return;
     const double nx = a.wnx[j], ny = a.wny[j], nz = a.wnz[j];
+    double b[9];
     wall_neumann_q(dx, dy, dz, nx, ny, nz, inv_re, b);
     const double ng = a.g[0] * nx + a.g[1] * ny + a.g[2] * nz;
     const double wall_rhs = a.re * a.density * ng;
 #pragma unroll
-    for (int k = 0; k < 9; ++k) rhs[k] += w * b[k] * wall_rhs;
-  }
-#pragma unroll
-  for (int k = 0; k < 9; ++k) rhs[k] = group_sum(rhs[k]);
+    for (int k = 0; k < 9; ++k) rhs[m][k] += w * b[k] * wall_rhs;
+  });
+  Sums::reduce(rhs);
   if (sub != 0) return;
   double r0 = 0.0, r1 = 0.0, r2 = 0.0;
 #pragma unroll
   for (int k = 0; k < 9; ++k) {
-    r0 += a.ops[static_cast<size_t>(k) * a.n + i] * rhs[k];
-    r1 += a.ops[static_cast<size_t>(9 + k) * a.n + i] * rhs[k];
-    r2 += a.ops[static_cast<size_t>(18 + k) * a.n + i] * rhs[k];
+    r0 += a.ops[static_cast<size_t>(k) * a.n + i] * rhs[0][k];
+    r1 += a.ops[static_cast<size_t>(9 + k) * a.n + i] * rhs[0][k];
+    r2 += a.ops[static_cast<size_t>(18 + k) * a.n + i] * rhs[0][k];
   }
   a.gx[i] = r0 / a.re;
   a.gy[i] = r1 / a.re;
@@ -146,10 +147,12 @@ struct ShiftArgs {
 };
 
 // compute_particle_shifting_kernel (pressure_correction.cu:278-367)
+template <int L>
 __global__ void __launch_bounds__(kBlock) shifting_kernel(ShiftArgs a) {
+  using Sums = SlotSums<L, kCorrSlots>;
   const size_t t = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  const size_t i = t / kCorrLanes;
-  const int sub = static_cast<int>(t % kCorrLanes);
+  const size_t i = t / L;
+  const int sub = static_cast<int>(t % L);
   if (i >= a.n) return;  // whole groups
   const int32_t st = a.st ? a.st[i] : LSMPS_SURFACE_INNER;
   if (a.scale <= 0.0 || st == LSMPS_SURFACE_SPLASH) {
@@ -160,37 +163,39 @@ __global__ void __launch_bounds__(kBlock) shifting_kernel(ShiftArgs a) {
     return;
   }
   const double xi = a.x[i], yi = a.y[i], zi = a.z[i];
-  double shx = 0.0, shy = 0.0, shz = 0.0, nx = 0.0, ny = 0.0, nz = 0.0;
-  for (int32_t c = a.f_off[i] + sub; c < a.f_off[i + 1]; c += kCorrLanes) {
+  // per slot: shift x, y, z, then the fluid normal sum x, y, z
+  double acc[Sums::M][6];
+#pragma unroll
+  for (int m = 0; m < Sums::M; ++m)
+#pragma unroll
+    for (int k = 0; k < 6; ++k) acc[m][k] = 0.0;
+  Sums::for_each(a.f_off[i], a.f_off[i + 1], sub, [&](int32_t c, int m) {
     const int32_t j = a.f_idx[c];
     const double dx = xi - a.x[j], dy = yi - a.y[j], dz = zi - a.z[j];
     const double d = sqrt(dx * dx + dy * dy + dz * dz);
-    if (d <= 0.0 || d >= a.min_distance) continue;
+    if (d <= 0.0 || d >= a.min_distance) return;
     const double push = a.scale * (a.min_distance - d) / d;
-    shx += push * dx;
-    shy += push * dy;
-    shz += push * dz;
-    nx += dx / d;
-    ny += dy / d;
-    nz += dz / d;
-  }
-  for (int32_t c = a.w_off[i] + sub; c < a.w_off[i + 1]; c += kCorrLanes) {
+    acc[m][0] += push * dx;
+    acc[m][1] += push * dy;
+    acc[m][2] += push * dz;
+    acc[m][3] += dx / d;
+    acc[m][4] += dy / d;
+    acc[m][5] += dz / d;
+  });
+  Sums::for_each(a.w_off[i], a.w_off[i + 1], sub, [&](int32_t c, int m) {
     const int32_t j = a.w_idx[c];
     const double dx = xi - a.wx[j], dy = yi - a.wy[j], dz = zi - a.wz[j];
     const double d = sqrt(dx * dx + dy * dy + dz * dz);
-    if (d <= 0.0 || d >= a.min_distance) continue;
+    if (d <= 0.0 || d >= a.min_distance) return;
     const double push = a.scale * (a.min_distance - d);
-    shx += push * a.wnx[j];
-    shy += push * a.wny[j];
-    shz += push * a.wnz[j];
-  }
-  shx = group_sum(shx);
-  shy = group_sum(shy);
-  shz = group_sum(shz);
-  nx = group_sum(nx);
-  ny = group_sum(ny);
-  nz = group_sum(nz);
+    acc[m][0] += push * a.wnx[j];
+    acc[m][1] += push * a.wny[j];
+    acc[m][2] += push * a.wnz[j];
+  });
+  Sums::reduce(acc);
   if (sub != 0) return;
+  double shx = acc[0][0], shy = acc[0][1], shz = acc[0][2];
+  double nx = acc[0][3], ny = acc[0][4], nz = acc[0][5];
   if (st == LSMPS_SURFACE_SURFACE || st == LSMPS_SURFACE_SPLASH) {
     const double nn = sqrt(nx * nx + ny * ny + nz * nz);
     if (nn > 0.0) {
@@ -278,14 +283,16 @@ __global__ void antipen_kernel(size_t n, double* x, double* y, double* z, double
 }
 
 // smooth_velocity_kernel (pressure_correction.cu:446-504)
+template <int L>
 __global__ void __launch_bounds__(kBlock) smooth_kernel(
     size_t n, const double* x, const double* y, const double* z, const double* vx,
     const double* vy, const double* vz, const int32_t* st, const int32_t* f_off,
     const int32_t* f_idx, double re, double strength, int32_t kernel, double* ox, double* oy,
     double* oz) {
+  using Sums = SlotSums<L, kCorrSlots>;
   const size_t t = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  const size_t i = t / kCorrLanes;
-  const int sub = static_cast<int>(t % kCorrLanes);
+  const size_t i = t / L;
+  const int sub = static_cast<int>(t % L);
   if (i >= n) return;  // whole groups
   if (strength <= 0.0 || is_splash(st, i)) {
     if (sub != 0) return;
@@ -294,24 +301,25 @@ __global__ void __launch_bounds__(kBlock) smooth_kernel(
     oz[i] = vz[i];
     return;
   }
-  double ws = 0.0, ax = 0.0, ay = 0.0, az = 0.0;
+  // per slot: weight sum, then the weighted velocity sums
+  double acc[Sums::M][4];
+#pragma unroll
+  for (int m = 0; m < Sums::M; ++m) acc[m][0] = acc[m][1] = acc[m][2] = acc[m][3] = 0.0;
   const double xi = x[i], yi = y[i], zi = z[i];
-  for (int32_t c = f_off[i] + sub; c < f_off[i + 1]; c += kCorrLanes) {
+  Sums::for_each(f_off[i], f_off[i + 1], sub, [&](int32_t c, int m) {
     const int32_t j = f_idx[c];
-    if (is_splash(st, j)) continue;
+    if (is_splash(st, j)) return;
     const double dx = x[j] - xi, dy = y[j] - yi, dz = z[j] - zi;
     const double w = correction_weight_of(sqrt(dx * dx + dy * dy + dz * dz), re, kernel);
-    if (w <= 0.0) continue;
-    ws += w;
-    ax += w * vx[j];
-    ay += w * vy[j];
-    az += w * vz[j];
-  }
-  ws = group_sum(ws);
-  ax = group_sum(ax);
-  ay = group_sum(ay);
-  az = group_sum(az);
+    if (w <= 0.0) return;
+    acc[m][0] += w;
+    acc[m][1] += w * vx[j];
+    acc[m][2] += w * vy[j];
+    acc[m][3] += w * vz[j];
+  });
+  Sums::reduce(acc);
   if (sub != 0) return;
+  const double ws = acc[0][0], ax = acc[0][1], ay = acc[0][2], az = acc[0][3];
   if (ws <= 0.0) {
     ox[i] = vx[i];
     oy[i] = vy[i];
@@ -391,8 +399,14 @@ void pressure_gradient(Context& ctx, const lsmps_config& cfg, const lsmps_fluid_
   a.gx = grad.x;
   a.gy = grad.y;
   a.gz = grad.z;
-  gradient_kernel<<<blocks_for(fluid.count * kCorrLanes), kBlock, 0, ctx.stream>>>(a);
-  LSMPS_LAUNCH_CHECK();
+  const size_t n = fluid.count;
+  launch_tuned(ctx, kTuneGradient, {4, 8, 16}, 8, n, fn.offsets, [&](int lanes) {
+    switch (lanes) {
+      case 4: gradient_kernel<4><<<blocks_for(n * 4), kBlock, 0, ctx.stream>>>(a); break;
+      case 8: gradient_kernel<8><<<blocks_for(n * 8), kBlock, 0, ctx.stream>>>(a); break;
+      default: gradient_kernel<16><<<blocks_for(n * 16), kBlock, 0, ctx.stream>>>(a); break;
+    }
+  });
 }
 
 void particle_shifting(Context& ctx, const lsmps_config& cfg, const lsmps_fluid_soa& fluid,
@@ -425,8 +439,14 @@ void particle_shifting(Context& ctx, const lsmps_config& cfg, const lsmps_fluid_
   a.sx = ps.x;
   a.sy = ps.y;
   a.sz = ps.z;
-  shifting_kernel<<<blocks_for(fluid.count * kCorrLanes), kBlock, 0, ctx.stream>>>(a);
-  LSMPS_LAUNCH_CHECK();
+  const size_t n = fluid.count;
+  launch_tuned(ctx, kTuneShifting, {4, 8, 16}, 8, n, fn.offsets, [&](int lanes) {
+    switch (lanes) {
+      case 4: shifting_kernel<4><<<blocks_for(n * 4), kBlock, 0, ctx.stream>>>(a); break;
+      case 8: shifting_kernel<8><<<blocks_for(n * 8), kBlock, 0, ctx.stream>>>(a); break;
+      default: shifting_kernel<16><<<blocks_for(n * 16), kBlock, 0, ctx.stream>>>(a); break;
+    }
+  });
 }
 
 void update_velocity_position(Context& ctx, const lsmps_config& cfg, const lsmps_fluid_soa& fluid,
@@ -463,11 +483,21 @@ void smooth_velocity(Context& ctx, const lsmps_config& cfg, const lsmps_fluid_so
           "Pressure correction requires a valid fluid-neighbor CSR table");
   require_vector(fluid, smoothed, "Velocity smoothing");
   if (fluid.count == 0) return;
-  smooth_kernel<<<blocks_for(fluid.count * kCorrLanes), kBlock, 0, ctx.stream>>>(
-      fluid.count, fluid.x, fluid.y, fluid.z, fluid.vx, fluid.vy, fluid.vz, fluid.surface_type,
-      fn.offsets, fn.indices, cfg.support_radius, cfg.velocity_smoothing_strength,
-      cfg.weight_kernel, smoothed.x, smoothed.y, smoothed.z);
-  LSMPS_LAUNCH_CHECK();
+  const size_t n = fluid.count;
+  auto smooth = [&](auto lanes_tag) {
+    constexpr int L = decltype(lanes_tag)::value;
+    smooth_kernel<L><<<blocks_for(n * L), kBlock, 0, ctx.stream>>>(
+        n, fluid.x, fluid.y, fluid.z, fluid.vx, fluid.vy, fluid.vz, fluid.surface_type,
+        fn.offsets, fn.indices, cfg.support_radius, cfg.velocity_smoothing_strength,
+        cfg.weight_kernel, smoothed.x, smoothed.y, smoothed.z);
+  };
+  launch_tuned(ctx, kTuneSmoothing, {4, 8, 16}, 8, n, fn.offsets, [&](int lanes) {
+    switch (lanes) {
+      case 4: smooth(std::integral_constant<int, 4>{}); break;
+      case 8: smooth(std::integral_constant<int, 8>{}); break;
+      default: smooth(std::integral_constant<int, 16>{}); break;
+    }
+  });
   copy3_kernel<<<blocks_for(fluid.count), kBlock, 0, ctx.stream>>>(
       fluid.count, smoothed.x, smoothed.y, smoothed.z, fluid.vx, fluid.vy, fluid.vz);
   LSMPS_LAUNCH_CHECK();

@@ -108,4 +108,58 @@ __device__ __forceinline__ void wall_neumann_q_jt(T nx, T ny, T nz, T inv_re, co
 constexpr int kOpRows = 36;
 constexpr int kOpLap = 27;
 
+// ---- lane-count-invariant neighbour sums ----------------------------------
+//
+// A particle's neighbour list is dealt round-robin onto kSlots virtual slots
+// (entry e of a list goes to slot e % kSlots); each slot sums its entries in
+// list order (the fluid list, then the wall list), and the slots meet in an xor
+// butterfly of strides kSlots / 2, ..., 2, 1.  A group of L adjacent lanes
+// (L divides kSlots and 64) runs one particle: lane `sub` holds the kSlots / L
+// slots sub, sub + L, ..., and takes list entries sub, sub + L, ... in turn.
+// The butterfly strides of at least L combine slots inside a lane and the
+// smaller ones go across lanes, so every L yields the same sums bit for bit;
+// L only changes speed, which is what the autotuner picks it by.
+template <int L, int kSlots>
+struct SlotSums {
+  static_assert(kSlots % L == 0 && 64 % L == 0, "lanes per particle must divide the slots");
+  static constexpr int M = kSlots / L;  // slots per lane
+
+  // body(entry, slot) for this lane's entries of [begin, end); slot in [0, M).
+  template <typename F>
+  __device__ __forceinline__ static void for_each(int32_t begin, int32_t end, int sub, F body) {
+    for (int32_t c0 = begin + sub; c0 < end; c0 += L * M) {
+#pragma unroll
+      for (int m = 0; m < M; ++m) {
+        const int32_t c = c0 + m * L;
+        if (c < end) body(c, m);
+      }
+    }
+  }
+
+  // Butterfly over the slots of v[m][k] (k < W): on return v[0][k] holds the
+  // total in every lane of the group.
+  template <int W>
+  __device__ __forceinline__ static void reduce(double (&v)[M][W]) {
+#pragma unroll
+    for (int q = M / 2; q > 0; q >>= 1) {  // stride q * L: inside the lane
+#pragma unroll
+      for (int m = 0; m < M; ++m) {
+        if ((m & q) == 0) {
+#pragma unroll
+          for (int k = 0; k < W; ++k) {
+            const double t = v[m][k] + v[m | q][k];
+            v[m][k] = t;
+            v[m | q][k] = t;
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int off = L / 2; off > 0; off >>= 1) {  // strides below L: across lanes
+#pragma unroll
+      for (int k = 0; k < W; ++k) v[0][k] += __shfl_xor(v[0][k], off, 64);
+    }
+  }
+};
+
 }  // namespace lsmps

@@ -15,8 +15,11 @@
 //      fluid cell order so a wavefront's 64 rows share the same neighbor cells.
 // Compiled with -ffp-contract=off: the distance test must round exactly like
 // the oracle's.
+#include <type_traits>
+
 #include "common.hpp"
 #include "stages.hpp"
+#include "tune.hpp"
 
 namespace lsmps {
 namespace {
@@ -248,14 +251,10 @@ __global__ void gather_points_kernel(size_t n, const int32_t* __restrict__ sorte
 // count_neighbors_kernel / write_neighbors_kernel (neighbor_search.cu:221-253) with
 // count/write_source_neighbors_for_fluid (:100-219) inlined.  Row t of the launch
 // handles fluid particle order[t] (cell order) so neighboring rows share cells.
-// Lanes per particle (a power of two dividing 64): the fill pass uses 8; the
-// count pass, which writes nothing per hit, is fastest with one.
-template <bool kWrite>
-constexpr int scan_lanes() {
-  return kWrite ? 8 : 1;
-}
-
-template <bool kWrite, bool kSeg>
+// kScanLanes lanes per particle (a power of two dividing 64, picked by the
+// autotuner; by default 8 for the fill pass and 1 for the count pass, which
+// writes nothing per hit).  The table is the same for every choice.
+template <bool kWrite, bool kSeg, int kScanLanes>
 __global__ void __launch_bounds__(kBlock) neighbor_scan_kernel(
     size_t n_fluid, const int32_t* __restrict__ order, const double* __restrict__ fx,
     const double* __restrict__ fy, const double* __restrict__ fz, GridDev g0,
@@ -263,7 +262,6 @@ __global__ void __launch_bounds__(kBlock) neighbor_scan_kernel(
     const int32_t* __restrict__ src_begin, const int32_t* __restrict__ src_end,
     const SortedPoint* __restrict__ src, double radius_squared, int exclude_self,
     int32_t* __restrict__ offsets, int32_t* __restrict__ indices) {
-  constexpr int kScanLanes = scan_lanes<kWrite>();
   constexpr uint32_t kScanMask = (1u << kScanLanes) - 1u;
   const size_t gt = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   const size_t t = gt / kScanLanes;
@@ -522,18 +520,32 @@ void one_neighbor_list(Context& ctx, const GeometryScratch& sc, const lsmps_flui
     LSMPS_LAUNCH_CHECK();
   }
   const int32_t* order = fluid_cells.sorted_indices;
-  if (grids) {
-    neighbor_scan_kernel<false, true><<<blocks_for(fluid.count * scan_lanes<false>()), kBlock, 0, ctx.stream>>>(
-        fluid.count, order, fluid.x, fluid.y, fluid.z, g, grids, fluid_seg,
-        source_cells.cell_begin, source_cells.cell_end, points, radius_squared,
-        exclude_self ? 1 : 0, nbrs.offsets, nullptr);
-  } else {
-    neighbor_scan_kernel<false, false><<<blocks_for(fluid.count * scan_lanes<false>()), kBlock, 0, ctx.stream>>>(
-        fluid.count, order, fluid.x, fluid.y, fluid.z, g, nullptr, nullptr,
-        source_cells.cell_begin, source_cells.cell_end, points, radius_squared,
-        exclude_self ? 1 : 0, nbrs.offsets, nullptr);
-  }
-  LSMPS_LAUNCH_CHECK();
+  const size_t nf = fluid.count;
+  const int self = exclude_self ? 1 : 0;
+  // one pass of the scan with L lanes per particle, over one grid or the segments' grids
+  auto scan = [&](auto write_tag, auto lanes_tag) {
+    constexpr bool W = decltype(write_tag)::value;
+    constexpr int L = decltype(lanes_tag)::value;
+    int32_t* idx = W ? nbrs.indices : nullptr;
+    if (grids) {
+      neighbor_scan_kernel<W, true, L><<<blocks_for(nf * L), kBlock, 0, ctx.stream>>>(
+          nf, order, fluid.x, fluid.y, fluid.z, g, grids, fluid_seg, source_cells.cell_begin,
+          source_cells.cell_end, points, radius_squared, self, nbrs.offsets, idx);
+    } else {
+      neighbor_scan_kernel<W, false, L><<<blocks_for(nf * L), kBlock, 0, ctx.stream>>>(
+          nf, order, fluid.x, fluid.y, fluid.z, g, nullptr, nullptr, source_cells.cell_begin,
+          source_cells.cell_end, points, radius_squared, self, nbrs.offsets, idx);
+    }
+  };
+  using Count = std::false_type;
+  using Write = std::true_type;
+  launch_tuned(ctx, kTuneScanCount, {1, 2, 4}, 1, nf, nullptr, [&](int lanes) {
+    switch (lanes) {
+      case 1: scan(Count{}, std::integral_constant<int, 1>{}); break;
+      case 2: scan(Count{}, std::integral_constant<int, 2>{}); break;
+      default: scan(Count{}, std::integral_constant<int, 4>{}); break;
+    }
+  });
   exclusive_scan_i32_with(ctx, nbrs.offsets, nbrs.offsets, fluid.count, sc.partials,
                           sc.partials_capacity);
   // Capacity check (require_neighbor_capacity, neighbor_search.cu:22-30,323-328).
@@ -547,17 +559,13 @@ void one_neighbor_list(Context& ctx, const GeometryScratch& sc, const lsmps_flui
                                   std::to_string(required) + ", capacity " +
                                   std::to_string(nbrs.neighbor_count));
   require(required == 0 || nbrs.indices != nullptr, "Neighbor list requires an indices buffer");
-  if (grids) {
-    neighbor_scan_kernel<true, true><<<blocks_for(fluid.count * scan_lanes<true>()), kBlock, 0, ctx.stream>>>(
-        fluid.count, order, fluid.x, fluid.y, fluid.z, g, grids, fluid_seg,
-        source_cells.cell_begin, source_cells.cell_end, points, radius_squared,
-        exclude_self ? 1 : 0, nbrs.offsets, nbrs.indices);
-  } else {
-    neighbor_scan_kernel<true, false><<<blocks_for(fluid.count * scan_lanes<true>()), kBlock, 0, ctx.stream>>>(
-        fluid.count, order, fluid.x, fluid.y, fluid.z, g, nullptr, nullptr,
-        source_cells.cell_begin, source_cells.cell_end, points, radius_squared,
-        exclude_self ? 1 : 0, nbrs.offsets, nbrs.indices);
-  }
+  launch_tuned(ctx, kTuneScanWrite, {4, 8, 16}, 8, nf, nbrs.offsets, [&](int lanes) {
+    switch (lanes) {
+      case 4: scan(Write{}, std::integral_constant<int, 4>{}); break;
+      case 8: scan(Write{}, std::integral_constant<int, 8>{}); break;
+      default: scan(Write{}, std::integral_constant<int, 16>{}); break;
+    }
+  });
   LSMPS_LAUNCH_CHECK();
 }
 }  // namespace

@@ -11,6 +11,7 @@
 #include "common.hpp"
 #include "device_math.hpp"
 #include "stages.hpp"
+#include "tune.hpp"
 
 namespace lsmps {
 namespace {
@@ -59,28 +60,28 @@ __global__ void row_offsets_kernel(size_t n, const int32_t* f_off, int32_t* row_
   row_offsets[i] = static_cast<int32_t>(i) + f_off[i];
 }
 
-// kAsmLanes adjacent lanes per row: lane l of a row's group takes neighbours
-// l, l + kAsmLanes, ..., so the index loads and the CSR writes of a wavefront
-// are contiguous runs and the neighbour gathers of a row are in flight
-// together; the per-row sums meet in a fixed xor butterfly.
-constexpr int kAsmLanes = 8;
+// L adjacent lanes per row take its neighbours round-robin, so the index loads
+// and the CSR writes of a wavefront are contiguous runs and the neighbour
+// gathers of a row are in flight together; the per-row sums follow the 8-slot
+// tree of SlotSums (the same result for L = 4 and 8).
+constexpr int kAsmSlots = 8;
 
-__device__ __forceinline__ double group_sum(double v) {
-#pragma unroll
-  for (int off = kAsmLanes / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
-}
+// per-slot sums: velocity rhs x (0..8), y (9..17), z (18..26), diagonal (27),
+// wall Neumann term (28)
+constexpr int kAsmSums = 29;
 
+template <int L>
 __global__ void __launch_bounds__(kBlock) assemble_kernel(PpeArgs a) {
+  using Sums = SlotSums<L, kAsmSlots>;
   const size_t t = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  const size_t i = t / kAsmLanes;
-  const int sub = static_cast<int>(t % kAsmLanes);
+  const size_t i = t / L;
+  const int sub = static_cast<int>(t % L);
   if (i >= a.n) return;  // whole groups
   const int32_t row_begin = static_cast<int32_t>(i) + a.f_off[i];
   const int32_t nb = a.f_off[i], ne = a.f_off[i + 1];
   const int32_t st = a.surface_type ? a.surface_type[i] : LSMPS_SURFACE_INNER;
   if (st == LSMPS_SURFACE_SURFACE || st == LSMPS_SURFACE_SPLASH) {
-    for (int32_t c = nb + sub; c < ne; c += kAsmLanes) {
+    for (int32_t c = nb + sub; c < ne; c += L) {
       const int32_t out = row_begin + 1 + c - nb;
       a.cols[out] = a.f_idx[c];
       a.vals[out] = 0.0;
@@ -94,90 +95,81 @@ __global__ void __launch_bounds__(kBlock) assemble_kernel(PpeArgs a) {
     }
     return;
   }
-  double L[9];
+  double Lap[9];
 #pragma unroll
-  for (int k = 0; k < 9; ++k) L[k] = a.pre_ops[static_cast<size_t>(kOpLap + k) * a.n + i];
+  for (int k = 0; k < 9; ++k) Lap[k] = a.pre_ops[static_cast<size_t>(kOpLap + k) * a.n + i];
   const double xi = a.x[i], yi = a.y[i], zi = a.z[i];
   const double ui = a.tvx[i], vi = a.tvy[i], wi = a.tvz[i];
   const double inv_re = 1.0 / a.re;
   const double coef = 2.0 / (a.re * a.re * a.density);
-  double rx[9], ry[9], rz[9];
+  double acc[Sums::M][kAsmSums];
 #pragma unroll
-  for (int k = 0; k < 9; ++k) {
-    rx[k] = 0.0;
-    ry[k] = 0.0;
-    rz[k] = 0.0;
-  }
-  double diagonal = 0.0;
-  double b[9];
-  for (int32_t c = nb + sub; c < ne; c += kAsmLanes) {
+  for (int m = 0; m < Sums::M; ++m)
+#pragma unroll
+    for (int k = 0; k < kAsmSums; ++k) acc[m][k] = 0.0;
+  Sums::for_each(nb, ne, sub, [&](int32_t c, int m) {
     const int32_t j = a.f_idx[c];
     const double dx = a.x[j] - xi, dy = a.y[j] - yi, dz = a.z[j] - zi;
     const double w = weight_of(sqrt(dx * dx + dy * dy + dz * dz), a.re, a.kernel);
     double off = 0.0;
     if (w > 0.0) {
+      double b[9];
       type_a_basis(dx, dy, dz, inv_re, b);
       double lb = 0.0;
 #pragma unroll
-      for (int k = 0; k < 9; ++k) lb += L[k] * b[k];
+      for (int k = 0; k < 9; ++k) lb += Lap[k] * b[k];
       off = coef * w * lb;
-      diagonal -= off;
+      acc[m][27] -= off;
       const double du = a.tvx[j] - ui, dv = a.tvy[j] - vi, dw = a.tvz[j] - wi;
 #pragma unroll
       for (int k = 0; k < 9; ++k) {
         const double wb = w * b[k];
-        rx[k] += wb * du;
-        ry[k] += wb * dv;
-        rz[k] += wb * dw;
+        acc[m][k] += wb * du;
+        acc[m][9 + k] += wb * dv;
+        acc[m][18 + k] += wb * dw;
       }
     }
     const int32_t out = row_begin + 1 + c - nb;
     a.cols[out] = j;
     a.vals[out] = off;
-  }
-  double wall_rhs = 0.0;
-  double q[9];
-  for (int32_t c = a.w_off[i] + sub; c < a.w_off[i + 1]; c += kAsmLanes) {
+  });
+  Sums::for_each(a.w_off[i], a.w_off[i + 1], sub, [&](int32_t c, int m) {
     const int32_t j = a.w_idx[c];
     const double dx = a.wx[j] - xi, dy = a.wy[j] - yi, dz = a.wz[j] - zi;
     const double bw = weight_of(sqrt(dx * dx + dy * dy + dz * dz), a.re, a.kernel);
-    if (bw <= 0.0) continue;
+    if (bw <= 0.0) return;
+    double b[9];
     type_a_basis(dx, dy, dz, inv_re, b);
     const double w = a.wall_scale * bw;
     const double du = a.twx[j] - ui, dv = a.twy[j] - vi, dw = a.twz[j] - wi;
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
       const double wb = w * b[k];
-      rx[k] += wb * du;
-      ry[k] += wb * dv;
-      rz[k] += wb * dw;
+      acc[m][k] += wb * du;
+      acc[m][9 + k] += wb * dv;
+      acc[m][18 + k] += wb * dw;
     }
     const double nx = a.wnx[j], ny = a.wny[j], nz = a.wnz[j];
     const double ng = a.g[0] * nx + a.g[1] * ny + a.g[2] * nz;
+    double q[9];
     wall_neumann_q(dx, dy, dz, nx, ny, nz, inv_re, q);
     double lq = 0.0;
 #pragma unroll
-    for (int k = 0; k < 9; ++k) lq += L[k] * q[k];
-    wall_rhs += 2.0 * a.wall_scale * bw * ng * lq / a.re;
-  }
-#pragma unroll
-  for (int k = 0; k < 9; ++k) {
-    rx[k] = group_sum(rx[k]);
-    ry[k] = group_sum(ry[k]);
-    rz[k] = group_sum(rz[k]);
-  }
-  diagonal = group_sum(diagonal);
-  wall_rhs = group_sum(wall_rhs);
+    for (int k = 0; k < 9; ++k) lq += Lap[k] * q[k];
+    acc[m][28] += 2.0 * a.wall_scale * bw * ng * lq / a.re;
+  });
+  Sums::reduce(acc);
   if (sub != 0) return;
   // divergence from velocity rows 0..2 (first_coefficient_from_inverse_rhs, ppe_matrix.cu:233-241)
   double d0 = 0.0, d1 = 0.0, d2 = 0.0;
 #pragma unroll
   for (int k = 0; k < 9; ++k) {
-    d0 += a.vel_ops[static_cast<size_t>(k) * a.n + i] * rx[k];
-    d1 += a.vel_ops[static_cast<size_t>(9 + k) * a.n + i] * ry[k];
-    d2 += a.vel_ops[static_cast<size_t>(18 + k) * a.n + i] * rz[k];
+    d0 += a.vel_ops[static_cast<size_t>(k) * a.n + i] * acc[0][k];
+    d1 += a.vel_ops[static_cast<size_t>(9 + k) * a.n + i] * acc[0][9 + k];
+    d2 += a.vel_ops[static_cast<size_t>(18 + k) * a.n + i] * acc[0][18 + k];
   }
   const double divergence = (d0 + d1 + d2) / a.re;
+  const double diagonal = acc[0][27], wall_rhs = acc[0][28];
   a.cols[row_begin] = static_cast<int32_t>(i);
   a.vals[row_begin] = diagonal;
   a.divergence[i] = divergence;
@@ -245,7 +237,14 @@ void ppe_assemble(Context& ctx, const lsmps_config& cfg, const lsmps_fluid_soa& 
   a.rhs = ws.rhs;
   a.divergence = ws.divergence;
   a.wall_term = ws.pressure_laplacian;
-  assemble_kernel<<<blocks_for(fluid.count * kAsmLanes), kBlock, 0, ctx.stream>>>(a);
+  const size_t n = fluid.count;
+  launch_tuned(ctx, kTuneAssembly, {4, 8}, 8, n, fn.offsets, [&](int lanes) {
+    if (lanes == 4) {
+      assemble_kernel<4><<<blocks_for(n * 4), kBlock, 0, ctx.stream>>>(a);
+    } else {
+      assemble_kernel<8><<<blocks_for(n * 8), kBlock, 0, ctx.stream>>>(a);
+    }
+  });
   LSMPS_LAUNCH_CHECK();
 }
 

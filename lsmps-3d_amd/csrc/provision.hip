@@ -10,6 +10,7 @@
 #include "common.hpp"
 #include "device_math.hpp"
 #include "stages.hpp"
+#include "tune.hpp"
 
 namespace lsmps {
 namespace {
@@ -48,21 +49,17 @@ struct ProvisionArgs {
   double* lap_z;
 };
 
-// kProvLanes adjacent lanes per particle: lane l takes neighbours l,
-// l + kProvLanes, ... (contiguous index loads per wavefront, a particle's
-// gathers in flight together); the sums meet in a fixed xor butterfly.
-constexpr int kProvLanes = 8;
+// L adjacent lanes per particle take its neighbours round-robin (contiguous
+// index loads per wavefront, a particle's gathers in flight together); the sums
+// follow the 16-slot tree of SlotSums, so every L gives the same result.
+constexpr int kProvSlots = 16;
 
-__device__ __forceinline__ double group_sum(double v) {
-#pragma unroll
-  for (int off = kProvLanes / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
-}
-
+template <int L>
 __global__ void __launch_bounds__(kBlock) provision_kernel(ProvisionArgs a) {
+  using Sums = SlotSums<L, kProvSlots>;
   const size_t t = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  const size_t i = t / kProvLanes;
-  const int sub = static_cast<int>(t % kProvLanes);
+  const size_t i = t / L;
+  const int sub = static_cast<int>(t % L);
   if (i >= a.n) return;  // whole groups
   double lap_row[9];
 #pragma unroll
@@ -72,40 +69,42 @@ __global__ void __launch_bounds__(kBlock) provision_kernel(ProvisionArgs a) {
   const double inv_re = 1.0 / a.re;
   // accumulate_velocity_rhs (explicit_update.cu:148-162) contracted with the
   // Laplacian row on the fly: lap = 2 (L . rhs) / re^2 and L . rhs = sum_j w (L . b_j) dv_j.
-  double sx = 0.0, sy = 0.0, sz = 0.0;
-  double b[9];
-  for (int32_t c = a.f_off[i] + sub; c < a.f_off[i + 1]; c += kProvLanes) {
+  double acc[Sums::M][3];
+#pragma unroll
+  for (int m = 0; m < Sums::M; ++m) acc[m][0] = acc[m][1] = acc[m][2] = 0.0;
+  Sums::for_each(a.f_off[i], a.f_off[i + 1], sub, [&](int32_t c, int m) {
     const int32_t j = a.f_idx[c];
     const double dx = a.x[j] - xi, dy = a.y[j] - yi, dz = a.z[j] - zi;
     const double w = weight_of(sqrt(dx * dx + dy * dy + dz * dz), a.re, a.kernel);
-    if (w <= 0.0) continue;
+    if (w <= 0.0) return;
+    double b[9];
     type_a_basis(dx, dy, dz, inv_re, b);
     double lb = 0.0;
 #pragma unroll
     for (int k = 0; k < 9; ++k) lb += lap_row[k] * b[k];
     const double wl = w * lb;
-    sx += wl * (a.vx[j] - vxi);
-    sy += wl * (a.vy[j] - vyi);
-    sz += wl * (a.vz[j] - vzi);
-  }
-  for (int32_t c = a.w_off[i] + sub; c < a.w_off[i + 1]; c += kProvLanes) {
+    acc[m][0] += wl * (a.vx[j] - vxi);
+    acc[m][1] += wl * (a.vy[j] - vyi);
+    acc[m][2] += wl * (a.vz[j] - vzi);
+  });
+  Sums::for_each(a.w_off[i], a.w_off[i + 1], sub, [&](int32_t c, int m) {
     const int32_t j = a.w_idx[c];
     const double dx = a.wx[j] - xi, dy = a.wy[j] - yi, dz = a.wz[j] - zi;
     const double w = a.wall_scale * weight_of(sqrt(dx * dx + dy * dy + dz * dz), a.re, a.kernel);
-    if (w <= 0.0) continue;
+    if (w <= 0.0) return;
+    double b[9];
     type_a_basis(dx, dy, dz, inv_re, b);
     double lb = 0.0;
 #pragma unroll
     for (int k = 0; k < 9; ++k) lb += lap_row[k] * b[k];
     const double wl = w * lb;
-    sx += wl * (a.wvx[j] - vxi);
-    sy += wl * (a.wvy[j] - vyi);
-    sz += wl * (a.wvz[j] - vzi);
-  }
-  sx = group_sum(sx);
-  sy = group_sum(sy);
-  sz = group_sum(sz);
+    acc[m][0] += wl * (a.wvx[j] - vxi);
+    acc[m][1] += wl * (a.wvy[j] - vyi);
+    acc[m][2] += wl * (a.wvz[j] - vzi);
+  });
+  Sums::reduce(acc);
   if (sub != 0) return;
+  const double sx = acc[0][0], sy = acc[0][1], sz = acc[0][2];
   // laplacian_from_inverse_rhs (explicit_update.cu:164-176)
   const double scale = 2.0 / (a.re * a.re);
   const double lx = scale * sx, ly = scale * sy, lz = scale * sz;
@@ -190,8 +189,14 @@ void provision_fluid(Context& ctx, const lsmps_config& cfg, const lsmps_fluid_so
   a.lap_x = lap ? lap->x : nullptr;
   a.lap_y = lap ? lap->y : nullptr;
   a.lap_z = lap ? lap->z : nullptr;
-  provision_kernel<<<blocks_for(fluid.count * kProvLanes), kBlock, 0, ctx.stream>>>(a);
-  LSMPS_LAUNCH_CHECK();
+  const size_t n = fluid.count;
+  launch_tuned(ctx, kTuneProvision, {4, 8, 16}, 8, n, fn.offsets, [&](int lanes) {
+    switch (lanes) {
+      case 4: provision_kernel<4><<<blocks_for(n * 4), kBlock, 0, ctx.stream>>>(a); break;
+      case 8: provision_kernel<8><<<blocks_for(n * 8), kBlock, 0, ctx.stream>>>(a); break;
+      default: provision_kernel<16><<<blocks_for(n * 16), kBlock, 0, ctx.stream>>>(a); break;
+    }
+  });
 }
 
 // compute_provision_temporary_velocity (explicit_update.cu:401-415)

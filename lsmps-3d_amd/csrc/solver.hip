@@ -1911,7 +1911,6 @@ int idr_solve(Context& ctx, PpeSolver& S, size_t n, const SellMatrix& M, const V
   T* G = work + 3 * ld;
   T* U = work + (3 + s) * ld;
   const bool schwarz = S.schwarz.blocks > 0 && S.schwarz.n == n;
-  static_assert(std::is_same<T, double>::value, "the IDR basis is FP64");
   const bool coarse = schwarz && S.coarse.ready && S.coarse.n == n;
   const V* dia = sizeof(V) == sizeof(double) ? reinterpret_cast<const V*>(M.dia64.data())
                                              : reinterpret_cast<const V*>(M.dia32.data());
@@ -2058,7 +2057,7 @@ int idr_solve(Context& ctx, PpeSolver& S, size_t n, const SellMatrix& M, const V
         reinterpret_cast<uintptr_t>(dia), reinterpret_cast<uintptr_t>(dinv),
         reinterpret_cast<uintptr_t>(A64.row_offsets), reinterpret_cast<uintptr_t>(A64.col_indices),
         reinterpret_cast<uintptr_t>(A64.values), reinterpret_cast<uintptr_t>(b64),
-        schwarz ? 1u : 0u, coarse ? 1u : 0u, sizeof(V)};
+        schwarz ? 1u : 0u, coarse ? 1u : 0u, sizeof(V), sizeof(T)};
     append_sell_signature(sig, M);
     if (schwarz) append_schwarz_signature(sig, S.schwarz);
     if (coarse) append_coarse_signature(sig, S.coarse);

@@ -39,7 +39,7 @@ EXPORTS = [
     "lsmps_config_load", "lsmps_config_save", "lsmps_read_fluid_csv", "lsmps_read_wall_csv",
     "lsmps_vtk_path", "lsmps_write_vtk", "lsmps_compute_virtual_light_diagnostics",
     "lsmps_ppe_assemble_views", "lsmps_compute_pressure_gradient_view",
-    "lsmps_build_neighbor_lists_noalloc",
+    "lsmps_build_neighbor_lists_noalloc", "lsmps_context_lanes", "lsmps_context_force_lanes",
 ]
 
 _LIB = None
@@ -320,6 +320,21 @@ class Context:
         if self.handle:
             self.lib.lsmps_context_destroy(self.handle)
             self.handle = C.c_void_p()
+
+    def lanes(self):
+        """{kernel: lanes of its latest launch}, and the number of tuned keys."""
+        out = {}
+        keys = C.c_int64()
+        for name, k in abi.TUNED_KERNELS.items():
+            v = C.c_int32()
+            check(self.lib.lsmps_context_lanes(self.handle, k, C.byref(v), C.byref(keys)),
+                  "context_lanes")
+            out[name] = v.value
+        return out, keys.value
+
+    def force_lanes(self, kernel, lanes):
+        check(self.lib.lsmps_context_force_lanes(self.handle, abi.TUNED_KERNELS[kernel], lanes),
+              "context_force_lanes")
 
     def __del__(self):
         try:
@@ -619,9 +634,12 @@ def default_solver(precision=abi.PRECISION_FP64):
 
 
 def run_step_on_device(case, precision=abi.PRECISION_FP64, solver=None, max_neighbors=256,
-                       device="cuda:0", flags=0):
-    """One fused step of `case` on the GPU; returns every inspectable output as numpy."""
+                       device="cuda:0", flags=0, lanes=None):
+    """One fused step of `case` on the GPU; returns every inspectable output as numpy.
+    `lanes` ({tuned kernel name: lanes per particle}) pins the kernels' shapes."""
     ctx = Context(precision, flags=flags)
+    for k, v in (lanes or {}).items():
+        ctx.force_lanes(k, v)
     state = DeviceState(case, device)
     st = Stepper(ctx, state, case.config, max_neighbors, solver or abi.default_solver_options())
     if "fluid_offsets" in case.extra:  # batched tanks
@@ -638,6 +656,7 @@ def run_step_on_device(case, precision=abi.PRECISION_FP64, solver=None, max_neig
         "provision": st.temporary_velocity(), "grad": grad, "ps": ps, "report": report,
         "stage_ms": st.stage_times(),
     }
+    out["lanes"], out["tuned_keys"] = ctx.lanes()
     st.close()
     ctx.close()
     return out
