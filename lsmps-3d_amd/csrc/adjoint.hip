@@ -27,6 +27,7 @@
 #include <algorithm>
 #include <cmath>
 
+#include "coarse.hpp"
 #include "objects.hpp"
 
 namespace lsmps {
@@ -1022,6 +1023,15 @@ void adjoint_step(Context& c, lsmps_stepper& S, const lsmps_fluid_soa& fluid,
   lsmps_solver_options opt = S.solver_options;
   opt.clamp_negative = 0;
   W.solver.segments = S.solver.segments;
+  // The coarse space of the forward solve, for A^T: with the same aggregation P
+  // the Galerkin operator of A^T is P^T A^T P = Ac^T, so the transpose solve
+  // gets the two-level preconditioner of the forward one (the aggregation needs
+  // the recorded step's labels and fluid cell lists, which are still current).
+  W.solver.coarse.count = 0;
+  if (S.segment_count == 0 && opt.preconditioner == LSMPS_PRECOND_SCHWARZ_COARSE && n > 0)
+    coarse_aggregate(c, W.solver.coarse, n, fluid.surface_type, S.fc_sorted.data(),
+                     S.fc_begin.data(), S.fc_end.data(), cfg.cell_dims,
+                     opt.coarse_cells > 0 ? opt.coarse_cells : 4);
   const uint32_t saved_flags = c.flags;
   c.flags |= LSMPS_FLAG_ASYNC;
   bool converged = true;
