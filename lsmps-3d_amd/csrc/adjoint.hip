@@ -104,10 +104,13 @@ __device__ __forceinline__ void load9(const double* ops, int row, size_t n, size
 }
 
 // mirrored CSR position of every fluid entry
+// One wavefront per row i, one lane per entry (i, j): each lane searches row j
+// for i, so a row's searches run side by side.
 __global__ void __launch_bounds__(kBlock) rev_kernel(AdjArgs a) {
-  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= a.n) return;
-  for (int32_t c = a.f_off[i]; c < a.f_off[i + 1]; ++c) {
+  const size_t i = (static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x) / 64;
+  const int lane = static_cast<int>(threadIdx.x & 63);
+  if (i >= a.n) return;  // whole wavefronts
+  for (int32_t c = a.f_off[i] + lane; c < a.f_off[i + 1]; c += 64) {
     const int32_t j = a.f_idx[c];
     int32_t found = -1;
     for (int32_t e = a.f_off[j]; e < a.f_off[j + 1]; ++e) {
@@ -1002,7 +1005,7 @@ void adjoint_step(Context& c, lsmps_stepper& S, const lsmps_fluid_soa& fluid,
   fill_i32(c.stream, W.flags.data(), 1, 0);
   const unsigned grid = blocks_for(n);
   const hipStream_t s = c.stream;
-  rev_kernel<<<grid, kBlock, 0, s>>>(a);
+  rev_kernel<<<blocks_for(n * 64), kBlock, 0, s>>>(a);
   forward_kernel<<<grid, kBlock, 0, s>>>(a);
   smooth_stats_kernel<<<grid, kBlock, 0, s>>>(a);
   smooth_adjoint_kernel<<<grid, kBlock, 0, s>>>(a);
@@ -1028,6 +1031,9 @@ void adjoint_step(Context& c, lsmps_stepper& S, const lsmps_fluid_soa& fluid,
   // gets the two-level preconditioner of the forward one (the aggregation needs
   // the recorded step's labels and fluid cell lists, which are still current).
   W.solver.coarse.count = 0;
+  // and its Schwarz blocks, transposed: (A^T)_b = (A_b)^T
+  W.solver.schwarz_transpose_source =
+      S.solver.schwarz.n == n && S.solver.schwarz.blocks > 0 ? &S.solver.schwarz : nullptr;
   if (S.segment_count == 0 && opt.preconditioner == LSMPS_PRECOND_SCHWARZ_COARSE && n > 0)
     coarse_aggregate(c, W.solver.coarse, n, fluid.surface_type, S.fc_sorted.data(),
                      S.fc_begin.data(), S.fc_end.data(), cfg.cell_dims,
@@ -1037,11 +1043,14 @@ void adjoint_step(Context& c, lsmps_stepper& S, const lsmps_fluid_soa& fluid,
   bool converged = true;
   try {
     ppe_solve(c, W.solver, at, a.psolve, lam, opt, &out.transpose_report);
+    W.solver.schwarz_transpose_source = nullptr;
   } catch (const Error& e) {
+    W.solver.schwarz_transpose_source = nullptr;
     c.flags = saved_flags;
     if (e.code != LSMPS_ERR_NOT_CONVERGED) throw;
     converged = false;
   } catch (...) {
+    W.solver.schwarz_transpose_source = nullptr;
     c.flags = saved_flags;
     throw;
   }

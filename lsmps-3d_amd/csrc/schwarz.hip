@@ -574,6 +574,58 @@ void schwarz_setup(Context& ctx, SchwarzPrec& P, const lsmps_csr& A, const doubl
   LSMPS_LAUNCH_CHECK();
 }
 
+namespace {
+// dst block b = transpose of src block b (column-major kMaxBlock^2 each).
+template <typename V>
+__global__ void __launch_bounds__(kBlock) transpose_blocks_kernel(size_t blocks, const V* src,
+                                                                  V* dst) {
+  constexpr size_t nn = static_cast<size_t>(kMaxBlock) * kMaxBlock;
+  const size_t total = blocks * nn;
+  for (size_t e = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < total;
+       e += static_cast<size_t>(gridDim.x) * blockDim.x) {
+    const size_t b = e / nn, k = e % nn;
+    const size_t r = k % kMaxBlock, c = k / kMaxBlock;
+    dst[b * nn + c + r * kMaxBlock] = src[e];
+  }
+}
+
+template <typename T>
+void copy_array(Context& ctx, DeviceArray<T>& dst, const DeviceArray<T>& src, size_t count) {
+  dst.reserve(std::max<size_t>(count, 1));
+  if (count)
+    LSMPS_HIP(hipMemcpyAsync(dst.data(), src.data(), count * sizeof(T), hipMemcpyDeviceToDevice,
+                             ctx.stream));
+}
+}  // namespace
+
+void schwarz_transpose_of(Context& ctx, SchwarzPrec& P, const SchwarzPrec& src) {
+  const size_t n = src.n, nb = src.blocks;
+  P.n = n;
+  P.blocks = nb;
+  P.fp32 = src.fp32;
+  P.covered_count = src.covered_count;
+  if (nb == 0) return;
+  const size_t dense = nb * kMaxBlock * kMaxBlock;
+  copy_array(ctx, P.members, src.members, nb * kMaxBlock);
+  copy_array(ctx, P.sizes, src.sizes, nb);
+  copy_array(ctx, P.cover_start, src.cover_start, n + 1);
+  copy_array(ctx, P.cover_entries, src.cover_entries, nb * kMaxBlock);
+  copy_array(ctx, P.covered, src.covered, std::max<size_t>(src.covered_count, 1));
+  copy_array(ctx, P.pdiag, src.pdiag, n);
+  P.ybuf.reserve(nb * kApplyParts * kMaxBlock);
+  const unsigned grid = static_cast<unsigned>(std::min<size_t>(blocks_for(dense), 8192));
+  if (src.fp32) {
+    P.inv32.reserve(dense);
+    transpose_blocks_kernel<float><<<grid, kBlock, 0, ctx.stream>>>(nb, src.inv32.data(),
+                                                                     P.inv32.data());
+  } else {
+    P.inv64.reserve(dense);
+    transpose_blocks_kernel<double><<<grid, kBlock, 0, ctx.stream>>>(nb, src.inv64.data(),
+                                                                      P.inv64.data());
+  }
+  LSMPS_LAUNCH_CHECK();
+}
+
 template <typename T>
 void schwarz_products(Context& ctx, const SchwarzPrec& P, const T* v, const int32_t* skip) {
   if (P.blocks == 0) return;

@@ -54,6 +54,11 @@ __device__ __forceinline__ double schwarz_row_mean(const int32_t* start, const i
 // Jacobi diagonal 1 / A_ii and the sum of the off-diagonal magnitudes.
 void schwarz_setup(Context& ctx, SchwarzPrec& P, const lsmps_csr& A, const double* diag,
                    const double* dinv, const double* offabs, bool fp32_inverse);
+// The blocks of A's preconditioner for A^T: the same member sets, whose dense
+// blocks of A^T are the transposes of A's, so their inverses are the
+// transposed inverses of src (restricted to the same covered rows; the
+// diagonal part 1 / A_ii is shared).
+void schwarz_transpose_of(Context& ctx, SchwarzPrec& P, const SchwarzPrec& src);
 // In-place completion of a preconditioned vector: on entry v = pdiag * w (w on
 // covered rows, Jacobi values elsewhere); on exit each covered row holds the mean
 // of its blocks' values (inv_b w_b), summed in block order.

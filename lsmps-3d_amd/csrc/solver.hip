@@ -2411,8 +2411,12 @@ void ppe_solve(Context& ctx, PpeSolver& S, const lsmps_csr& A, const double* b, 
     S.coarse.ready = false;
     if (opt.preconditioner == LSMPS_PRECOND_SCHWARZ ||
         opt.preconditioner == LSMPS_PRECOND_SCHWARZ_COARSE) {
-      schwarz_setup(ctx, S.schwarz, A, S.diag64.data(), S.dinv64.data(), S.offabs.data(),
-                    ctx.precision == LSMPS_PRECISION_MIXED);
+      if (S.schwarz_transpose_source && S.schwarz_transpose_source->n == n) {
+        schwarz_transpose_of(ctx, S.schwarz, *S.schwarz_transpose_source);
+      } else {
+        schwarz_setup(ctx, S.schwarz, A, S.diag64.data(), S.dinv64.data(), S.offabs.data(),
+                      ctx.precision == LSMPS_PRECISION_MIXED);
+      }
       int64_t blocks = 0, rows = 0;
       schwarz_stats(ctx, S.schwarz, &blocks, &rows);
       rep.schwarz_blocks = static_cast<int32_t>(blocks);

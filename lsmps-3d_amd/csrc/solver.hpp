@@ -149,6 +149,9 @@ class PpeSolver {
   size_t seg_host_count = 0;
   SellMatrix sell;
   SchwarzPrec schwarz;
+  // set by a caller solving A^T after a solve with A: the blocks are taken
+  // transposed from that solve's preconditioner instead of built from A^T
+  const SchwarzPrec* schwarz_transpose_source = nullptr;
   CoarseSpace coarse;          // aggregation set by the caller (count 0: none)
   DeviceArray<double> dinv64;  // Jacobi: 1 / diagonal
   DeviceArray<double> diag64;  // row diagonal
