@@ -1034,10 +1034,16 @@ void adjoint_step(Context& c, lsmps_stepper& S, const lsmps_fluid_soa& fluid,
   // and its Schwarz blocks, transposed: (A^T)_b = (A_b)^T
   W.solver.schwarz_transpose_source =
       S.solver.schwarz.n == n && S.solver.schwarz.blocks > 0 ? &S.solver.schwarz : nullptr;
-  if (S.segment_count == 0 && opt.preconditioner == LSMPS_PRECOND_SCHWARZ_COARSE && n > 0)
-    coarse_aggregate(c, W.solver.coarse, n, fluid.surface_type, S.fc_sorted.data(),
-                     S.fc_begin.data(), S.fc_end.data(), cfg.cell_dims,
-                     opt.coarse_cells > 0 ? opt.coarse_cells : 4);
+  if (opt.preconditioner == LSMPS_PRECOND_SCHWARZ_COARSE && n > 0) {
+    const int f = opt.coarse_cells > 0 ? opt.coarse_cells : 4;
+    if (S.segment_count > 0) {
+      coarse_aggregate_tanks(c, W.solver.coarse, n, fluid.surface_type, S.fc_sorted.data(),
+                             S.fc_begin.data(), S.fc_end.data(), S.coarse_grids, f);
+    } else {
+      coarse_aggregate(c, W.solver.coarse, n, fluid.surface_type, S.fc_sorted.data(),
+                       S.fc_begin.data(), S.fc_end.data(), cfg.cell_dims, f);
+    }
+  }
   const uint32_t saved_flags = c.flags;
   c.flags |= LSMPS_FLAG_ASYNC;
   bool converged = true;

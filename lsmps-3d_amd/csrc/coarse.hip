@@ -118,6 +118,94 @@ __global__ void fill_i32(int32_t* p, size_t n, int32_t v) {
   if (i < n) p[i] = v;
 }
 
+// ---- batched tanks: aggregation over each tank's own grid -----------------
+
+// Largest k in [0, tanks) with start[k] <= v (start ascending).
+__device__ __forceinline__ int tank_of(const int32_t* start, int tanks, int64_t v) {
+  int lo = 0, hi = tanks - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (start[mid] <= v) {
+      lo = mid;
+    } else {
+      hi = mid - 1;
+    }
+  }
+  return lo;
+}
+
+__device__ __forceinline__ CoarseGrid grid_of(const TankCoarseGrid& t) {
+  return CoarseGrid{t.nx, t.ny, t.nz, t.cx, t.cy, t.cz, t.f};
+}
+
+// count_members_kernel over the coarse cells of every tank (coarse cell cc
+// belongs to the tank whose range of tank_cc holds it; its fine cells are
+// that tank's, offset by the tank's first cell).
+__global__ void __launch_bounds__(kBlock) count_tank_members_kernel(
+    const TankCoarseGrid* grids, const int32_t* tank_cc, int tanks, size_t ccells,
+    const int32_t* labels, const int32_t* sorted, const int32_t* begin, const int32_t* end,
+    int32_t* count, int32_t* flag) {
+  const size_t cc = static_cast<size_t>(blockIdx.x) * kWaves + (threadIdx.x >> 6);
+  if (cc >= ccells) return;  // uniform per wavefront
+  const int lane = threadIdx.x & 63;
+  const TankCoarseGrid t = grids[tank_of(tank_cc, tanks, static_cast<int64_t>(cc))];
+  int c = 0;
+  for_each_candidate(grid_of(t), cc - static_cast<size_t>(t.cc_base), sorted, begin + t.base,
+                     end + t.base, lane, [&](int32_t row) {
+                       if (row >= 0 && aggregated_label(labels[row])) ++c;
+                     });
+  c = wave_sum_i(c);
+  if (lane == 0) {
+    count[cc] = c;
+    flag[cc] = c > 0 ? 1 : 0;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) fill_tank_members_kernel(
+    const TankCoarseGrid* grids, const int32_t* tank_cc, int tanks, size_t ccells,
+    const int32_t* labels, const int32_t* sorted, const int32_t* begin, const int32_t* end,
+    const int32_t* count, const int32_t* start, const int32_t* id, int32_t* agg_start,
+    int32_t* agg_rows, int32_t* agg_of_row) {
+  const size_t cc = static_cast<size_t>(blockIdx.x) * kWaves + (threadIdx.x >> 6);
+  if (cc >= ccells) return;
+  if (count[cc] == 0) return;  // uniform per wavefront
+  const int lane = threadIdx.x & 63;
+  const TankCoarseGrid t = grids[tank_of(tank_cc, tanks, static_cast<int64_t>(cc))];
+  const int32_t a = id[cc];
+  int32_t pos = start[cc];
+  if (lane == 0) agg_start[a] = pos;
+  for_each_candidate(grid_of(t), cc - static_cast<size_t>(t.cc_base), sorted, begin + t.base,
+                     end + t.base, lane, [&](int32_t row) {
+                       const bool take = row >= 0 && aggregated_label(labels[row]);
+                       const unsigned long long mask = __ballot(take);
+                       if (take) {
+                         const int before = __popcll(mask & ((1ull << lane) - 1ull));
+                         agg_rows[pos + before] = row;
+                         agg_of_row[row] = a;
+                       }
+                       pos += __popcll(mask);
+                     });
+}
+
+// tank_agg[k] = first aggregate of tank k (k < tanks), tank_agg[tanks] = the
+// aggregate count and tank_agg[tanks + 1] = the member count.
+__global__ void tank_ranges_kernel(const int32_t* tank_cc, int tanks, size_t ccells,
+                                   const int32_t* cell_count, const int32_t* cell_flag,
+                                   const int32_t* cell_start, const int32_t* cell_id,
+                                   int32_t* tank_agg) {
+  const int k = static_cast<int>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k < tanks) tank_agg[k] = cell_id[tank_cc[k]];
+  if (k == tanks) tank_agg[k] = cell_id[ccells - 1] + cell_flag[ccells - 1];
+  if (k == tanks + 1) tank_agg[k] = cell_start[ccells - 1] + cell_count[ccells - 1];
+}
+
+// agg_tank[I] = k for the aggregates of tank k (one block per tank).
+__global__ void __launch_bounds__(kBlock) agg_tank_kernel(const int32_t* tank_agg,
+                                                          int32_t* agg_tank) {
+  const int k = blockIdx.x;
+  for (int32_t I = tank_agg[k] + threadIdx.x; I < tank_agg[k + 1]; I += kBlock) agg_tank[I] = k;
+}
+
 // Ac = P^T A P in exact 64-bit fixed point.  Row I's scale 2^s_I is set from a
 // bound B_I >= |Ac_IJ| (the sum over its members of sum_j |a_ij|) so that
 // 2^s_I B_I < 2^61: every a_ij enters as round(a_ij 2^s_I) and all additions
@@ -185,6 +273,62 @@ __global__ void __launch_bounds__(kBlock) coarse_convert_kernel(int count, const
 __global__ void __launch_bounds__(kBlock) fill_u64(unsigned long long* p, size_t n) {
   const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0ull;
+}
+
+// coarse_matrix_kernel for batched tanks: row I of its tank's block (columns
+// local to the tank; the tanks' rows never couple, and an entry outside the
+// tank's range is not added).
+__global__ void __launch_bounds__(kBlock) tank_matrix_kernel(
+    const int32_t* agg_tank, const int32_t* tank_agg, const int64_t* block_offset,
+    const int32_t* agg_start, const int32_t* agg_rows, const int32_t* agg_of_row,
+    const int32_t* rows, const int32_t* cols, const double* vals, const double* bound,
+    unsigned long long* out) {
+  __shared__ unsigned long long acc[kMaxTankCoarse];
+  constexpr int kGroups = kBlock / kMatrixLanes;
+  const int I = blockIdx.x / kMatrixParts;
+  const int part = blockIdx.x % kMatrixParts;
+  const int group = threadIdx.x / kMatrixLanes;
+  const int l = threadIdx.x % kMatrixLanes;
+  const int tank = agg_tank[I];
+  const int32_t lo = tank_agg[tank];
+  const int32_t c = tank_agg[tank + 1] - lo;  // <= kMaxTankCoarse
+  const double scale = row_scale(bound[I]);
+  for (int J = threadIdx.x; J < c; J += kBlock) acc[J] = 0ull;
+  __syncthreads();
+  if (scale > 0.0) {
+    const int32_t me = agg_start[I + 1];
+    for (int32_t m = agg_start[I] + part * kGroups + group; m < me;
+         m += kMatrixParts * kGroups) {
+      const int32_t i = agg_rows[m];
+      for (int32_t e = rows[i] + l; e < rows[i + 1]; e += kMatrixLanes) {
+        const int32_t J = agg_of_row[cols[e]] - lo;
+        if (J >= 0 && J < c)
+          atomicAdd(&acc[J], static_cast<unsigned long long>(llrint(vals[e] * scale)));
+      }
+    }
+  }
+  __syncthreads();
+  unsigned long long* row = out + block_offset[tank] + static_cast<size_t>(I - lo) * c;
+  for (int J = threadIdx.x; J < c; J += kBlock)
+    if (acc[J] != 0ull) atomicAdd(&row[J], acc[J]);
+}
+
+// Integer row I of its tank's block to doubles (one block per aggregate).
+__global__ void __launch_bounds__(kBlock) tank_convert_kernel(const int32_t* agg_tank,
+                                                              const int32_t* tank_agg,
+                                                              const int64_t* block_offset,
+                                                              const double* bound,
+                                                              const unsigned long long* in,
+                                                              double* out) {
+  const int I = blockIdx.x;
+  const int tank = agg_tank[I];
+  const int32_t lo = tank_agg[tank];
+  const int32_t c = tank_agg[tank + 1] - lo;
+  const double scale = row_scale(bound[I]);
+  const size_t base = block_offset[tank] + static_cast<size_t>(I - lo) * c;
+  for (int J = threadIdx.x; J < c; J += kBlock)
+    out[base + J] = scale > 0.0 ? static_cast<double>(static_cast<long long>(in[base + J])) / scale
+                                : 0.0;
 }
 
 // Blocked Gauss-Jordan inverse with partial pivoting.  Step k: pivot row p >= k
@@ -447,6 +591,112 @@ __global__ void __launch_bounds__(kBlock) gj_permute_kernel(const double* a, int
   out[e] = a[r * count + idx[j]];
 }
 
+// Gauss-Jordan inverse of each tank's block (one workgroup per tank, c <=
+// kMaxTankCoarse = kBlock rows: thread t owns row t in the pivot search), with
+// the steps of the blocked elimination above applied one column at a time:
+// pivot p >= k of largest |a_pk| (lowest index on ties), rows k and p swapped,
+// row k scaled by inv = 1 / a_kk with a_kk := 1, then a_rj -= f a_kj for every
+// other row r (f = a_rk before the step, a_rk taken as 0).  work holds the
+// blocks on entry and is overwritten; ainv receives the column-permuted result.
+__global__ void __launch_bounds__(kBlock) tank_gj_kernel(const int32_t* tank_agg,
+                                                         const int64_t* block_offset,
+                                                         double* work, double* ainv,
+                                                         int32_t* status) {
+  static_assert(kMaxTankCoarse <= kBlock, "one pivot candidate per thread");
+  __shared__ double rowk[kMaxTankCoarse], colf[kMaxTankCoarse];
+  __shared__ int32_t perm[kMaxTankCoarse], idx[kMaxTankCoarse];
+  __shared__ double wv[kWaves];
+  __shared__ int wi[kWaves];
+  const int tank = blockIdx.x;
+  const int c = tank_agg[tank + 1] - tank_agg[tank];
+  if (c <= 0) return;  // uniform
+  double* a = work + block_offset[tank];
+  const int t = threadIdx.x;
+  const size_t cc = static_cast<size_t>(c);
+  for (int k = 0; k < c; ++k) {
+    double best = t >= k && t < c ? fabs(a[static_cast<size_t>(t) * cc + k]) : -1.0;
+    int bidx = t >= k && t < c ? t : c;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const double v2 = __shfl_xor(best, off, 64);
+      const int i2 = __shfl_xor(bidx, off, 64);
+      better_pivot(best, bidx, v2, i2);
+    }
+    if ((t & 63) == 0) {
+      wv[t >> 6] = best;
+      wi[t >> 6] = bidx;
+    }
+    __syncthreads();
+    best = wv[0];
+    bidx = wi[0];
+#pragma unroll
+    for (int w = 1; w < kWaves; ++w) better_pivot(best, bidx, wv[w], wi[w]);
+    const bool singular = !(best > 0.0) || !isfinite(best) || bidx >= c;
+    if (singular) {
+      if (t == 0) status[tank] = 1;
+      return;  // uniform: every thread reduced the same partials
+    }
+    const int p = bidx;
+    const double inv = 1.0 / a[static_cast<size_t>(p) * cc + k];
+    if (t == 0) perm[k] = p;
+    __syncthreads();  // every thread has read the pivot and the partials
+    for (int j = t; j < c; j += kBlock) {
+      const double xk = a[static_cast<size_t>(k) * cc + j];
+      const double xp = a[static_cast<size_t>(p) * cc + j];
+      if (p != k) a[static_cast<size_t>(p) * cc + j] = xk;
+      const double v = (j == k ? 1.0 : xp) * inv;
+      a[static_cast<size_t>(k) * cc + j] = v;
+      rowk[j] = v;
+    }
+    __syncthreads();
+    for (int r = t; r < c; r += kBlock) colf[r] = r == k ? 0.0 : a[static_cast<size_t>(r) * cc + k];
+    __syncthreads();
+    for (size_t e = t; e < cc * cc; e += kBlock) {
+      const int r = static_cast<int>(e / cc);
+      const int j = static_cast<int>(e % cc);
+      if (r == k) continue;
+      a[e] = (j == k ? 0.0 : a[e]) - colf[r] * rowk[j];
+    }
+    __syncthreads();
+  }
+  // the row swaps become column swaps of the inverse, in reverse step order
+  if (t == 0) {
+    for (int j = 0; j < c; ++j) idx[j] = j;
+    for (int k = c - 1; k >= 0; --k) {
+      const int p = perm[k];
+      const int32_t tmp = idx[k];
+      idx[k] = idx[p];
+      idx[p] = tmp;
+    }
+  }
+  __syncthreads();
+  double* out = ainv + block_offset[tank];
+  for (size_t e = t; e < cc * cc; e += kBlock)
+    out[e] = a[(e / cc) * cc + idx[e % cc]];
+}
+
+// yc[I] = (row I of its tank's inverse block) . rc over the tank's aggregates;
+// one wavefront per aggregate, lanes strided over the columns, fixed butterfly.
+__global__ void __launch_bounds__(kBlock) tank_product_kernel(
+    int count, const int32_t* agg_tank, const int32_t* tank_agg, const int64_t* block_offset,
+    const double* ainv, const double* rc, const int32_t* status, double* yc,
+    const int32_t* skip = nullptr) {
+  if (launch_skipped(skip)) return;
+  const int I = static_cast<int>(blockIdx.x) * kWaves + (threadIdx.x >> 6);
+  if (I >= count) return;  // uniform per wavefront
+  const int lane = threadIdx.x & 63;
+  const int tank = agg_tank[I];
+  const int32_t lo = tank_agg[tank];
+  const int32_t c = tank_agg[tank + 1] - lo;
+  double s = 0.0;
+  if (!status[tank]) {  // a singular block gives its tank no correction
+    const double* row = ainv + block_offset[tank] + static_cast<size_t>(I - lo) * c;
+    for (int32_t J = lane; J < c; J += 64) s += row[J] * rc[lo + J];
+  }
+  s = wave_sum(s);
+  if (lane == 0) yc[I] = s;
+}
+
 __global__ void __launch_bounds__(kBlock) inverse_kernel(size_t n, const double* pdiag,
                                                          double* wfac) {
   const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -550,6 +800,8 @@ void coarse_aggregate(Context& ctx, CoarseSpace& C, size_t n, const int32_t* lab
   C.factor = 0;
   C.n = n;
   C.ready = false;
+  C.tanks = 0;
+  C.block_entries = 0;
   if (n == 0 || factor < 1 || dims[0] <= 0 || dims[1] <= 0 || dims[2] <= 0) return;
   C.agg_of_row.reserve(n);
   const int32_t widest = std::max(dims[0], std::max(dims[1], dims[2]));
@@ -597,8 +849,117 @@ void coarse_aggregate(Context& ctx, CoarseSpace& C, size_t n, const int32_t* lab
   }
 }
 
+void coarse_aggregate_tanks(Context& ctx, CoarseSpace& C, size_t n, const int32_t* labels,
+                            const int32_t* cell_sorted, const int32_t* cell_begin,
+                            const int32_t* cell_end, const std::vector<CoarseTankGrid>& grids,
+                            int factor) {
+  C.count = 0;
+  C.factor = 0;
+  C.n = n;
+  C.ready = false;
+  C.tanks = 0;
+  C.block_entries = 0;
+  const int tanks = static_cast<int>(grids.size());
+  if (n == 0 || factor < 1 || tanks == 0) return;
+  int32_t widest = 1;
+  for (const CoarseTankGrid& g : grids) {
+    require(g.nx > 0 && g.ny > 0 && g.nz > 0 && g.base >= 0, "Invalid cell grid configuration");
+    widest = std::max(widest, std::max(g.nx, std::max(g.ny, g.nz)));
+  }
+  C.agg_of_row.reserve(n);
+  C.tank_grids.reserve(static_cast<size_t>(tanks));
+  C.tank_cc.reserve(static_cast<size_t>(tanks) + 1);
+  C.tank_agg.reserve(static_cast<size_t>(tanks) + 2);
+  C.block_offset.reserve(static_cast<size_t>(tanks) + 1);
+  C.grids_host.resize(tanks);
+  C.tank_cc_host.resize(static_cast<size_t>(tanks) + 1);
+  C.tank_agg_host.resize(static_cast<size_t>(tanks) + 2);
+  for (int f = factor; f <= widest; ++f) {
+    int64_t ccells = 0;
+    for (int k = 0; k < tanks; ++k) {
+      const CoarseTankGrid& g = grids[k];
+      TankCoarseGrid& t = C.grids_host[k];
+      t = TankCoarseGrid{g.nx, g.ny, g.nz, (g.nx + f - 1) / f, (g.ny + f - 1) / f,
+                         (g.nz + f - 1) / f, f, g.base, static_cast<int32_t>(ccells)};
+      C.tank_cc_host[k] = static_cast<int32_t>(ccells);
+      ccells += static_cast<int64_t>(t.cx) * t.cy * t.cz;
+      require(ccells < INT32_MAX, "Cell count exceeds 32-bit index capacity");
+    }
+    C.tank_cc_host[tanks] = static_cast<int32_t>(ccells);
+    const size_t nc = static_cast<size_t>(ccells);
+    C.cell_count.reserve(nc);
+    C.cell_start.reserve(nc);
+    C.cell_flag.reserve(nc);
+    C.cell_id.reserve(nc);
+    LSMPS_HIP(hipMemcpyAsync(C.tank_grids.data(), C.grids_host.data(),
+                             tanks * sizeof(TankCoarseGrid), hipMemcpyHostToDevice, ctx.stream));
+    LSMPS_HIP(hipMemcpyAsync(C.tank_cc.data(), C.tank_cc_host.data(), (tanks + 1) * sizeof(int32_t),
+                             hipMemcpyHostToDevice, ctx.stream));
+    count_tank_members_kernel<<<wave_blocks(nc), kBlock, 0, ctx.stream>>>(
+        C.tank_grids.data(), C.tank_cc.data(), tanks, nc, labels, cell_sorted, cell_begin,
+        cell_end, C.cell_count.data(), C.cell_flag.data());
+    LSMPS_LAUNCH_CHECK();
+    exclusive_scan_i32(ctx, C.cell_count.data(), C.cell_start.data(), nc);
+    exclusive_scan_i32(ctx, C.cell_flag.data(), C.cell_id.data(), nc);
+    tank_ranges_kernel<<<blocks_for(static_cast<size_t>(tanks) + 2), kBlock, 0, ctx.stream>>>(
+        C.tank_cc.data(), tanks, nc, C.cell_count.data(), C.cell_flag.data(),
+        C.cell_start.data(), C.cell_id.data(), C.tank_agg.data());
+    LSMPS_LAUNCH_CHECK();
+    LSMPS_HIP(hipMemcpyAsync(C.tank_agg_host.data(), C.tank_agg.data(),
+                             (tanks + 2) * sizeof(int32_t), hipMemcpyDeviceToHost, ctx.stream));
+    ctx.sync();
+    const int count = C.tank_agg_host[tanks];
+    const int members = C.tank_agg_host[tanks + 1];
+    if (count <= 0 || members <= 0) return;  // no aggregated rows
+    int widest_block = 0;
+    for (int k = 0; k < tanks; ++k)
+      widest_block = std::max(widest_block, C.tank_agg_host[k + 1] - C.tank_agg_host[k]);
+    if (widest_block > kMaxTankCoarse) continue;  // wider aggregates
+    C.agg_start.reserve(static_cast<size_t>(count) + 1);
+    C.agg_rows.reserve(static_cast<size_t>(members));
+    C.agg_tank.reserve(static_cast<size_t>(count));
+    fill_i32<<<blocks_for(n), kBlock, 0, ctx.stream>>>(C.agg_of_row.data(), n, -1);
+    fill_tank_members_kernel<<<wave_blocks(nc), kBlock, 0, ctx.stream>>>(
+        C.tank_grids.data(), C.tank_cc.data(), tanks, nc, labels, cell_sorted, cell_begin,
+        cell_end, C.cell_count.data(), C.cell_start.data(), C.cell_id.data(), C.agg_start.data(),
+        C.agg_rows.data(), C.agg_of_row.data());
+    set_last_kernel<<<1, 1, 0, ctx.stream>>>(C.agg_start.data(), count, C.cell_count.data(),
+                                             C.cell_start.data(), nc);
+    agg_tank_kernel<<<static_cast<unsigned>(tanks), kBlock, 0, ctx.stream>>>(C.tank_agg.data(),
+                                                                            C.agg_tank.data());
+    LSMPS_LAUNCH_CHECK();
+    C.offsets_host.resize(static_cast<size_t>(tanks) + 1);
+    int64_t entries = 0;
+    for (int k = 0; k < tanks; ++k) {
+      C.offsets_host[k] = entries;
+      const int64_t c = C.tank_agg_host[k + 1] - C.tank_agg_host[k];
+      entries += c * c;
+    }
+    C.offsets_host[tanks] = entries;
+    LSMPS_HIP(hipMemcpyAsync(C.block_offset.data(), C.offsets_host.data(),
+                             (tanks + 1) * sizeof(int64_t), hipMemcpyHostToDevice, ctx.stream));
+    ctx.sync();  // the host sources of the uploads stay valid until here anyway
+    C.count = count;
+    C.factor = f;
+    C.tanks = tanks;
+    C.block_entries = static_cast<size_t>(entries);
+    return;
+  }
+}
+
 void coarse_reserve(CoarseSpace& C) {
   const int count = C.count;
+  if (C.tanks > 0) {
+    if (count < 1) return;
+    C.work.reserve(std::max<size_t>(C.block_entries, 1));
+    C.ainv.reserve(std::max<size_t>(C.block_entries, 1));
+    C.status.reserve(static_cast<size_t>(C.tanks));
+    C.rc.reserve(static_cast<size_t>(count));
+    C.yc.reserve(static_cast<size_t>(count));
+    C.rowabs.reserve(C.n);
+    C.wfac.reserve(C.n);
+    return;
+  }
   if (count <= 1) return;
   const size_t nn = static_cast<size_t>(count) * count;
   C.work.reserve(nn);
@@ -617,6 +978,33 @@ void coarse_setup(Context& ctx, CoarseSpace& C, const lsmps_csr& A, const double
   (void)ctx;
   C.ready = false;
   const int count = C.count;
+  if (C.tanks > 0) {
+    if (count < 1 || C.n != A.rows) return;
+    require(C.work.size() >= C.block_entries && C.ainv.size() >= C.block_entries &&
+                C.rc.size() >= static_cast<size_t>(count) && C.rowabs.size() >= A.rows &&
+                C.status.size() >= static_cast<size_t>(C.tanks),
+            "coarse setup before coarse_reserve", LSMPS_ERR_INTERNAL);
+    const unsigned tb = static_cast<unsigned>(C.tanks);
+    fill_i32<<<blocks_for(C.tanks), kBlock, 0, st>>>(C.status.data(), C.tanks, 0);
+    unsigned long long* acci = reinterpret_cast<unsigned long long*>(C.ainv.data());
+    row_bound_kernel<<<blocks_for(A.rows), kBlock, 0, st>>>(
+        A.rows, C.agg_of_row.data(), diag, offabs, C.rowabs.data());
+    restrict_kernel<double><<<static_cast<unsigned>(count), kBlock, 0, st>>>(
+        count, C.agg_start.data(), C.agg_rows.data(), C.rowabs.data(), nullptr, C.rc.data());
+    fill_u64<<<blocks_for(C.block_entries), kBlock, 0, st>>>(acci, C.block_entries);
+    tank_matrix_kernel<<<static_cast<unsigned>(count) * kMatrixParts, kBlock, 0, st>>>(
+        C.agg_tank.data(), C.tank_agg.data(), C.block_offset.data(), C.agg_start.data(),
+        C.agg_rows.data(), C.agg_of_row.data(), A.row_offsets, A.col_indices, A.values,
+        C.rc.data(), acci);
+    tank_convert_kernel<<<static_cast<unsigned>(count), kBlock, 0, st>>>(
+        C.agg_tank.data(), C.tank_agg.data(), C.block_offset.data(), C.rc.data(), acci,
+        C.work.data());
+    tank_gj_kernel<<<tb, kBlock, 0, st>>>(C.tank_agg.data(), C.block_offset.data(),
+                                          C.work.data(), C.ainv.data(), C.status.data());
+    LSMPS_LAUNCH_CHECK();
+    C.ready = true;  // singular blocks are flagged per tank in C.status
+    return;
+  }
   if (count <= 1 || C.n != A.rows) return;
   const size_t nn = static_cast<size_t>(count) * count;
   require(C.work.size() >= nn && C.ainv.size() >= nn && C.rc.size() >= static_cast<size_t>(count) &&
@@ -672,8 +1060,17 @@ void coarse_set_weights(Context& ctx, CoarseSpace& C, const double* pdiag) {
   LSMPS_LAUNCH_CHECK();
 }
 
-bool coarse_usable(Context& ctx, const CoarseSpace& C) {
+bool coarse_usable(Context& ctx, CoarseSpace& C) {
   if (!C.ready) return false;
+  if (C.tanks > 0) {
+    C.status_host.resize(static_cast<size_t>(C.tanks));
+    LSMPS_HIP(hipMemcpyAsync(C.status_host.data(), C.status.data(), C.tanks * sizeof(int32_t),
+                             hipMemcpyDeviceToHost, ctx.stream));
+    ctx.sync();
+    for (int k = 0; k < C.tanks; ++k)
+      if (!C.status_host[k] && C.tank_agg_host[k + 1] > C.tank_agg_host[k]) return true;
+    return false;
+  }
   LSMPS_HIP(hipMemcpyAsync(ctx.host_i32, C.status.data(), sizeof(int32_t), hipMemcpyDeviceToHost,
                            ctx.stream));
   ctx.sync();
@@ -686,6 +1083,13 @@ void coarse_solve(Context& ctx, CoarseSpace& C, const W* w, bool scale_by_wfac,
   restrict_kernel<W><<<static_cast<unsigned>(C.count), kBlock, 0, ctx.stream>>>(
       C.count, C.agg_start.data(), C.agg_rows.data(), w, scale_by_wfac ? C.wfac.data() : nullptr,
       C.rc.data(), skip);
+  if (C.tanks > 0) {
+    tank_product_kernel<<<wave_blocks(static_cast<size_t>(C.count)), kBlock, 0, ctx.stream>>>(
+        C.count, C.agg_tank.data(), C.tank_agg.data(), C.block_offset.data(), C.ainv.data(),
+        C.rc.data(), C.status.data(), C.yc.data(), skip);
+    LSMPS_LAUNCH_CHECK();
+    return;
+  }
   coarse_product_kernel<<<static_cast<unsigned>(C.count), kBlock, 0, ctx.stream>>>(
       C.count, C.ainv.data(), C.rc.data(), C.status.data(), C.yc.data(), skip);
   LSMPS_LAUNCH_CHECK();

@@ -14,13 +14,34 @@
 // Determinism: Ac is accumulated in 64-bit fixed point (integer additions are
 // associative), and every per-aggregate or per-row sum runs in a fixed order, so
 // a solve is bit-reproducible run to run.
+//
+// Batched tanks (independent systems stacked in one matrix): each tank is
+// aggregated over its own cell grid and its aggregates are numbered after the
+// previous tank's, so Ac is block diagonal.  Each tank's block is stored
+// densely (packed one after the other), inverted by its own workgroup, and
+// applied by its own rows only: a tank's coarse correction depends on its rows
+// alone.
 #pragma once
+
+#include <vector>
 
 #include "common.hpp"
 
 namespace lsmps {
 
-constexpr int kMaxCoarse = 4096;  // LDS row of the coarse-matrix build (int64)
+constexpr int kMaxCoarse = 4096;      // LDS row of the coarse-matrix build (int64)
+constexpr int kMaxTankCoarse = 256;   // aggregates per tank of a batched coarse space
+
+// One tank's cell grid: dims and the index of its first cell in the batched
+// cell list (cells of a tank are numbered (z, y, x) from `base`).
+struct CoarseTankGrid {
+  int32_t nx, ny, nz, base;
+};
+// A tank's coarse grid on the device: fine dims, coarse dims, factor, first
+// fine cell and first coarse cell of the tank.
+struct TankCoarseGrid {
+  int32_t nx, ny, nz, cx, cy, cz, f, base, cc_base;
+};
 
 struct CoarseSpace {
   size_t n = 0;                      // fine rows the aggregation was built for
@@ -41,11 +62,27 @@ struct CoarseSpace {
   DeviceArray<double> rowabs;        // n: setup scratch (row bounds)
   DeviceArray<double> rc, yc;        // count
   bool ready = false;                // ainv valid for the current solve
+  // batched tanks (tanks > 0): ainv and work hold one dense block per tank,
+  // block k (c_k = tank_agg[k + 1] - tank_agg[k] rows, row-major) at entry
+  // block_offset[k]; status holds one singular-pivot flag per tank
+  int tanks = 0;
+  size_t block_entries = 0;                // sum of c_k^2
+  std::vector<int32_t> tank_agg_host;      // tanks + 2: tank_agg's host copy
+  std::vector<TankCoarseGrid> grids_host;  // sources of the uploads (kept alive)
+  std::vector<int32_t> tank_cc_host;
+  std::vector<int64_t> offsets_host;
+  std::vector<int32_t> status_host;
+  DeviceArray<TankCoarseGrid> tank_grids;  // tanks
+  DeviceArray<int32_t> tank_cc;            // tanks + 1: first coarse cell of each tank
+  DeviceArray<int32_t> tank_agg;           // tanks + 2: first aggregate of each tank, members
+  DeviceArray<int64_t> block_offset;       // tanks + 1
+  DeviceArray<int32_t> agg_tank;           // count: tank of an aggregate
   size_t bytes() const {
     return agg_of_row.bytes() + agg_start.bytes() + agg_rows.bytes() + cell_count.bytes() +
            cell_start.bytes() + cell_flag.bytes() + cell_id.bytes() + ainv.bytes() +
            work.bytes() + colk.bytes() + perm.bytes() + status.bytes() + wfac.bytes() + rowabs.bytes() +
-           rc.bytes() + yc.bytes();
+           rc.bytes() + yc.bytes() + tank_grids.bytes() + tank_cc.bytes() + tank_agg.bytes() +
+           block_offset.bytes() + agg_tank.bytes();
   }
 };
 
@@ -60,6 +97,15 @@ void coarse_aggregate(Context& ctx, CoarseSpace& C, size_t n, const int32_t* lab
                       const int32_t* cell_sorted, const int32_t* cell_begin,
                       const int32_t* cell_end, const int32_t dims[3], int factor);
 
+// Batched tanks: tank k's rows are aggregated as above over its own grid
+// (grids[k]; its cells start at grids[k].base in the batched cell list), with
+// one factor f for every tank, grown by one while some tank has more than
+// kMaxTankCoarse aggregates.  Aggregates are numbered tank by tank.
+void coarse_aggregate_tanks(Context& ctx, CoarseSpace& C, size_t n, const int32_t* labels,
+                            const int32_t* cell_sorted, const int32_t* cell_begin,
+                            const int32_t* cell_end, const std::vector<CoarseTankGrid>& grids,
+                            int factor);
+
 // Reserves every per-solve buffer of coarse_setup / coarse_set_weights for the
 // current aggregation (the only device allocations of the coarse solve).
 void coarse_reserve(CoarseSpace& C);
@@ -72,8 +118,10 @@ void coarse_setup(Context& ctx, CoarseSpace& C, const lsmps_csr& A, const double
                   const double* offabs, hipStream_t st);
 // wfac = 1 / pdiag (the fine preconditioner's diagonal part), on ctx's stream.
 void coarse_set_weights(Context& ctx, CoarseSpace& C, const double* pdiag);
-// After the setup has completed: whether the inverse is valid (host wait).
-bool coarse_usable(Context& ctx, const CoarseSpace& C);
+// After the setup has completed: whether the inverse is valid (host wait).  For
+// batched tanks: whether it is valid for at least one tank (a tank whose block
+// met a zero pivot gets no correction).
+bool coarse_usable(Context& ctx, CoarseSpace& C);
 
 // yc = Ac^-1 P^T (wfac * w) (wfac may be skipped: then yc = Ac^-1 P^T w).
 template <typename W>

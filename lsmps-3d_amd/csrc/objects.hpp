@@ -93,6 +93,7 @@ struct lsmps_stepper {
   lsmps::DeviceArray<lsmps::SegmentGrid> segment_grids;
   lsmps::DeviceArray<int32_t> fluid_segment, wall_segment;
   lsmps::DeviceArray<int32_t> segment_rows;  // fluid_offsets as int32 (the PPE's segments)
+  std::vector<lsmps::CoarseTankGrid> coarse_grids;  // per-tank grids of the coarse space
   hipEvent_t events[8] = {};
   int n_events = 0;
   double stage_ms[7] = {0, 0, 0, 0, 0, 0, 0};

@@ -1834,6 +1834,31 @@ void append_coarse_signature(std::vector<uintptr_t>& sig, const CoarseSpace& C) 
   sig.push_back(C.n);
 }
 
+// Algorithmic bytes of one SpMV y = A x over n rows: the stored nonzeros'
+// values, the column indices of the remainder entries (16-bit, or 32-bit in the
+// chunks that need it, in the layout's proportion; DIA columns are implicit),
+// the per-chunk descriptors, diagonal offsets and window bases, x read once and
+// y written once.  Padding is not counted.
+double spmv_algorithmic_bytes(const SellMatrix& M, size_t n, size_t value_bytes,
+                              size_t vector_bytes) {
+  const double col_bytes =
+      2.0 * static_cast<double>(M.col_units) / static_cast<double>(std::max<size_t>(M.entries, 1));
+  return static_cast<double>(M.nnz) * value_bytes + static_cast<double>(M.rem_nnz) * col_bytes +
+         M.chunks * (6.0 + kSellWindows) * sizeof(int32_t) +
+         static_cast<double>(M.dia_entries) / kSellChunk * sizeof(int32_t) +
+         2.0 * n * vector_bytes;
+}
+
+// Whether the SpMVs are timed (LSMPS_FLAG_TIMING); creates the events once.
+bool spmv_timing_ready(Context& ctx) {
+  const bool timing = ctx.timing();
+  if (timing && !ctx.timing_ready) {
+    for (auto& e : ctx.timing_events) LSMPS_HIP(hipEventCreate(&e));
+    ctx.timing_ready = true;
+  }
+  return timing;
+}
+
 // Makes S.cycle_exec[0..1] the captured cycle for `sig`, capturing it (twice:
 // SpMV event sets 0 and 1) when the signature changed.  enqueue(halt,
 // skip_replace, skip_reinit, set) launches one cycle on ctx.stream.  Returns
@@ -1911,7 +1936,7 @@ int idr_solve(Context& ctx, PpeSolver& S, size_t n, const SellMatrix& M, const V
   T* G = work + 3 * ld;
   T* U = work + (3 + s) * ld;
   const bool schwarz = S.schwarz.blocks > 0 && S.schwarz.n == n;
-  const bool coarse = schwarz && S.coarse.ready && S.coarse.n == n;
+  const bool coarse = schwarz && S.coarse.ready && S.coarse.n == n && S.coarse.tanks == 0;
   const V* dia = sizeof(V) == sizeof(double) ? reinterpret_cast<const V*>(M.dia64.data())
                                              : reinterpret_cast<const V*>(M.dia32.data());
   IdrVec<T> iv{n, ld, s, r, x, v, t, G, U, p0, S.scalars.data(), S.part_b.data(),
@@ -1933,23 +1958,8 @@ int idr_solve(Context& ctx, PpeSolver& S, size_t n, const SellMatrix& M, const V
   const double target2 = target * target;
   const double factor2 = replace_factor * replace_factor;
   // Optional per-launch timing of the SpMVs (events bracket each launch).
-  const bool timing = ctx.timing();
-  if (timing && !ctx.timing_ready) {
-    for (auto& e : ctx.timing_events) LSMPS_HIP(hipEventCreate(&e));
-    ctx.timing_ready = true;
-  }
-  // Algorithmic bytes of one SpMV: the stored nonzeros' values, the column
-  // indices of the remainder entries (16-bit, or 32-bit in the chunks that need
-  // it, in the layout's proportion; DIA columns are implicit), the per-chunk
-  // descriptors, diagonal offsets and window bases, x read once and y written
-  // once.  Padding is not counted.
-  const double col_bytes =
-      2.0 * static_cast<double>(M.col_units) / static_cast<double>(std::max<size_t>(M.entries, 1));
-  const double spmv_bytes = static_cast<double>(M.nnz) * sizeof(V) +
-                            static_cast<double>(M.rem_nnz) * col_bytes +
-                            M.chunks * (6.0 + kSellWindows) * sizeof(int32_t) +
-                            static_cast<double>(M.dia_entries) / kSellChunk * sizeof(int32_t) +
-                            2.0 * n * sizeof(T);
+  const bool timing = spmv_timing_ready(ctx);
+  const double spmv_bytes = spmv_algorithmic_bytes(M, n, sizeof(V), sizeof(T));
   // One IDR(s) cycle: s steps, each with one SpMV G_k = A U_k (and, with the
   // Schwarz blocks, their products and the coarse term before it), then the
   // omega step's SpMV and the residual norm.  `skip` (null when launched
@@ -2153,6 +2163,9 @@ void idr_solve_segmented(Context& ctx, PpeSolver& S, size_t n, const SellMatrix&
   T* G = work + 3 * ld;
   T* U = work + (3 + s) * ld;
   const bool schwarz = S.schwarz.blocks > 0 && S.schwarz.n == n;
+  // per-tank coarse space (block-diagonal Ac): applied like the single
+  // system's, M^-1 w = Schwarz/Jacobi(w) + P Ac^-1 P^T w
+  const bool coarse = schwarz && S.coarse.ready && S.coarse.n == n && S.coarse.tanks == nseg;
   const V* dia = sizeof(V) == sizeof(double) ? reinterpret_cast<const V*>(M.dia64.data())
                                              : reinterpret_cast<const V*>(M.dia32.data());
   // the segmented recurrence applies the Schwarz blocks with schwarz_apply, so
@@ -2188,13 +2201,34 @@ void idr_solve_segmented(Context& ctx, PpeSolver& S, size_t n, const SellMatrix&
     scalar_norm<<<nseg, kReduceThreads, 0, ctx.stream>>>(S.scalars.data(), part, kSegBlocks);
     LSMPS_LAUNCH_CHECK();
   };
+  // SpMV timing as in the single-system solve: events around each launch of a
+  // cycle (slots 2 k, 2 k + 1), read after the cycle's status copy has waited
+  const bool timing = spmv_timing_ready(ctx);
+  const double spmv_bytes = spmv_algorithmic_bytes(M, n, sizeof(V), sizeof(T));
+  int timed = 0;  // SpMVs recorded in the current cycle
   auto spmv = [&](const T* in, T* out) {
     SellArgs<T, V> a{n, M.chunks, M.dia_pairs.data(), M.dia_singles.data(), M.dia_plan.data(),
                      M.dia_start.data(),
                      dia, M.chunk_start.data(), M.chunk_width.data(), M.col_start.data(),
                      M.nwin.data(), M.windows.data(), M.cols.data(), vals, in, out, p0, s,
                      nullptr, nullptr};
+    const bool mark = timing && 2 * timed + 1 < Context::kTimingSet;
+    if (mark) LSMPS_HIP(hipEventRecord(ctx.timing_events[2 * timed], ctx.stream));
     sell_spmv_kernel<T, V, 0><<<sb, kSellBlock, 0, ctx.stream>>>(a);
+    if (mark) {
+      LSMPS_HIP(hipEventRecord(ctx.timing_events[2 * timed + 1], ctx.stream));
+      ++timed;
+    }
+  };
+  auto add_timing = [&]() {  // after a host wait that followed the cycle's SpMVs
+    for (int k = 0; k < timed; ++k) {
+      float ms = 0.0f;
+      LSMPS_HIP(hipEventElapsedTime(&ms, ctx.timing_events[2 * k], ctx.timing_events[2 * k + 1]));
+      ctx.spmv_ms += ms;
+      ctx.spmv_bytes += spmv_bytes;
+      ++ctx.spmv_launches;
+    }
+    timed = 0;
   };
   // |b_k|^2 per segment (r = b on entry, x = 0)
   std::vector<double> bnorm2(nseg), rr(nseg), rr_rep(nseg), target2(nseg);
@@ -2225,7 +2259,10 @@ void idr_solve_segmented(Context& ctx, PpeSolver& S, size_t n, const SellMatrix&
     for (int k = 0; k < s; ++k) {
       if (schwarz) {
         idr_a1_kernel<T, true><<<vb, kBlock, 0, ctx.stream>>>(iv, k);
+        // the coarse restriction reads w = v / pdiag before the blocks overwrite v
+        if (coarse) coarse_solve(ctx, S.coarse, v, true);
         schwarz_apply<T>(ctx, S.schwarz, v);
+        if (coarse) coarse_prolong_add(ctx, S.coarse, v);
         idr_a2_kernel<T, true><<<vb, kBlock, 0, ctx.stream>>>(iv, k);
       } else {
         idr_a_kernel<T, true><<<vb, kBlock, 0, ctx.stream>>>(iv, k);
@@ -2237,7 +2274,11 @@ void idr_solve_segmented(Context& ctx, PpeSolver& S, size_t n, const SellMatrix&
       idr_b_kernel<T, true><<<vb, kBlock, 0, ctx.stream>>>(iv, k, k == s - 1 ? 1 : 0);
       ++it;
     }
-    if (schwarz) schwarz_apply<T>(ctx, S.schwarz, v);
+    if (schwarz) {
+      if (coarse) coarse_solve(ctx, S.coarse, r, false);
+      schwarz_apply<T>(ctx, S.schwarz, v);
+      if (coarse) coarse_prolong_add(ctx, S.coarse, v);
+    }
     spmv(v, t);
     seg_reduce_kernel<T, 2><<<rb, kBlock, 0, ctx.stream>>>(g.start, s, p0, t, r, part);
     idr_scalar_omega<<<nseg, kReduceThreads, 0, ctx.stream>>>(S.scalars.data(), part, part,
@@ -2246,6 +2287,7 @@ void idr_solve_segmented(Context& ctx, PpeSolver& S, size_t n, const SellMatrix&
     segment_norms();
     ++it;
     read_status(rr, brk);
+    add_timing();
     bool need_true = false;
     for (int k = 0; k < nseg; ++k) {
       if (done[k]) continue;
@@ -2295,6 +2337,7 @@ void idr_solve_segmented(Context& ctx, PpeSolver& S, size_t n, const SellMatrix&
   rep.refinements = replacements;
   rep.rhs_norm = std::sqrt(bn);
   rep.relative_residual = worst;  // the largest per-segment relative residual
+  if (coarse && coarse_usable(ctx, S.coarse)) rep.coarse_unknowns = S.coarse.count;
   rep.converged = all ? 1 : 0;
   (void)total;
 }
@@ -2388,8 +2431,11 @@ void ppe_solve(Context& ctx, PpeSolver& S, const lsmps_csr& A, const double* b, 
   // s = 0 picks 14 when the coarse space is requested and 16 otherwise: with
   // the smooth modes removed by the coarse term, s = 12..16 cost about the same
   // (fewer iterations against more vector traffic per step).
-  const bool coarse_requested = opt.preconditioner == LSMPS_PRECOND_SCHWARZ_COARSE &&
-                                S.segments.count == 0 && S.coarse.count > 1 && S.coarse.n == n;
+  // (batched tanks: the caller's aggregation is per tank, one block each)
+  const bool coarse_requested =
+      opt.preconditioner == LSMPS_PRECOND_SCHWARZ_COARSE && S.coarse.n == n &&
+      (S.segments.count == 0 ? S.coarse.tanks == 0 && S.coarse.count > 1
+                             : S.coarse.tanks == S.segments.count && S.coarse.count > 0);
   if (shadow_dim <= 0) shadow_dim = coarse_requested ? 14 : 16;
   if (shadow_dim > kMaxShadow) shadow_dim = kMaxShadow;
   if (static_cast<size_t>(shadow_dim) > n) shadow_dim = static_cast<int>(n);
@@ -2433,8 +2479,8 @@ void ppe_solve(Context& ctx, PpeSolver& S, const lsmps_csr& A, const double* b, 
     } else {
       build_sell<double>(ctx, S.sell, A, nullptr);
     }
-    // The coarse space (it needs the caller's aggregation of these rows, one
-    // system) is set up on the library stream after the Schwarz blocks and the
+    // The coarse space (it needs the caller's aggregation of these rows: one
+    // system, or one block per tank) is set up on the library stream after the Schwarz blocks and the
     // SpMV layout: every buffer it touches is reserved first, on this thread,
     // and nothing is enqueued on another stream.
     if (coarse_requested && S.schwarz.n == n) {

@@ -150,6 +150,7 @@ void lsmps_stepper::set_segments(int count, const int64_t* fo, const int64_t* wo
     segment_cells = 0;
     fluid_offsets.clear();
     wall_offsets.clear();
+    coarse_grids.clear();
     return;
   }
   require(fo && wo && grids, "segments require fluid offsets, wall offsets and grids");
@@ -195,6 +196,9 @@ void lsmps_stepper::set_segments(int count, const int64_t* fo, const int64_t* wo
   c.sync();  // the host vectors go out of scope
   fluid_offsets.assign(fo, fo + count + 1);
   wall_offsets.assign(wo, wo + count + 1);
+  coarse_grids.resize(count);
+  for (int k = 0; k < count; ++k)
+    coarse_grids[k] = CoarseTankGrid{host[k].nx, host[k].ny, host[k].nz, host[k].base};
   segment_count = count;
   segment_cells = static_cast<size_t>(base);
 }
@@ -290,12 +294,19 @@ void lsmps_stepper::step(const lsmps_fluid_soa& fluid, const lsmps_wall_soa& wal
   }
   lsmps_solver_options opts = solver_options;
   if (record) opts.clamp_negative = 0;  // the correction clamps the same way
-  // coarse space: aggregates of coarse_cells^3 fluid cells (one system only)
+  // coarse space: aggregates of coarse_cells^3 fluid cells (batched tanks:
+  // per tank, over its own grid)
   solver.coarse.count = 0;
-  if (segment_count == 0 && opts.preconditioner == LSMPS_PRECOND_SCHWARZ_COARSE && nf > 0)
-    coarse_aggregate(c, solver.coarse, nf, fluid.surface_type, fc_sorted.data(), fc_begin.data(),
-                     fc_end.data(), config.cell_dims,
-                     opts.coarse_cells > 0 ? opts.coarse_cells : 4);
+  if (opts.preconditioner == LSMPS_PRECOND_SCHWARZ_COARSE && nf > 0) {
+    const int f = opts.coarse_cells > 0 ? opts.coarse_cells : 4;
+    if (segment_count > 0) {
+      coarse_aggregate_tanks(c, solver.coarse, nf, fluid.surface_type, fc_sorted.data(),
+                             fc_begin.data(), fc_end.data(), coarse_grids, f);
+    } else {
+      coarse_aggregate(c, solver.coarse, nf, fluid.surface_type, fc_sorted.data(),
+                       fc_begin.data(), fc_end.data(), config.cell_dims, f);
+    }
+  }
   try {
     ppe_solve(c, solver, ws.matrix, ws.rhs, ws.pressure, opts, &last_report);
   } catch (const Error& e) {
