@@ -56,6 +56,9 @@ def parse():
     p.add_argument("--no-autotune", action="store_true",
                    help="fixed lanes per particle instead of the autotuner's choice "
                         "(LSMPS_FLAG_AUTOTUNE, tuned in the warm-up steps)")
+    p.add_argument("--schwarz-fp32", action="store_true",
+                   help="mixed mode: apply the Schwarz block inverses from FP32 instead of "
+                        "their scaled FP16 copies (LSMPS_FLAG_SCHWARZ_FP32)")
     p.add_argument("--inner-tolerance", type=float, default=0.0,
                    help="mixed mode: residual drop between reliable updates (0: the default)")
     p.add_argument("--cpu-threads", type=int, default=0)
@@ -82,12 +85,13 @@ def make_case(cases, name):
 
 
 def run_gpu(pkg, rt, case, precision, steps, warmup, timing=True, adjoint=False, shadow=0,
-            coarse=0, graph=True, inner_tolerance=0.0, autotune=True):
+            coarse=0, graph=True, inner_tolerance=0.0, autotune=True, schwarz_fp32=False):
     import numpy as np
     import torch
     abi = pkg.abi
     flags = ((abi.FLAG_KERNEL_TIMING if timing else 0) | (abi.FLAG_GRAPH if graph else 0) |
-             (abi.FLAG_AUTOTUNE if autotune else 0))
+             (abi.FLAG_AUTOTUNE if autotune else 0) |
+             (abi.FLAG_SCHWARZ_FP32 if schwarz_fp32 else 0))
     ctx = rt.Context(precision, flags=flags)
     state = rt.DeviceState(case)
     opts = abi.default_solver_options()
@@ -319,7 +323,8 @@ def main():
     main_run = run_gpu(pkg, rt, case, prec, args.steps, args.warmup,
                        adjoint=not args.no_adjoint, shadow=args.shadow_dimension,
                        coarse=args.coarse_cells, graph=False,
-                       inner_tolerance=args.inner_tolerance, autotune=not args.no_autotune)
+                       inner_tolerance=args.inner_tolerance, autotune=not args.no_autotune,
+                       schwarz_fp32=args.schwarz_fp32)
     iters = [r["iterations"] for r in main_run["reports"]]
     print(f"[bench] {args.precision}: {main_run['ms_per_step']:.1f} ms/step, iterations {iters}, "
           f"stages {np.round(main_run['stage_ms'], 2).tolist()}", file=sys.stderr, flush=True)
@@ -356,7 +361,7 @@ def main():
                                 "C3": "64 batched tanks, 2k-60k fluid each, one launch per stage",
                                 "C5": "splash 160^3 (4,096,000 fluid), top 8 layers lifted"}[args.case],
                    "n_fluid": case.n_fluid, "n_wall": case.n_wall, "l0_m": case.config.particle_spacing,
-                   "re_over_l0": 3.1, "dt_s": case.config.time_step, "solver": f"IDR({args.shadow_dimension or (16 if args.coarse_cells < 0 else 14)}); Schwarz blocks on wrong-sign rows, Jacobi elsewhere" + ("" if args.coarse_cells < 0 else f", coarse space of {args.coarse_cells or pkg.abi.default_solver_options().coarse_cells}^3-cell aggregates"),
+                   "re_over_l0": 3.1, "dt_s": case.config.time_step, "solver": f"IDR({args.shadow_dimension or (16 if args.coarse_cells < 0 else 14)}); Schwarz blocks on wrong-sign rows, Jacobi elsewhere" + ("" if args.coarse_cells < 0 else f", coarse space of {args.coarse_cells or pkg.abi.default_solver_options().coarse_cells}^3-cell aggregates") + ("" if args.precision != "mixed" else ("; Schwarz inverses FP32" + ("" if args.schwarz_fp32 else ", applied from scaled FP16 copies"))),
                    "tolerance": 1e-10, "precision": args.precision},
         "solver_iterations": iters,
         "lanes_per_particle": main_run["lanes"],
@@ -388,7 +393,7 @@ def main():
         graph_run = run_gpu(pkg, rt, case, prec, 5, 1, timing=False,
                             shadow=args.shadow_dimension, coarse=args.coarse_cells, graph=True,
                             inner_tolerance=args.inner_tolerance,
-                            autotune=not args.no_autotune)
+                            autotune=not args.no_autotune, schwarz_fp32=args.schwarz_fp32)
         line["graph"] = {
             "ms_per_step": graph_run["ms_per_step"],
             "graph_cycles": [r["graph_cycles"] for r in graph_run["reports"]],

@@ -97,11 +97,17 @@ enum { LSMPS_SOLVER_PCG = 0, LSMPS_SOLVER_BICGSTAB = 1, LSMPS_SOLVER_IDRS = 2 };
  * passes, provision, PPE assembly, pressure gradient, shifting, smoothing) pick
  * their lanes per particle by timing each candidate once per neighbour-count
  * histogram of the list they read; every choice computes the same result. */
+/* LSMPS_FLAG_SCHWARZ_FP32: in mixed precision, apply the Schwarz block
+ * inverses from their FP32 values.  By default the application reads FP16
+ * copies with one power-of-two scale per block column (half the bytes of the
+ * dominant preconditioner pass; the inverses themselves are still computed in
+ * FP32).  No effect in FP64 precision. */
 enum {
   LSMPS_FLAG_ASYNC = 1,
   LSMPS_FLAG_KERNEL_TIMING = 2,
   LSMPS_FLAG_GRAPH = 4,
-  LSMPS_FLAG_AUTOTUNE = 8
+  LSMPS_FLAG_AUTOTUNE = 8,
+  LSMPS_FLAG_SCHWARZ_FP32 = 16
 };
 
 /* Kernels the autotuner shapes (lsmps_context_lanes / lsmps_context_force_lanes). */

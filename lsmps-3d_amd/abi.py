@@ -44,6 +44,7 @@ FLAG_ASYNC = 1
 FLAG_KERNEL_TIMING = 2
 FLAG_GRAPH = 4
 FLAG_AUTOTUNE = 8
+FLAG_SCHWARZ_FP32 = 16
 
 # kernels the autotuner shapes (LSMPS_TUNE_*) and their lanes-per-particle candidates
 TUNED_KERNELS = {"scan_count": 0, "scan_write": 1, "provision": 2, "assembly": 3,

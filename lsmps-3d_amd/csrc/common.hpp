@@ -180,6 +180,7 @@ struct Context {
   bool timing() const { return (flags & LSMPS_FLAG_KERNEL_TIMING) != 0; }
   bool graph() const { return (flags & LSMPS_FLAG_GRAPH) != 0; }
   bool autotune() const { return (flags & LSMPS_FLAG_AUTOTUNE) != 0; }
+  bool schwarz_half() const { return (flags & LSMPS_FLAG_SCHWARZ_FP32) == 0; }
   TuneTable tune;
 
   void activate() const { LSMPS_HIP(hipSetDevice(device)); }

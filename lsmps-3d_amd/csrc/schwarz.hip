@@ -438,11 +438,13 @@ __global__ void cover_sort_kernel(size_t n, const int32_t* start, int32_t* entri
 // range [p * 64, p * 64 + 64) and thread t (row t) writes its partial sum to
 // ybuf[(b * kApplyParts + p) * kMaxBlock + t].  The column slices are read
 // coalesced across the rows.
+// With FP16 inverses (V = _Float16) column j holds inv * 2^e_j and the input
+// slice takes the exact factor scale[j] = 2^-e_j instead (scale null: none).
 template <typename T, typename V>
 __global__ void __launch_bounds__(kMaxBlock) apply_kernel(const int32_t* members,
                                                           const int32_t* sizes, const V* inverse,
-                                                          const T* w, double* ybuf,
-                                                          const int32_t* skip) {
+                                                          const float* scale, const T* w,
+                                                          double* ybuf, const int32_t* skip) {
   if (launch_skipped(skip)) return;
   __shared__ double wb[kApplyCols];
   const int t = threadIdx.x;
@@ -454,7 +456,11 @@ __global__ void __launch_bounds__(kMaxBlock) apply_kernel(const int32_t* members
   constexpr int ld = kMaxBlock;
   const int32_t* mem = members + b * kMaxBlock;
   const V* inv = inverse + b * ld * ld;
-  if (t < kApplyCols) wb[t] = j0 + t < j1 ? static_cast<double>(w[mem[j0 + t]]) : 0.0;
+  if (t < kApplyCols) {
+    double x = j0 + t < j1 ? static_cast<double>(w[mem[j0 + t]]) : 0.0;
+    if (scale && j0 + t < j1) x *= static_cast<double>(scale[b * ld + j0 + t]);
+    wb[t] = x;
+  }
   __syncthreads();
   if (t >= n) return;
   double y[4] = {0.0, 0.0, 0.0, 0.0};
@@ -462,9 +468,12 @@ __global__ void __launch_bounds__(kMaxBlock) apply_kernel(const int32_t* members
   for (; j + 4 <= j1; j += 4) {
 #pragma unroll
     for (int u = 0; u < 4; ++u)
-      y[u] += static_cast<double>(inv[static_cast<size_t>(j + u) * ld + t]) * wb[j + u - j0];
+      y[u] += static_cast<double>(static_cast<float>(inv[static_cast<size_t>(j + u) * ld + t])) *
+              wb[j + u - j0];
   }
-  for (; j < j1; ++j) y[0] += static_cast<double>(inv[static_cast<size_t>(j) * ld + t]) * wb[j - j0];
+  for (; j < j1; ++j)
+    y[0] += static_cast<double>(static_cast<float>(inv[static_cast<size_t>(j) * ld + t])) *
+            wb[j - j0];
   ybuf[(b * kApplyParts + part) * kMaxBlock + t] = (y[0] + y[1]) + (y[2] + y[3]);
 }
 
@@ -479,6 +488,30 @@ __global__ void gather_kernel(size_t count, const int32_t* covered, const int32_
   if (k >= count) return;
   const int32_t i = covered[k];
   v[i] = static_cast<T>(schwarz_row_mean(start, entries, ybuf, i));
+}
+
+// FP16 copies of the FP32 inverses: one wavefront per block column c < n;
+// e_c puts the column's largest magnitude m in [2^14, 2^15) (e = 14 - ilogb m),
+// so every entry keeps FP16's 11 significant bits relative to m and nothing
+// overflows; scale[c] = 2^-e_c undoes it exactly in the application.
+__global__ void __launch_bounds__(kBlock) quantize_kernel(const int32_t* sizes, const float* inv32,
+                                                          _Float16* inv16, float* scale) {
+  constexpr int ld = kMaxBlock;
+  constexpr int kWaves = kBlock / 64;
+  const size_t b = blockIdx.x;
+  const int n = sizes[b];
+  const int lane = threadIdx.x & 63;
+  for (int c = threadIdx.x >> 6; c < n; c += kWaves) {
+    const size_t base = (b * ld + c) * ld;
+    float m = 0.0f;
+    for (int r = lane; r < n; r += 64) m = fmaxf(m, fabsf(inv32[base + r]));
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+    const int e = m > 0.0f && isfinite(m) ? 14 - ilogbf(m) : 0;
+    const float up = ldexpf(1.0f, e);
+    for (int r = lane; r < n; r += 64) inv16[base + r] = static_cast<_Float16>(inv32[base + r] * up);
+    if (lane == 0) scale[b * ld + c] = ldexpf(1.0f, -e);
+  }
 }
 
 // pdiag and the flags of covered rows.
@@ -498,12 +531,26 @@ __global__ void compact_kernel(size_t n, const int32_t* flag, const int32_t* pos
 
 }  // namespace
 
+namespace {
+void quantize_blocks(Context& ctx, SchwarzPrec& P) {
+  const size_t dense = P.blocks * kMaxBlock * kMaxBlock;
+  P.inv16.reserve(dense);
+  P.col_scale.reserve(P.blocks * kMaxBlock);
+  quantize_kernel<<<static_cast<unsigned>(P.blocks), kBlock, 0, ctx.stream>>>(
+      P.sizes.data(), P.inv32.data(), reinterpret_cast<_Float16*>(P.inv16.data()),
+      P.col_scale.data());
+  LSMPS_LAUNCH_CHECK();
+}
+}  // namespace
+
 void schwarz_setup(Context& ctx, SchwarzPrec& P, const lsmps_csr& A, const double* diag,
-                   const double* dinv, const double* offabs, bool fp32_inverse) {
+                   const double* dinv, const double* offabs, bool fp32_inverse,
+                   bool half_application) {
   const size_t n = A.rows;
   P.n = n;
   P.blocks = 0;
   P.fp32 = fp32_inverse;
+  P.half = fp32_inverse && half_application;
   P.eligible.reserve(n);
   P.seed_flag.reserve(n);
   P.seed_pos.reserve(n + 1);
@@ -533,6 +580,8 @@ void schwarz_setup(Context& ctx, SchwarzPrec& P, const lsmps_csr& A, const doubl
     gj_inverse_kernel<<<static_cast<unsigned>(nb), kGjThreads, 0, ctx.stream>>>(
         A.row_offsets, A.col_indices, A.values, P.members.data(), P.sizes.data(),
         P.inv32.data());
+    LSMPS_LAUNCH_CHECK();
+    if (P.half) quantize_blocks(ctx, P);
   } else {
     P.scratch.reserve(2 * dense);
     P.inv64.reserve(dense);
@@ -598,11 +647,13 @@ void copy_array(Context& ctx, DeviceArray<T>& dst, const DeviceArray<T>& src, si
 }
 }  // namespace
 
-void schwarz_transpose_of(Context& ctx, SchwarzPrec& P, const SchwarzPrec& src) {
+void schwarz_transpose_of(Context& ctx, SchwarzPrec& P, const SchwarzPrec& src,
+                          bool half_application) {
   const size_t n = src.n, nb = src.blocks;
   P.n = n;
   P.blocks = nb;
   P.fp32 = src.fp32;
+  P.half = src.fp32 && half_application;
   P.covered_count = src.covered_count;
   if (nb == 0) return;
   const size_t dense = nb * kMaxBlock * kMaxBlock;
@@ -618,6 +669,9 @@ void schwarz_transpose_of(Context& ctx, SchwarzPrec& P, const SchwarzPrec& src) 
     P.inv32.reserve(dense);
     transpose_blocks_kernel<float><<<grid, kBlock, 0, ctx.stream>>>(nb, src.inv32.data(),
                                                                      P.inv32.data());
+    LSMPS_LAUNCH_CHECK();
+    // the transposed columns are the source's rows: their scales are new
+    if (P.half) quantize_blocks(ctx, P);
   } else {
     P.inv64.reserve(dense);
     transpose_blocks_kernel<double><<<grid, kBlock, 0, ctx.stream>>>(nb, src.inv64.data(),
@@ -630,14 +684,18 @@ template <typename T>
 void schwarz_products(Context& ctx, const SchwarzPrec& P, const T* v, const int32_t* skip) {
   if (P.blocks == 0) return;
   const unsigned grid = static_cast<unsigned>(P.blocks * kApplyParts);
-  if (P.fp32) {
+  if (P.half) {
+    apply_kernel<T, _Float16><<<grid, kMaxBlock, 0, ctx.stream>>>(
+        P.members.data(), P.sizes.data(), reinterpret_cast<const _Float16*>(P.inv16.data()),
+        P.col_scale.data(), v, P.ybuf.data(), skip);
+  } else if (P.fp32) {
     apply_kernel<T, float><<<grid, kMaxBlock, 0, ctx.stream>>>(P.members.data(), P.sizes.data(),
-                                                               P.inv32.data(), v, P.ybuf.data(),
-                                                               skip);
+                                                               P.inv32.data(), nullptr, v,
+                                                               P.ybuf.data(), skip);
   } else {
     apply_kernel<T, double><<<grid, kMaxBlock, 0, ctx.stream>>>(P.members.data(), P.sizes.data(),
-                                                                P.inv64.data(), v, P.ybuf.data(),
-                                                                skip);
+                                                                P.inv64.data(), nullptr, v,
+                                                                P.ybuf.data(), skip);
   }
   LSMPS_LAUNCH_CHECK();
 }

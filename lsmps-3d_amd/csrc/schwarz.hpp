@@ -19,6 +19,12 @@ struct SchwarzPrec {
   DeviceArray<double> scratch;   // LU + column solves
   DeviceArray<double> inv64;     // blocks x kMaxBlock^2, column-major
   DeviceArray<float> inv32;
+  // mixed precision: FP16 copies of inv32 read by the application, column c
+  // of block b stored as inv32 * 2^e and col_scale[b * kMaxBlock + c] = 2^-e
+  // (the column's largest magnitude lands in [2^14, 2^15))
+  bool half = false;
+  DeviceArray<uint16_t> inv16;
+  DeviceArray<float> col_scale;
   DeviceArray<int32_t> cover;          // per-row counter (setup only)
   DeviceArray<int32_t> cover_start;    // n + 1: row i's slots are cover_entries[start[i], start[i+1])
   DeviceArray<int32_t> cover_entries;  // slots b * kMaxBlock + position, ascending per row
@@ -29,6 +35,7 @@ struct SchwarzPrec {
   size_t bytes() const {
     return eligible.bytes() + seed_flag.bytes() + seed_pos.bytes() + seeds.bytes() +
            members.bytes() + sizes.bytes() + scratch.bytes() + inv64.bytes() + inv32.bytes() +
+           inv16.bytes() + col_scale.bytes() +
            cover.bytes() + cover_start.bytes() + cover_entries.bytes() + covered.bytes() +
            pdiag.bytes() + ybuf.bytes();
   }
@@ -52,13 +59,17 @@ __device__ __forceinline__ double schwarz_row_mean(const int32_t* start, const i
 
 // Builds the blocks of matrix A from its per-row statistics: diagonal A_ii,
 // Jacobi diagonal 1 / A_ii and the sum of the off-diagonal magnitudes.
+// fp32_inverse: mixed precision (FP32 inverses; FP16 application copies when
+// half_application).
 void schwarz_setup(Context& ctx, SchwarzPrec& P, const lsmps_csr& A, const double* diag,
-                   const double* dinv, const double* offabs, bool fp32_inverse);
+                   const double* dinv, const double* offabs, bool fp32_inverse,
+                   bool half_application);
 // The blocks of A's preconditioner for A^T: the same member sets, whose dense
 // blocks of A^T are the transposes of A's, so their inverses are the
 // transposed inverses of src (restricted to the same covered rows; the
 // diagonal part 1 / A_ii is shared).
-void schwarz_transpose_of(Context& ctx, SchwarzPrec& P, const SchwarzPrec& src);
+void schwarz_transpose_of(Context& ctx, SchwarzPrec& P, const SchwarzPrec& src,
+                          bool half_application);
 // In-place completion of a preconditioned vector: on entry v = pdiag * w (w on
 // covered rows, Jacobi values elsewhere); on exit each covered row holds the mean
 // of its blocks' values (inv_b w_b), summed in block order.

@@ -957,19 +957,23 @@ __global__ void __launch_bounds__(kSellBlock) sell_spmv_kernel(SellArgs<T, V> a)
   }
 }
 
-// r = b - A x in FP64; optional FP32 copy of r; partial r.r.
+// r = b - A x in FP64; optional FP32 copy of r; partial r.r.  Batched systems
+// (row_seg non-null): only the rows of segments k with take[k] != 0.
 __global__ void __launch_bounds__(kBlock) residual_kernel(size_t n, const int32_t* rows,
                                                           const int32_t* cols, const double* vals,
                                                           const double* x, const double* b,
                                                           double* r, float* r32,
                                                           double* partials,
-                                                          const int32_t* skip = nullptr) {
+                                                          const int32_t* skip = nullptr,
+                                                          const int32_t* row_seg = nullptr,
+                                                          const int32_t* take = nullptr) {
   if (launch_skipped(skip)) return;
   const int lane = threadIdx.x & (kLanes - 1);
   const size_t group = (static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x) / kLanes;
   const size_t ngroups = static_cast<size_t>(gridDim.x) * blockDim.x / kLanes;
   double acc[1] = {0.0};
   for (size_t row = group; row < n; row += ngroups) {
+    if (row_seg && !take[row_seg[row]]) continue;  // uniform per lane group
     double sum = 0.0;
     for (int32_t c = rows[row] + lane; c < rows[row + 1]; c += kLanes) sum += vals[c] * x[cols[c]];
 #pragma unroll
@@ -1818,12 +1822,13 @@ void append_sell_signature(std::vector<uintptr_t>& sig, const SellMatrix& M) {
 
 void append_schwarz_signature(std::vector<uintptr_t>& sig, const SchwarzPrec& P) {
   const void* p[] = {P.members.data(), P.sizes.data(), P.inv32.data(), P.inv64.data(),
-                     P.ybuf.data(), P.covered.data(), P.cover_start.data(),
-                     P.cover_entries.data(), P.pdiag.data()};
+                     P.inv16.data(), P.col_scale.data(), P.ybuf.data(), P.covered.data(),
+                     P.cover_start.data(), P.cover_entries.data(), P.pdiag.data()};
   for (const void* q : p) sig.push_back(reinterpret_cast<uintptr_t>(q));
   sig.push_back(P.blocks);
   sig.push_back(P.covered_count);
   sig.push_back(P.fp32 ? 1 : 0);
+  sig.push_back(P.half ? 1 : 0);
 }
 
 void append_coarse_signature(std::vector<uintptr_t>& sig, const CoarseSpace& C) {
@@ -2179,7 +2184,7 @@ void idr_solve_segmented(Context& ctx, PpeSolver& S, size_t n, const SellMatrix&
   const unsigned fb = static_cast<unsigned>((nseg + kBlock - 1) / kBlock);
   double* part = S.part_a.data();
   S.seg_status.reserve(2 * static_cast<size_t>(nseg));
-  std::vector<int32_t> flags(nseg, 0);
+  std::vector<int32_t> flags(nseg, 0), replace(nseg, 0);
   DeviceArray<int32_t>& dflags = S.seg_flags;
   dflags.reserve(nseg);
   auto read_status = [&](std::vector<double>& rr, std::vector<int>& brk) {
@@ -2288,20 +2293,30 @@ void idr_solve_segmented(Context& ctx, PpeSolver& S, size_t n, const SellMatrix&
     ++it;
     read_status(rr, brk);
     add_timing();
+    // reliable updates of the segments whose residual met the target or fell
+    // by the replacement factor (the other segments' rows are not touched)
     bool need_true = false;
     for (int k = 0; k < nseg; ++k) {
+      replace[k] = 0;
       if (done[k]) continue;
       if (!std::isfinite(rr[k])) raise(LSMPS_ERR_NUMERICAL, "PPE solve produced a non-finite residual");
-      if (rr[k] <= target2[k] || rr[k] <= factor2 * rr_rep[k]) need_true = true;
+      if (rr[k] <= target2[k] || rr[k] <= factor2 * rr_rep[k]) {
+        replace[k] = 1;
+        need_true = true;
+      }
     }
     if (need_true) {
+      LSMPS_HIP(hipMemcpyAsync(dflags.data(), replace.data(), nseg * sizeof(int32_t),
+                               hipMemcpyHostToDevice, ctx.stream));
       const unsigned nb = spmv_blocks(n);
       residual_kernel<<<nb, kBlock, 0, ctx.stream>>>(n, A64.row_offsets, A64.col_indices,
                                                      A64.values, x, b64, r, nullptr,
-                                                     S.part_b.data());
+                                                     S.part_b.data(), nullptr, g.row_seg,
+                                                     dflags.data());
       segment_norms();
       read_status(rr, brk);
-      for (int k = 0; k < nseg; ++k) rr_rep[k] = rr[k];
+      for (int k = 0; k < nseg; ++k)
+        if (replace[k]) rr_rep[k] = rr[k];
       ++replacements;
     }
     bool restart = false;
@@ -2458,10 +2473,10 @@ void ppe_solve(Context& ctx, PpeSolver& S, const lsmps_csr& A, const double* b, 
     if (opt.preconditioner == LSMPS_PRECOND_SCHWARZ ||
         opt.preconditioner == LSMPS_PRECOND_SCHWARZ_COARSE) {
       if (S.schwarz_transpose_source && S.schwarz_transpose_source->n == n) {
-        schwarz_transpose_of(ctx, S.schwarz, *S.schwarz_transpose_source);
+        schwarz_transpose_of(ctx, S.schwarz, *S.schwarz_transpose_source, ctx.schwarz_half());
       } else {
         schwarz_setup(ctx, S.schwarz, A, S.diag64.data(), S.dinv64.data(), S.offabs.data(),
-                      ctx.precision == LSMPS_PRECISION_MIXED);
+                      ctx.precision == LSMPS_PRECISION_MIXED, ctx.schwarz_half());
       }
       int64_t blocks = 0, rows = 0;
       schwarz_stats(ctx, S.schwarz, &blocks, &rows);
