@@ -348,6 +348,12 @@ int lsmps_context_lanes(lsmps_context* ctx, int32_t kernel, int32_t* lanes,
  * count pass 1/2/4, fill pass 4/8/16, assembly 4/8, the others 4/8/16); 0
  * returns it to the default or the autotuner.  Results do not depend on it. */
 int lsmps_context_force_lanes(lsmps_context* ctx, int32_t kernel, int32_t lanes);
+/* No reference counterpart (the reference hands its PPE to AMGX).  Inverse of
+ * a dense n x n row-major FP64 matrix (device pointers, 1 <= n <= 4096) by the
+ * kernels that invert the PPE's coarse-space matrix: blocked Gauss-Jordan with
+ * partial pivoting, rank-b updates on the FP64 matrix cores.  Lets a caller
+ * check those kernels directly.  LSMPS_ERR_NUMERICAL on a zero pivot. */
+int lsmps_dense_inverse(lsmps_context* ctx, int32_t n, const double* matrix, double* inverse);
 /* PPE SpMV statistics gathered under LSMPS_FLAG_KERNEL_TIMING since the last
  * reset: launches, summed duration (ms) and summed algorithmic bytes
  * (values + column indices + row pointers, each vector element read or written

@@ -147,6 +147,12 @@ int lsmps_context_force_lanes(lsmps_context* ctx, int32_t kernel, int32_t lanes)
   API_END("lsmps_context_force_lanes")
 }
 
+int lsmps_dense_inverse(lsmps_context* ctx, int32_t n, const double* matrix, double* inverse) {
+  API_BEGIN
+  dense_inverse(C(ctx), n, matrix, inverse);
+  API_END("lsmps_dense_inverse")
+}
+
 int lsmps_context_synchronize(lsmps_context* ctx) {
   API_BEGIN
   C(ctx).sync();

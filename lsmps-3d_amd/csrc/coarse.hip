@@ -337,22 +337,32 @@ __global__ void __launch_bounds__(kBlock) tank_convert_kernel(const int32_t* agg
 // other row r, with f = a_rk before the step: a_rk := -f inv and
 // a_rj -= f a_kj (j != k).
 //
-// The steps run in panels of b columns [k0, k0 + b): gj_panel_kernel (one
-// block, a row's panel entries in its thread's registers) applies the b steps
-// to the panel's columns and records, per step, the pivot row, inv and the
-// column f; gj_update_kernel then applies the same swaps, scalings and
-// eliminations, in the same order and with the same arithmetic, to every other
-// column.  So the result equals the unblocked elimination's bit for bit, with
-// two launches per panel instead of two per column.
+// On the columns a step does not eliminate it is A := (I + (z - e_k) e_k^T) A,
+// z being column k after the step (z_k = inv, z_r = -f_r inv).  b consecutive
+// steps compose to A := (I + (W - E) E^T) A, where W holds the panel's b
+// columns after its b steps and E the identity's columns k0 .. k0 + b - 1.
+// So the steps run in panels of b columns:
+//  1. gj_panel_kernel (one workgroup, the panel's rows in registers) applies
+//     the b steps to the panel's columns and records their pivot rows;
+//  2. gj_swap_kernel applies the b row interchanges to every other column and
+//     copies the interchanged panel rows R = A[k0 .. k0 + b) of those columns
+//     aside;
+//  3. gj_rank_kernel adds (W - E) R to every other column: a rank-b update,
+//     count x b times b x count, on the FP64 matrix cores
+//     (v_mfma_f64_16x16x4f64, one wavefront per 16 x 16 tile).
+// Row k0 + i of the update is W[k0 + i] R (its identity term cancels the
+// interchanged row), every other row r adds W[r] R; each sum runs in the
+// matrix cores' fixed order, so the inverse is reproducible run to run.
 constexpr int kPanelThreads = 1024;
 constexpr int kPanelWaves = kPanelThreads / 64;
 static_assert(kPanelWaves <= 64 && (kPanelWaves & (kPanelWaves - 1)) == 0, "one partial per lane");
-constexpr int kPanelLds = 7680;   // doubles of the update's f columns (b x count)
-constexpr int kMaxPanel = 8;
-constexpr int kUpdateRows = 32;   // column entries per thread in the update
 static_assert(kMaxCoarse <= 4 * kPanelThreads, "panel rows per thread");
+constexpr int kMaxPanel = 16;  // widest panel (the R buffer holds kMaxPanel rows)
 
-int panel_width(int count) { return std::max(1, std::min(kMaxPanel, kPanelLds / count)); }
+// Panel width: at most 64 register doubles per thread (rows per thread x
+// width) and 16 unrolled steps (a 32-step unroll is not taken by the compiler,
+// which then keeps the panel in scratch memory).
+int panel_width(int count) { return count <= 2 * kPanelThreads ? 16 : 8; }
 
 __device__ __forceinline__ void better_pivot(double& v, int& i, double v2, int i2) {
   if (v2 > v || (v2 == v && i2 < i)) {
@@ -361,224 +371,215 @@ __device__ __forceinline__ void better_pivot(double& v, int& i, double v2, int i
   }
 }
 
-// Entry c of a register row (unrolled selects: no dynamic register indexing).
-__device__ __forceinline__ double pick(const double (&row)[kMaxPanel], int c) {
-  double x = row[0];
-#pragma unroll
-  for (int i = 1; i < kMaxPanel; ++i) x = c == i ? row[i] : x;
-  return x;
-}
-
-// panel: steps k0 .. k0 + b - 1 on columns [k0, k0 + b); fcol (b x count) and
-// pinv (b) receive f and inv of each step, perm[k] the pivot row.  Thread t
-// holds rows t, t + kPanelThreads, ... (kPanelRows of them, covering count).
-template <int kPanelRows>
-__global__ void __launch_bounds__(kPanelThreads) gj_panel_kernel(double* a, int count, int k0,
-                                                                 int b, double* fcol,
-                                                                 double* pinv, int32_t* perm,
-                                                                 int32_t* status) {
-  __shared__ double wv[kPanelWaves];
-  __shared__ int wi[kPanelWaves];
-  __shared__ double xk[kMaxPanel], xp[kMaxPanel], rk[kMaxPanel];
-  if (status[0]) return;  // an earlier panel met a zero pivot
+// The panel's w <= B steps on its columns [k0, k0 + w); perm[k] receives the
+// pivot row of step k.  Thread t of T holds rows t, t + T, ... (R of
+// them, covering count).  Per step the pivot partials, the old row k and the
+// new row k (row p scaled by its owner) pass through LDS, double-buffered by
+// step parity: two barriers per step.  The step loop is unrolled, so that
+// every register index is a constant (a run-time column index would send the
+// panel to scratch memory).
+template <int R, int B, int T>
+__global__ void __launch_bounds__(T) gj_panel_kernel(double* a, int count, int k0, int w,
+                                                     int32_t* perm, int32_t* status) {
+  constexpr int kWavesT = T / 64;
+  static_assert(kWavesT <= 64 && (kWavesT & (kWavesT - 1)) == 0, "one partial per lane");
+  __shared__ double wv[2][kWavesT];
+  __shared__ int wi[2][kWavesT];
+  __shared__ double xk[2][B], xr[2][B];
+  if (status[0]) return;  // an earlier panel met a zero pivot (uniform)
   const int t = threadIdx.x;
-  double P[kPanelRows][kMaxPanel];
+  double P[R][B];
 #pragma unroll
-  for (int q = 0; q < kPanelRows; ++q) {
-    const int r = t + q * kPanelThreads;
+  for (int q = 0; q < R; ++q) {
+    const int r = t + q * T;
 #pragma unroll
-    for (int c = 0; c < kMaxPanel; ++c)
-      P[q][c] = r < count && c < b ? a[static_cast<size_t>(r) * count + k0 + c] : 0.0;
+    for (int c = 0; c < B; ++c)
+      P[q][c] = r < count && c < w ? a[static_cast<size_t>(r) * count + k0 + c] : 0.0;
   }
-  for (int s = 0; s < b; ++s) {
-    const int k = k0 + s;
-    double best = -1.0;
-    int bidx = count;
 #pragma unroll
-    for (int q = 0; q < kPanelRows; ++q) {
-      const int r = t + q * kPanelThreads;
-      const double v = fabs(pick(P[q], s));
-      if (r >= k && r < count && v > best) {
-        best = v;
-        bidx = r;
+  for (int s = 0; s < B; ++s) {
+    if (s < w) {  // uniform; a constant trip count lets the loop unroll
+      const int k = k0 + s;
+      const int buf = s & 1;
+      double best = -1.0;
+      int bidx = count;
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        const int r = t + q * T;
+        const double v = fabs(P[q][s]);
+        if (r >= k && r < count && v > best) {
+          best = v;
+          bidx = r;
+        }
       }
-    }
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      const double v2 = __shfl_xor(best, off, 64);
-      const int i2 = __shfl_xor(bidx, off, 64);
-      better_pivot(best, bidx, v2, i2);
-    }
-    if ((t & 63) == 0) {
-      wv[t >> 6] = best;
-      wi[t >> 6] = bidx;
-    }
-    __syncthreads();
-    {  // every wavefront reduces the kPanelWaves partials the same way
-      const int l = t & 63;
-      best = l < kPanelWaves ? wv[l] : -1.0;
-      bidx = l < kPanelWaves ? wi[l] : count;
-#pragma unroll
-      for (int off = kPanelWaves / 2; off > 0; off >>= 1) {
+      for (int off = 32; off > 0; off >>= 1) {
         const double v2 = __shfl_xor(best, off, 64);
         const int i2 = __shfl_xor(bidx, off, 64);
         better_pivot(best, bidx, v2, i2);
       }
-      best = __shfl(best, 0, 64);
-      bidx = __shfl(bidx, 0, 64);
-    }
-    const int p = bidx;
-    const bool singular = !(best > 0.0) || !isfinite(best);
-    if (t == 0) {
-      perm[k] = p < count ? p : k;
-      if (singular) status[0] = 1;
-    }
-    if (singular) return;  // uniform: every thread read the same reduction
-    // rows k and p through LDS
-#pragma unroll
-    for (int q = 0; q < kPanelRows; ++q) {
-      const int r = t + q * kPanelThreads;
-      if (r == k) {
-#pragma unroll
-        for (int c = 0; c < kMaxPanel; ++c) xk[c] = P[q][c];
+      if ((t & 63) == 0) {
+        wv[buf][t >> 6] = best;
+        wi[buf][t >> 6] = bidx;
       }
-      if (r == p) {
+      __syncthreads();
+      {  // every wavefront reduces the kWavesT partials the same way
+        const int l = t & 63;
+        best = l < kWavesT ? wv[buf][l] : -1.0;
+        bidx = l < kWavesT ? wi[buf][l] : count;
 #pragma unroll
-        for (int c = 0; c < kMaxPanel; ++c) xp[c] = P[q][c];
+        for (int off = kWavesT / 2; off > 0; off >>= 1) {
+          const double v2 = __shfl_xor(best, off, 64);
+          const int i2 = __shfl_xor(bidx, off, 64);
+          better_pivot(best, bidx, v2, i2);
+        }
+        best = __shfl(best, 0, 64);
+        bidx = __shfl(bidx, 0, 64);
       }
-    }
-    __syncthreads();
-    const double inv = 1.0 / xp[s];  // the pivot, now at row k
-#pragma unroll
-    for (int q = 0; q < kPanelRows; ++q) {
-      const int r = t + q * kPanelThreads;
-      if (r == p) {
-#pragma unroll
-        for (int c = 0; c < kMaxPanel; ++c) P[q][c] = xk[c];
+      const bool singular = !(best > 0.0) || !isfinite(best) || bidx >= count;
+      if (singular) {
+        if (t == 0) {
+          perm[k] = k;
+          status[0] = 1;
+        }
+        return;  // uniform: every thread reduced the same partials
       }
-      if (r == k) {
+      const int p = bidx;
+      if (t == 0) perm[k] = p;
 #pragma unroll
-        for (int c = 0; c < kMaxPanel; ++c) {
-          P[q][c] = (c == s ? 1.0 : xp[c]) * inv;
-          rk[c] = P[q][c];
+      for (int q = 0; q < R; ++q) {
+        const int r = t + q * T;
+        if (r == k) {
+#pragma unroll
+          for (int c = 0; c < B; ++c) xk[buf][c] = P[q][c];
+        }
+        if (r == p) {
+          const double inv = 1.0 / P[q][s];
+#pragma unroll
+          for (int c = 0; c < B; ++c) xr[buf][c] = (c == s ? 1.0 : P[q][c]) * inv;
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        const int r = t + q * T;
+        if (r >= count) continue;
+        if (r == k) {
+#pragma unroll
+          for (int c = 0; c < B; ++c) P[q][c] = xr[buf][c];
+          continue;
+        }
+        const bool is_p = r == p;  // row p now holds the old row k
+        const double f = is_p ? xk[buf][s] : P[q][s];
+#pragma unroll
+        for (int c = 0; c < B; ++c) {
+          const double src = is_p ? xk[buf][c] : P[q][c];
+          P[q][c] = (c == s ? 0.0 : src) - f * xr[buf][c];
         }
       }
     }
-    if (t == 0) pinv[s] = inv;
-    __syncthreads();
-    double kr[kMaxPanel];
-#pragma unroll
-    for (int c = 0; c < kMaxPanel; ++c) kr[c] = rk[c];
-    const double ks = rk[s];
-#pragma unroll
-    for (int q = 0; q < kPanelRows; ++q) {
-      const int r = t + q * kPanelThreads;
-      if (r >= count) continue;
-      if (r == k) {
-        fcol[static_cast<size_t>(s) * count + r] = 0.0;
-        continue;
-      }
-      const double f = pick(P[q], s);
-      fcol[static_cast<size_t>(s) * count + r] = f;
-#pragma unroll
-      for (int c = 0; c < kMaxPanel; ++c) P[q][c] = c == s ? -f * ks : P[q][c] - f * kr[c];
-    }
-    __syncthreads();  // xk / xp / rk / wv are rewritten by the next step
   }
 #pragma unroll
-  for (int q = 0; q < kPanelRows; ++q) {
-    const int r = t + q * kPanelThreads;
+  for (int q = 0; q < R; ++q) {
+    const int r = t + q * T;
     if (r >= count) continue;
 #pragma unroll
-    for (int c = 0; c < kMaxPanel; ++c)
-      if (c < b) a[static_cast<size_t>(r) * count + k0 + c] = P[q][c];
+    for (int c = 0; c < B; ++c)
+      if (c < w) a[static_cast<size_t>(r) * count + k0 + c] = P[q][c];
   }
 }
 
-// The panel's steps on every column outside it.  A block takes tj adjacent
-// columns; its threads hold rows g, g + G, ... (G = kBlock / tj) of one column
-// in registers for all b steps, with the panel's f columns in LDS.
-__global__ void __launch_bounds__(kBlock) gj_update_kernel(double* a, int count, int k0, int b,
-                                                           int tj, const double* fcol,
-                                                           const double* pinv,
-                                                           const int32_t* perm,
-                                                           const int32_t* status) {
-  __shared__ double F[kPanelLds];
-  __shared__ double xk[64], xp[64], rk[64];
-  if (status[0]) return;  // uniform: written before this launch
-  const int t = threadIdx.x;
-  for (int e = t; e < b * count; e += kBlock) F[e] = fcol[e];
-  const int cj = t % tj;
-  const int g = t / tj;
-  const int G = kBlock / tj;
-  const int u = static_cast<int>(blockIdx.x) * tj + cj;  // column among the count - b others
-  const bool live = u < count - b;
-  const int j = u < k0 ? u : u + b;
-  double v[kUpdateRows];
-#pragma unroll
-  for (int q = 0; q < kUpdateRows; ++q) {
-    const int r = g + q * G;
-    v[q] = live && r < count ? a[static_cast<size_t>(r) * count + j] : 0.0;
-  }
-  __syncthreads();
-  for (int s = 0; s < b; ++s) {
+// The panel's row interchanges on every column outside it (one thread per
+// column, the interchanges in step order), then its rows of those columns
+// copied to panel_rows (w x count).
+__global__ void __launch_bounds__(kBlock) gj_swap_kernel(double* a, int count, int k0, int w,
+                                                         const int32_t* perm,
+                                                         const int32_t* status,
+                                                         double* panel_rows) {
+  if (status[0]) return;
+  const int j = static_cast<int>(blockIdx.x) * kBlock + threadIdx.x;
+  if (j >= count || (j >= k0 && j < k0 + w)) return;
+  for (int s = 0; s < w; ++s) {
     const int k = k0 + s;
     const int p = perm[k];
-    const double inv = pinv[s];
-    // swap rows k and p of this column, then scale row k
-#pragma unroll
-    for (int q = 0; q < kUpdateRows; ++q) {
-      const int r = g + q * G;
-      if (r == k) xk[cj] = v[q];
-      if (r == p) xp[cj] = v[q];
+    if (p != k) {
+      const double x = a[static_cast<size_t>(k) * count + j];
+      a[static_cast<size_t>(k) * count + j] = a[static_cast<size_t>(p) * count + j];
+      a[static_cast<size_t>(p) * count + j] = x;
     }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < kUpdateRows; ++q) {
-      const int r = g + q * G;
-      if (r == p) v[q] = xk[cj];
-      if (r == k) {
-        v[q] = xp[cj] * inv;
-        rk[cj] = v[q];
-      }
-    }
-    __syncthreads();
-    const double pk = rk[cj];
-    const double* f = F + s * count;
-#pragma unroll
-    for (int q = 0; q < kUpdateRows; ++q) {
-      const int r = g + q * G;
-      if (r < count && r != k) v[q] = v[q] - f[r] * pk;
-    }
-    __syncthreads();  // xk / xp / rk are rewritten by the next step
   }
-  if (!live) return;
-#pragma unroll
-  for (int q = 0; q < kUpdateRows; ++q) {
-    const int r = g + q * G;
-    if (r < count) a[static_cast<size_t>(r) * count + j] = v[q];
-  }
+  for (int s = 0; s < w; ++s)
+    panel_rows[static_cast<size_t>(s) * count + j] = a[static_cast<size_t>(k0 + s) * count + j];
 }
 
-// Columns per update block: the largest power of two tj <= 64 whose rows fit
-// kUpdateRows per thread.
-int update_columns(int count) {
-  int tj = 64;
-  while (tj > 1 && (count + kBlock / tj - 1) / (kBlock / tj) > kUpdateRows) tj >>= 1;
-  return tj;
+typedef double double4_t __attribute__((ext_vector_type(4)));
+
+// a[:, j] += (W - E) R[:, j] for the columns j outside the panel.  Wavefront
+// per 16 x 16 tile (rows i0.., columns j0..); per K = 4 slice of the panel
+// lane l supplies (W - E)[i0 + l % 16][kk + l / 16] and R[kk + l / 16][j0 + l % 16]
+// and accumulates rows i0 + l / 16 + 4 v (v < 4) of column j0 + l % 16 (the
+// instruction's result layout, checked on the device).
+__global__ void __launch_bounds__(kBlock) gj_rank_kernel(double* a, int count, int k0, int w,
+                                                         const double* panel_rows,
+                                                         const int32_t* status) {
+  if (status[0]) return;
+  const int tiles = (count + 15) / 16;
+  const int tile = static_cast<int>(blockIdx.x) * kWaves + (threadIdx.x >> 6);
+  if (tile >= tiles * tiles) return;  // uniform per wavefront
+  const int i0 = (tile / tiles) * 16, j0 = (tile % tiles) * 16;
+  if (j0 >= k0 && j0 + 16 <= k0 + w) return;  // every column in the panel
+  const int lane = threadIdx.x & 63;
+  const int li = lane & 15, lk = lane >> 4;
+  const int jc = j0 + li;
+  const bool col_ok = jc < count && !(jc >= k0 && jc < k0 + w);
+  double4_t acc;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const int i = i0 + lk + 4 * v;
+    acc[v] = col_ok && i < count ? a[static_cast<size_t>(i) * count + jc] : 0.0;
+  }
+  const int ia = i0 + li;
+  for (int kk = 0; kk < w; kk += 4) {
+    const int ka = kk + lk;
+    double av = 0.0, bv = 0.0;
+    if (ia < count && ka < w)
+      av = a[static_cast<size_t>(ia) * count + k0 + ka] - (ia == k0 + ka ? 1.0 : 0.0);
+    if (col_ok && ka < w) bv = panel_rows[static_cast<size_t>(ka) * count + jc];
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+  }
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const int i = i0 + lk + 4 * v;
+    if (col_ok && i < count) a[static_cast<size_t>(i) * count + jc] = acc[v];
+  }
 }
 
 // The row swaps of the pivoting become column swaps of the inverse, applied in
 // reverse step order: out[:, j] = a[:, idx[j]] with idx the composed order.
-__global__ void gj_order_kernel(int count, const int32_t* perm, int32_t* idx) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  for (int j = 0; j < count; ++j) idx[j] = j;
-  for (int k = count - 1; k >= 0; --k) {
-    const int p = perm[k];
-    const int32_t tmp = idx[k];
-    idx[k] = idx[p];
-    idx[p] = tmp;
+// After a zero pivot (status set) the later steps never ran and their perm
+// entries were never written: the order is then left as the identity (the
+// inverse is unusable anyway and its products are zero).
+// (One workgroup; the order is composed in LDS by one thread.)
+__global__ void __launch_bounds__(kBlock) gj_order_kernel(int count, const int32_t* perm,
+                                                          const int32_t* status, int32_t* idx) {
+  __shared__ int32_t pv[kMaxCoarse], order[kMaxCoarse];
+  const bool valid = status[0] == 0;  // uniform
+  for (int j = threadIdx.x; j < count; j += kBlock) {
+    pv[j] = valid ? perm[j] : j;
+    order[j] = j;
   }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = count - 1; k >= 0; --k) {
+      const int p = pv[k];
+      if (p < k || p >= count) continue;  // a pivot row is never above its step
+      const int32_t tmp = order[k];
+      order[k] = order[p];
+      order[p] = tmp;
+    }
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < count; j += kBlock) idx[j] = order[j];
 }
 
 __global__ void __launch_bounds__(kBlock) gj_permute_kernel(const double* a, int count,
@@ -790,6 +791,37 @@ __global__ void __launch_bounds__(kBlock) prolong_add_kernel(size_t n, const int
 }
 
 unsigned wave_blocks(size_t items) { return static_cast<unsigned>((items + kWaves - 1) / kWaves); }
+
+// The blocked Gauss-Jordan elimination of the count x count row-major matrix a
+// in place (panel_rows: kMaxPanel x count doubles; perm: count; status[0] set
+// on a zero pivot, after which the remaining launches return at once).
+void gauss_jordan(double* a, int count, double* panel_rows, int32_t* perm, int32_t* status,
+                  hipStream_t st) {
+  const int b = panel_width(count);
+  const int tiles = (count + 15) / 16;
+  const unsigned rank_blocks =
+      static_cast<unsigned>((static_cast<size_t>(tiles) * tiles + kWaves - 1) / kWaves);
+  for (int k0 = 0; k0 < count; k0 += b) {
+    const int w = std::min(b, count - k0);
+    // (four waves with four rows a thread measured no faster at 729 rows)
+    if (count <= kPanelThreads) {
+      gj_panel_kernel<1, 16, kPanelThreads><<<1, kPanelThreads, 0, st>>>(a, count, k0, w, perm,
+                                                                          status);
+    } else if (count <= 2 * kPanelThreads) {
+      gj_panel_kernel<2, 16, kPanelThreads><<<1, kPanelThreads, 0, st>>>(a, count, k0, w, perm,
+                                                                          status);
+    } else {
+      gj_panel_kernel<4, 8, kPanelThreads><<<1, kPanelThreads, 0, st>>>(a, count, k0, w, perm,
+                                                                         status);
+    }
+    if (count > w) {
+      gj_swap_kernel<<<blocks_for(count), kBlock, 0, st>>>(a, count, k0, w, perm, status,
+                                                          panel_rows);
+      gj_rank_kernel<<<rank_blocks, kBlock, 0, st>>>(a, count, k0, w, panel_rows, status);
+    }
+  }
+  LSMPS_LAUNCH_CHECK();
+}
 
 }  // namespace
 
@@ -1025,33 +1057,38 @@ void coarse_setup(Context& ctx, CoarseSpace& C, const lsmps_csr& A, const double
                                                                    C.work.data());
   LSMPS_LAUNCH_CHECK();
   const unsigned eb = blocks_for(nn);
-  const int b = panel_width(count);
-  const int tj = update_columns(count);
-  double* pinv = C.colk.data() + static_cast<size_t>(kMaxPanel) * count;
-  for (int k0 = 0; k0 < count; k0 += b) {
-    const int w = std::min(b, count - k0);
-    if (count <= kPanelThreads) {
-      gj_panel_kernel<1><<<1, kPanelThreads, 0, st>>>(C.work.data(), count, k0, w, C.colk.data(),
-                                                      pinv, C.perm.data(), C.status.data());
-    } else if (count <= 2 * kPanelThreads) {
-      gj_panel_kernel<2><<<1, kPanelThreads, 0, st>>>(C.work.data(), count, k0, w, C.colk.data(),
-                                                      pinv, C.perm.data(), C.status.data());
-    } else {
-      gj_panel_kernel<4><<<1, kPanelThreads, 0, st>>>(C.work.data(), count, k0, w, C.colk.data(),
-                                                      pinv, C.perm.data(), C.status.data());
-    }
-    if (count > w) {
-      gj_update_kernel<<<static_cast<unsigned>((count - w + tj - 1) / tj), kBlock, 0, st>>>(
-          C.work.data(), count, k0, w, tj, C.colk.data(), pinv, C.perm.data(), C.status.data());
-    }
-  }
+  gauss_jordan(C.work.data(), count, C.colk.data(), C.perm.data(), C.status.data(), st);
   LSMPS_LAUNCH_CHECK();
   // idx reuses colk's storage as int32 (count entries fit in count doubles)
   int32_t* idx = reinterpret_cast<int32_t*>(C.colk.data());
-  gj_order_kernel<<<1, 1, 0, st>>>(count, C.perm.data(), idx);
+  gj_order_kernel<<<1, kBlock, 0, st>>>(count, C.perm.data(), C.status.data(), idx);
   gj_permute_kernel<<<eb, kBlock, 0, st>>>(C.work.data(), count, idx, C.ainv.data());
   LSMPS_LAUNCH_CHECK();
   C.ready = true;  // a zero pivot is reported in C.status (the product then yields 0)
+}
+
+void dense_inverse(Context& ctx, int n, const double* matrix, double* inverse) {
+  require(n >= 1 && n <= kMaxCoarse, "dense inverse size must be in 1..4096");
+  require(matrix && inverse, "dense inverse needs device matrix and output pointers");
+  const size_t nn = static_cast<size_t>(n) * n;
+  DeviceArray<double> work, panel_rows;
+  DeviceArray<int32_t> perm, status;
+  work.reserve(nn);
+  panel_rows.reserve(static_cast<size_t>(kMaxPanel) * n + kMaxPanel);
+  perm.reserve(static_cast<size_t>(n));
+  status.reserve(1);
+  LSMPS_HIP(hipMemcpyAsync(work.data(), matrix, nn * sizeof(double), hipMemcpyDeviceToDevice,
+                           ctx.stream));
+  fill_i32<<<1, 1, 0, ctx.stream>>>(status.data(), 1, 0);
+  gauss_jordan(work.data(), n, panel_rows.data(), perm.data(), status.data(), ctx.stream);
+  int32_t* idx = reinterpret_cast<int32_t*>(panel_rows.data());
+  gj_order_kernel<<<1, kBlock, 0, ctx.stream>>>(n, perm.data(), status.data(), idx);
+  gj_permute_kernel<<<blocks_for(nn), kBlock, 0, ctx.stream>>>(work.data(), n, idx, inverse);
+  LSMPS_LAUNCH_CHECK();
+  LSMPS_HIP(hipMemcpyAsync(ctx.host_i32, status.data(), sizeof(int32_t), hipMemcpyDeviceToHost,
+                           ctx.stream));
+  ctx.sync();
+  if (ctx.host_i32[0]) raise(LSMPS_ERR_NUMERICAL, "dense inverse: zero pivot");
 }
 
 void coarse_set_weights(Context& ctx, CoarseSpace& C, const double* pdiag) {

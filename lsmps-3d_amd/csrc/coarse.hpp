@@ -123,6 +123,10 @@ void coarse_set_weights(Context& ctx, CoarseSpace& C, const double* pdiag);
 // met a zero pivot gets no correction).
 bool coarse_usable(Context& ctx, CoarseSpace& C);
 
+// inverse = matrix^-1 (n x n, row-major, device memory) by the coarse matrix's
+// Gauss-Jordan kernels; raises LSMPS_ERR_NUMERICAL on a zero pivot.
+void dense_inverse(Context& ctx, int n, const double* matrix, double* inverse);
+
 // yc = Ac^-1 P^T (wfac * w) (wfac may be skipped: then yc = Ac^-1 P^T w).
 template <typename W>
 void coarse_solve(Context& ctx, CoarseSpace& C, const W* w, bool scale_by_wfac,

@@ -40,6 +40,7 @@ EXPORTS = [
     "lsmps_vtk_path", "lsmps_write_vtk", "lsmps_compute_virtual_light_diagnostics",
     "lsmps_ppe_assemble_views", "lsmps_compute_pressure_gradient_view",
     "lsmps_build_neighbor_lists_noalloc", "lsmps_context_lanes", "lsmps_context_force_lanes",
+    "lsmps_dense_inverse",
 ]
 
 _LIB = None
@@ -335,6 +336,18 @@ class Context:
     def force_lanes(self, kernel, lanes):
         check(self.lib.lsmps_context_force_lanes(self.handle, abi.TUNED_KERNELS[kernel], lanes),
               "context_force_lanes")
+
+    def dense_inverse(self, matrix):
+        """Inverse of a square float64 torch tensor on the GPU (row-major), by
+        the coarse-space Gauss-Jordan kernels."""
+        import torch
+        a = matrix.to(torch.float64).contiguous()
+        n = a.shape[0]
+        assert a.dim() == 2 and a.shape[1] == n and a.is_cuda
+        out = torch.empty_like(a)
+        check(self.lib.lsmps_dense_inverse(self.handle, n, abi.dptr(a.data_ptr()),
+                                           abi.dptr(out.data_ptr())), "dense_inverse")
+        return out  # the entry point waits for its stream
 
     def __del__(self):
         try:

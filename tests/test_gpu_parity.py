@@ -347,3 +347,23 @@ def test_batched_tanks_coarse_space_per_tank(rt, abi, cases):
         a, b = fo[k], fo[k + 1]
         assert rel_l2(gc["pressure"][a:b], gs["pressure"][a:b]) <= 1e-8, k
     assert gc["report"]["iterations"] < gs["report"]["iterations"]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("batched", [False, True])
+def test_schwarz_fp16_application_matches_fp32(rt, abi, cases, batched):
+    """Mixed precision applies the Schwarz inverses from FP16 copies scaled per
+    column by a power of two.  It is a slightly different preconditioner of the
+    same FP64 system: the solve must reach the same 1e-10 residual, the pressure
+    must agree with the FP32 application's to the solve's accuracy, and the
+    iteration count must stay within a few percent."""
+    case = (cases.batched_tanks(count=4, min_fluid=4000, max_fluid=20000) if batched
+            else cases.dam_break(block=(40, 40, 40), tank=(80, 40, 80), l0=0.01, seed=2))
+    prec = abi.PRECISION_MIXED
+    g16 = rt.run_step_on_device(case, precision=prec)
+    g32 = rt.run_step_on_device(case, precision=prec, flags=abi.FLAG_SCHWARZ_FP32)
+    print("fp16", g16["report"], "fp32", g32["report"], flush=True)
+    assert g16["report"]["converged"] == 1 and g32["report"]["converged"] == 1
+    assert g16["report"]["schwarz_blocks"] == g32["report"]["schwarz_blocks"] > 0
+    assert rel_l2(g16["pressure"], g32["pressure"]) <= 1e-8
+    assert g16["report"]["iterations"] <= 1.1 * g32["report"]["iterations"] + 16
