@@ -271,9 +271,9 @@ def test_preconditioners_agree(rt, abi, cases, precision):
 
 
 @pytest.mark.parametrize("block,tank,cells,lo,hi", [
-    ((60, 40, 40), (120, 40, 80), 1, 2048, 4096),  # 4 rows per panel thread, 1-column panels
-    ((44, 28, 28), (88, 28, 56), 1, 1024, 2048),   # 2 rows per panel thread, 5-column panels
-    ((60, 40, 40), (120, 40, 80), 2, 64, 1024),    # 1 row per panel thread, 8-column panels
+    ((60, 40, 40), (120, 40, 80), 1, 2048, 4096),  # 4 rows per panel thread, 8-column panels
+    ((44, 28, 28), (88, 28, 56), 1, 1024, 2048),   # 2 rows per panel thread, 16-column panels
+    ((60, 40, 40), (120, 40, 80), 2, 64, 1024),    # 1 row per panel thread, 16-column panels
 ])
 def test_coarse_space_sizes(rt, abi, cases, block, tank, cells, lo, hi):
     """The coarse inverse runs in Gauss-Jordan panels whose width and rows per
