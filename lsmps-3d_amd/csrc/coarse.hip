@@ -9,6 +9,7 @@
 #include <cmath>
 
 #include "coarse.hpp"
+#include "schwarz.hpp"
 #include "stages.hpp"
 
 namespace lsmps {
@@ -716,20 +717,16 @@ __device__ __forceinline__ double block_sum(double s, double* red) {
   return t;
 }
 
-// rc[I] = sum over the members i of aggregate I of w_i (times wfac_i); one
-// block per aggregate, two members per thread in flight.
+// rc[I] = sum over the members i of aggregate I of w_i (times wfac_i), by one
+// kBlock-thread workgroup: up to kInFlight members per thread in flight (an
+// aggregate has ~1400 rows, so usually one round), summed in member order, then
+// the fixed block tree.
 template <typename W>
-__global__ void __launch_bounds__(kBlock) restrict_kernel(int count, const int32_t* agg_start,
-                                                          const int32_t* agg_rows,
-                                                          const W* w, const double* wfac,
-                                                          double* rc,
-                                                          const int32_t* skip = nullptr) {
-  if (launch_skipped(skip)) return;
+__device__ __forceinline__ void restrict_block(int I, const int32_t* agg_start,
+                                               const int32_t* agg_rows, const W* w,
+                                               const double* wfac, double* rc) {
   __shared__ double red[kWaves];
-  const int I = blockIdx.x;
   const int32_t mb = agg_start[I], me = agg_start[I + 1];
-  // up to kInFlight members per thread in flight (an aggregate has ~1400 rows,
-  // so usually one round), summed in member order
   constexpr int kInFlight = 8;
   double s = 0.0;
   for (int32_t m = mb + threadIdx.x; m < me; m += kInFlight * kBlock) {
@@ -758,6 +755,37 @@ __global__ void __launch_bounds__(kBlock) restrict_kernel(int count, const int32
   s = block_sum(s, red);
   if (threadIdx.x == 0) rc[I] = s;
 }
+
+// One block per aggregate.
+template <typename W>
+__global__ void __launch_bounds__(kBlock) restrict_kernel(int count, const int32_t* agg_start,
+                                                          const int32_t* agg_rows,
+                                                          const W* w, const double* wfac,
+                                                          double* rc,
+                                                          const int32_t* skip = nullptr) {
+  if (launch_skipped(skip)) return;
+  (void)count;
+  restrict_block<W>(blockIdx.x, agg_start, agg_rows, w, wfac, rc);
+}
+
+// The Schwarz block products of wa and the coarse restriction of wr in one
+// launch: workgroups [0, product_groups) are the products' (b, part) groups,
+// the rest one aggregate each.  Both read a finished vector and write disjoint
+// buffers, so the two passes overlap instead of following each other.
+template <typename T, typename V>
+__global__ void __launch_bounds__(kBlock) products_restrict_kernel(
+    unsigned product_groups, const int32_t* members, const int32_t* sizes, const V* inverse,
+    const float* scale, const T* wa, double* ybuf, const int32_t* agg_start,
+    const int32_t* agg_rows, const T* wr, const double* wfac, double* rc, const int32_t* skip) {
+  if (launch_skipped(skip)) return;
+  if (blockIdx.x < product_groups) {  // uniform per workgroup
+    schwarz_block_product<T, V>(blockIdx.x, members, sizes, inverse, scale, wa, ybuf);
+  } else {
+    restrict_block<T>(static_cast<int>(blockIdx.x - product_groups), agg_start, agg_rows, wr,
+                      wfac, rc);
+  }
+}
+static_assert(kBlock == kMaxBlock, "one workgroup shape for the fused products and restriction");
 
 // yc = Ainv rc, one block per row.
 __global__ void __launch_bounds__(kBlock) coarse_product_kernel(int count, const double* ainv,
@@ -1132,6 +1160,41 @@ void coarse_solve(Context& ctx, CoarseSpace& C, const W* w, bool scale_by_wfac,
   LSMPS_LAUNCH_CHECK();
 }
 
+void coarse_product(Context& ctx, CoarseSpace& C, const int32_t* skip) {
+  if (C.tanks > 0) {
+    tank_product_kernel<<<wave_blocks(static_cast<size_t>(C.count)), kBlock, 0, ctx.stream>>>(
+        C.count, C.agg_tank.data(), C.tank_agg.data(), C.block_offset.data(), C.ainv.data(),
+        C.rc.data(), C.status.data(), C.yc.data(), skip);
+  } else {
+    coarse_product_kernel<<<static_cast<unsigned>(C.count), kBlock, 0, ctx.stream>>>(
+        C.count, C.ainv.data(), C.rc.data(), C.status.data(), C.yc.data(), skip);
+  }
+  LSMPS_LAUNCH_CHECK();
+}
+
+template <typename T>
+void products_and_restrict(Context& ctx, const SchwarzPrec& P, CoarseSpace& C, const T* wa,
+                           const T* wr, bool scale_by_wfac, const int32_t* skip) {
+  const unsigned groups = static_cast<unsigned>(P.blocks * kApplyParts);
+  const unsigned grid = groups + static_cast<unsigned>(C.count);
+  const double* wfac = scale_by_wfac ? C.wfac.data() : nullptr;
+  if (P.half) {
+    products_restrict_kernel<T, _Float16><<<grid, kBlock, 0, ctx.stream>>>(
+        groups, P.members.data(), P.sizes.data(),
+        reinterpret_cast<const _Float16*>(P.inv16.data()), P.col_scale.data(), wa, P.ybuf.data(),
+        C.agg_start.data(), C.agg_rows.data(), wr, wfac, C.rc.data(), skip);
+  } else if (P.fp32) {
+    products_restrict_kernel<T, float><<<grid, kBlock, 0, ctx.stream>>>(
+        groups, P.members.data(), P.sizes.data(), P.inv32.data(), nullptr, wa, P.ybuf.data(),
+        C.agg_start.data(), C.agg_rows.data(), wr, wfac, C.rc.data(), skip);
+  } else {
+    products_restrict_kernel<T, double><<<grid, kBlock, 0, ctx.stream>>>(
+        groups, P.members.data(), P.sizes.data(), P.inv64.data(), nullptr, wa, P.ybuf.data(),
+        C.agg_start.data(), C.agg_rows.data(), wr, wfac, C.rc.data(), skip);
+  }
+  LSMPS_LAUNCH_CHECK();
+}
+
 template <typename W>
 void coarse_prolong_add(Context& ctx, const CoarseSpace& C, W* v, const int32_t* skip) {
   prolong_add_kernel<W><<<blocks_for(C.n), kBlock, 0, ctx.stream>>>(C.n, C.agg_of_row.data(),
@@ -1139,6 +1202,8 @@ void coarse_prolong_add(Context& ctx, const CoarseSpace& C, W* v, const int32_t*
   LSMPS_LAUNCH_CHECK();
 }
 
+template void products_and_restrict<double>(Context&, const SchwarzPrec&, CoarseSpace&,
+                                            const double*, const double*, bool, const int32_t*);
 template void coarse_solve<double>(Context&, CoarseSpace&, const double*, bool, const int32_t*);
 template void coarse_solve<float>(Context&, CoarseSpace&, const float*, bool, const int32_t*);
 template void coarse_prolong_add<double>(Context&, const CoarseSpace&, double*, const int32_t*);

@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "schwarz.hpp"
 
 namespace lsmps {
 
@@ -131,6 +132,16 @@ void dense_inverse(Context& ctx, int n, const double* matrix, double* inverse);
 template <typename W>
 void coarse_solve(Context& ctx, CoarseSpace& C, const W* w, bool scale_by_wfac,
                   const int32_t* skip = nullptr);
+
+// yc = Ac^-1 rc (the product half of coarse_solve, rc already restricted).
+void coarse_product(Context& ctx, CoarseSpace& C, const int32_t* skip = nullptr);
+
+// The Schwarz block products of wa (into P.ybuf, as schwarz_products) and the
+// restriction rc = P^T (wfac * wr) (wfac skipped unless scale_by_wfac) in one
+// launch; P must have blocks and C an aggregation.
+template <typename T>
+void products_and_restrict(Context& ctx, const SchwarzPrec& P, CoarseSpace& C, const T* wa,
+                           const T* wr, bool scale_by_wfac, const int32_t* skip = nullptr);
 
 // v[i] += yc[agg_of_row[i]] on the aggregated rows.
 template <typename W>

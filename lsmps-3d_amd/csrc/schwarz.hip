@@ -433,48 +433,13 @@ __global__ void cover_sort_kernel(size_t n, const int32_t* start, int32_t* entri
   }
 }
 
-// y_b = inv_b w_b, split over kApplyParts workgroups per block b so that a
-// few hundred blocks still fill the GPU: workgroup (b, p) multiplies the column
-// range [p * 64, p * 64 + 64) and thread t (row t) writes its partial sum to
-// ybuf[(b * kApplyParts + p) * kMaxBlock + t].  The column slices are read
-// coalesced across the rows.
-// With FP16 inverses (V = _Float16) column j holds inv * 2^e_j and the input
-// slice takes the exact factor scale[j] = 2^-e_j instead (scale null: none).
 template <typename T, typename V>
 __global__ void __launch_bounds__(kMaxBlock) apply_kernel(const int32_t* members,
                                                           const int32_t* sizes, const V* inverse,
                                                           const float* scale, const T* w,
                                                           double* ybuf, const int32_t* skip) {
   if (launch_skipped(skip)) return;
-  __shared__ double wb[kApplyCols];
-  const int t = threadIdx.x;
-  const size_t b = blockIdx.x / kApplyParts;
-  const int part = static_cast<int>(blockIdx.x % kApplyParts);
-  const int n = sizes[b];
-  const int j0 = part * kApplyCols;
-  const int j1 = min(n, j0 + kApplyCols);
-  constexpr int ld = kMaxBlock;
-  const int32_t* mem = members + b * kMaxBlock;
-  const V* inv = inverse + b * ld * ld;
-  if (t < kApplyCols) {
-    double x = j0 + t < j1 ? static_cast<double>(w[mem[j0 + t]]) : 0.0;
-    if (scale && j0 + t < j1) x *= static_cast<double>(scale[b * ld + j0 + t]);
-    wb[t] = x;
-  }
-  __syncthreads();
-  if (t >= n) return;
-  double y[4] = {0.0, 0.0, 0.0, 0.0};
-  int j = j0;
-  for (; j + 4 <= j1; j += 4) {
-#pragma unroll
-    for (int u = 0; u < 4; ++u)
-      y[u] += static_cast<double>(static_cast<float>(inv[static_cast<size_t>(j + u) * ld + t])) *
-              wb[j + u - j0];
-  }
-  for (; j < j1; ++j)
-    y[0] += static_cast<double>(static_cast<float>(inv[static_cast<size_t>(j) * ld + t])) *
-            wb[j - j0];
-  ybuf[(b * kApplyParts + part) * kMaxBlock + t] = (y[0] + y[1]) + (y[2] + y[3]);
+  schwarz_block_product<T, V>(blockIdx.x, members, sizes, inverse, scale, w, ybuf);
 }
 
 // Covered rows take the mean of their blocks' values (each the sum of its
@@ -701,15 +666,23 @@ void schwarz_products(Context& ctx, const SchwarzPrec& P, const T* v, const int3
 }
 
 template <typename T>
-void schwarz_apply(Context& ctx, const SchwarzPrec& P, T* v, const int32_t* skip) {
+void schwarz_complete(Context& ctx, const SchwarzPrec& P, T* v, const int32_t* skip) {
   if (P.blocks == 0) return;
-  schwarz_products<T>(ctx, P, v, skip);
   gather_kernel<T><<<blocks_for(std::max<size_t>(P.covered_count, 1)), kBlock, 0, ctx.stream>>>(
       P.covered_count, P.covered.data(), P.cover_start.data(), P.cover_entries.data(),
       P.ybuf.data(), v, skip);
   LSMPS_LAUNCH_CHECK();
 }
 
+template <typename T>
+void schwarz_apply(Context& ctx, const SchwarzPrec& P, T* v, const int32_t* skip) {
+  if (P.blocks == 0) return;
+  schwarz_products<T>(ctx, P, v, skip);
+  schwarz_complete<T>(ctx, P, v, skip);
+}
+
+template void schwarz_complete<double>(Context&, const SchwarzPrec&, double*, const int32_t*);
+template void schwarz_complete<float>(Context&, const SchwarzPrec&, float*, const int32_t*);
 template void schwarz_apply<double>(Context&, const SchwarzPrec&, double*, const int32_t*);
 template void schwarz_apply<float>(Context&, const SchwarzPrec&, float*, const int32_t*);
 template void schwarz_products<double>(Context&, const SchwarzPrec&, const double*,

@@ -57,6 +57,50 @@ __device__ __forceinline__ double schwarz_row_mean(const int32_t* start, const i
   return z / static_cast<double>(hi - lo);
 }
 
+// y_b = inv_b w_b, split over kApplyParts workgroups per block b so that a
+// few hundred blocks still fill the GPU: workgroup (b, p) = group multiplies
+// the column range [p * 64, p * 64 + 64) and thread t (row t) writes its
+// partial sum to ybuf[(b * kApplyParts + p) * kMaxBlock + t].  The column
+// slices are read coalesced across the rows.  With FP16 inverses
+// (V = _Float16) column j holds inv * 2^e_j and the input slice takes the
+// exact factor scale[j] = 2^-e_j instead (scale null: none).  Called by every
+// thread of a kMaxBlock-thread workgroup.
+template <typename T, typename V>
+__device__ __forceinline__ void schwarz_block_product(unsigned group, const int32_t* members,
+                                                      const int32_t* sizes, const V* inverse,
+                                                      const float* scale, const T* w,
+                                                      double* ybuf) {
+  __shared__ double wb[kApplyCols];
+  const int t = threadIdx.x;
+  const size_t b = group / kApplyParts;
+  const int part = static_cast<int>(group % kApplyParts);
+  const int n = sizes[b];
+  const int j0 = part * kApplyCols;
+  const int j1 = min(n, j0 + kApplyCols);
+  constexpr int ld = kMaxBlock;
+  const int32_t* mem = members + b * kMaxBlock;
+  const V* inv = inverse + b * ld * ld;
+  if (t < kApplyCols) {
+    double x = j0 + t < j1 ? static_cast<double>(w[mem[j0 + t]]) : 0.0;
+    if (scale && j0 + t < j1) x *= static_cast<double>(scale[b * ld + j0 + t]);
+    wb[t] = x;
+  }
+  __syncthreads();
+  if (t >= n) return;
+  double y[4] = {0.0, 0.0, 0.0, 0.0};
+  int j = j0;
+  for (; j + 4 <= j1; j += 4) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      y[u] += static_cast<double>(static_cast<float>(inv[static_cast<size_t>(j + u) * ld + t])) *
+              wb[j + u - j0];
+  }
+  for (; j < j1; ++j)
+    y[0] += static_cast<double>(static_cast<float>(inv[static_cast<size_t>(j) * ld + t])) *
+            wb[j - j0];
+  ybuf[(b * kApplyParts + part) * kMaxBlock + t] = (y[0] + y[1]) + (y[2] + y[3]);
+}
+
 // Builds the blocks of matrix A from its per-row statistics: diagonal A_ii,
 // Jacobi diagonal 1 / A_ii and the sum of the off-diagonal magnitudes.
 // fp32_inverse: mixed precision (FP32 inverses; FP16 application copies when
@@ -75,6 +119,10 @@ void schwarz_transpose_of(Context& ctx, SchwarzPrec& P, const SchwarzPrec& src,
 // of its blocks' values (inv_b w_b), summed in block order.
 template <typename T>
 void schwarz_apply(Context& ctx, const SchwarzPrec& P, T* v, const int32_t* skip = nullptr);
+// The completion alone (covered rows of v take their blocks' mean from P.ybuf),
+// after the products were formed by schwarz_products or products_and_restrict.
+template <typename T>
+void schwarz_complete(Context& ctx, const SchwarzPrec& P, T* v, const int32_t* skip = nullptr);
 // The block products alone (into P.ybuf); a consumer that reads v row by row
 // can then take covered rows from schwarz_row_mean instead of the completion.
 template <typename T>

@@ -113,34 +113,47 @@ __device__ __forceinline__ void block_reduce_store(const double (&acc)[NQ], int 
 }
 
 // Fixed-order sums over b < nb of partials[b][q] for every q < kPartialStride
-// at once, by one block of kReduceThreads.  The partials are one contiguous
-// array of nb * kPartialStride doubles and kReduceThreads is a multiple of
-// kPartialStride, so thread t always reads column t % kPartialStride: every
-// pass is one coalesced sweep.  Results land in out[0 .. kPartialStride).
+// (nb * kPartialStride doubles, record b at b * kPartialStride), by the
+// kReduceThreads threads of one block.  Thread t owns column t % kPartialStride
+// and the records t / kPartialStride + 64 j: its kReduceLoads loads of a round
+// (all of nb <= 2048 in one) are issued before the first add and summed in a
+// fixed pairwise tree, so every sweep over j is coalesced and one memory
+// latency long; the four same-column lanes of a wavefront then meet in a fixed
+// butterfly and the 16 wavefronts in LDS, in wavefront order.  Results land in
+// out[0 .. kPartialStride).
 constexpr int kReduceThreads = 1024;
-static_assert(kReduceThreads % kPartialStride == 0, "column-stable sweep");
+constexpr int kReduceRows = kReduceThreads / kPartialStride;  // records per sweep
+constexpr int kReduceLoads = 32;                               // sweeps in flight
+static_assert(kReduceThreads % 64 == 0 && 64 % kPartialStride == 0, "column-stable lanes");
 __device__ void reduce_partials(const double* partials, int nb, double* out) {
-  __shared__ double red[kReduceThreads];
+  __shared__ double red[kReduceThreads / 64][kPartialStride];
   const int t = threadIdx.x;
-  const int total = nb * kPartialStride;
-  // eight independent loads in flight per thread
-  constexpr int L = 8;
-  double acc[L];
+  const int q = t % kPartialStride;
+  const int r0 = t / kPartialStride;
+  double s = 0.0;
+  for (int base = 0; base < nb; base += kReduceRows * kReduceLoads) {
+    double v[kReduceLoads];
 #pragma unroll
-  for (int u = 0; u < L; ++u) acc[u] = 0.0;
-  int e = t;
-  for (; e + (L - 1) * kReduceThreads < total; e += L * kReduceThreads) {
+    for (int j = 0; j < kReduceLoads; ++j) {
+      const int b = base + r0 + j * kReduceRows;
+      v[j] = b < nb ? partials[static_cast<size_t>(b) * kPartialStride + q] : 0.0;
+    }
 #pragma unroll
-    for (int u = 0; u < L; ++u) acc[u] += partials[e + u * kReduceThreads];
+    for (int w = kReduceLoads / 2; w > 0; w >>= 1)
+#pragma unroll
+      for (int j = 0; j < w; ++j) v[j] += v[j + w];
+    s += v[0];
   }
-  for (; e < total; e += kReduceThreads) acc[0] += partials[e];
-  red[t] = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+#pragma unroll
+  for (int off = kPartialStride; off < 64; off <<= 1) s += __shfl_xor(s, off, 64);
+  if ((t & 63) < kPartialStride) red[t >> 6][t & 63] = s;
   __syncthreads();
-  for (int w = kReduceThreads / 2; w >= kPartialStride; w >>= 1) {
-    if (t < w) red[t] += red[t + w];
-    __syncthreads();
+  if (t < kPartialStride) {
+    double tot = 0.0;
+#pragma unroll
+    for (int w = 0; w < kReduceThreads / 64; ++w) tot += red[w][t];
+    out[t] = tot;
   }
-  if (t < kPartialStride) out[t] = red[t];
   __syncthreads();
 }
 
@@ -1988,8 +2001,12 @@ int idr_solve(Context& ctx, PpeSolver& S, size_t n, const SellMatrix& M, const V
     }
     for (int k = 0; k < s; ++k) {
       if (schwarz) {
-        schwarz_products<T>(ctx, S.schwarz, v, skip);
-        if (coarse) coarse_solve(ctx, S.coarse, v, true, skip);
+        if (coarse) {  // block products and restriction in one launch, then yc
+          products_and_restrict<T>(ctx, S.schwarz, S.coarse, v, v, true, skip);
+          coarse_product(ctx, S.coarse, skip);
+        } else {
+          schwarz_products<T>(ctx, S.schwarz, v, skip);
+        }
         idr_a2g_kernel<T><<<vb, kBlock, 0, ctx.stream>>>(iv, k);
       }
       SellArgs<T, V> sa{n, M.chunks, M.dia_pairs.data(), M.dia_singles.data(), M.dia_plan.data(),
@@ -2011,10 +2028,13 @@ int idr_solve(Context& ctx, PpeSolver& S, size_t n, const SellMatrix& M, const V
       }
     }
     if (schwarz) {
-      schwarz_apply<T>(ctx, S.schwarz, v, skip);
-      if (coarse) {
-        coarse_solve(ctx, S.coarse, r, false, skip);
+      if (coarse) {  // products of v and restriction of r in one launch
+        products_and_restrict<T>(ctx, S.schwarz, S.coarse, v, r, false, skip);
+        schwarz_complete<T>(ctx, S.schwarz, v, skip);
+        coarse_product(ctx, S.coarse, skip);
         coarse_prolong_add(ctx, S.coarse, v, skip);
+      } else {
+        schwarz_apply<T>(ctx, S.schwarz, v, skip);
       }
     }
     SellArgs<T, V> st{n, M.chunks, M.dia_pairs.data(), M.dia_singles.data(), M.dia_plan.data(),
