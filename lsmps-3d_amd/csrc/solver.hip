@@ -83,7 +83,6 @@ template <typename T>
 __device__ __forceinline__ double shadow(const T* p0, size_t row, int q) {
   return q == 0 ? static_cast<double>(p0[row]) : hashed_sign(static_cast<uint32_t>(row), q, kShadowSeed);
 }
-
 template <int NQ, int BLOCK = kBlock>
 __device__ __forceinline__ void block_reduce_store(const double (&acc)[NQ], int nq,
                                                    double* partials) {
@@ -1175,10 +1174,11 @@ __global__ void __launch_bounds__(kBlock) idr_a1_kernel(IdrVec<T> a, int k) {
 template <typename T, bool kSeg = false>
 __global__ void __launch_bounds__(kBlock) idr_a2_kernel(IdrVec<T> a, int k) {
   if (launch_skipped(a.skip)) return;
+  const double om1 = a.sc->om;  // one system: loaded once
   for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < a.n;
        i += static_cast<size_t>(gridDim.x) * blockDim.x) {
     const SolverScalars* sc = row_sc<kSeg>(a, i);
-    const double om = sc->om;
+    const double om = kSeg ? sc->om : om1;
     const double u = comb_add(0.0, sc->c, a.U, a.ld, i, k, a.s);
     a.U[k * a.ld + i] = static_cast<T>(u + om * static_cast<double>(a.v[i]));
   }
@@ -1186,13 +1186,15 @@ __global__ void __launch_bounds__(kBlock) idr_a2_kernel(IdrVec<T> a, int k) {
 
 // idr_a2_kernel with the Schwarz completion fused: covered rows take their
 // blocks' mean (schwarz_row_mean) in place of v, which is not written back.
-template <typename T>
+template <typename T, bool kSeg = false>
 __global__ void __launch_bounds__(kBlock) idr_a2g_kernel(IdrVec<T> a, int k) {
   if (launch_skipped(a.skip)) return;
-  const double om = a.sc->om;
+  const double om1 = a.sc->om;  // one system: loaded once
   for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < a.n;
        i += static_cast<size_t>(gridDim.x) * blockDim.x) {
-    const double u = comb_add(0.0, a.sc->c, a.U, a.ld, i, k, a.s);
+    const SolverScalars* sc = row_sc<kSeg>(a, i);
+    const double om = kSeg ? sc->om : om1;
+    const double u = comb_add(0.0, sc->c, a.U, a.ld, i, k, a.s);
     double v = static_cast<double>(a.v[i]);
     if (a.cover_start[i + 1] > a.cover_start[i])
       v = static_cast<double>(static_cast<T>(
@@ -1441,14 +1443,15 @@ __global__ void __launch_bounds__(kBlock) idr_b_kernel(IdrVec<T> a, int k, int w
 // scalars idr_scalar_after(k) wrote): after the biorthogonalisation and the r, x
 // updates, v = pdiag (r - sum_{q>k} c_q G_q); without Schwarz blocks also
 // U_{k+1} = sum_{q>k} c_q U_q + om v (idr_a_kernel).
-template <typename T, bool kBlocks>
+template <typename T, bool kBlocks, bool kSeg = false>
 __global__ void __launch_bounds__(kBlock) idr_ba_kernel(IdrVec<T> a, int k) {
   if (launch_skipped(a.skip)) return;
   double acc[1] = {0.0};
-  const SolverScalars* sc = a.sc;
-  const double beta = sc->beta, om = sc->om;
+  const double beta1 = a.sc->beta, om1 = a.sc->om;  // one system: loaded once
   for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < a.n;
        i += static_cast<size_t>(gridDim.x) * blockDim.x) {
+    const SolverScalars* sc = row_sc<kSeg>(a, i);
+    const double beta = kSeg ? sc->beta : beta1, om = kSeg ? sc->om : om1;
     double g = static_cast<double>(a.G[k * a.ld + i]);
     double u = static_cast<double>(a.U[k * a.ld + i]);
     comb_sub2(g, u, sc->alpha, a.G, a.U, a.ld, i, k);
@@ -2193,11 +2196,27 @@ void idr_solve_segmented(Context& ctx, PpeSolver& S, size_t n, const SellMatrix&
   const bool coarse = schwarz && S.coarse.ready && S.coarse.n == n && S.coarse.tanks == nseg;
   const V* dia = sizeof(V) == sizeof(double) ? reinterpret_cast<const V*>(M.dia64.data())
                                              : reinterpret_cast<const V*>(M.dia32.data());
-  // the segmented recurrence applies the Schwarz blocks with schwarz_apply, so
-  // the fused completion pointers stay null
-  IdrVec<T> iv{n,  ld, s, r, x, v, t, G, U, p0, S.scalars.data(), S.part_b.data(),
-               schwarz ? S.schwarz.pdiag.data() : dinv, g.row_seg, nullptr, nullptr, nullptr,
-               nullptr, nullptr};
+  // the same fused kernels as one system (b + next a1; a2 with the Schwarz
+  // completion and the coarse term), each row reading its tank's scalars
+  IdrVec<T> iv{n,
+               ld,
+               s,
+               r,
+               x,
+               v,
+               t,
+               G,
+               U,
+               p0,
+               S.scalars.data(),
+               S.part_b.data(),
+               schwarz ? S.schwarz.pdiag.data() : dinv,
+               g.row_seg,
+               schwarz ? S.schwarz.cover_start.data() : nullptr,
+               schwarz ? S.schwarz.cover_entries.data() : nullptr,
+               schwarz ? S.schwarz.ybuf.data() : nullptr,
+               coarse ? S.coarse.agg_of_row.data() : nullptr,
+               coarse ? S.coarse.yc.data() : nullptr};
   const unsigned vb = vec_blocks(n);
   const unsigned sb = sell_blocks(M.chunks);
   const unsigned rb = static_cast<unsigned>(nseg * kSegBlocks);
@@ -2281,14 +2300,17 @@ void idr_solve_segmented(Context& ctx, PpeSolver& S, size_t n, const SellMatrix&
   while (active() && it < max_iterations) {
     seg_reduce_kernel<T, 4><<<rb, kBlock, 0, ctx.stream>>>(g.start, s, p0, nullptr, r, part);
     idr_scalar_f<<<nseg, kReduceThreads, 0, ctx.stream>>>(S.scalars.data(), part, kSegBlocks, s);
+    // step 0's first half; later steps get it fused into the previous step's b
+    if (schwarz) idr_a1_kernel<T, true><<<vb, kBlock, 0, ctx.stream>>>(iv, 0);
     for (int k = 0; k < s; ++k) {
       if (schwarz) {
-        idr_a1_kernel<T, true><<<vb, kBlock, 0, ctx.stream>>>(iv, k);
-        // the coarse restriction reads w = v / pdiag before the blocks overwrite v
-        if (coarse) coarse_solve(ctx, S.coarse, v, true);
-        schwarz_apply<T>(ctx, S.schwarz, v);
-        if (coarse) coarse_prolong_add(ctx, S.coarse, v);
-        idr_a2_kernel<T, true><<<vb, kBlock, 0, ctx.stream>>>(iv, k);
+        if (coarse) {  // block products and restriction in one launch, then yc
+          products_and_restrict<T>(ctx, S.schwarz, S.coarse, v, v, true);
+          coarse_product(ctx, S.coarse);
+        } else {
+          schwarz_products<T>(ctx, S.schwarz, v);
+        }
+        idr_a2g_kernel<T, true><<<vb, kBlock, 0, ctx.stream>>>(iv, k);
       } else {
         idr_a_kernel<T, true><<<vb, kBlock, 0, ctx.stream>>>(iv, k);
       }
@@ -2296,13 +2318,22 @@ void idr_solve_segmented(Context& ctx, PpeSolver& S, size_t n, const SellMatrix&
       seg_reduce_kernel<T, 1><<<rb, kBlock, 0, ctx.stream>>>(g.start, s, p0, G + k * ld, r, part);
       idr_scalar_after<<<nseg, kReduceThreads, 0, ctx.stream>>>(S.scalars.data(), part,
                                                                 kSegBlocks, s, k);
-      idr_b_kernel<T, true><<<vb, kBlock, 0, ctx.stream>>>(iv, k, k == s - 1 ? 1 : 0);
+      if (schwarz && k < s - 1) {
+        idr_ba_kernel<T, true, true><<<vb, kBlock, 0, ctx.stream>>>(iv, k);
+      } else {
+        idr_b_kernel<T, true><<<vb, kBlock, 0, ctx.stream>>>(iv, k, k == s - 1 ? 1 : 0);
+      }
       ++it;
     }
     if (schwarz) {
-      if (coarse) coarse_solve(ctx, S.coarse, r, false);
-      schwarz_apply<T>(ctx, S.schwarz, v);
-      if (coarse) coarse_prolong_add(ctx, S.coarse, v);
+      if (coarse) {  // products of v and restriction of r in one launch
+        products_and_restrict<T>(ctx, S.schwarz, S.coarse, v, r, false);
+        schwarz_complete<T>(ctx, S.schwarz, v);
+        coarse_product(ctx, S.coarse);
+        coarse_prolong_add(ctx, S.coarse, v);
+      } else {
+        schwarz_apply<T>(ctx, S.schwarz, v);
+      }
     }
     spmv(v, t);
     seg_reduce_kernel<T, 2><<<rb, kBlock, 0, ctx.stream>>>(g.start, s, p0, t, r, part);
