@@ -83,19 +83,60 @@ template <typename T>
 __device__ __forceinline__ double shadow(const T* p0, size_t row, int q) {
   return q == 0 ? static_cast<double>(p0[row]) : hashed_sign(static_cast<uint32_t>(row), q, kShadowSeed);
 }
+// Sums of kMaxShadow (16) values over the 64 lanes of a wavefront by recursive
+// halving: at offset 32, 16, 8, 4 each lane keeps half of its remaining sums
+// (the half its lane bit selects) and adds the partner's copy of that half,
+// then the last two offsets add whole values.  Lane l ends with the sum of
+// value l >> 2 (17 double shuffles instead of 16 x 6 for one butterfly per
+// value); the order of the additions is fixed, so the sums are reproducible.
+__device__ __forceinline__ double wave_sum16(const double (&v)[kMaxShadow], int lane) {
+  static_assert(kMaxShadow == 16, "four halving steps");
+  double a8[8], a4[4], a2[2];
+  const bool b5 = (lane & 32) != 0, b4 = (lane & 16) != 0, b3 = (lane & 8) != 0,
+             b2 = (lane & 4) != 0;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const double keep = b5 ? v[8 + i] : v[i], give = b5 ? v[i] : v[8 + i];
+    a8[i] = keep + __shfl_xor(give, 32, 64);
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const double keep = b4 ? a8[4 + i] : a8[i], give = b4 ? a8[i] : a8[4 + i];
+    a4[i] = keep + __shfl_xor(give, 16, 64);
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const double keep = b3 ? a4[2 + i] : a4[i], give = b3 ? a4[i] : a4[2 + i];
+    a2[i] = keep + __shfl_xor(give, 8, 64);
+  }
+  double z;
+  {
+    const double keep = b2 ? a2[1] : a2[0], give = b2 ? a2[0] : a2[1];
+    z = keep + __shfl_xor(give, 4, 64);
+  }
+  z += __shfl_xor(z, 2, 64);
+  z += __shfl_xor(z, 1, 64);
+  return z;
+}
+
 template <int NQ, int BLOCK = kBlock>
 __device__ __forceinline__ void block_reduce_store(const double (&acc)[NQ], int nq,
                                                    double* partials) {
   __shared__ double red[BLOCK / 64][kPartialStride];
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
+  if constexpr (NQ == kMaxShadow) {  // the 16 shadow dots: recursive halving
+    const double v = wave_sum16(acc, lane);
+    if ((lane & 3) == 0) red[wave][lane >> 2] = v;
+  } else {
 #pragma unroll
-  for (int q = 0; q < NQ; ++q) {
-    if (q < nq) {
-      double v = acc[q];
+    for (int q = 0; q < NQ; ++q) {
+      if (q < nq) {
+        double v = acc[q];
 #pragma unroll
-      for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-      if (lane == 0) red[wave][q] = v;
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        if (lane == 0) red[wave][q] = v;
+      }
     }
   }
   __syncthreads();
@@ -945,20 +986,29 @@ __global__ void __launch_bounds__(kSellBlock) sell_spmv_kernel(SellArgs<T, V> a)
       a.y[row] = yv;
       ys = static_cast<double>(yv);
     }
-    for (int q = 0; q < nq; ++q) {  // uniform trip count
-      double d = 0.0;
-      if (row < a.n) {
-        if (MODE == 1) d = shadow(a.p0, row, q) * ys;
-        else d = (q == 0 ? a.r[row] : ys) * ys;
-      }
+    if (MODE == 1) {  // the shadow dots: lane l accumulates dot l >> 2
+      double d[kMaxShadow];
 #pragma unroll
-      for (int off = 32; off > 0; off >>= 1) d += __shfl_xor(d, off, 64);
-      if (lane == q) lane_acc += d;
+      for (int q = 0; q < kMaxShadow; ++q)
+        d[q] = row < a.n && q < nq ? shadow(a.p0, row, q) * ys : 0.0;
+      lane_acc += wave_sum16(d, lane);
+    } else {
+      for (int q = 0; q < nq; ++q) {  // uniform trip count
+        double d = 0.0;
+        if (row < a.n) d = (q == 0 ? a.r[row] : ys) * ys;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) d += __shfl_xor(d, off, 64);
+        if (lane == q) lane_acc += d;
+      }
     }
   }
   if (MODE != 0) {
     __shared__ double red[kSellBlock / 64][kPartialStride];
-    if (lane < kPartialStride) red[wave_in_block][lane] = lane < nq ? lane_acc : 0.0;
+    if (MODE == 1) {  // dot q sits in lanes 4q .. 4q + 3
+      if ((lane & 3) == 0) red[wave_in_block][lane >> 2] = (lane >> 2) < nq ? lane_acc : 0.0;
+    } else if (lane < kPartialStride) {
+      red[wave_in_block][lane] = lane < nq ? lane_acc : 0.0;
+    }
     __syncthreads();
     if (static_cast<int>(threadIdx.x) < kPartialStride) {
       double t = 0.0;
