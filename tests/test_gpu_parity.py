@@ -367,3 +367,66 @@ def test_schwarz_fp16_application_matches_fp32(rt, abi, cases, batched):
     assert g16["report"]["schwarz_blocks"] == g32["report"]["schwarz_blocks"] > 0
     assert rel_l2(g16["pressure"], g32["pressure"]) <= 1e-8
     assert g16["report"]["iterations"] <= 1.1 * g32["report"]["iterations"] + 16
+
+
+def _batch_of(cases, tanks):
+    """Tanks concatenated into one batched case (as cases.batched_tanks does)."""
+    fluid = {key: np.concatenate([t.fluid[key] for t in tanks]) for key in tanks[0].fluid}
+    walls = {key: np.concatenate([t.walls[key] for t in tanks]) for key in tanks[0].walls}
+    fo = np.concatenate([[0], np.cumsum([t.n_fluid for t in tanks])]).astype(np.int64)
+    wo = np.concatenate([[0], np.cumsum([t.n_wall for t in tanks])]).astype(np.int64)
+    return cases.Case("batch", fluid, walls, tanks[0].config.copy(),
+                      {"fluid_offsets": fo, "wall_offsets": wo,
+                       "grids": [t.config.grid() for t in tanks], "tanks": tanks})
+
+
+@pytest.mark.gpu
+def test_batched_coarse_space_widens_and_skips_empty_tanks(rt, abi, cases, oracle):
+    """Per-tank coarse space at its edges: 2-cell aggregates give the two large
+    tanks more than the 256 aggregates a tank's dense block may hold, so the
+    factor widens for the whole batch; a 2 x 2 x 2 droplet in mid-air between
+    them has only Surface rows and no aggregate at all.  The batch must
+    converge to the pressure of the Schwarz-only solve, with a coarse space no
+    larger than 256 unknowns per tank, and match the oracle tank by tank.
+    (A 2 x 2 x 2 block filling a 2 x 2 x 2 tank would not do: its rows are all
+    Inner, so its PPE has no Dirichlet row and is singular.)"""
+    l0 = 0.005
+    x, y, z = cases.lattice_block(2, 2, 2, l0, origin=(6 * l0, 6 * l0, 6 * l0))
+    drop_fluid, drop_walls = cases._fluid_dict(x, y, z), cases.tank_walls(16, 16, 16, l0)
+    drop_cfg = cases.case_config(l0)
+    cases.derive_cell_grid(drop_cfg, drop_fluid, drop_walls)
+    tanks = [cases.block_in_tank("big0", (40, 40, 30), (80, 40, 60), l0, 0.05, 3),
+             cases.Case("droplet", drop_fluid, drop_walls, drop_cfg, {"l0": l0}),
+             cases.block_in_tank("big1", (36, 36, 36), (72, 36, 72), l0, 0.05, 4)]
+    case = _batch_of(cases, tanks)
+    prec = abi.PRECISION_FP64
+    sch = abi.default_solver_options()
+    sch.preconditioner = abi.PRECOND_SCHWARZ
+    crs = abi.default_solver_options()
+    crs.coarse_cells = 2
+    gs = rt.run_step_on_device(case, precision=prec, solver=sch)
+    gc = rt.run_step_on_device(case, precision=prec, solver=crs)
+    print("schwarz", gs["report"], "coarse", gc["report"], flush=True)
+    assert gs["report"]["converged"] == 1 and gc["report"]["converged"] == 1
+    # the oracle's aggregation of each tank: the smallest factor >= 2 at which
+    # no tank exceeds 256 aggregates is the batch's factor
+    refs = [oracle.pipeline(t) for t in tanks]
+    counts = {f: [oracle.coarse_aggregation(t, r["neighbors"], r["surface"]["surface_type"],
+                                            factor=f)[1] for t, r in zip(tanks, refs)]
+              for f in (2, 3, 4)}
+    f_used = min(f for f in counts if max(counts[f]) <= 256)
+    print("aggregates per factor", counts, "factor used", f_used, flush=True)
+    assert f_used > 2 and counts[2][1] == 0
+    assert gc["report"]["coarse_unknowns"] == sum(counts[f_used])
+    fo = case.extra["fluid_offsets"]
+    tiny = gc["surface_type"][fo[1]:fo[2]]
+    assert np.all((tiny == abi.SURFACE) | (tiny == abi.SPLASH)), tiny
+    assert rel_l2(gc["pressure"], gs["pressure"]) <= 1e-8
+    assert gc["report"]["iterations"] < gs["report"]["iterations"]
+    for k, (tank, ref) in enumerate(zip(tanks, refs)):
+        a, b = fo[k], fo[k + 1]
+        assert np.array_equal(gc["surface_type"][a:b], ref["surface"]["surface_type"]), k
+        if np.any(ref["pressure"] != 0.0):
+            assert rel_l2(gc["pressure"][a:b], ref["pressure"]) <= 1e-7, k
+        else:
+            assert np.all(gc["pressure"][a:b] == 0.0), k
