@@ -354,6 +354,15 @@ int lsmps_context_force_lanes(lsmps_context* ctx, int32_t kernel, int32_t lanes)
  * partial pivoting, rank-b updates on the FP64 matrix cores.  Lets a caller
  * check those kernels directly.  LSMPS_ERR_NUMERICAL on a zero pivot. */
 int lsmps_dense_inverse(lsmps_context* ctx, int32_t n, const double* matrix, double* inverse);
+/* No reference counterpart (the reference hands its PPE to AMGX).  y = A x for
+ * a square CSR matrix (device pointers, FP64 x and y) through the layout and
+ * kernel the PPE solve's Krylov SpMVs use: diagonals of 256-row groups read as
+ * x windows, the rest in sliced ELLPACK, values stored in FP64
+ * (LSMPS_PRECISION_FP64) or rounded to FP32 (LSMPS_PRECISION_MIXED), products
+ * accumulated in FP64.  Builds the layout on each call; lets a caller check it
+ * against a CSR product. */
+int lsmps_ppe_spmv(lsmps_context* ctx, const lsmps_csr* matrix, const double* x, double* y,
+                   int32_t value_precision);
 /* PPE SpMV statistics gathered under LSMPS_FLAG_KERNEL_TIMING since the last
  * reset: launches, summed duration (ms) and summed algorithmic bytes
  * (values + column indices + row pointers, each vector element read or written

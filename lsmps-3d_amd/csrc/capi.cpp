@@ -153,6 +153,16 @@ int lsmps_dense_inverse(lsmps_context* ctx, int32_t n, const double* matrix, dou
   API_END("lsmps_dense_inverse")
 }
 
+int lsmps_ppe_spmv(lsmps_context* ctx, const lsmps_csr* matrix, const double* x, double* y,
+                   int32_t value_precision) {
+  API_BEGIN
+  require(matrix != nullptr, "null matrix");
+  require(value_precision == LSMPS_PRECISION_FP64 || value_precision == LSMPS_PRECISION_MIXED,
+          "value precision must be LSMPS_PRECISION_FP64 or LSMPS_PRECISION_MIXED");
+  sell_spmv(C(ctx), *matrix, x, y, value_precision == LSMPS_PRECISION_MIXED);
+  API_END("lsmps_ppe_spmv")
+}
+
 int lsmps_context_synchronize(lsmps_context* ctx) {
   API_BEGIN
   C(ctx).sync();

@@ -354,9 +354,10 @@ __device__ __forceinline__ int dia_find(const int32_t* list, int nd, int32_t d) 
 }
 
 constexpr int kDiaTable = 2048;  // LDS hash slots (power of two)
-// Stored entries of one row that a lane reads with one vector load: 4 values
-// (16 bytes in FP32) and 4 16-bit columns (8 bytes).  Slot k of a chunk part is
-// at (k / kQuad) * kQuad * kSellChunk + lane * kQuad + k % kQuad.
+// Remainder entries of one row that a lane reads with one vector load: 4 values
+// (16 bytes in FP32) and 4 16-bit columns (8 bytes).  Slot k of a chunk's
+// remainder is at (k / kQuad) * kQuad * kSellChunk + lane * kQuad + k % kQuad.
+// A DIA quad holds one diagonal's values of a lane's 4 consecutive rows.
 constexpr int kQuad = 4;
 __device__ __forceinline__ int64_t quad_slot(int32_t k, int lane) {
   return static_cast<int64_t>(k / kQuad) * (kQuad * kSellChunk) + lane * kQuad + k % kQuad;
@@ -443,64 +444,73 @@ __device__ __forceinline__ SegmentRank segment_rank(int lane, int row, bool flag
   return {static_cast<int>(__popcll(flags & seg)), tail};
 }
 
-// One 64-thread block per chunk: counts the offsets column - row of the
-// chunk's entries in an LDS hash table, keeps those present in at least
-// kDiaMinRows rows as the chunk's diagonals (ascending), sizes the remainder,
-// plans the diagonals' storage and lays the remainder's column windows.  A
-// table overflow or more than kMaxDia qualifying offsets leaves
-// the chunk without diagonals, so the selection never depends on the order of
-// the atomics.
-__global__ void __launch_bounds__(kSellChunk) dia_select_kernel(
+// One 256-thread block per group of four chunks: counts the offsets
+// column - row of the group's entries in an LDS hash table (wave w hashes the
+// entries of chunk 4g + w), keeps those present in at least kDiaMinRows of the
+// group's rows as its diagonals (ascending) and plans their storage.  A table
+// overflow or more than kMaxDia qualifying offsets leaves the group without
+// diagonals, so the selection never depends on the order of the atomics.
+//
+// Plan (thread 0, nd <= kMaxDia): the sorted offsets are cut into runs of
+// consecutive offsets of at most kMaxRun; a run of length m is class m when the
+// window of every lane (x[row + d] for the group's first row up to the last
+// lane's row + d + 2 * ((m + 4) / 2) - 1) lies inside [0, n), else each of its
+// offsets is class 0.  Plan order and storage slots: classes kMaxRun .. 1 (one
+// plan entry per run, m slots), then class 0 (one entry and slot per offset);
+// within a class, ascending offsets.
+__global__ void __launch_bounds__(kGroupRows) dia_group_kernel(
     size_t n, const int32_t* rows, const int32_t* cols, int32_t* dia_count,
-    int32_t* dia_offsets, int32_t* dia_slot, int32_t* dia_plan, int32_t* dia_pairs,
-    int32_t* dia_singles, int32_t* dia_slots, int32_t* rem_width, int32_t* rem_scaled,
-    unsigned long long* rem_total, int32_t* nwin, int32_t* windows, int32_t* col_rows) {
+    int32_t* dia_offsets, int32_t* dia_slot, int32_t* dia_plan, int32_t* dia_runs,
+    int32_t* dia_slots) {
   __shared__ int32_t keys[kDiaTable];
   __shared__ int32_t counts[kDiaTable];
   __shared__ int32_t list[kMaxDia];
   __shared__ int32_t nlist;
   __shared__ int32_t failed;
-  __shared__ int32_t rs[kSellChunk + 1];
-  __shared__ int32_t rem_count[kSellChunk + 1];
-  const size_t chunk = blockIdx.x;
-  const int lane = threadIdx.x;
+  __shared__ int32_t rs[kDiaGroup][kSellChunk + 1];
+  const size_t g = blockIdx.x;
+  const int wave = static_cast<int>(threadIdx.x) >> 6;
+  const int lane = static_cast<int>(threadIdx.x) & 63;
+  const size_t chunk = g * kDiaGroup + wave;
   const int32_t row0 = static_cast<int32_t>(chunk * kSellChunk);
-  load_row_starts(n, chunk, rows, rs, lane);
-  rem_count[lane] = 0;
-  for (int t = lane; t < kDiaTable; t += kSellChunk) {
+  load_row_starts(n, chunk, rows, rs[wave], lane);
+  for (int t = threadIdx.x; t < kDiaTable; t += kGroupRows) {
     keys[t] = kDiaEmpty;
     counts[t] = 0;
   }
-  if (lane == 0) {
+  if (threadIdx.x == 0) {
     nlist = 0;
     failed = 0;
   }
   __syncthreads();
-  const int32_t e0 = rs[0], e1 = rs[kSellChunk];
-  // the next group's column is loaded while this group's entry is hashed
-  int32_t col_next = e0 + lane < e1 ? cols[e0 + lane] : 0;
-  for (int32_t base = e0; base < e1; base += kSellChunk) {
-    const int32_t c = base + lane;
-    if (c >= e1) break;
-    const int32_t col = col_next;
-    if (c + kSellChunk < e1) col_next = cols[c + kSellChunk];
-    const int32_t d = col - (row0 + chunk_row(rs, c));
-    uint32_t h = (static_cast<uint32_t>(d) * 2654435761u) & (kDiaTable - 1);
-    bool placed = false;
-    for (int probe = 0; probe < kDiaTable; ++probe) {
-      const int32_t prev = atomicCAS(&keys[h], kDiaEmpty, d);
-      if (prev == kDiaEmpty || prev == d) {
-        atomicAdd(&counts[h], 1);
-        placed = true;
-        break;
+  {
+    const int32_t* r = rs[wave];
+    const int32_t e0 = r[0], e1 = r[kSellChunk];
+    // the next group's column is loaded while this group's entry is hashed
+    int32_t col_next = e0 + lane < e1 ? cols[e0 + lane] : 0;
+    for (int32_t base = e0; base < e1; base += kSellChunk) {
+      const int32_t c = base + lane;
+      if (c >= e1) break;
+      const int32_t col = col_next;
+      if (c + kSellChunk < e1) col_next = cols[c + kSellChunk];
+      const int32_t d = col - (row0 + chunk_row(r, c));
+      uint32_t h = (static_cast<uint32_t>(d) * 2654435761u) & (kDiaTable - 1);
+      bool placed = false;
+      for (int probe = 0; probe < kDiaTable; ++probe) {
+        const int32_t prev = atomicCAS(&keys[h], kDiaEmpty, d);
+        if (prev == kDiaEmpty || prev == d) {
+          atomicAdd(&counts[h], 1);
+          placed = true;
+          break;
+        }
+        h = (h + 1) & (kDiaTable - 1);
       }
-      h = (h + 1) & (kDiaTable - 1);
+      if (!placed) failed = 1;
     }
-    if (!placed) failed = 1;
   }
   __syncthreads();
   if (!failed) {
-    for (int t = lane; t < kDiaTable; t += kSellChunk) {
+    for (int t = threadIdx.x; t < kDiaTable; t += kGroupRows) {
       if (keys[t] != kDiaEmpty && counts[t] >= kDiaMinRows) {
         const int32_t slot = atomicAdd(&nlist, 1);
         if (slot < kMaxDia) list[slot] = keys[t]; else failed = 1;
@@ -510,7 +520,7 @@ __global__ void __launch_bounds__(kSellChunk) dia_select_kernel(
   __syncthreads();
   const int nd = failed ? 0 : nlist;
   // sort the list ascending (signed order via the sign-flipped high word)
-  {
+  if (wave == 0) {
     uint64_t key[kMaxDia / kSellChunk];
 #pragma unroll
     for (int q = 0; q < kMaxDia / kSellChunk; ++q) {
@@ -519,25 +529,96 @@ __global__ void __launch_bounds__(kSellChunk) dia_select_kernel(
                       : ~0ull;
     }
     wave_bitonic_sort<kMaxDia / kSellChunk>(key, lane);
-    __syncthreads();
 #pragma unroll
     for (int q = 0; q < kMaxDia / kSellChunk; ++q) {
       const int i = lane * (kMaxDia / kSellChunk) + q;
       if (i < nd) {
         const int32_t d = static_cast<int32_t>(static_cast<uint32_t>(key[q] >> 32) ^ 0x80000000u);
         list[i] = d;
-        dia_offsets[chunk * kMaxDia + i] = d;
+        dia_offsets[g * kMaxDia + i] = d;
       }
     }
   }
   __syncthreads();
-  // Remainder entries per row (the chunk's widest remainder row sizes the
-  // remainder), and the 2048-wide column buckets the remainder touches, as a
-  // 128-bit mask relative to the chunk's first row.
+  if (threadIdx.x == 0) {
+    // runs (first sorted index, length, class) in ascending order, in the
+    // hash table's memory, which is free now
+    int32_t* run_first = keys;
+    int32_t* run_len = keys + kMaxDia;
+    int32_t* run_cls = counts;
+    const int64_t grow0 = static_cast<int64_t>(g) * kGroupRows;
+    const int64_t last_lane_row = grow0 + kGroupRows - 4;
+    int32_t per_class[kRunClasses];
+#pragma unroll
+    for (int c = 0; c < kRunClasses; ++c) per_class[c] = 0;
+    int nr = 0;
+    for (int i = 0; i < nd;) {
+      int m = 1;
+      while (i + m < nd && m < kMaxRun && list[i + m] == list[i] + m) ++m;
+      const int64_t lo = grow0 + list[i];
+      const int64_t hi = last_lane_row + list[i] + 2 * ((m + 4) / 2) - 1;
+      const int cls = lo >= 0 && hi <= static_cast<int64_t>(n) - 1 ? m : 0;
+      run_first[nr] = i;
+      run_len[nr] = m;
+      run_cls[nr] = cls;
+      per_class[cls] += cls > 0 ? 1 : m;
+      ++nr;
+      i += m;
+    }
+    // plan and slot positions of each class: kMaxRun .. 1, then 0
+    int32_t plan_at[kRunClasses], slot_at[kRunClasses];
+    int32_t p = 0, sl = 0;
+    for (int c = kMaxRun; c >= 0; --c) {
+      plan_at[c] = p;
+      slot_at[c] = sl;
+      p += per_class[c];
+      sl += c > 0 ? per_class[c] * c : per_class[c];
+    }
+    int32_t* plan = dia_plan + g * kMaxDia;
+    int32_t* slot_of = dia_slot + g * kMaxDia;
+    for (int r = 0; r < nr; ++r) {
+      const int i = run_first[r], m = run_len[r], c = run_cls[r];
+      if (c > 0) {
+        plan[plan_at[c]++] = list[i];
+        for (int q = 0; q < m; ++q) slot_of[i + q] = slot_at[c]++;
+      } else {
+        for (int q = 0; q < m; ++q) {
+          plan[plan_at[0]++] = list[i + q];
+          slot_of[i + q] = slot_at[0]++;
+        }
+      }
+    }
+    for (int c = 0; c < kRunClasses; ++c) dia_runs[g * kRunClasses + c] = per_class[c];
+    dia_count[g] = nd;
+    dia_slots[g] = nd * kGroupRows;
+  }
+}
+
+// One 64-thread block per chunk: the entries that are not on one of its
+// group's diagonals form the remainder.  Sizes it (the chunk's widest remainder
+// row) and lays its column windows over the 2048-wide column buckets it
+// touches (a 128-bit mask relative to the chunk's first row).
+__global__ void __launch_bounds__(kSellChunk) rem_select_kernel(
+    size_t n, const int32_t* rows, const int32_t* cols, const int32_t* dia_count,
+    const int32_t* dia_offsets, int32_t* rem_width, int32_t* rem_scaled,
+    unsigned long long* rem_total, int32_t* nwin, int32_t* windows, int32_t* col_rows) {
+  __shared__ int32_t list[kMaxDia];
+  __shared__ int32_t rs[kSellChunk + 1];
+  __shared__ int32_t rem_count[kSellChunk + 1];
+  const size_t chunk = blockIdx.x;
+  const size_t g = chunk / kDiaGroup;
+  const int lane = threadIdx.x;
+  const int32_t row0 = static_cast<int32_t>(chunk * kSellChunk);
+  const int nd = dia_count[g];
+  for (int t = lane; t < nd; t += kSellChunk) list[t] = dia_offsets[g * kMaxDia + t];
+  load_row_starts(n, chunk, rows, rs, lane);
+  rem_count[lane] = 0;
+  __syncthreads();
+  const int32_t e0 = rs[0], e1 = rs[kSellChunk];
   const int64_t first_bucket = static_cast<int64_t>(row0 >> kBucketBits) - 64;
   uint64_t lo = 0, hi = 0;
   int outside = 0;
-  col_next = e0 + lane < e1 ? cols[e0 + lane] : 0;
+  int32_t col_next = e0 + lane < e1 ? cols[e0 + lane] : 0;
   for (int32_t base = e0; base < e1; base += kSellChunk) {
     const int32_t c = base + lane;
     const bool in = c < e1;
@@ -573,84 +654,33 @@ __global__ void __launch_bounds__(kSellChunk) dia_select_kernel(
   // Windows of two buckets (4096 columns), laid greedily from the lowest
   // uncovered bucket; columns outside the 128-bucket span, or more than
   // kSellWindows windows, give the chunk 32-bit columns.
-  {
-    int32_t* wb = windows + chunk * kSellWindows;
-    int count = 0;
-    bool wide = outside != 0;
-    for (int bit = 0; bit < 128 && !wide; ++bit) {
-      const bool set = bit < 64 ? ((lo >> bit) & 1ull) : ((hi >> (bit - 64)) & 1ull);
-      if (!set) continue;
-      if (count == kSellWindows) {
-        wide = true;
-        break;
-      }
-      if (lane == 0) wb[count] = static_cast<int32_t>((first_bucket + bit) << kBucketBits);
-      ++count;
-      ++bit;  // the window also covers the next bucket
+  int32_t* wb = windows + chunk * kSellWindows;
+  int count = 0;
+  bool wide = outside != 0;
+  for (int bit = 0; bit < 128 && !wide; ++bit) {
+    const bool set = bit < 64 ? ((lo >> bit) & 1ull) : ((hi >> (bit - 64)) & 1ull);
+    if (!set) continue;
+    if (count == kSellWindows) {
+      wide = true;
+      break;
     }
-    if (!wide && count == 0) {  // a chunk without remainder: one window at 0 for the padding
-      if (lane == 0) wb[0] = 0;
-      count = 1;
-    }
-    if (wide) {  // 32-bit columns; padding points at the chunk's first row
-      if (lane == 0) wb[0] = row0;
-      count = 1;
-    }
-    if (lane == 0) {
-      for (int j = count; j < kSellWindows; ++j) wb[j] = wb[count - 1];
-      nwin[chunk] = wide ? -1 : count;
-      const int32_t w4 = (width + kQuad - 1) / kQuad * kQuad;
-      col_rows[chunk] = w4 * (wide ? 2 : 1);
-    }
+    if (lane == 0) wb[count] = static_cast<int32_t>((first_bucket + bit) << kBucketBits);
+    ++count;
+    ++bit;  // the window also covers the next bucket
   }
-  // Storage plan (lane 0; nd <= kMaxDia): pairs (d, d + 1) greedily along the
-  // runs of the sorted list, kept only when every pair reads inside [0, n) for
-  // every real row of the chunk; then the singles.  Pair entries are padded to
-  // an even count and singles to a multiple of 4, so both parts fill whole
-  // quads; padding repeats an offset of its part (value 0).
-  const int32_t width4 = (width + kQuad - 1) / kQuad * kQuad;
+  if (!wide && count == 0) {  // a chunk without remainder: one window at 0 for the padding
+    if (lane == 0) wb[0] = 0;
+    count = 1;
+  }
+  if (wide) {  // 32-bit columns; padding points at the chunk's first row
+    if (lane == 0) wb[0] = row0;
+    count = 1;
+  }
   if (lane == 0) {
-    const int64_t rend = static_cast<int64_t>(row0) + kSellChunk - 1;
-    const int64_t rlast = rend < static_cast<int64_t>(n) - 1 ? rend : static_cast<int64_t>(n) - 1;
-    bool safe = true;
-    int pairs = 0;
-    for (int i = 0; i < nd;) {
-      if (i + 1 < nd && list[i + 1] == list[i] + 1) {
-        if (static_cast<int64_t>(row0) + list[i] < 0 ||
-            rlast + list[i] + 1 > static_cast<int64_t>(n) - 1)
-          safe = false;
-        ++pairs;
-        i += 2;
-      } else {
-        ++i;
-      }
-    }
-    if (!safe) pairs = 0;
-    const int p2 = (pairs + 1) / 2 * 2;
-    int32_t* plan = dia_plan + chunk * kMaxPlan;
-    int32_t* slot_of = dia_slot + chunk * kMaxDia;
-    int np = 0, ns = 0;
-    for (int i = 0; i < nd;) {
-      if (pairs > 0 && i + 1 < nd && list[i + 1] == list[i] + 1) {
-        plan[np] = list[i];
-        slot_of[i] = 2 * np;
-        slot_of[i + 1] = 2 * np + 1;
-        ++np;
-        i += 2;
-      } else {
-        plan[p2 + ns] = list[i];
-        slot_of[i] = 2 * p2 + ns;
-        ++ns;
-        i += 1;
-      }
-    }
-    for (int q = np; q < p2; ++q) plan[q] = plan[0];
-    const int s4 = (ns + kQuad - 1) / kQuad * kQuad;
-    for (int q = ns; q < s4; ++q) plan[p2 + q] = ns > 0 ? plan[p2 + ns - 1] : 0;
-    dia_count[chunk] = nd;
-    dia_pairs[chunk] = p2;
-    dia_singles[chunk] = s4;
-    dia_slots[chunk] = (2 * p2 + s4) * kSellChunk;
+    for (int j = count; j < kSellWindows; ++j) wb[j] = wb[count - 1];
+    nwin[chunk] = wide ? -1 : count;
+    const int32_t width4 = (width + kQuad - 1) / kQuad * kQuad;
+    col_rows[chunk] = width4 * (wide ? 2 : 1);
     rem_width[chunk] = width4;
     rem_scaled[chunk] = width4 * kSellChunk;
     atomicAdd(rem_total, total);
@@ -658,10 +688,11 @@ __global__ void __launch_bounds__(kSellChunk) dia_select_kernel(
 }
 
 // One 64-thread block per chunk writes its entries, 64 at a time in CSR order
-// (coalesced loads): DIA values into their diagonal slots (the DIA array is
-// zeroed beforehand), every other entry into the next remainder slot of its row
-// with its column encoded against the chunk's windows (or as a 32-bit index);
-// then each lane pads its row's remainder to the chunk's width (base_0, value 0).
+// (coalesced loads): values on one of its group's diagonals into their slots
+// (the DIA array is zeroed beforehand), every other entry into the next
+// remainder slot of its row with its column encoded against the chunk's
+// windows (or as a 32-bit index); then each lane pads its row's remainder to
+// the chunk's width (base_0, value 0).
 template <typename V>
 __global__ void __launch_bounds__(kSellChunk) sell_fill_kernel(
     size_t n, const int32_t* __restrict__ rows, const int32_t* __restrict__ cols,
@@ -678,18 +709,20 @@ __global__ void __launch_bounds__(kSellChunk) sell_fill_kernel(
   __shared__ int32_t rs[kSellChunk + 1];
   __shared__ int32_t rem_count[kSellChunk + 1];
   const size_t chunk = blockIdx.x;
+  const size_t g = chunk / kDiaGroup;
   const int lane = threadIdx.x;
   const int32_t row0 = static_cast<int32_t>(chunk * kSellChunk);
-  const int nd = dia_count[chunk];
+  const int nd = dia_count[g];
   for (int t = lane; t < nd; t += kSellChunk) {
-    s_off[t] = dia_offsets[chunk * kMaxDia + t];
-    s_slot[t] = dia_slot[chunk * kMaxDia + t];
+    s_off[t] = dia_offsets[g * kMaxDia + t];
+    s_slot[t] = dia_slot[g * kMaxDia + t];
   }
   if (lane < kSellWindows) s_win[lane] = windows[chunk * kSellWindows + lane];
   load_row_starts(n, chunk, rows, rs, lane);
   rem_count[lane] = 0;
   __syncthreads();
-  const int64_t dbase = dia_start[chunk];
+  // row r of the chunk is row (chunk % 4) * 64 + r of its group
+  const int64_t dbase = static_cast<int64_t>(dia_start[g]) + (chunk % kDiaGroup) * kSellChunk;
   const int64_t vbase = chunk_start[chunk];
   const int64_t cbase = static_cast<int64_t>(col_start[chunk]) * kSellChunk;
   const int32_t w = width[chunk];
@@ -721,7 +754,7 @@ __global__ void __launch_bounds__(kSellChunk) sell_fill_kernel(
     const int j = in ? dia_find(s_off, nd, c - (row0 + lr)) : -1;
     const bool rem = in && j < 0;
     const SegmentRank sr = segment_rank(lane, lr, rem);
-    if (in && j >= 0) dia_vals[dbase + quad_slot(s_slot[j], lr)] = static_cast<V>(v);
+    if (in && j >= 0) dia_vals[dbase + static_cast<int64_t>(s_slot[j]) * kGroupRows + lr] = static_cast<V>(v);
     if (rem) put(rem_count[lr] + sr.before, lr, c, static_cast<V>(v));
     __syncthreads();  // every lane has read rem_count[lr]
     if (in && sr.tail) rem_count[lr] += sr.before + (rem ? 1 : 0);
@@ -734,8 +767,8 @@ template <typename T, typename V>
 struct SellArgs {
   size_t n;
   size_t chunks;
-  const int32_t* dia_pairs;
-  const int32_t* dia_singles;
+  size_t groups;
+  const int32_t* dia_runs;
   const int32_t* dia_plan;
   const int32_t* dia_start;
   const V* dia_vals;
@@ -812,173 +845,176 @@ struct XGather<float> {
 // Largest system the 32-bit buffer offsets address.
 constexpr size_t kMaxGatherRows = (size_t{1} << 31) / sizeof(double) - 1;
 
-// One wavefront per 64-row chunk, lane = row.  Each workgroup owns a contiguous
-// range of chunks (XCD-aware order above); its kSellBlock/64 waves interleave
-// within it.  MODE 1: shadow dots P_q . y.  MODE 2: y.r and y.y.
+// Runs of class M (M consecutive offsets each) of one group.  The lane owning
+// rows i .. i + 3 reads x[i + d .. i + d + M + 2] as (M + 4) / 2 16-byte loads
+// and the run's M value quads (4 rows each), then adds its 4 M products.
+// Wave w of the block takes every fourth run, starting at `first`.
+template <int M, typename T, typename V>
+__device__ __forceinline__ void dia_run_class(int count, int first, const int32_t* plan,
+                                              const V* vp, uint32_t rowb, const XGather<T>& xg,
+                                              double (&acc)[4]) {
+  constexpr int NL = (M + 4) / 2;
+  for (int i = first; i < count; i += kDiaGroup) {
+    const uint32_t off = rowb + static_cast<uint32_t>(plan[i]) * static_cast<uint32_t>(sizeof(T));
+    T xw[2 * NL];
+#pragma unroll
+    for (int j = 0; j < NL; ++j) xg.pair(off + 16u * j, xw[2 * j], xw[2 * j + 1]);
+    V v[M][kQuad];
+#pragma unroll
+    for (int q = 0; q < M; ++q) load_quad(vp + static_cast<int64_t>(i * M + q) * kGroupRows, v[q]);
+#pragma unroll
+    for (int q = 0; q < M; ++q)
+#pragma unroll
+      for (int t = 0; t < kQuad; ++t)
+        acc[t] += static_cast<double>(v[q][t]) * static_cast<double>(xw[t + q]);
+  }
+}
+
+// One 256-thread workgroup per 256-row group at a time; each workgroup owns a
+// contiguous range of groups (an even split in the XCD-aware order above).
+// DIA part: lane l of every wave holds rows 4 l .. 4 l + 3 of the group and the
+// four waves share out the group's runs (run j of the plan goes to wave j mod 4);
+// the four partial sums of a row meet in LDS.  Remainder: wave w takes chunk
+// 4 g + w with one row per lane, which is also the row whose sum, y and dots
+// that lane finishes.  MODE 1: shadow dots P_q . y.  MODE 2: y.r and y.y.
 //
-// The chunk index is made wave-uniform (readfirstlane), so the chunk
-// descriptors and the diagonal offsets come through scalar loads and the loops
-// branch on scalars.  Each DIA step issues the gathers of 8 entries, then the
-// next step's value quads and offsets, then consumes the gathers: the
-// prefetches stay in flight across the wait.
-constexpr int kSellBlock = 512;
+// The group index is wave-uniform, so the run counts, the plan offsets and the
+// chunk descriptors come through scalar loads and the loops branch on scalars.
+constexpr int kSellBlock = kGroupRows;
 template <typename T, typename V, int MODE>
 __global__ void __launch_bounds__(kSellBlock) sell_spmv_kernel(SellArgs<T, V> a) {
+  static_assert(sizeof(T) == sizeof(double), "x windows are read as 16-byte pairs of doubles");
   if (launch_skipped(a.skip)) return;
+  __shared__ double part[kDiaGroup][kGroupRows];
   const int lane = threadIdx.x & 63;
-  const unsigned wave_in_block = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const size_t per_block = (a.chunks + gridDim.x - 1) / gridDim.x;
-  const size_t first = static_cast<size_t>(xcd_block()) * per_block;
-  const size_t last = first + per_block < a.chunks ? first + per_block : a.chunks;
+  const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+  const size_t b = xcd_block();
+  const size_t first = b * a.groups / gridDim.x;
+  const size_t last = (b + 1) * a.groups / gridDim.x;
   const int nq = MODE == 1 ? a.s : (MODE == 2 ? 2 : 0);
   const XGather<T> xg(a.x, a.n);
   // Lane q keeps the wavefront's running dot product q (one register instead of
   // one per dot in every lane, which keeps occupancy up).
   double lane_acc = 0.0;
-  for (size_t chunk = first + wave_in_block; chunk < last; chunk += kSellBlock / 64) {
-    const size_t row = chunk * kSellChunk + lane;
-    const uint32_t rowb = static_cast<uint32_t>(row) * static_cast<uint32_t>(sizeof(T));
-    const int32_t w = a.width[chunk];
-    const int32_t nw = a.nwin[chunk];
-    const V* vp = a.vals + a.chunk_start[chunk] + lane * kQuad;
-    const int64_t cbase = static_cast<int64_t>(a.col_start[chunk]) * kSellChunk;
-    // FP64 accumulation whatever the storage types: the SpMV is bandwidth bound.
-    constexpr int U = 8;  // entries (two quads) per step
-    double s[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) s[u] = 0.0;
-    // DIA part, in the plan's order: pair entries (two slots, one 16-byte
-    // gather each), then single entries.  Both counts fill whole quads.
+  for (size_t g = first; g < last; ++g) {
     {
-      const int np = a.dia_pairs[chunk];
-      const int ns = a.dia_singles[chunk];
-      const int32_t* pl = a.dia_plan + chunk * kMaxPlan;
-      const V* dp = a.dia_vals + a.dia_start[chunk] + lane * kQuad;
-      auto off = [&](int32_t d) {
-        return rowb + static_cast<uint32_t>(d) * static_cast<uint32_t>(sizeof(T));
+      const uint32_t rowb = static_cast<uint32_t>(g * kGroupRows + kQuad * lane) *
+                            static_cast<uint32_t>(sizeof(T));
+      const int32_t* cnt = a.dia_runs + g * kRunClasses;
+      const int32_t* plan = a.dia_plan + g * kMaxDia;
+      const V* vp = a.dia_vals + a.dia_start[g] + kQuad * lane;
+      double acc[kQuad] = {0.0, 0.0, 0.0, 0.0};
+      int before = 0;  // runs of the classes already done: sets each class's first run per wave
+      auto first_of = [&](int runs_before) {
+        return (wave - runs_before % kDiaGroup + kDiaGroup) % kDiaGroup;
       };
-      // Pairs: a step is 4 pair entries = 8 slots = 2 value quads.  Two register
-      // sets alternate; each step's gathers are issued before the next step's
-      // value and offset loads, so the wait for the gathers leaves those in flight.
-      constexpr int P = 4;
-      auto load_pairs = [&](V* v, int32_t* o, int at) {  // at: pair index
-        load_quad(dp + (2 * at) * kSellChunk, v);
-        load_quad(dp + (2 * at + kQuad) * kSellChunk, v + kQuad);
+#define LSMPS_DIA_CLASS(M)                                                        \
+  {                                                                               \
+    const int c = cnt[M];                                                         \
+    if (c > 0) {                                                                  \
+      dia_run_class<M>(c, first_of(before), plan, vp, rowb, xg, acc);             \
+      plan += c;                                                                  \
+      vp += static_cast<int64_t>(c) * (M * kGroupRows);                           \
+      before += c;                                                                \
+    }                                                                             \
+  }
+      static_assert(kMaxRun == 8, "one case per run class");
+      LSMPS_DIA_CLASS(8)
+      LSMPS_DIA_CLASS(7)
+      LSMPS_DIA_CLASS(6)
+      LSMPS_DIA_CLASS(5)
+      LSMPS_DIA_CLASS(4)
+      LSMPS_DIA_CLASS(3)
+      LSMPS_DIA_CLASS(2)
+      LSMPS_DIA_CLASS(1)
+#undef LSMPS_DIA_CLASS
+      // class 0: offsets whose windows would leave [0, n) for some lane, one
+      // value at a time.  Each column is formed and range-tested on its own:
+      // a byte offset that wrapped below zero plus a constant increment may be
+      // split by the compiler into the instruction's immediate, and the buffer
+      // range check then sees the wrapped base and returns 0 for a column
+      // inside [0, n).
+      const int64_t lane_row = static_cast<int64_t>(g) * kGroupRows + kQuad * lane;
+      for (int i = first_of(before); i < cnt[0]; i += kDiaGroup) {
+        const int64_t base = lane_row + plan[i];
+        V v[kQuad];
+        load_quad(vp + static_cast<int64_t>(i) * kGroupRows, v);
 #pragma unroll
-        for (int u = 0; u < P; ++u) o[u] = pl[at + u];
-      };
-      auto gather_pairs = [&](const int32_t* o, T* xv) {
+        for (int t = 0; t < kQuad; ++t) {
+          const int64_t col = base + t;
+          const double xe =
+              col >= 0 && col < static_cast<int64_t>(a.n)
+                  ? static_cast<double>(xg(static_cast<uint32_t>(col) * static_cast<uint32_t>(sizeof(T))))
+                  : 0.0;
+          acc[t] += static_cast<double>(v[t]) * xe;
+        }
+      }
 #pragma unroll
-        for (int u = 0; u < P; ++u) xg.pair(off(o[u]), xv[2 * u], xv[2 * u + 1]);
-      };
-      auto fma8 = [&](const V* v, const T* xv) {
+      for (int t = 0; t < kQuad; ++t) part[wave][kQuad * lane + t] = acc[t];
+    }
+    const size_t chunk = g * kDiaGroup + wave;
+    const size_t row = chunk * kSellChunk + lane;
+    double s[8];
 #pragma unroll
-        for (int u = 0; u < U; ++u) s[u] += static_cast<double>(v[u]) * static_cast<double>(xv[u]);
-      };
-      int j = 0;
-      if (np >= P) {
-        V va[U], vb[U];
-        int32_t oa[P], ob[P];
-        T xv[U];
-        load_pairs(va, oa, 0);
-        while (j + 2 * P <= np) {
-          gather_pairs(oa, xv);
-          load_pairs(vb, ob, j + P);
-          fma8(va, xv);
-          j += P;
-          if (j + 2 * P > np) {  // set B holds the last full step
-            gather_pairs(ob, xv);
-            fma8(vb, xv);
-            j += P;
-            goto pair_tail;
+    for (int u = 0; u < 8; ++u) s[u] = 0.0;
+    if (chunk < a.chunks) {
+      constexpr int U = 8;  // entries (two quads) per step
+      const int32_t w = a.width[chunk];
+      const int32_t nw = a.nwin[chunk];
+      const V* vp = a.vals + a.chunk_start[chunk] + lane * kQuad;
+      const int64_t cbase = static_cast<int64_t>(a.col_start[chunk]) * kSellChunk;
+      int32_t k = 0;
+      if (nw >= 0) {
+        // lane l holds window base l % 16; an entry's base is fetched across lanes
+        const int32_t wbase = a.windows[chunk * kSellWindows + (lane & (kSellWindows - 1))];
+        const uint16_t* cp = a.cols + cbase + lane * kQuad;
+        constexpr int kMask = (1 << kSellWindowBits) - 1;
+        for (; k + U <= w; k += U) {
+          uint16_t c[U];
+          V v[U];
+          load_quad(cp + k * kSellChunk, c);
+          load_quad(cp + (k + kQuad) * kSellChunk, c + kQuad);
+          load_quad(vp + k * kSellChunk, v);
+          load_quad(vp + (k + kQuad) * kSellChunk, v + kQuad);
+#pragma unroll
+          for (int u = 0; u < U; ++u) {
+            const int32_t col = __shfl(wbase, c[u] >> kSellWindowBits, 64) + (c[u] & kMask);
+            const T xv = xg(static_cast<uint32_t>(col) * static_cast<uint32_t>(sizeof(T)));
+            s[u] += static_cast<double>(v[u]) * static_cast<double>(xv);
           }
-          gather_pairs(ob, xv);
-          load_pairs(va, oa, j + P);
-          fma8(vb, xv);
-          j += P;
         }
-        gather_pairs(oa, xv);  // set A holds the last full step
-        fma8(va, xv);
-        j += P;
-      }
-    pair_tail:
-      if (j < np) {  // two pair entries left (np is even): one quad
-        V v[kQuad];
-        load_quad(dp + (2 * j) * kSellChunk, v);
-        T xv[kQuad];
-        xg.pair(off(pl[j]), xv[0], xv[1]);
-        xg.pair(off(pl[j + 1]), xv[2], xv[3]);
+        for (; k < w; k += kQuad) {
+          uint16_t c[kQuad];
+          V v[kQuad];
+          load_quad(cp + k * kSellChunk, c);
+          load_quad(vp + k * kSellChunk, v);
 #pragma unroll
-        for (int u = 0; u < kQuad; ++u) s[u] += static_cast<double>(v[u]) * static_cast<double>(xv[u]);
-      }
-      // Singles: 8 per step (2 quads), then at most one quad.
-      const V* sp = dp + (2 * np) * kSellChunk;
-      const int32_t* sl = pl + np;
-      int q = 0;
-      for (; q + U <= ns; q += U) {
-        V v[U];
-        load_quad(sp + q * kSellChunk, v);
-        load_quad(sp + (q + kQuad) * kSellChunk, v + kQuad);
-        T xv[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) xv[u] = xg(off(sl[q + u]));
-        fma8(v, xv);
-      }
-      for (; q < ns; q += kQuad) {
-        V v[kQuad];
-        load_quad(sp + q * kSellChunk, v);
-#pragma unroll
-        for (int u = 0; u < kQuad; ++u)
-          s[u] += static_cast<double>(v[u]) * static_cast<double>(xg(off(sl[q + u])));
-      }
-    }
-    int32_t k = 0;
-    if (nw >= 0) {
-      // lane l holds window base l % 16; an entry's base is fetched across lanes
-      const int32_t wbase = a.windows[chunk * kSellWindows + (lane & (kSellWindows - 1))];
-      const uint16_t* cp = a.cols + cbase + lane * kQuad;
-      constexpr int kMask = (1 << kSellWindowBits) - 1;
-      for (; k + U <= w; k += U) {
-        uint16_t c[U];
-        V v[U];
-        load_quad(cp + k * kSellChunk, c);
-        load_quad(cp + (k + kQuad) * kSellChunk, c + kQuad);
-        load_quad(vp + k * kSellChunk, v);
-        load_quad(vp + (k + kQuad) * kSellChunk, v + kQuad);
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-          const int32_t col = __shfl(wbase, c[u] >> kSellWindowBits, 64) + (c[u] & kMask);
-          const T xv = xg(static_cast<uint32_t>(col) * static_cast<uint32_t>(sizeof(T)));
-          s[u] += static_cast<double>(v[u]) * static_cast<double>(xv);
+          for (int u = 0; u < kQuad; ++u) {
+            const int32_t col = __shfl(wbase, c[u] >> kSellWindowBits, 64) + (c[u] & kMask);
+            const T xv = xg(static_cast<uint32_t>(col) * static_cast<uint32_t>(sizeof(T)));
+            s[u] += static_cast<double>(v[u]) * static_cast<double>(xv);
+          }
         }
-      }
-      for (; k < w; k += kQuad) {
-        uint16_t c[kQuad];
-        V v[kQuad];
-        load_quad(cp + k * kSellChunk, c);
-        load_quad(vp + k * kSellChunk, v);
+      } else {
+        const int32_t* cp = reinterpret_cast<const int32_t*>(a.cols + cbase) + lane * kQuad;
+        for (; k < w; k += kQuad) {
+          int32_t c[kQuad];
+          V v[kQuad];
+          load_quad(cp + k * kSellChunk, c);
+          load_quad(vp + k * kSellChunk, v);
 #pragma unroll
-        for (int u = 0; u < kQuad; ++u) {
-          const int32_t col = __shfl(wbase, c[u] >> kSellWindowBits, 64) + (c[u] & kMask);
-          const T xv = xg(static_cast<uint32_t>(col) * static_cast<uint32_t>(sizeof(T)));
-          s[u] += static_cast<double>(v[u]) * static_cast<double>(xv);
-        }
-      }
-    } else {
-      const int32_t* cp = reinterpret_cast<const int32_t*>(a.cols + cbase) + lane * kQuad;
-      for (; k < w; k += kQuad) {
-        int32_t c[kQuad];
-        V v[kQuad];
-        load_quad(cp + k * kSellChunk, c);
-        load_quad(vp + k * kSellChunk, v);
-#pragma unroll
-        for (int u = 0; u < kQuad; ++u) {
-          const T xv = xg(static_cast<uint32_t>(c[u]) * static_cast<uint32_t>(sizeof(T)));
-          s[u] += static_cast<double>(v[u]) * static_cast<double>(xv);
+          for (int u = 0; u < kQuad; ++u) {
+            const T xv = xg(static_cast<uint32_t>(c[u]) * static_cast<uint32_t>(sizeof(T)));
+            s[u] += static_cast<double>(v[u]) * static_cast<double>(xv);
+          }
         }
       }
     }
-    const double sum = ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
+    __syncthreads();
+    const int r = wave * kSellChunk + lane;  // the row within the group
+    const double dia = (part[0][r] + part[1][r]) + (part[2][r] + part[3][r]);
+    __syncthreads();  // part is rewritten by the next group
+    const double sum = dia + (((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7])));
     double ys = 0.0;
     if (row < a.n) {
       // the dots use the stored (rounded) value, which the later updates read
@@ -1005,9 +1041,9 @@ __global__ void __launch_bounds__(kSellBlock) sell_spmv_kernel(SellArgs<T, V> a)
   if (MODE != 0) {
     __shared__ double red[kSellBlock / 64][kPartialStride];
     if (MODE == 1) {  // dot q sits in lanes 4q .. 4q + 3
-      if ((lane & 3) == 0) red[wave_in_block][lane >> 2] = (lane >> 2) < nq ? lane_acc : 0.0;
+      if ((lane & 3) == 0) red[wave][lane >> 2] = (lane >> 2) < nq ? lane_acc : 0.0;
     } else if (lane < kPartialStride) {
-      red[wave_in_block][lane] = lane < nq ? lane_acc : 0.0;
+      red[wave][lane] = lane < nq ? lane_acc : 0.0;
     }
     __syncthreads();
     if (static_cast<int>(threadIdx.x) < kPartialStride) {
@@ -1697,15 +1733,15 @@ unsigned spmv_blocks(size_t n) {
 }
 
 // A multiple of the XCD count (see xcd_block), at most kMaxReduceBlocks.
-unsigned sell_blocks(size_t chunks) {
-  const size_t waves_per_block = kSellBlock / 64;
-  size_t b = (chunks + waves_per_block - 1) / waves_per_block;
-  b = std::min<size_t>(b, kMaxReduceBlocks);
+// SpMV grid: one workgroup per group up to kMaxReduceBlocks (each writes one
+// partial record), a multiple of the 8 XCDs.
+unsigned sell_blocks(size_t groups) {
+  size_t b = std::min<size_t>(std::max<size_t>(groups, 1), kMaxReduceBlocks);
   b = (b + kXcds - 1) / kXcds * kXcds;
-  return static_cast<unsigned>(std::max<size_t>(kXcds, std::min<size_t>(b, kMaxReduceBlocks)));
+  return static_cast<unsigned>(std::min<size_t>(b, kMaxReduceBlocks));
 }
 
-// Converts the CSR (FP64 values) into the SELL-64 layout with values of type V.
+// Converts the CSR (FP64 values) into the DIA + SELL-64 layout with values of type V.
 template <typename V>
 void build_sell(Context& ctx, SellMatrix& M, const lsmps_csr& A_in, V* out_vals_dummy_type) {
   (void)out_vals_dummy_type;
@@ -1732,15 +1768,16 @@ void build_sell(Context& ctx, SellMatrix& M, const lsmps_csr& A_in, V* out_vals_
   A.values = M.sorted_vals.data();
   M.n = n;
   M.chunks = (n + kSellChunk - 1) / kSellChunk;
+  M.groups = (M.chunks + kDiaGroup - 1) / kDiaGroup;
   const size_t chunks = M.chunks;
-  M.dia_count.reserve(chunks);
-  M.dia_offsets.reserve(chunks * kMaxDia);
-  M.dia_slot.reserve(chunks * kMaxDia);
-  M.dia_plan.reserve(chunks * kMaxPlan);
-  M.dia_pairs.reserve(chunks);
-  M.dia_singles.reserve(chunks);
-  M.dia_slots.reserve(chunks);
-  M.dia_start.reserve(chunks + 1);
+  const size_t groups = M.groups;
+  M.dia_count.reserve(groups);
+  M.dia_offsets.reserve(groups * kMaxDia);
+  M.dia_slot.reserve(groups * kMaxDia);
+  M.dia_plan.reserve(groups * kMaxDia);
+  M.dia_runs.reserve(groups * kRunClasses);
+  M.dia_slots.reserve(groups);
+  M.dia_start.reserve(groups + 1);
   M.chunk_width.reserve(chunks);
   M.chunk_scaled.reserve(chunks);
   M.chunk_start.reserve(chunks + 1);
@@ -1750,14 +1787,18 @@ void build_sell(Context& ctx, SellMatrix& M, const lsmps_csr& A_in, V* out_vals_
   M.windows.reserve(chunks * kSellWindows);
   M.rem_counter.reserve(1);
   fill_kernel<unsigned long long><<<1, kBlock, 0, ctx.stream>>>(1, M.rem_counter.data(), 0ull);
-  // diagonals, remainder widths and column windows
-  dia_select_kernel<<<static_cast<unsigned>(chunks), kSellChunk, 0, ctx.stream>>>(
+  // each group's diagonals and their plan, then each chunk's remainder widths
+  // and column windows
+  dia_group_kernel<<<static_cast<unsigned>(groups), kGroupRows, 0, ctx.stream>>>(
       n, A.row_offsets, A.col_indices, M.dia_count.data(), M.dia_offsets.data(),
-      M.dia_slot.data(), M.dia_plan.data(), M.dia_pairs.data(), M.dia_singles.data(),
-      M.dia_slots.data(), M.chunk_width.data(), M.chunk_scaled.data(), M.rem_counter.data(),
-      M.nwin.data(), M.windows.data(), M.col_rows.data());
+      M.dia_slot.data(), M.dia_plan.data(), M.dia_runs.data(), M.dia_slots.data());
   LSMPS_LAUNCH_CHECK();
-  exclusive_scan_i32(ctx, M.dia_slots.data(), M.dia_start.data(), chunks);
+  rem_select_kernel<<<static_cast<unsigned>(chunks), kSellChunk, 0, ctx.stream>>>(
+      n, A.row_offsets, A.col_indices, M.dia_count.data(), M.dia_offsets.data(),
+      M.chunk_width.data(), M.chunk_scaled.data(), M.rem_counter.data(), M.nwin.data(),
+      M.windows.data(), M.col_rows.data());
+  LSMPS_LAUNCH_CHECK();
+  exclusive_scan_i32(ctx, M.dia_slots.data(), M.dia_start.data(), groups);
   exclusive_scan_i32(ctx, M.chunk_scaled.data(), M.chunk_start.data(), chunks);
   exclusive_scan_i32(ctx, M.col_rows.data(), M.col_start.data(), chunks);
   LSMPS_HIP(hipMemcpyAsync(ctx.host_i32, M.chunk_start.data() + chunks, sizeof(int32_t),
@@ -1766,7 +1807,7 @@ void build_sell(Context& ctx, SellMatrix& M, const lsmps_csr& A_in, V* out_vals_
                            hipMemcpyDeviceToHost, ctx.stream));
   LSMPS_HIP(hipMemcpyAsync(ctx.host_i32 + 2, M.col_start.data() + chunks, sizeof(int32_t),
                            hipMemcpyDeviceToHost, ctx.stream));
-  LSMPS_HIP(hipMemcpyAsync(ctx.host_i32 + 3, M.dia_start.data() + chunks, sizeof(int32_t),
+  LSMPS_HIP(hipMemcpyAsync(ctx.host_i32 + 3, M.dia_start.data() + groups, sizeof(int32_t),
                            hipMemcpyDeviceToHost, ctx.stream));
   LSMPS_HIP(hipMemcpyAsync(ctx.host_f64, M.rem_counter.data(), sizeof(unsigned long long),
                            hipMemcpyDeviceToHost, ctx.stream));
@@ -1879,7 +1920,7 @@ __global__ void idr_control_after_replace(IdrControl* c, const SolverScalars* sc
 }
 
 void append_sell_signature(std::vector<uintptr_t>& sig, const SellMatrix& M) {
-  const void* p[] = {M.dia_pairs.data(), M.dia_singles.data(), M.dia_plan.data(),
+  const void* p[] = {M.dia_runs.data(), M.dia_plan.data(),
                      M.dia_start.data(), M.chunk_start.data(), M.chunk_width.data(),
                      M.col_start.data(), M.nwin.data(), M.windows.data(), M.cols.data()};
   for (const void* q : p) sig.push_back(reinterpret_cast<uintptr_t>(q));
@@ -1915,8 +1956,8 @@ double spmv_algorithmic_bytes(const SellMatrix& M, size_t n, size_t value_bytes,
   const double col_bytes =
       2.0 * static_cast<double>(M.col_units) / static_cast<double>(std::max<size_t>(M.entries, 1));
   return static_cast<double>(M.nnz) * value_bytes + static_cast<double>(M.rem_nnz) * col_bytes +
-         M.chunks * (6.0 + kSellWindows) * sizeof(int32_t) +
-         static_cast<double>(M.dia_entries) / kSellChunk * sizeof(int32_t) +
+         M.chunks * (5.0 + kSellWindows) * sizeof(int32_t) + M.groups * (1.0 + kRunClasses) * sizeof(int32_t) +
+         static_cast<double>(M.dia_entries) / kGroupRows * sizeof(int32_t) +
          2.0 * n * vector_bytes;
 }
 
@@ -2018,7 +2059,7 @@ int idr_solve(Context& ctx, PpeSolver& S, size_t n, const SellMatrix& M, const V
                coarse ? S.coarse.agg_of_row.data() : nullptr,
                coarse ? S.coarse.yc.data() : nullptr};
   const unsigned vb = vec_blocks(n);
-  const unsigned sb = sell_blocks(M.chunks);
+  const unsigned sb = sell_blocks(M.groups);
   fill_kernel<T><<<blocks_for(2 * s * ld), kBlock, 0, ctx.stream>>>(2 * s * ld, G, T(0));
   copy_kernel<double, T><<<blocks_for(n), kBlock, 0, ctx.stream>>>(n, r, p0);
   idr_init_scalars<<<1, kBlock, 0, ctx.stream>>>(S.scalars.data(), s);
@@ -2062,7 +2103,7 @@ int idr_solve(Context& ctx, PpeSolver& S, size_t n, const SellMatrix& M, const V
         }
         idr_a2g_kernel<T><<<vb, kBlock, 0, ctx.stream>>>(iv, k);
       }
-      SellArgs<T, V> sa{n, M.chunks, M.dia_pairs.data(), M.dia_singles.data(), M.dia_plan.data(),
+      SellArgs<T, V> sa{n, M.chunks, M.groups, M.dia_runs.data(), M.dia_plan.data(),
                         M.dia_start.data(), dia, M.chunk_start.data(), M.chunk_width.data(),
                         M.col_start.data(), M.nwin.data(), M.windows.data(), M.cols.data(),
                         vals, U + k * ld, G + k * ld, p0, s, nullptr, S.part_a.data(), skip};
@@ -2090,7 +2131,7 @@ int idr_solve(Context& ctx, PpeSolver& S, size_t n, const SellMatrix& M, const V
         schwarz_apply<T>(ctx, S.schwarz, v, skip);
       }
     }
-    SellArgs<T, V> st{n, M.chunks, M.dia_pairs.data(), M.dia_singles.data(), M.dia_plan.data(),
+    SellArgs<T, V> st{n, M.chunks, M.groups, M.dia_runs.data(), M.dia_plan.data(),
                       M.dia_start.data(), dia, M.chunk_start.data(), M.chunk_width.data(),
                       M.col_start.data(), M.nwin.data(), M.windows.data(), M.cols.data(), vals,
                       v, t, nullptr, s, r, S.part_a.data(), skip};
@@ -2268,7 +2309,7 @@ void idr_solve_segmented(Context& ctx, PpeSolver& S, size_t n, const SellMatrix&
                coarse ? S.coarse.agg_of_row.data() : nullptr,
                coarse ? S.coarse.yc.data() : nullptr};
   const unsigned vb = vec_blocks(n);
-  const unsigned sb = sell_blocks(M.chunks);
+  const unsigned sb = sell_blocks(M.groups);
   const unsigned rb = static_cast<unsigned>(nseg * kSegBlocks);
   const unsigned fb = static_cast<unsigned>((nseg + kBlock - 1) / kBlock);
   double* part = S.part_a.data();
@@ -2301,7 +2342,7 @@ void idr_solve_segmented(Context& ctx, PpeSolver& S, size_t n, const SellMatrix&
   const double spmv_bytes = spmv_algorithmic_bytes(M, n, sizeof(V), sizeof(T));
   int timed = 0;  // SpMVs recorded in the current cycle
   auto spmv = [&](const T* in, T* out) {
-    SellArgs<T, V> a{n, M.chunks, M.dia_pairs.data(), M.dia_singles.data(), M.dia_plan.data(),
+    SellArgs<T, V> a{n, M.chunks, M.groups, M.dia_runs.data(), M.dia_plan.data(),
                      M.dia_start.data(),
                      dia, M.chunk_start.data(), M.chunk_width.data(), M.col_start.data(),
                      M.nwin.data(), M.windows.data(), M.cols.data(), vals, in, out, p0, s,
@@ -2498,6 +2539,37 @@ void pcg_solve(Context& ctx, PpeSolver& S, const lsmps_csr& A, const double* b, 
 }
 
 }  // namespace
+
+// y = A x through the Krylov layout and SpMV kernel, for direct checks of the
+// layout against a CSR product: FP32 values (fp32_values) or FP64.
+void sell_spmv(Context& ctx, const lsmps_csr& A, const double* x, double* y, bool fp32_values) {
+  require(A.rows == A.cols, "SpMV check: the matrix must be square");
+  require(x != nullptr && y != nullptr, "SpMV check: null vector");
+  const size_t n = A.rows;
+  if (n == 0) return;
+  SellMatrix M;
+  const unsigned sb = [&] {
+    if (fp32_values) build_sell<float>(ctx, M, A, nullptr); else build_sell<double>(ctx, M, A, nullptr);
+    return sell_blocks((n + kGroupRows - 1) / kGroupRows);
+  }();
+  if (fp32_values) {
+    SellArgs<double, float> a{n, M.chunks, M.groups, M.dia_runs.data(), M.dia_plan.data(),
+                              M.dia_start.data(), M.dia32.data(), M.chunk_start.data(),
+                              M.chunk_width.data(), M.col_start.data(), M.nwin.data(),
+                              M.windows.data(), M.cols.data(), M.vals32.data(), x, y, nullptr, 0,
+                              nullptr, nullptr};
+    sell_spmv_kernel<double, float, 0><<<sb, kSellBlock, 0, ctx.stream>>>(a);
+  } else {
+    SellArgs<double, double> a{n, M.chunks, M.groups, M.dia_runs.data(), M.dia_plan.data(),
+                               M.dia_start.data(), M.dia64.data(), M.chunk_start.data(),
+                               M.chunk_width.data(), M.col_start.data(), M.nwin.data(),
+                               M.windows.data(), M.cols.data(), M.vals64.data(), x, y, nullptr, 0,
+                               nullptr, nullptr};
+    sell_spmv_kernel<double, double, 0><<<sb, kSellBlock, 0, ctx.stream>>>(a);
+  }
+  LSMPS_LAUNCH_CHECK();
+  ctx.sync();  // M's buffers are freed on return
+}
 
 void ppe_solve(Context& ctx, PpeSolver& S, const lsmps_csr& A, const double* b, double* x,
                const lsmps_solver_options& opt, lsmps_solve_report* report) {

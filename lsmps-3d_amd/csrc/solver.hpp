@@ -54,31 +54,31 @@ struct IdrControl {
   double factor2;
 };
 
-// PPE matrix for the Krylov SpMVs: a hybrid of a per-chunk diagonal (DIA)
-// part and a sliced-ELLPACK remainder, over 64-row chunks (one wavefront per
-// chunk, one row per lane).
+// PPE matrix for the Krylov SpMVs: a hybrid of a diagonal (DIA) part over
+// 256-row groups and a sliced-ELLPACK remainder over 64-row chunks (four chunks
+// to a group).  One 256-thread workgroup owns a group: in the DIA part each lane
+// computes four consecutive rows, in the remainder one row.
 //
 // DIA part: the offsets d = column - row that at least kDiaMinRows of the
-// chunk's rows have (particles in lattice or cell order share the ~123 offsets
-// of the 7x7x7 neighbourhood) are listed per chunk, ascending (for lookups).
-// Columns are implicit, and the gathers x[row + d] of a wavefront are
-// contiguous.  The stored diagonals are ordered for the gathers: consecutive
-// offsets (d, d + 1) are paired greedily along each run of the sorted list and
-// stored first, two slots per pair, so one 16-byte load per lane fetches both
-// x values; the unpaired offsets follow.  The pair count is padded to even and
-// the single count to a multiple of 4 (value 0).  A chunk where some pair would
-// read x outside [0, n) for one of its rows keeps every offset single.  The
-// per-chunk plan lists the pair offsets d, then the single offsets, in slot
-// order; dia_slot maps a sorted offset to its storage slot.
+// group's rows have (particles in lattice or cell order share the ~123 offsets
+// of the 7x7x7 neighbourhood) are listed per group, ascending (for lookups).
+// Columns are implicit.  The sorted offsets are cut into runs of consecutive
+// offsets d, d + 1, ..., d + m - 1 (m <= kMaxRun): the lane owning rows
+// i .. i + 3 reads the m + 3 values x[i + d .. i + d + m + 2] as one window of
+// 16-byte loads and takes all 4m products from it.  The plan lists the runs by
+// length, longest first (class m), then the offsets whose window would read
+// partly outside [0, n) for some row of the group (class 0, gathered one value
+// at a time); dia_runs holds the count of each class and dia_slot maps a sorted
+// offset to its storage slot (plan order, one slot per offset).  Slot k of the
+// group's row r sits at dia_start[g] + k * 256 + r, so a lane's four rows are
+// one 16-byte quad in FP32.
 //
 // Remainder: every other entry, in row-sorted order, padded to the chunk's
-// widest remainder row rounded up to a multiple of 4 (value 0).
-//
-// Both parts are stored in per-lane quads so that a lane reads 4 entries of its
-// row with one 16-byte load (8 bytes for 16-bit columns): entry k of the chunk's
-// row r sits at start + (k / 4) * 256 + r * 4 + k % 4, with start = dia_start[c]
-// for diagonal k, chunk_start[c] for remainder entry k, and the same slot
-// numbering for the remainder columns.  Its column indices are stored in 16 bits where
+// widest remainder row rounded up to a multiple of 4 (value 0), stored in
+// per-lane quads so that a lane reads 4 entries of its row with one 16-byte load
+// (8 bytes for 16-bit columns): entry k of the chunk's row r sits at
+// chunk_start[c] + (k / 4) * 256 + r * 4 + k % 4, with the same slot numbering
+// for the columns.  Its column indices are stored in 16 bits where
 // the chunk allows it: the chunk's columns are covered by at most kSellWindows
 // windows [base_j, base_j + 4096), each two aligned 2048-column buckets wide
 // and laid greedily over the buckets the chunk touches, and an entry holds
@@ -86,11 +86,14 @@ struct IdrControl {
 // more than 64 buckets from its first row, keeps 32-bit columns (nwin = -1) in
 // the same array, two 16-bit slots per entry.  Padding points at base_0.
 constexpr int kSellChunk = 64;
+constexpr int kDiaGroup = 4;                       // chunks per DIA group
+constexpr int kGroupRows = kDiaGroup * kSellChunk;  // 256
 constexpr int kSellWindows = 16;
 constexpr int kSellWindowBits = 12;
 constexpr int kMaxDia = 256;
-constexpr int kMaxPlan = kMaxDia + 8;  // plan entries incl. padding
-constexpr int kDiaMinRows = 16;
+constexpr int kMaxRun = 8;
+constexpr int kRunClasses = kMaxRun + 1;  // class 0: single offsets gathered value by value
+constexpr int kDiaMinRows = 64;           // of the group's 256 rows
 struct SellMatrix {
   size_t n = 0;
   size_t chunks = 0;
@@ -99,13 +102,13 @@ struct SellMatrix {
   size_t col_units = 0;   // 16-bit column slots (entries + entries of 32-bit chunks)
   size_t nnz = 0;         // stored CSR nonzeros
   size_t rem_nnz = 0;     // of which in the remainder
-  DeviceArray<int32_t> dia_count;    // diagonals per chunk (sorted list length)
-  DeviceArray<int32_t> dia_offsets;  // chunks x kMaxDia, ascending
-  DeviceArray<int32_t> dia_slot;     // chunks x kMaxDia: storage slot of sorted entry i
-  DeviceArray<int32_t> dia_plan;     // chunks x kMaxPlan: pair offsets, then single offsets
-  DeviceArray<int32_t> dia_pairs;    // plan pair entries per chunk (even)
-  DeviceArray<int32_t> dia_singles;  // plan single entries per chunk (multiple of 4)
-  DeviceArray<int32_t> dia_slots;    // stored slots * 64 (scan input)
+  size_t groups = 0;
+  DeviceArray<int32_t> dia_count;    // diagonals per group (sorted list length)
+  DeviceArray<int32_t> dia_offsets;  // groups x kMaxDia, ascending
+  DeviceArray<int32_t> dia_slot;     // groups x kMaxDia: storage slot of sorted entry i
+  DeviceArray<int32_t> dia_plan;     // groups x kMaxDia: run start offsets by class, then class 0
+  DeviceArray<int32_t> dia_runs;     // groups x kRunClasses: runs per class (class 0: offsets)
+  DeviceArray<int32_t> dia_slots;    // stored slots * 256 (scan input)
   DeviceArray<int32_t> dia_start;    // scan of dia_slots
   DeviceArray<int32_t> chunk_start;  // remainder value slot offset (< 2^31 is checked)
   DeviceArray<int32_t> chunk_width;  // remainder width
@@ -122,7 +125,7 @@ struct SellMatrix {
   DeviceArray<double> sorted_vals;
   size_t bytes() const {
     return dia_count.bytes() + dia_offsets.bytes() + dia_slot.bytes() + dia_plan.bytes() +
-           dia_pairs.bytes() + dia_singles.bytes() + dia_slots.bytes() + dia_start.bytes() +
+           dia_runs.bytes() + dia_slots.bytes() + dia_start.bytes() +
            chunk_start.bytes() + chunk_width.bytes() + chunk_scaled.bytes() + col_rows.bytes() +
            col_start.bytes() + nwin.bytes() + windows.bytes() + rem_counter.bytes() +
            cols.bytes() + vals64.bytes() + dia64.bytes() + vals32.bytes() + dia32.bytes() +

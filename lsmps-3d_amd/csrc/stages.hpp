@@ -142,6 +142,7 @@ void ppe_assemble(Context& ctx, const lsmps_config& cfg, const lsmps_fluid_soa& 
 
 // ---- solver.hip
 class PpeSolver;  // owns Krylov workspaces; defined in solver.hpp
+void sell_spmv(Context& ctx, const lsmps_csr& A, const double* x, double* y, bool fp32_values);
 void ppe_solve(Context& ctx, PpeSolver& solver, const lsmps_csr& a, const double* b, double* x,
                const lsmps_solver_options& opt, lsmps_solve_report* report);
 

@@ -40,7 +40,7 @@ EXPORTS = [
     "lsmps_vtk_path", "lsmps_write_vtk", "lsmps_compute_virtual_light_diagnostics",
     "lsmps_ppe_assemble_views", "lsmps_compute_pressure_gradient_view",
     "lsmps_build_neighbor_lists_noalloc", "lsmps_context_lanes", "lsmps_context_force_lanes",
-    "lsmps_dense_inverse",
+    "lsmps_dense_inverse", "lsmps_ppe_spmv",
 ]
 
 _LIB = None
@@ -312,6 +312,7 @@ class Context:
         torch = _torch()
         self.lib = load()
         self.handle = C.c_void_p()
+        self.precision = precision
         check(self.lib.lsmps_context_create(device, precision, flags, C.byref(self.handle)),
               "context_create")
         s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
@@ -336,6 +337,25 @@ class Context:
     def force_lanes(self, kernel, lanes):
         check(self.lib.lsmps_context_force_lanes(self.handle, abi.TUNED_KERNELS[kernel], lanes),
               "context_force_lanes")
+
+    def ppe_spmv(self, row_offsets, col_indices, values, x, precision=None):
+        """y = A x on the GPU through the PPE solve's SpMV layout and kernel;
+        precision LSMPS_PRECISION_MIXED rounds the stored values to FP32.  The
+        CSR arrays and x are torch tensors (int32, int32, float64, float64)."""
+        import torch
+        rows = row_offsets.to(torch.int32).contiguous()
+        cols = col_indices.to(torch.int32).contiguous()
+        vals = values.to(torch.float64).contiguous()
+        xv = x.to(torch.float64).contiguous()
+        n = rows.shape[0] - 1
+        assert rows.is_cuda and cols.is_cuda and vals.is_cuda and xv.is_cuda and xv.shape[0] == n
+        y = torch.empty(n, dtype=torch.float64, device=xv.device)
+        csr = abi.Csr(n, n, vals.shape[0], abi.iptr(rows.data_ptr()), abi.iptr(cols.data_ptr()),
+                      abi.dptr(vals.data_ptr()))
+        prec = self.precision if precision is None else precision
+        check(self.lib.lsmps_ppe_spmv(self.handle, C.byref(csr), abi.dptr(xv.data_ptr()),
+                                      abi.dptr(y.data_ptr()), prec), "ppe_spmv")
+        return y
 
     def dense_inverse(self, matrix):
         """Inverse of a square float64 torch tensor on the GPU (row-major), by
