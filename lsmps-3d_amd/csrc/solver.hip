@@ -849,13 +849,32 @@ constexpr size_t kMaxGatherRows = (size_t{1} << 31) / sizeof(double) - 1;
 // rows i .. i + 3 reads x[i + d .. i + d + M + 2] as (M + 4) / 2 16-byte loads
 // and the run's M value quads (4 rows each), then adds its 4 M products.
 // Wave w of the block takes every fourth run, starting at `first`.
+// A group's plan held across the lanes of a wavefront (lane j holds entries
+// j, j + 64, j + 128, j + 192), read back one entry at a time with readlane:
+// the offsets then live in scalar registers and a run's window loads issue
+// without first waiting for a vector load of its offset.
+struct GroupPlan {
+  int32_t v0, v1, v2, v3;
+  __device__ int32_t operator[](int p) const {  // p is wave-uniform
+    const int lane = p & 63;
+    if (p < 128) return __builtin_amdgcn_readlane(p < 64 ? v0 : v1, lane);
+    return __builtin_amdgcn_readlane(p < 192 ? v2 : v3, lane);
+  }
+};
+static_assert(kMaxDia == 256, "four plan registers per lane");
+
+// Runs of class M (M consecutive offsets each) of one group, plan entries
+// [pbase, pbase + count).  The lane owning rows i .. i + 3 reads
+// x[i + d .. i + d + M + 2] as (M + 4) / 2 16-byte loads and the run's M value
+// quads (4 rows each), then adds its 4 M products.  Wave w of the block takes
+// every fourth run, starting at `first`.
 template <int M, typename T, typename V>
-__device__ __forceinline__ void dia_run_class(int count, int first, const int32_t* plan,
-                                              const V* vp, uint32_t rowb, const XGather<T>& xg,
-                                              double (&acc)[4]) {
+__device__ __forceinline__ void dia_run_class(int count, int first, const GroupPlan& plan,
+                                              int pbase, const V* vp, uint32_t rowb,
+                                              const XGather<T>& xg, double (&acc)[4]) {
   constexpr int NL = (M + 4) / 2;
   for (int i = first; i < count; i += kDiaGroup) {
-    const uint32_t off = rowb + static_cast<uint32_t>(plan[i]) * static_cast<uint32_t>(sizeof(T));
+    const uint32_t off = rowb + static_cast<uint32_t>(plan[pbase + i]) * static_cast<uint32_t>(sizeof(T));
     T xw[2 * NL];
 #pragma unroll
     for (int j = 0; j < NL; ++j) xg.pair(off + 16u * j, xw[2 * j], xw[2 * j + 1]);
@@ -897,23 +916,37 @@ __global__ void __launch_bounds__(kSellBlock) sell_spmv_kernel(SellArgs<T, V> a)
   // one per dot in every lane, which keeps occupancy up).
   double lane_acc = 0.0;
   for (size_t g = first; g < last; ++g) {
+    // this wave's remainder chunk and its descriptors, loaded ahead of the DIA part
+    const size_t chunk = g * kDiaGroup + wave;
+    const size_t row = chunk * kSellChunk + lane;
+    const bool has_chunk = chunk < a.chunks;
+    const int32_t w = has_chunk ? __builtin_amdgcn_readfirstlane(a.width[chunk]) : 0;
+    const int32_t nw = has_chunk ? __builtin_amdgcn_readfirstlane(a.nwin[chunk]) : 0;
+    const int32_t rstart = has_chunk ? __builtin_amdgcn_readfirstlane(a.chunk_start[chunk]) : 0;
+    const int32_t cstart = has_chunk ? __builtin_amdgcn_readfirstlane(a.col_start[chunk]) : 0;
+    const int32_t wbase =
+        has_chunk && nw >= 0 ? a.windows[chunk * kSellWindows + (lane & (kSellWindows - 1))] : 0;
     {
+      // the group's plan, run counts and DIA start in lanes, read back as scalars
+      const int32_t* pg = a.dia_plan + g * kMaxDia + lane;
+      const GroupPlan plan{pg[0], pg[64], pg[128], pg[192]};
+      const int32_t cnt_lane = lane < kRunClasses ? a.dia_runs[g * kRunClasses + lane] : 0;
+      const int32_t dstart = __builtin_amdgcn_readfirstlane(a.dia_start[g]);
       const uint32_t rowb = static_cast<uint32_t>(g * kGroupRows + kQuad * lane) *
                             static_cast<uint32_t>(sizeof(T));
-      const int32_t* cnt = a.dia_runs + g * kRunClasses;
-      const int32_t* plan = a.dia_plan + g * kMaxDia;
-      const V* vp = a.dia_vals + a.dia_start[g] + kQuad * lane;
+      const V* vp = a.dia_vals + dstart + kQuad * lane;
       double acc[kQuad] = {0.0, 0.0, 0.0, 0.0};
+      int pbase = 0;   // plan entries of the classes already done
       int before = 0;  // runs of the classes already done: sets each class's first run per wave
       auto first_of = [&](int runs_before) {
         return (wave - runs_before % kDiaGroup + kDiaGroup) % kDiaGroup;
       };
 #define LSMPS_DIA_CLASS(M)                                                        \
   {                                                                               \
-    const int c = cnt[M];                                                         \
+    const int c = __builtin_amdgcn_readlane(cnt_lane, M);                         \
     if (c > 0) {                                                                  \
-      dia_run_class<M>(c, first_of(before), plan, vp, rowb, xg, acc);             \
-      plan += c;                                                                  \
+      dia_run_class<M>(c, first_of(before), plan, pbase, vp, rowb, xg, acc);      \
+      pbase += c;                                                                 \
       vp += static_cast<int64_t>(c) * (M * kGroupRows);                           \
       before += c;                                                                \
     }                                                                             \
@@ -935,8 +968,9 @@ __global__ void __launch_bounds__(kSellBlock) sell_spmv_kernel(SellArgs<T, V> a)
       // range check then sees the wrapped base and returns 0 for a column
       // inside [0, n).
       const int64_t lane_row = static_cast<int64_t>(g) * kGroupRows + kQuad * lane;
-      for (int i = first_of(before); i < cnt[0]; i += kDiaGroup) {
-        const int64_t base = lane_row + plan[i];
+      const int c0 = __builtin_amdgcn_readlane(cnt_lane, 0);
+      for (int i = first_of(before); i < c0; i += kDiaGroup) {
+        const int64_t base = lane_row + plan[pbase + i];
         V v[kQuad];
         load_quad(vp + static_cast<int64_t>(i) * kGroupRows, v);
 #pragma unroll
@@ -952,21 +986,16 @@ __global__ void __launch_bounds__(kSellBlock) sell_spmv_kernel(SellArgs<T, V> a)
 #pragma unroll
       for (int t = 0; t < kQuad; ++t) part[wave][kQuad * lane + t] = acc[t];
     }
-    const size_t chunk = g * kDiaGroup + wave;
-    const size_t row = chunk * kSellChunk + lane;
     double s[8];
 #pragma unroll
     for (int u = 0; u < 8; ++u) s[u] = 0.0;
-    if (chunk < a.chunks) {
+    if (has_chunk) {
       constexpr int U = 8;  // entries (two quads) per step
-      const int32_t w = a.width[chunk];
-      const int32_t nw = a.nwin[chunk];
-      const V* vp = a.vals + a.chunk_start[chunk] + lane * kQuad;
-      const int64_t cbase = static_cast<int64_t>(a.col_start[chunk]) * kSellChunk;
+      const V* vp = a.vals + rstart + lane * kQuad;
+      const int64_t cbase = static_cast<int64_t>(cstart) * kSellChunk;
       int32_t k = 0;
       if (nw >= 0) {
         // lane l holds window base l % 16; an entry's base is fetched across lanes
-        const int32_t wbase = a.windows[chunk * kSellWindows + (lane & (kSellWindows - 1))];
         const uint16_t* cp = a.cols + cbase + lane * kQuad;
         constexpr int kMask = (1 << kSellWindowBits) - 1;
         for (; k + U <= w; k += U) {
