@@ -1588,6 +1588,130 @@ __global__ void __launch_bounds__(kBlock) idr_ba_kernel(IdrVec<T> a, int k) {
   block_reduce_store<1>(acc, 1, a.partials);
 }
 
+// ---- one system, shadow dimension fixed at compile time ---------------------
+//
+// The generic kernels above loop over a run-time number of basis columns and
+// read the coefficients from the scalars struct in global memory, which the
+// compiler must reload after every store (it may alias), so each row waits
+// for several memory round trips.  For the default shadow dimensions the
+// launch below picks a kernel with S fixed and dispatches on k inside: the
+// coefficients are read once into registers and every column a row needs is
+// loaded before the first use.  The arithmetic is the generic kernels', term
+// for term in the same order, so results are bit-identical.
+
+// Calls f(std::integral_constant<int, K>{}) for K = k, 0 <= k < S.
+template <int S, typename F>
+__device__ __forceinline__ void with_step(int k, F&& f) {
+  static_assert(S >= 1 && S <= kMaxShadow, "shadow dimension");
+  switch (k) {
+#define LSMPS_STEP(K) \
+  case K:             \
+    if constexpr (K < S) f(std::integral_constant<int, K>{}); \
+    break;
+    LSMPS_STEP(0) LSMPS_STEP(1) LSMPS_STEP(2) LSMPS_STEP(3) LSMPS_STEP(4) LSMPS_STEP(5)
+    LSMPS_STEP(6) LSMPS_STEP(7) LSMPS_STEP(8) LSMPS_STEP(9) LSMPS_STEP(10) LSMPS_STEP(11)
+    LSMPS_STEP(12) LSMPS_STEP(13) LSMPS_STEP(14) LSMPS_STEP(15)
+#undef LSMPS_STEP
+    default:
+      break;
+  }
+}
+
+// idr_a2g_kernel for one system with s = S.
+template <typename T, int S>
+__global__ void __launch_bounds__(kBlock) idr_a2g_fixed_kernel(IdrVec<T> a, int k) {
+  if (launch_skipped(a.skip)) return;
+  const double om = a.sc->om;
+  double cs[kMaxShadow];  // cs[j] = c[k + j]
+#pragma unroll
+  for (int j = 0; j < kMaxShadow; ++j) cs[j] = j < S - k ? a.sc->c[k + j] : 0.0;
+  with_step<S>(k, [&](auto kc) {
+    constexpr int K = decltype(kc)::value;
+    constexpr int N = S - K;
+    const T* U = a.U + static_cast<size_t>(K) * a.ld;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < a.n;
+         i += static_cast<size_t>(gridDim.x) * blockDim.x) {
+      double e[N];
+#pragma unroll
+      for (int j = 0; j < N; ++j) e[j] = static_cast<double>(U[static_cast<size_t>(j) * a.ld + i]);
+      double v = static_cast<double>(a.v[i]);
+      const int32_t lo = a.cover_start[i], hi = a.cover_start[i + 1];
+      const int32_t g = a.yc ? a.agg_of_row[i] : -1;
+      double u = 0.0;
+#pragma unroll
+      for (int j = 0; j < N; ++j) u += cs[j] * e[j];
+      if (hi > lo)
+        v = static_cast<double>(static_cast<T>(
+            schwarz_row_mean(a.cover_start, a.cover_entries, a.ybuf, static_cast<int32_t>(i))));
+      if (g >= 0) v += a.yc[g];
+      a.U[static_cast<size_t>(K) * a.ld + i] = static_cast<T>(u + om * v);
+    }
+  });
+}
+
+// idr_ba_kernel for one system with s = S.
+template <typename T, bool kBlocks, int S>
+__global__ void __launch_bounds__(kBlock) idr_ba_fixed_kernel(IdrVec<T> a, int k) {
+  if (launch_skipped(a.skip)) return;
+  double acc[1] = {0.0};
+  const double beta = a.sc->beta, om = a.sc->om;
+  double al[kMaxShadow], cs[kMaxShadow];  // al[q] = alpha[q] (q < k), cs[j] = c[k + 1 + j]
+#pragma unroll
+  for (int j = 0; j < kMaxShadow; ++j) {
+    al[j] = j < k ? a.sc->alpha[j] : 0.0;
+    cs[j] = j < S - 1 - k ? a.sc->c[k + 1 + j] : 0.0;
+  }
+  with_step<S>(k, [&](auto kc) {
+    constexpr int K = decltype(kc)::value;
+    constexpr int N = S - 1 - K;  // columns after k
+    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < a.n;
+         i += static_cast<size_t>(gridDim.x) * blockDim.x) {
+      const size_t ok = static_cast<size_t>(K) * a.ld + i;
+      double g = static_cast<double>(a.G[ok]);
+      double u = static_cast<double>(a.U[ok]);
+      double eg[K > 0 ? K : 1], eu[K > 0 ? K : 1], en[N > 0 ? N : 1], un_e[N > 0 && !kBlocks ? N : 1];
+#pragma unroll
+      for (int q = 0; q < K; ++q) {
+        eg[q] = static_cast<double>(a.G[static_cast<size_t>(q) * a.ld + i]);
+        eu[q] = static_cast<double>(a.U[static_cast<size_t>(q) * a.ld + i]);
+      }
+#pragma unroll
+      for (int j = 0; j < N; ++j) en[j] = static_cast<double>(a.G[static_cast<size_t>(K + 1 + j) * a.ld + i]);
+      if constexpr (!kBlocks) {
+#pragma unroll
+        for (int j = 0; j < N; ++j) un_e[j] = static_cast<double>(a.U[static_cast<size_t>(K + 1 + j) * a.ld + i]);
+      }
+      const double ri = a.r[i];
+      const double xi = a.x[i];
+      const double pd = a.pdiag[i];
+#pragma unroll
+      for (int q = 0; q < K; ++q) {
+        g -= al[q] * eg[q];
+        u -= al[q] * eu[q];
+      }
+      a.G[ok] = static_cast<T>(g);
+      a.U[ok] = static_cast<T>(u);
+      const double r = ri - beta * g;
+      a.r[i] = r;
+      a.x[i] = xi + beta * u;
+      acc[0] += r * r;
+      double v = r;
+#pragma unroll
+      for (int j = 0; j < N; ++j) v -= cs[j] * en[j];
+      v *= pd;
+      if constexpr (kBlocks) {
+        a.v[i] = static_cast<T>(v);
+      } else {
+        double un = 0.0;
+#pragma unroll
+        for (int j = 0; j < N; ++j) un += cs[j] * un_e[j];
+        a.U[static_cast<size_t>(K + 1) * a.ld + i] = static_cast<T>(un + om * v);
+      }
+    }
+  });
+  block_reduce_store<1>(acc, 1, a.partials);
+}
+
 // x += om v; r -= om t; partial r.r
 template <typename T, bool kSeg = false>
 __global__ void __launch_bounds__(kBlock) idr_c_kernel(IdrVec<T> a) {
@@ -2130,7 +2254,13 @@ int idr_solve(Context& ctx, PpeSolver& S, size_t n, const SellMatrix& M, const V
         } else {
           schwarz_products<T>(ctx, S.schwarz, v, skip);
         }
-        idr_a2g_kernel<T><<<vb, kBlock, 0, ctx.stream>>>(iv, k);
+        if (s == 14) {
+          idr_a2g_fixed_kernel<T, 14><<<vb, kBlock, 0, ctx.stream>>>(iv, k);
+        } else if (s == 16) {
+          idr_a2g_fixed_kernel<T, 16><<<vb, kBlock, 0, ctx.stream>>>(iv, k);
+        } else {
+          idr_a2g_kernel<T><<<vb, kBlock, 0, ctx.stream>>>(iv, k);
+        }
       }
       SellArgs<T, V> sa{n, M.chunks, M.groups, M.dia_runs.data(), M.dia_plan.data(),
                         M.dia_start.data(), dia, M.chunk_start.data(), M.chunk_width.data(),
@@ -2145,9 +2275,21 @@ int idr_solve(Context& ctx, PpeSolver& S, size_t n, const SellMatrix& M, const V
       if (k == s - 1) {
         idr_b_kernel<T><<<vb, kBlock, 0, ctx.stream>>>(iv, k, 1);
       } else if (schwarz) {
-        idr_ba_kernel<T, true><<<vb, kBlock, 0, ctx.stream>>>(iv, k);
+        if (s == 14) {
+          idr_ba_fixed_kernel<T, true, 14><<<vb, kBlock, 0, ctx.stream>>>(iv, k);
+        } else if (s == 16) {
+          idr_ba_fixed_kernel<T, true, 16><<<vb, kBlock, 0, ctx.stream>>>(iv, k);
+        } else {
+          idr_ba_kernel<T, true><<<vb, kBlock, 0, ctx.stream>>>(iv, k);
+        }
       } else {
-        idr_ba_kernel<T, false><<<vb, kBlock, 0, ctx.stream>>>(iv, k);
+        if (s == 14) {
+          idr_ba_fixed_kernel<T, false, 14><<<vb, kBlock, 0, ctx.stream>>>(iv, k);
+        } else if (s == 16) {
+          idr_ba_fixed_kernel<T, false, 16><<<vb, kBlock, 0, ctx.stream>>>(iv, k);
+        } else {
+          idr_ba_kernel<T, false><<<vb, kBlock, 0, ctx.stream>>>(iv, k);
+        }
       }
     }
     if (schwarz) {
