@@ -687,12 +687,14 @@ __global__ void __launch_bounds__(kSellChunk) rem_select_kernel(
   }
 }
 
-// One 64-thread block per chunk writes its entries, 64 at a time in CSR order
-// (coalesced loads): values on one of its group's diagonals into their slots
-// (the DIA array is zeroed beforehand), every other entry into the next
-// remainder slot of its row with its column encoded against the chunk's
-// windows (or as a 32-bit index); then each lane pads its row's remainder to
-// the chunk's width (base_0, value 0).
+// One 64-thread block per chunk, one lane per row: each lane walks its row's
+// entries in CSR order (four loads ahead) and writes a value on one of the
+// group's diagonals into its slot (the DIA array is zeroed beforehand) and
+// every other entry into its row's next remainder slot, with the column
+// encoded against the chunk's windows (or as a 32-bit index); then pads its
+// row's remainder to the chunk's width (base_0, value 0).  The rows of a
+// lattice-ordered chunk mostly sit on the same diagonal at the same step, so
+// the lanes' DIA stores land in consecutive words of one slot.
 template <typename V>
 __global__ void __launch_bounds__(kSellChunk) sell_fill_kernel(
     size_t n, const int32_t* __restrict__ rows, const int32_t* __restrict__ cols,
@@ -706,29 +708,24 @@ __global__ void __launch_bounds__(kSellChunk) sell_fill_kernel(
   __shared__ int32_t s_off[kMaxDia];
   __shared__ int32_t s_slot[kMaxDia];
   __shared__ int32_t s_win[kSellWindows];
-  __shared__ int32_t rs[kSellChunk + 1];
-  __shared__ int32_t rem_count[kSellChunk + 1];
   const size_t chunk = blockIdx.x;
   const size_t g = chunk / kDiaGroup;
   const int lane = threadIdx.x;
-  const int32_t row0 = static_cast<int32_t>(chunk * kSellChunk);
   const int nd = dia_count[g];
   for (int t = lane; t < nd; t += kSellChunk) {
     s_off[t] = dia_offsets[g * kMaxDia + t];
     s_slot[t] = dia_slot[g * kMaxDia + t];
   }
   if (lane < kSellWindows) s_win[lane] = windows[chunk * kSellWindows + lane];
-  load_row_starts(n, chunk, rows, rs, lane);
-  rem_count[lane] = 0;
   __syncthreads();
   // row r of the chunk is row (chunk % 4) * 64 + r of its group
-  const int64_t dbase = static_cast<int64_t>(dia_start[g]) + (chunk % kDiaGroup) * kSellChunk;
+  V* dia = dia_vals + static_cast<int64_t>(dia_start[g]) + (chunk % kDiaGroup) * kSellChunk + lane;
   const int64_t vbase = chunk_start[chunk];
   const int64_t cbase = static_cast<int64_t>(col_start[chunk]) * kSellChunk;
   const int32_t w = width[chunk];
   const int32_t nw = nwin[chunk];
-  auto put = [&](int32_t k, int row, int32_t c, V v) {
-    const int64_t slot = quad_slot(k, row);
+  auto put = [&](int32_t k, int32_t c, V v) {
+    const int64_t slot = quad_slot(k, lane);
     out_vals[vbase + slot] = v;
     if (nw < 0) {
       reinterpret_cast<int32_t*>(out_cols + cbase)[slot] = c;
@@ -738,29 +735,32 @@ __global__ void __launch_bounds__(kSellChunk) sell_fill_kernel(
       out_cols[cbase + slot] = static_cast<uint16_t>((j << kSellWindowBits) | (c - s_win[j]));
     }
   };
-  const int32_t e0 = rs[0], e1 = rs[kSellChunk];
-  int32_t c_next = e0 + lane < e1 ? cols[e0 + lane] : 0;
-  double v_next = e0 + lane < e1 ? vals[e0 + lane] : 0.0;
-  for (int32_t base = e0; base < e1; base += kSellChunk) {
-    const int32_t e = base + lane;
-    const bool in = e < e1;
-    const int32_t c = c_next;
-    const double v = v_next;
-    if (e + kSellChunk < e1) {  // next group's entry, in flight during this one
-      c_next = cols[e + kSellChunk];
-      v_next = vals[e + kSellChunk];
+  const size_t row = chunk * kSellChunk + lane;
+  const int32_t b = row < n ? rows[row] : 0, e = row < n ? rows[row + 1] : 0;
+  const int32_t r32 = static_cast<int32_t>(row);
+  int32_t rem = 0;
+  auto take = [&](int32_t c, double v) {
+    const int j = dia_find(s_off, nd, c - r32);
+    if (j >= 0) {
+      dia[static_cast<int64_t>(s_slot[j]) * kGroupRows] = static_cast<V>(v);
+    } else {
+      put(rem++, c, static_cast<V>(v));
     }
-    const int lr = in ? chunk_row(rs, e) : kSellChunk;
-    const int j = in ? dia_find(s_off, nd, c - (row0 + lr)) : -1;
-    const bool rem = in && j < 0;
-    const SegmentRank sr = segment_rank(lane, lr, rem);
-    if (in && j >= 0) dia_vals[dbase + static_cast<int64_t>(s_slot[j]) * kGroupRows + lr] = static_cast<V>(v);
-    if (rem) put(rem_count[lr] + sr.before, lr, c, static_cast<V>(v));
-    __syncthreads();  // every lane has read rem_count[lr]
-    if (in && sr.tail) rem_count[lr] += sr.before + (rem ? 1 : 0);
-    __syncthreads();
+  };
+  int32_t q = b;
+  for (; q + 4 <= e; q += 4) {
+    int32_t c[4];
+    double v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      c[u] = cols[q + u];
+      v[u] = vals[q + u];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) take(c[u], v[u]);
   }
-  for (int32_t k = rem_count[lane]; k < w; ++k) put(k, lane, s_win[0], V(0));
+  for (; q < e; ++q) take(cols[q], vals[q]);
+  for (int32_t k = rem; k < w; ++k) put(k, s_win[0], V(0));
 }
 
 template <typename T, typename V>
