@@ -111,6 +111,20 @@ inline unsigned blocks_for(size_t n, int block = kBlock) {
   return static_cast<unsigned>(b > 0 ? b : 1);
 }
 
+// Workgroups are dispatched round-robin over the GPU's 8 XCDs, each with its
+// own L2.  Kernels whose threads gather the data of spatial neighbours
+// (particles in lattice or cell order) number their work by this index instead
+// of blockIdx.x: XCD x (= blockIdx.x % 8) gets the contiguous range of about
+// gridDim.x / 8 blocks starting at x * (g / 8) + min(x, g % 8), so the
+// neighbour data a block gathers was mostly brought into that XCD's L2 by the
+// blocks before it.  A permutation of [0, gridDim.x) for any grid size.
+constexpr unsigned kXcdCount = 8;
+__device__ __forceinline__ size_t xcd_slab_block() {
+  const unsigned g = gridDim.x, b = blockIdx.x;
+  const unsigned x = b % kXcdCount, q = g / kXcdCount, r = g % kXcdCount;
+  return static_cast<size_t>(x) * q + (x < r ? x : r) + b / kXcdCount;
+}
+
 // Kernels that a captured solver cycle replays take an optional `skip` word:
 // when it points at a non-zero value the whole launch returns at once (the
 // value is uniform, so no barrier is split).  Eager launches pass null.
