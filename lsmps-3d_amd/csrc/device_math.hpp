@@ -119,6 +119,7 @@ constexpr int kOpLap = 27;
 // The butterfly strides of at least L combine slots inside a lane and the
 // smaller ones go across lanes, so every L yields the same sums bit for bit;
 // L only changes speed, which is what the autotuner picks it by.
+constexpr int kNbBatch = 4;  // entries per slot whose gathers for_each_batched issues together
 template <int L, int kSlots>
 struct SlotSums {
   static_assert(kSlots % L == 0 && 64 % L == 0, "lanes per particle must divide the slots");
@@ -132,6 +133,34 @@ struct SlotSums {
       for (int m = 0; m < M; ++m) {
         const int32_t c = c0 + m * L;
         if (c < end) body(c, m);
+      }
+    }
+  }
+
+  // for_each with this lane's entries taken in batches of about B (whole
+  // rounds of its M slots): load(c) returns what the body needs of entry c (its
+  // neighbour index and gathered fields), and the loads of a batch are all
+  // issued before its first body runs, so a lane waits for one round of gathers
+  // per batch instead of one per entry; body(c, slot, loaded) runs in
+  // for_each's order.
+  template <int B, typename Load, typename F>
+  __device__ __forceinline__ static void for_each_batched(int32_t begin, int32_t end, int sub,
+                                                          Load load, F body) {
+    using Loaded = decltype(load(int32_t{0}));
+    constexpr int Q = (B / M > 0 ? B / M : 1) * M;  // entries per batch
+    int32_t c0 = begin + sub;
+    for (; c0 + (Q - 1) * L < end; c0 += Q * L) {
+      Loaded v[Q];
+#pragma unroll
+      for (int q = 0; q < Q; ++q) v[q] = load(c0 + q * L);
+#pragma unroll
+      for (int q = 0; q < Q; ++q) body(c0 + q * L, q % M, v[q]);
+    }
+    for (; c0 < end; c0 += L * M) {
+#pragma unroll
+      for (int m = 0; m < M; ++m) {
+        const int32_t c = c0 + m * L;
+        if (c < end) body(c, m, load(c));
       }
     }
   }

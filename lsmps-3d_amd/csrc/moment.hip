@@ -182,6 +182,7 @@ __device__ __forceinline__ void rhs_vector(int which, double e[N]) {
 // the same total in every lane of the group (each pairwise sum is commutative),
 // so every lane factors the same matrix and the lanes share the solves.
 constexpr int kMomLanes = 4;
+constexpr int kMomBatch = 4;  // neighbours per lane loaded together
 
 __device__ __forceinline__ double group_sum(double v) {
 #pragma unroll
@@ -202,16 +203,40 @@ __global__ void __launch_bounds__(kBlock) moment_kernel(MomentArgs a) {
   const double xi = a.x[i], yi = a.y[i], zi = a.z[i];
   const double inv_re = 1.0 / a.re;
   double v[N];
-  // fluid samples: add_sample_to_matrix (moment_matrix.cu:211-229)
-  for (int32_t c = a.f_off[i] + sub; c < a.f_off[i + 1]; c += kMomLanes) {
-    const int32_t j = a.f_idx[c];
-    const double dx = a.x[j] - xi, dy = a.y[j] - yi, dz = a.z[j] - zi;
+  // fluid samples: add_sample_to_matrix (moment_matrix.cu:211-229).  A lane
+  // takes its neighbours in batches of kMomBatch: the batch's indices, then its
+  // positions, are loaded before the first is used (the sums keep the order
+  // of a one-at-a-time loop).
+  auto fluid_sample = [&](double px, double py, double pz) {
+    const double dx = px - xi, dy = py - yi, dz = pz - zi;
     const double d2 = dx * dx + dy * dy + dz * dz;
-    if (d2 <= 0.0) continue;
+    if (d2 <= 0.0) return;
     const double w = weight_of(sqrt(d2), a.re, a.kernel);
-    if (w <= 0.0) continue;
+    if (w <= 0.0) return;
     basis_full<N>(dx, dy, dz, inv_re, v);
     add_outer<N>(m, v, w);
+  };
+  {
+    const int32_t c1 = a.f_off[i + 1];
+    int32_t c = a.f_off[i] + sub;
+    for (; c + (kMomBatch - 1) * kMomLanes < c1; c += kMomBatch * kMomLanes) {
+      int32_t j[kMomBatch];
+#pragma unroll
+      for (int b = 0; b < kMomBatch; ++b) j[b] = a.f_idx[c + b * kMomLanes];
+      double px[kMomBatch], py[kMomBatch], pz[kMomBatch];
+#pragma unroll
+      for (int b = 0; b < kMomBatch; ++b) {
+        px[b] = a.x[j[b]];
+        py[b] = a.y[j[b]];
+        pz[b] = a.z[j[b]];
+      }
+#pragma unroll
+      for (int b = 0; b < kMomBatch; ++b) fluid_sample(px[b], py[b], pz[b]);
+    }
+    for (; c < c1; c += kMomLanes) {
+      const int32_t j = a.f_idx[c];
+      fluid_sample(a.x[j], a.y[j], a.z[j]);
+    }
   }
   if (a.boundary != 0) {
     for (int32_t c = a.w_off[i] + sub; c < a.w_off[i + 1]; c += kMomLanes) {

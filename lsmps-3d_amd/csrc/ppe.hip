@@ -110,9 +110,19 @@ __global__ void __launch_bounds__(kBlock) assemble_kernel(PpeArgs a) {
   for (int m = 0; m < Sums::M; ++m)
 #pragma unroll
     for (int k = 0; k < kAsmSums; ++k) acc[m][k] = 0.0;
-  Sums::for_each(nb, ne, sub, [&](int32_t c, int m) {
-    const int32_t j = a.f_idx[c];
-    const double dx = a.x[j] - xi, dy = a.y[j] - yi, dz = a.z[j] - zi;
+  struct Nb {
+    int32_t j;
+    double x, y, z, u, v, w;
+  };
+  Sums::template for_each_batched<kNbBatch>(
+      nb, ne, sub,
+      [&](int32_t c) {
+        const int32_t j = a.f_idx[c];
+        return Nb{j, a.x[j], a.y[j], a.z[j], a.tvx[j], a.tvy[j], a.tvz[j]};
+      },
+      [&](int32_t c, int m, const Nb& q) {
+    const int32_t j = q.j;
+    const double dx = q.x - xi, dy = q.y - yi, dz = q.z - zi;
     const double w = weight_of(sqrt(dx * dx + dy * dy + dz * dz), a.re, a.kernel);
     double off = 0.0;
     if (w > 0.0) {
@@ -123,7 +133,7 @@ __global__ void __launch_bounds__(kBlock) assemble_kernel(PpeArgs a) {
       for (int k = 0; k < 9; ++k) lb += Lap[k] * b[k];
       off = coef * w * lb;
       acc[m][27] -= off;
-      const double du = a.tvx[j] - ui, dv = a.tvy[j] - vi, dw = a.tvz[j] - wi;
+      const double du = q.u - ui, dv = q.v - vi, dw = q.w - wi;
 #pragma unroll
       for (int k = 0; k < 9; ++k) {
         const double wb = w * b[k];

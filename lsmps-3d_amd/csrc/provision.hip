@@ -72,21 +72,29 @@ __global__ void __launch_bounds__(kBlock) provision_kernel(ProvisionArgs a) {
   double acc[Sums::M][3];
 #pragma unroll
   for (int m = 0; m < Sums::M; ++m) acc[m][0] = acc[m][1] = acc[m][2] = 0.0;
-  Sums::for_each(a.f_off[i], a.f_off[i + 1], sub, [&](int32_t c, int m) {
-    const int32_t j = a.f_idx[c];
-    const double dx = a.x[j] - xi, dy = a.y[j] - yi, dz = a.z[j] - zi;
-    const double w = weight_of(sqrt(dx * dx + dy * dy + dz * dz), a.re, a.kernel);
-    if (w <= 0.0) return;
-    double b[9];
-    type_a_basis(dx, dy, dz, inv_re, b);
-    double lb = 0.0;
+  struct Nb {
+    double x, y, z, vx, vy, vz;
+  };
+  Sums::template for_each_batched<kNbBatch>(
+      a.f_off[i], a.f_off[i + 1], sub,
+      [&](int32_t c) {
+        const int32_t j = a.f_idx[c];
+        return Nb{a.x[j], a.y[j], a.z[j], a.vx[j], a.vy[j], a.vz[j]};
+      },
+      [&](int32_t, int m, const Nb& q) {
+        const double dx = q.x - xi, dy = q.y - yi, dz = q.z - zi;
+        const double w = weight_of(sqrt(dx * dx + dy * dy + dz * dz), a.re, a.kernel);
+        if (w <= 0.0) return;
+        double b[9];
+        type_a_basis(dx, dy, dz, inv_re, b);
+        double lb = 0.0;
 #pragma unroll
-    for (int k = 0; k < 9; ++k) lb += lap_row[k] * b[k];
-    const double wl = w * lb;
-    acc[m][0] += wl * (a.vx[j] - vxi);
-    acc[m][1] += wl * (a.vy[j] - vyi);
-    acc[m][2] += wl * (a.vz[j] - vzi);
-  });
+        for (int k = 0; k < 9; ++k) lb += lap_row[k] * b[k];
+        const double wl = w * lb;
+        acc[m][0] += wl * (q.vx - vxi);
+        acc[m][1] += wl * (q.vy - vyi);
+        acc[m][2] += wl * (q.vz - vzi);
+      });
   Sums::for_each(a.w_off[i], a.w_off[i + 1], sub, [&](int32_t c, int m) {
     const int32_t j = a.w_idx[c];
     const double dx = a.wx[j] - xi, dy = a.wy[j] - yi, dz = a.wz[j] - zi;
