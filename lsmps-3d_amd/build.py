@@ -72,7 +72,10 @@ def build(verbose=False, jobs=None):
                 print(warn, flush=True)
             objs.append(obj)
     tmp = LIB + ".tmp"
-    cmd = [hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-o", tmp]
+    # roctx (rocprofiler-sdk) marks the stepper's stages for rocprofv3 --marker-trace
+    rocm_lib = os.path.join(os.path.dirname(os.path.dirname(os.path.realpath(hipcc()))), "lib")
+    cmd = [hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, f"-L{rocm_lib}",
+           "-lrocprofiler-sdk-roctx", f"-Wl,-rpath,{rocm_lib}", "-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)

@@ -9,6 +9,8 @@
 #include <algorithm>
 #include <cstring>
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 #include "objects.hpp"
 
 using namespace lsmps;
@@ -203,7 +205,33 @@ void lsmps_stepper::set_segments(int count, const int64_t* fo, const int64_t* wo
   segment_cells = static_cast<size_t>(base);
 }
 
+namespace {
+// roctx ranges over the host side of the step and of its stages (rocprofv3
+// --marker-trace); the stages' GPU times are the events' (stage_ms).  Every
+// range pushed is popped, also when a stage throws.
+struct StepRanges {
+  bool stage_open = false;
+  StepRanges() { roctxRangePushA("lsmps step"); }
+  void stage(const char* name) {
+    end_stage();
+    roctxRangePushA(name);
+    stage_open = true;
+  }
+  void end_stage() {
+    if (stage_open) roctxRangePop();
+    stage_open = false;
+  }
+  ~StepRanges() {
+    end_stage();
+    roctxRangePop();
+  }
+  StepRanges(const StepRanges&) = delete;
+  StepRanges& operator=(const StepRanges&) = delete;
+};
+}  // namespace
+
 void lsmps_stepper::step(const lsmps_fluid_soa& fluid, const lsmps_wall_soa& walls) {
+  StepRanges ranges;
   Context& c = *ctx;
   const size_t nf = fluid.count, nw = walls.count;
   if (segment_count > 0)
@@ -241,6 +269,7 @@ void lsmps_stepper::step(const lsmps_fluid_soa& fluid, const lsmps_wall_soa& wal
   }
   LSMPS_HIP(hipEventRecord(events[0], c.stream));
   // 1. geometry: rebuild_geometry (simulation_driver.cu:252-264)
+  ranges.stage("geometry");
   if (frozen) {
     // lists and labels of the previous step
   } else if (segment_count > 0) {
@@ -261,10 +290,12 @@ void lsmps_stepper::step(const lsmps_fluid_soa& fluid, const lsmps_wall_soa& wal
   geometry_walls = nw;
   LSMPS_HIP(hipEventRecord(events[1], c.stream));
   // 2. moment operators (prepare_matrices, moment_matrix.cu:615-654)
+  ranges.stage("moments");
   moments.config = config;
   moments.prepare(fluid, walls, fn, wn, generation);
   LSMPS_HIP(hipEventRecord(events[2], c.stream));
   // 3. provision (explicit_update.cu:366-399)
+  ranges.stage("provision");
   double* tv = tmp_fluid.data();
   double* tw = tmp_wall.data();
   lsmps_fluid_soa tmp{nf, nullptr, nullptr, nullptr, tv, tv + nf1, tv + 2 * nf1, nullptr, nullptr};
@@ -274,11 +305,13 @@ void lsmps_stepper::step(const lsmps_fluid_soa& fluid, const lsmps_wall_soa& wal
   provision_walls(c, config, walls, tmpw);
   LSMPS_HIP(hipEventRecord(events[3], c.stream));
   // 4. PPE assembly (ppe_matrix.cu:486-515)
+  ranges.stage("assembly");
   const lsmps_ppe_workspace ws = ppe_workspace(nf);
   ppe_assemble(c, config, fluid, walls, fn, wn, tmp, tmpw, moments.velocity_ops(),
                moments.pressure_ops(), ws);
   LSMPS_HIP(hipEventRecord(events[4], c.stream));
   // 5. PPE solve (replaces the AMGX solve, simulation_driver.cu:364-370)
+  ranges.stage("solve");
   // As in the reference driver, the step continues with whatever pressure the
   // solver produced; a non-converged solve is reported after the correction.
   last_report = lsmps_solve_report{};
@@ -321,6 +354,7 @@ void lsmps_stepper::step(const lsmps_fluid_soa& fluid, const lsmps_wall_soa& wal
   if (record && nf > 0) copy_f64(c.stream, raw_pressure.data(), ws.pressure, nf);
   LSMPS_HIP(hipEventRecord(events[5], c.stream));
   // 6. correction (pressure_correction.cu:598-645)
+  ranges.stage("correction");
   double* cv = corr.data();
   lsmps_vector3 grad{nf, cv, cv + nf1, cv + 2 * nf1};
   lsmps_vector3 ps{nf, cv + 3 * nf1, cv + 4 * nf1, cv + 5 * nf1};
@@ -331,6 +365,7 @@ void lsmps_stepper::step(const lsmps_fluid_soa& fluid, const lsmps_wall_soa& wal
   update_velocity_position(c, config, fluid, tmp, grad, ps);
   anti_penetration(c, config, fluid, walls, wn);
   smooth_velocity(c, config, fluid, fn, sm);
+  ranges.end_stage();
   LSMPS_HIP(hipEventRecord(events[6], c.stream));
   LSMPS_HIP(hipEventSynchronize(events[6]));
   for (int k = 0; k < 6; ++k) {
