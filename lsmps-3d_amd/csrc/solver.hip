@@ -688,13 +688,14 @@ __global__ void __launch_bounds__(kSellChunk) rem_select_kernel(
 }
 
 // One 64-thread block per chunk, one lane per row: each lane walks its row's
-// entries in CSR order (four loads ahead) and writes a value on one of the
+// entries in CSR order and writes a value on one of the
 // group's diagonals into its slot (the DIA array is zeroed beforehand) and
 // every other entry into its row's next remainder slot, with the column
 // encoded against the chunk's windows (or as a 32-bit index); then pads its
 // row's remainder to the chunk's width (base_0, value 0).  The rows of a
 // lattice-ordered chunk mostly sit on the same diagonal at the same step, so
 // the lanes' DIA stores land in consecutive words of one slot.
+constexpr int kFillStep = 16;
 template <typename V>
 __global__ void __launch_bounds__(kSellChunk) sell_fill_kernel(
     size_t n, const int32_t* __restrict__ rows, const int32_t* __restrict__ cols,
@@ -747,17 +748,19 @@ __global__ void __launch_bounds__(kSellChunk) sell_fill_kernel(
       put(rem++, c, static_cast<V>(v));
     }
   };
+  // 16 entries (64 bytes of columns, 128 of values) per step, read together,
+  // so that each cache line of the lane's row is fetched about once
   int32_t q = b;
-  for (; q + 4 <= e; q += 4) {
-    int32_t c[4];
-    double v[4];
+  for (; q + kFillStep <= e; q += kFillStep) {
+    int32_t c[kFillStep];
+    double v[kFillStep];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < kFillStep; ++u) {
       c[u] = cols[q + u];
       v[u] = vals[q + u];
     }
 #pragma unroll
-    for (int u = 0; u < 4; ++u) take(c[u], v[u]);
+    for (int u = 0; u < kFillStep; ++u) take(c[u], v[u]);
   }
   for (; q < e; ++q) take(cols[q], vals[q]);
   for (int32_t k = rem; k < w; ++k) put(k, s_win[0], V(0));
@@ -845,10 +848,6 @@ struct XGather<float> {
 // Largest system the 32-bit buffer offsets address.
 constexpr size_t kMaxGatherRows = (size_t{1} << 31) / sizeof(double) - 1;
 
-// Runs of class M (M consecutive offsets each) of one group.  The lane owning
-// rows i .. i + 3 reads x[i + d .. i + d + M + 2] as (M + 4) / 2 16-byte loads
-// and the run's M value quads (4 rows each), then adds its 4 M products.
-// Wave w of the block takes every fourth run, starting at `first`.
 // A group's plan held across the lanes of a wavefront (lane j holds entries
 // j, j + 64, j + 128, j + 192), read back one entry at a time with readlane:
 // the offsets then live in scalar registers and a run's window loads issue
@@ -863,21 +862,57 @@ struct GroupPlan {
 };
 static_assert(kMaxDia == 256, "four plan registers per lane");
 
+// x[i + 1] of lane i in lane i, for lanes 0..62 (DPP wave_shl:1 on both
+// halves); lane 63, whose source would be lane 64, keeps `old`.
+__device__ __forceinline__ double from_next_lane(double old, double src) {
+  const uint64_t o = __builtin_bit_cast(uint64_t, old), v = __builtin_bit_cast(uint64_t, src);
+  constexpr int kWaveShl1 = 0x130;
+  const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_update_dpp(
+      static_cast<int>(static_cast<uint32_t>(o)), static_cast<int>(static_cast<uint32_t>(v)),
+      kWaveShl1, 0xF, 0xF, false));
+  const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_update_dpp(
+      static_cast<int>(static_cast<uint32_t>(o >> 32)), static_cast<int>(static_cast<uint32_t>(v >> 32)),
+      kWaveShl1, 0xF, 0xF, false));
+  return __builtin_bit_cast(double, static_cast<uint64_t>(lo) | (static_cast<uint64_t>(hi) << 32));
+}
+
 // Runs of class M (M consecutive offsets each) of one group, plan entries
-// [pbase, pbase + count).  The lane owning rows i .. i + 3 reads
-// x[i + d .. i + d + M + 2] as (M + 4) / 2 16-byte loads and the run's M value
-// quads (4 rows each), then adds its 4 M products.  Wave w of the block takes
-// every fourth run, starting at `first`.
+// [pbase, pbase + count).  The lane owning rows i .. i + 3 needs the window
+// x[i + d .. i + d + M + 2] and the run's M value quads (4 rows each), then adds
+// its 4 M products.  Wave w of the block takes every fourth run, starting at
+// `first`.  FP64 values: each lane reads its window as (M + 4) / 2 16-byte
+// loads.  FP32 values: each lane loads only its own quad x[i + d .. i + d + 3]
+// and takes the next quads of the window from the lanes above it (DPP), so the
+// address unit sees two x loads per run instead of up to six; lane 63 loads the
+// rest of its window itself (110 against 114 us per launch at 1M rows).
 template <int M, typename T, typename V>
 __device__ __forceinline__ void dia_run_class(int count, int first, const GroupPlan& plan,
                                               int pbase, const V* vp, uint32_t rowb,
                                               const XGather<T>& xg, double (&acc)[4]) {
-  constexpr int NL = (M + 4) / 2;
+  constexpr bool kShift = sizeof(V) == 4;
+  constexpr int NL = (M + 4) / 2;                  // 16-byte loads of a whole window
+  constexpr int NQ = (M + 6) / 4;                  // quads holding the M + 3 window elements
+  constexpr int NW = kShift ? 4 * NQ : 2 * NL;     // window registers
+  const bool top_lane = (threadIdx.x & 63) == 63;
   for (int i = first; i < count; i += kDiaGroup) {
     const uint32_t off = rowb + static_cast<uint32_t>(plan[pbase + i]) * static_cast<uint32_t>(sizeof(T));
-    T xw[2 * NL];
+    T xw[NW];
+    if constexpr (kShift) {
+      xg.pair(off, xw[0], xw[1]);
+      xg.pair(off + 16u, xw[2], xw[3]);
 #pragma unroll
-    for (int j = 0; j < NL; ++j) xg.pair(off + 16u * j, xw[2 * j], xw[2 * j + 1]);
+      for (int e = 4; e < NW; ++e) xw[e] = 0.0;
+      if (top_lane) {
+        // the last quad may reach past the window: those columns are unused (0 past n)
+#pragma unroll
+        for (int j = 2; j < NW / 2; ++j) xg.pair(off + 16u * j, xw[2 * j], xw[2 * j + 1]);
+      }
+#pragma unroll
+      for (int e = 4; e < NW; ++e) xw[e] = from_next_lane(xw[e], xw[e - 4]);
+    } else {
+#pragma unroll
+      for (int j = 0; j < NL; ++j) xg.pair(off + 16u * j, xw[2 * j], xw[2 * j + 1]);
+    }
     V v[M][kQuad];
 #pragma unroll
     for (int q = 0; q < M; ++q) load_quad(vp + static_cast<int64_t>(i * M + q) * kGroupRows, v[q]);
