@@ -775,11 +775,13 @@ __global__ void __launch_bounds__(kBlock) restrict_kernel(int count, const int32
 template <typename T, typename V>
 __global__ void __launch_bounds__(kBlock) products_restrict_kernel(
     unsigned product_groups, const int32_t* members, const int32_t* sizes, const V* inverse,
-    const float* scale, const T* wa, double* ybuf, const int32_t* agg_start,
+    const float* scale, const T* wa, const int32_t* entry_of, double* ybuf, size_t stride,
+    const int32_t* agg_start,
     const int32_t* agg_rows, const T* wr, const double* wfac, double* rc, const int32_t* skip) {
   if (launch_skipped(skip)) return;
   if (blockIdx.x < product_groups) {  // uniform per workgroup
-    schwarz_block_product<T, V>(blockIdx.x, members, sizes, inverse, scale, wa, ybuf);
+    schwarz_block_product<T, V>(blockIdx.x, members, sizes, inverse, scale, wa, entry_of, ybuf,
+                                stride);
   } else {
     restrict_block<T>(static_cast<int>(blockIdx.x - product_groups), agg_start, agg_rows, wr,
                       wfac, rc);
@@ -1181,15 +1183,18 @@ void products_and_restrict(Context& ctx, const SchwarzPrec& P, CoarseSpace& C, c
   if (P.half) {
     products_restrict_kernel<T, _Float16><<<grid, kBlock, 0, ctx.stream>>>(
         groups, P.members.data(), P.sizes.data(),
-        reinterpret_cast<const _Float16*>(P.inv16.data()), P.col_scale.data(), wa, P.ybuf.data(),
+        reinterpret_cast<const _Float16*>(P.inv16.data()), P.col_scale.data(), wa,
+        P.entry_of.data(), P.ybuf.data(), P.ybuf_stride,
         C.agg_start.data(), C.agg_rows.data(), wr, wfac, C.rc.data(), skip);
   } else if (P.fp32) {
     products_restrict_kernel<T, float><<<grid, kBlock, 0, ctx.stream>>>(
-        groups, P.members.data(), P.sizes.data(), P.inv32.data(), nullptr, wa, P.ybuf.data(),
+        groups, P.members.data(), P.sizes.data(), P.inv32.data(), nullptr, wa, P.entry_of.data(),
+        P.ybuf.data(), P.ybuf_stride,
         C.agg_start.data(), C.agg_rows.data(), wr, wfac, C.rc.data(), skip);
   } else {
     products_restrict_kernel<T, double><<<grid, kBlock, 0, ctx.stream>>>(
-        groups, P.members.data(), P.sizes.data(), P.inv64.data(), nullptr, wa, P.ybuf.data(),
+        groups, P.members.data(), P.sizes.data(), P.inv64.data(), nullptr, wa, P.entry_of.data(),
+        P.ybuf.data(), P.ybuf_stride,
         C.agg_start.data(), C.agg_rows.data(), wr, wfac, C.rc.data(), skip);
   }
   LSMPS_LAUNCH_CHECK();

@@ -437,22 +437,31 @@ template <typename T, typename V>
 __global__ void __launch_bounds__(kMaxBlock) apply_kernel(const int32_t* members,
                                                           const int32_t* sizes, const V* inverse,
                                                           const float* scale, const T* w,
-                                                          double* ybuf, const int32_t* skip) {
+                                                          const int32_t* entry_of, double* ybuf,
+                                                          size_t stride, const int32_t* skip) {
   if (launch_skipped(skip)) return;
-  schwarz_block_product<T, V>(blockIdx.x, members, sizes, inverse, scale, w, ybuf);
+  schwarz_block_product<T, V>(blockIdx.x, members, sizes, inverse, scale, w, entry_of, ybuf,
+                              stride);
 }
 
 // Covered rows take the mean of their blocks' values (each the sum of its
 // kApplyParts partials), in ascending block order.
 template <typename T>
 __global__ void gather_kernel(size_t count, const int32_t* covered, const int32_t* start,
-                              const int32_t* entries, const double* ybuf, T* v,
-                              const int32_t* skip) {
+                              const double* ybuf, size_t stride, T* v, const int32_t* skip) {
   if (launch_skipped(skip)) return;
   const size_t k = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (k >= count) return;
   const int32_t i = covered[k];
-  v[i] = static_cast<T>(schwarz_row_mean(start, entries, ybuf, i));
+  v[i] = static_cast<T>(schwarz_row_mean(start, ybuf, stride, i));
+}
+
+// entry_of[slot] = e for every cover entry e (one thread per row).
+__global__ void entry_of_kernel(size_t n, const int32_t* start, const int32_t* entries,
+                                int32_t* entry_of) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int32_t e = start[i]; e < start[i + 1]; ++e) entry_of[entries[e]] = e;
 }
 
 // FP16 copies of the FP32 inverses: one wavefront per block column c < n;
@@ -474,7 +483,8 @@ __global__ void __launch_bounds__(kBlock) quantize_kernel(const int32_t* sizes, 
     for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
     const int e = m > 0.0f && isfinite(m) ? 14 - ilogbf(m) : 0;
     const float up = ldexpf(1.0f, e);
-    for (int r = lane; r < n; r += 64) inv16[base + r] = static_cast<_Float16>(inv32[base + r] * up);
+    for (int r = lane; r < n; r += 64)
+      inv16[half_index(b, r, c)] = static_cast<_Float16>(inv32[base + r] * up);
     if (lane == 0) scale[b * ld + c] = ldexpf(1.0f, -e);
   }
 }
@@ -558,6 +568,7 @@ void schwarz_setup(Context& ctx, SchwarzPrec& P, const lsmps_csr& A, const doubl
   // Row -> covering slots (CSR over rows), sorted per row.
   P.cover.reserve(n);
   P.cover_start.reserve(n + 1);
+  P.ybuf_stride = nb * kMaxBlock;
   P.ybuf.reserve(nb * kApplyParts * kMaxBlock);
   fill_i32(ctx.stream, P.cover.data(), n, 0);
   cover_count_kernel<<<static_cast<unsigned>(nb), kMaxBlock, 0, ctx.stream>>>(
@@ -571,6 +582,13 @@ void schwarz_setup(Context& ctx, SchwarzPrec& P, const lsmps_csr& A, const doubl
       P.cover_entries.data());
   cover_sort_kernel<<<blocks_for(n), kBlock, 0, ctx.stream>>>(n, P.cover_start.data(),
                                                              P.cover_entries.data());
+  LSMPS_LAUNCH_CHECK();
+  // every slot b * kMaxBlock + t with t < sizes[b] is one row's entry; the
+  // others are never written by the products
+  P.entry_of.reserve(nb * kMaxBlock);
+  entry_of_kernel<<<blocks_for(n), kBlock, 0, ctx.stream>>>(n, P.cover_start.data(),
+                                                           P.cover_entries.data(),
+                                                           P.entry_of.data());
   // preconditioner diagonal and the list of covered rows (flags and scan reuse
   // the seed arrays, which are no longer needed)
   P.pdiag.reserve(n);
@@ -626,8 +644,10 @@ void schwarz_transpose_of(Context& ctx, SchwarzPrec& P, const SchwarzPrec& src,
   copy_array(ctx, P.sizes, src.sizes, nb);
   copy_array(ctx, P.cover_start, src.cover_start, n + 1);
   copy_array(ctx, P.cover_entries, src.cover_entries, nb * kMaxBlock);
+  copy_array(ctx, P.entry_of, src.entry_of, nb * kMaxBlock);
   copy_array(ctx, P.covered, src.covered, std::max<size_t>(src.covered_count, 1));
   copy_array(ctx, P.pdiag, src.pdiag, n);
+  P.ybuf_stride = nb * kMaxBlock;
   P.ybuf.reserve(nb * kApplyParts * kMaxBlock);
   const unsigned grid = static_cast<unsigned>(std::min<size_t>(blocks_for(dense), 8192));
   if (src.fp32) {
@@ -652,15 +672,17 @@ void schwarz_products(Context& ctx, const SchwarzPrec& P, const T* v, const int3
   if (P.half) {
     apply_kernel<T, _Float16><<<grid, kMaxBlock, 0, ctx.stream>>>(
         P.members.data(), P.sizes.data(), reinterpret_cast<const _Float16*>(P.inv16.data()),
-        P.col_scale.data(), v, P.ybuf.data(), skip);
+        P.col_scale.data(), v, P.entry_of.data(), P.ybuf.data(), P.ybuf_stride, skip);
   } else if (P.fp32) {
     apply_kernel<T, float><<<grid, kMaxBlock, 0, ctx.stream>>>(P.members.data(), P.sizes.data(),
                                                                P.inv32.data(), nullptr, v,
-                                                               P.ybuf.data(), skip);
+                                                               P.entry_of.data(), P.ybuf.data(),
+                                                               P.ybuf_stride, skip);
   } else {
     apply_kernel<T, double><<<grid, kMaxBlock, 0, ctx.stream>>>(P.members.data(), P.sizes.data(),
                                                                 P.inv64.data(), nullptr, v,
-                                                                P.ybuf.data(), skip);
+                                                                P.entry_of.data(), P.ybuf.data(),
+                                                                P.ybuf_stride, skip);
   }
   LSMPS_LAUNCH_CHECK();
 }
@@ -669,8 +691,8 @@ template <typename T>
 void schwarz_complete(Context& ctx, const SchwarzPrec& P, T* v, const int32_t* skip) {
   if (P.blocks == 0) return;
   gather_kernel<T><<<blocks_for(std::max<size_t>(P.covered_count, 1)), kBlock, 0, ctx.stream>>>(
-      P.covered_count, P.covered.data(), P.cover_start.data(), P.cover_entries.data(),
-      P.ybuf.data(), v, skip);
+      P.covered_count, P.covered.data(), P.cover_start.data(), P.ybuf.data(), P.ybuf_stride, v,
+      skip);
   LSMPS_LAUNCH_CHECK();
 }
 

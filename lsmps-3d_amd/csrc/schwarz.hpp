@@ -1,6 +1,8 @@
 // Restricted additive Schwarz blocks around the PPE's wrong-sign rows (schwarz.hip).
 #pragma once
 
+#include <type_traits>
+
 #include "common.hpp"
 
 namespace lsmps {
@@ -21,38 +23,50 @@ struct SchwarzPrec {
   DeviceArray<float> inv32;
   // mixed precision: FP16 copies of inv32 read by the application, column c
   // of block b stored as inv32 * 2^e and col_scale[b * kMaxBlock + c] = 2^-e
-  // (the column's largest magnitude lands in [2^14, 2^15))
+  // (the column's largest magnitude lands in [2^14, 2^15)); entry (r, c) at
+  // half_index(b, r, c): the four columns of a quad side by side in each row
   bool half = false;
   DeviceArray<uint16_t> inv16;
   DeviceArray<float> col_scale;
   DeviceArray<int32_t> cover;          // per-row counter (setup only)
   DeviceArray<int32_t> cover_start;    // n + 1: row i's slots are cover_entries[start[i], start[i+1])
   DeviceArray<int32_t> cover_entries;  // slots b * kMaxBlock + position, ascending per row
+  DeviceArray<int32_t> entry_of;       // blocks x kMaxBlock: slot -> its index in cover_entries
   DeviceArray<int32_t> covered;        // the covered rows, ascending
   size_t covered_count = 0;
   DeviceArray<double> pdiag;           // 1 on covered rows, 1 / A_ii elsewhere
-  DeviceArray<double> ybuf;            // blocks x kApplyParts x kMaxBlock: partial products
+  DeviceArray<double> ybuf;            // kApplyParts x ybuf_stride: partial p of entry e at p * stride + e
+  size_t ybuf_stride = 0;              // blocks x kMaxBlock (at least the number of cover entries)
   size_t bytes() const {
     return eligible.bytes() + seed_flag.bytes() + seed_pos.bytes() + seeds.bytes() +
            members.bytes() + sizes.bytes() + scratch.bytes() + inv64.bytes() + inv32.bytes() +
            inv16.bytes() + col_scale.bytes() +
-           cover.bytes() + cover_start.bytes() + cover_entries.bytes() + covered.bytes() +
+           cover.bytes() + cover_start.bytes() + cover_entries.bytes() + entry_of.bytes() +
+           covered.bytes() +
            pdiag.bytes() + ybuf.bytes();
   }
 };
 
+// FP16 inverses: each quad of columns is a kMaxBlock x 4 row-major tile, so a
+// row's four entries of the quad are one 8-byte load.
+__host__ __device__ __forceinline__ size_t half_index(size_t b, int r, int c) {
+  return (b * kMaxBlock + static_cast<size_t>(c & ~3)) * kMaxBlock + static_cast<size_t>(r) * 4 +
+         static_cast<size_t>(c & 3);
+}
+
 // Value of covered row i: the mean of its blocks' products (each the sum of
-// its kApplyParts partials in ybuf), in ascending block order.
-__device__ __forceinline__ double schwarz_row_mean(const int32_t* start, const int32_t* entries,
-                                                   const double* ybuf, int32_t i) {
+// its kApplyParts partials), in ascending block order.  The partials are
+// stored in cover-entry order (partial p of entry e at p * stride + e), so
+// row i reads kApplyParts contiguous pieces [start[i], start[i + 1]) right
+// after its range; a gather through the block slots instead cost a chain of
+// dependent loads per block (11 us per IDR step at 1M rows).
+__device__ __forceinline__ double schwarz_row_mean(const int32_t* start, const double* ybuf,
+                                                   size_t stride, int32_t i) {
   const int32_t lo = start[i], hi = start[i + 1];
   double z = 0.0;
   for (int32_t e = lo; e < hi; ++e) {
-    const int32_t slot = entries[e];  // block * kMaxBlock + position
-    const size_t base = static_cast<size_t>(slot / kMaxBlock) * kApplyParts * kMaxBlock +
-                        static_cast<size_t>(slot % kMaxBlock);
 #pragma unroll
-    for (int p = 0; p < kApplyParts; ++p) z += ybuf[base + static_cast<size_t>(p) * kMaxBlock];
+    for (int p = 0; p < kApplyParts; ++p) z += ybuf[static_cast<size_t>(p) * stride + e];
   }
   return z / static_cast<double>(hi - lo);
 }
@@ -60,7 +74,7 @@ __device__ __forceinline__ double schwarz_row_mean(const int32_t* start, const i
 // y_b = inv_b w_b, split over kApplyParts workgroups per block b so that a
 // few hundred blocks still fill the GPU: workgroup (b, p) = group multiplies
 // the column range [p * 64, p * 64 + 64) and thread t (row t) writes its
-// partial sum to ybuf[(b * kApplyParts + p) * kMaxBlock + t].  The column
+// partial sum to ybuf[p * stride + entry_of[b * kMaxBlock + t]].  The column
 // slices are read coalesced across the rows.  With FP16 inverses
 // (V = _Float16) column j holds inv * 2^e_j and the input slice takes the
 // exact factor scale[j] = 2^-e_j instead (scale null: none).  Called by every
@@ -69,7 +83,8 @@ template <typename T, typename V>
 __device__ __forceinline__ void schwarz_block_product(unsigned group, const int32_t* members,
                                                       const int32_t* sizes, const V* inverse,
                                                       const float* scale, const T* w,
-                                                      double* ybuf) {
+                                                      const int32_t* entry_of, double* ybuf,
+                                                      size_t stride) {
   __shared__ double wb[kApplyCols];
   const int t = threadIdx.x;
   const size_t b = group / kApplyParts;
@@ -89,16 +104,31 @@ __device__ __forceinline__ void schwarz_block_product(unsigned group, const int3
   if (t >= n) return;
   double y[4] = {0.0, 0.0, 0.0, 0.0};
   int j = j0;
-  for (; j + 4 <= j1; j += 4) {
+  if constexpr (std::is_same<V, _Float16>::value) {
+    // quad tiles (half_index): one 8-byte load per row and quad; j0 is a multiple of 4
+    typedef _Float16 half4 __attribute__((ext_vector_type(4)));
+    const V* hb = inverse + b * ld * ld;
+    for (; j + 4 <= j1; j += 4) {
+      const half4 q = *reinterpret_cast<const half4*>(hb + static_cast<size_t>(j) * ld + 4 * t);
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
-      y[u] += static_cast<double>(static_cast<float>(inv[static_cast<size_t>(j + u) * ld + t])) *
-              wb[j + u - j0];
+      for (int u = 0; u < 4; ++u)
+        y[u] += static_cast<double>(static_cast<float>(q[u])) * wb[j + u - j0];
+    }
+    for (; j < j1; ++j)
+      y[0] += static_cast<double>(static_cast<float>(hb[half_index(0, t, j)])) * wb[j - j0];
+  } else {
+    for (; j + 4 <= j1; j += 4) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        y[u] += static_cast<double>(static_cast<float>(inv[static_cast<size_t>(j + u) * ld + t])) *
+                wb[j + u - j0];
+    }
+    for (; j < j1; ++j)
+      y[0] += static_cast<double>(static_cast<float>(inv[static_cast<size_t>(j) * ld + t])) *
+              wb[j - j0];
   }
-  for (; j < j1; ++j)
-    y[0] += static_cast<double>(static_cast<float>(inv[static_cast<size_t>(j) * ld + t])) *
-            wb[j - j0];
-  ybuf[(b * kApplyParts + part) * kMaxBlock + t] = (y[0] + y[1]) + (y[2] + y[3]);
+  ybuf[static_cast<size_t>(part) * stride + entry_of[b * kMaxBlock + t]] =
+      (y[0] + y[1]) + (y[2] + y[3]);
 }
 
 // Builds the blocks of matrix A from its per-row statistics: diagonal A_ii,

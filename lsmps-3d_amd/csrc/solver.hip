@@ -1232,8 +1232,8 @@ struct IdrVec {
   const int32_t* row_seg;  // batched systems: each row's segment (scalars sc[row_seg[i]])
   // Schwarz completion read by idr_a2g_kernel (null without blocks)
   const int32_t* cover_start;
-  const int32_t* cover_entries;
   const double* ybuf;
+  size_t ybuf_stride;
   // coarse-space term added by idr_a2g_kernel (null without a coarse space)
   const int32_t* agg_of_row;
   const double* yc;
@@ -1348,7 +1348,7 @@ __global__ void __launch_bounds__(kBlock) idr_a2g_kernel(IdrVec<T> a, int k) {
     double v = static_cast<double>(a.v[i]);
     if (a.cover_start[i + 1] > a.cover_start[i])
       v = static_cast<double>(static_cast<T>(
-          schwarz_row_mean(a.cover_start, a.cover_entries, a.ybuf, static_cast<int32_t>(i))));
+          schwarz_row_mean(a.cover_start, a.ybuf, a.ybuf_stride, static_cast<int32_t>(i))));
     if (a.yc) {
       const int32_t g = a.agg_of_row[i];
       if (g >= 0) v += a.yc[g];
@@ -1677,7 +1677,7 @@ __global__ void __launch_bounds__(kBlock) idr_a2g_fixed_kernel(IdrVec<T> a, int 
       for (int j = 0; j < N; ++j) u += cs[j] * e[j];
       if (hi > lo)
         v = static_cast<double>(static_cast<T>(
-            schwarz_row_mean(a.cover_start, a.cover_entries, a.ybuf, static_cast<int32_t>(i))));
+            schwarz_row_mean(a.cover_start, a.ybuf, a.ybuf_stride, static_cast<int32_t>(i))));
       if (g >= 0) v += a.yc[g];
       a.U[static_cast<size_t>(K) * a.ld + i] = static_cast<T>(u + om * v);
     }
@@ -2118,7 +2118,7 @@ void append_sell_signature(std::vector<uintptr_t>& sig, const SellMatrix& M) {
 void append_schwarz_signature(std::vector<uintptr_t>& sig, const SchwarzPrec& P) {
   const void* p[] = {P.members.data(), P.sizes.data(), P.inv32.data(), P.inv64.data(),
                      P.inv16.data(), P.col_scale.data(), P.ybuf.data(), P.covered.data(),
-                     P.cover_start.data(), P.cover_entries.data(), P.pdiag.data()};
+                     P.cover_start.data(), P.entry_of.data(), P.pdiag.data()};
   for (const void* q : p) sig.push_back(reinterpret_cast<uintptr_t>(q));
   sig.push_back(P.blocks);
   sig.push_back(P.covered_count);
@@ -2242,8 +2242,8 @@ int idr_solve(Context& ctx, PpeSolver& S, size_t n, const SellMatrix& M, const V
   IdrVec<T> iv{n, ld, s, r, x, v, t, G, U, p0, S.scalars.data(), S.part_b.data(),
                schwarz ? S.schwarz.pdiag.data() : dinv, nullptr,
                schwarz ? S.schwarz.cover_start.data() : nullptr,
-               schwarz ? S.schwarz.cover_entries.data() : nullptr,
                schwarz ? S.schwarz.ybuf.data() : nullptr,
+               schwarz ? S.schwarz.ybuf_stride : 0,
                coarse ? S.coarse.agg_of_row.data() : nullptr,
                coarse ? S.coarse.yc.data() : nullptr};
   const unsigned vb = vec_blocks(n);
@@ -2510,8 +2510,8 @@ void idr_solve_segmented(Context& ctx, PpeSolver& S, size_t n, const SellMatrix&
                schwarz ? S.schwarz.pdiag.data() : dinv,
                g.row_seg,
                schwarz ? S.schwarz.cover_start.data() : nullptr,
-               schwarz ? S.schwarz.cover_entries.data() : nullptr,
                schwarz ? S.schwarz.ybuf.data() : nullptr,
+               schwarz ? S.schwarz.ybuf_stride : 0,
                coarse ? S.coarse.agg_of_row.data() : nullptr,
                coarse ? S.coarse.yc.data() : nullptr};
   const unsigned vb = vec_blocks(n);
