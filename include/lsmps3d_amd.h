@@ -83,9 +83,11 @@ enum { LSMPS_WEIGHT_LINEAR = 0, LSMPS_WEIGHT_QUADRATIC = 1 };
  * PCG (Jacobi only) is only meaningful for symmetric positive systems. */
 enum { LSMPS_SOLVER_PCG = 0, LSMPS_SOLVER_BICGSTAB = 1, LSMPS_SOLVER_IDRS = 2 };
 
-/* Context flags.  KERNEL_TIMING brackets every PPE SpMV launch with hipEvents on
- * the context stream and accumulates durations and algorithmic bytes
- * (lsmps_context_kernel_stats). */
+/* Context flags.  KERNEL_TIMING brackets the PPE SpMV launches of every eighth
+ * IDR(s) cycle (the first of each solve included) with hipEvents on the context
+ * stream and accumulates their durations and algorithmic bytes
+ * (lsmps_context_kernel_stats); each event pair stalls the stream for a few
+ * microseconds, which timing every launch would add to every iteration. */
 /* LSMPS_FLAG_GRAPH: the solver captures one IDR(s) cycle (its vector kernels,
  * SpMVs, preconditioner, scalar recurrences and the loop's convergence,
  * reliable-update and restart decisions, made on the device) in a HIP graph and

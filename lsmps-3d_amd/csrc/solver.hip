@@ -2150,6 +2150,9 @@ double spmv_algorithmic_bytes(const SellMatrix& M, size_t n, size_t value_bytes,
 }
 
 // Whether the SpMVs are timed (LSMPS_FLAG_TIMING); creates the events once.
+// With LSMPS_FLAG_KERNEL_TIMING the SpMVs of one IDR cycle in this many are timed.
+constexpr int kSpmvTimingStride = 8;
+
 bool spmv_timing_ready(Context& ctx) {
   const bool timing = ctx.timing();
   if (timing && !ctx.timing_ready) {
@@ -2257,17 +2260,20 @@ int idr_solve(Context& ctx, PpeSolver& S, size_t n, const SellMatrix& M, const V
   int it = 0, replacements = 0;
   const double target2 = target * target;
   const double factor2 = replace_factor * replace_factor;
-  // Optional per-launch timing of the SpMVs (events bracket each launch).
+  // Optional per-launch timing of the SpMVs: events bracket each launch of
+  // every kSpmvTimingStride-th cycle (each event pair stalls the stream for a
+  // few us, so timing every cycle would slow the solve it measures).
   const bool timing = spmv_timing_ready(ctx);
   const double spmv_bytes = spmv_algorithmic_bytes(M, n, sizeof(V), sizeof(T));
   // One IDR(s) cycle: s steps, each with one SpMV G_k = A U_k (and, with the
   // Schwarz blocks, their products and the coarse term before it), then the
   // omega step's SpMV and the residual norm.  `skip` (null when launched
-  // directly) is the captured graphs' halt word; `set` picks the SpMV events.
-  auto cycle_body = [&](const int32_t* skip, int set) {
+  // directly) is the captured graphs' halt word; `set` picks the SpMV events,
+  // `timed` whether this cycle records them.
+  auto cycle_body = [&](const int32_t* skip, int set, bool timed) {
     // SpMV timing: HIP events around each launch (direct launches only)
     auto mark = [&](int slot) {
-      if (timing)
+      if (timed)
         LSMPS_HIP(hipEventRecord(ctx.timing_events[set * Context::kTimingSet + slot], ctx.stream));
     };
     iv.skip = skip;
@@ -2402,7 +2408,7 @@ int idr_solve(Context& ctx, PpeSolver& S, size_t n, const SellMatrix& M, const V
             int set) {
           IdrControl* ctl = S.control.data();
           idr_control_begin<<<1, 1, 0, ctx.stream>>>(ctl);
-          cycle_body(halt, set);
+          cycle_body(halt, set, false);
           idr_control_cycle<<<1, 1, 0, ctx.stream>>>(ctl, S.scalars.data());
           replace_body(skip_replace);
           idr_control_after_replace<<<1, 1, 0, ctx.stream>>>(ctl, S.scalars.data());
@@ -2442,15 +2448,16 @@ int idr_solve(Context& ctx, PpeSolver& S, size_t n, const SellMatrix& M, const V
     S.last_graph_cycles = replays;
   } else {
     S.last_graph_cycles = 0;
-    while (rr > target2 && it < max_iterations) {
-      cycle_body(nullptr, 0);
+    for (int cycle = 0; rr > target2 && it < max_iterations; ++cycle) {
+      const bool timed = timing && cycle % kSpmvTimingStride == 0;
+      cycle_body(nullptr, 0, timed);
       it += s + 1;
       LSMPS_HIP(hipMemcpyAsync(S.host, &S.scalars.data()->rr, sizeof(double),
                                hipMemcpyDeviceToHost, ctx.stream));
       LSMPS_HIP(hipMemcpyAsync(S.host + 2, &S.scalars.data()->breakdown, sizeof(int32_t),
                                hipMemcpyDeviceToHost, ctx.stream));
       ctx.sync();
-      if (timing) add_timing(0);
+      if (timed) add_timing(0);
       rr = S.host[0];
       int32_t breakdown = 0;
       std::memcpy(&breakdown, S.host + 2, sizeof(int32_t));
@@ -2542,18 +2549,20 @@ void idr_solve_segmented(Context& ctx, PpeSolver& S, size_t n, const SellMatrix&
     scalar_norm<<<nseg, kReduceThreads, 0, ctx.stream>>>(S.scalars.data(), part, kSegBlocks);
     LSMPS_LAUNCH_CHECK();
   };
-  // SpMV timing as in the single-system solve: events around each launch of a
-  // cycle (slots 2 k, 2 k + 1), read after the cycle's status copy has waited
+  // SpMV timing as in the single-system solve: events around each launch of
+  // every kSpmvTimingStride-th cycle (slots 2 k, 2 k + 1), read after the
+  // cycle's status copy has waited
   const bool timing = spmv_timing_ready(ctx);
   const double spmv_bytes = spmv_algorithmic_bytes(M, n, sizeof(V), sizeof(T));
-  int timed = 0;  // SpMVs recorded in the current cycle
+  int timed = 0;        // SpMVs recorded in the current cycle
+  bool cycle_timed = timing;  // the first cycle is timed
   auto spmv = [&](const T* in, T* out) {
     SellArgs<T, V> a{n, M.chunks, M.groups, M.dia_runs.data(), M.dia_plan.data(),
                      M.dia_start.data(),
                      dia, M.chunk_start.data(), M.chunk_width.data(), M.col_start.data(),
                      M.nwin.data(), M.windows.data(), M.cols.data(), vals, in, out, p0, s,
                      nullptr, nullptr};
-    const bool mark = timing && 2 * timed + 1 < Context::kTimingSet;
+    const bool mark = cycle_timed && 2 * timed + 1 < Context::kTimingSet;
     if (mark) LSMPS_HIP(hipEventRecord(ctx.timing_events[2 * timed], ctx.stream));
     sell_spmv_kernel<T, V, 0><<<sb, kSellBlock, 0, ctx.stream>>>(a);
     if (mark) {
@@ -2594,7 +2603,8 @@ void idr_solve_segmented(Context& ctx, PpeSolver& S, size_t n, const SellMatrix&
       if (!done[k]) return true;
     return false;
   };
-  while (active() && it < max_iterations) {
+  for (int cycle = 0; active() && it < max_iterations; ++cycle) {
+    cycle_timed = timing && cycle % kSpmvTimingStride == 0;
     seg_reduce_kernel<T, 4><<<rb, kBlock, 0, ctx.stream>>>(g.start, s, p0, nullptr, r, part);
     idr_scalar_f<<<nseg, kReduceThreads, 0, ctx.stream>>>(S.scalars.data(), part, kSegBlocks, s);
     // step 0's first half; later steps get it fused into the previous step's b
