@@ -147,4 +147,10 @@ void products_and_restrict(Context& ctx, const SchwarzPrec& P, CoarseSpace& C, c
 template <typename W>
 void coarse_prolong_add(Context& ctx, const CoarseSpace& C, W* v, const int32_t* skip = nullptr);
 
+// schwarz_complete followed by coarse_prolong_add, in one pass over the rows:
+// covered rows take their blocks' mean, then aggregated rows add their yc.
+template <typename T>
+void schwarz_complete_prolong_add(Context& ctx, const SchwarzPrec& P, const CoarseSpace& C, T* v,
+                                  const int32_t* skip = nullptr);
+
 }  // namespace lsmps

@@ -2313,16 +2313,18 @@ int idr_solve(Context& ctx, PpeSolver& S, size_t n, const SellMatrix& M, const V
       idr_scalar_after<<<1, kReduceThreads, 0, ctx.stream>>>(S.scalars.data(), S.part_a.data(),
                                                              static_cast<int>(sb), s, k, skip);
       iv.partials = S.part_b.data();
-      if (k == s - 1) {
-        idr_b_kernel<T><<<vb, kBlock, 0, ctx.stream>>>(iv, k, 1);
-      } else if (schwarz) {
+      if (schwarz && (s == 14 || s == 16)) {
+        // at k = s - 1 there are no later columns: the fixed kernel's a part
+        // is then v = pdiag r, what idr_b_kernel(k, write_v) writes
         if (s == 14) {
           idr_ba_fixed_kernel<T, true, 14><<<vb, kBlock, 0, ctx.stream>>>(iv, k);
-        } else if (s == 16) {
-          idr_ba_fixed_kernel<T, true, 16><<<vb, kBlock, 0, ctx.stream>>>(iv, k);
         } else {
-          idr_ba_kernel<T, true><<<vb, kBlock, 0, ctx.stream>>>(iv, k);
+          idr_ba_fixed_kernel<T, true, 16><<<vb, kBlock, 0, ctx.stream>>>(iv, k);
         }
+      } else if (k == s - 1) {
+        idr_b_kernel<T><<<vb, kBlock, 0, ctx.stream>>>(iv, k, 1);
+      } else if (schwarz) {
+        idr_ba_kernel<T, true><<<vb, kBlock, 0, ctx.stream>>>(iv, k);
       } else {
         if (s == 14) {
           idr_ba_fixed_kernel<T, false, 14><<<vb, kBlock, 0, ctx.stream>>>(iv, k);
@@ -2336,9 +2338,8 @@ int idr_solve(Context& ctx, PpeSolver& S, size_t n, const SellMatrix& M, const V
     if (schwarz) {
       if (coarse) {  // products of v and restriction of r in one launch
         products_and_restrict<T>(ctx, S.schwarz, S.coarse, v, r, false, skip);
-        schwarz_complete<T>(ctx, S.schwarz, v, skip);
         coarse_product(ctx, S.coarse, skip);
-        coarse_prolong_add(ctx, S.coarse, v, skip);
+        schwarz_complete_prolong_add<T>(ctx, S.schwarz, S.coarse, v, skip);
       } else {
         schwarz_apply<T>(ctx, S.schwarz, v, skip);
       }
@@ -2452,15 +2453,19 @@ int idr_solve(Context& ctx, PpeSolver& S, size_t n, const SellMatrix& M, const V
       const bool timed = timing && cycle % kSpmvTimingStride == 0;
       cycle_body(nullptr, 0, timed);
       it += s + 1;
-      LSMPS_HIP(hipMemcpyAsync(S.host, &S.scalars.data()->rr, sizeof(double),
-                               hipMemcpyDeviceToHost, ctx.stream));
-      LSMPS_HIP(hipMemcpyAsync(S.host + 2, &S.scalars.data()->breakdown, sizeof(int32_t),
-                               hipMemcpyDeviceToHost, ctx.stream));
+      // rr through breakdown in one copy (each copy is a kernel of ~9 us here)
+      constexpr size_t kFrom = offsetof(SolverScalars, rr);
+      constexpr size_t kBytes = offsetof(SolverScalars, breakdown) + sizeof(int32_t) - kFrom;
+      static_assert(kBytes <= 8 * sizeof(double), "fits the pinned status buffer");
+      LSMPS_HIP(hipMemcpyAsync(S.host, &S.scalars.data()->rr, kBytes, hipMemcpyDeviceToHost,
+                               ctx.stream));
       ctx.sync();
       if (timed) add_timing(0);
       rr = S.host[0];
       int32_t breakdown = 0;
-      std::memcpy(&breakdown, S.host + 2, sizeof(int32_t));
+      std::memcpy(&breakdown,
+                  reinterpret_cast<const char*>(S.host) + offsetof(SolverScalars, breakdown) - kFrom,
+                  sizeof(int32_t));
       if (!std::isfinite(rr)) break;
       if (rr <= target2 || rr <= factor2 * rr_replaced) {
         replace_body(nullptr);
@@ -2635,9 +2640,8 @@ void idr_solve_segmented(Context& ctx, PpeSolver& S, size_t n, const SellMatrix&
     if (schwarz) {
       if (coarse) {  // products of v and restriction of r in one launch
         products_and_restrict<T>(ctx, S.schwarz, S.coarse, v, r, false);
-        schwarz_complete<T>(ctx, S.schwarz, v);
         coarse_product(ctx, S.coarse);
-        coarse_prolong_add(ctx, S.coarse, v);
+        schwarz_complete_prolong_add<T>(ctx, S.schwarz, S.coarse, v);
       } else {
         schwarz_apply<T>(ctx, S.schwarz, v);
       }
