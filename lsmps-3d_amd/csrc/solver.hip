@@ -862,8 +862,8 @@ struct GroupPlan {
 };
 static_assert(kMaxDia == 256, "four plan registers per lane");
 
-// x[i + 1] of lane i in lane i, for lanes 0..62 (DPP wave_shl:1 on both
-// halves); lane 63, whose source would be lane 64, keeps `old`.
+// Lane i receives `src` of lane i + 1 (DPP wave_shl:1 on both 32-bit halves);
+// lane 63, whose source would be lane 64, keeps `old`.
 __device__ __forceinline__ double from_next_lane(double old, double src) {
   const uint64_t o = __builtin_bit_cast(uint64_t, old), v = __builtin_bit_cast(uint64_t, src);
   constexpr int kWaveShl1 = 0x130;
@@ -882,9 +882,10 @@ __device__ __forceinline__ double from_next_lane(double old, double src) {
 // its 4 M products.  Wave w of the block takes every fourth run, starting at
 // `first`.  FP64 values: each lane reads its window as (M + 4) / 2 16-byte
 // loads.  FP32 values: each lane loads only its own quad x[i + d .. i + d + 3]
-// and takes the next quads of the window from the lanes above it (DPP), so the
-// address unit sees two x loads per run instead of up to six; lane 63 loads the
-// rest of its window itself (110 against 114 us per launch at 1M rows).
+// and takes the next quads of the window from the lanes above it (DPP), so a
+// run costs two full-wave x loads instead of up to six; lane 63 loads the rest
+// of its window itself, in loads of that one lane (111 against 115 us per
+// launch at 1M rows).
 template <int M, typename T, typename V>
 __device__ __forceinline__ void dia_run_class(int count, int first, const GroupPlan& plan,
                                               int pbase, const V* vp, uint32_t rowb,
