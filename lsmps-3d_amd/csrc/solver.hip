@@ -1389,6 +1389,15 @@ __device__ void freeze(SolverScalars* sc) {
   sc->om = 0.0;
 }
 
+// Lane `l` (wave-uniform) of a double, through two v_readlane: the scalar
+// recurrences broadcast one lane at a time, so this replaces a ds_bpermute
+// round trip per step with register moves (same value).
+__device__ __forceinline__ double lane_value(double v, int l) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+  return __hiloint2double(hi, lo);
+}
+
 // Lower-triangular solve M[k:, k:] c[k:] = f[k:] (single thread).
 // The s x s recurrences run on the first wavefront of the block, lane i owning
 // row i: forward substitutions go column by column (lane l finishes its row,
@@ -1407,7 +1416,7 @@ __device__ void solve_c(SolverScalars* sc, int s, int k) {
       sc->c[l] = cl;
       singular = d == 0.0;
     }
-    cl = __shfl(cl, l, 64);
+    cl = lane_value(cl, l);
     if (i > l && i < s) t -= sc->M[i * kMaxShadow + l] * cl;
   }
   if (__any(singular) && i == 0) sc->breakdown = 1;
@@ -1416,6 +1425,18 @@ __device__ void solve_c(SolverScalars* sc, int s, int k) {
 // The single-thread recurrences below work on an LDS copy of the system's
 // scalars (all threads copy in and out), not on global memory.
 static_assert(sizeof(SolverScalars) % sizeof(double) == 0, "copied as doubles");
+constexpr int kScalarWords = static_cast<int>(sizeof(SolverScalars) / sizeof(double));
+static_assert(kScalarWords <= kReduceThreads, "one scalar word per thread");
+// The scalar kernels' copy-in in two halves: each thread's word is loaded
+// before the partial loads are issued (both round trips overlap) and stored to
+// LDS after the reduction.
+__device__ __forceinline__ double scalars_word(const SolverScalars* from) {
+  return threadIdx.x < kScalarWords ? reinterpret_cast<const double*>(from)[threadIdx.x] : 0.0;
+}
+__device__ __forceinline__ void scalars_store(double w, SolverScalars* to) {
+  if (threadIdx.x < kScalarWords) reinterpret_cast<double*>(to)[threadIdx.x] = w;
+  __syncthreads();
+}
 __device__ void scalars_copy(const SolverScalars* from, SolverScalars* to) {
   const double* src = reinterpret_cast<const double*>(from);
   double* dst = reinterpret_cast<double*>(to);
@@ -1434,8 +1455,9 @@ __global__ void __launch_bounds__(kReduceThreads)
   partials += static_cast<size_t>(blockIdx.x) * nb * kPartialStride;
   __shared__ double d[kPartialStride];
   __shared__ SolverScalars lds;
+  const double word = scalars_word(sc_g);
   reduce_partials(partials, nb, d);
-  scalars_copy(sc_g, &lds);
+  scalars_store(word, &lds);
   SolverScalars* sc = &lds;
   if (threadIdx.x < 64) {  // the first wavefront
     if (sc->done) {
@@ -1461,8 +1483,9 @@ __global__ void __launch_bounds__(kReduceThreads)
   partials += static_cast<size_t>(blockIdx.x) * nb * kPartialStride;
   __shared__ double d[kPartialStride];
   __shared__ SolverScalars lds;
+  const double word = scalars_word(sc_g);
   reduce_partials(partials, nb, d);
-  scalars_copy(sc_g, &lds);
+  scalars_store(word, &lds);
   if (threadIdx.x < 64) {  // the first wavefront
     if (lds.done) {
       if (threadIdx.x == 0) freeze(&lds);
@@ -1487,17 +1510,17 @@ __device__ void scalar_after_body(SolverScalars* sc, const double* d, int s, int
       sc->alpha[l] = al;
       my_alpha = al;
     }
-    al = __shfl(al, l, 64);
+    al = lane_value(al, l);
     if (i > l && i < k) t -= al * sc->M[i * kMaxShadow + l];
   }
   // new column k of M: d(k:s) minus the projections on the earlier columns
   double mk = i >= k && i < s ? d[i] : 0.0;
   for (int l = 0; l < k; ++l) {
-    const double al = __shfl(my_alpha, l, 64);
+    const double al = lane_value(my_alpha, l);
     if (i >= k && i < s) mk -= al * sc->M[i * kMaxShadow + l];
   }
   if (i >= k && i < s) sc->M[i * kMaxShadow + k] = mk;
-  const double mkk = __shfl(mk, k, 64);
+  const double mkk = lane_value(mk, k);
   double beta = 0.0;
   if (mkk == 0.0 || !isfinite(mkk)) {
     if (i == 0) sc->breakdown = 1;

@@ -234,23 +234,23 @@ def test_metric_case_against_oracle(rt, abi, cases, oracle):
 
 @pytest.mark.parametrize("case_name", ["C2", "M1"])
 def test_mixed_error_target(rt, abi, cases, oracle, case_name):
-    """The throughput configuration: the mixed step with the PPE solved to a
-    1e-7 relative true residual (ten times tighter than the reference's AMGX
-    1e-6, ppe_matrix.cu:607) against the oracle's FP64 step solved to 1e-10.
+    """The throughput configuration: the mixed step with the PPE solved to the
+    1e-6 relative true residual of the reference's own AMGX solve
+    (ppe_matrix.cu:607) against the oracle's FP64 step solved to 1e-10.
     Pressure, grad p and the corrected velocity (relative to the velocity
     itself and to u*) stay within the 1e-5 relative-L2 bar; CSR and labels are
     bit-exact as in every step."""
     case = cases.dam_break() if case_name == "C2" else cases.metric_case()
     ref = oracle.pipeline(case)
     opts = abi.default_solver_options()
-    opts.tolerance = 1e-7
+    opts.tolerance = 1e-6
     g = rt.run_step_on_device(case, precision=abi.PRECISION_MIXED, solver=opts)
-    print(case_name, "mixed at 1e-7", g["report"], "oracle", ref["report"], flush=True)
+    print(case_name, "mixed at 1e-6", g["report"], "oracle", ref["report"], flush=True)
     _check_geometry(g, ref)
     assert g["report"]["converged"] == 1
-    assert g["report"]["relative_residual"] <= 1e-7
+    assert g["report"]["relative_residual"] <= 1e-6
     errs = field_errors(g, ref)
-    print_field_errors(f"{case_name} MIXED 1e-7", errs)
+    print_field_errors(f"{case_name} MIXED 1e-6", errs)
     v = np.concatenate([g["fluid"][k] for k in ("vx", "vy", "vz")])
     vr = np.concatenate([ref["correction"]["fluid"][k] for k in ("vx", "vy", "vz")])
     v_rel = rel_l2(v, vr)
