@@ -2280,7 +2280,6 @@ int idr_solve(Context& ctx, PpeSolver& S, size_t n, const SellMatrix& M, const V
   idr_init_scalars<<<1, kBlock, 0, ctx.stream>>>(S.scalars.data(), s);
   LSMPS_LAUNCH_CHECK();
   double rr = rr0;
-  double rr_replaced = rr0;
   int it = 0, replacements = 0;
   const double target2 = target * target;
   const double factor2 = replace_factor * replace_factor;
@@ -2472,32 +2471,51 @@ int idr_solve(Context& ctx, PpeSolver& S, size_t n, const SellMatrix& M, const V
     rr = last.rr;
     S.last_graph_cycles = replays;
   } else {
+    // Direct launches under the same device-side loop control as the graph
+    // replays (the loop tests run in one-thread kernels, every later launch of
+    // a halted solve returns at once), with two cycles in flight: while cycle
+    // c runs, the host waits for cycle c - 1's control record and enqueues
+    // cycle c + 1.  Cycle c's SpMV events are event set c & 1; a set is read
+    // back after its cycle's record and before the set is recorded again.
     S.last_graph_cycles = 0;
-    for (int cycle = 0; rr > target2 && it < max_iterations; ++cycle) {
-      const bool timed = timing && cycle % kSpmvTimingStride == 0;
-      cycle_body(nullptr, 0, timed);
-      it += s + 1;
-      // rr through breakdown in one copy (each copy is a kernel of ~9 us here)
-      constexpr size_t kFrom = offsetof(SolverScalars, rr);
-      constexpr size_t kBytes = offsetof(SolverScalars, breakdown) + sizeof(int32_t) - kFrom;
-      static_assert(kBytes <= 8 * sizeof(double), "fits the pinned status buffer");
-      LSMPS_HIP(hipMemcpyAsync(S.host, &S.scalars.data()->rr, kBytes, hipMemcpyDeviceToHost,
-                               ctx.stream));
-      ctx.sync();
-      if (timed) add_timing(0);
-      rr = S.host[0];
-      int32_t breakdown = 0;
-      std::memcpy(&breakdown,
-                  reinterpret_cast<const char*>(S.host) + offsetof(SolverScalars, breakdown) - kFrom,
-                  sizeof(int32_t));
-      if (!std::isfinite(rr)) break;
-      if (rr <= target2 || rr <= factor2 * rr_replaced) {
-        replace_body(nullptr);
-        rr = read_rr(ctx, S);
-        rr_replaced = rr;
-        ++replacements;
+    S.control.reserve(1);
+    if (!S.host_control)
+      LSMPS_HIP(hipHostMalloc(reinterpret_cast<void**>(&S.host_control), 2 * sizeof(IdrControl)));
+    for (auto& e : S.cycle_done)
+      if (!e) LSMPS_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    if (rr > target2 && max_iterations > 0) {
+      IdrControl* ctl = S.control.data();
+      idr_control_init<<<1, 1, 0, ctx.stream>>>(ctl, rr, target2, factor2, max_iterations, s);
+      LSMPS_LAUNCH_CHECK();
+      auto launch = [&](int c) {
+        const int slot = c & 1;
+        idr_control_begin<<<1, 1, 0, ctx.stream>>>(ctl);
+        cycle_body(&ctl->halt, slot, timing && c % kSpmvTimingStride == 0);
+        idr_control_cycle<<<1, 1, 0, ctx.stream>>>(ctl, S.scalars.data());
+        replace_body(&ctl->skip_replace);
+        idr_control_after_replace<<<1, 1, 0, ctx.stream>>>(ctl, S.scalars.data());
+        reinit_body(&ctl->skip_reinit);
+        LSMPS_LAUNCH_CHECK();
+        LSMPS_HIP(hipMemcpyAsync(S.host_control + slot, ctl, sizeof(IdrControl),
+                                 hipMemcpyDeviceToHost, ctx.stream));
+        LSMPS_HIP(hipEventRecord(S.cycle_done[slot], ctx.stream));
+      };
+      launch(0);
+      launch(1);
+      IdrControl last{};
+      for (int c = 0;; ++c) {
+        const int slot = c & 1;
+        LSMPS_HIP(hipEventSynchronize(S.cycle_done[slot]));
+        last = S.host_control[slot];
+        // cycle c ran: a cycle that starts halted is never waited for (the
+        // loop stops at the record that first shows the halt)
+        if (timing && c % kSpmvTimingStride == 0) add_timing(slot);
+        if (last.halt) break;
+        launch(c + 2);
       }
-      if (breakdown) reinit_body(nullptr);
+      it = last.iterations;
+      replacements = last.replacements;
+      rr = last.rr;
     }
   }
   *iterations_out = it;
