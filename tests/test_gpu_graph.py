@@ -1,7 +1,7 @@
-"""Captured IDR cycles (LSMPS_FLAG_GRAPH) against the launch path, through the
-C-ABI: the device-side loop control makes the host loop's decisions from the
-same values, so pressures are bit-identical and the iteration and
-reliable-update counts equal.  The graphs are replayed over several steps with
+"""Captured IDR cycles (LSMPS_FLAG_GRAPH) against the direct launches, through
+the C-ABI: both run under the same device-side loop control (two cycles in
+flight), so pressures are bit-identical and the iteration and reliable-update
+counts equal.  The graphs are replayed over several steps with
 the program's own eager work and synchronisations between them, and every
 replay is checked against the launch path."""
 import numpy as np
@@ -71,3 +71,44 @@ def test_graph_iteration_limit(rt, abi, cases):
     assert eager["report"]["converged"] == 0 and graph["report"]["converged"] == 0
     assert eager["report"]["iterations"] == graph["report"]["iterations"] >= 40
     assert np.array_equal(eager["pressure"], graph["pressure"])
+
+
+def test_direct_loop_across_solves_with_timing(rt, abi, cases):
+    """The direct launches' loop control and SpMV event sets carried across
+    solves of one stepper: tolerances 1e-6, 1e-10, 1e-6 in turn (a halted
+    cycle of each solve is still in flight when the next solve starts), with
+    kernel timing on.  Each step equals a fresh context's step at its
+    tolerance bit for bit, and the timed SpMV launches are counted."""
+    import ctypes as C
+    import torch
+    case = cases.dam_break(block=(30, 20, 20), tank=(60, 20, 40), l0=0.01, seed=0)
+    loose = abi.default_solver_options()
+    loose.tolerance = 1e-6
+    tight = abi.default_solver_options()
+    refs = {1e-6: rt.run_step_on_device(case, precision=abi.PRECISION_MIXED, solver=loose),
+            1e-10: rt.run_step_on_device(case, precision=abi.PRECISION_MIXED, solver=tight)}
+    assert refs[1e-6]["report"]["iterations"] < refs[1e-10]["report"]["iterations"]
+    ctx = rt.Context(abi.PRECISION_MIXED, flags=abi.FLAG_KERNEL_TIMING)
+    state = rt.DeviceState(case)
+    st = rt.Stepper(ctx, state, case.config, 256, loose)
+    snap = state.snapshot()
+    lib = rt.load()
+    rt.check(lib.lsmps_context_kernel_stats(ctx.handle, None, None, None, 1))
+    for tol, opts in ((1e-6, loose), (1e-10, tight), (1e-6, loose)):
+        st.set_solver(opts)
+        state.restore(snap)
+        rep = st.step()
+        torch.cuda.synchronize()
+        print("tolerance", tol, rep, flush=True)
+        ref = refs[tol]
+        assert rep["graph_cycles"] == 0
+        for k in ("iterations", "refinements", "converged", "relative_residual"):
+            assert rep[k] == ref["report"][k], (tol, k)
+        assert np.array_equal(state.fluid["pressure"].cpu().numpy(), ref["pressure"]), tol
+    launches, ms, nbytes = C.c_int64(), C.c_double(), C.c_double()
+    rt.check(lib.lsmps_context_kernel_stats(ctx.handle, C.byref(launches), C.byref(ms),
+                                            C.byref(nbytes), 0))
+    print("timed SpMV launches", launches.value, "ms", ms.value, flush=True)
+    assert launches.value > 0 and ms.value > 0.0 and nbytes.value > 0.0
+    st.close()
+    ctx.close()
